@@ -1,0 +1,189 @@
+"""Throughput of the coherent-dedispersion + fold hot path on one MI355X.
+
+One step = one block of the 128-channel workload through the whole path
+(unpack -> overlap-save convolution -> full-Stokes detection -> fold into 1024
+bins x 8 sub-integrations), raw int8 input resident in HBM before timing.
+Prints one JSON line (see the keys below).
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload multichan|single]
+"""
+import argparse
+import importlib
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "Msamples/s dedispersed+folded; % HBM roofline; fused-MFMA vs plain FP32"
+HBM_PEAK_GBS = 8000.0
+
+WORKLOADS = {
+    # 128 channels x 2 pols x 2^20 complex int8, 400 MHz at 1400 MHz, DM 300,
+    # P = 253 ms, 2% duty at 0.1 sigma, full Stokes, 1024 bins x 8 subints
+    "multichan": dict(nchan=128, npol=2, ndat=1 << 20, cfreq=1400.0, bw=400.0, dm=300.0, period=0.253,
+                      duty=0.02, peak=0.1, sigma=10.0, nbit=8, state=3, ndim=4, nbin=1024, nsub=8),
+    # 1 channel x 2 pols x 2^24 complex int8, 16 MHz at 1405 MHz, DM 56.77,
+    # P = 89.33 ms, 3% duty at 0.2 sigma, total intensity, 1024 bins
+    "single": dict(nchan=1, npol=2, ndat=1 << 24, cfreq=1405.0, bw=16.0, dm=56.77, period=0.08933,
+                   duty=0.03, peak=0.2, sigma=10.0, nbit=8, state=0, ndim=1, nbin=1024, nsub=1),
+}
+
+
+def algorithmic_bytes(w, geom, npart):
+    """Per-stage algorithmic HBM bytes of one block: stage A reads the raw
+    samples of every FFT window (B_in = 2*nbit/8 per sample) and writes 8 B;
+    stage B reads and writes 8 B; stage C reads 8 B (the chirp is generated on
+    chip; profiles are O(nbin))."""
+    n = w["nchan"] * w["npol"] * npart * geom["nfft"]
+    b_in = 2 * w["nbit"] / 8
+    return {"stageA_unpack_colfft": n * (b_in + 8), "stageB_rowfft_chirp": n * 16,
+            "stageC_colifft_detect_fold": n * 8}
+
+
+def time_path(dspx, ctx, w, geom, raw, form, steps, warmup, timing_steps=3):
+    import torch
+    L = dspx.LoadToFold(ctx, w["nbit"], w["nchan"], w["npol"], w["ndat"], w["cfreq"], w["bw"], w["dm"],
+                        w["nbin"], state=w["state"], ndim=w["ndim"], form=form, geom=geom)
+    rate = abs(w["bw"]) / w["nchan"] * 1e6
+    chunks = dspx.fold_chunks(L.ndat_out, rate, w["period"], geom["impulse_pos"], nsub=w["nsub"])
+    L.set_fold((1.0 / rate) / w["period"], chunks)
+    nval = L.nval
+    prof = torch.zeros((w["nsub"], w["nchan"], nval * w["nbin"]), dtype=torch.float32, device="cuda")
+    hits = torch.zeros((w["nsub"], w["nbin"]), dtype=torch.int32, device="cuda")
+    for _ in range(warmup):
+        L.execute(raw, prof, hits)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        L.execute(raw, prof, hits)
+    torch.cuda.synchronize()
+    t = (time.perf_counter() - t0) / steps
+    # per-kernel durations from HIP events on the library's stream
+    L.set_timing(True)
+    stage = {}
+    for _ in range(timing_steps):
+        L.execute(raw, prof, hits)
+        for name, ms in L.stage_times():
+            stage.setdefault(name, []).append(ms)
+    L.set_timing(False)
+    torch.cuda.synchronize()
+    prof_h = prof.cpu().numpy()
+    hits_h = hits.cpu().numpy()
+    npart = L.npart
+    L.close()
+    return t, {k: sum(v) / len(v) for k, v in stage.items()}, prof_h, hits_h, npart
+
+
+def cpu_baseline(w, geom, raw_np, budget_s=20.0):
+    """The oracle chain (unpack, FP32 overlap-save convolution with its own
+    FFT, detection, fold) on the host, one core, over as many leading
+    channels of the same block as fit the time budget."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from oracle import oracle as O
+    from helpers import oracle_load_to_fold
+    rate = abs(w["bw"]) / w["nchan"] * 1e6
+    step = geom["step"]
+    npart = (w["ndat"] - geom["noverlap"]) // step
+    import importlib as il
+    dspx = il.import_module("dspsr-kanyixia_amd")
+    chunks = dspx.fold_chunks(npart * step, rate, w["period"], geom["impulse_pos"], nsub=w["nsub"])
+    pps = (1.0 / rate) / w["period"]
+    done, t_total = 0, 0.0
+    # channels are independent; time one at a time until the budget is spent
+    while done < w["nchan"] and t_total < budget_s:
+        sub = raw_np[:, done:done + 1]
+        sub = sub.copy()
+        t0 = time.perf_counter()
+        oracle_load_to_fold(O, sub, w["nbit"], 1, w["npol"], w["ndat"], w["cfreq"], w["bw"], w["dm"], geom,
+                            w["state"], w["ndim"], w["nbin"], pps, chunks, w["nsub"])
+        t_total += time.perf_counter() - t0
+        done += 1
+    samples = done * w["npol"] * w["ndat"]
+    return {"value": samples / t_total / 1e6, "unit": "Msamples/s", "cores": 1, "kind": "port",
+            "sample": "%d of %d channels of the same block (%d pol x %d samples each), oracle chain "
+                      "unpack+convolution+detection+fold, %.1f s" % (done, w["nchan"], w["npol"], w["ndat"],
+                                                                     t_total)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="multichan", choices=list(WORKLOADS))
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--cpu-budget", type=float, default=20.0)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: this path has no multi-GPU form here "
+              "(channels and beams shard without exchange; see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    dspx = importlib.import_module("dspsr-kanyixia_amd")
+    synth = importlib.import_module("dspsr-kanyixia_amd.synth")
+    w = WORKLOADS[args.workload]
+    torch.cuda.init()
+    ctx = dspx.Context(0, torch.cuda.current_stream().cuda_stream)
+    geom = dspx.geometry(w["cfreq"], w["bw"], w["nchan"], w["dm"])
+    print("[bench] workload %s geometry %s" % (args.workload, geom), file=sys.stderr, flush=True)
+    raw = synth.make_baseband_torch(w["nchan"], w["npol"], w["ndat"], w["cfreq"], w["bw"], w["dm"], w["period"],
+                                    w["duty"], w["peak"], sigma=w["sigma"], nbit=w["nbit"], seed=1)
+    torch.cuda.synchronize()
+    print("[bench] input generated", file=sys.stderr, flush=True)
+    nsamp = w["nchan"] * w["npol"] * w["ndat"]
+
+    t_f, st_f, prof_f, hits_f, npart = time_path(dspx, ctx, w, geom, raw, dspx.FORM_FUSED, args.steps, args.warmup)
+    print("[bench] fused %.3f ms/step stages %s" % (t_f * 1e3, st_f), file=sys.stderr, flush=True)
+    plain = None
+    if not args.no_plain:
+        sp = max(3, args.steps // 4)
+        t_p, st_p, prof_p, hits_p, _ = time_path(dspx, ctx, w, geom, raw, dspx.FORM_PLAIN, sp, 1)
+        nf, npl = args.warmup + args.steps + 3, 1 + sp + 3  # executions accumulated in each
+        pf, pp = prof_f / nf, prof_p / npl
+        plain = {"value": nsamp / t_p / 1e6, "ms_per_step": t_p * 1e3, "stages_ms": st_p,
+                 "profile_rel_rms_vs_fused": float(np.sqrt(np.sum((pf - pp) ** 2) / np.sum(pp ** 2))),
+                 "hits_equal": bool(np.array_equal(hits_f // nf, hits_p // npl))}
+        print("[bench] plain %.3f ms/step stages %s" % (t_p * 1e3, st_p), file=sys.stderr, flush=True)
+
+    value = nsamp / t_f / 1e6
+    abytes = algorithmic_bytes(w, geom, npart)
+    dom = max(st_f, key=lambda k: st_f[k] if k in abytes else -1)
+    ach = abytes[dom] / (st_f[dom] * 1e-3) / 1e9
+    total_alg = sum(abytes.values())
+    r = geom["nfft"] / geom["step"]
+    roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom,
+            "kernel_ms": round(st_f[dom], 4), "algorithmic_bytes_per_launch": int(abytes[dom]),
+            "path_achieved": round(total_alg / t_f / 1e9, 1),
+            "path_frac": round(total_alg / t_f / 1e9 / HBM_PEAK_GBS, 4),
+            "bytes_per_sample": round(total_alg / nsamp, 3), "r": round(r, 5)}
+    out = {"metric": METRIC, "value": round(value, 2), "unit": "Msamples/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": round(t_f * 1e3, 4), "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+           "config": {"workload": args.workload + ": %d chan x %d pol x %d complex int%d, %g MHz at %g MHz, DM %g, "
+                                                 "state %d ndim %d, %d bins x %d subints" % (
+                                                     w["nchan"], w["npol"], w["ndat"], w["nbit"], w["bw"], w["cfreq"],
+                                                     w["dm"], w["state"], w["ndim"], w["nbin"], w["nsub"]),
+                      "nfft": geom["nfft"], "noverlap": geom["noverlap"], "npart": npart, "form": "fused"},
+           "roofline": roof, "stages_ms": {k: round(v, 4) for k, v in st_f.items()}}
+    if plain is not None:
+        out["plain_fp32"] = {"value": round(plain["value"], 2), "ms_per_step": round(plain["ms_per_step"], 4),
+                             "stages_ms": {k: round(v, 4) for k, v in plain["stages_ms"].items()},
+                             "profile_rel_rms_vs_fused": plain["profile_rel_rms_vs_fused"],
+                             "hits_equal": plain["hits_equal"]}
+        out["fused_vs_plain"] = round(value / plain["value"], 3)
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(w, geom, raw.cpu().numpy(), args.cpu_budget)
+    print(json.dumps(out))
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,321 @@
+"""dspx -- MI355X-native coherent dedispersion and folding for dspsr pipelines.
+
+Python bindings of the C ABI in include/dspx.h (libdspx.so, built in this
+directory by the Makefile).  Device buffers are plain device pointers; torch
+tensors on a HIP device are accepted wherever a pointer is expected.
+
+The compute path is the HIP library only: if libdspx.so is missing every call
+raises, there is no CPU fallback.
+"""
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libdspx.so")
+_lib = None
+
+OK, E_PARAM, E_STATE, E_HIP, E_UNSUPPORTED, E_INTERNAL = range(6)
+CHIRP_TABLE, CHIRP_PARITY, CHIRP_TRUTH = 0, 1, 2
+FORM_FUSED, FORM_PLAIN = 0, 1
+INTENSITY, PPQQ, COHERENCE, STOKES = 0, 1, 2, 3
+
+
+class DspxError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("dspx error %d: %s" % (code, msg))
+        self.code = code
+
+
+class DedispDesc(C.Structure):
+    _fields_ = [("centre_frequency", C.c_double), ("bandwidth", C.c_double),
+                ("dispersion_measure", C.c_double), ("nchan", C.c_uint32),
+                ("dc_centred", C.c_uint32), ("nfft", C.c_uint32), ("reserved", C.c_uint32)]
+
+
+class Geometry(C.Structure):
+    _fields_ = [("impulse_neg", C.c_uint32), ("impulse_pos", C.c_uint32), ("nfft", C.c_uint32),
+                ("nsamp_overlap", C.c_uint32), ("nsamp_step", C.c_uint32),
+                ("nunsupported", C.c_uint32)]
+
+
+class ConvDesc(C.Structure):
+    _fields_ = [("dedisp", DedispDesc), ("nchan", C.c_uint32), ("npol", C.c_uint32),
+                ("nfft", C.c_uint32), ("nfilt_pos", C.c_uint32), ("nfilt_neg", C.c_uint32),
+                ("chirp_mode", C.c_uint32), ("form", C.c_uint32), ("reserved", C.c_uint32)]
+
+
+class FoldChunk(C.Structure):
+    _fields_ = [("idat_start", C.c_uint64), ("ndat", C.c_uint64), ("phi", C.c_double),
+                ("isub", C.c_uint32), ("reserved", C.c_uint32)]
+
+
+class LtfDesc(C.Structure):
+    _fields_ = [("nbit", C.c_uint32), ("offset_binary", C.c_uint32), ("scale", C.c_float),
+                ("nchan", C.c_uint32), ("npol", C.c_uint32), ("reserved0", C.c_uint32),
+                ("ndat", C.c_uint64), ("dedisp", DedispDesc), ("nfft", C.c_uint32),
+                ("nfilt_pos", C.c_uint32), ("nfilt_neg", C.c_uint32), ("chirp_mode", C.c_uint32),
+                ("state", C.c_uint32), ("ndim", C.c_uint32), ("nbin", C.c_uint32),
+                ("form", C.c_uint32)]
+
+
+P = C.c_void_p
+U32, U64, I32, D = C.c_uint32, C.c_uint64, C.c_int, C.c_double
+
+_SIGS = {
+    "dspx_version": (C.c_char_p, []),
+    "dspx_ctx_create": (I32, [I32, P, C.POINTER(P)]),
+    "dspx_ctx_destroy": (None, [P]),
+    "dspx_last_error": (C.c_char_p, [P]),
+    "dspx_ctx_synchronize": (I32, [P]),
+    "dspx_ctx_stream": (P, [P]),
+    "dspx_dedisp_geometry": (I32, [C.POINTER(DedispDesc), C.POINTER(Geometry)]),
+    "dspx_unpack": (I32, [P, P, U32, U32, U32, U32, U64, C.c_float, P, U64, U64]),
+    "dspx_conv_create": (I32, [P, C.POINTER(ConvDesc), C.POINTER(P)]),
+    "dspx_conv_destroy": (None, [P]),
+    "dspx_conv_set_response": (I32, [P, P]),
+    "dspx_conv_perform": (I32, [P, P, U64, U64, U64, P, U64, U64, P, U32]),
+    "dspx_detect": (I32, [P, U32, U32, P, U64, U64, U32, U32, U64, P, U64, U64]),
+    "dspx_fold_create": (I32, [P, U64, D, U32, C.POINTER(FoldChunk), U32, C.POINTER(P)]),
+    "dspx_fold_destroy": (None, [P]),
+    "dspx_fold_binplan": (I32, [P, P]),
+    "dspx_fold_binplan_host": (I32, [D, D, U32, U64, P]),
+    "dspx_fold_nsegments_host": (I32, [D, D, U64, P]),
+    "dspx_fold_hits": (I32, [P, P]),
+    "dspx_fold_execute": (I32, [P, P, U64, U64, U32, U32, U32, P]),
+    "dspx_ltf_create": (I32, [P, C.POINTER(LtfDesc), C.POINTER(P)]),
+    "dspx_ltf_destroy": (None, [P]),
+    "dspx_ltf_geometry": (I32, [P, C.POINTER(U64), C.POINTER(U64)]),
+    "dspx_ltf_set_fold": (I32, [P, D, C.POINTER(FoldChunk), U32]),
+    "dspx_ltf_execute": (I32, [P, P, P, P]),
+    "dspx_ltf_filterbank": (I32, [P, P, P]),
+    "dspx_ltf_set_timing": (I32, [P, I32]),
+    "dspx_ltf_stage_times": (I32, [P, P, P, I32, C.POINTER(I32)]),
+}
+
+
+def lib():
+    """The loaded libdspx.so; raises when the HIP library has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError("libdspx.so not built (run __graft_entry__.build() or make -C %s)" % _HERE)
+    L = C.CDLL(LIB_PATH)
+    for name, (res, args) in _SIGS.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def exported_symbols():
+    return sorted(_SIGS)
+
+
+def ptr(x):
+    """Device pointer of a torch tensor, or an int / None passed through."""
+    if x is None:
+        return None
+    if isinstance(x, int):
+        return x
+    return x.data_ptr()
+
+
+def _check(rc, ctx=None):
+    if rc != OK:
+        msg = lib().dspx_last_error(ctx).decode() if ctx else "(no context)"
+        raise DspxError(rc, msg)
+
+
+def geometry(centre_frequency, bandwidth, nchan, dm, nfft=0, dc_centred=False):
+    """Convolution geometry by the reference rule (host only)."""
+    d = DedispDesc(centre_frequency, bandwidth, dm, nchan, int(dc_centred), nfft, 0)
+    g = Geometry()
+    rc = lib().dspx_dedisp_geometry(C.byref(d), C.byref(g))
+    if rc != OK:
+        raise DspxError(rc, "geometry: smearing exceeds the limit or nfft too small")
+    return {"impulse_neg": g.impulse_neg, "impulse_pos": g.impulse_pos, "nfft": g.nfft,
+            "noverlap": g.nsamp_overlap, "step": g.nsamp_step, "nunsupported": g.nunsupported}
+
+
+def binplan_host(phi, pps, nbin, ndat):
+    import numpy as np
+    out = np.zeros(max(ndat, 1), np.uint32)
+    _check(lib().dspx_fold_binplan_host(phi, pps, nbin, ndat, out.ctypes.data))
+    return out[:ndat]
+
+
+class Context:
+    def __init__(self, device=0, stream=None):
+        h = P()
+        _check(lib().dspx_ctx_create(device, stream, C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            lib().dspx_ctx_destroy(self.h)
+            self.h = None
+
+    __del__ = close
+
+    def check(self, rc):
+        _check(rc, self.h)
+
+    def synchronize(self):
+        self.check(lib().dspx_ctx_synchronize(self.h))
+
+    @property
+    def stream(self):
+        return lib().dspx_ctx_stream(self.h)
+
+    # ---- engines
+    def unpack(self, raw, nbit, nchan, npol, ndat, out, chan_stride, pol_stride, scale=1.0,
+               offset_binary=False):
+        self.check(lib().dspx_unpack(self.h, ptr(raw), nbit, int(offset_binary), nchan, npol, ndat,
+                                     scale, ptr(out), chan_stride, pol_stride))
+
+    def detect(self, state, ndim, x, in_chan_stride, in_pol_stride, nchan, npol, ndat, out,
+               out_chan_stride, out_pol_stride):
+        self.check(lib().dspx_detect(self.h, state, ndim, ptr(x), in_chan_stride, in_pol_stride,
+                                     nchan, npol, ndat, ptr(out), out_chan_stride, out_pol_stride))
+
+
+def _dedisp(centre_frequency, bandwidth, dm, nchan, dc_centred=False):
+    return DedispDesc(centre_frequency, bandwidth, dm, nchan, int(dc_centred), 0, 0)
+
+
+class Convolution:
+    """Convolution::Engine (fused or plain form)."""
+
+    def __init__(self, ctx, nchan, npol, nfft, nfilt_pos, nfilt_neg, chirp_mode=CHIRP_PARITY,
+                 form=FORM_FUSED, centre_frequency=0.0, bandwidth=0.0, dm=0.0, dc_centred=False):
+        self.ctx = ctx
+        d = ConvDesc(_dedisp(centre_frequency, bandwidth, dm, nchan, dc_centred), nchan, npol, nfft,
+                     nfilt_pos, nfilt_neg, chirp_mode, form, 0)
+        h = P()
+        ctx.check(lib().dspx_conv_create(ctx.h, C.byref(d), C.byref(h)))
+        self.h = h
+        self.step = nfft - nfilt_pos - nfilt_neg
+
+    def close(self):
+        if self.h:
+            lib().dspx_conv_destroy(self.h)
+            self.h = None
+
+    __del__ = close
+
+    def set_response(self, resp):
+        import numpy as np
+        r = np.ascontiguousarray(resp, np.complex64)
+        self.ctx.check(lib().dspx_conv_set_response(self.h, r.ctypes.data))
+
+    def perform(self, x, in_chan_stride, in_pol_stride, ndat, out, out_chan_stride, out_pol_stride,
+                npart, zero_dm=None):
+        self.ctx.check(lib().dspx_conv_perform(self.h, ptr(x), in_chan_stride, in_pol_stride, ndat,
+                                               ptr(out), out_chan_stride, out_pol_stride,
+                                               ptr(zero_dm), npart))
+
+
+class Fold:
+    """Fold::Engine: device bin plan + accumulation."""
+
+    def __init__(self, ctx, ndat, pps, nbin, chunks):
+        self.ctx = ctx
+        arr = (FoldChunk * len(chunks))(*[FoldChunk(c[0], c[1], c[2], c[3], 0) for c in chunks])
+        h = P()
+        ctx.check(lib().dspx_fold_create(ctx.h, ndat, pps, nbin, arr, len(chunks), C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            lib().dspx_fold_destroy(self.h)
+            self.h = None
+
+    __del__ = close
+
+    def binplan(self, out):
+        self.ctx.check(lib().dspx_fold_binplan(self.h, ptr(out)))
+
+    def hits(self, out):
+        self.ctx.check(lib().dspx_fold_hits(self.h, ptr(out)))
+
+    def execute(self, x, chan_stride, pol_stride, nchan, npol, ndim, prof):
+        self.ctx.check(lib().dspx_fold_execute(self.h, ptr(x), chan_stride, pol_stride, nchan, npol,
+                                               ndim, ptr(prof)))
+
+
+class LoadToFold:
+    """Unpack -> Convolution -> Detection -> Fold on one block, fused or plain."""
+
+    def __init__(self, ctx, nbit, nchan, npol, ndat, centre_frequency, bandwidth, dm, nbin,
+                 state=INTENSITY, ndim=1, chirp_mode=CHIRP_PARITY, form=FORM_FUSED, nfft=0,
+                 scale=1.0, offset_binary=False, geom=None):
+        self.ctx = ctx
+        g = geom or geometry(centre_frequency, bandwidth, nchan, dm, nfft)
+        self.geom = g
+        d = LtfDesc(nbit, int(offset_binary), scale, nchan, npol, 0, ndat,
+                    _dedisp(centre_frequency, bandwidth, dm, nchan), g["nfft"], g["impulse_pos"],
+                    g["impulse_neg"], chirp_mode, state, ndim, nbin, form)
+        h = P()
+        ctx.check(lib().dspx_ltf_create(ctx.h, C.byref(d), C.byref(h)))
+        self.h = h
+        npart, nout = U64(), U64()
+        lib().dspx_ltf_geometry(h, C.byref(npart), C.byref(nout))
+        self.npart, self.ndat_out = npart.value, nout.value
+        self.nbin, self.state, self.ndim, self.nchan, self.npol = nbin, state, ndim, nchan, npol
+        self.nsub = 1
+
+    def close(self):
+        if self.h:
+            lib().dspx_ltf_destroy(self.h)
+            self.h = None
+
+    __del__ = close
+
+    @property
+    def nval(self):
+        return {INTENSITY: 1, PPQQ: self.npol}.get(self.state, 4)
+
+    def set_fold(self, pps, chunks):
+        arr = (FoldChunk * len(chunks))(*[FoldChunk(c[0], c[1], c[2], c[3], 0) for c in chunks])
+        self.ctx.check(lib().dspx_ltf_set_fold(self.h, pps, arr, len(chunks)))
+        self.nsub = max(c[3] for c in chunks) + 1
+
+    def execute(self, raw, prof, hits=None):
+        self.ctx.check(lib().dspx_ltf_execute(self.h, ptr(raw), ptr(prof), ptr(hits)))
+
+    def filterbank(self, raw, out):
+        self.ctx.check(lib().dspx_ltf_filterbank(self.h, ptr(raw), ptr(out)))
+
+    def set_timing(self, on=True):
+        self.ctx.check(lib().dspx_ltf_set_timing(self.h, int(on)))
+
+    def stage_times(self):
+        ms = (C.c_float * 8)()
+        names = (C.c_char_p * 8)()
+        n = I32()
+        self.ctx.check(lib().dspx_ltf_stage_times(self.h, ms, names, 8, C.byref(n)))
+        return [(names[i].decode(), ms[i]) for i in range(n.value)]
+
+
+def fold_phi(t_seconds, period, reference_phase=0.0):
+    """Fold::get_phi for a constant folding period (Fold.C:975-982), with the
+    reference epoch at t = 0."""
+    import math
+    return math.fmod(t_seconds, period) / period - reference_phase
+
+
+def fold_chunks(ndat_out, rate, period, nfilt_pos, nsub=1, t0=0.0, reference_phase=0.0):
+    """Sub-integration chunks of one block of convolution output: output
+    sample i is input sample nfilt_pos + i (Convolution.C:611); the block is
+    divided into nsub equal spans (boundaries rounded with rint, as
+    TimeDivide does, TimeDivide.C:193-304).  Returns [(idat_start, ndat, phi,
+    isub)] with phi of the midpoint of each chunk's first sample."""
+    chunks = []
+    bounds = [int(round(ndat_out * k / nsub)) for k in range(nsub + 1)]
+    for k in range(nsub):
+        i0, i1 = bounds[k], bounds[k + 1]
+        t = t0 + (nfilt_pos + i0 + 0.5) / rate
+        chunks.append((i0, i1 - i0, fold_phi(t, period, reference_phase), k))
+    return chunks

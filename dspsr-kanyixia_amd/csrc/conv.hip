@@ -1,0 +1,610 @@
+// conv.hip -- Convolution::Engine (fused three-stage form) and the fused
+// LoadToFold pipeline (unpack + convolution + detection + fold in three
+// kernels).
+//
+// Reference: Convolution::transformation (Signal/General/Convolution.C:646-813),
+// CUDA::ConvolutionEngine::perform (ConvolutionCUDA.cu:567-770), the LoadToFold
+// chain (Signal/Pulsar/LoadToFold1.C:109-258).
+#include <algorithm>
+#include <cmath>
+
+#include "dspx_internal.hpp"
+#include "fold_impl.hpp"
+#include "plain.hpp"
+#include "stages.hpp"
+
+namespace dspx {
+
+// N = N1 * N2 with N2 <= 4096 (one LDS row) and N1 <= 256 (column tile).
+static bool split_n(uint32_t N, uint32_t& N1, uint32_t& N2, uint32_t& m)
+{
+  if (N == 0 || (N & (N - 1))) return false;
+  m = 0;
+  while ((1u << m) < N) m++;
+  if (m < 12 || m > 20) return false;
+  if (m >= 16) {
+    N1 = 1u << (m - 12);
+    N2 = 4096;
+  } else {
+    N1 = 16;
+    N2 = 1u << (m - 4);
+  }
+  return true;
+}
+
+// Everything the three stages share for one geometry.
+struct ConvCore {
+  dspx_ctx* ctx = nullptr;
+  uint32_t nchan = 0, npol = 0, N = 0, N1 = 0, N2 = 0, m = 0;
+  uint32_t nfilt_pos = 0, nfilt_neg = 0, step = 0;
+  uint32_t chirp_mode = 0;
+  dspx_dedisp_desc dd{};
+  DevBuf tw2;      // lo[2^h] then hi[N >> h]
+  int h = 0;
+  DevBuf chan_par; // coeff[nchan], cf[nchan]
+  DevBuf table;    // [nchan][N1][N2]
+  DevBuf work;     // [nchan][npart][npol][N]
+  uint32_t work_npart = 0;
+  double binwidth = 0, half_cw = 0;
+  float snorm = 0;
+  bool have_table = false;
+
+  void init(dspx_ctx* c, uint32_t nchan_, uint32_t npol_, uint32_t N_, uint32_t pos, uint32_t neg,
+            uint32_t mode, const dspx_dedisp_desc& d)
+  {
+    ctx = c;
+    nchan = nchan_;
+    npol = npol_;
+    N = N_;
+    nfilt_pos = pos;
+    nfilt_neg = neg;
+    chirp_mode = mode;
+    dd = d;
+    DSPX_REQUIRE(npol == 1 || npol == 2, DSPX_E_PARAM, "npol=%u", npol);
+    DSPX_REQUIRE(nchan >= 1, DSPX_E_PARAM, "nchan=0");
+    DSPX_REQUIRE(split_n(N, N1, N2, m), DSPX_E_UNSUPPORTED,
+                 "nfft=%u: the fused form supports powers of two 2^12..2^20", N);
+    DSPX_REQUIRE(uint64_t(pos) + neg < N, DSPX_E_STATE, "error nfft=%u < nfilt=%u", N, pos + neg);
+    step = N - pos - neg;
+    DSPX_REQUIRE(mode <= DSPX_CHIRP_TRUTH, DSPX_E_PARAM, "chirp_mode=%u", mode);
+    // two-level twiddle table W_N^m = lo[m & (2^h-1)] * hi[m >> h]
+    h = (int(m) + 1) / 2;
+    uint32_t nlo = 1u << h, nhi = N >> h;
+    std::vector<float2> t(nlo + nhi);
+    for (uint32_t i = 0; i < nlo; i++) {
+      double a = -2.0 * M_PI * double(i) / double(N);
+      t[i] = make_float2(float(std::cos(a)), float(std::sin(a)));
+    }
+    for (uint32_t i = 0; i < nhi; i++) {
+      double a = -2.0 * M_PI * double(uint64_t(i) << h) / double(N);
+      t[nlo + i] = make_float2(float(std::cos(a)), float(std::sin(a)));
+    }
+    tw2.reserve(t.size() * sizeof(float2));
+    DSPX_HIP(hipMemcpy(tw2.p, t.data(), t.size() * sizeof(float2), hipMemcpyHostToDevice));
+    snorm = float(1.0 / std::sqrt(double(N)));  // Convolution.C:401-414
+    if (mode != DSPX_CHIRP_TABLE) {
+      DSPX_REQUIRE(d.nchan == nchan, DSPX_E_PARAM, "dedisp.nchan=%u != nchan=%u", d.nchan, nchan);
+      DSPX_REQUIRE(d.bandwidth != 0.0 && d.centre_frequency > 0.0, DSPX_E_PARAM, "invalid band");
+      double chanwidth = d.bandwidth / double(nchan);
+      binwidth = chanwidth / double(N);
+      half_cw = 0.5 * chanwidth;
+      std::vector<double> pc(2 * nchan);
+      for (uint32_t ic = 0; ic < nchan; ic++) {
+        double cf = channel_centre(d.centre_frequency, d.bandwidth, nchan, d.dc_centred, ic);
+        pc[ic] = chirp_coeff(d.bandwidth, d.dispersion_measure, cf);
+        pc[nchan + ic] = cf;
+      }
+      chan_par.reserve(pc.size() * sizeof(double));
+      DSPX_HIP(hipMemcpy(chan_par.p, pc.data(), pc.size() * sizeof(double), hipMemcpyHostToDevice));
+    }
+  }
+
+  void set_table(const float* host)
+  {
+    // transpose bin k = k1 + N1*k2 to [k1][k2] so stage B reads rows
+    std::vector<float2> t(size_t(nchan) * N);
+    const float2* hp = reinterpret_cast<const float2*>(host);
+    for (uint32_t ic = 0; ic < nchan; ic++)
+      for (uint32_t k1 = 0; k1 < N1; k1++)
+        for (uint32_t k2 = 0; k2 < N2; k2++)
+          t[(size_t(ic) * N1 + k1) * N2 + k2] = hp[size_t(ic) * N + k1 + size_t(N1) * k2];
+    table.reserve(t.size() * sizeof(float2));
+    DSPX_HIP(hipMemcpy(table.p, t.data(), t.size() * sizeof(float2), hipMemcpyHostToDevice));
+    have_table = true;
+  }
+
+  void reserve_work(uint32_t npart)
+  {
+    work.reserve(size_t(nchan) * npart * npol * N * sizeof(float2));
+    work_npart = npart;
+  }
+
+  ConvShape shape(uint32_t npart) const
+  {
+    ConvShape s;
+    s.work = work.as<float2>();
+    s.N1 = N1;
+    s.N2 = N2;
+    s.log2N = m;
+    s.nchan = nchan;
+    s.npol = npol;
+    s.npart = npart;
+    s.step = step;
+    s.nfilt_pos = nfilt_pos;
+    s.tw.lo = tw2.as<float2>();
+    s.tw.hi = tw2.as<float2>() + (1u << h);
+    s.tw.h = h;
+    return s;
+  }
+
+  ChirpArgs chirp() const
+  {
+    ChirpArgs c;
+    c.mode = chirp_mode;
+    c.snorm = snorm;
+    c.binwidth = binwidth;
+    c.half_cw = half_cw;
+    c.coeff = chan_par.as<double>();
+    c.cf = chan_par.as<double>() ? chan_par.as<double>() + nchan : nullptr;
+    c.table = table.as<float2>();
+    return c;
+  }
+
+  uint32_t ncb() const { return N2 / (kColTile / N1); }
+
+  template <class LD> void stage_a(const LD& ld, uint32_t npart)
+  {
+    ConvShape s = shape(npart);
+    uint64_t nb = uint64_t(ncb()) * npart * nchan;
+    DSPX_REQUIRE(nb < (1ull << 31), DSPX_E_UNSUPPORTED, "grid too large");
+    const float2* twc = twiddles(ctx, N1);
+    switch (N1) {
+#define CASE_A(n) \
+  case n: k_stageA<n, LD><<<unsigned(nb), kColThreads, 0, ctx->stream>>>(ld, s, twc); break;
+      CASE_A(16) CASE_A(32) CASE_A(64) CASE_A(128) CASE_A(256)
+#undef CASE_A
+      default: fail(DSPX_E_UNSUPPORTED, "N1=%u", N1);
+    }
+    DSPX_HIP(hipGetLastError());
+  }
+
+  void stage_b(uint32_t npart)
+  {
+    DSPX_REQUIRE(chirp_mode != DSPX_CHIRP_TABLE || have_table, DSPX_E_STATE,
+                 "no response: call dspx_conv_set_response first");
+    ConvShape s = shape(npart);
+    ChirpArgs c = chirp();
+    uint64_t nb = uint64_t(N1) * npart * nchan;
+    DSPX_REQUIRE(nb < (1ull << 31), DSPX_E_UNSUPPORTED, "grid too large");
+    const float2* twr = twiddles(ctx, N2);
+    switch (N2) {
+#define CASE_B(n) \
+  case n: k_stageB<n><<<unsigned(nb), n / 16, 0, ctx->stream>>>(s, c, twr); break;
+      CASE_B(256) CASE_B(512) CASE_B(1024) CASE_B(2048) CASE_B(4096)
+#undef CASE_B
+      default: fail(DSPX_E_UNSUPPORTED, "N2=%u", N2);
+    }
+    DSPX_HIP(hipGetLastError());
+  }
+
+  template <int MODE> void stage_c(uint32_t npart, const FilterbankArgs& fb, const FoldArgs& fa)
+  {
+    ConvShape s = shape(npart);
+    uint32_t G = MODE == kStageCFold ? fa.G : ncb();
+    uint64_t nb = uint64_t(G) * npart * nchan;
+    DSPX_REQUIRE(nb < (1ull << 31), DSPX_E_UNSUPPORTED, "grid too large");
+    size_t shm = kColTile * sizeof(float2);
+    if (MODE == kStageCFold) shm += size_t(fa.nslot) * fa.nval * fa.nbin * sizeof(float);
+    DSPX_REQUIRE(shm <= 160 * 1024, DSPX_E_UNSUPPORTED, "LDS request %zu too large", shm);
+    const float2* twc = twiddles(ctx, N1);
+    switch (N1) {
+#define CASE_C(n)                                                                  \
+  case n:                                                                          \
+    if (shm > 64 * 1024)                                                           \
+      DSPX_HIP(hipFuncSetAttribute((const void*) k_stageC<n, MODE>,                \
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, int(shm))); \
+    k_stageC<n, MODE><<<unsigned(nb), kColThreads, shm, ctx->stream>>>(s, fb, fa, twc);   \
+    break;
+      CASE_C(16) CASE_C(32) CASE_C(64) CASE_C(128) CASE_C(256)
+#undef CASE_C
+      default: fail(DSPX_E_UNSUPPORTED, "N1=%u", N1);
+    }
+    DSPX_HIP(hipGetLastError());
+  }
+};
+
+// zero-DM copy (Convolution.C:792-799)
+__global__ void k_zero_dm(const float* __restrict__ in, uint64_t ics, uint64_t ips, float* __restrict__ out,
+                          uint64_t ocs, uint64_t ops, uint32_t npol, uint32_t step, uint32_t nfilt_pos,
+                          uint64_t nout)
+{
+  uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  uint32_t cp = blockIdx.y;
+  if (i >= nout) return;
+  uint32_t chan = cp / npol, pol = cp % npol;
+  uint64_t p = i / step, j = i % step;
+  const float* x = in + chan * ics + pol * ips + 2 * (p * step + nfilt_pos + j);
+  float* o = out + chan * ocs + pol * ops + 2 * i;
+  *reinterpret_cast<float2*>(o) = *reinterpret_cast<const float2*>(x);
+}
+
+// Sum the fused stage C partial histograms into the profiles:
+// prof[isub][chan][v] += sum over parts p touching isub and groups g.
+__global__ void k_fold_reduce(const float* __restrict__ partial, const uint32_t* __restrict__ sub_first,
+                              uint32_t npart, uint32_t G, uint32_t nslot, uint32_t hl, uint32_t nchan,
+                              uint32_t nsub, float* __restrict__ prof)
+{
+  uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t chan = blockIdx.y;
+  uint32_t isub = blockIdx.z;
+  if (e >= hl) return;
+  float acc = 0.f;
+  bool any = false;
+  for (uint32_t p = 0; p < npart; p++) {
+    uint32_t s0 = sub_first[p];
+    if (isub < s0 || isub >= s0 + nslot) continue;
+    const float* base = partial + ((uint64_t(chan) * npart + p) * G) * nslot * hl + uint64_t(isub - s0) * hl + e;
+    for (uint32_t g = 0; g < G; g++) acc += base[uint64_t(g) * nslot * hl];
+    any = true;
+  }
+  if (any) prof[(uint64_t(isub) * nchan + chan) * hl + e] += acc;
+}
+
+}  // namespace dspx
+
+using namespace dspx;
+
+// ------------------------------------------------------- Convolution engine
+
+struct dspx_conv {
+  dspx_ctx* ctx = nullptr;
+  dspx_conv_desc d{};
+  ConvCore core;
+  PlainConv plain;
+};
+
+extern "C" {
+
+int dspx_conv_create(dspx_ctx* ctx, const dspx_conv_desc* d, dspx_conv** out)
+{
+  return guard(ctx, [&] {
+    DSPX_REQUIRE(ctx && d && out, DSPX_E_PARAM, "null argument");
+    DSPX_HIP(hipSetDevice(ctx->device));
+    auto* c = new dspx_conv;
+    c->ctx = ctx;
+    c->d = *d;
+    try {
+      if (d->form == DSPX_FORM_PLAIN)
+        c->plain.init(ctx, d->nchan, d->npol, d->nfft, d->nfilt_pos, d->nfilt_neg, d->chirp_mode, d->dedisp);
+      else if (d->form == DSPX_FORM_FUSED)
+        c->core.init(ctx, d->nchan, d->npol, d->nfft, d->nfilt_pos, d->nfilt_neg, d->chirp_mode, d->dedisp);
+      else
+        fail(DSPX_E_PARAM, "form=%u", d->form);
+    } catch (...) {
+      delete c;
+      throw;
+    }
+    *out = c;
+  });
+}
+
+void dspx_conv_destroy(dspx_conv* c)
+{
+  if (!c) return;
+  (void) hipSetDevice(c->ctx->device);
+  (void) hipStreamSynchronize(c->ctx->stream);
+  delete c;
+}
+
+int dspx_conv_set_response(dspx_conv* c, const float* host)
+{
+  dspx_ctx* ctx = c ? c->ctx : nullptr;
+  return guard(ctx, [&] {
+    DSPX_REQUIRE(c && host, DSPX_E_PARAM, "null argument");
+    if (c->d.form == DSPX_FORM_PLAIN)
+      c->plain.set_table(host);
+    else
+      c->core.set_table(host);
+  });
+}
+
+int dspx_conv_perform(dspx_conv* c, const float* d_in, uint64_t ics, uint64_t ips, uint64_t ndat, float* d_out,
+                      uint64_t ocs, uint64_t ops, float* d_zdm, uint32_t npart)
+{
+  dspx_ctx* ctx = c ? c->ctx : nullptr;
+  return guard(ctx, [&] {
+    DSPX_REQUIRE(c && d_in && d_out, DSPX_E_PARAM, "null argument");
+    const uint32_t N = c->d.nfft, nov = c->d.nfilt_pos + c->d.nfilt_neg, step = N - nov;
+    uint64_t max_part = ndat < N ? 0 : (ndat - nov) / step;
+    DSPX_REQUIRE(npart <= max_part, DSPX_E_PARAM, "npart=%u > %llu parts available in ndat=%llu", npart,
+                 (unsigned long long) max_part, (unsigned long long) ndat);
+    if (npart == 0) return;
+    if (c->d.form == DSPX_FORM_PLAIN) {
+      c->plain.perform(d_in, ics, ips, d_out, ocs, ops, npart);
+    } else {
+      ConvCore& k = c->core;
+      k.reserve_work(npart);
+      LoadF32 ld{d_in, ics, ips, k.npol};
+      k.stage_a(ld, npart);
+      k.stage_b(npart);
+      FilterbankArgs fb{d_out, ocs, ops};
+      k.stage_c<kStageCFilterbank>(npart, fb, FoldArgs{});
+    }
+    if (d_zdm) {
+      uint64_t nout = uint64_t(npart) * step;
+      dim3 grid(unsigned((nout + 255) / 256), c->d.nchan * c->d.npol);
+      k_zero_dm<<<grid, 256, 0, ctx->stream>>>(d_in, ics, ips, d_zdm, ocs, ops, c->d.npol, step, c->d.nfilt_pos,
+                                               nout);
+      DSPX_HIP(hipGetLastError());
+    }
+  });
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------ fused LoadToFold
+
+struct dspx_ltf {
+  dspx_ctx* ctx = nullptr;
+  dspx_ltf_desc d{};
+  uint64_t npart = 0, ndat_out = 0;
+  uint32_t step = 0;
+  uint32_t nval = 1, npol_out = 1, ndim_out = 1;
+  ConvCore core;
+  PlainConv plain;
+  FoldPlan fold;
+  bool have_fold = false;
+  // fused fold partials
+  uint32_t G = 1, nslot = 1;
+  DevBuf sub_first;
+  DevBuf partial;
+  // plain form intermediates
+  DevBuf unpacked, filtered, detected;
+  // timing
+  bool timing = false;
+  int nstage = 0;
+  const char* stage_name[8] = {};
+};
+
+namespace {
+
+void ltf_record(dspx_ltf* L, int i)
+{
+  if (L->timing) DSPX_HIP(hipEventRecord(L->ctx->ev[i], L->ctx->stream));
+}
+
+template <class F> void with_loader(dspx_ltf* L, const void* raw, F&& f)
+{
+  const auto& d = L->d;
+  if (d.nbit == 8) {
+    LoadI8 ld{static_cast<const uint8_t*>(raw), uint64_t(d.nchan) * d.npol * 2, d.npol, d.offset_binary, d.scale};
+    f(ld);
+  } else {
+    LoadI16 ld{static_cast<const int16_t*>(raw), uint64_t(d.nchan) * d.npol * 2, d.npol, d.offset_binary};
+    f(ld);
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int dspx_ltf_create(dspx_ctx* ctx, const dspx_ltf_desc* d, dspx_ltf** out)
+{
+  return guard(ctx, [&] {
+    DSPX_REQUIRE(ctx && d && out, DSPX_E_PARAM, "null argument");
+    DSPX_REQUIRE(d->nbit == 8 || d->nbit == 16, DSPX_E_PARAM, "nbit=%u", d->nbit);
+    DSPX_REQUIRE(d->npol == 1 || d->npol == 2, DSPX_E_PARAM, "npol=%u", d->npol);
+    DSPX_REQUIRE(d->state <= DSPX_STOKES, DSPX_E_PARAM, "state=%u", d->state);
+    DSPX_REQUIRE(d->state <= DSPX_PPQQ || d->npol == 2, DSPX_E_STATE, "Coherence/Stokes need npol=2");
+    DSPX_REQUIRE(d->state <= DSPX_PPQQ || d->ndim == 1 || d->ndim == 2 || d->ndim == 4, DSPX_E_STATE,
+                 "invalid ndim=%u", d->ndim);
+    DSPX_REQUIRE(d->chirp_mode == DSPX_CHIRP_PARITY || d->chirp_mode == DSPX_CHIRP_TRUTH, DSPX_E_PARAM,
+                 "chirp_mode=%u (on-chip modes only)", d->chirp_mode);
+    DSPX_REQUIRE(d->nbin >= 1 && d->nbin <= 65535, DSPX_E_PARAM, "nbin=%u", d->nbin);
+    DSPX_HIP(hipSetDevice(ctx->device));
+    auto* L = new dspx_ltf;
+    L->ctx = ctx;
+    L->d = *d;
+    try {
+      uint32_t nov = d->nfilt_pos + d->nfilt_neg;
+      DSPX_REQUIRE(nov < d->nfft, DSPX_E_STATE, "error nfft=%u < nfilt=%u", d->nfft, nov);
+      L->step = d->nfft - nov;
+      L->npart = d->ndat < d->nfft ? 0 : (d->ndat - nov) / L->step;
+      L->ndat_out = L->npart * L->step;
+      switch (d->state) {
+        case DSPX_INTENSITY: L->nval = 1; L->npol_out = 1; L->ndim_out = 1; break;
+        case DSPX_PPQQ: L->nval = d->npol; L->npol_out = d->npol; L->ndim_out = 1; break;
+        default: L->nval = 4; L->ndim_out = d->ndim; L->npol_out = 4 / d->ndim; break;
+      }
+      if (d->form == DSPX_FORM_PLAIN) {
+        L->plain.init(ctx, d->nchan, d->npol, d->nfft, d->nfilt_pos, d->nfilt_neg, d->chirp_mode, d->dedisp);
+      } else {
+        DSPX_REQUIRE(d->form == DSPX_FORM_FUSED, DSPX_E_PARAM, "form=%u", d->form);
+        L->core.init(ctx, d->nchan, d->npol, d->nfft, d->nfilt_pos, d->nfilt_neg, d->chirp_mode, d->dedisp);
+        DSPX_REQUIRE(L->npart < (1u << 20), DSPX_E_UNSUPPORTED, "npart too large");
+        L->core.reserve_work(uint32_t(L->npart));
+      }
+    } catch (...) {
+      delete L;
+      throw;
+    }
+    *out = L;
+  });
+}
+
+void dspx_ltf_destroy(dspx_ltf* L)
+{
+  if (!L) return;
+  (void) hipSetDevice(L->ctx->device);
+  (void) hipStreamSynchronize(L->ctx->stream);
+  delete L;
+}
+
+int dspx_ltf_geometry(const dspx_ltf* L, uint64_t* npart, uint64_t* ndat_out)
+{
+  if (!L || !npart || !ndat_out) return DSPX_E_PARAM;
+  *npart = L->npart;
+  *ndat_out = L->ndat_out;
+  return DSPX_OK;
+}
+
+int dspx_ltf_set_fold(dspx_ltf* L, double pps, const dspx_fold_chunk* chunks, uint32_t nchunk)
+{
+  dspx_ctx* ctx = L ? L->ctx : nullptr;
+  return guard(ctx, [&] {
+    DSPX_REQUIRE(L && chunks, DSPX_E_PARAM, "null argument");
+    L->fold.build(ctx, L->ndat_out, pps, L->d.nbin, chunks, nchunk);
+    L->have_fold = true;
+    if (L->d.form == DSPX_FORM_FUSED && L->npart) {
+      // sub-integration slots per part: the kept outputs of part p are
+      // samples [p*step, (p+1)*step)
+      std::vector<uint32_t> sf(L->npart);
+      uint32_t nslot = 1;
+      for (uint64_t p = 0; p < L->npart; p++) {
+        uint32_t first, last;
+        L->fold.sub_range(p * L->step, (p + 1) * L->step, first, last);
+        if (first == 0xffffffffu) {
+          first = 0;
+          last = 0;
+        }
+        sf[p] = first;
+        nslot = std::max(nslot, last - first + 1);
+      }
+      L->nslot = nslot;
+      size_t hist_bytes = size_t(nslot) * L->nval * L->d.nbin * sizeof(float);
+      DSPX_REQUIRE(kColTile * sizeof(float2) + hist_bytes <= 160 * 1024, DSPX_E_UNSUPPORTED,
+                   "fold histogram of %zu bytes does not fit in LDS", hist_bytes);
+      uint32_t ncb = L->core.ncb();
+      uint64_t units = uint64_t(L->d.nchan) * L->npart;
+      uint32_t G = uint32_t(std::min<uint64_t>(ncb, std::max<uint64_t>(1, (2048 + units - 1) / units)));
+      L->G = G;
+      L->sub_first.reserve(sf.size() * sizeof(uint32_t));
+      DSPX_HIP(hipMemcpy(L->sub_first.p, sf.data(), sf.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+      L->partial.reserve(units * G * hist_bytes);
+    }
+  });
+}
+
+int dspx_ltf_execute(dspx_ltf* L, const void* d_raw, float* d_prof, uint32_t* d_hits)
+{
+  dspx_ctx* ctx = L ? L->ctx : nullptr;
+  return guard(ctx, [&] {
+    DSPX_REQUIRE(L && d_raw && d_prof, DSPX_E_PARAM, "null argument");
+    DSPX_REQUIRE(L->have_fold, DSPX_E_STATE, "no fold plan: call dspx_ltf_set_fold");
+    const auto& d = L->d;
+    if (!L->npart) return;
+    int ev = 0;
+    L->nstage = 0;
+    ltf_record(L, ev++);
+    if (d.form == DSPX_FORM_FUSED) {
+      ConvCore& k = L->core;
+      uint32_t np = uint32_t(L->npart);
+      with_loader(L, d_raw, [&](auto& ld) { k.stage_a(ld, np); });
+      ltf_record(L, ev++);
+      L->stage_name[L->nstage++] = "stageA_unpack_colfft";
+      k.stage_b(np);
+      ltf_record(L, ev++);
+      L->stage_name[L->nstage++] = "stageB_rowfft_chirp";
+      FoldArgs fa;
+      fa.binplan = L->fold.plan.as<uint32_t>();
+      fa.partial = L->partial.as<float>();
+      fa.sub_first = L->sub_first.as<uint32_t>();
+      fa.nbin = d.nbin;
+      fa.nval = L->nval;
+      fa.ndim = L->ndim_out;
+      fa.nslot = L->nslot;
+      fa.G = L->G;
+      fa.state = d.state;
+      fa.npol_out = L->npol_out;
+      k.stage_c<kStageCFold>(np, FilterbankArgs{}, fa);
+      ltf_record(L, ev++);
+      L->stage_name[L->nstage++] = "stageC_colifft_detect_fold";
+      uint32_t hl = L->nval * d.nbin;
+      dim3 grid((hl + 255) / 256, d.nchan, L->fold.nsub);
+      k_fold_reduce<<<grid, 256, 0, ctx->stream>>>(L->partial.as<float>(), L->sub_first.as<uint32_t>(), np, L->G,
+                                                   L->nslot, hl, d.nchan, L->fold.nsub, d_prof);
+      DSPX_HIP(hipGetLastError());
+      if (d_hits) L->fold.hits(d_hits);
+      ltf_record(L, ev++);
+      L->stage_name[L->nstage++] = "fold_reduce_hits";
+    } else {
+      // plain: unpack -> hipFFT convolution -> detect -> fold, each a pass
+      uint64_t nin = d.ndat, nout = L->ndat_out;
+      L->unpacked.reserve(size_t(d.nchan) * d.npol * nin * sizeof(float2));
+      L->filtered.reserve(size_t(d.nchan) * d.npol * nout * sizeof(float2));
+      L->detected.reserve(size_t(d.nchan) * L->nval * nout * sizeof(float));
+      unpack(ctx, d_raw, d.nbit, d.offset_binary, d.nchan, d.npol, nin, d.scale, L->unpacked.as<float>(),
+             uint64_t(d.npol) * nin * 2, nin * 2);
+      ltf_record(L, ev++);
+      L->stage_name[L->nstage++] = "unpack";
+      L->plain.perform(L->unpacked.as<float>(), uint64_t(d.npol) * nin * 2, nin * 2, L->filtered.as<float>(),
+                       uint64_t(d.npol) * nout * 2, nout * 2, uint32_t(L->npart));
+      ltf_record(L, ev++);
+      L->stage_name[L->nstage++] = "convolution_hipfft";
+      uint64_t ops = nout * L->ndim_out;  // floats per output pol
+      uint64_t ocs = uint64_t(L->npol_out) * ops;
+      detect(ctx, d.state, L->ndim_out, L->filtered.as<float>(), uint64_t(d.npol) * nout * 2, nout * 2, d.nchan,
+             d.npol, nout, L->detected.as<float>(), ocs, ops);
+      ltf_record(L, ev++);
+      L->stage_name[L->nstage++] = "detect";
+      L->fold.fold_fpt(L->detected.as<float>(), ocs, ops, d.nchan, L->npol_out, L->ndim_out, d_prof);
+      if (d_hits) L->fold.hits(d_hits);
+      ltf_record(L, ev++);
+      L->stage_name[L->nstage++] = "fold_hits";
+    }
+  });
+}
+
+int dspx_ltf_filterbank(dspx_ltf* L, const void* d_raw, float* d_out)
+{
+  dspx_ctx* ctx = L ? L->ctx : nullptr;
+  return guard(ctx, [&] {
+    DSPX_REQUIRE(L && d_raw && d_out, DSPX_E_PARAM, "null argument");
+    const auto& d = L->d;
+    if (!L->npart) return;
+    uint64_t nout = L->ndat_out;
+    if (d.form == DSPX_FORM_FUSED) {
+      ConvCore& k = L->core;
+      uint32_t np = uint32_t(L->npart);
+      with_loader(L, d_raw, [&](auto& ld) { k.stage_a(ld, np); });
+      k.stage_b(np);
+      FilterbankArgs fb{d_out, uint64_t(d.npol) * nout * 2, nout * 2};
+      k.stage_c<kStageCFilterbank>(np, fb, FoldArgs{});
+    } else {
+      uint64_t nin = d.ndat;
+      L->unpacked.reserve(size_t(d.nchan) * d.npol * nin * sizeof(float2));
+      unpack(ctx, d_raw, d.nbit, d.offset_binary, d.nchan, d.npol, nin, d.scale, L->unpacked.as<float>(),
+             uint64_t(d.npol) * nin * 2, nin * 2);
+      L->plain.perform(L->unpacked.as<float>(), uint64_t(d.npol) * nin * 2, nin * 2, d_out,
+                       uint64_t(d.npol) * nout * 2, nout * 2, uint32_t(L->npart));
+    }
+  });
+}
+
+int dspx_ltf_set_timing(dspx_ltf* L, int enable)
+{
+  if (!L) return DSPX_E_PARAM;
+  L->timing = enable != 0;
+  return DSPX_OK;
+}
+
+int dspx_ltf_stage_times(dspx_ltf* L, float* ms, const char** names, int cap, int* n)
+{
+  dspx_ctx* ctx = L ? L->ctx : nullptr;
+  return guard(ctx, [&] {
+    DSPX_REQUIRE(L && n, DSPX_E_PARAM, "null argument");
+    DSPX_REQUIRE(L->timing, DSPX_E_STATE, "timing not enabled");
+    DSPX_HIP(hipEventSynchronize(ctx->ev[L->nstage]));
+    int k = std::min(cap, L->nstage);
+    for (int i = 0; i < k; i++) {
+      float t = 0;
+      DSPX_HIP(hipEventElapsedTime(&t, ctx->ev[i], ctx->ev[i + 1]));
+      if (ms) ms[i] = t;
+      if (names) names[i] = L->stage_name[i];
+    }
+    *n = L->nstage;
+  });
+}
+
+}  // extern "C"

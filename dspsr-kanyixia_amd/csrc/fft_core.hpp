@@ -1,0 +1,272 @@
+// fft_core.hpp -- register/LDS Stockham FFT building blocks for gfx950.
+//
+// A workgroup of NT threads transforms B independent sequences of length L
+// held in LDS ("batched columns", element (n, b) at n*RS + b) or one row
+// (B = 1, element n at n + n/16: one pad slot every 16 complex values keeps
+// the radix-16 scatter of the first Stockham pass conflict-free across the
+// 64 LDS banks).  Every thread owns E = B*L/NT complex values in registers;
+// a radix-R pass gives each thread E/R butterflies g = tid + q*NT.
+//
+// Stockham autosort pass (size Ns already transformed, radix R):
+//   j-th butterfly reads  x[j + r*L/R],            r = 0..R-1
+//   multiplies by         W_{Ns*R}^{r*(j%Ns)}
+//   applies a DFT_R and writes y[(j/Ns)*Ns*R + j%Ns + r*Ns].
+// The output is in natural order.  The last pass (Ns = L/R) leaves the
+// thread holding y[j + r*L/R] -- the very elements the first pass of a
+// transform with the same radix reads, so a forward transform, a pointwise
+// product and an inverse transform chain without an LDS round trip.
+//
+// Twiddles come from a table of exp(-2 pi i m / L) computed in double on the
+// host and rounded once to float; DIR = +1 uses the conjugate.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace dspx {
+
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ float2 cmul(float2 a, float2 b)
+{
+  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+// a * conj(b)
+__device__ __forceinline__ float2 cmulc(float2 a, float2 b)
+{
+  return make_float2(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
+}
+// multiply by DIR*i  (forward DIR=-1: -i ; inverse DIR=+1: +i)
+template <int DIR> __device__ __forceinline__ float2 mul_dir_i(float2 a)
+{
+  return DIR < 0 ? make_float2(a.y, -a.x) : make_float2(-a.y, a.x);
+}
+
+// cos/sin(2 pi m / 16), m = 0..15, correctly rounded to float
+__device__ constexpr float kCos16[16] = {
+    1.0f, 0.92387953251128674f, 0.70710678118654752f, 0.38268343236508977f,
+    0.0f, -0.38268343236508977f, -0.70710678118654752f, -0.92387953251128674f,
+    -1.0f, -0.92387953251128674f, -0.70710678118654752f, -0.38268343236508977f,
+    0.0f, 0.38268343236508977f, 0.70710678118654752f, 0.92387953251128674f};
+__device__ constexpr float kSin16[16] = {
+    0.0f, 0.38268343236508977f, 0.70710678118654752f, 0.92387953251128674f,
+    1.0f, 0.92387953251128674f, 0.70710678118654752f, 0.38268343236508977f,
+    0.0f, -0.38268343236508977f, -0.70710678118654752f, -0.92387953251128674f,
+    -1.0f, -0.92387953251128674f, -0.70710678118654752f, -0.38268343236508977f};
+
+// x * W_16^m with W_16 = exp(DIR * 2 pi i / 16)
+template <int DIR> __device__ __forceinline__ float2 mul_w16(float2 x, int m)
+{
+  m &= 15;
+  if (m == 0) return x;
+  if (m == 4) return mul_dir_i<DIR>(x);
+  if (m == 8) return make_float2(-x.x, -x.y);
+  if (m == 12) return mul_dir_i<-DIR>(x);
+  float2 w = make_float2(kCos16[m], DIR * kSin16[m]);
+  return cmul(x, w);
+}
+
+template <int DIR> __device__ __forceinline__ void dft2(float2& a, float2& b)
+{
+  float2 t = a;
+  a = cadd(t, b);
+  b = csub(t, b);
+}
+
+template <int DIR> __device__ __forceinline__ void dft4(float2& a0, float2& a1, float2& a2, float2& a3)
+{
+  float2 t0 = cadd(a0, a2), t1 = csub(a0, a2);
+  float2 t2 = cadd(a1, a3), t3 = mul_dir_i<DIR>(csub(a1, a3));
+  a0 = cadd(t0, t2);
+  a2 = csub(t0, t2);
+  a1 = cadd(t1, t3);
+  a3 = csub(t1, t3);
+}
+
+// In-register DFT of length R (2, 4, 8, 16) on v[0..R-1], natural order in
+// and out.  R = P*Q with P = 4 first: y[n2][k1] = DFT_P_n1 x[Q n1 + n2],
+// y *= W_R^{n2 k1}, X[k1 + P k2] = DFT_Q_n2 y[n2][k1].
+template <int R, int DIR> __device__ __forceinline__ void dft(float2* v)
+{
+  if constexpr (R == 2) {
+    dft2<DIR>(v[0], v[1]);
+  } else if constexpr (R == 4) {
+    dft4<DIR>(v[0], v[1], v[2], v[3]);
+  } else if constexpr (R == 8) {
+    // Q = 2: x[2 n1 + n2]
+    dft4<DIR>(v[0], v[2], v[4], v[6]);
+    dft4<DIR>(v[1], v[3], v[5], v[7]);
+    // y[1][k1] *= W8^{k1} = W16^{2 k1}
+    v[3] = mul_w16<DIR>(v[3], 2);
+    v[5] = mul_w16<DIR>(v[5], 4);
+    v[7] = mul_w16<DIR>(v[7], 6);
+    float2 o[8];
+#pragma unroll
+    for (int k1 = 0; k1 < 4; k1++) {
+      float2 a = v[2 * k1], b = v[2 * k1 + 1];
+      o[k1] = cadd(a, b);
+      o[k1 + 4] = csub(a, b);
+    }
+#pragma unroll
+    for (int i = 0; i < 8; i++) v[i] = o[i];
+  } else if constexpr (R == 16) {
+    // Q = 4: y[n2][k1] from x[4 n1 + n2]
+#pragma unroll
+    for (int n2 = 0; n2 < 4; n2++) dft4<DIR>(v[n2], v[4 + n2], v[8 + n2], v[12 + n2]);
+    // now v[4*k1 + n2] = y[n2][k1]; twiddle W16^{n2 k1}
+#pragma unroll
+    for (int k1 = 1; k1 < 4; k1++)
+#pragma unroll
+      for (int n2 = 1; n2 < 4; n2++) v[4 * k1 + n2] = mul_w16<DIR>(v[4 * k1 + n2], n2 * k1);
+    float2 o[16];
+#pragma unroll
+    for (int k1 = 0; k1 < 4; k1++) {
+      float2 a0 = v[4 * k1], a1 = v[4 * k1 + 1], a2 = v[4 * k1 + 2], a3 = v[4 * k1 + 3];
+      dft4<DIR>(a0, a1, a2, a3);
+      o[k1] = a0;
+      o[k1 + 4] = a1;
+      o[k1 + 8] = a2;
+      o[k1 + 12] = a3;
+    }
+#pragma unroll
+    for (int i = 0; i < 16; i++) v[i] = o[i];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Batched-column layout: element (n, b) at lds[n * RS + b].
+// Butterfly g -> b = g % B, j = g / B (consecutive lanes -> consecutive b).
+// ---------------------------------------------------------------------------
+template <int L, int B, int RS, int NT> struct ColLayout {
+  static constexpr int E = L * B / NT;
+  __device__ static __forceinline__ int idx(int n, int b) { return n * RS + b; }
+};
+
+// Row layout (B = 1): element n at n + (n >> 4).
+template <int L, int NT> struct RowLayout {
+  static constexpr int E = L / NT;
+  __device__ static __forceinline__ int idx(int n, int /*b*/) { return n + (n >> 4); }
+};
+
+// Compute one Stockham pass on registers v (which hold, for each of the
+// thread's E/R butterflies q, the R inputs x[j + r*L/R]).  After the call
+// v[q*R + r] holds the output destined for index (j/Ns)*Ns*R + j%Ns + r*Ns.
+template <int L, int B, int NT, int R, int NS, int DIR>
+__device__ __forceinline__ void stockham_compute(float2* v, const float2* __restrict__ tw, int tid)
+{
+  constexpr int E = L * B / NT;
+  static_assert(E % R == 0, "radix must divide the per-thread count");
+#pragma unroll
+  for (int q = 0; q < E / R; q++) {
+    int g = tid + q * NT;
+    int j = g / B;
+    if constexpr (NS > 1) {
+      int k = j % NS;
+#pragma unroll
+      for (int r = 1; r < R; r++) {
+        float2 w = tw[r * k * (L / (NS * R))];
+        v[q * R + r] = DIR < 0 ? cmul(v[q * R + r], w) : cmulc(v[q * R + r], w);
+      }
+    }
+    dft<R, DIR>(v + q * R);
+  }
+}
+
+template <class LAY, int L, int B, int NT, int R>
+__device__ __forceinline__ void lds_load_pass(const float2* lds, float2* v, int tid)
+{
+  constexpr int E = L * B / NT;
+#pragma unroll
+  for (int q = 0; q < E / R; q++) {
+    int g = tid + q * NT;
+    int b = g % B, j = g / B;
+#pragma unroll
+    for (int r = 0; r < R; r++) v[q * R + r] = lds[LAY::idx(j + r * (L / R), b)];
+  }
+}
+
+template <class LAY, int L, int B, int NT, int R, int NS>
+__device__ __forceinline__ void lds_store_pass(float2* lds, const float2* v, int tid)
+{
+  constexpr int E = L * B / NT;
+#pragma unroll
+  for (int q = 0; q < E / R; q++) {
+    int g = tid + q * NT;
+    int b = g % B, j = g / B;
+    int base = (j / NS) * NS * R + (j % NS);
+#pragma unroll
+    for (int r = 0; r < R; r++) lds[LAY::idx(base + r * NS, b)] = v[q * R + r];
+  }
+}
+
+// Radix plans.  The first radix of the forward plan equals the last radix of
+// the inverse plan and vice versa (inverse = reversed forward), so a forward
+// transform's last pass and the inverse's first pass share thread ownership.
+template <int L> struct RadixPlan;
+template <> struct RadixPlan<2>    { static constexpr int n = 1; static constexpr int r[4] = {2, 0, 0, 0}; };
+template <> struct RadixPlan<4>    { static constexpr int n = 1; static constexpr int r[4] = {4, 0, 0, 0}; };
+template <> struct RadixPlan<8>    { static constexpr int n = 1; static constexpr int r[4] = {8, 0, 0, 0}; };
+template <> struct RadixPlan<16>   { static constexpr int n = 1; static constexpr int r[4] = {16, 0, 0, 0}; };
+template <> struct RadixPlan<32>   { static constexpr int n = 2; static constexpr int r[4] = {2, 16, 0, 0}; };
+template <> struct RadixPlan<64>   { static constexpr int n = 2; static constexpr int r[4] = {4, 16, 0, 0}; };
+template <> struct RadixPlan<128>  { static constexpr int n = 2; static constexpr int r[4] = {8, 16, 0, 0}; };
+template <> struct RadixPlan<256>  { static constexpr int n = 2; static constexpr int r[4] = {16, 16, 0, 0}; };
+template <> struct RadixPlan<512>  { static constexpr int n = 3; static constexpr int r[4] = {2, 16, 16, 0}; };
+template <> struct RadixPlan<1024> { static constexpr int n = 3; static constexpr int r[4] = {4, 16, 16, 0}; };
+template <> struct RadixPlan<2048> { static constexpr int n = 3; static constexpr int r[4] = {8, 16, 16, 0}; };
+template <> struct RadixPlan<4096> { static constexpr int n = 3; static constexpr int r[4] = {16, 16, 16, 0}; };
+template <> struct RadixPlan<8192> { static constexpr int n = 4; static constexpr int r[4] = {2, 16, 16, 16}; };
+
+// Ns before pass p of the forward plan (product of earlier radices).
+template <int L, int P> struct FwdNs {
+  static constexpr int v = (P == 0) ? 1 : FwdNs<L, P - 1>::v * RadixPlan<L>::r[P - 1 < 0 ? 0 : P - 1];
+};
+template <int L> struct FwdNs<L, 0> { static constexpr int v = 1; };
+
+// Inverse plan uses the reversed radix order: pass p has radix r[n-1-p].
+template <int L, int P> struct InvRadix { static constexpr int v = RadixPlan<L>::r[RadixPlan<L>::n - 1 - P]; };
+template <int L, int P> struct InvNs {
+  static constexpr int v = (P == 0) ? 1 : InvNs<L, P - 1>::v * InvRadix<L, (P - 1 < 0 ? 0 : P - 1)>::v;
+};
+template <int L> struct InvNs<L, 0> { static constexpr int v = 1; };
+
+// Run passes [P0, P1) of the forward (DIR=-1, forward plan) or inverse
+// (DIR=+1, reversed plan) transform.  On entry v holds the inputs of pass P0
+// in pass-P0 ownership; on exit v holds the outputs of pass P1-1 in its
+// output ownership.  Passes after the first go through LDS.
+template <class LAY, int L, int B, int NT, int DIR, int P, int PEND>
+__device__ __forceinline__ void run_passes(float2* lds, float2* v, const float2* __restrict__ tw, int tid)
+{
+  if constexpr (P < PEND) {
+    constexpr int R = DIR < 0 ? RadixPlan<L>::r[P] : InvRadix<L, P>::v;
+    constexpr int NS = DIR < 0 ? FwdNs<L, P>::v : InvNs<L, P>::v;
+    if constexpr (P > 0) {
+      // previous pass left outputs in registers: scatter them, then gather
+      constexpr int RP = DIR < 0 ? RadixPlan<L>::r[P - 1] : InvRadix<L, P - 1>::v;
+      constexpr int NSP = DIR < 0 ? FwdNs<L, P - 1>::v : InvNs<L, P - 1>::v;
+      __syncthreads();
+      lds_store_pass<LAY, L, B, NT, RP, NSP>(lds, v, tid);
+      __syncthreads();
+      lds_load_pass<LAY, L, B, NT, R>(lds, v, tid);
+    }
+    stockham_compute<L, B, NT, R, NS, DIR>(v, tw, tid);
+    run_passes<LAY, L, B, NT, DIR, P + 1, PEND>(lds, v, tw, tid);
+  }
+}
+
+// Full transform: v holds pass-0 inputs (x[j + r*L/R0] for the thread's
+// butterflies) and ends holding outputs X[j + r*L/Rlast] of the last pass.
+template <class LAY, int L, int B, int NT, int DIR>
+__device__ __forceinline__ void fft_regs(float2* lds, float2* v, const float2* __restrict__ tw, int tid)
+{
+  run_passes<LAY, L, B, NT, DIR, 0, RadixPlan<L>::n>(lds, v, tw, tid);
+}
+
+// Radix of the first / last pass of each direction.
+template <int L, int DIR> struct FirstRadix {
+  static constexpr int v = DIR < 0 ? RadixPlan<L>::r[0] : RadixPlan<L>::r[RadixPlan<L>::n - 1];
+};
+template <int L, int DIR> struct LastRadix {
+  static constexpr int v = DIR < 0 ? RadixPlan<L>::r[RadixPlan<L>::n - 1] : RadixPlan<L>::r[0];
+};
+
+}  // namespace dspx

@@ -1,0 +1,408 @@
+// stages.hpp -- the three fused kernels of the overlap-save convolution.
+//
+// An nfft-point transform is split N = N1 * N2 (four-step):
+//   input  x[N2*n1 + n2]          (n1 < N1, n2 < N2)
+//   stage A: per column n2, forward N1-point FFT over n1, times W_N^{n2*k1}
+//            -> work[k1][n2]                             (unpack fused in)
+//   stage B: per row k1, forward N2-point FFT over n2 (bin k = k1 + N1*k2),
+//            times the chirp H[k], inverse N2-point FFT, times W_N^{-n2*k1}
+//            (chirp generated on chip in FP64)
+//   stage C: per column n2, inverse N1-point FFT over k1 -> y[N2*n1 + n2],
+//            overlap discard, then either the filterbank store or detection
+//            plus phase-bin accumulation in LDS.
+// Each stage moves 8 bytes per complex sample per direction through HBM once.
+#pragma once
+#include "dspx_internal.hpp"
+#include "fft_core.hpp"
+
+namespace dspx {
+
+constexpr int kColTile = 4096;  // complex values per pol in a column tile
+constexpr int kColThreads = 256;
+
+// ------------------------------------------------------------ input loaders
+
+// 8-bit complex TFP.  Both pols of (t, chan) in one 4-byte word.
+struct LoadI8 {
+  const uint8_t* raw;
+  uint64_t tstride;  // bytes per time sample (nchan*npol*2)
+  uint32_t npol;
+  uint32_t offset_binary;
+  float scale;
+  __device__ __forceinline__ float cvt(uint32_t byte) const
+  {
+    return offset_binary ? (float(byte) - 127.5f) * scale
+                         : (float(int8_t(uint8_t(byte))) + 0.5f) * scale;
+  }
+  __device__ __forceinline__ void operator()(uint32_t chan, uint64_t t, float2& a, float2& b) const
+  {
+    const uint8_t* q = raw + t * tstride + uint64_t(chan) * npol * 2;
+    if (npol == 2) {
+      uint32_t w = *reinterpret_cast<const uint32_t*>(q);
+      a = make_float2(cvt(w & 0xff), cvt((w >> 8) & 0xff));
+      b = make_float2(cvt((w >> 16) & 0xff), cvt(w >> 24));
+    } else {
+      uint32_t w = *reinterpret_cast<const uint16_t*>(q);
+      a = make_float2(cvt(w & 0xff), cvt(w >> 8));
+      b = make_float2(0.f, 0.f);
+    }
+  }
+};
+
+// 16-bit complex TFP: float(v) + 0.5 (offset binary: v ^ 0x8000).
+struct LoadI16 {
+  const int16_t* raw;
+  uint64_t tstride;  // int16 per time sample
+  uint32_t npol;
+  uint32_t offset_binary;
+  __device__ __forceinline__ float cvt(int16_t v) const
+  {
+    if (offset_binary) v = int16_t(v ^ 0x8000);
+    return float(v) + 0.5f;
+  }
+  __device__ __forceinline__ void operator()(uint32_t chan, uint64_t t, float2& a, float2& b) const
+  {
+    const int16_t* q = raw + t * tstride + uint64_t(chan) * npol * 2;
+    if (npol == 2) {
+      short4 w = *reinterpret_cast<const short4*>(q);
+      a = make_float2(cvt(w.x), cvt(w.y));
+      b = make_float2(cvt(w.z), cvt(w.w));
+    } else {
+      short2 w = *reinterpret_cast<const short2*>(q);
+      a = make_float2(cvt(w.x), cvt(w.y));
+      b = make_float2(0.f, 0.f);
+    }
+  }
+};
+
+// FPT float complex (the Convolution engine's unpacked TimeSeries).
+struct LoadF32 {
+  const float* in;
+  uint64_t chan_stride;  // floats
+  uint64_t pol_stride;   // floats
+  uint32_t npol;
+  __device__ __forceinline__ void operator()(uint32_t chan, uint64_t t, float2& a, float2& b) const
+  {
+    const float* q = in + uint64_t(chan) * chan_stride + 2 * t;
+    a = *reinterpret_cast<const float2*>(q);
+    b = npol == 2 ? *reinterpret_cast<const float2*>(q + pol_stride) : make_float2(0.f, 0.f);
+  }
+};
+
+// --------------------------------------------------------------- parameters
+
+struct ConvShape {
+  float2* work;        // [nchan][npart][npol][N1][N2]
+  uint32_t N1, N2, log2N;
+  uint32_t nchan, npol, npart;
+  uint32_t step, nfilt_pos;
+  Tw2 tw;              // W_N^m
+};
+
+struct ChirpArgs {
+  uint32_t mode;       // DSPX_CHIRP_*
+  float snorm;         // float(1/sqrt(N))
+  double binwidth;     // chanwidth / N
+  double half_cw;      // 0.5 * chanwidth
+  const double* coeff; // per channel
+  const double* cf;    // per channel centre frequency (MHz)
+  const float2* table; // DSPX_CHIRP_TABLE: [nchan][N1][N2] (transposed bins)
+};
+
+struct FoldArgs {
+  const uint32_t* binplan;   // per output sample: (isub << 16) | bin, ~0 skip
+  float* partial;            // [nchan][npart][G][nslot][nval*nbin]
+  const uint32_t* sub_first; // per part: first sub-integration its outputs touch
+  uint32_t nbin, nval, ndim, nslot, G;
+  uint32_t state, npol_out;
+};
+
+struct FilterbankArgs {
+  float* out;            // FPT complex
+  uint64_t chan_stride;  // floats
+  uint64_t pol_stride;   // floats
+};
+
+__device__ __forceinline__ float2 tw_N(const Tw2& t, uint32_t m)
+{
+  return cmul(t.lo[m & ((1u << t.h) - 1u)], t.hi[m >> t.h]);
+}
+
+// ------------------------------------------------------------------ stage A
+// grid.x = ncb * npart * nchan, block = 256.  Tile: N1 rows x C columns.
+template <int N1, class LD>
+__global__ __launch_bounds__(kColThreads) void k_stageA(LD ld, ConvShape s, const float2* __restrict__ twc)
+{
+  constexpr int C = kColTile / N1;
+  constexpr int E = kColTile / kColThreads;  // 16
+  constexpr int R0 = FirstRadix<N1, -1>::v;
+  constexpr int RL = LastRadix<N1, -1>::v;
+  using LAY = ColLayout<N1, C, C, kColThreads>;
+  __shared__ float2 lds[kColTile];
+
+  const int tid = threadIdx.x;
+  const uint32_t ncb = s.N2 / C;
+  uint32_t bid = blockIdx.x;
+  const uint32_t cb = bid % ncb;
+  bid /= ncb;
+  const uint32_t p = bid % s.npart;
+  const uint32_t chan = bid / s.npart;
+  const uint32_t c0 = cb * C;
+  const uint64_t t0 = uint64_t(p) * s.step + c0;
+
+  float2 va[E], vb[E];
+#pragma unroll
+  for (int q = 0; q < E / R0; q++) {
+    int g = tid + q * kColThreads;
+    int b = g % C, j = g / C;
+#pragma unroll
+    for (int r = 0; r < R0; r++) {
+      uint64_t t = t0 + uint64_t(s.N2) * uint32_t(j + r * (N1 / R0)) + b;
+      ld(chan, t, va[q * R0 + r], vb[q * R0 + r]);
+    }
+  }
+
+  const uint64_t plane = uint64_t(s.N1) * s.N2;
+  float2* wbase = s.work + (uint64_t(chan) * s.npart + p) * s.npol * plane;
+  for (uint32_t pol = 0; pol < s.npol; pol++) {
+    float2* v = pol == 0 ? va : vb;
+    fft_regs<LAY, N1, C, kColThreads, -1>(lds, v, twc, tid);
+    float2* w = wbase + pol * plane;
+#pragma unroll
+    for (int q = 0; q < E / RL; q++) {
+      int g = tid + q * kColThreads;
+      int b = g % C, j = g / C;
+#pragma unroll
+      for (int r = 0; r < RL; r++) {
+        uint32_t k1 = j + r * (N1 / RL);
+        uint32_t n2 = c0 + b;
+        float2 y = cmul(v[q * RL + r], tw_N(s.tw, n2 * k1));
+        w[uint64_t(k1) * s.N2 + n2] = y;
+      }
+    }
+  }
+}
+
+// ------------------------------------------------------------------ stage B
+
+// Chirp of FFT bin k of channel chan (reference parity or FP64 truth).
+__device__ __forceinline__ float2 chirp_onchip(const ChirpArgs& c, uint32_t chan, uint32_t k, uint32_t N)
+{
+#pragma clang fp contract(off)
+  uint32_t ipt = (k + (N >> 1)) & (N - 1);
+  double freq = double(ipt) * c.binwidth - c.half_cw;
+  double cf = c.cf[chan];
+  double phase = c.coeff[chan] * (freq * freq) / (cf + freq);
+  double sn, cs;
+  if (c.mode == DSPX_CHIRP_PARITY)
+    sincos(double(float(phase)), &sn, &cs);
+  else
+    sincos(phase, &sn, &cs);
+  return make_float2(c.snorm * float(cs), c.snorm * float(sn));
+}
+
+// grid.x = N1 * npart * nchan, block = N2/16.
+template <int N2>
+__global__ __launch_bounds__(N2 / 16) void k_stageB(ConvShape s, ChirpArgs c, const float2* __restrict__ twr)
+{
+  constexpr int NT = N2 / 16;
+  constexpr int E = 16;
+  using LAY = RowLayout<N2, NT>;
+  __shared__ float2 lds[N2 + N2 / 16];
+  constexpr int RF = FirstRadix<N2, -1>::v;   // forward first pass
+  constexpr int RI = FirstRadix<N2, +1>::v;   // inverse first pass (= forward last)
+  constexpr int RL = LastRadix<N2, +1>::v;    // inverse last pass
+
+  const int tid = threadIdx.x;
+  uint32_t bid = blockIdx.x;
+  const uint32_t k1 = bid % s.N1;
+  bid /= s.N1;
+  const uint32_t p = bid % s.npart;
+  const uint32_t chan = bid / s.npart;
+  const uint32_t N = s.N1 * N2;
+  const uint64_t plane = uint64_t(s.N1) * N2;
+  float2* rowbase = s.work + (uint64_t(chan) * s.npart + p) * s.npol * plane + uint64_t(k1) * N2;
+
+  // chirp for the bins this thread holds after the forward transform:
+  // k2 = tid + r * N2/RI (one butterfly per thread)
+  static_assert(E / RI == 1, "row plan must end with a radix-16 pass");
+  float2 h[RI];
+#pragma unroll
+  for (int r = 0; r < RI; r++) {
+    uint32_t k2 = tid + r * (N2 / RI);
+    uint32_t k = k1 + s.N1 * k2;
+    if (c.mode == DSPX_CHIRP_TABLE)
+      h[r] = c.table[(uint64_t(chan) * s.N1 + k1) * N2 + k2];
+    else
+      h[r] = chirp_onchip(c, chan, k, N);
+  }
+
+  for (uint32_t pol = 0; pol < s.npol; pol++) {
+    float2* row = rowbase + pol * plane;
+    float2 v[E];
+#pragma unroll
+    for (int q = 0; q < E / RF; q++) {
+      int j = tid + q * NT;
+#pragma unroll
+      for (int r = 0; r < RF; r++) v[q * RF + r] = row[j + r * (N2 / RF)];
+    }
+    if (pol > 0) __syncthreads();  // previous pol's last LDS reads are done
+    fft_regs<LAY, N2, 1, NT, -1>(lds, v, twr, tid);
+#pragma unroll
+    for (int r = 0; r < RI; r++) v[r] = cmul(v[r], h[r]);
+    fft_regs<LAY, N2, 1, NT, +1>(lds, v, twr, tid);
+#pragma unroll
+    for (int q = 0; q < E / RL; q++) {
+      int j = tid + q * NT;
+#pragma unroll
+      for (int r = 0; r < RL; r++) row[j + r * (N2 / RL)] = v[q * RL + r];
+    }
+  }
+}
+
+// ------------------------------------------------------------------ stage C
+
+enum { kStageCFilterbank = 0, kStageCFold = 1 };
+
+// Detected values of one sample (Detection.C:252-318; cross_detect.ic,
+// stokes_detect.ic), evaluated without contraction like the reference.
+__device__ __forceinline__ int detect_values(uint32_t state, uint32_t npol, float2 a, float2 b, float* d)
+{
+#pragma clang fp contract(off)
+  float pp = a.x * a.x;
+  pp += a.y * a.y;
+  if (state == DSPX_INTENSITY) {
+    if (npol == 2) {
+      pp += b.x * b.x;
+      pp += b.y * b.y;
+    }
+    d[0] = pp;
+    return 1;
+  }
+  float qq = b.x * b.x;
+  qq += b.y * b.y;
+  if (state == DSPX_PPQQ) {
+    d[0] = pp;
+    d[1] = qq;
+    return 2;
+  }
+  float pq_r = a.x * b.x + a.y * b.y;
+  float pq_i = a.x * b.y - a.y * b.x;
+  if (state == DSPX_STOKES) {
+    float p2 = a.x * a.x + a.y * a.y;
+    float q2 = b.x * b.x + b.y * b.y;
+    d[0] = p2 + q2;
+    d[1] = p2 - q2;
+    d[2] = 2.0f * pq_r;
+    d[3] = 2.0f * pq_i;
+  } else {
+    d[0] = a.x * a.x + a.y * a.y;
+    d[1] = b.x * b.x + b.y * b.y;
+    d[2] = pq_r;
+    d[3] = pq_i;
+  }
+  return 4;
+}
+
+// grid.x = G * npart * nchan (fold) or ncb * npart * nchan (filterbank).
+template <int N1, int MODE>
+__global__ __launch_bounds__(kColThreads) void k_stageC(ConvShape s, FilterbankArgs fb, FoldArgs fa,
+                                                       const float2* __restrict__ twc)
+{
+  constexpr int C = kColTile / N1;
+  constexpr int E = kColTile / kColThreads;
+  constexpr int RI = FirstRadix<N1, +1>::v;
+  constexpr int RL = LastRadix<N1, +1>::v;
+  using LAY = ColLayout<N1, C, C, kColThreads>;
+  extern __shared__ float2 smem[];
+  float2* lds = smem;
+  float* hist = reinterpret_cast<float*>(smem + kColTile);
+
+  const int tid = threadIdx.x;
+  const uint32_t ncb = s.N2 / C;
+  const uint32_t G = MODE == kStageCFold ? fa.G : ncb;
+  uint32_t bid = blockIdx.x;
+  const uint32_t g = bid % G;
+  bid /= G;
+  const uint32_t p = bid % s.npart;
+  const uint32_t chan = bid / s.npart;
+  uint32_t cb_begin, cb_end;
+  if (MODE == kStageCFold) {
+    cb_begin = uint32_t(uint64_t(ncb) * g / G);
+    cb_end = uint32_t(uint64_t(ncb) * (g + 1) / G);
+  } else {
+    cb_begin = g;
+    cb_end = g + 1;
+  }
+
+  const uint32_t hist_len = MODE == kStageCFold ? fa.nslot * fa.nval * fa.nbin : 0;
+  if (MODE == kStageCFold) {
+    for (uint32_t i = tid; i < hist_len; i += kColThreads) hist[i] = 0.f;
+  }
+  const uint32_t sub0 = MODE == kStageCFold ? fa.sub_first[p] : 0;
+
+  const uint64_t plane = uint64_t(s.N1) * s.N2;
+  const float2* wbase = s.work + (uint64_t(chan) * s.npart + p) * s.npol * plane;
+
+  for (uint32_t cb = cb_begin; cb < cb_end; cb++) {
+    const uint32_t c0 = cb * C;
+    float2 va[E], vb[E];
+#pragma unroll
+    for (int q = 0; q < E / RI; q++) {
+      int gg = tid + q * kColThreads;
+      int b = gg % C, j = gg / C;
+#pragma unroll
+      for (int r = 0; r < RI; r++) {
+        uint32_t k1 = j + r * (N1 / RI);
+        uint32_t n2 = c0 + b;
+        float2 w = tw_N(s.tw, n2 * k1);
+        uint64_t off = uint64_t(k1) * s.N2 + n2;
+        va[q * RI + r] = cmulc(wbase[off], w);
+        vb[q * RI + r] = s.npol == 2 ? cmulc(wbase[plane + off], w) : make_float2(0.f, 0.f);
+      }
+    }
+    __syncthreads();  // previous tile's LDS reads are complete
+    fft_regs<LAY, N1, C, kColThreads, +1>(lds, va, twc, tid);
+    if (s.npol == 2) {
+      __syncthreads();
+      fft_regs<LAY, N1, C, kColThreads, +1>(lds, vb, twc, tid);
+    }
+#pragma unroll
+    for (int q = 0; q < E / RL; q++) {
+      int gg = tid + q * kColThreads;
+      int b = gg % C, j = gg / C;
+#pragma unroll
+      for (int r = 0; r < RL; r++) {
+        uint32_t n1 = j + r * (N1 / RL);
+        uint32_t t = s.N2 * n1 + c0 + b;
+        if (t < s.nfilt_pos || t >= s.nfilt_pos + s.step) continue;
+        uint64_t idat = uint64_t(p) * s.step + (t - s.nfilt_pos);
+        if (MODE == kStageCFilterbank) {
+          float* o = fb.out + uint64_t(chan) * fb.chan_stride + 2 * idat;
+          *reinterpret_cast<float2*>(o) = va[q * RL + r];
+          if (s.npol == 2) *reinterpret_cast<float2*>(o + fb.pol_stride) = vb[q * RL + r];
+        } else {
+          uint32_t plan = fa.binplan[idat];
+          if (plan == 0xffffffffu) continue;
+          uint32_t bin = plan & 0xffffu;
+          uint32_t slot = (plan >> 16) - sub0;
+          float d[4];
+          int nv = detect_values(fa.state, s.npol, va[q * RL + r], vb[q * RL + r], d);
+          float* hs = hist + slot * fa.nval * fa.nbin;
+          for (int i = 0; i < nv; i++) {
+            uint32_t off = (i / fa.ndim) * fa.nbin * fa.ndim + bin * fa.ndim + (i % fa.ndim);
+            atomicAdd(hs + off, d[i]);
+          }
+        }
+      }
+    }
+  }
+
+  if (MODE == kStageCFold) {
+    __syncthreads();
+    float* part = fa.partial + ((uint64_t(chan) * s.npart + p) * fa.G + g) * hist_len;
+    for (uint32_t i = tid; i < hist_len; i += kColThreads) part[i] = hist[i];
+  }
+}
+
+}  // namespace dspx

@@ -1,0 +1,234 @@
+/*
+ * dspx.h -- C ABI of libdspx, the MI355X (gfx950) coherent-dedispersion
+ * hot path.  Plain pointers and sizes only; no exceptions cross this ABI.
+ * Every call returns a dspx status (0 = DSPX_OK) and, on failure, leaves a
+ * message readable with dspx_last_error().
+ *
+ * All device pointers are HIP device memory on the context's device; all work
+ * is enqueued on the context's stream and is asynchronous unless a function
+ * says otherwise.  A context is NOT thread-safe (one per pipeline thread, like
+ * the reference's one cudaStream per SingleThread, SingleThread.C:135-169).
+ *
+ * Reference interfaces replaced (paths relative to the dspsr source tree):
+ *   dspx_unpack          GenericEightBitUnpacker::unpack_on_gpu
+ *                        (Kernel/Classes/GenericEightBitUnpacker.C:92-131) and
+ *                        SixteenBitUnpacker::unpack (SixteenBitUnpacker.C:23-78)
+ *   dspx_dedisp_geometry PlasmaResponse::prepare (PlasmaResponse.C:91-119),
+ *                        Response::set_optimal_ndat (Response.C:553-588),
+ *                        Convolution::prepare (Convolution.C:316-455)
+ *   dspx_conv_*          Convolution::Engine prepare / copy_response / perform
+ *                        (Convolution.C:446-451, 656-667, 690-697;
+ *                        ConvolutionCUDA.cu:174-216, 219-284, 567-770)
+ *   dspx_detect          Detection::Engine square_law / polarimetry
+ *                        (Detection.C:237-243, 483-490; DetectionCUDA.cu:143,294)
+ *   dspx_fold_*          Fold::Engine set_bins / fold / get_bin_hits
+ *                        (Signal/Pulsar/dsp/Fold.h:262-345; Fold.C:760-887)
+ *   dspx_sk_*            SpectralKurtosis::Engine compute / detect_ft / mask
+ *                        (SpectralKurtosis.C:699-703, 1130-1134, 1518-1524)
+ *   dspx_ltf_*           the LoadToFold operation chain Unpack -> Convolution
+ *                        -> Detection -> Fold (LoadToFold1.C:109-258,
+ *                        SingleThread.C:662-769) fused into three kernels.
+ */
+#ifndef DSPX_H
+#define DSPX_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  DSPX_OK = 0,
+  DSPX_E_PARAM = 1,       /* reference: Error(InvalidParam, ...) */
+  DSPX_E_STATE = 2,       /* reference: Error(InvalidState, ...) */
+  DSPX_E_HIP = 3,         /* reference: Error(FailedCall, ...) / CUFFTError */
+  DSPX_E_UNSUPPORTED = 4, /* shape outside what the kernels were built for */
+  DSPX_E_INTERNAL = 5
+};
+
+typedef struct dspx_ctx dspx_ctx;
+
+/* Context: device, stream (nullable: the library creates one) and the
+   twiddle tables shared by every engine on it. */
+int dspx_ctx_create(int device, void* hip_stream, dspx_ctx** out);
+void dspx_ctx_destroy(dspx_ctx* ctx);
+const char* dspx_last_error(const dspx_ctx* ctx);
+int dspx_ctx_synchronize(dspx_ctx* ctx);
+void* dspx_ctx_stream(dspx_ctx* ctx);
+/* Library version string, e.g. "dspx 0.1 gfx950". */
+const char* dspx_version(void);
+
+/* ---------------------------------------------------------------- geometry */
+
+typedef struct {
+  double centre_frequency;   /* MHz */
+  double bandwidth;          /* MHz; negative for an inverted band */
+  double dispersion_measure; /* pc cm^-3 */
+  uint32_t nchan;
+  uint32_t dc_centred;       /* Observation::get_dc_centred */
+  uint32_t nfft;             /* 0: reference rule; else user-set resolution */
+  uint32_t reserved;
+} dspx_dedisp_desc;
+
+typedef struct {
+  uint32_t impulse_neg;   /* nfilt_neg: samples lost at the end of each FFT */
+  uint32_t impulse_pos;   /* nfilt_pos: samples lost at the start */
+  uint32_t nfft;
+  uint32_t nsamp_overlap; /* impulse_neg + impulse_pos */
+  uint32_t nsamp_step;    /* nfft - nsamp_overlap */
+  uint32_t nunsupported;  /* leading channels whose smear exceeds the limit */
+} dspx_geometry;
+
+/* Host-only.  Complex (Analytic) input. */
+int dspx_dedisp_geometry(const dspx_dedisp_desc* desc, dspx_geometry* out);
+
+/* ------------------------------------------------------------------ unpack */
+
+/* Raw TFP [ndat][nchan][npol][2] (8-bit: (v + 0.5) * scale, offset binary
+   (v - 127.5) * scale; 16-bit: float(v) + 0.5, scale ignored, offset binary
+   XOR 0x8000) -> FPT float complex out[ichan*chan_stride + ipol*pol_stride
+   + 2*idat (+1)], strides in floats. */
+int dspx_unpack(dspx_ctx* ctx, const void* d_raw, uint32_t nbit, uint32_t offset_binary,
+                uint32_t nchan, uint32_t npol, uint64_t ndat, float scale, float* d_out,
+                uint64_t out_chan_stride, uint64_t out_pol_stride);
+
+/* ------------------------------------------------------------- convolution */
+
+enum {
+  DSPX_CHIRP_TABLE = 0,  /* response copied from the host (copy_response) */
+  DSPX_CHIRP_PARITY = 1, /* on-chip FP64 phase rounded to float, as the
+                            reference stores it (Dedispersion.C:104-111) */
+  DSPX_CHIRP_TRUTH = 2   /* on-chip FP64 phase, no float rounding */
+};
+
+enum {
+  DSPX_FORM_FUSED = 0, /* hand-written three-stage Stockham path */
+  DSPX_FORM_PLAIN = 1  /* hipFFT + separate multiply / discard kernels */
+};
+
+typedef struct dspx_conv dspx_conv;
+
+typedef struct {
+  dspx_dedisp_desc dedisp; /* band and DM for the on-chip chirp */
+  uint32_t nchan;
+  uint32_t npol;
+  uint32_t nfft;
+  uint32_t nfilt_pos;
+  uint32_t nfilt_neg;
+  uint32_t chirp_mode;
+  uint32_t form;
+  uint32_t reserved;
+} dspx_conv_desc;
+
+int dspx_conv_create(dspx_ctx* ctx, const dspx_conv_desc* desc, dspx_conv** out);
+void dspx_conv_destroy(dspx_conv* conv);
+/* Engine::copy_response: [nchan][nfft] complex float in FFT-bin order, the
+   normaliser already multiplied in.  Required for DSPX_CHIRP_TABLE. */
+int dspx_conv_set_response(dspx_conv* conv, const float* host_response);
+/* Engine::perform: FPT complex input (ndat samples per chan/pol), npart
+   overlapping FFTs of nfft with step nfft - nfilt_pos - nfilt_neg; output
+   npart*step samples.  d_zero_dm (nullable) receives the zero-DM copy.
+   Strides in floats. */
+int dspx_conv_perform(dspx_conv* conv, const float* d_in, uint64_t in_chan_stride,
+                      uint64_t in_pol_stride, uint64_t ndat, float* d_out,
+                      uint64_t out_chan_stride, uint64_t out_pol_stride, float* d_zero_dm,
+                      uint32_t npart);
+
+/* --------------------------------------------------------------- detection */
+
+enum { DSPX_INTENSITY = 0, DSPX_PPQQ = 1, DSPX_COHERENCE = 2, DSPX_STOKES = 3 };
+
+/* FPT complex input -> FPT detected output.  Intensity: out[ichan*chan_stride
+   + idat]; PPQQ: ipol*pol_stride added; Coherence/Stokes with ndim 1, 2 or 4
+   lay the four products out as Detection::get_result_pointers does
+   (Detection.C:578-624): value i of sample idat at
+   (i/ndim)*pol_stride + idat*ndim + i%ndim. */
+int dspx_detect(dspx_ctx* ctx, uint32_t state, uint32_t ndim, const float* d_in,
+                uint64_t in_chan_stride, uint64_t in_pol_stride, uint32_t nchan, uint32_t npol,
+                uint64_t ndat, float* d_out, uint64_t out_chan_stride, uint64_t out_pol_stride);
+
+/* -------------------------------------------------------------------- fold */
+
+/* One Fold::fold call: samples [idat_start, idat_start + ndat) of the block
+   go to sub-integration isub, starting from phase phi = get_phi(start time of
+   the midpoint of sample idat_start) (Fold.C:686-693). */
+typedef struct {
+  uint64_t idat_start;
+  uint64_t ndat;
+  double phi;
+  uint32_t isub;
+  uint32_t reserved;
+} dspx_fold_chunk;
+
+typedef struct dspx_fold dspx_fold;
+
+/* Builds the bin plan on the device for a block of ndat samples.  The bins
+   reproduce the reference recurrence phi -= floor(phi); ibin =
+   unsigned(phi*nbin); phi += pps (Fold.C:790-838) bit for bit. */
+int dspx_fold_create(dspx_ctx* ctx, uint64_t ndat, double phase_per_sample, uint32_t nbin,
+                     const dspx_fold_chunk* chunks, uint32_t nchunk, dspx_fold** out);
+void dspx_fold_destroy(dspx_fold* fold);
+/* Per-sample plan: (isub << 16) | ibin, or 0xffffffff for samples outside
+   every chunk.  d_out holds ndat uint32. */
+int dspx_fold_binplan(dspx_fold* fold, uint32_t* d_out);
+/* hits[isub][ibin] += number of samples folded into ibin (Fold.C:827-836). */
+int dspx_fold_hits(dspx_fold* fold, uint32_t* d_hits);
+/* Host-only evaluation of the same plan for one chunk starting at sample 0
+   (bins[j] for j < ndat), and the number of arithmetic runs it uses; for
+   checking the plan against the serial recurrence without a device. */
+int dspx_fold_binplan_host(double phi, double phase_per_sample, uint32_t nbin, uint64_t ndat,
+                           uint32_t* bins);
+int dspx_fold_nsegments_host(double phi, double phase_per_sample, uint64_t ndat, uint64_t* nseg);
+/* Engine::fold: detected FPT input (value idim of sample idat of (ichan, ipol)
+   at ichan*chan_stride + ipol*pol_stride + idat*ndim + idim) is added into
+   prof[isub][ichan][ipol][ibin*ndim + idim] (accumulated). */
+int dspx_fold_execute(dspx_fold* fold, const float* d_in, uint64_t in_chan_stride,
+                      uint64_t in_pol_stride, uint32_t nchan, uint32_t npol, uint32_t ndim,
+                      float* d_prof);
+
+/* ------------------------------------------------------ fused LoadToFold */
+
+typedef struct dspx_ltf dspx_ltf;
+
+typedef struct {
+  uint32_t nbit;          /* 8 or 16: complex TFP [ndat][nchan][npol][2] */
+  uint32_t offset_binary;
+  float scale;            /* 8-bit unpack scale */
+  uint32_t nchan;
+  uint32_t npol;          /* 1 or 2 */
+  uint32_t reserved0;
+  uint64_t ndat;          /* input samples per channel and polarisation */
+  dspx_dedisp_desc dedisp;
+  uint32_t nfft, nfilt_pos, nfilt_neg;
+  uint32_t chirp_mode;    /* DSPX_CHIRP_PARITY or DSPX_CHIRP_TRUTH */
+  uint32_t state;         /* DSPX_INTENSITY ... DSPX_STOKES */
+  uint32_t ndim;          /* 1, 2 or 4 for Coherence / Stokes */
+  uint32_t nbin;
+  uint32_t form;          /* DSPX_FORM_FUSED or DSPX_FORM_PLAIN */
+} dspx_ltf_desc;
+
+int dspx_ltf_create(dspx_ctx* ctx, const dspx_ltf_desc* desc, dspx_ltf** out);
+void dspx_ltf_destroy(dspx_ltf* ltf);
+/* Number of FFT parts and output samples of one block (Convolution.C:485-487,
+   601-608). */
+int dspx_ltf_geometry(const dspx_ltf* ltf, uint64_t* npart, uint64_t* ndat_out);
+/* Fold plan of the block (output sample indices). */
+int dspx_ltf_set_fold(dspx_ltf* ltf, double phase_per_sample, const dspx_fold_chunk* chunks,
+                      uint32_t nchunk);
+/* Raw device input -> profiles prof[isub][nchan][npol_out][nbin*ndim_out]
+   and hits[isub][nbin], both accumulated. */
+int dspx_ltf_execute(dspx_ltf* ltf, const void* d_raw, float* d_prof, uint32_t* d_hits);
+/* Raw device input -> dedispersed filterbank out[ichan][ipol][npart*step]
+   complex (what the Convolution operation outputs). */
+int dspx_ltf_filterbank(dspx_ltf* ltf, const void* d_raw, float* d_out);
+/* Per-kernel durations (ms) of the last execute when timing is enabled;
+   names[i] points at static strings.  Returns the count via *n. */
+int dspx_ltf_set_timing(dspx_ltf* ltf, int enable);
+int dspx_ltf_stage_times(dspx_ltf* ltf, float* ms, const char** names, int cap, int* n);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DSPX_H */

@@ -1,0 +1,256 @@
+"""ctypes wrapper of the CPU oracle (oracle.c) and of the reference's own C
+files compiled into oracle/_ref/libref.so.
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, as the checker.  The product library never
+imports this module.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_lib = None
+_ref = None
+
+f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
+f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+u8p = np.ctypeslib.ndpointer(np.uint8, flags="C_CONTIGUOUS")
+i16p = np.ctypeslib.ndpointer(np.int16, flags="C_CONTIGUOUS")
+u16p = np.ctypeslib.ndpointer(np.uint16, flags="C_CONTIGUOUS")
+u32p = np.ctypeslib.ndpointer(np.uint32, flags="C_CONTIGUOUS")
+u64p = np.ctypeslib.ndpointer(np.uint64, flags="C_CONTIGUOUS")
+
+
+def build():
+    """Compile liboracle.so (and _ref/libref.so when the reference tree is present)."""
+    import subprocess
+    subprocess.run(["make", "-s", "-C", _HERE, "all"], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    path = os.path.join(_HERE, "liboracle.so")
+    if not os.path.exists(path):
+        build()
+    L = C.CDLL(path)
+    u, i, d, U = C.c_uint, C.c_int, C.c_double, C.c_uint64
+    L.ox_delay_time.restype = d
+    L.ox_delay_time.argtypes = [d, d]
+    L.ox_smearing_samples.restype = u
+    L.ox_smearing_samples.argtypes = [d, d, u, d, i, u]
+    L.ox_optimal_fft_length.restype = C.c_int64
+    L.ox_optimal_fft_length.argtypes = [U, U, u]
+    L.ox_conv_geometry.restype = i
+    L.ox_conv_geometry.argtypes = [d, d, u, d, u, u64p]
+    L.ox_conv_npart.restype = U
+    L.ox_conv_npart.argtypes = [U, U, U]
+    L.ox_unpack8.restype = None
+    L.ox_unpack8.argtypes = [u8p, u, u, U, i, C.c_float, f32p]
+    L.ox_unpack16.restype = None
+    L.ox_unpack16.argtypes = [i16p, u, u, U, i, f32p]
+    L.ox_fft_f32.restype = None
+    L.ox_fft_f32.argtypes = [U, f32p, i]
+    L.ox_fft_f64.restype = None
+    L.ox_fft_f64.argtypes = [U, f64p, i]
+    L.ox_channel_centre.restype = d
+    L.ox_channel_centre.argtypes = [d, d, u, i, u]
+    L.ox_chirp_phase.restype = d
+    L.ox_chirp_phase.argtypes = [d, d, d, d]
+    L.ox_chirp_freq.restype = d
+    L.ox_chirp_freq.argtypes = [d, u, u]
+    L.ox_dedisp_response_f32.restype = None
+    L.ox_dedisp_response_f32.argtypes = [d, d, u, i, d, u, i, f32p]
+    L.ox_dedisp_response_f64.restype = None
+    L.ox_dedisp_response_f64.argtypes = [d, d, u, i, d, u, f64p]
+    for name, p in (("ox_convolve_f32", f32p), ("ox_convolve_f64", f64p)):
+        fn = getattr(L, name)
+        fn.restype = U
+        fn.argtypes = [p, u, u, U, p, u, u, u, p, C.c_void_p]
+    L.ox_detect_square_law.restype = None
+    L.ox_detect_square_law.argtypes = [f32p, u, u, U, i, f32p]
+    L.ox_detect_polarimetry.restype = None
+    L.ox_detect_polarimetry.argtypes = [f32p, u, U, u, i, f32p]
+    L.ox_fold.restype = U
+    L.ox_fold.argtypes = [f32p, u, u, u, U, U, U, d, d, u, f32p, u32p, u, i,
+                          C.c_void_p, U, u, u, C.c_void_p]
+    L.ox_sk_compute.restype = U
+    L.ox_sk_compute.argtypes = [f32p, u, u, U, u, i, f32p, u32p]
+    L.ox_sk_detect_skfb.restype = U
+    L.ox_sk_detect_skfb.argtypes = [f32p, u32p, U, u, u, u, f32p, f32p, u8p]
+    _lib = L
+    return L
+
+
+def ref():
+    """The reference's own optimize_fft.c / stokes_detect.c / cross_detect.c,
+    or None where oracle/_ref was not built."""
+    global _ref
+    if _ref is not None:
+        return _ref
+    path = os.path.join(_HERE, "_ref", "libref.so")
+    if not os.path.exists(path):
+        return None
+    R = C.CDLL(path)
+    R.optimal_fft_length.restype = C.c_int64
+    R.optimal_fft_length.argtypes = [C.c_uint64, C.c_uint64, C.c_uint, C.c_char]
+    for name in ("stokes_detect", "cross_detect"):
+        fn = getattr(R, name)
+        fn.restype = None
+        fn.argtypes = [C.c_uint, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                       C.c_void_p, C.c_void_p, C.c_uint]
+    _ref = R
+    return R
+
+
+# ---------------------------------------------------------------- helpers
+
+
+def geometry(cfreq, bw, nchan, dm, nfft=0):
+    """dict(impulse_neg, impulse_pos, nfft, noverlap, step, nunsupported)."""
+    out = np.zeros(6, np.uint64)
+    rc = lib().ox_conv_geometry(cfreq, bw, nchan, dm, nfft, out)
+    if rc != 0:
+        raise ValueError("oracle geometry failed rc=%d" % rc)
+    keys = ("impulse_neg", "impulse_pos", "nfft", "noverlap", "step", "nunsupported")
+    return {k: int(v) for k, v in zip(keys, out)}
+
+
+def unpack8(raw, nchan, npol, ndat, scale=1.0, offset_binary=False):
+    out = np.empty(nchan * npol * ndat * 2, np.float32)
+    lib().ox_unpack8(np.ascontiguousarray(raw).view(np.uint8).ravel(), nchan, npol,
+                     ndat, int(offset_binary), scale, out)
+    return out.reshape(nchan, npol, ndat, 2)
+
+
+def unpack16(raw, nchan, npol, ndat, offset_binary=False):
+    out = np.empty(nchan * npol * ndat * 2, np.float32)
+    lib().ox_unpack16(np.ascontiguousarray(raw).view(np.int16).ravel(), nchan, npol,
+                      ndat, int(offset_binary), out)
+    return out.reshape(nchan, npol, ndat, 2)
+
+
+def fft(x, sign=-1):
+    """Unnormalised complex FFT of a complex64/complex128 vector."""
+    x = np.ascontiguousarray(x)
+    if x.dtype == np.complex64:
+        buf = x.view(np.float32).copy()
+        lib().ox_fft_f32(x.size, buf, sign)
+        return buf.view(np.complex64)
+    buf = x.astype(np.complex128).view(np.float64).copy()
+    lib().ox_fft_f64(x.size, buf, sign)
+    return buf.view(np.complex128)
+
+
+def response(cfreq, bw, nchan, dm, nfft, mode=0, dc_centred=False):
+    """[nchan][nfft] complex response in FFT-bin order (normaliser included).
+    mode 0: float phase (reference parity); 1: FP64 phase rounded to float;
+    'f64': FP64 truth in double."""
+    if mode == "f64":
+        out = np.empty(nchan * nfft * 2, np.float64)
+        lib().ox_dedisp_response_f64(cfreq, bw, nchan, int(dc_centred), dm, nfft, out)
+        return out.view(np.complex128).reshape(nchan, nfft)
+    out = np.empty(nchan * nfft * 2, np.float32)
+    lib().ox_dedisp_response_f32(cfreq, bw, nchan, int(dc_centred), dm, nfft, mode, out)
+    return out.view(np.complex64).reshape(nchan, nfft)
+
+
+def convolve(x, resp, nfilt_pos, nfilt_neg, zero_dm=False):
+    """Overlap-save convolution.  x: [nchan][npol][ndat] complex64/128;
+    resp: [nchan][nfft] of the same precision.  Returns out (and zdm)."""
+    x = np.ascontiguousarray(x)
+    nchan, npol, ndat = x.shape
+    nfft = resp.shape[1]
+    step = nfft - nfilt_pos - nfilt_neg
+    npart = 0 if ndat < nfft else (ndat - nfilt_pos - nfilt_neg) // step
+    if x.dtype == np.complex64:
+        ft, fn = np.float32, lib().ox_convolve_f32
+        resp = np.ascontiguousarray(resp, np.complex64)
+    else:
+        ft, fn = np.float64, lib().ox_convolve_f64
+        x = x.astype(np.complex128)
+        resp = np.ascontiguousarray(resp, np.complex128)
+    out = np.zeros(nchan * npol * npart * step * 2, ft)
+    zdm = np.zeros_like(out) if zero_dm else None
+    fn(x.view(ft).ravel(), nchan, npol, ndat, resp.view(ft).ravel(), nfft,
+       nfilt_pos, nfilt_neg, out, zdm.ctypes.data if zero_dm else None)
+    cdt = np.complex64 if ft == np.float32 else np.complex128
+    out = out.view(cdt).reshape(nchan, npol, npart * step)
+    if zero_dm:
+        return out, zdm.view(cdt).reshape(nchan, npol, npart * step)
+    return out
+
+
+INTENSITY, PPQQ, COHERENCE, STOKES = 0, 1, 2, 3
+
+
+def detect(x, state, ndim=4):
+    """x: [nchan][npol][ndat] complex64.  Returns the detected FPT array
+    [nchan][npol_out][ndat*ndim_out] as the reference lays it out."""
+    x = np.ascontiguousarray(x, np.complex64)
+    nchan, npol, ndat = x.shape
+    flat = x.view(np.float32).ravel()
+    if state in (INTENSITY, PPQQ):
+        onpol = npol if state == PPQQ else 1
+        out = np.zeros(nchan * onpol * ndat, np.float32)
+        lib().ox_detect_square_law(flat, nchan, npol, ndat, int(state == PPQQ), out)
+        return out.reshape(nchan, onpol, ndat)
+    out = np.zeros(nchan * 4 * ndat, np.float32)
+    lib().ox_detect_polarimetry(flat, nchan, ndat, ndim, int(state == STOKES), out)
+    return out.reshape(nchan, 4 // ndim, ndat * ndim)
+
+
+def fold(data, ndim, phi, pps, nbin, idat_start=0, ndat_fold=None, prof=None,
+         hits=None, zeroed=False, weights=None, ndatperweight=0, weight_idat=0,
+         hits_nchan=1, want_binplan=False):
+    """data: [nchan][npol][ndat*ndim] float32 (detected, FPT).  Accumulates
+    into prof [nchan][npol][nbin*ndim] and hits [hits_nchan][nbin]."""
+    data = np.ascontiguousarray(data, np.float32)
+    nchan, npol, nf = data.shape
+    ndat = nf // ndim
+    if ndat_fold is None:
+        ndat_fold = ndat - idat_start
+    if prof is None:
+        prof = np.zeros((nchan, npol, nbin * ndim), np.float32)
+    if hits is None:
+        hits = np.zeros((hits_nchan, nbin), np.uint32)
+    binplan = np.zeros(max(ndat_fold, 1), np.uint32) if want_binplan else None
+    w = None
+    nweights = 0
+    if weights is not None:
+        w = np.ascontiguousarray(weights, np.uint16)
+        nweights = w.size
+    nfolded = lib().ox_fold(data.ravel(), nchan, npol, ndim, ndat, idat_start, ndat_fold,
+                            phi, pps, nbin, prof.reshape(-1), hits.reshape(-1), hits_nchan,
+                            int(zeroed), w.ctypes.data if w is not None else None, nweights,
+                            ndatperweight, weight_idat,
+                            binplan.ctypes.data if want_binplan else None)
+    if want_binplan:
+        return prof, hits, nfolded, binplan[:ndat_fold]
+    return prof, hits, nfolded
+
+
+def sk_compute(x, M, zeroed=False):
+    """x: [nchan][npol][ndat] complex64 -> sums [npart][nchan][npol][2] float32,
+    counts [npart][nchan] uint32."""
+    x = np.ascontiguousarray(x, np.complex64)
+    nchan, npol, ndat = x.shape
+    npart = ndat // M
+    sums = np.zeros(npart * nchan * npol * 2, np.float32)
+    counts = np.zeros(npart * nchan, np.uint32)
+    lib().ox_sk_compute(x.view(np.float32).ravel(), nchan, npol, ndat, M, int(zeroed),
+                        sums, counts)
+    return sums.reshape(npart, nchan, npol, 2), counts.reshape(npart, nchan)
+
+
+def sk_detect_skfb(sums, counts, lower, upper, Nd=1):
+    npart, nchan, npol, _ = sums.shape
+    mask = np.zeros(npart * nchan, np.uint8)
+    n = lib().ox_sk_detect_skfb(np.ascontiguousarray(sums, np.float32).ravel(),
+                                np.ascontiguousarray(counts, np.uint32).ravel(), npart,
+                                nchan, npol, Nd, np.ascontiguousarray(lower, np.float32),
+                                np.ascontiguousarray(upper, np.float32), mask)
+    return mask.reshape(npart, nchan), int(n)

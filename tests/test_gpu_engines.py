@@ -1,0 +1,167 @@
+"""Each engine on the GPU through the C ABI against the oracle: unpack,
+detection, fold (bins and hits bit-exact), convolution (the reference's tukey
+known-answer test; dedispersion within 1e-5 relative RMS)."""
+import math
+
+import numpy as np
+import pytest
+
+from helpers import from_dev, make_test_response, rel_rms, to_dev, tukey_kat_check, tukey_kat_input
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("nbit,nchan,npol,ob", [(8, 1, 2, False), (8, 5, 2, True), (16, 3, 2, False),
+                                                (16, 2, 1, True), (8, 4, 1, False)])
+def test_unpack(ctx, oracle, nbit, nchan, npol, ob):
+    import torch
+    rng = np.random.default_rng(nbit + nchan)
+    ndat = 3000
+    if nbit == 8:
+        raw = rng.integers(0, 256, (ndat, nchan, npol, 2)).astype(np.uint8)
+        expect = oracle.unpack8(raw, nchan, npol, ndat, scale=0.5, offset_binary=ob)
+    else:
+        raw = rng.integers(-32768, 32768, (ndat, nchan, npol, 2)).astype(np.int16)
+        expect = oracle.unpack16(raw, nchan, npol, ndat, offset_binary=ob)
+    out = torch.zeros((nchan, npol, ndat, 2), dtype=torch.float32, device="cuda")
+    print("unpack", nbit, nchan, npol, ob, flush=True)
+    ctx.unpack(to_dev(raw), nbit, nchan, npol, ndat, out, npol * ndat * 2, ndat * 2, scale=0.5,
+               offset_binary=ob)
+    assert np.array_equal(from_dev(out), expect)
+
+
+@pytest.mark.parametrize("state,ndim", [(0, 1), (1, 1), (2, 1), (2, 2), (2, 4), (3, 1), (3, 2), (3, 4)])
+def test_detect(ctx, oracle, state, ndim):
+    import torch
+    rng = np.random.default_rng(state * 10 + ndim)
+    nchan, ndat = 3, 5000
+    x = (rng.standard_normal((nchan, 2, ndat)) + 1j * rng.standard_normal((nchan, 2, ndat))).astype(np.complex64)
+    expect = oracle.detect(x, state, ndim)
+    onpol, onf = expect.shape[1], expect.shape[2]
+    out = torch.zeros(expect.shape, dtype=torch.float32, device="cuda")
+    print("detect", state, ndim, flush=True)
+    ctx.detect(state, ndim, to_dev(x.view(np.float32)), 2 * ndat * 2, ndat * 2, nchan, 2, ndat, out,
+               onpol * onf, onf)
+    assert np.array_equal(from_dev(out), expect)
+
+
+@pytest.mark.parametrize("seed", range(3))
+def test_fold_engine(ctx, dspx, oracle, seed):
+    import torch
+    rng = np.random.default_rng(seed)
+    nchan, npol, ndim, ndat = 3, 2, 2, 200_000
+    nbin = [1024, 333, 64][seed]
+    pps = float(10 ** rng.uniform(-6, -3))
+    data = rng.standard_normal((nchan, npol, ndat * ndim)).astype(np.float32)
+    # three sub-integrations with their own start phases, and a gap
+    chunks = [(0, 70_000, 0.31, 0), (70_000, 60_000, -2.2, 1), (140_000, 50_000, 0.9, 2)]
+    prof_o = np.zeros((3, nchan, npol, nbin * ndim), np.float32)
+    hits_o = np.zeros((3, 1, nbin), np.uint32)
+    plans = []
+    for (i0, n, phi, isub) in chunks:
+        _, _, _, bp = oracle.fold(data, ndim, phi, pps, nbin, idat_start=i0, ndat_fold=n,
+                                  prof=prof_o[isub], hits=hits_o[isub], want_binplan=True)
+        plans.append((i0, n, isub, bp))
+    print("fold", seed, nbin, pps, flush=True)
+    f = dspx.Fold(ctx, ndat, pps, nbin, chunks)
+    plan = torch.zeros(ndat, dtype=torch.int32, device="cuda")
+    f.binplan(plan)
+    plan_h = from_dev(plan).view(np.uint32)
+    for (i0, n, isub, bp) in plans:
+        assert np.array_equal(plan_h[i0:i0 + n] & 0xffff, bp)
+        assert np.all((plan_h[i0:i0 + n] >> 16) == isub)
+    assert np.all(plan_h[190_000:] == 0xffffffff)
+    hits = torch.zeros((3, nbin), dtype=torch.int32, device="cuda")
+    f.hits(hits)
+    assert np.array_equal(from_dev(hits).view(np.uint32), hits_o[:, 0])
+    prof = torch.zeros((3, nchan, npol, nbin * ndim), dtype=torch.float32, device="cuda")
+    f.execute(to_dev(data), npol * ndat * ndim, ndat * ndim, nchan, npol, ndim, prof)
+    assert rel_rms(from_dev(prof), prof_o) < 1e-6
+
+
+def _conv_kat(ctx, dspx, oracle, form):
+    import torch
+    n_neg, n_pos = 128, 256
+    nfilt = 4096
+    temporal, spectral = make_test_response(oracle, nfilt, n_neg, n_pos)
+    resp = (spectral * np.float32(1.0 / math.sqrt(nfilt))).astype(np.complex64)
+    n_loop = 269
+    x, offsets, expected_ndat = tukey_kat_input(nfilt, n_neg, n_pos, n_loop)
+    ndat = x.shape[2]
+    step = nfilt - n_neg - n_pos
+    npart = (ndat - n_neg - n_pos) // step
+    assert npart * step == expected_ndat
+    conv = dspx.Convolution(ctx, 1, 2, nfilt, n_pos, n_neg, chirp_mode=dspx.CHIRP_TABLE, form=form)
+    conv.set_response(resp[None, :])
+    out = torch.zeros((2, expected_ndat, 2), dtype=torch.float32, device="cuda")
+    conv.perform(to_dev(x.view(np.float32)), 2 * ndat * 2, ndat * 2, ndat, out, 2 * expected_ndat * 2,
+                 expected_ndat * 2, npart)
+    res = from_dev(out).view(np.complex64).reshape(2, expected_ndat)
+    return tukey_kat_check(res, temporal, offsets, n_neg + n_pos)
+
+
+def test_convolution_tukey_known_answer_fused(ctx, dspx, oracle):
+    """ConvolutionTest.test_operate_tukey (ConvolutionTest.cpp:148-308) with
+    the reference's thresholds, fused form."""
+    print("tukey fused", flush=True)
+    err, rms = _conv_kat(ctx, dspx, oracle, dspx.FORM_FUSED)
+    assert err < 5e-8 and rms < 5e-8, (err, rms)
+
+
+def test_convolution_tukey_known_answer_plain(ctx, dspx, oracle):
+    print("tukey plain", flush=True)
+    err, rms = _conv_kat(ctx, dspx, oracle, dspx.FORM_PLAIN)
+    assert err < 5e-8 and rms < 5e-8, (err, rms)
+
+
+@pytest.mark.parametrize("form", [0, 1])
+@pytest.mark.parametrize("cfreq,bw,nchan,dm,ndat", [
+    (1405.0, 16.0, 1, 56.77, 3 << 19),      # single channel, N = 2^19
+    (1400.0, 400.0, 8, 300.0, 1 << 19),     # N = 2^18 per channel
+    (1500.0, 64.0, 16, 10.0, 1 << 15),      # N = 2^12
+    (1400.0, 50.0, 16, 20.0, 1 << 15),      # N = 2^13
+])
+def test_convolution_dedispersion_parity(ctx, dspx, oracle, form, cfreq, bw, nchan, dm, ndat):
+    """Reference-parity chirp: output within 1e-5 relative RMS of the FP32
+    oracle; error vs the FP64 oracle stated."""
+    import torch
+    g = oracle.geometry(cfreq, bw, nchan, dm)
+    N, pos, neg, step = g["nfft"], g["impulse_pos"], g["impulse_neg"], g["step"]
+    if N < 4096:
+        pytest.skip("fused form starts at 2^12")
+    rng = np.random.default_rng(11)
+    x = (rng.standard_normal((nchan, 2, ndat)) + 1j * rng.standard_normal((nchan, 2, ndat))).astype(np.complex64)
+    resp = oracle.response(cfreq, bw, nchan, dm, N, 0)
+    expect = oracle.convolve(x, resp, pos, neg)
+    npart = expect.shape[2] // step
+    nout = npart * step
+    print("conv parity", form, cfreq, bw, nchan, dm, ndat, N, flush=True)
+    conv = dspx.Convolution(ctx, nchan, 2, N, pos, neg, chirp_mode=dspx.CHIRP_PARITY, form=form,
+                            centre_frequency=cfreq, bandwidth=bw, dm=dm)
+    out = torch.zeros((nchan, 2, nout, 2), dtype=torch.float32, device="cuda")
+    conv.perform(to_dev(x.view(np.float32)), 2 * ndat * 2, ndat * 2, ndat, out, 2 * nout * 2, nout * 2, npart)
+    res = from_dev(out).view(np.complex64).reshape(nchan, 2, nout)
+    err = rel_rms(res, expect)
+    truth = oracle.convolve(x.astype(np.complex128), oracle.response(cfreq, bw, nchan, dm, N, "f64"), pos, neg)
+    print("rel rms vs FP32 oracle %.3g, vs FP64 truth %.3g" % (err, rel_rms(res, truth)))
+    assert err < 1e-5
+
+
+def test_convolution_truth_mode_tracks_fp64(ctx, dspx, oracle):
+    import torch
+    cfreq, bw, nchan, dm, ndat = 1405.0, 16.0, 1, 56.77, 3 << 19
+    g = oracle.geometry(cfreq, bw, nchan, dm)
+    N, pos, neg, step = g["nfft"], g["impulse_pos"], g["impulse_neg"], g["step"]
+    rng = np.random.default_rng(12)
+    x = (rng.standard_normal((nchan, 2, ndat)) + 1j * rng.standard_normal((nchan, 2, ndat))).astype(np.complex64)
+    truth = oracle.convolve(x.astype(np.complex128), oracle.response(cfreq, bw, nchan, dm, N, "f64"), pos, neg)
+    npart = truth.shape[2] // step
+    nout = npart * step
+    conv = dspx.Convolution(ctx, nchan, 2, N, pos, neg, chirp_mode=dspx.CHIRP_TRUTH, form=dspx.FORM_FUSED,
+                            centre_frequency=cfreq, bandwidth=bw, dm=dm)
+    out = torch.zeros((nchan, 2, nout, 2), dtype=torch.float32, device="cuda")
+    conv.perform(to_dev(x.view(np.float32)), 2 * ndat * 2, ndat * 2, ndat, out, 2 * nout * 2, nout * 2, npart)
+    res = from_dev(out).view(np.complex64).reshape(nchan, 2, nout)
+    err = rel_rms(res, truth)
+    print("truth-mode rel rms vs FP64 oracle %.3g" % err)
+    assert err < 1e-5

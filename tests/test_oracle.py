@@ -1,0 +1,346 @@
+"""The oracle pinned against the reference's own C sources (oracle/_ref), its
+own known-answer tests and the mathematics of each step.  CPU only."""
+import math
+from fractions import Fraction
+
+import numpy as np
+import pytest
+
+from helpers import rel_rms, make_test_response, tukey_kat_check, tukey_kat_input
+
+
+# ----------------------------------------------------------------- geometry
+
+def test_optimal_fft_length_matches_reference_source(oracle):
+    ref = oracle.ref()
+    if ref is None:
+        pytest.skip("oracle/_ref not built (reference tree absent)")
+    rng = np.random.default_rng(0)
+    vals = list(range(1, 2000)) + list(rng.integers(1, 5_000_000, 500))
+    for n in vals:
+        for factor in (1, 2, 8):
+            a = oracle.lib().ox_optimal_fft_length(int(n), 0, factor)
+            b = ref.optimal_fft_length(int(n), 0, factor, b"\0")
+            assert a == b, (n, factor, a, b)
+    # bounded search (nfft_max) and the error return
+    for n, mx in ((1000, 4096), (5000, 8192), (100, 64)):
+        assert oracle.lib().ox_optimal_fft_length(n, mx, 1) == ref.optimal_fft_length(n, mx, 1, b"\0")
+
+
+def _smear_closed_form(cfreq, bw, nchan, dm, half, skip=0):
+    """Dispersion delay across half of the lowest supported channel, +10%,
+    in samples."""
+    k = dm / 2.41e-4
+    cw = abs(bw) / nchan
+    lo = cfreq - (abs(bw) - cw) / 2.0 + skip * cw
+    hw = cw / 2
+    c = lo + half * hw
+    f1, f2 = c - hw / 2, c + hw / 2
+    t = k / f1 ** 2 - k / f2 ** 2
+    return math.ceil(t * (1.0 + np.float32(0.1)) * cw * 1e6)
+
+
+def _smear_pair(cfreq, bw, nchan, dm):
+    """Channels whose smear exceeds 16 Mi / nchan samples are skipped
+    (PlasmaResponse.C:89-106)."""
+    skip = 0
+    while _smear_closed_form(cfreq, bw, nchan, dm, -1, skip) > (16 << 20) // nchan:
+        skip += 1
+    return _smear_closed_form(cfreq, bw, nchan, dm, -1, skip), _smear_closed_form(cfreq, bw, nchan, dm, 1, skip), skip
+
+
+@pytest.mark.parametrize("cfreq,bw,nchan,dm", [
+    (1405.0, 16.0, 1, 56.77), (1400.0, 400.0, 128, 300.0), (800.0, 256.0, 1024, 750.0),
+    (300.0, 75.0, 64, 1500.0), (1500.0, 375.0, 512, 5.0), (1400.0, -400.0, 128, 300.0)])
+def test_smearing_closed_form(oracle, cfreq, bw, nchan, dm):
+    g = oracle.geometry(cfreq, bw, nchan, dm)
+    neg, pos, skip = _smear_pair(cfreq, bw, nchan, dm)
+    assert (g["impulse_neg"], g["impulse_pos"], g["nunsupported"]) == (neg, pos, skip)
+    assert g["noverlap"] == g["impulse_neg"] + g["impulse_pos"]
+    assert g["step"] == g["nfft"] - g["noverlap"]
+    n = g["nfft"]
+    assert n & (n - 1) == 0 and n > g["noverlap"]
+
+
+def test_geometry_baseline_configs(oracle):
+    # the values of the single-channel and 128-channel workloads
+    g1 = oracle.geometry(1405.0, 16.0, 1, 56.77)
+    assert (g1["impulse_neg"], g1["impulse_pos"], g1["nfft"], g1["step"]) == (24331, 23514, 524288, 476443)
+    assert oracle.lib().ox_conv_npart(1 << 24, g1["nfft"], g1["noverlap"]) == 35
+    g2 = oracle.geometry(1400.0, 400.0, 128, 300.0)
+    assert (g2["impulse_neg"], g2["impulse_pos"], g2["nfft"], g2["step"]) == (7739, 7679, 262144, 246726)
+    assert oracle.lib().ox_conv_npart(1 << 20, g2["nfft"], g2["noverlap"]) == 4
+
+
+# ---------------------------------------------------------------------- FFT
+
+@pytest.mark.parametrize("n", [2, 4, 8, 16, 64, 256, 1024, 4096])
+def test_fft_brute_force(oracle, n):
+    rng = np.random.default_rng(n)
+    x = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    W = np.exp(-2j * np.pi * np.outer(np.arange(n), np.arange(n)) / n)
+    ref = W @ x
+    assert rel_rms(oracle.fft(x.astype(np.complex128), -1), ref) < 1e-12
+    assert rel_rms(oracle.fft(x.astype(np.complex128), +1), np.conj(W) @ x) < 1e-12
+    assert rel_rms(oracle.fft(x.astype(np.complex64), -1), ref) < 1e-6
+
+
+def test_fft_large_vs_numpy(oracle):
+    rng = np.random.default_rng(7)
+    for n in (1 << 16, 1 << 19):
+        x = (rng.standard_normal(n) + 1j * rng.standard_normal(n))
+        ref = np.fft.fft(x)
+        assert rel_rms(oracle.fft(x.astype(np.complex128)), ref) < 1e-14
+        assert rel_rms(oracle.fft(x.astype(np.complex64)), ref) < 5e-7
+
+
+# -------------------------------------------------------------- convolution
+
+def test_tukey_known_answer(oracle):
+    """ConvolutionTest.test_operate_tukey on the oracle (FP32)."""
+    n_neg, n_pos = 128, 256
+    nfilt = oracle.lib().ox_optimal_fft_length(n_neg + n_pos, 0, 1)
+    assert nfilt == 4096
+    temporal, spectral = make_test_response(oracle, nfilt, n_neg, n_pos)
+    s = np.float32(1.0 / math.sqrt(nfilt))  # Convolution normaliser
+    resp = (spectral * s).astype(np.complex64)[None, :]
+    x, offsets, expected_ndat = tukey_kat_input(nfilt, n_neg, n_pos, 40)
+    out = oracle.convolve(x, resp, n_pos, n_neg)
+    assert out.shape[2] == expected_ndat
+    err, rms = tukey_kat_check(out[0], temporal, offsets, n_neg + n_pos)
+    assert err < 5e-8 and rms < 5e-8, (err, rms)
+
+
+def test_convolution_is_circular_convolution(oracle):
+    """Each kept sample equals the direct circular convolution sum (FP64)."""
+    rng = np.random.default_rng(3)
+    n, pos, neg = 256, 20, 12
+    h = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    H = np.fft.fft(h) / n  # unnormalised FFTs: the response carries 1/n
+    x = (rng.standard_normal(3 * n) + 1j * rng.standard_normal(3 * n))[None, None, :]
+    out = oracle.convolve(x, H[None, :].astype(np.complex128), pos, neg)
+    step = n - pos - neg
+    for p in range(out.shape[2] // step):
+        seg = x[0, 0, p * step:p * step + n]
+        for j in (0, 7, step - 1):
+            t = pos + j
+            direct = sum(seg[m] * h[(t - m) % n] for m in range(n))
+            assert abs(out[0, 0, p * step + j] - direct) < 1e-10
+
+
+def test_zero_dm_copy(oracle):
+    """test_Convolution.cpp:151-186: the zero-DM output is the input, bit for bit."""
+    rng = np.random.default_rng(4)
+    n, pos, neg = 512, 40, 30
+    x = (rng.standard_normal((2, 2, 2000)) + 1j * rng.standard_normal((2, 2, 2000))).astype(np.complex64)
+    resp = oracle.response(1400.0, 8.0, 2, 10.0, n, 0)
+    out, zdm = oracle.convolve(x, resp, pos, neg, zero_dm=True)
+    step = n - pos - neg
+    npart = out.shape[2] // step
+    for p in range(npart):
+        assert np.array_equal(zdm[:, :, p * step:(p + 1) * step], x[:, :, p * step + pos:p * step + pos + step])
+
+
+# ---------------------------------------------------------------- the chirp
+
+def test_chirp_group_delay_is_cold_plasma_delay(oracle):
+    """(1/2pi) d(phase)/df of the dedispersion response equals the cold-plasma
+    delay difference D/(fc+f)^2 - D/fc^2 (microseconds): pins the sign, the
+    coefficient and the frequency grid of the chirp."""
+    cfreq, bw, dm, n = 1405.0, 16.0, 56.77, 1 << 19
+    H = oracle.response(cfreq, bw, 1, dm, n, "f64")[0]
+    f = np.fft.fftfreq(n, d=1.0) * n * (bw / n)  # MHz offset of FFT bin k
+    order = np.argsort(f)
+    ph = np.unwrap(np.angle(H[order]))
+    fs = f[order]
+    dpdf = np.gradient(ph, fs)
+    tau = dpdf / (2 * np.pi)
+    D = 1e6 * dm / 2.41e-4
+    expect = D / (cfreq + fs) ** 2 - D / cfreq ** 2
+    inner = slice(n // 8, n - n // 8)
+    assert np.max(np.abs(tau[inner] - expect[inner])) < 1e-3 * np.max(np.abs(expect))
+
+
+def test_dedispersion_restores_dispersed_impulse(oracle):
+    """An impulse dispersed by the conjugate chirp (FP64) comes back as an
+    impulse at its original sample after the FP32 oracle convolution."""
+    cfreq, bw, dm = 1405.0, 16.0, 5.0
+    g = oracle.geometry(cfreq, bw, 1, dm)
+    n = g["nfft"]
+    ndat = 3 * n
+    H = oracle.response(cfreq, bw, 1, dm, ndat, "f64")[0] * math.sqrt(ndat)
+    x = np.zeros(ndat, np.complex128)
+    t0 = g["impulse_pos"] + n // 2 + 17
+    x[t0] = 1.0
+    disp = np.fft.ifft(np.fft.fft(x) * np.conj(H))
+    resp = oracle.response(cfreq, bw, 1, dm, n, 0)
+    out = oracle.convolve(disp.astype(np.complex64)[None, None, :], resp, g["impulse_pos"], g["impulse_neg"])
+    y = np.abs(out[0, 0]) / math.sqrt(n)  # unnormalised FFTs leave a factor sqrt(N)
+    peak = int(np.argmax(y))
+    assert peak == t0 - g["impulse_pos"]
+    assert y[peak] > 0.99
+    others = np.delete(y, peak)
+    assert others.max() < 0.02
+
+
+def test_parity_chirp_differs_from_truth_by_float_phase_rounding(oracle):
+    """The reference stores the phase as float (Dedispersion.C:104-111): the
+    parity response differs from the FP64 truth by at most half an ulp of the
+    phase, and not otherwise."""
+    cfreq, bw, dm, n = 1405.0, 16.0, 56.77, 1 << 16
+    Hp = oracle.response(cfreq, bw, 1, dm, n, 0)[0].astype(np.complex128)
+    Ht = oracle.response(cfreq, bw, 1, dm, n, "f64")[0]
+    lib = oracle.lib()
+    cw = bw
+    cf = lib.ox_channel_centre(cfreq, bw, 1, 0, 0)
+    ph = np.array([lib.ox_chirp_phase(bw, dm, cf, lib.ox_chirp_freq(cw, n, (k + n // 2) % n)) for k in range(0, n, 97)])
+    half_ulp = np.spacing(np.abs(ph).astype(np.float32)).astype(np.float64) / 2
+    dphi = np.abs(np.angle(Hp[::97] / Ht[::97]))
+    assert np.all(dphi <= half_ulp + 1e-6)
+
+
+# ---------------------------------------------------------------- detection
+
+def test_detect_matches_reference_source(oracle):
+    ref = oracle.ref()
+    if ref is None:
+        pytest.skip("oracle/_ref not built")
+    rng = np.random.default_rng(5)
+    ndat = 1000
+    x = (rng.standard_normal((1, 2, ndat)) + 1j * rng.standard_normal((1, 2, ndat))).astype(np.complex64)
+    p = np.ascontiguousarray(x[0, 0]).view(np.float32)
+    q = np.ascontiguousarray(x[0, 1]).view(np.float32)
+    for state, fn in ((oracle.STOKES, ref.stokes_detect), (oracle.COHERENCE, ref.cross_detect)):
+        out = np.zeros(4 * ndat, np.float32)
+        fn(ndat, p.ctypes.data, q.ctypes.data, out.ctypes.data, out[1:].ctypes.data,
+           out[2:].ctypes.data, out[3:].ctypes.data, 4)
+        mine = oracle.detect(x, state, ndim=4)
+        assert np.array_equal(mine.ravel(), out), state
+
+
+def test_detect_closed_forms(oracle):
+    """test/Detection/test_Detection.cpp:52-236 index fillers."""
+    nchan, npol, ndat = 8, 2, 100
+    f = np.zeros((nchan, npol, ndat), np.complex64)
+    for c in range(nchan):
+        for p in range(npol):
+            v = np.float32(c * npol * ndat + p * ndat) + np.arange(ndat, dtype=np.float32)
+            f[c, p] = v + 1j * v
+    inten = oracle.detect(f, oracle.INTENSITY)
+    expect = sum(2 * f.real[:, p].astype(np.float64) ** 2 for p in range(npol))
+    assert np.array_equal(inten[:, 0], expect.astype(np.float32))
+    ppqq = oracle.detect(f, oracle.PPQQ)
+    assert np.array_equal(ppqq, (2 * f.real.astype(np.float64) ** 2).astype(np.float32))
+    coh = oracle.detect(f, oracle.COHERENCE, ndim=1)
+    pr, qr = f.real[:, 0].astype(np.float64), f.real[:, 1].astype(np.float64)
+    assert np.array_equal(coh[:, 2], (2 * pr * qr).astype(np.float32))
+    assert np.array_equal(coh[:, 3], np.zeros_like(coh[:, 3]))
+
+
+# ------------------------------------------------------------------- unpack
+
+def test_unpack_closed_forms(oracle):
+    rng = np.random.default_rng(6)
+    raw = rng.integers(-128, 128, (50, 3, 2, 2)).astype(np.int8)
+    out = oracle.unpack8(raw, 3, 2, 50, scale=0.25)
+    expect = (raw.astype(np.float32) + 0.5) * np.float32(0.25)
+    assert np.array_equal(out, np.transpose(expect, (1, 2, 0, 3)))
+    ob = oracle.unpack8(raw.view(np.uint8), 3, 2, 50, scale=1.0, offset_binary=True)
+    assert np.array_equal(ob, np.transpose(raw.view(np.uint8).astype(np.float32) - 127.5, (1, 2, 0, 3)))
+    r16 = rng.integers(-32768, 32768, (40, 2, 2, 2)).astype(np.int16)
+    o16 = oracle.unpack16(r16, 2, 2, 40)
+    assert np.array_equal(o16, np.transpose(r16.astype(np.float32) + 0.5, (1, 2, 0, 3)))
+    o16b = oracle.unpack16(r16, 2, 2, 40, offset_binary=True)
+    assert np.array_equal(o16b, np.transpose((r16 ^ np.int16(-32768)).astype(np.float32) + 0.5, (1, 2, 0, 3)))
+
+
+# --------------------------------------------------------------------- fold
+
+def _fold_python(data, ndim, phi, pps, nbin, weights=None, ndpw=0, zeroed=False):
+    """Fold.C:662-949 in plain Python (small inputs)."""
+    nchan, npol, nf = data.shape
+    ndat = nf // ndim
+    prof = np.zeros((nchan, npol, nbin * ndim), np.float32)
+    hits = np.zeros(nbin, np.uint32)
+    plan = []
+    for i in range(ndat):
+        bad = weights is not None and not zeroed and weights[i // ndpw] == 0
+        phi -= math.floor(phi)
+        b = int(phi * float(nbin))
+        phi += pps
+        plan.append(nbin if bad else b)
+        if not bad and not zeroed:
+            hits[b] += 1
+    for c in range(nchan):
+        for p in range(npol):
+            for i in range(ndat):
+                if plan[i] != nbin:
+                    for d in range(ndim):
+                        prof[c, p, plan[i] * ndim + d] += data[c, p, i * ndim + d]
+    return prof, hits, plan
+
+
+def test_fold_brute_force(oracle):
+    rng = np.random.default_rng(8)
+    data = rng.standard_normal((2, 2, 3 * 600)).astype(np.float32)
+    w = rng.integers(0, 3, 600 // 50).astype(np.uint16)
+    for phi, pps, nbin, weights in ((0.3, 0.0137, 64, None), (-1.7, 0.0091, 100, w), (5.25, 0.003, 17, None)):
+        prof, hits, nfold, plan = oracle.fold(data, 3, phi, pps, nbin, weights=weights,
+                                              ndatperweight=50 if weights is not None else 0,
+                                              want_binplan=True)
+        p2, h2, plan2 = _fold_python(data, 3, phi, pps, nbin, weights, 50)
+        assert list(plan) == plan2
+        assert np.array_equal(hits[0], h2)
+        assert rel_rms(prof, p2) < 1e-6
+
+
+def test_fold_constant_period_closed_form(oracle):
+    """Bins follow floor(frac(phi0 + j*pps) * nbin) in exact arithmetic, except
+    for samples within rounding distance of a bin edge."""
+    phi0, pps, nbin, n = 0.123456789, 6.996529721258255e-07, 1024, 400000
+    _, _, _, plan = oracle.fold(np.zeros((1, 1, n), np.float32), 1, phi0, pps, nbin, want_binplan=True)
+    F0, Fp = Fraction(phi0), Fraction(pps)
+    mism = 0
+    for j in range(0, n, 7):
+        x = (F0 + j * Fp) % 1
+        b = math.floor(x * nbin)
+        edge = abs(float(x * nbin) - round(float(x * nbin)))
+        if plan[j] != b:
+            assert edge < 1e-6
+            mism += 1
+    assert mism < 10
+
+
+def test_fold_zeroed_hits(oracle):
+    """With zeroed data (SK), hits count non-zero samples per channel
+    (Fold.C:911-930)."""
+    data = np.ones((2, 1, 200), np.float32)
+    data[0, 0, ::3] = 0
+    prof, hits, nfold = oracle.fold(data, 1, 0.0, 0.01, 10, zeroed=True, hits_nchan=2)
+    assert hits[0].sum() == 200 - 67 and hits[1].sum() == 200
+    assert nfold == (200 - 67) + 200
+
+
+# ----------------------------------------------------------------------- SK
+
+def test_sk_estimator_closed_form(oracle):
+    """Constant-power blocks give SK = 0 (M S2 / S1^2 = 1); a Gaussian
+    block's estimator is near 1 (Nita & Gary 2010)."""
+    M = 1024
+    x = np.ones((1, 1, 4 * M), np.complex64)
+    sums, counts = oracle.sk_compute(x, M)
+    assert np.all(counts == M)
+    for b in range(4):
+        s1, s2 = sums[b, 0, 0]
+        V = np.float32(np.float32(M + 1) / (M - 1)) * (M * (s2 / (s1 * s1)) - 1)
+        assert abs(V) < 1e-4
+    rng = np.random.default_rng(9)
+    g = (rng.standard_normal((1, 1, 64 * M)) + 1j * rng.standard_normal((1, 1, 64 * M))).astype(np.complex64)
+    sums, counts = oracle.sk_compute(g, M)
+    V = [(M + 1) / (M - 1) * (M * sums[b, 0, 0, 1] / sums[b, 0, 0, 0] ** 2 - 1) for b in range(64)]
+    assert abs(np.mean(V) - 1.0) < 0.02
+    lower = np.full(M + 1, 0.8, np.float32)
+    upper = np.full(M + 1, 1.2, np.float32)
+    mask, nz = oracle.sk_detect_skfb(sums, counts, lower, upper)
+    expect = sum(1 for v in V if v < 0.8 or v > 1.2)
+    assert nz == expect
