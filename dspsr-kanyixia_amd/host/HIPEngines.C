@@ -1,0 +1,311 @@
+// HIPEngines.C -- dsp::HIP engines over the libdspx C ABI.
+#include "dsp/HIPEngines.h"
+
+#include <hip/hip_runtime_api.h>
+
+#include <algorithm>
+#include <cmath>
+
+namespace dsp {
+namespace HIP {
+
+Context::Context(int device, void* stream)
+{
+  int rc = dspx_ctx_create(device, stream, &ctx);
+  if (rc != DSPX_OK) throw Error(FailedCall, "dsp::HIP::Context", "dspx_ctx_create failed (%d)", rc);
+}
+
+Context::~Context() { dspx_ctx_destroy(ctx); }
+
+void Context::synchronize() { check(dspx_ctx_synchronize(ctx), "dsp::HIP::Context::synchronize"); }
+
+void Context::check(int status, const char* where) const
+{
+  if (status == DSPX_OK) return;
+  ErrorCode code = status == DSPX_E_PARAM ? InvalidParam : status == DSPX_E_STATE ? InvalidState : FailedCall;
+  throw Error(code, where, "%s", dspx_last_error(ctx));
+}
+
+static void require_device(const void* p, bool on_device, const char* where)
+{
+  if (!on_device) throw Error(InvalidState, where, "buffer is not in device memory");
+  (void) p;
+}
+
+// ---------------------------------------------------------------- Unpacker
+
+void Unpacker::unpack(const BitSeries* in, TimeSeries* out)
+{
+  require_device(in->get_rawptr(), in->get_memory()->on_device, "dsp::HIP::Unpacker::unpack");
+  out->copy_configuration(in);
+  out->set_state(Signal::Analytic);
+  out->set_ndim(2);
+  out->resize(in->get_ndat());
+  require_device(out->get_datptr(), out->get_memory()->on_device, "dsp::HIP::Unpacker::unpack");
+  ctx->check(dspx_unpack(ctx->get(), in->get_rawptr(), in->get_nbit(), in->offset_binary ? 1 : 0, in->get_nchan(),
+                         in->get_npol(), in->get_ndat(), scale, out->get_datptr(), out->chan_span(),
+                         out->get_pol_span()),
+             "dsp::HIP::Unpacker::unpack");
+}
+
+// ------------------------------------------------------ ConvolutionEngine
+
+ConvolutionEngine::~ConvolutionEngine() { dspx_conv_destroy(conv); }
+
+void ConvolutionEngine::prepare(Convolution* convolution)
+{
+  parent = convolution;
+  create(convolution->get_response());
+}
+
+void ConvolutionEngine::create(const Response* response)
+{
+  if (conv) {
+    dspx_conv_destroy(conv);
+    conv = nullptr;
+  }
+  const TimeSeries* in = parent->get_input();
+  dspx_conv_desc d{};
+  d.nchan = in->get_nchan();
+  d.npol = in->get_npol();
+  d.nfft = parent->get_minimum_samples();
+  d.nfilt_pos = parent->get_nfilt_pos();
+  d.nfilt_neg = parent->get_nfilt_neg();
+  d.form = form;
+  auto* dd = dynamic_cast<const Dedispersion*>(response);
+  if (dd) {
+    // the chirp is evaluated on the device from the band and DM
+    d.chirp_mode = DSPX_CHIRP_PARITY;
+    d.dedisp = dspx_dedisp_desc{dd->get_centre_frequency(), dd->get_bandwidth(), dd->get_dispersion_measure(),
+                                dd->get_nchan(), dd->get_dc_centred() ? 1u : 0u, d.nfft, 0};
+  } else {
+    d.chirp_mode = DSPX_CHIRP_TABLE;
+  }
+  ctx->check(dspx_conv_create(ctx->get(), &d, &conv), "dsp::HIP::ConvolutionEngine::prepare");
+}
+
+void ConvolutionEngine::copy_response(const Response* response)
+{
+  if (dynamic_cast<const Dedispersion*>(response)) {
+    // a DM change rebuilds the on-chip chirp parameters
+    create(response);
+    return;
+  }
+  const auto& data = response->get_data();
+  if (data.size() != size_t(response->get_nchan()) * response->get_ndat() * 2)
+    throw Error(InvalidState, "dsp::HIP::ConvolutionEngine::copy_response", "response has no host data");
+  ctx->check(dspx_conv_set_response(conv, data.data()), "dsp::HIP::ConvolutionEngine::copy_response");
+}
+
+void ConvolutionEngine::perform(const TimeSeries* in, TimeSeries* out, unsigned npart)
+{
+  perform(in, out, nullptr, npart);
+}
+
+void ConvolutionEngine::perform(const TimeSeries* in, TimeSeries* out, TimeSeries* zdm, unsigned npart)
+{
+  ctx->check(dspx_conv_perform(conv, in->get_datptr(), in->chan_span(), in->get_pol_span(), in->get_ndat(),
+                               out->get_datptr(), out->chan_span(), out->get_pol_span(),
+                               zdm ? zdm->get_datptr() : nullptr, npart),
+             "dsp::HIP::ConvolutionEngine::perform");
+}
+
+// -------------------------------------------------------- DetectionEngine
+
+void DetectionEngine::square_law(const TimeSeries* in, TimeSeries* out)
+{
+  uint32_t st = out->get_npol() == 1 ? DSPX_INTENSITY : DSPX_PPQQ;
+  ctx->check(dspx_detect(ctx->get(), st, 1, in->get_datptr(), in->chan_span(), in->get_pol_span(), in->get_nchan(),
+                         in->get_npol(), in->get_ndat(), out->get_datptr(), out->chan_span(), out->get_pol_span()),
+             "dsp::HIP::DetectionEngine::square_law");
+}
+
+void DetectionEngine::polarimetry(unsigned ndim, const TimeSeries* in, TimeSeries* out)
+{
+  uint32_t st = out->get_state() == Signal::Stokes ? DSPX_STOKES : DSPX_COHERENCE;
+  ctx->check(dspx_detect(ctx->get(), st, ndim, in->get_datptr(), in->chan_span(), in->get_pol_span(),
+                         in->get_nchan(), in->get_npol(), in->get_ndat(), out->get_datptr(), out->chan_span(),
+                         out->get_pol_span()),
+             "dsp::HIP::DetectionEngine::polarimetry");
+}
+
+// ------------------------------------------------------------- FoldEngine
+
+FoldEngine::~FoldEngine()
+{
+  dspx_fold_destroy(plan);
+  if (d_prof) (void) hipFree(d_prof);
+  if (d_hits) (void) hipFree(d_hits);
+}
+
+void FoldEngine::set_nbin(unsigned n)
+{
+  if (n == nbin) return;
+  nbin = n;
+  if (d_hits) (void) hipFree(d_hits);
+  d_hits = nullptr;
+  if (hipMalloc(&d_hits, nbin * sizeof(uint32_t)) != hipSuccess)
+    throw Error(FailedCall, "dsp::HIP::FoldEngine::set_nbin", "hipMalloc");
+  hits_host.assign(nbin, 0);
+}
+
+void FoldEngine::set_bin(uint64_t, double, double)
+{
+  throw Error(InvalidState, "dsp::HIP::FoldEngine::set_bin", "this engine builds its plan with set_bins");
+}
+
+uint64_t FoldEngine::set_bins(double phi, double pps, uint64_t ndat, uint64_t idat_start)
+{
+  dspx_fold_destroy(plan);
+  plan = nullptr;
+  // the plan covers the input block; only [idat_start, idat_start + ndat) folds
+  uint64_t block = in->get_ndat();
+  dspx_fold_chunk chunk{idat_start, ndat, phi, 0, 0};
+  ctx->check(dspx_fold_create(ctx->get(), block, pps, nbin, &chunk, 1, &plan), "dsp::HIP::FoldEngine::set_bins");
+  plan_ndat = block;
+  // hits of this call, for Fold::fold's hits[ibin] += get_bin_hits(ibin)
+  hipStream_t s = static_cast<hipStream_t>(dspx_ctx_stream(ctx->get()));
+  if (hipMemsetAsync(d_hits, 0, nbin * sizeof(uint32_t), s) != hipSuccess)
+    throw Error(FailedCall, "dsp::HIP::FoldEngine::set_bins", "hipMemsetAsync");
+  ctx->check(dspx_fold_hits(plan, d_hits), "dsp::HIP::FoldEngine::set_bins");
+  if (hipMemcpyAsync(hits_host.data(), d_hits, nbin * sizeof(uint32_t), hipMemcpyDeviceToHost, s) != hipSuccess)
+    throw Error(FailedCall, "dsp::HIP::FoldEngine::set_bins", "hipMemcpyAsync");
+  ctx->synchronize();
+  ndat_folded = 0;
+  for (auto h : hits_host) ndat_folded += h;
+  return ndat_folded;
+}
+
+uint64_t FoldEngine::get_bin_hits(int ibin) { return hits_host.at(size_t(ibin)); }
+
+void FoldEngine::fold()
+{
+  size_t need = size_t(in->get_nchan()) * in->get_npol() * nbin * in->get_ndim();
+  if (need != prof_floats) {
+    if (d_prof) (void) hipFree(d_prof);
+    d_prof = nullptr;
+    if (hipMalloc(&d_prof, need * sizeof(float)) != hipSuccess)
+      throw Error(FailedCall, "dsp::HIP::FoldEngine::fold", "hipMalloc");
+    prof_floats = need;
+    zero();
+  }
+  ctx->check(dspx_fold_execute(plan, in->get_datptr(), in->chan_span(), in->get_pol_span(), in->get_nchan(),
+                               in->get_npol(), in->get_ndim(), d_prof),
+             "dsp::HIP::FoldEngine::fold");
+}
+
+void FoldEngine::zero()
+{
+  if (!d_prof) return;
+  hipStream_t s = static_cast<hipStream_t>(dspx_ctx_stream(ctx->get()));
+  if (hipMemsetAsync(d_prof, 0, prof_floats * sizeof(float), s) != hipSuccess)
+    throw Error(FailedCall, "dsp::HIP::FoldEngine::zero", "hipMemsetAsync");
+}
+
+void FoldEngine::synch(PhaseSeries* output)
+{
+  // TransferPhaseSeriesCUDA D2H (FoldCUDA.cu:121-146); host profiles are FPT
+  if (!d_prof) return;
+  ctx->synchronize();
+  std::vector<float> h(prof_floats);
+  if (hipMemcpy(h.data(), d_prof, prof_floats * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess)
+    throw Error(FailedCall, "dsp::HIP::FoldEngine::synch", "hipMemcpy");
+  size_t per = size_t(nbin) * output->get_ndim();
+  for (unsigned c = 0; c < output->get_nchan(); c++)
+    for (unsigned p = 0; p < output->get_npol(); p++)
+      std::copy(h.begin() + (size_t(c) * output->get_npol() + p) * per,
+                h.begin() + (size_t(c) * output->get_npol() + p + 1) * per, output->get_datptr(c, p));
+}
+
+// ------------------------------------------------------- LoadToFoldEngine
+
+LoadToFoldEngine::~LoadToFoldEngine()
+{
+  dspx_ltf_destroy(ltf);
+  if (d_prof) (void) hipFree(d_prof);
+  if (d_hits) (void) hipFree(d_hits);
+}
+
+static uint32_t state_code(Signal::State s)
+{
+  switch (s) {
+    case Signal::Intensity: return DSPX_INTENSITY;
+    case Signal::PPQQ: return DSPX_PPQQ;
+    case Signal::Coherence: return DSPX_COHERENCE;
+    case Signal::Stokes: return DSPX_STOKES;
+    default: throw Error(InvalidState, "dsp::HIP::LoadToFoldEngine", "invalid output state");
+  }
+}
+
+void LoadToFoldEngine::prepare(const BitSeries* raw)
+{
+  dspx_dedisp_desc dd{raw->get_centre_frequency(), raw->get_bandwidth(), raw->get_dispersion_measure(),
+                      raw->get_nchan(), raw->get_dc_centred() ? 1u : 0u, cfg.nfft, 0};
+  ctx->check(dspx_dedisp_geometry(&dd, &geom), "dsp::HIP::LoadToFoldEngine::prepare");
+  dspx_ltf_desc d{};
+  d.nbit = raw->get_nbit();
+  d.offset_binary = raw->offset_binary ? 1 : 0;
+  d.scale = cfg.scale;
+  d.nchan = raw->get_nchan();
+  d.npol = raw->get_npol();
+  d.ndat = raw->get_ndat();
+  d.dedisp = dd;
+  d.nfft = geom.nfft;
+  d.nfilt_pos = geom.impulse_pos;
+  d.nfilt_neg = geom.impulse_neg;
+  d.chirp_mode = cfg.chirp_mode;
+  d.state = state_code(cfg.state);
+  d.ndim = cfg.ndim;
+  d.nbin = cfg.nbin;
+  d.form = cfg.form;
+  dspx_ltf_destroy(ltf);
+  ltf = nullptr;
+  ctx->check(dspx_ltf_create(ctx->get(), &d, &ltf), "dsp::HIP::LoadToFoldEngine::prepare");
+  dspx_ltf_geometry(ltf, &npart, &ndat_out);
+  // sub-integrations: equal spans of the block, each folded from the phase
+  // of its first sample's midpoint (Fold.C:686-693, TimeDivide)
+  double rate = raw->get_rate();
+  std::vector<dspx_fold_chunk> chunks(cfg.nsub);
+  for (unsigned k = 0; k < cfg.nsub; k++) {
+    uint64_t i0 = uint64_t(std::llround(double(ndat_out) * k / cfg.nsub));
+    uint64_t i1 = uint64_t(std::llround(double(ndat_out) * (k + 1) / cfg.nsub));
+    double t = raw->get_start_time() + (double(geom.impulse_pos) + double(i0) + 0.5) / rate;
+    double phi = std::fmod(t, cfg.folding_period) / cfg.folding_period - cfg.reference_phase;
+    chunks[k] = dspx_fold_chunk{i0, i1 - i0, phi, k, 0};
+  }
+  ctx->check(dspx_ltf_set_fold(ltf, (1.0 / rate) / cfg.folding_period, chunks.data(), cfg.nsub),
+             "dsp::HIP::LoadToFoldEngine::prepare");
+  unsigned nval = cfg.state == Signal::Intensity ? 1 : cfg.state == Signal::PPQQ ? raw->get_npol() : 4;
+  prof_floats = size_t(cfg.nsub) * raw->get_nchan() * nval * cfg.nbin;
+  hits_count = size_t(cfg.nsub) * cfg.nbin;
+  if (d_prof) (void) hipFree(d_prof);
+  if (d_hits) (void) hipFree(d_hits);
+  d_prof = nullptr;
+  d_hits = nullptr;
+  if (hipMalloc(&d_prof, prof_floats * sizeof(float)) != hipSuccess ||
+      hipMalloc(&d_hits, hits_count * sizeof(uint32_t)) != hipSuccess)
+    throw Error(FailedCall, "dsp::HIP::LoadToFoldEngine::prepare", "hipMalloc");
+  hipStream_t s = static_cast<hipStream_t>(dspx_ctx_stream(ctx->get()));
+  if (hipMemsetAsync(d_prof, 0, prof_floats * sizeof(float), s) != hipSuccess ||
+      hipMemsetAsync(d_hits, 0, hits_count * sizeof(uint32_t), s) != hipSuccess)
+    throw Error(FailedCall, "dsp::HIP::LoadToFoldEngine::prepare", "hipMemsetAsync");
+}
+
+void LoadToFoldEngine::operate(const BitSeries* raw)
+{
+  if (!ltf) prepare(raw);
+  ctx->check(dspx_ltf_execute(ltf, raw->get_rawptr(), d_prof, d_hits), "dsp::HIP::LoadToFoldEngine::operate");
+}
+
+void LoadToFoldEngine::get_result(std::vector<float>& prof, std::vector<uint32_t>& hits)
+{
+  ctx->synchronize();
+  prof.resize(prof_floats);
+  hits.resize(hits_count);
+  if (hipMemcpy(prof.data(), d_prof, prof_floats * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess ||
+      hipMemcpy(hits.data(), d_hits, hits_count * sizeof(uint32_t), hipMemcpyDeviceToHost) != hipSuccess)
+    throw Error(FailedCall, "dsp::HIP::LoadToFoldEngine::get_result", "hipMemcpy");
+}
+
+}  // namespace HIP
+}  // namespace dsp

@@ -1,0 +1,254 @@
+// Operations.C -- data model and Transformation logic of the hot path
+// (engine branches only; see dsp/Operations.h).
+#include <hip/hip_runtime_api.h>
+
+#include <cmath>
+#include <cstring>
+
+#include "dsp/Operations.h"
+#include "dspx.h"
+
+namespace dsp {
+
+// ------------------------------------------------------------------ Memory
+
+static Memory g_host{false};
+static Memory g_device{true};
+Memory* Memory::host() { return &g_host; }
+Memory* Memory::device() { return &g_device; }
+
+void* Memory::allocate(size_t bytes)
+{
+  if (!bytes) return nullptr;
+  void* p = nullptr;
+  if (on_device) {
+    if (hipMalloc(&p, bytes) != hipSuccess) throw Error(FailedCall, "dsp::Memory::allocate", "hipMalloc(%zu)", bytes);
+  } else {
+    p = std::malloc(bytes);
+    if (!p) throw Error(FailedCall, "dsp::Memory::allocate", "malloc(%zu)", bytes);
+  }
+  return p;
+}
+
+void Memory::free(void* p)
+{
+  if (!p) return;
+  if (on_device)
+    (void) hipFree(p);
+  else
+    std::free(p);
+}
+
+void Memory::zero(void* p, size_t bytes, void* stream)
+{
+  if (!p || !bytes) return;
+  if (on_device) {
+    if (hipMemsetAsync(p, 0, bytes, static_cast<hipStream_t>(stream)) != hipSuccess)
+      throw Error(FailedCall, "dsp::Memory::zero", "hipMemsetAsync");
+  } else {
+    std::memset(p, 0, bytes);
+  }
+}
+
+void TimeSeries::release()
+{
+  memory->free(buffer);
+  buffer = nullptr;
+  capacity = 0;
+}
+
+void TimeSeries::resize(uint64_t n)
+{
+  uint64_t need = n * ndim;
+  if (need > capacity || uint64_t(npol) * nchan * need > uint64_t(npol) * nchan * capacity) {
+    release();
+    buffer = static_cast<float*>(memory->allocate(size_t(nchan) * npol * need * sizeof(float)));
+    capacity = need;
+  }
+  pol_span = need;
+  ndat = n;
+}
+
+void TimeSeries::zero() { memory->zero(buffer, size_t(nchan) * npol * pol_span * sizeof(float)); }
+
+BitSeries::~BitSeries() { memory->free(data); }
+
+void BitSeries::resize(uint64_t n, unsigned nb)
+{
+  memory->free(data);
+  ndat = n;
+  nbit = nb;
+  data = static_cast<unsigned char*>(memory->allocate(size_t(n) * nchan * npol * 2 * nbit / 8));
+}
+
+// ----------------------------------------------------------- Dedispersion
+
+void Dedispersion::prepare(const Observation* input, unsigned channels)
+{
+  unsigned n = channels ? channels : input->get_nchan();
+  if (n != nchan || input->get_centre_frequency() != centre_frequency || input->get_bandwidth() != bandwidth ||
+      input->get_dc_centred() != dc_centred)
+    changed = true;
+  nchan = n;
+  centre_frequency = input->get_centre_frequency();
+  bandwidth = input->get_bandwidth();
+  dc_centred = input->get_dc_centred();
+  dspx_dedisp_desc d{centre_frequency, bandwidth, dispersion_measure, nchan, dc_centred ? 1u : 0u, user_ndat, 0};
+  dspx_geometry g;
+  int rc = dspx_dedisp_geometry(&d, &g);
+  if (rc == DSPX_E_STATE)
+    throw Error(InvalidState, "dsp::PlasmaResponse::prepare", "smearing exceeds the threshold in every channel");
+  if (rc != DSPX_OK) throw Error(InvalidParam, "dsp::Dedispersion::prepare", "invalid band or DM");
+  impulse_neg = g.impulse_neg;
+  impulse_pos = g.impulse_pos;
+  ndat = g.nfft;
+}
+
+// ------------------------------------------------------------ Convolution
+
+void Convolution::prepare()
+{
+  if (prepared) return;
+  if (!response) throw Error(InvalidState, "dsp::Convolution::prepare", "no frequency response");
+  if (input->get_detected()) throw Error(InvalidState, "dsp::Convolution::prepare", "input data are detected");
+  if (input->get_state() != Signal::Analytic)
+    throw Error(InvalidState, "dsp::Convolution::prepare", "only Analytic (complex) input is on this path");
+  if (!engine) throw Error(InvalidState, "dsp::Convolution::prepare", "no engine (the CPU branch is the oracle)");
+  response->prepare(input);
+  if (response->get_ndat() < 2)
+    throw Error(InvalidState, "dsp::Convolution::prepare", "invalid response size = %u", response->get_ndat());
+  if (response->get_nchan() != input->get_nchan())
+    throw Error(InvalidState, "dsp::Convolution::prepare", "invalid response nsub=%d != nchan=%d",
+                response->get_nchan(), input->get_nchan());
+  nsamp_fft = response->get_ndat();
+  nfilt_pos = response->get_impulse_pos();
+  nfilt_neg = response->get_impulse_neg();
+  nsamp_overlap = nfilt_pos + nfilt_neg;
+  if (nsamp_fft < nsamp_overlap)
+    throw Error(InvalidState, "dsp::Convolution::prepare", "error nfft=%d < nfilt=%d", nsamp_fft, nsamp_overlap);
+  prepare_output();
+  engine->prepare(this);
+  prepared = true;
+}
+
+void Convolution::prepare_output()
+{
+  nsamp_step = nsamp_fft - nsamp_overlap;
+  uint64_t ndat = input->get_ndat();
+  npart = 0;
+  if (ndat >= nsamp_fft) npart = (ndat - nsamp_overlap) / nsamp_step;  // Convolution.C:485-487
+  output->copy_configuration(input);
+  output->set_state(Signal::Analytic);
+  output->set_ndim(2);
+}
+
+void Convolution::operate()
+{
+  if (!prepared) prepare();
+  if (response->has_changed()) {  // Response::match -> Engine::copy_response
+    engine->copy_response(response);
+    response->set_changed(false);
+  }
+  prepare_output();
+  uint64_t output_ndat = npart * nsamp_step;  // Convolution::reserve (:601-611)
+  output->resize(output_ndat);
+  output->set_start_time(input->get_start_time());
+  output->change_start_time(nfilt_pos);
+  if (zero_DM) {
+    if (!zero_DM_output) throw Error(InvalidState, "dsp::Convolution::operate", "zero_DM without output");
+    zero_DM_output->copy_configuration(output);
+    zero_DM_output->resize(output_ndat);
+    engine->perform(input, output, zero_DM_output, unsigned(npart));
+  } else {
+    engine->perform(input, output, unsigned(npart));
+  }
+}
+
+// -------------------------------------------------------------- Detection
+
+void Detection::operate()
+{
+  if (!engine) throw Error(InvalidState, "dsp::Detection::operate", "no engine");
+  if (input->get_ndat() == 0) return;
+  unsigned npol = input->get_npol();
+  output->copy_configuration(input);
+  output->set_state(state);
+  if (state == Signal::Intensity) {
+    output->set_npol(1);
+    output->set_ndim(1);
+  } else if (state == Signal::PPQQ) {
+    output->set_npol(npol);
+    output->set_ndim(1);
+  } else if (state == Signal::Coherence || state == Signal::Stokes) {
+    if (npol != 2)
+      throw Error(InvalidState, "dsp::Detection::checks", "invalid npol=%d for %s formation", npol,
+                  state == Signal::Stokes ? "Stokes" : "Coherence");
+    if (!(ndim == 1 || ndim == 2 || ndim == 4))
+      throw Error(InvalidState, "dsp::Detection::checks", "invalid ndim=%d", ndim);
+    output->set_npol(4 / ndim);
+    output->set_ndim(ndim);
+  } else {
+    throw Error(InvalidState, "dsp::Detection::operate", "unsupported output state");
+  }
+  output->resize(input->get_ndat());
+  if (state == Signal::Intensity || state == Signal::PPQQ)
+    engine->square_law(input, output);
+  else
+    engine->polarimetry(ndim, input, output);
+}
+
+// ------------------------------------------------------------------- Fold
+
+double Fold::get_phi(double seconds) const
+{
+  return std::fmod(seconds, folding_period) / folding_period - reference_phase;
+}
+
+void Fold::operate()
+{
+  if (input->get_ndat() == 0) return;
+  if (folding_period <= 0) throw Error(InvalidState, "dsp::Fold::transformation", "no folding period");
+  if (!folding_nbin) throw Error(InvalidState, "dsp::Fold::fold", "nbin not set");
+  if (!engine) throw Error(InvalidState, "dsp::Fold::fold", "no engine");
+  if (!limits_set) {  // Fold::set_limits (Fold.C:993-997)
+    idat_start = 0;
+    ndat_fold = input->get_ndat();
+  }
+  uint64_t idat_end = idat_start + ndat_fold;
+  if (idat_end > input->get_ndat())
+    throw Error(InvalidParam, "dsp::Fold:fold", "idat_start=%llu + ndat_fold=%llu > ndat=%llu",
+                (unsigned long long) idat_start, (unsigned long long) ndat_fold,
+                (unsigned long long) input->get_ndat());
+  if (output->get_nbin() != folding_nbin || output->get_nchan() != input->get_nchan() ||
+      output->get_npol() != input->get_npol() || output->get_ndim() != input->get_ndim()) {
+    output->copy_configuration(input);
+    output->resize(folding_nbin);
+    output->zero();
+    output->set_hits_nchan(1);
+    output->resize_hits();
+    engine->zero();
+  }
+  // midpoint of the first sample (Fold.C:686-693)
+  double start = input->get_start_time() + (double(idat_start) + 0.5) / input->get_rate();
+  double phi = get_phi(start);
+  double pps = (1.0 / input->get_rate()) / folding_period;
+  engine->set_input(input);
+  engine->set_profiles(output);
+  engine->set_nbin(folding_nbin);
+  engine->set_ndat(ndat_fold, idat_start);
+  uint64_t ndat_folded = engine->set_bins(phi, pps, ndat_fold, idat_start);
+  unsigned* hits = output->get_hits();
+  for (unsigned ibin = 0; ibin < folding_nbin; ibin++) hits[ibin] += unsigned(engine->get_bin_hits(int(ibin)));
+  output->integration_length += double(ndat_folded) / input->get_rate();
+  output->ndat_total += ndat_fold;
+  output->folding_period = folding_period;
+  engine->fold();
+}
+
+PhaseSeries* Fold::get_result()
+{
+  if (engine) engine->synch(output);
+  return output;
+}
+
+}  // namespace dsp

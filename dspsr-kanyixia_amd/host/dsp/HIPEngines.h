@@ -1,0 +1,141 @@
+// HIPEngines.h -- MI355X engines behind the dsp Transformation interfaces.
+//
+// Each engine is a thin C++ shell over the C ABI of libdspx (include/dspx.h):
+// it holds a dspx handle, converts dsp:: objects into pointers and strides,
+// and turns a non-zero dspx status into
+//   Error(FailedCall, "dsp::HIP::<Engine>::<method>", dspx_last_error(ctx))
+// like the reference's CUDA engines throw on cuFFT / CUDA failures
+// (ConvolutionCUDA.cu:105-108).  Attachment mirrors the reference builders:
+//   convolution->set_engine (new dsp::HIP::ConvolutionEngine (ctx));
+//     (DedispersionPipe.C:413-427)
+//   detect->set_engine (new dsp::HIP::DetectionEngine (ctx));
+//     (LoadToFold1.C:843-852)
+//   fold->set_engine (new dsp::HIP::FoldEngine (ctx));
+//     (LoadToFold1.C:969-978)
+#pragma once
+#include "dsp/Operations.h"
+#include "dspx.h"
+
+namespace dsp {
+namespace HIP {
+
+//! One per pipeline thread: device, stream and shared tables
+class Context {
+ public:
+  explicit Context(int device = 0, void* stream = nullptr);
+  ~Context();
+  dspx_ctx* get() const { return ctx; }
+  void synchronize();
+  void check(int status, const char* where) const;
+
+ private:
+  dspx_ctx* ctx = nullptr;
+};
+
+//! GenericEightBitUnpacker::unpack_on_gpu / SixteenBitUnpacker (device)
+class Unpacker {
+ public:
+  explicit Unpacker(Context* c) : ctx(c) {}
+  void set_scale(float s) { scale = s; }
+  void unpack(const BitSeries* in, TimeSeries* out);
+
+ private:
+  Context* ctx;
+  float scale = 1.0f;
+};
+
+class ConvolutionEngine : public Convolution::Engine {
+ public:
+  explicit ConvolutionEngine(Context* c, unsigned form = DSPX_FORM_FUSED) : ctx(c), form(form) {}
+  ~ConvolutionEngine() override;
+  void prepare(Convolution* convolution) override;
+  void copy_response(const Response* response) override;
+  void perform(const TimeSeries* in, TimeSeries* out, unsigned npart) override;
+  void perform(const TimeSeries* in, TimeSeries* out, TimeSeries* zero_DM_out, unsigned npart) override;
+
+ private:
+  void create(const Response* response);
+  Context* ctx;
+  unsigned form;
+  dspx_conv* conv = nullptr;
+  Convolution* parent = nullptr;
+};
+
+class DetectionEngine : public Detection::Engine {
+ public:
+  explicit DetectionEngine(Context* c) : ctx(c) {}
+  void square_law(const TimeSeries* in, TimeSeries* out) override;
+  void polarimetry(unsigned ndim, const TimeSeries* in, TimeSeries* out) override;
+  //! the output state square_law forms (Intensity or PPQQ)
+  void set_state(Signal::State s) { state = s; }
+
+ private:
+  Context* ctx;
+  Signal::State state = Signal::Intensity;
+  friend class dsp::Detection;
+};
+
+//! Fold::Engine with the bin plan evaluated on the device (use_set_bins)
+class FoldEngine : public Fold::Engine {
+ public:
+  explicit FoldEngine(Context* c) : ctx(c) { use_set_bins = true; }
+  ~FoldEngine() override;
+  void set_nbin(unsigned nbin) override;
+  void set_bin(uint64_t idat, double ibin, double bins_per_samp) override;
+  uint64_t set_bins(double phi, double phase_per_sample, uint64_t ndat, uint64_t idat_start) override;
+  uint64_t get_bin_hits(int ibin) override;
+  uint64_t get_ndat_folded() const override { return ndat_folded; }
+  PhaseSeries* get_profiles() override { return out; }
+  void fold() override;
+  void synch(PhaseSeries* output) override;
+  void zero() override;
+
+ private:
+  Context* ctx;
+  unsigned nbin = 0;
+  dspx_fold* plan = nullptr;
+  uint64_t ndat_folded = 0, plan_ndat = 0;
+  std::vector<uint32_t> hits_host;
+  float* d_prof = nullptr;  // [nchan][npol][nbin*ndim] accumulated on the device
+  size_t prof_floats = 0;
+  uint32_t* d_hits = nullptr;
+};
+
+//! The LoadToFold chain Unpack -> Convolution -> Detection -> Fold fused
+//! into three kernels (or the plain form), for one block of raw samples.
+class LoadToFoldEngine {
+ public:
+  struct Config {
+    double folding_period = 0, reference_phase = 0;
+    unsigned nbin = 1024, nsub = 1;
+    Signal::State state = Signal::Intensity;
+    unsigned ndim = 1;
+    unsigned form = DSPX_FORM_FUSED;
+    unsigned chirp_mode = DSPX_CHIRP_PARITY;
+    float scale = 1.0f;
+    unsigned nfft = 0;  // 0: reference rule
+  };
+  LoadToFoldEngine(Context* c, const Config& cfg) : ctx(c), cfg(cfg) {}
+  ~LoadToFoldEngine();
+  //! build for the shape of this raw block (dedispersion of its band at its DM)
+  void prepare(const BitSeries* raw);
+  //! fold one block; profiles accumulate in the engine
+  void operate(const BitSeries* raw);
+  //! copy the profiles [nsub][nchan][npol_out][nbin*ndim] and hits [nsub][nbin] to the host
+  void get_result(std::vector<float>& prof, std::vector<uint32_t>& hits);
+  uint64_t get_ndat_out() const { return ndat_out; }
+  const dspx_geometry& get_geometry() const { return geom; }
+
+ private:
+  Context* ctx;
+  Config cfg;
+  dspx_ltf* ltf = nullptr;
+  dspx_geometry geom{};
+  uint64_t npart = 0, ndat_out = 0;
+  float* d_prof = nullptr;
+  uint32_t* d_hits = nullptr;
+  size_t prof_floats = 0, hits_count = 0;
+};
+
+}  // namespace HIP
+}  // namespace dsp

@@ -1,0 +1,193 @@
+// Operations.h -- the Transformations of the hot path and their Engine
+// interfaces, restated from the reference's call sites:
+//   Convolution + Convolution::Engine  Signal/General/Convolution.C:316-813
+//     (engine calls :446-451, :656-667, :690-697; Engine methods as used by
+//      ConvolutionCUDA.cu:174-216, 219, 567-595)
+//   Detection + Detection::Engine      Signal/General/Detection.C:225-573
+//     (engine calls :237-243, :483-490)
+//   Fold + Fold::Engine                Signal/Pulsar/Fold.C:546-990,
+//                                      Signal/Pulsar/dsp/Fold.h:262-345
+// The CPU branches of these Transformations are not restated here: on this
+// tree they are the oracle (oracle/), and the product requires an engine.
+#pragma once
+#include "dsp/Types.h"
+
+namespace dsp {
+
+//! Frequency response applied by the Convolution, [nchan][ndat] complex
+//! float in FFT-bin order (Response.C, after swap_as_needed)
+class Response {
+ public:
+  virtual ~Response() = default;
+  unsigned get_nchan() const { return nchan; }
+  unsigned get_ndat() const { return ndat; }
+  unsigned get_impulse_pos() const { return impulse_pos; }
+  unsigned get_impulse_neg() const { return impulse_neg; }
+  //! Response::prepare: fix nchan/ndat/impulse for this input
+  virtual void prepare(const Observation* input, unsigned channels = 0) = 0;
+  //! host copy of the response (empty when the engine builds it on chip)
+  const std::vector<float>& get_data() const { return data; }
+  bool has_changed() const { return changed; }
+  void set_changed(bool c) { changed = c; }
+
+ protected:
+  unsigned nchan = 0, ndat = 0, impulse_pos = 0, impulse_neg = 0;
+  std::vector<float> data;
+  bool changed = true;
+};
+
+//! Coherent dedispersion kernel (Dedispersion.C / PlasmaResponse.C).  The
+//! geometry follows the reference rule; the chirp itself is evaluated on the
+//! device by the HIP engine (FP64 phase rounded to float, as the reference
+//! stores it, Dedispersion.C:104-111).
+class Dedispersion : public Response {
+ public:
+  void set_dispersion_measure(double dm)
+  {
+    if (dm != dispersion_measure) changed = true;
+    dispersion_measure = dm;
+  }
+  double get_dispersion_measure() const { return dispersion_measure; }
+  //! user-set frequency resolution (DedispersionPipeConfig -x)
+  void set_frequency_resolution(unsigned n) { user_ndat = n; }
+  void prepare(const Observation* input, unsigned channels = 0) override;
+  double get_centre_frequency() const { return centre_frequency; }
+  double get_bandwidth() const { return bandwidth; }
+  bool get_dc_centred() const { return dc_centred; }
+
+ protected:
+  double dispersion_measure = 0, centre_frequency = 0, bandwidth = 0;
+  bool dc_centred = false;
+  unsigned user_ndat = 0;
+};
+
+class Convolution {
+ public:
+  class Engine {
+   public:
+    virtual ~Engine() = default;
+    virtual void set_scratch(void*) {}
+    virtual void prepare(Convolution* convolution) = 0;
+    virtual void copy_response(const Response* response) = 0;
+    virtual void perform(const TimeSeries* in, TimeSeries* out, unsigned npart) = 0;
+    virtual void perform(const TimeSeries* in, TimeSeries* out, TimeSeries* zero_DM_out, unsigned npart) = 0;
+  };
+
+  void set_input(const TimeSeries* in) { input = in; }
+  void set_output(TimeSeries* out) { output = out; }
+  void set_response(Response* r) { response = r; }
+  Response* get_response() const { return response; }
+  void set_engine(Engine* e) { engine.reset(e); }
+  void set_zero_DM(bool z) { zero_DM = z; }
+  void set_zero_DM_output(TimeSeries* z) { zero_DM_output = z; }
+  const TimeSeries* get_input() const { return input; }
+  TimeSeries* get_output() const { return output; }
+
+  //! Convolution.C:316-455 (Analytic input)
+  void prepare();
+  //! Transformation::operate -> Convolution::transformation (Convolution.C:646-698)
+  void operate();
+
+  unsigned get_minimum_samples() const { return nsamp_fft; }
+  unsigned get_minimum_samples_lost() const { return nsamp_overlap; }
+  unsigned get_nfilt_pos() const { return nfilt_pos; }
+  unsigned get_nfilt_neg() const { return nfilt_neg; }
+  uint64_t get_npart() const { return npart; }
+
+ protected:
+  void prepare_output();
+  const TimeSeries* input = nullptr;
+  TimeSeries* output = nullptr;
+  TimeSeries* zero_DM_output = nullptr;
+  Response* response = nullptr;
+  std::unique_ptr<Engine> engine;
+  bool prepared = false, zero_DM = false;
+  unsigned nsamp_fft = 0, nsamp_overlap = 0, nsamp_step = 0, nfilt_pos = 0, nfilt_neg = 0;
+  uint64_t npart = 0;
+};
+
+class Detection {
+ public:
+  class Engine {
+   public:
+    virtual ~Engine() = default;
+    virtual void square_law(const TimeSeries* in, TimeSeries* out) = 0;
+    virtual void polarimetry(unsigned ndim, const TimeSeries* in, TimeSeries* out) = 0;
+  };
+  void set_input(const TimeSeries* in) { input = in; }
+  void set_output(TimeSeries* out) { output = out; }
+  void set_output_state(Signal::State s) { state = s; }
+  Signal::State get_output_state() const { return state; }
+  void set_output_ndim(unsigned n) { ndim = n; }
+  unsigned get_output_ndim() const { return ndim; }
+  void set_engine(Engine* e) { engine.reset(e); }
+  //! Detection::transformation: output shape, then square_law / polarimetry
+  void operate();
+
+ protected:
+  const TimeSeries* input = nullptr;
+  TimeSeries* output = nullptr;
+  Signal::State state = Signal::Intensity;
+  unsigned ndim = 4;
+  std::unique_ptr<Engine> engine;
+};
+
+class Fold {
+ public:
+  //! Signal/Pulsar/dsp/Fold.h:262-345
+  class Engine {
+   public:
+    virtual ~Engine() = default;
+    virtual void set_input(const TimeSeries* input) { in = input; }
+    virtual void set_profiles(PhaseSeries* o) { out = o; }
+    virtual void set_nbin(unsigned nbin) = 0;
+    virtual void set_bin(uint64_t idat, double ibin, double bins_per_samp) = 0;
+    virtual uint64_t set_bins(double phi, double phase_per_sample, uint64_t ndat, uint64_t idat_start) = 0;
+    bool use_set_bins = false;
+    virtual uint64_t get_bin_hits(int ibin) = 0;
+    virtual uint64_t get_ndat_folded() const = 0;
+    virtual PhaseSeries* get_profiles() = 0;
+    virtual void fold() = 0;
+    virtual void synch(PhaseSeries*) = 0;
+    virtual void zero() = 0;
+    virtual void set_ndat(uint64_t, uint64_t) {}
+
+   protected:
+    const TimeSeries* in = nullptr;
+    PhaseSeries* out = nullptr;
+  };
+
+  void set_input(const TimeSeries* in) { input = in; }
+  void set_output(PhaseSeries* out) { output = out; }
+  void set_engine(Engine* e) { engine.reset(e); }
+  Engine* get_engine() const { return engine.get(); }
+  void set_folding_period(double p) { folding_period = p; }
+  double get_folding_period() const { return folding_period; }
+  void set_nbin(unsigned n) { folding_nbin = n; }
+  unsigned get_nbin() const { return folding_nbin; }
+  void set_reference_phase(double r) { reference_phase = r; }
+  //! fold samples [idat_start, idat_start + ndat_fold) of the input
+  void set_limits(uint64_t idat_start_, uint64_t ndat_fold_)
+  {
+    idat_start = idat_start_;
+    ndat_fold = ndat_fold_;
+    limits_set = true;
+  }
+  //! Fold::transformation + Fold::fold (Fold.C:546-779, 862-887)
+  void operate();
+  //! Fold::get_result: engine->synch into the output (Fold.C:138-150)
+  PhaseSeries* get_result();
+  //! Fold::get_phi for a constant period (Fold.C:975-982)
+  double get_phi(double seconds) const;
+
+ protected:
+  const TimeSeries* input = nullptr;
+  PhaseSeries* output = nullptr;
+  std::unique_ptr<Engine> engine;
+  double folding_period = 0, reference_phase = 0;
+  unsigned folding_nbin = 0;
+  uint64_t idat_start = 0, ndat_fold = 0;
+  bool limits_set = false;
+};
+
+}  // namespace dsp

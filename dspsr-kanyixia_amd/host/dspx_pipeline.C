@@ -1,0 +1,151 @@
+// dspx_pipeline -- a LoadToFold-style driver over the dsp::HIP engines.
+//
+// Reads one block of raw TFP samples from a file, runs
+//   chain : Unpacker -> Convolution[HIP::ConvolutionEngine] ->
+//           Detection[HIP::DetectionEngine] -> Fold[HIP::FoldEngine] per sub-integration
+//   fused : HIP::LoadToFoldEngine (three fused kernels)
+//   plain : HIP::LoadToFoldEngine, plain FP32 form
+// and writes the folded profiles [nsub][nchan][npol][nbin*ndim] (float) and
+// hits [nsub][nbin] (uint32) to the output file.  The construction mirrors
+// LoadToFold::construct (LoadToFold1.C:109-258) with engines attached as
+// DedispersionPipe.C:413-427 and LoadToFold1.C:843-852, 969-978 attach them.
+#include <hip/hip_runtime_api.h>
+
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <iostream>
+#include <map>
+
+#include "dsp/HIPEngines.h"
+
+using namespace dsp;
+
+static std::map<std::string, std::string> parse(int argc, char** argv)
+{
+  std::map<std::string, std::string> a;
+  for (int i = 1; i + 1 < argc; i += 2) {
+    if (std::strncmp(argv[i], "--", 2)) throw Error(InvalidParam, "dspx_pipeline", "bad option %s", argv[i]);
+    a[argv[i] + 2] = argv[i + 1];
+  }
+  return a;
+}
+
+static Signal::State parse_state(const std::string& s)
+{
+  if (s == "Intensity") return Signal::Intensity;
+  if (s == "PPQQ") return Signal::PPQQ;
+  if (s == "Coherence") return Signal::Coherence;
+  if (s == "Stokes") return Signal::Stokes;
+  throw Error(InvalidParam, "dspx_pipeline", "unknown state %s", s.c_str());
+}
+
+int main(int argc, char** argv)
+try {
+  auto a = parse(argc, argv);
+  auto need = [&](const char* k) -> std::string {
+    if (!a.count(k)) throw Error(InvalidParam, "dspx_pipeline", "missing --%s", k);
+    return a[k];
+  };
+  unsigned nbit = std::stoul(need("nbit")), nchan = std::stoul(need("nchan")), npol = std::stoul(need("npol"));
+  uint64_t ndat = std::stoull(need("ndat"));
+  double cfreq = std::stod(need("cfreq")), bw = std::stod(need("bw")), dm = std::stod(need("dm"));
+  double period = std::stod(need("period"));
+  unsigned nbin = std::stoul(need("nbin")), nsub = std::stoul(need("nsub"));
+  Signal::State state = parse_state(need("state"));
+  unsigned ndim = std::stoul(a.count("ndim") ? a["ndim"] : "1");
+  std::string mode = need("mode");
+
+  HIP::Context ctx(0);
+  BitSeries raw(Memory::device());
+  raw.set_nchan(nchan);
+  raw.set_npol(npol);
+  raw.set_centre_frequency(cfreq);
+  raw.set_bandwidth(bw);
+  raw.set_dispersion_measure(dm);
+  raw.set_rate(std::fabs(bw) / nchan * 1e6);
+  raw.set_start_time(0.0);
+  raw.resize(ndat, nbit);
+  {
+    size_t bytes = size_t(ndat) * nchan * npol * 2 * nbit / 8;
+    std::vector<char> h(bytes);
+    std::ifstream f(need("raw"), std::ios::binary);
+    if (!f.read(h.data(), std::streamsize(bytes))) throw Error(FailedCall, "dspx_pipeline", "short read");
+    if (hipMemcpy(raw.get_rawptr(), h.data(), bytes, hipMemcpyHostToDevice) != hipSuccess)
+      throw Error(FailedCall, "dspx_pipeline", "hipMemcpy");
+  }
+
+  std::vector<float> prof;
+  std::vector<uint32_t> hits;
+  if (mode == "chain") {
+    TimeSeries unpacked(Memory::device()), filtered(Memory::device()), detected(Memory::device());
+    HIP::Unpacker unpacker(&ctx);
+    unpacker.unpack(&raw, &unpacked);
+
+    Dedispersion kernel;
+    kernel.set_dispersion_measure(dm);
+    Convolution convolution;
+    convolution.set_input(&unpacked);
+    convolution.set_output(&filtered);
+    convolution.set_response(&kernel);
+    convolution.set_engine(new HIP::ConvolutionEngine(&ctx));
+    convolution.operate();
+
+    Detection detect;
+    detect.set_input(&filtered);
+    detect.set_output(&detected);
+    detect.set_output_state(state);
+    detect.set_output_ndim(ndim);
+    detect.set_engine(new HIP::DetectionEngine(&ctx));
+    detect.operate();
+
+    unsigned npol_out = detected.get_npol(), ndim_out = detected.get_ndim();
+    uint64_t nout = detected.get_ndat();
+    size_t per_sub = size_t(nchan) * npol_out * nbin * ndim_out;
+    prof.assign(per_sub * nsub, 0.f);
+    hits.assign(size_t(nsub) * nbin, 0);
+    for (unsigned k = 0; k < nsub; k++) {
+      uint64_t i0 = uint64_t(std::llround(double(nout) * k / nsub));
+      uint64_t i1 = uint64_t(std::llround(double(nout) * (k + 1) / nsub));
+      PhaseSeries sub(Memory::host());
+      Fold fold;
+      fold.set_input(&detected);
+      fold.set_output(&sub);
+      fold.set_folding_period(period);
+      fold.set_nbin(nbin);
+      fold.set_limits(i0, i1 - i0);
+      fold.set_engine(new HIP::FoldEngine(&ctx));
+      fold.operate();
+      fold.get_result();
+      for (unsigned c = 0; c < nchan; c++)
+        for (unsigned p = 0; p < npol_out; p++)
+          std::memcpy(&prof[k * per_sub + (size_t(c) * npol_out + p) * nbin * ndim_out], sub.get_datptr(c, p),
+                      size_t(nbin) * ndim_out * sizeof(float));
+      std::memcpy(&hits[size_t(k) * nbin], sub.get_hits(), nbin * sizeof(uint32_t));
+    }
+  } else if (mode == "fused" || mode == "plain") {
+    HIP::LoadToFoldEngine::Config cfg;
+    cfg.folding_period = period;
+    cfg.nbin = nbin;
+    cfg.nsub = nsub;
+    cfg.state = state;
+    cfg.ndim = ndim;
+    cfg.form = mode == "fused" ? DSPX_FORM_FUSED : DSPX_FORM_PLAIN;
+    HIP::LoadToFoldEngine ltf(&ctx, cfg);
+    ltf.prepare(&raw);
+    ltf.operate(&raw);
+    ltf.get_result(prof, hits);
+  } else {
+    throw Error(InvalidParam, "dspx_pipeline", "unknown mode %s", mode.c_str());
+  }
+  std::ofstream o(need("out"), std::ios::binary);
+  o.write(reinterpret_cast<const char*>(prof.data()), std::streamsize(prof.size() * sizeof(float)));
+  o.write(reinterpret_cast<const char*>(hits.data()), std::streamsize(hits.size() * sizeof(uint32_t)));
+  std::cout << "dspx_pipeline " << mode << ": " << prof.size() << " profile values, " << hits.size() << " hits"
+            << std::endl;
+  return 0;
+} catch (const Error& e) {
+  std::cerr << "Error: " << e.what() << std::endl;
+  return 1;
+}
