@@ -148,6 +148,7 @@ def binplan_host(phi, pps, nbin, ndat):
 
 class Context:
     def __init__(self, device=0, stream=None):
+        self.h = None
         h = P()
         _check(lib().dspx_ctx_create(device, stream, C.byref(h)))
         self.h = h
@@ -191,6 +192,7 @@ class Convolution:
     def __init__(self, ctx, nchan, npol, nfft, nfilt_pos, nfilt_neg, chirp_mode=CHIRP_PARITY,
                  form=FORM_FUSED, centre_frequency=0.0, bandwidth=0.0, dm=0.0, dc_centred=False):
         self.ctx = ctx
+        self.h = None
         d = ConvDesc(_dedisp(centre_frequency, bandwidth, dm, nchan, dc_centred), nchan, npol, nfft,
                      nfilt_pos, nfilt_neg, chirp_mode, form, 0)
         h = P()
@@ -222,6 +224,7 @@ class Fold:
 
     def __init__(self, ctx, ndat, pps, nbin, chunks):
         self.ctx = ctx
+        self.h = None
         arr = (FoldChunk * len(chunks))(*[FoldChunk(c[0], c[1], c[2], c[3], 0) for c in chunks])
         h = P()
         ctx.check(lib().dspx_fold_create(ctx.h, ndat, pps, nbin, arr, len(chunks), C.byref(h)))
@@ -252,6 +255,7 @@ class LoadToFold:
                  state=INTENSITY, ndim=1, chirp_mode=CHIRP_PARITY, form=FORM_FUSED, nfft=0,
                  scale=1.0, offset_binary=False, geom=None):
         self.ctx = ctx
+        self.h = None
         g = geom or geometry(centre_frequency, bandwidth, nchan, dm, nfft)
         self.geom = g
         d = LtfDesc(nbit, int(offset_binary), scale, nchan, npol, 0, ndat,
@@ -313,6 +317,7 @@ def fold_chunks(ndat_out, rate, period, nfilt_pos, nsub=1, t0=0.0, reference_pha
     TimeDivide does, TimeDivide.C:193-304).  Returns [(idat_start, ndat, phi,
     isub)] with phi of the midpoint of each chunk's first sample."""
     chunks = []
+    # rint: round half to even, as Python's round does
     bounds = [int(round(ndat_out * k / nsub)) for k in range(nsub + 1)]
     for k in range(nsub):
         i0, i1 = bounds[k], bounds[k + 1]

@@ -267,8 +267,8 @@ void LoadToFoldEngine::prepare(const BitSeries* raw)
   double rate = raw->get_rate();
   std::vector<dspx_fold_chunk> chunks(cfg.nsub);
   for (unsigned k = 0; k < cfg.nsub; k++) {
-    uint64_t i0 = uint64_t(std::llround(double(ndat_out) * k / cfg.nsub));
-    uint64_t i1 = uint64_t(std::llround(double(ndat_out) * (k + 1) / cfg.nsub));
+    uint64_t i0 = uint64_t(std::rint(double(ndat_out) * k / cfg.nsub));
+    uint64_t i1 = uint64_t(std::rint(double(ndat_out) * (k + 1) / cfg.nsub));
     double t = raw->get_start_time() + (double(geom.impulse_pos) + double(i0) + 0.5) / rate;
     double phi = std::fmod(t, cfg.folding_period) / cfg.folding_period - cfg.reference_phase;
     chunks[k] = dspx_fold_chunk{i0, i1 - i0, phi, k, 0};

@@ -106,8 +106,8 @@ try {
     prof.assign(per_sub * nsub, 0.f);
     hits.assign(size_t(nsub) * nbin, 0);
     for (unsigned k = 0; k < nsub; k++) {
-      uint64_t i0 = uint64_t(std::llround(double(nout) * k / nsub));
-      uint64_t i1 = uint64_t(std::llround(double(nout) * (k + 1) / nsub));
+      uint64_t i0 = uint64_t(std::rint(double(nout) * k / nsub));
+      uint64_t i1 = uint64_t(std::rint(double(nout) * (k + 1) / nsub));
       PhaseSeries sub(Memory::host());
       Fold fold;
       fold.set_input(&detected);
