@@ -213,6 +213,47 @@ struct ConvCore {
   }
 };
 
+// Raw TFP [ndat][nchan] words (one word = both pols of one sample) to
+// channel-major [nchan][ndat] words, so that stage A reads each channel's
+// time samples contiguously.  64 x 64 word tiles through LDS; rows of the
+// input and of the output are both read / written as contiguous runs.
+template <class W>
+__global__ __launch_bounds__(256) void k_tfp_to_ftp(const W* __restrict__ in, uint64_t ndat, uint32_t nchan,
+                                                    W* __restrict__ out)
+{
+  __shared__ W tile[64][65];
+  const uint64_t t0 = uint64_t(blockIdx.x) * 64;
+  const uint32_t c0 = blockIdx.y * 64;
+  const uint32_t tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+#pragma unroll
+  for (uint32_t k = 0; k < 16; k++) {
+    const uint32_t i = ty + 4 * k;
+    uint64_t t = t0 + i;
+    uint32_t c = c0 + tx;
+    if (t < ndat && c < nchan) tile[i][tx] = in[t * nchan + c];
+  }
+  __syncthreads();
+#pragma unroll
+  for (uint32_t k = 0; k < 16; k++) {
+    const uint32_t i = ty + 4 * k;
+    uint32_t c = c0 + i;
+    uint64_t t = t0 + tx;
+    if (t < ndat && c < nchan) out[uint64_t(c) * ndat + t] = tile[tx][i];
+  }
+}
+
+void tfp_to_ftp(dspx_ctx* ctx, const void* in, uint64_t ndat, uint32_t nchan, uint32_t wbytes, void* out)
+{
+  dim3 grid(unsigned((ndat + 63) / 64), (nchan + 63) / 64);
+  switch (wbytes) {
+    case 2: k_tfp_to_ftp<uint16_t><<<grid, 256, 0, ctx->stream>>>((const uint16_t*) in, ndat, nchan, (uint16_t*) out); break;
+    case 4: k_tfp_to_ftp<uint32_t><<<grid, 256, 0, ctx->stream>>>((const uint32_t*) in, ndat, nchan, (uint32_t*) out); break;
+    case 8: k_tfp_to_ftp<uint64_t><<<grid, 256, 0, ctx->stream>>>((const uint64_t*) in, ndat, nchan, (uint64_t*) out); break;
+    default: fail(DSPX_E_INTERNAL, "word size %u", wbytes);
+  }
+  DSPX_HIP(hipGetLastError());
+}
+
 // zero-DM copy (Convolution.C:792-799)
 __global__ void k_zero_dm(const float* __restrict__ in, uint64_t ics, uint64_t ips, float* __restrict__ out,
                           uint64_t ocs, uint64_t ops, uint32_t npol, uint32_t step, uint32_t nfilt_pos,
@@ -358,29 +399,49 @@ struct dspx_ltf {
   uint32_t G = 1, nslot = 1;
   DevBuf sub_first;
   DevBuf partial;
+  // channel-major copy of the raw block (nchan > 1)
+  DevBuf ftp;
   // plain form intermediates
   DevBuf unpacked, filtered, detected;
   // timing
   bool timing = false;
+  bool nstage_open = false;
   int nstage = 0;
   const char* stage_name[8] = {};
 };
 
 namespace {
 
-void ltf_record(dspx_ltf* L, int i)
+void ltf_record(dspx_ltf* L, int /*unused*/)
 {
-  if (L->timing) DSPX_HIP(hipEventRecord(L->ctx->ev[i], L->ctx->stream));
+  // event k closes stage k-1; event 0 opens the first stage
+  if (L->timing) DSPX_HIP(hipEventRecord(L->ctx->ev[L->nstage + (L->nstage_open ? 1 : 0)], L->ctx->stream));
+  L->nstage_open = true;
 }
 
+// Calls f(loader) with a loader over the raw block.  Multichannel TFP input
+// is first transposed to channel-major order (one extra 2*B_in bytes per
+// sample) so that stage A reads each channel contiguously.
 template <class F> void with_loader(dspx_ltf* L, const void* raw, F&& f)
 {
   const auto& d = L->d;
+  uint64_t wbytes = uint64_t(d.npol) * 2 * d.nbit / 8;  // one sample of one channel, all pols
+  uint64_t tstride = uint64_t(d.nchan) * d.npol * 2, cstride = uint64_t(d.npol) * 2;  // in elements
+  const void* src = raw;
+  if (d.nchan > 1) {
+    L->ftp.reserve(d.ndat * d.nchan * wbytes);
+    tfp_to_ftp(L->ctx, raw, d.ndat, d.nchan, uint32_t(wbytes), L->ftp.p);
+    src = L->ftp.p;
+    tstride = uint64_t(d.npol) * 2;
+    cstride = d.ndat * d.npol * 2;
+    ltf_record(L, 1);
+    if (L->nstage < 7) L->stage_name[L->nstage++] = "transpose_tfp_to_ftp";
+  }
   if (d.nbit == 8) {
-    LoadI8 ld{static_cast<const uint8_t*>(raw), uint64_t(d.nchan) * d.npol * 2, d.npol, d.offset_binary, d.scale};
+    LoadI8 ld{static_cast<const uint8_t*>(src), tstride, cstride, d.npol, d.offset_binary, d.scale};
     f(ld);
   } else {
-    LoadI16 ld{static_cast<const int16_t*>(raw), uint64_t(d.nchan) * d.npol * 2, d.npol, d.offset_binary};
+    LoadI16 ld{static_cast<const int16_t*>(src), tstride, cstride, d.npol, d.offset_binary};
     f(ld);
   }
 }
@@ -496,6 +557,7 @@ int dspx_ltf_execute(dspx_ltf* L, const void* d_raw, float* d_prof, uint32_t* d_
     if (!L->npart) return;
     int ev = 0;
     L->nstage = 0;
+    L->nstage_open = false;
     ltf_record(L, ev++);
     if (d.form == DSPX_FORM_FUSED) {
       ConvCore& k = L->core;
@@ -564,6 +626,8 @@ int dspx_ltf_filterbank(dspx_ltf* L, const void* d_raw, float* d_out)
     const auto& d = L->d;
     if (!L->npart) return;
     uint64_t nout = L->ndat_out;
+    L->nstage = 0;
+    L->nstage_open = false;
     if (d.form == DSPX_FORM_FUSED) {
       ConvCore& k = L->core;
       uint32_t np = uint32_t(L->npart);

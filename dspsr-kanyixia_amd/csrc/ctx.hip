@@ -43,10 +43,11 @@ void unpack(dspx_ctx* ctx, const void* raw, uint32_t nbit, uint32_t offset_binar
   if (!ndat) return;
   dim3 grid(unsigned((ndat + 255) / 256), nchan);
   if (nbit == 8) {
-    LoadI8 ld{static_cast<const uint8_t*>(raw), uint64_t(nchan) * npol * 2, npol, offset_binary, scale};
+    LoadI8 ld{static_cast<const uint8_t*>(raw), uint64_t(nchan) * npol * 2, uint64_t(npol) * 2, npol, offset_binary,
+              scale};
     k_unpack<<<grid, 256, 0, ctx->stream>>>(ld, ndat, out, cs, ps, npol);
   } else if (nbit == 16) {
-    LoadI16 ld{static_cast<const int16_t*>(raw), uint64_t(nchan) * npol * 2, npol, offset_binary};
+    LoadI16 ld{static_cast<const int16_t*>(raw), uint64_t(nchan) * npol * 2, uint64_t(npol) * 2, npol, offset_binary};
     k_unpack<<<grid, 256, 0, ctx->stream>>>(ld, ndat, out, cs, ps, npol);
   } else {
     fail(DSPX_E_PARAM, "nbit=%u (8 or 16 supported)", nbit);

@@ -22,10 +22,13 @@ constexpr int kColThreads = 256;
 
 // ------------------------------------------------------------ input loaders
 
-// 8-bit complex TFP.  Both pols of (t, chan) in one 4-byte word.
+// 8-bit complex samples; both pols of (t, chan) in one word at
+// raw + chan*cstride + t*tstride (TFP: cstride = npol*2, tstride = nchan*npol*2;
+// channel-major after the transpose pass: cstride = ndat*npol*2, tstride = npol*2).
 struct LoadI8 {
   const uint8_t* raw;
-  uint64_t tstride;  // bytes per time sample (nchan*npol*2)
+  uint64_t tstride;  // bytes between time samples
+  uint64_t cstride;  // bytes between channels
   uint32_t npol;
   uint32_t offset_binary;
   float scale;
@@ -36,7 +39,7 @@ struct LoadI8 {
   }
   __device__ __forceinline__ void operator()(uint32_t chan, uint64_t t, float2& a, float2& b) const
   {
-    const uint8_t* q = raw + t * tstride + uint64_t(chan) * npol * 2;
+    const uint8_t* q = raw + t * tstride + uint64_t(chan) * cstride;
     if (npol == 2) {
       uint32_t w = *reinterpret_cast<const uint32_t*>(q);
       a = make_float2(cvt(w & 0xff), cvt((w >> 8) & 0xff));
@@ -49,10 +52,12 @@ struct LoadI8 {
   }
 };
 
-// 16-bit complex TFP: float(v) + 0.5 (offset binary: v ^ 0x8000).
+// 16-bit complex samples: float(v) + 0.5 (offset binary: v ^ 0x8000); strides
+// in int16 elements, laid out as for LoadI8.
 struct LoadI16 {
   const int16_t* raw;
-  uint64_t tstride;  // int16 per time sample
+  uint64_t tstride;
+  uint64_t cstride;
   uint32_t npol;
   uint32_t offset_binary;
   __device__ __forceinline__ float cvt(int16_t v) const
@@ -62,7 +67,7 @@ struct LoadI16 {
   }
   __device__ __forceinline__ void operator()(uint32_t chan, uint64_t t, float2& a, float2& b) const
   {
-    const int16_t* q = raw + t * tstride + uint64_t(chan) * npol * 2;
+    const int16_t* q = raw + t * tstride + uint64_t(chan) * cstride;
     if (npol == 2) {
       short4 w = *reinterpret_cast<const short4*>(q);
       a = make_float2(cvt(w.x), cvt(w.y));
@@ -304,6 +309,32 @@ __device__ __forceinline__ int detect_values(uint32_t state, uint32_t npol, floa
   return 4;
 }
 
+// Sums d[0..nv) over runs of consecutive lanes of the wavefront that carry the
+// same key (a segmented inclusive scan).  Returns true on the last lane of each
+// run, which then holds the run's totals.  Every lane must call it.
+__device__ __forceinline__ bool wave_run_sum(uint32_t key, float* d, int nv)
+{
+  const int lane = int(threadIdx.x & 63u);
+  uint32_t kprev = __shfl_up(key, 1);
+  int head = (lane == 0 || kprev != key) ? lane : 0;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    int h = __shfl_up(head, off);
+    if (lane >= off) head = max(head, h);
+  }
+  for (int i = 0; i < nv; i++) {
+    float v = d[i];
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      float u = __shfl_up(v, off);
+      if (lane - off >= head) v += u;
+    }
+    d[i] = v;
+  }
+  uint32_t knext = __shfl_down(key, 1);
+  return lane == 63 || knext != key;
+}
+
 // grid.x = G * npart * nchan (fold) or ncb * npart * nchan (filterbank).
 template <int N1, int MODE>
 __global__ __launch_bounds__(kColThreads) void k_stageC(ConvShape s, FilterbankArgs fb, FoldArgs fa,
@@ -375,23 +406,30 @@ __global__ __launch_bounds__(kColThreads) void k_stageC(ConvShape s, FilterbankA
       for (int r = 0; r < RL; r++) {
         uint32_t n1 = j + r * (N1 / RL);
         uint32_t t = s.N2 * n1 + c0 + b;
-        if (t < s.nfilt_pos || t >= s.nfilt_pos + s.step) continue;
+        const bool kept = t >= s.nfilt_pos && t < s.nfilt_pos + s.step;
         uint64_t idat = uint64_t(p) * s.step + (t - s.nfilt_pos);
         if (MODE == kStageCFilterbank) {
+          if (!kept) continue;
           float* o = fb.out + uint64_t(chan) * fb.chan_stride + 2 * idat;
           *reinterpret_cast<float2*>(o) = va[q * RL + r];
           if (s.npol == 2) *reinterpret_cast<float2*>(o + fb.pol_stride) = vb[q * RL + r];
         } else {
-          uint32_t plan = fa.binplan[idat];
-          if (plan == 0xffffffffu) continue;
-          uint32_t bin = plan & 0xffffu;
-          uint32_t slot = (plan >> 16) - sub0;
-          float d[4];
+          // consecutive lanes hold consecutive samples, which mostly share a
+          // phase bin: sum each run of equal (sub-integration, bin) across the
+          // wavefront, then one LDS atomic per run and value
+          uint32_t plan = kept ? fa.binplan[idat] : 0xffffffffu;
+          uint32_t key = plan == 0xffffffffu ? 0xffffffffu : (((plan >> 16) - sub0) << 16) | (plan & 0xffffu);
+          float d[4] = {0.f, 0.f, 0.f, 0.f};
           int nv = detect_values(fa.state, s.npol, va[q * RL + r], vb[q * RL + r], d);
-          float* hs = hist + slot * fa.nval * fa.nbin;
-          for (int i = 0; i < nv; i++) {
-            uint32_t off = (i / fa.ndim) * fa.nbin * fa.ndim + bin * fa.ndim + (i % fa.ndim);
-            atomicAdd(hs + off, d[i]);
+          if (key == 0xffffffffu) d[0] = d[1] = d[2] = d[3] = 0.f;
+          bool tail = wave_run_sum(key, d, int(fa.nval));
+          if (tail && key != 0xffffffffu) {
+            uint32_t bin = key & 0xffffu;
+            float* hs = hist + (key >> 16) * fa.nval * fa.nbin;
+            for (int i = 0; i < nv; i++) {
+              uint32_t off = (i / fa.ndim) * fa.nbin * fa.ndim + bin * fa.ndim + (i % fa.ndim);
+              atomicAdd(hs + off, d[i]);
+            }
           }
         }
       }
