@@ -174,12 +174,17 @@ struct ConvCore {
                  "no response: call dspx_conv_set_response first");
     ConvShape s = shape(npart);
     ChirpArgs c = chirp();
-    uint64_t nb = uint64_t(N1) * npart * nchan;
+    // parts per workgroup: as many as keep >= 2048 workgroups in flight
+    uint64_t rows = uint64_t(N1) * nchan;
+    uint32_t npg = rows >= 2048 ? 1u : uint32_t(std::min<uint64_t>(npart, (2048 + rows - 1) / rows));
+    uint32_t ppg = (npart + npg - 1) / npg;
+    npg = (npart + ppg - 1) / ppg;
+    uint64_t nb = rows * npg;
     DSPX_REQUIRE(nb < (1ull << 31), DSPX_E_UNSUPPORTED, "grid too large");
     const float2* twr = twiddles(ctx, N2);
     switch (N2) {
 #define CASE_B(n) \
-  case n: k_stageB<n><<<unsigned(nb), n / 16, 0, ctx->stream>>>(s, c, twr); break;
+  case n: k_stageB<n><<<unsigned(nb), n / 16, 0, ctx->stream>>>(s, c, twr, ppg); break;
       CASE_B(256) CASE_B(512) CASE_B(1024) CASE_B(2048) CASE_B(4096)
 #undef CASE_B
       default: fail(DSPX_E_UNSUPPORTED, "N2=%u", N2);
@@ -198,15 +203,25 @@ struct ConvCore {
     DSPX_REQUIRE(shm <= 160 * 1024, DSPX_E_UNSUPPORTED, "LDS request %zu too large", shm);
     const float2* twc = twiddles(ctx, N1);
     switch (N1) {
-#define CASE_C(n)                                                                  \
-  case n:                                                                          \
-    if (shm > 64 * 1024)                                                           \
-      DSPX_HIP(hipFuncSetAttribute((const void*) k_stageC<n, MODE>,                \
-                                   hipFuncAttributeMaxDynamicSharedMemorySize, int(shm))); \
-    k_stageC<n, MODE><<<unsigned(nb), kColThreads, shm, ctx->stream>>>(s, fb, fa, twc);   \
+#define LAUNCH_C(n, nv)                                                                         \
+  do {                                                                                          \
+    if (shm > 64 * 1024)                                                                        \
+      DSPX_HIP(hipFuncSetAttribute((const void*) k_stageC<n, MODE, nv>,                         \
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, int(shm)));      \
+    k_stageC<n, MODE, nv><<<unsigned(nb), kColThreads, shm, ctx->stream>>>(s, fb, fa, twc);     \
+  } while (0)
+#define CASE_C(n)                                    \
+  case n:                                            \
+    if (MODE == kStageCFilterbank || fa.nval == 1)   \
+      LAUNCH_C(n, 1);                                \
+    else if (fa.nval == 2)                           \
+      LAUNCH_C(n, 2);                                \
+    else                                             \
+      LAUNCH_C(n, 4);                                \
     break;
       CASE_C(16) CASE_C(32) CASE_C(64) CASE_C(128) CASE_C(256)
 #undef CASE_C
+#undef LAUNCH_C
       default: fail(DSPX_E_UNSUPPORTED, "N1=%u", N1);
     }
     DSPX_HIP(hipGetLastError());
@@ -579,6 +594,7 @@ int dspx_ltf_execute(dspx_ltf* L, const void* d_raw, float* d_prof, uint32_t* d_
       fa.G = L->G;
       fa.state = d.state;
       fa.npol_out = L->npol_out;
+      for (uint32_t i = 0; i < 4; i++) fa.voff[i] = (i / L->ndim_out) * d.nbin * L->ndim_out + (i % L->ndim_out);
       k.stage_c<kStageCFold>(np, FilterbankArgs{}, fa);
       ltf_record(L, ev++);
       L->stage_name[L->nstage++] = "stageC_colifft_detect_fold";

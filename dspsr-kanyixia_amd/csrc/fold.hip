@@ -36,9 +36,12 @@ __global__ void k_binplan(const FoldSegment* __restrict__ seg, uint32_t nseg, ui
   out[i] = (sg.isub << 16) | bin;
 }
 
-// hits[isub][bin] += count; a workgroup covers 4096 samples and keeps a
-// private histogram for the first two sub-integrations it meets.
+// hits[isub][bin] += count.  A workgroup covers 4096 samples; each thread
+// walks 16 consecutive samples, counting runs of equal plan entries (the bin
+// changes rarely from one sample to the next), and adds each run once into a
+// private histogram of the first two sub-integrations the workgroup meets.
 constexpr int kHitsChunk = 4096;
+constexpr int kHitsRun = kHitsChunk / 256;
 __global__ __launch_bounds__(256) void k_hits(const uint32_t* __restrict__ plan, uint64_t ndat,
                                               uint32_t nbin, uint32_t* __restrict__ hits)
 {
@@ -57,15 +60,27 @@ __global__ __launch_bounds__(256) void k_hits(const uint32_t* __restrict__ plan,
     sub0 = s;
   }
   __syncthreads();
-  for (uint64_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
-    uint32_t v = plan[i];
-    if (v == 0xffffffffu) continue;
+  auto flush = [&](uint32_t v, uint32_t n) {
+    if (v == 0xffffffffu || n == 0) return;
     uint32_t sub = v >> 16, bin = v & 0xffffu;
     if (sub - sub0 < 2)
-      atomicAdd(&h[(sub - sub0) * nbin + bin], 1u);
+      atomicAdd(&h[(sub - sub0) * nbin + bin], n);
     else
-      atomicAdd(&hits[uint64_t(sub) * nbin + bin], 1u);
+      atomicAdd(&hits[uint64_t(sub) * nbin + bin], n);
+  };
+  uint64_t a = i0 + uint64_t(threadIdx.x) * kHitsRun;
+  uint64_t b = min(i1, a + kHitsRun);
+  uint32_t cur = 0xffffffffu, n = 0;
+  for (uint64_t i = a; i < b; i++) {
+    uint32_t v = plan[i];
+    if (v != cur) {
+      flush(cur, n);
+      cur = v;
+      n = 0;
+    }
+    n++;
   }
+  flush(cur, n);
   __syncthreads();
   if (sub0 == 0xffffffffu) return;
   for (uint32_t k = threadIdx.x; k < 2 * nbin; k += blockDim.x)
@@ -107,18 +122,33 @@ __global__ __launch_bounds__(256) void k_fold_fpt(const float* __restrict__ in, 
   const float* x = in + uint64_t(chan) * chan_stride + uint64_t(pol) * pol_stride;
   const uint64_t prof_sub = uint64_t(nchan) * npol * hl;
   float* pc = prof + (uint64_t(chan) * npol + pol) * hl;
-  for (uint64_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
-    uint32_t v = plan[i];
-    if (v == 0xffffffffu) continue;
+  // each thread folds a run of consecutive samples, summing in registers
+  // while the plan entry is unchanged
+  auto flush = [&](uint32_t v, const float* acc) {
+    if (v == 0xffffffffu) return;
     uint32_t sub = v >> 16, bin = v & 0xffffu;
     for (uint32_t d = 0; d < ndim; d++) {
-      float val = x[i * ndim + d];
       if (sub - sub0 < 2)
-        atomicAdd(&hf[(sub - sub0) * hl + bin * ndim + d], val);
+        atomicAdd(&hf[(sub - sub0) * hl + bin * ndim + d], acc[d]);
       else
-        atomicAdd(&pc[uint64_t(sub) * prof_sub + bin * ndim + d], val);
+        atomicAdd(&pc[uint64_t(sub) * prof_sub + bin * ndim + d], acc[d]);
     }
+  };
+  constexpr int run = kFoldChunk / 256;
+  uint64_t a = i0 + uint64_t(threadIdx.x) * run;
+  uint64_t b = min(i1, a + run);
+  uint32_t cur = 0xffffffffu;
+  float acc[4] = {0.f, 0.f, 0.f, 0.f};
+  for (uint64_t i = a; i < b; i++) {
+    uint32_t v = plan[i];
+    if (v != cur) {
+      flush(cur, acc);
+      cur = v;
+      acc[0] = acc[1] = acc[2] = acc[3] = 0.f;
+    }
+    for (uint32_t d = 0; d < ndim; d++) acc[d] += x[i * ndim + d];
   }
+  flush(cur, acc);
   __syncthreads();
   if (sub0 == 0xffffffffu) return;
   for (uint32_t k = threadIdx.x; k < 2 * hl; k += blockDim.x)

@@ -37,18 +37,23 @@ struct LoadI8 {
     return offset_binary ? (float(byte) - 127.5f) * scale
                          : (float(int8_t(uint8_t(byte))) + 0.5f) * scale;
   }
-  __device__ __forceinline__ void operator()(uint32_t chan, uint64_t t, float2& a, float2& b) const
+  // packed raw word of both pols of one sample
+  using Word = uint32_t;
+  __device__ __forceinline__ Word word(uint32_t chan, uint64_t t) const
   {
     const uint8_t* q = raw + t * tstride + uint64_t(chan) * cstride;
-    if (npol == 2) {
-      uint32_t w = *reinterpret_cast<const uint32_t*>(q);
-      a = make_float2(cvt(w & 0xff), cvt((w >> 8) & 0xff));
-      b = make_float2(cvt((w >> 16) & 0xff), cvt(w >> 24));
-    } else {
-      uint32_t w = *reinterpret_cast<const uint16_t*>(q);
-      a = make_float2(cvt(w & 0xff), cvt(w >> 8));
-      b = make_float2(0.f, 0.f);
-    }
+    return npol == 2 ? *reinterpret_cast<const uint32_t*>(q) : uint32_t(*reinterpret_cast<const uint16_t*>(q));
+  }
+  __device__ __forceinline__ float2 pol(Word w, uint32_t ipol) const
+  {
+    w >>= 16 * ipol;
+    return make_float2(cvt(w & 0xff), cvt((w >> 8) & 0xff));
+  }
+  __device__ __forceinline__ void operator()(uint32_t chan, uint64_t t, float2& a, float2& b) const
+  {
+    Word w = word(chan, t);
+    a = pol(w, 0);
+    b = npol == 2 ? pol(w, 1) : make_float2(0.f, 0.f);
   }
 };
 
@@ -65,18 +70,22 @@ struct LoadI16 {
     if (offset_binary) v = int16_t(v ^ 0x8000);
     return float(v) + 0.5f;
   }
-  __device__ __forceinline__ void operator()(uint32_t chan, uint64_t t, float2& a, float2& b) const
+  using Word = uint2;
+  __device__ __forceinline__ Word word(uint32_t chan, uint64_t t) const
   {
     const int16_t* q = raw + t * tstride + uint64_t(chan) * cstride;
-    if (npol == 2) {
-      short4 w = *reinterpret_cast<const short4*>(q);
-      a = make_float2(cvt(w.x), cvt(w.y));
-      b = make_float2(cvt(w.z), cvt(w.w));
-    } else {
-      short2 w = *reinterpret_cast<const short2*>(q);
-      a = make_float2(cvt(w.x), cvt(w.y));
-      b = make_float2(0.f, 0.f);
-    }
+    return npol == 2 ? *reinterpret_cast<const uint2*>(q) : make_uint2(*reinterpret_cast<const uint32_t*>(q), 0u);
+  }
+  __device__ __forceinline__ float2 pol(Word w, uint32_t ipol) const
+  {
+    uint32_t v = ipol ? w.y : w.x;
+    return make_float2(cvt(int16_t(v & 0xffff)), cvt(int16_t(v >> 16)));
+  }
+  __device__ __forceinline__ void operator()(uint32_t chan, uint64_t t, float2& a, float2& b) const
+  {
+    Word w = word(chan, t);
+    a = pol(w, 0);
+    b = npol == 2 ? pol(w, 1) : make_float2(0.f, 0.f);
   }
 };
 
@@ -86,11 +95,18 @@ struct LoadF32 {
   uint64_t chan_stride;  // floats
   uint64_t pol_stride;   // floats
   uint32_t npol;
+  // the "word" is the sample's offset; each pol is loaded when converted
+  using Word = uint64_t;
+  __device__ __forceinline__ Word word(uint32_t chan, uint64_t t) const { return uint64_t(chan) * chan_stride + 2 * t; }
+  __device__ __forceinline__ float2 pol(Word w, uint32_t ipol) const
+  {
+    return *reinterpret_cast<const float2*>(in + w + (ipol ? pol_stride : 0));
+  }
   __device__ __forceinline__ void operator()(uint32_t chan, uint64_t t, float2& a, float2& b) const
   {
-    const float* q = in + uint64_t(chan) * chan_stride + 2 * t;
-    a = *reinterpret_cast<const float2*>(q);
-    b = npol == 2 ? *reinterpret_cast<const float2*>(q + pol_stride) : make_float2(0.f, 0.f);
+    Word w = word(chan, t);
+    a = pol(w, 0);
+    b = npol == 2 ? pol(w, 1) : make_float2(0.f, 0.f);
   }
 };
 
@@ -120,6 +136,8 @@ struct FoldArgs {
   const uint32_t* sub_first; // per part: first sub-integration its outputs touch
   uint32_t nbin, nval, ndim, nslot, G;
   uint32_t state, npol_out;
+  // profile offset of detected value i of bin b: voff[i] + b * ndim
+  uint32_t voff[4];
 };
 
 struct FilterbankArgs {
@@ -155,24 +173,24 @@ __global__ __launch_bounds__(kColThreads) void k_stageA(LD ld, ConvShape s, cons
   const uint32_t c0 = cb * C;
   const uint64_t t0 = uint64_t(p) * s.step + c0;
 
-  float2 va[E], vb[E];
+  // raw words of both pols stay packed until their pol is transformed
+  typename LD::Word w[E];
 #pragma unroll
   for (int q = 0; q < E / R0; q++) {
     int g = tid + q * kColThreads;
     int b = g % C, j = g / C;
 #pragma unroll
-    for (int r = 0; r < R0; r++) {
-      uint64_t t = t0 + uint64_t(s.N2) * uint32_t(j + r * (N1 / R0)) + b;
-      ld(chan, t, va[q * R0 + r], vb[q * R0 + r]);
-    }
+    for (int r = 0; r < R0; r++) w[q * R0 + r] = ld.word(chan, t0 + uint64_t(s.N2) * uint32_t(j + r * (N1 / R0)) + b);
   }
 
   const uint64_t plane = uint64_t(s.N1) * s.N2;
   float2* wbase = s.work + (uint64_t(chan) * s.npart + p) * s.npol * plane;
   for (uint32_t pol = 0; pol < s.npol; pol++) {
-    float2* v = pol == 0 ? va : vb;
+    float2 v[E];
+#pragma unroll
+    for (int i = 0; i < E; i++) v[i] = ld.pol(w[i], pol);
     fft_regs<LAY, N1, C, kColThreads, -1>(lds, v, twc, tid);
-    float2* w = wbase + pol * plane;
+    float2* out = wbase + pol * plane;
 #pragma unroll
     for (int q = 0; q < E / RL; q++) {
       int g = tid + q * kColThreads;
@@ -181,8 +199,7 @@ __global__ __launch_bounds__(kColThreads) void k_stageA(LD ld, ConvShape s, cons
       for (int r = 0; r < RL; r++) {
         uint32_t k1 = j + r * (N1 / RL);
         uint32_t n2 = c0 + b;
-        float2 y = cmul(v[q * RL + r], tw_N(s.tw, n2 * k1));
-        w[uint64_t(k1) * s.N2 + n2] = y;
+        out[uint64_t(k1) * s.N2 + n2] = cmul(v[q * RL + r], tw_N(s.tw, n2 * k1));
       }
     }
   }
@@ -190,7 +207,12 @@ __global__ __launch_bounds__(kColThreads) void k_stageA(LD ld, ConvShape s, cons
 
 // ------------------------------------------------------------------ stage B
 
-// Chirp of FFT bin k of channel chan (reference parity or FP64 truth).
+// Chirp of FFT bin k of channel chan.  The phase is evaluated in FP64 with
+// the reference's operation order (Dedispersion.C:166-186); in parity mode it
+// is rounded to float as the reference stores it (Dedispersion.C:104-111).
+// The (possibly huge) angle is reduced modulo 2 pi in FP64 (two-term
+// Cody-Waite with FMA), and only the reduced angle, |r| <= pi, goes through
+// single-precision sincos.
 __device__ __forceinline__ float2 chirp_onchip(const ChirpArgs& c, uint32_t chan, uint32_t k, uint32_t N)
 {
 #pragma clang fp contract(off)
@@ -198,17 +220,21 @@ __device__ __forceinline__ float2 chirp_onchip(const ChirpArgs& c, uint32_t chan
   double freq = double(ipt) * c.binwidth - c.half_cw;
   double cf = c.cf[chan];
   double phase = c.coeff[chan] * (freq * freq) / (cf + freq);
-  double sn, cs;
-  if (c.mode == DSPX_CHIRP_PARITY)
-    sincos(double(float(phase)), &sn, &cs);
-  else
-    sincos(phase, &sn, &cs);
-  return make_float2(c.snorm * float(cs), c.snorm * float(sn));
+  double th = c.mode == DSPX_CHIRP_PARITY ? double(float(phase)) : phase;
+  double n = rint(th * 0.15915494309189535);          // 1 / (2 pi)
+  double r = fma(-n, 6.283185307179586, th);           // 2 pi, high part
+  r = fma(-n, 2.4492935982947064e-16, r);              // 2 pi - high part
+  float sn, cs;
+  sincosf(float(r), &sn, &cs);
+  return make_float2(c.snorm * cs, c.snorm * sn);
 }
 
-// grid.x = N1 * npart * nchan, block = N2/16.
+// grid.x = N1 * npg * nchan, block = N2/16.  A workgroup owns row k1 of
+// channel chan for the parts [pg*ppg, min(npart, (pg+1)*ppg)): the chirp of
+// the row is generated once and applied to every part and polarisation.
 template <int N2>
-__global__ __launch_bounds__(N2 / 16) void k_stageB(ConvShape s, ChirpArgs c, const float2* __restrict__ twr)
+__global__ __launch_bounds__(N2 / 16) void k_stageB(ConvShape s, ChirpArgs c, const float2* __restrict__ twr,
+                                                    uint32_t ppg)
 {
   constexpr int NT = N2 / 16;
   constexpr int E = 16;
@@ -219,14 +245,16 @@ __global__ __launch_bounds__(N2 / 16) void k_stageB(ConvShape s, ChirpArgs c, co
   constexpr int RL = LastRadix<N2, +1>::v;    // inverse last pass
 
   const int tid = threadIdx.x;
+  const uint32_t npg = (s.npart + ppg - 1) / ppg;
   uint32_t bid = blockIdx.x;
   const uint32_t k1 = bid % s.N1;
   bid /= s.N1;
-  const uint32_t p = bid % s.npart;
-  const uint32_t chan = bid / s.npart;
+  const uint32_t pg = bid % npg;
+  const uint32_t chan = bid / npg;
+  const uint32_t p_begin = pg * ppg;
+  const uint32_t p_end = min(s.npart, p_begin + ppg);
   const uint32_t N = s.N1 * N2;
   const uint64_t plane = uint64_t(s.N1) * N2;
-  float2* rowbase = s.work + (uint64_t(chan) * s.npart + p) * s.npol * plane + uint64_t(k1) * N2;
 
   // chirp for the bins this thread holds after the forward transform:
   // k2 = tid + r * N2/RI (one butterfly per thread)
@@ -242,25 +270,27 @@ __global__ __launch_bounds__(N2 / 16) void k_stageB(ConvShape s, ChirpArgs c, co
       h[r] = chirp_onchip(c, chan, k, N);
   }
 
-  for (uint32_t pol = 0; pol < s.npol; pol++) {
-    float2* row = rowbase + pol * plane;
-    float2 v[E];
+  for (uint32_t p = p_begin; p < p_end; p++) {
+    float2* rowbase = s.work + (uint64_t(chan) * s.npart + p) * s.npol * plane + uint64_t(k1) * N2;
+    for (uint32_t pol = 0; pol < s.npol; pol++) {
+      float2* row = rowbase + pol * plane;
+      float2 v[E];
 #pragma unroll
-    for (int q = 0; q < E / RF; q++) {
-      int j = tid + q * NT;
+      for (int q = 0; q < E / RF; q++) {
+        int j = tid + q * NT;
 #pragma unroll
-      for (int r = 0; r < RF; r++) v[q * RF + r] = row[j + r * (N2 / RF)];
-    }
-    if (pol > 0) __syncthreads();  // previous pol's last LDS reads are done
-    fft_regs<LAY, N2, 1, NT, -1>(lds, v, twr, tid);
+        for (int r = 0; r < RF; r++) v[q * RF + r] = row[j + r * (N2 / RF)];
+      }
+      fft_regs<LAY, N2, 1, NT, -1>(lds, v, twr, tid);
 #pragma unroll
-    for (int r = 0; r < RI; r++) v[r] = cmul(v[r], h[r]);
-    fft_regs<LAY, N2, 1, NT, +1>(lds, v, twr, tid);
+      for (int r = 0; r < RI; r++) v[r] = cmul(v[r], h[r]);
+      fft_regs<LAY, N2, 1, NT, +1>(lds, v, twr, tid);
 #pragma unroll
-    for (int q = 0; q < E / RL; q++) {
-      int j = tid + q * NT;
+      for (int q = 0; q < E / RL; q++) {
+        int j = tid + q * NT;
 #pragma unroll
-      for (int r = 0; r < RL; r++) row[j + r * (N2 / RL)] = v[q * RL + r];
+        for (int r = 0; r < RL; r++) row[j + r * (N2 / RL)] = v[q * RL + r];
+      }
     }
   }
 }
@@ -309,11 +339,64 @@ __device__ __forceinline__ int detect_values(uint32_t state, uint32_t npol, floa
   return 4;
 }
 
-// Sums d[0..nv) over runs of consecutive lanes of the wavefront that carry the
-// same key (a segmented inclusive scan).  Returns true on the last lane of each
-// run, which then holds the run's totals.  Every lane must call it.
-__device__ __forceinline__ bool wave_run_sum(uint32_t key, float* d, int nv)
+// Detection specialised on the number of detected values: NV = 1 Intensity,
+// 2 PPQQ, 4 Coherence (stokes = false) or Stokes.
+template <int NV>
+__device__ __forceinline__ void detect_nv(bool stokes, uint32_t npol, float2 a, float2 b, float* d)
 {
+#pragma clang fp contract(off)
+  float pp = a.x * a.x;
+  pp += a.y * a.y;
+  if constexpr (NV == 1) {
+    if (npol == 2) {
+      pp += b.x * b.x;
+      pp += b.y * b.y;
+    }
+    d[0] = pp;
+  } else {
+    float qq = b.x * b.x;
+    qq += b.y * b.y;
+    if constexpr (NV == 2) {
+      d[0] = pp;
+      d[1] = qq;
+    } else {
+      float pq_r = a.x * b.x + a.y * b.y;
+      float pq_i = a.x * b.y - a.y * b.x;
+      float p2 = a.x * a.x + a.y * a.y;
+      float q2 = b.x * b.x + b.y * b.y;
+      d[0] = stokes ? p2 + q2 : p2;
+      d[1] = stokes ? p2 - q2 : q2;
+      d[2] = stokes ? 2.0f * pq_r : pq_r;
+      d[3] = stokes ? 2.0f * pq_i : pq_i;
+    }
+  }
+}
+
+// Adds the NV values of each lane into the LDS histogram at key = (slot <<
+// 16) | bin, summing equal keys across the wavefront first.  The common case
+// (every lane of the wavefront in one bin) is a butterfly reduction and one
+// atomic per value; otherwise a segmented scan over runs of equal keys.
+// Every lane must call it.
+template <int NV>
+__device__ __forceinline__ void wave_fold(uint32_t key, float* d, float* hist, const FoldArgs& fa)
+{
+  const uint32_t k0 = __builtin_amdgcn_readfirstlane(key);
+  if (__all(key == k0)) {
+    if (k0 == 0xffffffffu) return;
+#pragma unroll
+    for (int i = 0; i < NV; i++) {
+      float v = d[i];
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+      d[i] = v;
+    }
+    if ((threadIdx.x & 63u) == 0) {
+      float* hs = hist + (k0 >> 16) * (NV * fa.nbin) + (k0 & 0xffffu) * fa.ndim;
+#pragma unroll
+      for (int i = 0; i < NV; i++) atomicAdd(hs + fa.voff[i], d[i]);
+    }
+    return;
+  }
   const int lane = int(threadIdx.x & 63u);
   uint32_t kprev = __shfl_up(key, 1);
   int head = (lane == 0 || kprev != key) ? lane : 0;
@@ -322,7 +405,8 @@ __device__ __forceinline__ bool wave_run_sum(uint32_t key, float* d, int nv)
     int h = __shfl_up(head, off);
     if (lane >= off) head = max(head, h);
   }
-  for (int i = 0; i < nv; i++) {
+#pragma unroll
+  for (int i = 0; i < NV; i++) {
     float v = d[i];
 #pragma unroll
     for (int off = 1; off < 64; off <<= 1) {
@@ -332,11 +416,16 @@ __device__ __forceinline__ bool wave_run_sum(uint32_t key, float* d, int nv)
     d[i] = v;
   }
   uint32_t knext = __shfl_down(key, 1);
-  return lane == 63 || knext != key;
+  if ((lane == 63 || knext != key) && key != 0xffffffffu) {
+    float* hs = hist + (key >> 16) * (NV * fa.nbin) + (key & 0xffffu) * fa.ndim;
+#pragma unroll
+    for (int i = 0; i < NV; i++) atomicAdd(hs + fa.voff[i], d[i]);
+  }
 }
 
 // grid.x = G * npart * nchan (fold) or ncb * npart * nchan (filterbank).
-template <int N1, int MODE>
+// NV: detected values per sample in fold mode (1, 2 or 4).
+template <int N1, int MODE, int NV = 1>
 __global__ __launch_bounds__(kColThreads) void k_stageC(ConvShape s, FilterbankArgs fb, FoldArgs fa,
                                                        const float2* __restrict__ twc)
 {
@@ -419,18 +508,13 @@ __global__ __launch_bounds__(kColThreads) void k_stageC(ConvShape s, FilterbankA
           // wavefront, then one LDS atomic per run and value
           uint32_t plan = kept ? fa.binplan[idat] : 0xffffffffu;
           uint32_t key = plan == 0xffffffffu ? 0xffffffffu : (((plan >> 16) - sub0) << 16) | (plan & 0xffffu);
-          float d[4] = {0.f, 0.f, 0.f, 0.f};
-          int nv = detect_values(fa.state, s.npol, va[q * RL + r], vb[q * RL + r], d);
-          if (key == 0xffffffffu) d[0] = d[1] = d[2] = d[3] = 0.f;
-          bool tail = wave_run_sum(key, d, int(fa.nval));
-          if (tail && key != 0xffffffffu) {
-            uint32_t bin = key & 0xffffu;
-            float* hs = hist + (key >> 16) * fa.nval * fa.nbin;
-            for (int i = 0; i < nv; i++) {
-              uint32_t off = (i / fa.ndim) * fa.nbin * fa.ndim + bin * fa.ndim + (i % fa.ndim);
-              atomicAdd(hs + off, d[i]);
-            }
+          float d[NV];
+          detect_nv<NV>(fa.state == DSPX_STOKES, s.npol, va[q * RL + r], vb[q * RL + r], d);
+          if (key == 0xffffffffu) {
+#pragma unroll
+            for (int i = 0; i < NV; i++) d[i] = 0.f;
           }
+          wave_fold<NV>(key, d, hist, fa);
         }
       }
     }
