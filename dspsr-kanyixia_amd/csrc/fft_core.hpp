@@ -269,4 +269,25 @@ template <int L, int DIR> struct LastRadix {
   static constexpr int v = DIR < 0 ? RadixPlan<L>::r[RadixPlan<L>::n - 1] : RadixPlan<L>::r[0];
 };
 
+// Wavefront sum delivered in lane 63, by DPP row shifts and row broadcasts
+// (no LDS traffic): partial sums of 4 lanes, then of each 16-lane row in its
+// lane 15, then rows 0+1 into lane 31 and all four rows into lane 63.
+template <int CTRL, int ROW_MASK, int BANK_MASK>
+__device__ __forceinline__ float dpp_shift(float v)
+{
+  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL, ROW_MASK,
+                                                               BANK_MASK, true));
+}
+__device__ __forceinline__ float wave_sum_lane63(float v)
+{
+  float s = v + dpp_shift<0x111, 0xf, 0xf>(v);  // row_shr:1
+  s += dpp_shift<0x112, 0xf, 0xf>(v);           // row_shr:2
+  s += dpp_shift<0x113, 0xf, 0xf>(v);           // row_shr:3
+  s += dpp_shift<0x114, 0xf, 0xe>(s);           // row_shr:4, lanes 4..15 of each row
+  s += dpp_shift<0x118, 0xf, 0xc>(s);           // row_shr:8, lanes 8..15
+  s += dpp_shift<0x142, 0xa, 0xf>(s);           // row_bcast:15 into rows 1 and 3
+  s += dpp_shift<0x143, 0xc, 0xf>(s);           // row_bcast:31 into rows 2 and 3
+  return s;
+}
+
 }  // namespace dspx

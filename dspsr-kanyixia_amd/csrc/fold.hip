@@ -8,6 +8,7 @@
 #include <cmath>
 
 #include "dspx_internal.hpp"
+#include "fft_core.hpp"
 #include "fold_impl.hpp"
 
 namespace dspx {
@@ -36,12 +37,9 @@ __global__ void k_binplan(const FoldSegment* __restrict__ seg, uint32_t nseg, ui
   out[i] = (sg.isub << 16) | bin;
 }
 
-// hits[isub][bin] += count.  A workgroup covers 4096 samples; each thread
-// walks 16 consecutive samples, counting runs of equal plan entries (the bin
-// changes rarely from one sample to the next), and adds each run once into a
-// private histogram of the first two sub-integrations the workgroup meets.
+// hits[isub][bin] += count (Fold.C:827-836 for unzapped data).  A workgroup
+// covers 4096 consecutive samples.
 constexpr int kHitsChunk = 4096;
-constexpr int kHitsRun = kHitsChunk / 256;
 __global__ __launch_bounds__(256) void k_hits(const uint32_t* __restrict__ plan, uint64_t ndat,
                                               uint32_t nbin, uint32_t* __restrict__ hits)
 {
@@ -60,27 +58,28 @@ __global__ __launch_bounds__(256) void k_hits(const uint32_t* __restrict__ plan,
     sub0 = s;
   }
   __syncthreads();
-  auto flush = [&](uint32_t v, uint32_t n) {
-    if (v == 0xffffffffu || n == 0) return;
-    uint32_t sub = v >> 16, bin = v & 0xffffu;
-    if (sub - sub0 < 2)
-      atomicAdd(&h[(sub - sub0) * nbin + bin], n);
-    else
-      atomicAdd(&hits[uint64_t(sub) * nbin + bin], n);
-  };
-  uint64_t a = i0 + uint64_t(threadIdx.x) * kHitsRun;
-  uint64_t b = min(i1, a + kHitsRun);
-  uint32_t cur = 0xffffffffu, n = 0;
-  for (uint64_t i = a; i < b; i++) {
-    uint32_t v = plan[i];
-    if (v != cur) {
-      flush(cur, n);
-      cur = v;
-      n = 0;
+  for (uint64_t base = i0; base < i1; base += blockDim.x) {
+    uint64_t i = base + threadIdx.x;
+    uint32_t key = i < i1 ? plan[i] : 0xffffffffu;
+    // count as a float sum: exact for counts below 2^24
+    const uint32_t k0 = __builtin_amdgcn_readfirstlane(key);
+    if (__all(key == k0)) {
+      if (k0 == 0xffffffffu) continue;
+      if ((threadIdx.x & 63u) == 63) {
+        uint32_t sub = k0 >> 16, bin = k0 & 0xffffu;
+        if (sub - sub0 < 2)
+          atomicAdd(&h[(sub - sub0) * nbin + bin], 64u);
+        else
+          atomicAdd(&hits[uint64_t(sub) * nbin + bin], 64u);
+      }
+    } else if (key != 0xffffffffu) {
+      uint32_t sub = key >> 16, bin = key & 0xffffu;
+      if (sub - sub0 < 2)
+        atomicAdd(&h[(sub - sub0) * nbin + bin], 1u);
+      else
+        atomicAdd(&hits[uint64_t(sub) * nbin + bin], 1u);
     }
-    n++;
   }
-  flush(cur, n);
   __syncthreads();
   if (sub0 == 0xffffffffu) return;
   for (uint32_t k = threadIdx.x; k < 2 * nbin; k += blockDim.x)
@@ -89,8 +88,9 @@ __global__ __launch_bounds__(256) void k_hits(const uint32_t* __restrict__ plan,
 
 // Detected FPT input -> prof[isub][chan][pol][bin*ndim + idim].  A workgroup
 // folds a contiguous run of samples of one (chan, pol) into an LDS copy of
-// the (at most two) sub-integrations the run meets, then adds the non-zero
-// bins into the profile.
+// the (at most two) sub-integrations the run meets, summing each wavefront's
+// samples first when they share a bin, then adds the non-zero bins into the
+// profile.
 constexpr int kFoldChunk = 8192;
 __global__ __launch_bounds__(256) void k_fold_fpt(const float* __restrict__ in, uint64_t chan_stride,
                                                   uint64_t pol_stride, uint32_t npol, uint32_t ndim,
@@ -122,33 +122,29 @@ __global__ __launch_bounds__(256) void k_fold_fpt(const float* __restrict__ in, 
   const float* x = in + uint64_t(chan) * chan_stride + uint64_t(pol) * pol_stride;
   const uint64_t prof_sub = uint64_t(nchan) * npol * hl;
   float* pc = prof + (uint64_t(chan) * npol + pol) * hl;
-  // each thread folds a run of consecutive samples, summing in registers
-  // while the plan entry is unchanged
-  auto flush = [&](uint32_t v, const float* acc) {
-    if (v == 0xffffffffu) return;
-    uint32_t sub = v >> 16, bin = v & 0xffffu;
+  for (uint64_t base = i0; base < i1; base += blockDim.x) {
+    uint64_t i = base + threadIdx.x;
+    uint32_t key = i < i1 ? plan[i] : 0xffffffffu;
+    const uint32_t k0 = __builtin_amdgcn_readfirstlane(key);
+    const bool uniform = __all(key == k0);
     for (uint32_t d = 0; d < ndim; d++) {
+      float v = (key != 0xffffffffu) ? x[i * ndim + d] : 0.f;
+      uint32_t k = key;
+      if (uniform) {
+        if (k0 == 0xffffffffu) break;
+        v = wave_sum_lane63(v);
+        if ((threadIdx.x & 63u) != 63) continue;
+        k = k0;
+      } else if (key == 0xffffffffu) {
+        continue;
+      }
+      uint32_t sub = k >> 16, bin = k & 0xffffu;
       if (sub - sub0 < 2)
-        atomicAdd(&hf[(sub - sub0) * hl + bin * ndim + d], acc[d]);
+        atomicAdd(&hf[(sub - sub0) * hl + bin * ndim + d], v);
       else
-        atomicAdd(&pc[uint64_t(sub) * prof_sub + bin * ndim + d], acc[d]);
+        atomicAdd(&pc[uint64_t(sub) * prof_sub + bin * ndim + d], v);
     }
-  };
-  constexpr int run = kFoldChunk / 256;
-  uint64_t a = i0 + uint64_t(threadIdx.x) * run;
-  uint64_t b = min(i1, a + run);
-  uint32_t cur = 0xffffffffu;
-  float acc[4] = {0.f, 0.f, 0.f, 0.f};
-  for (uint64_t i = a; i < b; i++) {
-    uint32_t v = plan[i];
-    if (v != cur) {
-      flush(cur, acc);
-      cur = v;
-      acc[0] = acc[1] = acc[2] = acc[3] = 0.f;
-    }
-    for (uint32_t d = 0; d < ndim; d++) acc[d] += x[i * ndim + d];
   }
-  flush(cur, acc);
   __syncthreads();
   if (sub0 == 0xffffffffu) return;
   for (uint32_t k = threadIdx.x; k < 2 * hl; k += blockDim.x)

@@ -384,13 +384,8 @@ __device__ __forceinline__ void wave_fold(uint32_t key, float* d, float* hist, c
   if (__all(key == k0)) {
     if (k0 == 0xffffffffu) return;
 #pragma unroll
-    for (int i = 0; i < NV; i++) {
-      float v = d[i];
-#pragma unroll
-      for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
-      d[i] = v;
-    }
-    if ((threadIdx.x & 63u) == 0) {
+    for (int i = 0; i < NV; i++) d[i] = wave_sum_lane63(d[i]);
+    if ((threadIdx.x & 63u) == 63) {
       float* hs = hist + (k0 >> 16) * (NV * fa.nbin) + (k0 & 0xffffu) * fa.ndim;
 #pragma unroll
       for (int i = 0; i < NV; i++) atomicAdd(hs + fa.voff[i], d[i]);
