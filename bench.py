@@ -130,7 +130,11 @@ def main():
     synth = importlib.import_module("dspsr-kanyixia_amd.synth")
     w = WORKLOADS[args.workload]
     torch.cuda.init()
-    ctx = dspx.Context(0, torch.cuda.current_stream().cuda_stream)
+    # torch's fills and the library's kernels on one stream (the default
+    # stream's handle is 0, for which the library would create its own)
+    stream = torch.cuda.Stream()
+    torch.cuda.set_stream(stream)
+    ctx = dspx.Context(0, stream.cuda_stream)
     geom = dspx.geometry(w["cfreq"], w["bw"], w["nchan"], w["dm"])
     print("[bench] workload %s geometry %s" % (args.workload, geom), file=sys.stderr, flush=True)
     raw = synth.make_baseband_torch(w["nchan"], w["npol"], w["ndat"], w["cfreq"], w["bw"], w["dm"], w["period"],

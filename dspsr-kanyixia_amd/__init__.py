@@ -147,6 +147,10 @@ def binplan_host(phi, pps, nbin, ndat):
 
 
 class Context:
+    """dspx_ctx.  ``stream``: a HIP stream handle to enqueue on; None or 0
+    makes the library create a stream of its own (then torch work on the
+    default stream is NOT ordered with the library's)."""
+
     def __init__(self, device=0, stream=None):
         self.h = None
         h = P()

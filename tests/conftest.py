@@ -30,7 +30,12 @@ def ctx(dspx):
     import torch
     assert torch.cuda.is_available(), "GPU test needs a HIP device"
     torch.cuda.init()
-    c = dspx.Context(0, torch.cuda.current_stream().cuda_stream)
+    # one stream for torch's allocations / fills and for the library's kernels
+    # (the default stream's handle is 0, which the library would replace with a
+    # stream of its own, unordered with torch's work)
+    s = torch.cuda.Stream()
+    torch.cuda.set_stream(s)
+    c = dspx.Context(0, s.cuda_stream)
     yield c
     c.synchronize()
     c.close()
