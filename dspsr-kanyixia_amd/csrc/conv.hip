@@ -284,11 +284,24 @@ __global__ void k_zero_dm(const float* __restrict__ in, uint64_t ics, uint64_t i
   *reinterpret_cast<float2*>(o) = *reinterpret_cast<const float2*>(x);
 }
 
-// Sum the fused stage C partial histograms into the profiles:
-// prof[isub][chan][v] += sum over parts p touching isub and groups g.
-__global__ void k_fold_reduce(const float* __restrict__ partial, const uint32_t* __restrict__ sub_first,
-                              uint32_t npart, uint32_t G, uint32_t nslot, uint32_t hl, uint32_t nchan,
-                              uint32_t nsub, float* __restrict__ prof)
+// Sum the fused stage C partial histograms into the profiles, in a fixed
+// order (the result does not depend on scheduling):
+//   k_fold_reduce_g: part[chan][p][slot][e] = sum over groups g
+//   k_fold_reduce_p: prof[isub][chan][e] += sum over parts p touching isub
+__global__ void k_fold_reduce_g(const float* __restrict__ partial, uint32_t G, uint32_t nslot_hl, uint64_t n,
+                                float* __restrict__ out)
+{
+  uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;  // (chan, p, slot, e)
+  if (i >= n) return;
+  uint64_t cp = i / nslot_hl, r = i % nslot_hl;
+  const float* b = partial + cp * G * nslot_hl + r;
+  float acc = 0.f;
+  for (uint32_t g = 0; g < G; g++) acc += b[uint64_t(g) * nslot_hl];
+  out[i] = acc;
+}
+
+__global__ void k_fold_reduce_p(const float* __restrict__ part, const uint32_t* __restrict__ sub_first,
+                                uint32_t npart, uint32_t nslot, uint32_t hl, uint32_t nchan, float* __restrict__ prof)
 {
   uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
   uint32_t chan = blockIdx.y;
@@ -299,8 +312,7 @@ __global__ void k_fold_reduce(const float* __restrict__ partial, const uint32_t*
   for (uint32_t p = 0; p < npart; p++) {
     uint32_t s0 = sub_first[p];
     if (isub < s0 || isub >= s0 + nslot) continue;
-    const float* base = partial + ((uint64_t(chan) * npart + p) * G) * nslot * hl + uint64_t(isub - s0) * hl + e;
-    for (uint32_t g = 0; g < G; g++) acc += base[uint64_t(g) * nslot * hl];
+    acc += part[((uint64_t(chan) * npart + p) * nslot + (isub - s0)) * hl + e];
     any = true;
   }
   if (any) prof[(uint64_t(isub) * nchan + chan) * hl + e] += acc;
@@ -413,7 +425,8 @@ struct dspx_ltf {
   // fused fold partials
   uint32_t G = 1, nslot = 1;
   DevBuf sub_first;
-  DevBuf partial;
+  DevBuf partial;   // [nchan][npart][G][nslot][nval*nbin]
+  DevBuf partial2;  // [nchan][npart][nslot][nval*nbin]
   // channel-major copy of the raw block (nchan > 1)
   DevBuf ftp;
   // plain form intermediates
@@ -599,9 +612,14 @@ int dspx_ltf_execute(dspx_ltf* L, const void* d_raw, float* d_prof, uint32_t* d_
       ltf_record(L, ev++);
       L->stage_name[L->nstage++] = "stageC_colifft_detect_fold";
       uint32_t hl = L->nval * d.nbin;
+      uint64_t n2 = uint64_t(d.nchan) * np * L->nslot * hl;
+      L->partial2.reserve(n2 * sizeof(float));
+      k_fold_reduce_g<<<unsigned((n2 + 255) / 256), 256, 0, ctx->stream>>>(L->partial.as<float>(), L->G,
+                                                                          L->nslot * hl, n2, L->partial2.as<float>());
+      DSPX_HIP(hipGetLastError());
       dim3 grid((hl + 255) / 256, d.nchan, L->fold.nsub);
-      k_fold_reduce<<<grid, 256, 0, ctx->stream>>>(L->partial.as<float>(), L->sub_first.as<uint32_t>(), np, L->G,
-                                                   L->nslot, hl, d.nchan, L->fold.nsub, d_prof);
+      k_fold_reduce_p<<<grid, 256, 0, ctx->stream>>>(L->partial2.as<float>(), L->sub_first.as<uint32_t>(), np,
+                                                     L->nslot, hl, d.nchan, d_prof);
       DSPX_HIP(hipGetLastError());
       if (d_hits) L->fold.hits(d_hits);
       ltf_record(L, ev++);

@@ -290,4 +290,15 @@ __device__ __forceinline__ float wave_sum_lane63(float v)
   return s;
 }
 
+// Sum over the 16 lanes of each DPP row, delivered in the row's lane 15.
+__device__ __forceinline__ float row_sum_lane15(float v)
+{
+  float s = v + dpp_shift<0x111, 0xf, 0xf>(v);  // row_shr:1
+  s += dpp_shift<0x112, 0xf, 0xf>(v);           // row_shr:2
+  s += dpp_shift<0x113, 0xf, 0xf>(v);           // row_shr:3
+  s += dpp_shift<0x114, 0xf, 0xe>(s);           // row_shr:4, lanes 4..15
+  s += dpp_shift<0x118, 0xf, 0xc>(s);           // row_shr:8, lanes 8..15
+  return s;
+}
+
 }  // namespace dspx

@@ -418,6 +418,44 @@ __device__ __forceinline__ void wave_fold(uint32_t key, float* d, float* hist, c
   }
 }
 
+// Inverse column transform whose last pass hands each thread QN = E/RL
+// consecutive columns b = (tid % (C/QN))*QN + q of the rows
+// j + r*L/RL, j = tid / (C/QN).  On entry v holds the first pass's inputs in
+// the standard ownership; on exit v[q*RL + r] holds y(row j + r*L/RL,
+// column (tid % (C/QN))*QN + q).
+template <class LAY, int L, int C>
+__device__ __forceinline__ void last_pass_cols(float2* lds, float2* v, const float2* __restrict__ tw, int tid)
+{
+  constexpr int NPASS = RadixPlan<L>::n;
+  constexpr int E = kColTile / kColThreads;
+  constexpr int RL = LastRadix<L, +1>::v;
+  constexpr int QN = E / RL;
+  if constexpr (NPASS == 1) {
+    // single pass: the standard ownership already is one column per thread
+    static_assert(QN == 1, "single-pass column transform owns one column");
+    stockham_compute<L, C, kColThreads, RL, 1, +1>(v, tw, tid);
+  } else {
+    run_passes<LAY, L, C, kColThreads, +1, 0, NPASS - 1>(lds, v, tw, tid);
+    constexpr int RP = InvRadix<L, NPASS - 2>::v;
+    constexpr int NSP = InvNs<L, NPASS - 2>::v;
+    constexpr int NS = InvNs<L, NPASS - 1>::v;
+    __syncthreads();
+    lds_store_pass<LAY, L, C, kColThreads, RP, NSP>(lds, v, tid);
+    __syncthreads();
+    const int b0 = (tid % (C / QN)) * QN;
+    const int j = tid / (C / QN);
+    const int k = j % NS;
+#pragma unroll
+    for (int q = 0; q < QN; q++) {
+#pragma unroll
+      for (int r = 0; r < RL; r++) v[q * RL + r] = lds[LAY::idx(j + r * (L / RL), b0 + q)];
+#pragma unroll
+      for (int r = 1; r < RL; r++) v[q * RL + r] = cmulc(v[q * RL + r], tw[r * k * (L / (NS * RL))]);
+      dft<RL, +1>(v + q * RL);
+    }
+  }
+}
+
 // grid.x = G * npart * nchan (fold) or ncb * npart * nchan (filterbank).
 // NV: detected values per sample in fold mode (1, 2 or 4).
 template <int N1, int MODE, int NV = 1>
@@ -476,40 +514,89 @@ __global__ __launch_bounds__(kColThreads) void k_stageC(ConvShape s, FilterbankA
         vb[q * RI + r] = s.npol == 2 ? cmulc(wbase[plane + off], w) : make_float2(0.f, 0.f);
       }
     }
-    __syncthreads();  // previous tile's LDS reads are complete
-    fft_regs<LAY, N1, C, kColThreads, +1>(lds, va, twc, tid);
-    if (s.npol == 2) {
-      __syncthreads();
-      fft_regs<LAY, N1, C, kColThreads, +1>(lds, vb, twc, tid);
-    }
+    if constexpr (MODE == kStageCFilterbank) {
+      __syncthreads();  // previous tile's LDS reads are complete
+      fft_regs<LAY, N1, C, kColThreads, +1>(lds, va, twc, tid);
+      if (s.npol == 2) {
+        __syncthreads();
+        fft_regs<LAY, N1, C, kColThreads, +1>(lds, vb, twc, tid);
+      }
 #pragma unroll
-    for (int q = 0; q < E / RL; q++) {
-      int gg = tid + q * kColThreads;
-      int b = gg % C, j = gg / C;
+      for (int q = 0; q < E / RL; q++) {
+        int gg = tid + q * kColThreads;
+        int b = gg % C, j = gg / C;
 #pragma unroll
-      for (int r = 0; r < RL; r++) {
-        uint32_t n1 = j + r * (N1 / RL);
-        uint32_t t = s.N2 * n1 + c0 + b;
-        const bool kept = t >= s.nfilt_pos && t < s.nfilt_pos + s.step;
-        uint64_t idat = uint64_t(p) * s.step + (t - s.nfilt_pos);
-        if (MODE == kStageCFilterbank) {
-          if (!kept) continue;
+        for (int r = 0; r < RL; r++) {
+          uint32_t n1 = j + r * (N1 / RL);
+          uint32_t t = s.N2 * n1 + c0 + b;
+          if (t < s.nfilt_pos || t >= s.nfilt_pos + s.step) continue;
+          uint64_t idat = uint64_t(p) * s.step + (t - s.nfilt_pos);
           float* o = fb.out + uint64_t(chan) * fb.chan_stride + 2 * idat;
           *reinterpret_cast<float2*>(o) = va[q * RL + r];
           if (s.npol == 2) *reinterpret_cast<float2*>(o + fb.pol_stride) = vb[q * RL + r];
-        } else {
-          // consecutive lanes hold consecutive samples, which mostly share a
-          // phase bin: sum each run of equal (sub-integration, bin) across the
-          // wavefront, then one LDS atomic per run and value
-          uint32_t plan = kept ? fa.binplan[idat] : 0xffffffffu;
-          uint32_t key = plan == 0xffffffffu ? 0xffffffffu : (((plan >> 16) - sub0) << 16) | (plan & 0xffffu);
+        }
+      }
+    } else {
+      // all passes but the last in the standard ownership; the last pass
+      // gives each thread QN consecutive columns (consecutive samples) of the
+      // rows j + r*N1/RL, so a 16-lane DPP row spans whole tile rows
+      __syncthreads();
+      last_pass_cols<LAY, N1, C>(lds, va, twc, tid);
+      if (s.npol == 2) {
+        __syncthreads();
+        last_pass_cols<LAY, N1, C>(lds, vb, twc, tid);
+      }
+      constexpr int QN = E / RL;
+      const uint32_t bq0 = (tid % (C / QN)) * QN;
+      const uint32_t jr = tid / (C / QN);
+      const int lane = int(threadIdx.x & 63u);
+#pragma unroll
+      for (int r = 0; r < RL; r++) {
+        const uint32_t n1 = jr + r * (N1 / RL);
+        const uint32_t t0 = s.N2 * n1 + c0 + bq0;
+        float acc[NV];
+#pragma unroll
+        for (int i = 0; i < NV; i++) acc[i] = 0.f;
+        uint32_t lane_key = 0xffffffffu;
+        bool divergent = false;
+#pragma unroll
+        for (int q = 0; q < QN; q++) {
+          const uint32_t t = t0 + q;
+          const bool kept = t >= s.nfilt_pos && t < s.nfilt_pos + s.step;
+          const uint64_t idat = uint64_t(p) * s.step + (t - s.nfilt_pos);
+          const uint32_t plan = kept ? fa.binplan[idat] : 0xffffffffu;
+          const uint32_t key = plan == 0xffffffffu ? 0xffffffffu : (((plan >> 16) - sub0) << 16) | (plan & 0xffffu);
           float d[NV];
           detect_nv<NV>(fa.state == DSPX_STOKES, s.npol, va[q * RL + r], vb[q * RL + r], d);
-          if (key == 0xffffffffu) {
+          if (q == 0) lane_key = key;
+          if (key == lane_key) {
 #pragma unroll
-            for (int i = 0; i < NV; i++) d[i] = 0.f;
+            for (int i = 0; i < NV; i++) acc[i] += d[i];
+          } else if (key != 0xffffffffu) {
+            divergent = true;  // rare: a bin edge inside this lane's samples
+            float* hs = hist + (key >> 16) * (NV * fa.nbin) + (key & 0xffffu) * fa.ndim;
+#pragma unroll
+            for (int i = 0; i < NV; i++) atomicAdd(hs + fa.voff[i], d[i]);
           }
-          wave_fold<NV>(key, d, hist, fa);
+        }
+        (void) divergent;
+        // is the whole 16-lane row in one bin?
+        const uint32_t k15 = __shfl(lane_key, lane | 15);
+        const uint64_t same = __ballot(lane_key == k15);
+        const bool row_uniform = ((same >> (lane & 48)) & 0xffffull) == 0xffffull;
+        float rs[NV];
+#pragma unroll
+        for (int i = 0; i < NV; i++) rs[i] = row_sum_lane15(acc[i]);
+        if (row_uniform) {
+          if ((lane & 15) == 15 && k15 != 0xffffffffu) {
+            float* hs = hist + (k15 >> 16) * (NV * fa.nbin) + (k15 & 0xffffu) * fa.ndim;
+#pragma unroll
+            for (int i = 0; i < NV; i++) atomicAdd(hs + fa.voff[i], rs[i]);
+          }
+        } else if (lane_key != 0xffffffffu) {
+          float* hs = hist + (lane_key >> 16) * (NV * fa.nbin) + (lane_key & 0xffffu) * fa.ndim;
+#pragma unroll
+          for (int i = 0; i < NV; i++) atomicAdd(hs + fa.voff[i], acc[i]);
         }
       }
     }
