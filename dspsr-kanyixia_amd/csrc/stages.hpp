@@ -2,11 +2,12 @@
 //
 // An nfft-point transform is split N = N1 * N2 (four-step):
 //   input  x[N2*n1 + n2]          (n1 < N1, n2 < N2)
-//   stage A: per column n2, forward N1-point FFT over n1, times W_N^{n2*k1}
-//            -> work[k1][n2]                             (unpack fused in)
-//   stage B: per row k1, forward N2-point FFT over n2 (bin k = k1 + N1*k2),
-//            times the chirp H[k], inverse N2-point FFT, times W_N^{-n2*k1}
-//            (chirp generated on chip in FP64)
+//   stage A: per column n2, forward N1-point FFT over n1 -> work[k1][n2]
+//            (unpack fused in)
+//   stage B: per row k1, times W_N^{n2*k1}, forward N2-point FFT over n2
+//            (bin k = k1 + N1*k2), times the chirp H[k], inverse N2-point FFT,
+//            times W_N^{-n2*k1} (chirp and twiddles generated once per row and
+//            reused for every part and polarisation)
 //   stage C: per column n2, inverse N1-point FFT over k1 -> y[N2*n1 + n2],
 //            overlap discard, then either the filterbank store or detection
 //            plus phase-bin accumulation in LDS.
@@ -199,7 +200,7 @@ __global__ __launch_bounds__(kColThreads) void k_stageA(LD ld, ConvShape s, cons
       for (int r = 0; r < RL; r++) {
         uint32_t k1 = j + r * (N1 / RL);
         uint32_t n2 = c0 + b;
-        out[uint64_t(k1) * s.N2 + n2] = cmul(v[q * RL + r], tw_N(s.tw, n2 * k1));
+        out[uint64_t(k1) * s.N2 + n2] = v[q * RL + r];
       }
     }
   }
@@ -270,6 +271,18 @@ __global__ __launch_bounds__(N2 / 16) void k_stageB(ConvShape s, ChirpArgs c, co
       h[r] = chirp_onchip(c, chan, k, N);
   }
 
+  // inter-stage twiddles W_N^{n2*k1} of the elements this thread loads and
+  // stores (the forward plan's first pass and the inverse plan's last pass
+  // use the same radix, hence the same elements n2 = j + r*N2/RF)
+  static_assert(RF == RL, "first forward and last inverse radix must agree");
+  float2 w[E];
+#pragma unroll
+  for (int q = 0; q < E / RF; q++) {
+    int j = tid + q * NT;
+#pragma unroll
+    for (int r = 0; r < RF; r++) w[q * RF + r] = tw_N(s.tw, uint32_t(j + r * (N2 / RF)) * k1);
+  }
+
   for (uint32_t p = p_begin; p < p_end; p++) {
     float2* rowbase = s.work + (uint64_t(chan) * s.npart + p) * s.npol * plane + uint64_t(k1) * N2;
     for (uint32_t pol = 0; pol < s.npol; pol++) {
@@ -279,7 +292,7 @@ __global__ __launch_bounds__(N2 / 16) void k_stageB(ConvShape s, ChirpArgs c, co
       for (int q = 0; q < E / RF; q++) {
         int j = tid + q * NT;
 #pragma unroll
-        for (int r = 0; r < RF; r++) v[q * RF + r] = row[j + r * (N2 / RF)];
+        for (int r = 0; r < RF; r++) v[q * RF + r] = cmul(row[j + r * (N2 / RF)], w[q * RF + r]);
       }
       fft_regs<LAY, N2, 1, NT, -1>(lds, v, twr, tid);
 #pragma unroll
@@ -289,7 +302,7 @@ __global__ __launch_bounds__(N2 / 16) void k_stageB(ConvShape s, ChirpArgs c, co
       for (int q = 0; q < E / RL; q++) {
         int j = tid + q * NT;
 #pragma unroll
-        for (int r = 0; r < RL; r++) row[j + r * (N2 / RL)] = v[q * RL + r];
+        for (int r = 0; r < RL; r++) row[j + r * (N2 / RL)] = cmulc(v[q * RL + r], w[q * RL + r]);
       }
     }
   }
@@ -507,11 +520,9 @@ __global__ __launch_bounds__(kColThreads) void k_stageC(ConvShape s, FilterbankA
 #pragma unroll
       for (int r = 0; r < RI; r++) {
         uint32_t k1 = j + r * (N1 / RI);
-        uint32_t n2 = c0 + b;
-        float2 w = tw_N(s.tw, n2 * k1);
-        uint64_t off = uint64_t(k1) * s.N2 + n2;
-        va[q * RI + r] = cmulc(wbase[off], w);
-        vb[q * RI + r] = s.npol == 2 ? cmulc(wbase[plane + off], w) : make_float2(0.f, 0.f);
+        uint64_t off = uint64_t(k1) * s.N2 + c0 + b;
+        va[q * RI + r] = wbase[off];
+        vb[q * RI + r] = s.npol == 2 ? wbase[plane + off] : make_float2(0.f, 0.f);
       }
     }
     if constexpr (MODE == kStageCFilterbank) {
