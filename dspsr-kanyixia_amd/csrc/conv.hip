@@ -184,7 +184,7 @@ struct ConvCore {
     const float2* twr = twiddles(ctx, N2);
     switch (N2) {
 #define CASE_B(n) \
-  case n: k_stageB<n><<<unsigned(nb), n / 16, 0, ctx->stream>>>(s, c, twr, ppg); break;
+  case n: k_stageB<n><<<unsigned(nb), n / 8, 0, ctx->stream>>>(s, c, twr, ppg); break;
       CASE_B(256) CASE_B(512) CASE_B(1024) CASE_B(2048) CASE_B(4096)
 #undef CASE_B
       default: fail(DSPX_E_UNSUPPORTED, "N2=%u", N2);

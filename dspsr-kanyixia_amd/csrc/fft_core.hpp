@@ -198,36 +198,44 @@ __device__ __forceinline__ void lds_store_pass(float2* lds, const float2* v, int
   }
 }
 
-// Radix plans.  The first radix of the forward plan equals the last radix of
-// the inverse plan and vice versa (inverse = reversed forward), so a forward
-// transform's last pass and the inverse's first pass share thread ownership.
-template <int L> struct RadixPlan;
-template <> struct RadixPlan<2>    { static constexpr int n = 1; static constexpr int r[4] = {2, 0, 0, 0}; };
-template <> struct RadixPlan<4>    { static constexpr int n = 1; static constexpr int r[4] = {4, 0, 0, 0}; };
-template <> struct RadixPlan<8>    { static constexpr int n = 1; static constexpr int r[4] = {8, 0, 0, 0}; };
-template <> struct RadixPlan<16>   { static constexpr int n = 1; static constexpr int r[4] = {16, 0, 0, 0}; };
-template <> struct RadixPlan<32>   { static constexpr int n = 2; static constexpr int r[4] = {2, 16, 0, 0}; };
-template <> struct RadixPlan<64>   { static constexpr int n = 2; static constexpr int r[4] = {4, 16, 0, 0}; };
-template <> struct RadixPlan<128>  { static constexpr int n = 2; static constexpr int r[4] = {8, 16, 0, 0}; };
-template <> struct RadixPlan<256>  { static constexpr int n = 2; static constexpr int r[4] = {16, 16, 0, 0}; };
-template <> struct RadixPlan<512>  { static constexpr int n = 3; static constexpr int r[4] = {2, 16, 16, 0}; };
-template <> struct RadixPlan<1024> { static constexpr int n = 3; static constexpr int r[4] = {4, 16, 16, 0}; };
-template <> struct RadixPlan<2048> { static constexpr int n = 3; static constexpr int r[4] = {8, 16, 16, 0}; };
-template <> struct RadixPlan<4096> { static constexpr int n = 3; static constexpr int r[4] = {16, 16, 16, 0}; };
-template <> struct RadixPlan<8192> { static constexpr int n = 4; static constexpr int r[4] = {2, 16, 16, 16}; };
+// Radix plans, keyed by the transform length L and the complex values each
+// thread owns (E = 16 for the column tiles, 8 for the row transforms).  The
+// inverse uses the reversed plan, so a forward transform's last pass and the
+// inverse's first pass share thread ownership.
+template <int L, int E> struct RadixPlan;
+#define DSPX_PLAN(L, E, N, R0, R1, R2, R3) \
+  template <> struct RadixPlan<L, E> { static constexpr int n = N; static constexpr int r[4] = {R0, R1, R2, R3}; };
+DSPX_PLAN(2, 16, 1, 2, 0, 0, 0)
+DSPX_PLAN(4, 16, 1, 4, 0, 0, 0)
+DSPX_PLAN(8, 16, 1, 8, 0, 0, 0)
+DSPX_PLAN(16, 16, 1, 16, 0, 0, 0)
+DSPX_PLAN(32, 16, 2, 2, 16, 0, 0)
+DSPX_PLAN(64, 16, 2, 4, 16, 0, 0)
+DSPX_PLAN(128, 16, 2, 8, 16, 0, 0)
+DSPX_PLAN(256, 16, 2, 16, 16, 0, 0)
+DSPX_PLAN(512, 16, 3, 2, 16, 16, 0)
+DSPX_PLAN(1024, 16, 3, 4, 16, 16, 0)
+DSPX_PLAN(2048, 16, 3, 8, 16, 16, 0)
+DSPX_PLAN(4096, 16, 3, 16, 16, 16, 0)
+DSPX_PLAN(256, 8, 3, 4, 8, 8, 0)
+DSPX_PLAN(512, 8, 3, 8, 8, 8, 0)
+DSPX_PLAN(1024, 8, 4, 2, 8, 8, 8)
+DSPX_PLAN(2048, 8, 4, 4, 8, 8, 8)
+DSPX_PLAN(4096, 8, 4, 8, 8, 8, 8)
+#undef DSPX_PLAN
 
 // Ns before pass p of the forward plan (product of earlier radices).
-template <int L, int P> struct FwdNs {
-  static constexpr int v = (P == 0) ? 1 : FwdNs<L, P - 1>::v * RadixPlan<L>::r[P - 1 < 0 ? 0 : P - 1];
+template <int L, int E, int P> struct FwdNs {
+  static constexpr int v = FwdNs<L, E, P - 1>::v * RadixPlan<L, E>::r[P - 1];
 };
-template <int L> struct FwdNs<L, 0> { static constexpr int v = 1; };
+template <int L, int E> struct FwdNs<L, E, 0> { static constexpr int v = 1; };
 
 // Inverse plan uses the reversed radix order: pass p has radix r[n-1-p].
-template <int L, int P> struct InvRadix { static constexpr int v = RadixPlan<L>::r[RadixPlan<L>::n - 1 - P]; };
-template <int L, int P> struct InvNs {
-  static constexpr int v = (P == 0) ? 1 : InvNs<L, P - 1>::v * InvRadix<L, (P - 1 < 0 ? 0 : P - 1)>::v;
+template <int L, int E, int P> struct InvRadix { static constexpr int v = RadixPlan<L, E>::r[RadixPlan<L, E>::n - 1 - P]; };
+template <int L, int E, int P> struct InvNs {
+  static constexpr int v = InvNs<L, E, P - 1>::v * InvRadix<L, E, P - 1>::v;
 };
-template <int L> struct InvNs<L, 0> { static constexpr int v = 1; };
+template <int L, int E> struct InvNs<L, E, 0> { static constexpr int v = 1; };
 
 // Run passes [P0, P1) of the forward (DIR=-1, forward plan) or inverse
 // (DIR=+1, reversed plan) transform.  On entry v holds the inputs of pass P0
@@ -236,13 +244,14 @@ template <int L> struct InvNs<L, 0> { static constexpr int v = 1; };
 template <class LAY, int L, int B, int NT, int DIR, int P, int PEND>
 __device__ __forceinline__ void run_passes(float2* lds, float2* v, const float2* __restrict__ tw, int tid)
 {
+  constexpr int E = L * B / NT;
   if constexpr (P < PEND) {
-    constexpr int R = DIR < 0 ? RadixPlan<L>::r[P] : InvRadix<L, P>::v;
-    constexpr int NS = DIR < 0 ? FwdNs<L, P>::v : InvNs<L, P>::v;
+    constexpr int R = DIR < 0 ? RadixPlan<L, E>::r[P] : InvRadix<L, E, P>::v;
+    constexpr int NS = DIR < 0 ? FwdNs<L, E, P>::v : InvNs<L, E, P>::v;
     if constexpr (P > 0) {
       // previous pass left outputs in registers: scatter them, then gather
-      constexpr int RP = DIR < 0 ? RadixPlan<L>::r[P - 1] : InvRadix<L, P - 1>::v;
-      constexpr int NSP = DIR < 0 ? FwdNs<L, P - 1>::v : InvNs<L, P - 1>::v;
+      constexpr int RP = DIR < 0 ? RadixPlan<L, E>::r[P - 1] : InvRadix<L, E, P - 1>::v;
+      constexpr int NSP = DIR < 0 ? FwdNs<L, E, P - 1>::v : InvNs<L, E, P - 1>::v;
       __syncthreads();
       lds_store_pass<LAY, L, B, NT, RP, NSP>(lds, v, tid);
       __syncthreads();
@@ -258,15 +267,24 @@ __device__ __forceinline__ void run_passes(float2* lds, float2* v, const float2*
 template <class LAY, int L, int B, int NT, int DIR>
 __device__ __forceinline__ void fft_regs(float2* lds, float2* v, const float2* __restrict__ tw, int tid)
 {
-  run_passes<LAY, L, B, NT, DIR, 0, RadixPlan<L>::n>(lds, v, tw, tid);
+  run_passes<LAY, L, B, NT, DIR, 0, RadixPlan<L, L * B / NT>::n>(lds, v, tw, tid);
+}
+
+// Returns p unchanged, opaquely: loads through the result cannot be hoisted
+// out of the loop that calls this.  Keeps per-pass twiddle loads inside the
+// loop over rows instead of holding every pass's twiddles in registers.
+template <class T> __device__ __forceinline__ const T* opaque(const T* p)
+{
+  asm volatile("" : "+s"(p));
+  return p;
 }
 
 // Radix of the first / last pass of each direction.
-template <int L, int DIR> struct FirstRadix {
-  static constexpr int v = DIR < 0 ? RadixPlan<L>::r[0] : RadixPlan<L>::r[RadixPlan<L>::n - 1];
+template <int L, int E, int DIR> struct FirstRadix {
+  static constexpr int v = DIR < 0 ? RadixPlan<L, E>::r[0] : RadixPlan<L, E>::r[RadixPlan<L, E>::n - 1];
 };
-template <int L, int DIR> struct LastRadix {
-  static constexpr int v = DIR < 0 ? RadixPlan<L>::r[RadixPlan<L>::n - 1] : RadixPlan<L>::r[0];
+template <int L, int E, int DIR> struct LastRadix {
+  static constexpr int v = DIR < 0 ? RadixPlan<L, E>::r[RadixPlan<L, E>::n - 1] : RadixPlan<L, E>::r[0];
 };
 
 // Wavefront sum delivered in lane 63, by DPP row shifts and row broadcasts

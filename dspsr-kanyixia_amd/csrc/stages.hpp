@@ -159,8 +159,8 @@ __global__ __launch_bounds__(kColThreads) void k_stageA(LD ld, ConvShape s, cons
 {
   constexpr int C = kColTile / N1;
   constexpr int E = kColTile / kColThreads;  // 16
-  constexpr int R0 = FirstRadix<N1, -1>::v;
-  constexpr int RL = LastRadix<N1, -1>::v;
+  constexpr int R0 = FirstRadix<N1, 16, -1>::v;
+  constexpr int RL = LastRadix<N1, 16, -1>::v;
   using LAY = ColLayout<N1, C, C, kColThreads>;
   __shared__ float2 lds[kColTile];
 
@@ -234,16 +234,16 @@ __device__ __forceinline__ float2 chirp_onchip(const ChirpArgs& c, uint32_t chan
 // channel chan for the parts [pg*ppg, min(npart, (pg+1)*ppg)): the chirp of
 // the row is generated once and applied to every part and polarisation.
 template <int N2>
-__global__ __launch_bounds__(N2 / 16) void k_stageB(ConvShape s, ChirpArgs c, const float2* __restrict__ twr,
-                                                    uint32_t ppg)
+__global__ __launch_bounds__(N2 / 8) void k_stageB(ConvShape s, ChirpArgs c, const float2* __restrict__ twr,
+                                                   uint32_t ppg)
 {
-  constexpr int NT = N2 / 16;
-  constexpr int E = 16;
+  constexpr int NT = N2 / 8;
+  constexpr int E = 8;
   using LAY = RowLayout<N2, NT>;
   __shared__ float2 lds[N2 + N2 / 16];
-  constexpr int RF = FirstRadix<N2, -1>::v;   // forward first pass
-  constexpr int RI = FirstRadix<N2, +1>::v;   // inverse first pass (= forward last)
-  constexpr int RL = LastRadix<N2, +1>::v;    // inverse last pass
+  constexpr int RF = FirstRadix<N2, E, -1>::v;   // forward first pass
+  constexpr int RI = FirstRadix<N2, E, +1>::v;   // inverse first pass (= forward last)
+  constexpr int RL = LastRadix<N2, E, +1>::v;    // inverse last pass
 
   const int tid = threadIdx.x;
   const uint32_t npg = (s.npart + ppg - 1) / ppg;
@@ -257,36 +257,48 @@ __global__ __launch_bounds__(N2 / 16) void k_stageB(ConvShape s, ChirpArgs c, co
   const uint32_t N = s.N1 * N2;
   const uint64_t plane = uint64_t(s.N1) * N2;
 
-  // chirp for the bins this thread holds after the forward transform:
-  // k2 = tid + r * N2/RI (one butterfly per thread)
-  static_assert(E / RI == 1, "row plan must end with a radix-16 pass");
-  float2 h[RI];
-#pragma unroll
+  // chirp for the bins this thread holds after the forward transform
+  // (k2 = tid + r * N2/RI, one butterfly per thread) and the inter-stage
+  // twiddles W_N^{n2*k1} of the elements it loads and stores (the forward
+  // plan's first pass and the inverse plan's last pass use the same radix,
+  // hence the same elements n2 = j + r*N2/RF).  Each is evaluated one value
+  // at a time into the LDS row buffer (bounding the registers the FP64 chirp
+  // needs) and then gathered into registers.
+  static_assert(E / RI == 1, "row plan must end with a pass of radix E");
+  static_assert(RF == RL, "first forward and last inverse radix must agree");
+#pragma unroll 1
   for (int r = 0; r < RI; r++) {
     uint32_t k2 = tid + r * (N2 / RI);
-    uint32_t k = k1 + s.N1 * k2;
-    if (c.mode == DSPX_CHIRP_TABLE)
-      h[r] = c.table[(uint64_t(chan) * s.N1 + k1) * N2 + k2];
-    else
-      h[r] = chirp_onchip(c, chan, k, N);
+    float2 hv = c.mode == DSPX_CHIRP_TABLE ? c.table[(uint64_t(chan) * s.N1 + k1) * N2 + k2]
+                                           : chirp_onchip(c, chan, k1 + s.N1 * k2, N);
+    lds[LAY::idx(k2, 0)] = hv;
   }
-
-  // inter-stage twiddles W_N^{n2*k1} of the elements this thread loads and
-  // stores (the forward plan's first pass and the inverse plan's last pass
-  // use the same radix, hence the same elements n2 = j + r*N2/RF)
-  static_assert(RF == RL, "first forward and last inverse radix must agree");
+  __syncthreads();
+  float2 h[RI];
+#pragma unroll
+  for (int r = 0; r < RI; r++) h[r] = lds[LAY::idx(tid + r * (N2 / RI), 0)];
+  __syncthreads();
+#pragma unroll 1
+  for (int e = 0; e < E; e++) {
+    int q = e / RF, r = e % RF;
+    uint32_t n2 = uint32_t(tid + q * NT + r * (N2 / RF));
+    lds[LAY::idx(n2, 0)] = tw_N(s.tw, n2 * k1);
+  }
+  __syncthreads();
   float2 w[E];
 #pragma unroll
   for (int q = 0; q < E / RF; q++) {
     int j = tid + q * NT;
 #pragma unroll
-    for (int r = 0; r < RF; r++) w[q * RF + r] = tw_N(s.tw, uint32_t(j + r * (N2 / RF)) * k1);
+    for (int r = 0; r < RF; r++) w[q * RF + r] = lds[LAY::idx(j + r * (N2 / RF), 0)];
   }
+  __syncthreads();
 
   for (uint32_t p = p_begin; p < p_end; p++) {
     float2* rowbase = s.work + (uint64_t(chan) * s.npart + p) * s.npol * plane + uint64_t(k1) * N2;
     for (uint32_t pol = 0; pol < s.npol; pol++) {
       float2* row = rowbase + pol * plane;
+      const float2* tw = opaque(twr);
       float2 v[E];
 #pragma unroll
       for (int q = 0; q < E / RF; q++) {
@@ -294,10 +306,10 @@ __global__ __launch_bounds__(N2 / 16) void k_stageB(ConvShape s, ChirpArgs c, co
 #pragma unroll
         for (int r = 0; r < RF; r++) v[q * RF + r] = cmul(row[j + r * (N2 / RF)], w[q * RF + r]);
       }
-      fft_regs<LAY, N2, 1, NT, -1>(lds, v, twr, tid);
+      fft_regs<LAY, N2, 1, NT, -1>(lds, v, tw, tid);
 #pragma unroll
       for (int r = 0; r < RI; r++) v[r] = cmul(v[r], h[r]);
-      fft_regs<LAY, N2, 1, NT, +1>(lds, v, twr, tid);
+      fft_regs<LAY, N2, 1, NT, +1>(lds, v, opaque(tw), tid);
 #pragma unroll
       for (int q = 0; q < E / RL; q++) {
         int j = tid + q * NT;
@@ -439,9 +451,9 @@ __device__ __forceinline__ void wave_fold(uint32_t key, float* d, float* hist, c
 template <class LAY, int L, int C>
 __device__ __forceinline__ void last_pass_cols(float2* lds, float2* v, const float2* __restrict__ tw, int tid)
 {
-  constexpr int NPASS = RadixPlan<L>::n;
+  constexpr int NPASS = RadixPlan<L, 16>::n;
   constexpr int E = kColTile / kColThreads;
-  constexpr int RL = LastRadix<L, +1>::v;
+  constexpr int RL = LastRadix<L, 16, +1>::v;
   constexpr int QN = E / RL;
   if constexpr (NPASS == 1) {
     // single pass: the standard ownership already is one column per thread
@@ -449,9 +461,9 @@ __device__ __forceinline__ void last_pass_cols(float2* lds, float2* v, const flo
     stockham_compute<L, C, kColThreads, RL, 1, +1>(v, tw, tid);
   } else {
     run_passes<LAY, L, C, kColThreads, +1, 0, NPASS - 1>(lds, v, tw, tid);
-    constexpr int RP = InvRadix<L, NPASS - 2>::v;
-    constexpr int NSP = InvNs<L, NPASS - 2>::v;
-    constexpr int NS = InvNs<L, NPASS - 1>::v;
+    constexpr int RP = InvRadix<L, 16, NPASS - 2>::v;
+    constexpr int NSP = InvNs<L, 16, NPASS - 2>::v;
+    constexpr int NS = InvNs<L, 16, NPASS - 1>::v;
     __syncthreads();
     lds_store_pass<LAY, L, C, kColThreads, RP, NSP>(lds, v, tid);
     __syncthreads();
@@ -477,8 +489,8 @@ __global__ __launch_bounds__(kColThreads) void k_stageC(ConvShape s, FilterbankA
 {
   constexpr int C = kColTile / N1;
   constexpr int E = kColTile / kColThreads;
-  constexpr int RI = FirstRadix<N1, +1>::v;
-  constexpr int RL = LastRadix<N1, +1>::v;
+  constexpr int RI = FirstRadix<N1, 16, +1>::v;
+  constexpr int RL = LastRadix<N1, 16, +1>::v;
   using LAY = ColLayout<N1, C, C, kColThreads>;
   extern __shared__ float2 smem[];
   float2* lds = smem;
@@ -512,6 +524,7 @@ __global__ __launch_bounds__(kColThreads) void k_stageC(ConvShape s, FilterbankA
 
   for (uint32_t cb = cb_begin; cb < cb_end; cb++) {
     const uint32_t c0 = cb * C;
+    const float2* tw = opaque(twc);
     float2 va[E], vb[E];
 #pragma unroll
     for (int q = 0; q < E / RI; q++) {
@@ -527,10 +540,10 @@ __global__ __launch_bounds__(kColThreads) void k_stageC(ConvShape s, FilterbankA
     }
     if constexpr (MODE == kStageCFilterbank) {
       __syncthreads();  // previous tile's LDS reads are complete
-      fft_regs<LAY, N1, C, kColThreads, +1>(lds, va, twc, tid);
+      fft_regs<LAY, N1, C, kColThreads, +1>(lds, va, tw, tid);
       if (s.npol == 2) {
         __syncthreads();
-        fft_regs<LAY, N1, C, kColThreads, +1>(lds, vb, twc, tid);
+        fft_regs<LAY, N1, C, kColThreads, +1>(lds, vb, opaque(tw), tid);
       }
 #pragma unroll
       for (int q = 0; q < E / RL; q++) {
@@ -552,10 +565,10 @@ __global__ __launch_bounds__(kColThreads) void k_stageC(ConvShape s, FilterbankA
       // gives each thread QN consecutive columns (consecutive samples) of the
       // rows j + r*N1/RL, so a 16-lane DPP row spans whole tile rows
       __syncthreads();
-      last_pass_cols<LAY, N1, C>(lds, va, twc, tid);
+      last_pass_cols<LAY, N1, C>(lds, va, tw, tid);
       if (s.npol == 2) {
         __syncthreads();
-        last_pass_cols<LAY, N1, C>(lds, vb, twc, tid);
+        last_pass_cols<LAY, N1, C>(lds, vb, opaque(tw), tid);
       }
       constexpr int QN = E / RL;
       const uint32_t bq0 = (tid % (C / QN)) * QN;
