@@ -153,13 +153,14 @@ class Context:
 
     def __init__(self, device=0, stream=None):
         self.h = None
+        self._lib = lib()  # held so that close() still works at interpreter exit
         h = P()
         _check(lib().dspx_ctx_create(device, stream, C.byref(h)))
         self.h = h
 
     def close(self):
         if self.h:
-            lib().dspx_ctx_destroy(self.h)
+            self._lib.dspx_ctx_destroy(self.h)
             self.h = None
 
     __del__ = close
@@ -197,6 +198,7 @@ class Convolution:
                  form=FORM_FUSED, centre_frequency=0.0, bandwidth=0.0, dm=0.0, dc_centred=False):
         self.ctx = ctx
         self.h = None
+        self._lib = lib()
         d = ConvDesc(_dedisp(centre_frequency, bandwidth, dm, nchan, dc_centred), nchan, npol, nfft,
                      nfilt_pos, nfilt_neg, chirp_mode, form, 0)
         h = P()
@@ -206,7 +208,7 @@ class Convolution:
 
     def close(self):
         if self.h:
-            lib().dspx_conv_destroy(self.h)
+            self._lib.dspx_conv_destroy(self.h)
             self.h = None
 
     __del__ = close
@@ -229,6 +231,7 @@ class Fold:
     def __init__(self, ctx, ndat, pps, nbin, chunks):
         self.ctx = ctx
         self.h = None
+        self._lib = lib()
         arr = (FoldChunk * len(chunks))(*[FoldChunk(c[0], c[1], c[2], c[3], 0) for c in chunks])
         h = P()
         ctx.check(lib().dspx_fold_create(ctx.h, ndat, pps, nbin, arr, len(chunks), C.byref(h)))
@@ -236,7 +239,7 @@ class Fold:
 
     def close(self):
         if self.h:
-            lib().dspx_fold_destroy(self.h)
+            self._lib.dspx_fold_destroy(self.h)
             self.h = None
 
     __del__ = close
@@ -260,6 +263,7 @@ class LoadToFold:
                  scale=1.0, offset_binary=False, geom=None):
         self.ctx = ctx
         self.h = None
+        self._lib = lib()
         g = geom or geometry(centre_frequency, bandwidth, nchan, dm, nfft)
         self.geom = g
         d = LtfDesc(nbit, int(offset_binary), scale, nchan, npol, 0, ndat,
@@ -276,7 +280,7 @@ class LoadToFold:
 
     def close(self):
         if self.h:
-            lib().dspx_ltf_destroy(self.h)
+            self._lib.dspx_ltf_destroy(self.h)
             self.h = None
 
     __del__ = close

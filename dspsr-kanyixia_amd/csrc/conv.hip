@@ -7,6 +7,7 @@
 // chain (Signal/Pulsar/LoadToFold1.C:109-258).
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 
 #include "dspx_internal.hpp"
 #include "fold_impl.hpp"
@@ -48,6 +49,9 @@ struct ConvCore {
   double binwidth = 0, half_cw = 0;
   float snorm = 0;
   bool have_table = false;
+  // complex values per thread in the row transform (16 or 8); a tuning
+  // parameter (environment DSPX_ROW_E overrides the default)
+  int row_e = 16;
 
   void init(dspx_ctx* c, uint32_t nchan_, uint32_t npol_, uint32_t N_, uint32_t pos, uint32_t neg,
             uint32_t mode, const dspx_dedisp_desc& d)
@@ -82,6 +86,8 @@ struct ConvCore {
     tw2.reserve(t.size() * sizeof(float2));
     DSPX_HIP(hipMemcpy(tw2.p, t.data(), t.size() * sizeof(float2), hipMemcpyHostToDevice));
     snorm = float(1.0 / std::sqrt(double(N)));  // Convolution.C:401-414
+    if (const char* e = std::getenv("DSPX_ROW_E")) row_e = std::atoi(e) == 8 ? 8 : 16;
+    if (N2 < 256 * 2 && row_e == 8) row_e = 16;
     if (mode != DSPX_CHIRP_TABLE) {
       DSPX_REQUIRE(d.nchan == nchan, DSPX_E_PARAM, "dedisp.nchan=%u != nchan=%u", d.nchan, nchan);
       DSPX_REQUIRE(d.bandwidth != 0.0 && d.centre_frequency > 0.0, DSPX_E_PARAM, "invalid band");
@@ -182,10 +188,11 @@ struct ConvCore {
     uint64_t nb = rows * npg;
     DSPX_REQUIRE(nb < (1ull << 31), DSPX_E_UNSUPPORTED, "grid too large");
     const float2* twr = twiddles(ctx, N2);
-    switch (N2) {
-#define CASE_B(n) \
-  case n: k_stageB<n><<<unsigned(nb), n / 8, 0, ctx->stream>>>(s, c, twr, ppg); break;
-      CASE_B(256) CASE_B(512) CASE_B(1024) CASE_B(2048) CASE_B(4096)
+#define CASE_B(n, e) \
+  case n * 100 + e: k_stageB<n, e><<<unsigned(nb), n / e, 0, ctx->stream>>>(s, c, twr, ppg); break;
+    switch (N2 * 100 + row_e) {
+      CASE_B(256, 16) CASE_B(512, 16) CASE_B(1024, 16) CASE_B(2048, 16) CASE_B(4096, 16)
+      CASE_B(512, 8) CASE_B(1024, 8) CASE_B(2048, 8) CASE_B(4096, 8)
 #undef CASE_B
       default: fail(DSPX_E_UNSUPPORTED, "N2=%u", N2);
     }

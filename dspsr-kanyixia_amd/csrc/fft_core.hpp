@@ -217,7 +217,6 @@ DSPX_PLAN(512, 16, 3, 2, 16, 16, 0)
 DSPX_PLAN(1024, 16, 3, 4, 16, 16, 0)
 DSPX_PLAN(2048, 16, 3, 8, 16, 16, 0)
 DSPX_PLAN(4096, 16, 3, 16, 16, 16, 0)
-DSPX_PLAN(256, 8, 3, 4, 8, 8, 0)
 DSPX_PLAN(512, 8, 3, 8, 8, 8, 0)
 DSPX_PLAN(1024, 8, 4, 2, 8, 8, 8)
 DSPX_PLAN(2048, 8, 4, 4, 8, 8, 8)

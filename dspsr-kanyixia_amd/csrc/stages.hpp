@@ -230,15 +230,14 @@ __device__ __forceinline__ float2 chirp_onchip(const ChirpArgs& c, uint32_t chan
   return make_float2(c.snorm * cs, c.snorm * sn);
 }
 
-// grid.x = N1 * npg * nchan, block = N2/16.  A workgroup owns row k1 of
+// grid.x = N1 * npg * nchan, block = N2/E.  A workgroup owns row k1 of
 // channel chan for the parts [pg*ppg, min(npart, (pg+1)*ppg)): the chirp of
 // the row is generated once and applied to every part and polarisation.
-template <int N2>
-__global__ __launch_bounds__(N2 / 8) void k_stageB(ConvShape s, ChirpArgs c, const float2* __restrict__ twr,
+template <int N2, int E>
+__global__ __launch_bounds__(N2 / E) void k_stageB(ConvShape s, ChirpArgs c, const float2* __restrict__ twr,
                                                    uint32_t ppg)
 {
-  constexpr int NT = N2 / 8;
-  constexpr int E = 8;
+  constexpr int NT = N2 / E;
   using LAY = RowLayout<N2, NT>;
   __shared__ float2 lds[N2 + N2 / 16];
   constexpr int RF = FirstRadix<N2, E, -1>::v;   // forward first pass
