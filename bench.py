@@ -6,6 +6,13 @@ bins x 8 sub-integrations), raw int8 input resident in HBM before timing.
 Prints one JSON line (see the keys below).
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload multichan|single]
+  python bench.py --copy-probe MIB   # a plain device copy of MIB MiB, for
+                                     # calibrating the HBM byte counters
+
+roofline.traffic is the HBM bytes per launch of the dominant kernel read by
+FETCH_SIZE / WRITE_SIZE counter runs (scaled by the same counters' reading of
+a copy kernel of known size); it is taken from profiles/traffic_<workload>.json
+when that file exists and is null otherwise.
 """
 import argparse
 import importlib
@@ -109,6 +116,31 @@ def cpu_baseline(w, geom, raw_np, budget_s=20.0):
                                                                      t_total)}
 
 
+def load_traffic(workload, kernel):
+    """HBM bytes per launch of ``kernel`` from the committed counter summary."""
+    path = os.path.join(ROOT, "profiles", "traffic_%s.json" % workload)
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        t = json.load(f)
+    k = t.get("kernels", {}).get(kernel)
+    return None if k is None else int(k["hbm_bytes_per_launch"])
+
+
+def copy_probe(mib, reps=5):
+    """A device-to-device copy of ``mib`` MiB: reads and writes exactly that
+    many bytes, the yardstick for the FETCH_SIZE / WRITE_SIZE counters."""
+    import torch
+    n = mib * (1 << 20) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    for _ in range(reps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes": n * 4, "reps": reps}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -118,12 +150,16 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this path has no multi-GPU form here "
               "(channels and beams shard without exchange; see DESIGN.md)", file=sys.stderr)
         sys.exit(2)
 
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
+        return
     import numpy as np
     import torch
     dspx = importlib.import_module("dspsr-kanyixia_amd")
@@ -163,7 +199,7 @@ def main():
     total_alg = sum(abytes.values())
     r = geom["nfft"] / geom["step"]
     roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom,
+            "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": load_traffic(args.workload, dom), "kernel": dom,
             "kernel_ms": round(st_f[dom], 4), "algorithmic_bytes_per_launch": int(abytes[dom]),
             "path_achieved": round(total_alg / t_f / 1e9, 1),
             "path_frac": round(total_alg / t_f / 1e9 / HBM_PEAK_GBS, 4),

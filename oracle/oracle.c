@@ -11,7 +11,7 @@
  * floating-point operation is the separate IEEE operation the reference
  * source writes (no fused multiply-add contraction).
  *
- * Pinning (what checks each function; see tests/test_oracle_*.py):
+ * Pinning (what checks each function; see tests/test_oracle.py):
  *   ox_optimal_fft_length   -- against the reference's own optimize_fft.c,
  *                              compiled from its sources into oracle/_ref.
  *   ox_detect_* (Coherence/Stokes) -- against the reference's cross_detect.c /
