@@ -35,20 +35,65 @@ __global__ void k_unpack(LD ld, uint64_t ndat, float* __restrict__ out, uint64_t
   if (npol == 2) *reinterpret_cast<float2*>(o + pol_stride) = b;
 }
 
+// Multichannel input: a 32-channel x 64-sample tile is read with consecutive
+// threads on consecutive channels (the TFP word order) and written with
+// consecutive threads on consecutive samples (the FPT order), through LDS.
+template <class LD>
+__global__ __launch_bounds__(256) void k_unpack_tile(LD ld, uint32_t nchan, uint64_t ndat, float* __restrict__ out,
+                                                     uint64_t chan_stride, uint64_t pol_stride, uint32_t npol)
+{
+  __shared__ typename LD::Word tile[32][65];
+  const uint32_t c0 = blockIdx.y * 32;
+  const uint64_t t0 = uint64_t(blockIdx.x) * 64;
+  const int tid = threadIdx.x;
+  const uint32_t cl = c0 + (tid & 31);
+#pragma unroll
+  for (int i = 0; i < 8; i++) {
+    int tt = (tid >> 5) + 8 * i;
+    if (cl < nchan && t0 + tt < ndat) tile[tid & 31][tt] = ld.word(cl, t0 + tt);
+  }
+  __syncthreads();
+  const int tt = tid & 63;
+  const uint64_t t = t0 + tt;
+  if (t >= ndat) return;
+#pragma unroll
+  for (int i = 0; i < 8; i++) {
+    int cc = (tid >> 6) + 4 * i;
+    if (c0 + cc >= nchan) break;
+    typename LD::Word w = tile[cc][tt];
+    float* o = out + uint64_t(c0 + cc) * chan_stride + 2 * t;
+    *reinterpret_cast<float2*>(o) = ld.pol(w, 0);
+    if (npol == 2) *reinterpret_cast<float2*>(o + pol_stride) = ld.pol(w, 1);
+  }
+}
+
+template <class LD>
+static void launch_unpack(dspx_ctx* ctx, const LD& ld, uint32_t nchan, uint32_t npol, uint64_t ndat, float* out,
+                          uint64_t cs, uint64_t ps)
+{
+  if (nchan >= 8) {
+    dim3 grid(unsigned((ndat + 63) / 64), (nchan + 31) / 32);
+    k_unpack_tile<<<grid, 256, 0, ctx->stream>>>(ld, nchan, ndat, out, cs, ps, npol);
+  } else {
+    dim3 grid(unsigned((ndat + 255) / 256), nchan);
+    k_unpack<<<grid, 256, 0, ctx->stream>>>(ld, ndat, out, cs, ps, npol);
+  }
+}
+
 void unpack(dspx_ctx* ctx, const void* raw, uint32_t nbit, uint32_t offset_binary, uint32_t nchan,
             uint32_t npol, uint64_t ndat, float scale, float* out, uint64_t cs, uint64_t ps)
 {
   DSPX_REQUIRE(npol == 1 || npol == 2, DSPX_E_PARAM, "npol=%u", npol);
   DSPX_REQUIRE(nchan >= 1 && nchan <= 65535, DSPX_E_PARAM, "nchan=%u", nchan);
   if (!ndat) return;
-  dim3 grid(unsigned((ndat + 255) / 256), nchan);
+  DSPX_REQUIRE((ndat + 63) / 64 < (1ull << 31), DSPX_E_UNSUPPORTED, "ndat=%llu too large", (unsigned long long) ndat);
   if (nbit == 8) {
     LoadI8 ld{static_cast<const uint8_t*>(raw), uint64_t(nchan) * npol * 2, uint64_t(npol) * 2, npol, offset_binary,
               scale};
-    k_unpack<<<grid, 256, 0, ctx->stream>>>(ld, ndat, out, cs, ps, npol);
+    launch_unpack(ctx, ld, nchan, npol, ndat, out, cs, ps);
   } else if (nbit == 16) {
     LoadI16 ld{static_cast<const int16_t*>(raw), uint64_t(nchan) * npol * 2, uint64_t(npol) * 2, npol, offset_binary};
-    k_unpack<<<grid, 256, 0, ctx->stream>>>(ld, ndat, out, cs, ps, npol);
+    launch_unpack(ctx, ld, nchan, npol, ndat, out, cs, ps);
   } else {
     fail(DSPX_E_PARAM, "nbit=%u (8 or 16 supported)", nbit);
   }
@@ -117,14 +162,20 @@ int dspx_ctx_create(int device, void* stream, dspx_ctx** out)
         DSPX_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
         c->own_stream = true;
       }
-      // exp(-2 pi i m / L) for L = 2, 4, ..., 8192 at offset L
-      const int total = 2 * 8192;
+      // exp(-2 pi i m / L) for L = 2, 4, ..., 8192 at offset L, then the
+      // power records of each span (layout in fft_core.hpp), all in FP64
+      // rounded once to float
+      const int total = kTwTotal;
       std::vector<float2> h(total, make_float2(0.f, 0.f));
-      for (int L = 2; L <= 8192; L *= 2)
-        for (int m = 0; m < L; m++) {
-          double a = -2.0 * M_PI * double(m) / double(L);
-          h[L + m] = make_float2(float(std::cos(a)), float(std::sin(a)));
-        }
+      auto w = [](int64_t m, int64_t L) {
+        double a = -2.0 * M_PI * double(m % L) / double(L);
+        return make_float2(float(std::cos(a)), float(std::sin(a)));
+      };
+      for (int L = 2; L <= kTwMaxL; L *= 2)
+        for (int m = 0; m < L; m++) h[L + m] = w(m, L);
+      for (int S = 2; S <= kTwMaxL; S *= 2)
+        for (int k = 0; k < S / 2; k++)
+          for (int p = 0; p < 4; p++) h[kTwPowBase + 4 * (S / 2 + k) + p] = w(int64_t(k) << p, S);
       DSPX_HIP(hipMalloc(&c->d_tw, total * sizeof(float2)));
       DSPX_HIP(hipMemcpy(c->d_tw, h.data(), total * sizeof(float2), hipMemcpyHostToDevice));
       for (auto& e : c->ev) DSPX_HIP(hipEventCreate(&e));

@@ -150,6 +150,15 @@ template <int L, int NT> struct RowLayout {
 // Compute one Stockham pass on registers v (which hold, for each of the
 // thread's E/R butterflies q, the R inputs x[j + r*L/R]).  After the call
 // v[q*R + r] holds the output destined for index (j/Ns)*Ns*R + j%Ns + r*Ns.
+// Twiddle memory (float2 units from the base of the table the context
+// builds): exp(-2 pi i m / L) for L = 2..8192 at offset L, m < L; from
+// kTwPowBase, power records of 4 float2 {W_S^k, W_S^2k, W_S^4k, W_S^8k}
+// (W_S = exp(-2 pi i / S)) for spans S = 2..8192 at record S/2 + k, k < S/2.
+// Kernels are handed the pointer to the exp table of their length L.
+constexpr int kTwMaxL = 8192;
+constexpr int kTwPowBase = 2 * kTwMaxL;
+constexpr int kTwTotal = kTwPowBase + 4 * kTwMaxL;
+
 template <int L, int B, int NT, int R, int NS, int DIR>
 __device__ __forceinline__ void stockham_compute(float2* v, const float2* __restrict__ tw, int tid)
 {
@@ -161,11 +170,35 @@ __device__ __forceinline__ void stockham_compute(float2* v, const float2* __rest
     int j = g / B;
     if constexpr (NS > 1) {
       int k = j % NS;
+      float2 w[R];
+      if constexpr (R >= 8 && B == 1 && NS >= 8) {
+        // Row transforms (one butterfly index per lane, so lanes gather
+        // different twiddles): power record {W^k, W^2k, W^4k, W^8k} of span
+        // S = NS*R, two contiguous 16-byte loads, the other powers by at most
+        // three products each, instead of R-1 scattered table loads.  Column
+        // transforms (B > 1) share k across the lanes of a column group and
+        // keep the direct loads.
+        constexpr int S = NS * R;
+        const float4* rec = reinterpret_cast<const float4*>(tw - L + kTwPowBase) + 2 * (S / 2 + k);
+        float4 a = rec[0], b = rec[1];
+        w[1] = make_float2(a.x, a.y);
+        w[2] = make_float2(a.z, a.w);
+        w[4] = make_float2(b.x, b.y);
+        w[3] = cmul(w[2], w[1]);
+        w[5] = cmul(w[4], w[1]);
+        w[6] = cmul(w[4], w[2]);
+        w[7] = cmul(w[4], w[3]);
+        if constexpr (R == 16) {
+          w[8] = make_float2(b.z, b.w);
 #pragma unroll
-      for (int r = 1; r < R; r++) {
-        float2 w = tw[r * k * (L / (NS * R))];
-        v[q * R + r] = DIR < 0 ? cmul(v[q * R + r], w) : cmulc(v[q * R + r], w);
+          for (int r = 9; r < 16; r++) w[r] = cmul(w[8], w[r - 8]);
+        }
+      } else {
+#pragma unroll
+        for (int r = 1; r < R; r++) w[r] = tw[r * k * (L / (NS * R))];
       }
+#pragma unroll
+      for (int r = 1; r < R; r++) v[q * R + r] = DIR < 0 ? cmul(v[q * R + r], w[r]) : cmulc(v[q * R + r], w[r]);
     }
     dft<R, DIR>(v + q * R);
   }
@@ -276,6 +309,21 @@ template <class T> __device__ __forceinline__ const T* opaque(const T* p)
 {
   asm volatile("" : "+s"(p));
   return p;
+}
+
+// Returns x unchanged, opaquely (a vector-register value): arithmetic on the
+// result cannot be hoisted out of the loop that calls this.
+__device__ __forceinline__ float2 opaque_v(float2 x)
+{
+  asm volatile("" : "+v"(x.x), "+v"(x.y));
+  return x;
+}
+
+// A workgroup-uniform value moved to scalar registers.
+__device__ __forceinline__ float2 uniform(float2 x)
+{
+  return make_float2(__builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, x.x))),
+                     __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, x.y))));
 }
 
 // Radix of the first / last pass of each direction.

@@ -256,65 +256,74 @@ __global__ __launch_bounds__(N2 / E) void k_stageB(ConvShape s, ChirpArgs c, con
   const uint32_t N = s.N1 * N2;
   const uint64_t plane = uint64_t(s.N1) * N2;
 
-  // chirp for the bins this thread holds after the forward transform
-  // (k2 = tid + r * N2/RI, one butterfly per thread) and the inter-stage
-  // twiddles W_N^{n2*k1} of the elements it loads and stores (the forward
-  // plan's first pass and the inverse plan's last pass use the same radix,
-  // hence the same elements n2 = j + r*N2/RF).  Each is evaluated one value
-  // at a time into the LDS row buffer (bounding the registers the FP64 chirp
-  // needs) and then gathered into registers.
+  // Chirp: the bins this thread holds after the forward transform
+  // (k2 = tid + r * N2/RI, one butterfly per thread), evaluated once per
+  // workgroup into LDS and read back per row.
+  //
+  // Inter-stage twiddles W_N^{n2*k1} of the elements a thread loads and
+  // stores (n2 = j + r*N2/RF; the forward plan's first pass and the inverse
+  // plan's last pass have the same radix) factor as
+  //   W_N^{j*k1} * W_N^{r*(N2/RF)*k1}:
+  // a per-thread base times RF workgroup-uniform steps held in scalar
+  // registers, recomposed per row (4 multiply-adds per element) so that the
+  // registers go to prefetching the next row instead.
   static_assert(E / RI == 1, "row plan must end with a pass of radix E");
   static_assert(RF == RL, "first forward and last inverse radix must agree");
+  __shared__ float2 hlds[N2];
 #pragma unroll 1
   for (int r = 0; r < RI; r++) {
     uint32_t k2 = tid + r * (N2 / RI);
-    float2 hv = c.mode == DSPX_CHIRP_TABLE ? c.table[(uint64_t(chan) * s.N1 + k1) * N2 + k2]
-                                           : chirp_onchip(c, chan, k1 + s.N1 * k2, N);
-    lds[LAY::idx(k2, 0)] = hv;
+    hlds[k2] = c.mode == DSPX_CHIRP_TABLE ? c.table[(uint64_t(chan) * s.N1 + k1) * N2 + k2]
+                                          : chirp_onchip(c, chan, k1 + s.N1 * k2, N);
   }
-  __syncthreads();
-  float2 h[RI];
+  constexpr int Q = E / RF;
+  float2 base[Q];
 #pragma unroll
-  for (int r = 0; r < RI; r++) h[r] = lds[LAY::idx(tid + r * (N2 / RI), 0)];
-  __syncthreads();
-#pragma unroll 1
-  for (int e = 0; e < E; e++) {
-    int q = e / RF, r = e % RF;
-    uint32_t n2 = uint32_t(tid + q * NT + r * (N2 / RF));
-    lds[LAY::idx(n2, 0)] = tw_N(s.tw, n2 * k1);
-  }
-  __syncthreads();
-  float2 w[E];
+  for (int q = 0; q < Q; q++) base[q] = tw_N(s.tw, uint32_t(tid + q * NT) * k1);
+  float2 step[RF];
 #pragma unroll
-  for (int q = 0; q < E / RF; q++) {
-    int j = tid + q * NT;
-#pragma unroll
-    for (int r = 0; r < RF; r++) w[q * RF + r] = lds[LAY::idx(j + r * (N2 / RF), 0)];
-  }
+  for (int r = 0; r < RF; r++) step[r] = uniform(tw_N(s.tw, uint32_t(r * (N2 / RF)) * k1));
   __syncthreads();
 
-  for (uint32_t p = p_begin; p < p_end; p++) {
-    float2* rowbase = s.work + (uint64_t(chan) * s.npart + p) * s.npol * plane + uint64_t(k1) * N2;
-    for (uint32_t pol = 0; pol < s.npol; pol++) {
-      float2* row = rowbase + pol * plane;
-      const float2* tw = opaque(twr);
-      float2 v[E];
+  const uint32_t nrow = (p_end - p_begin) * s.npol;
+  auto row_ptr = [&](uint32_t i) {
+    uint32_t p = p_begin + i / s.npol, pol = i % s.npol;
+    return s.work + ((uint64_t(chan) * s.npart + p) * s.npol + pol) * plane + uint64_t(k1) * N2;
+  };
+  float2 pf[E];
+  {
+    const float2* row = row_ptr(0);
 #pragma unroll
-      for (int q = 0; q < E / RF; q++) {
-        int j = tid + q * NT;
+    for (int q = 0; q < Q; q++)
 #pragma unroll
-        for (int r = 0; r < RF; r++) v[q * RF + r] = cmul(row[j + r * (N2 / RF)], w[q * RF + r]);
-      }
-      fft_regs<LAY, N2, 1, NT, -1>(lds, v, tw, tid);
+      for (int r = 0; r < RF; r++) pf[q * RF + r] = row[tid + q * NT + r * (N2 / RF)];
+  }
+  for (uint32_t i = 0; i < nrow; i++) {
+    float2* row = row_ptr(i);
+    const float2* tw = opaque(twr);
+    float2 v[E];
 #pragma unroll
-      for (int r = 0; r < RI; r++) v[r] = cmul(v[r], h[r]);
-      fft_regs<LAY, N2, 1, NT, +1>(lds, v, opaque(tw), tid);
+    for (int q = 0; q < Q; q++) {
+      float2 b = opaque_v(base[q]);
 #pragma unroll
-      for (int q = 0; q < E / RL; q++) {
-        int j = tid + q * NT;
+      for (int r = 0; r < RF; r++) v[q * RF + r] = cmul(pf[q * RF + r], cmul(b, step[r]));
+    }
+    if (i + 1 < nrow) {
+      const float2* nxt = row_ptr(i + 1);
 #pragma unroll
-        for (int r = 0; r < RL; r++) row[j + r * (N2 / RL)] = cmulc(v[q * RL + r], w[q * RL + r]);
-      }
+      for (int q = 0; q < Q; q++)
+#pragma unroll
+        for (int r = 0; r < RF; r++) pf[q * RF + r] = nxt[tid + q * NT + r * (N2 / RF)];
+    }
+    fft_regs<LAY, N2, 1, NT, -1>(lds, v, tw, tid);
+#pragma unroll
+    for (int r = 0; r < RI; r++) v[r] = cmul(v[r], hlds[tid + r * (N2 / RI)]);
+    fft_regs<LAY, N2, 1, NT, +1>(lds, v, opaque(tw), tid);
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
+      float2 b = opaque_v(base[q]);
+#pragma unroll
+      for (int r = 0; r < RL; r++) row[tid + q * NT + r * (N2 / RL)] = cmulc(v[q * RL + r], cmul(b, step[r]));
     }
   }
 }

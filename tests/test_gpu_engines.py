@@ -12,7 +12,8 @@ pytestmark = pytest.mark.gpu
 
 
 @pytest.mark.parametrize("nbit,nchan,npol,ob", [(8, 1, 2, False), (8, 5, 2, True), (16, 3, 2, False),
-                                                (16, 2, 1, True), (8, 4, 1, False)])
+                                                (16, 2, 1, True), (8, 4, 1, False), (8, 40, 2, False),
+                                                (16, 33, 2, True), (8, 9, 1, True), (16, 64, 1, False)])
 def test_unpack(ctx, oracle, nbit, nchan, npol, ob):
     import torch
     rng = np.random.default_rng(nbit + nchan)
