@@ -530,10 +530,9 @@ __global__ __launch_bounds__(kColThreads) void k_stageC(ConvShape s, FilterbankA
   const uint64_t plane = uint64_t(s.N1) * s.N2;
   const float2* wbase = s.work + (uint64_t(chan) * s.npart + p) * s.npol * plane;
 
-  for (uint32_t cb = cb_begin; cb < cb_end; cb++) {
+  // the tile's first-pass inputs, loaded one tile ahead of the transform
+  auto load_tile = [&](uint32_t cb, float2* xa, float2* xb) {
     const uint32_t c0 = cb * C;
-    const float2* tw = opaque(twc);
-    float2 va[E], vb[E];
 #pragma unroll
     for (int q = 0; q < E / RI; q++) {
       int gg = tid + q * kColThreads;
@@ -542,8 +541,39 @@ __global__ __launch_bounds__(kColThreads) void k_stageC(ConvShape s, FilterbankA
       for (int r = 0; r < RI; r++) {
         uint32_t k1 = j + r * (N1 / RI);
         uint64_t off = uint64_t(k1) * s.N2 + c0 + b;
-        va[q * RI + r] = wbase[off];
-        vb[q * RI + r] = s.npol == 2 ? wbase[plane + off] : make_float2(0.f, 0.f);
+        xa[q * RI + r] = wbase[off];
+        xb[q * RI + r] = s.npol == 2 ? wbase[plane + off] : make_float2(0.f, 0.f);
+      }
+    }
+  };
+  // (prefetch only where its 64 registers do not cost occupancy: long
+  // columns already need more registers for the column transform)
+  constexpr bool PF = MODE == kStageCFold && N1 <= 64;
+  float2 pa[E], pb[E];
+  if (PF && cb_begin < cb_end) load_tile(cb_begin, pa, pb);
+  for (uint32_t cb = cb_begin; cb < cb_end; cb++) {
+    const uint32_t c0 = cb * C;
+    const float2* tw = opaque(twc);
+    float2 va[E], vb[E];
+    if constexpr (PF) {
+#pragma unroll
+      for (int e = 0; e < E; e++) {
+        va[e] = pa[e];
+        vb[e] = pb[e];
+      }
+      if (cb + 1 < cb_end) load_tile(cb + 1, pa, pb);
+    } else {
+#pragma unroll
+      for (int q = 0; q < E / RI; q++) {
+        int gg = tid + q * kColThreads;
+        int b = gg % C, j = gg / C;
+#pragma unroll
+        for (int r = 0; r < RI; r++) {
+          uint32_t k1 = j + r * (N1 / RI);
+          uint64_t off = uint64_t(k1) * s.N2 + c0 + b;
+          va[q * RI + r] = wbase[off];
+          vb[q * RI + r] = s.npol == 2 ? wbase[plane + off] : make_float2(0.f, 0.f);
+        }
       }
     }
     if constexpr (MODE == kStageCFilterbank) {
