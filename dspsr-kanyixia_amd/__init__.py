@@ -7,8 +7,10 @@ tensors on a HIP device are accepted wherever a pointer is expected.
 The compute path is the HIP library only: if libdspx.so is missing every call
 raises, there is no CPU fallback.
 """
+import atexit
 import ctypes as C
 import os
+import weakref
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libdspx.so")
@@ -58,6 +60,13 @@ class LtfDesc(C.Structure):
                 ("form", C.c_uint32)]
 
 
+class SkDesc(C.Structure):
+    _fields_ = [("nchan", C.c_uint32), ("npol", C.c_uint32), ("M", C.c_uint32), ("Nd", C.c_uint32),
+                ("omit_outliers", C.c_uint32), ("reserved", C.c_uint32)]
+
+
+SK_SKFB, SK_FSCR, SK_TSCR = 1, 2, 4
+
 P = C.c_void_p
 U32, U64, I32, D = C.c_uint32, C.c_uint64, C.c_int, C.c_double
 
@@ -82,6 +91,13 @@ _SIGS = {
     "dspx_fold_nsegments_host": (I32, [D, D, U64, P]),
     "dspx_fold_hits": (I32, [P, P]),
     "dspx_fold_execute": (I32, [P, P, U64, U64, U32, U32, U32, P]),
+    "dspx_sk_create": (I32, [P, C.POINTER(SkDesc), C.POINTER(P)]),
+    "dspx_sk_destroy": (None, [P]),
+    "dspx_sk_set_limits": (I32, [P, P, P, P, D]),
+    "dspx_sk_compute": (I32, [P, P, U64, U64, U64, I32, P, P]),
+    "dspx_sk_compute_raw": (I32, [P, P, U32, U32, C.c_float, U64, U64, P, P]),
+    "dspx_sk_detect": (I32, [P, P, P, U64, U32, P]),
+    "dspx_sk_mask": (I32, [P, P, P, U64, U64, P, U64, U64, U64]),
     "dspx_ltf_create": (I32, [P, C.POINTER(LtfDesc), C.POINTER(P)]),
     "dspx_ltf_destroy": (None, [P]),
     "dspx_ltf_geometry": (I32, [P, C.POINTER(U64), C.POINTER(U64)]),
@@ -91,6 +107,32 @@ _SIGS = {
     "dspx_ltf_set_timing": (I32, [P, I32]),
     "dspx_ltf_stage_times": (I32, [P, P, P, I32, C.POINTER(I32)]),
 }
+
+
+# Live handles, closed at interpreter exit (engines before their contexts),
+# while the HIP runtime and the streams a context was given still exist:
+# finalizers that run during interpreter teardown may come after the runtime
+# is gone.
+_live_engines = weakref.WeakSet()
+_live_contexts = weakref.WeakSet()
+_atexit_registered = False
+
+
+def _close_all():
+    for o in list(_live_engines):
+        o.close()
+    for o in list(_live_contexts):
+        o.close()
+
+
+def _register(obj, is_context=False):
+    global _atexit_registered
+    if not _atexit_registered:
+        # registered when the first handle is made (after torch has set up
+        # the runtime), so it runs before torch's own exit handlers
+        atexit.register(_close_all)
+        _atexit_registered = True
+    (_live_contexts if is_context else _live_engines).add(obj)
 
 
 def lib():
@@ -157,6 +199,7 @@ class Context:
         h = P()
         _check(lib().dspx_ctx_create(device, stream, C.byref(h)))
         self.h = h
+        _register(self, True)
 
     def close(self):
         if self.h:
@@ -204,6 +247,7 @@ class Convolution:
         h = P()
         ctx.check(lib().dspx_conv_create(ctx.h, C.byref(d), C.byref(h)))
         self.h = h
+        _register(self, False)
         self.step = nfft - nfilt_pos - nfilt_neg
 
     def close(self):
@@ -236,6 +280,7 @@ class Fold:
         h = P()
         ctx.check(lib().dspx_fold_create(ctx.h, ndat, pps, nbin, arr, len(chunks), C.byref(h)))
         self.h = h
+        _register(self, False)
 
     def close(self):
         if self.h:
@@ -255,6 +300,49 @@ class Fold:
                                                ndim, ptr(prof)))
 
 
+class SpectralKurtosis:
+    """dspx_sk: one resolution of SpectralKurtosis (compute / detect / mask).
+    Tensors are device buffers: sums [npart][nchan][npol][2] float32, counts
+    [npart][nchan] int32 (read as uint32), mask [npart][nchan] uint8."""
+
+    def __init__(self, ctx, nchan, npol, M, Nd=1, omit_outliers=False):
+        self.ctx = ctx
+        self.h = None
+        self._lib = lib()
+        self.nchan, self.npol, self.M = nchan, npol, M
+        h = P()
+        ctx.check(lib().dspx_sk_create(ctx.h, C.byref(SkDesc(nchan, npol, M, Nd, int(omit_outliers), 0)), C.byref(h)))
+        self.h = h
+        _register(self, False)
+
+    def close(self):
+        if self.h:
+            self._lib.dspx_sk_destroy(self.h)
+            self.h = None
+
+    __del__ = close
+
+    def set_limits(self, lower, upper, symmetric, pcsd):
+        import numpy as np
+        lo, up, sy = (np.ascontiguousarray(a, np.float32) for a in (lower, upper, symmetric))
+        assert lo.size == up.size == sy.size == self.M + 1
+        self.ctx.check(lib().dspx_sk_set_limits(self.h, lo.ctypes.data, up.ctypes.data, sy.ctypes.data, pcsd))
+
+    def compute(self, x, chan_stride, pol_stride, npart, sums, counts, zeroed=False):
+        self.ctx.check(lib().dspx_sk_compute(self.h, ptr(x), chan_stride, pol_stride, npart, int(zeroed),
+                                             ptr(sums), ptr(counts)))
+
+    def compute_raw(self, raw, nbit, t0, npart, sums, counts, scale=1.0, offset_binary=False):
+        self.ctx.check(lib().dspx_sk_compute_raw(self.h, ptr(raw), nbit, int(offset_binary), scale, t0, npart,
+                                                 ptr(sums), ptr(counts)))
+
+    def detect(self, sums, counts, npart, flags, mask):
+        self.ctx.check(lib().dspx_sk_detect(self.h, ptr(sums), ptr(counts), npart, flags, ptr(mask)))
+
+    def mask(self, mask, x, ics, ips, out, ocs, ops, npart):
+        self.ctx.check(lib().dspx_sk_mask(self.h, ptr(mask), ptr(x), ics, ips, ptr(out), ocs, ops, npart))
+
+
 class LoadToFold:
     """Unpack -> Convolution -> Detection -> Fold on one block, fused or plain."""
 
@@ -272,6 +360,7 @@ class LoadToFold:
         h = P()
         ctx.check(lib().dspx_ltf_create(ctx.h, C.byref(d), C.byref(h)))
         self.h = h
+        _register(self, False)
         npart, nout = U64(), U64()
         lib().dspx_ltf_geometry(h, C.byref(npart), C.byref(nout))
         self.npart, self.ndat_out = npart.value, nout.value

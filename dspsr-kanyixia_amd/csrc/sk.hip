@@ -1,0 +1,420 @@
+// sk.hip -- spectral kurtosis engine: block sums, the three detectors of
+// SpectralKurtosis::detect and the mask, for one resolution (noverlap 1,
+// every channel included).
+//
+// Reference: SpectralKurtosis.C:674-798 (compute), 830-909 (detect),
+// 1105-1222 (detect_skfb), 1312-1462 (detect_fscr), 1014-1103 (detect_tscr),
+// 1465-1566 (mask).  Arithmetic is kept in the reference's types and order
+// where a decision depends on it: float sq = re*re + im*im and sq*sq, both
+// summed in double; the float estimator of detect_skfb; double averages of
+// detect_fscr; sequential double sums of detect_tscr.
+#include "dspx_internal.hpp"
+#include "stages.hpp"
+
+struct dspx_sk {
+  dspx_ctx* ctx = nullptr;
+  dspx_sk_desc d{};
+  dspx::DevBuf lim;  // lower[M+1], upper[M+1], symmetric[M+1]
+  double pcsd = 0;
+  bool have_limits = false;
+};
+
+namespace dspx {
+
+namespace {
+
+constexpr int kSkThreads = 256;
+
+// Fixed-order tree sum over the block (deterministic, like a sequential sum
+// to within double rounding; the float results match the reference's
+// sequential double sums except within 1e-16 of a float rounding boundary).
+__device__ __forceinline__ void block_sum(double* a, double* b, uint32_t* c, int tid)
+{
+  for (int s = kSkThreads / 2; s > 0; s >>= 1) {
+    __syncthreads();
+    if (tid < s) {
+      a[tid] += a[tid + s];
+      b[tid] += b[tid + s];
+      c[tid] += c[tid + s];
+    }
+  }
+  __syncthreads();
+}
+
+// grid (npart, nchan): S1 = sum sq, S2 = sum sq^2 over block ipart of each pol
+// (SpectralKurtosis.C:752-766), counts of pol 0 (:781-787).
+__global__ __launch_bounds__(kSkThreads) void k_sk_sums_fpt(const float* __restrict__ in, uint64_t cs, uint64_t ps,
+                                                           uint32_t nchan, uint32_t npol, uint32_t M, int zeroed,
+                                                           float* __restrict__ sums, uint32_t* __restrict__ counts)
+{
+#pragma clang fp contract(off)
+  __shared__ double r1[kSkThreads], r2[kSkThreads];
+  __shared__ uint32_t rc[kSkThreads];
+  const int tid = threadIdx.x;
+  const uint64_t part = blockIdx.x;
+  const uint32_t chan = blockIdx.y;
+  for (uint32_t pol = 0; pol < npol; pol++) {
+    const float2* x = reinterpret_cast<const float2*>(in + uint64_t(chan) * cs + uint64_t(pol) * ps) + part * M;
+    double s1 = 0, s2 = 0;
+    uint32_t c = 0;
+    for (uint32_t i = tid; i < M; i += kSkThreads) {
+      float2 v = x[i];
+      float sq = (v.x * v.x) + (v.y * v.y);
+      if (!zeroed || sq != 0) {
+        s1 += sq;
+        s2 += sq * sq;
+        c++;
+      }
+    }
+    r1[tid] = s1;
+    r2[tid] = s2;
+    rc[tid] = c;
+    block_sum(r1, r2, rc, tid);
+    if (tid == 0) {
+      float* o = sums + ((part * nchan + chan) * npol + pol) * 2;
+      o[0] = float(r1[0]);
+      o[1] = float(r2[0]);
+      if (pol == 0) counts[part * nchan + chan] = rc[0];
+    }
+  }
+}
+
+// Raw TFP input (the unpacked zero-DM samples): grid (npart, ceil(nchan/32));
+// lanes span 32 consecutive channels of a time sample (one TFP row segment),
+// 8 row groups stride the block; the row groups are summed in order.
+template <class LD>
+__global__ __launch_bounds__(kSkThreads) void k_sk_sums_raw(LD ld, uint64_t t0, uint32_t nchan, uint32_t M,
+                                                           float* __restrict__ sums, uint32_t* __restrict__ counts)
+{
+#pragma clang fp contract(off)
+  constexpr int CH = 32, RG = kSkThreads / CH;
+  __shared__ double r1[RG][2][CH], r2[RG][2][CH];
+  const int tid = threadIdx.x;
+  const int cl = tid % CH, rg = tid / CH;
+  const uint64_t part = blockIdx.x;
+  const uint32_t chan = blockIdx.y * CH + cl;
+  const uint32_t npol = ld.npol;
+  double s1[2] = {0, 0}, s2[2] = {0, 0};
+  if (chan < nchan) {
+    const uint64_t tb = t0 + part * M;
+    for (uint32_t i = rg; i < M; i += RG) {
+      typename LD::Word w = ld.word(chan, tb + i);
+#pragma unroll
+      for (uint32_t pol = 0; pol < 2; pol++) {
+        if (pol < npol) {
+          float2 v = ld.pol(w, pol);
+          float sq = (v.x * v.x) + (v.y * v.y);
+          s1[pol] += sq;
+          s2[pol] += sq * sq;
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int pol = 0; pol < 2; pol++) {
+    r1[rg][pol][cl] = s1[pol];
+    r2[rg][pol][cl] = s2[pol];
+  }
+  __syncthreads();
+  if (rg == 0 && chan < nchan) {
+    for (uint32_t pol = 0; pol < npol; pol++) {
+      double a = 0, b = 0;
+      for (int g = 0; g < RG; g++) {
+        a += r1[g][pol][cl];
+        b += r2[g][pol][cl];
+      }
+      float* o = sums + ((part * nchan + chan) * npol + pol) * 2;
+      o[0] = float(a);
+      o[1] = float(b);
+    }
+    counts[part * nchan + chan] = M;  // raw samples are never zeroed data
+  }
+}
+
+__global__ void k_fill_u8(uint8_t* p, uint64_t n, uint8_t v)
+{
+  uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+// detect_skfb (SpectralKurtosis.C:1148-1210): one thread per (block, channel).
+// Counts above M (never produced by the compute kernels) are not tested.
+__global__ void k_sk_skfb(float* __restrict__ sums, uint32_t* __restrict__ counts, uint64_t n, uint32_t npol,
+                          uint32_t M, uint32_t Nd, int omit, const float* __restrict__ lower, const float* __restrict__ upper,
+                          uint8_t* __restrict__ mask)
+{
+#pragma clang fp contract(off)
+  uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const unsigned Mprime = counts[i];
+  if (Mprime == 0 || Mprime > M) return;
+  float* sd = sums + i * npol * 2;
+  bool zap = true;
+  if (Mprime > 2) {
+    const float M_fac = float(Mprime * Nd + 1) / (Mprime - 1);
+    zap = false;
+    for (uint32_t pol = 0; pol < npol; pol++) {
+      float S1 = sd[2 * pol], S2 = sd[2 * pol + 1];
+      float V = M_fac * (Mprime * (S2 / (S1 * S1)) - 1);
+      if (V > upper[Mprime] || V < lower[Mprime]) zap = true;
+    }
+  }
+  if (zap) {
+    if (omit) {
+      counts[i] = 0;
+      for (uint32_t pol = 0; pol < npol; pol++) sd[2 * pol] = sd[2 * pol + 1] = 0.f;
+    }
+    mask[i] = 1;
+  }
+}
+
+// detect_fscr (SpectralKurtosis.C:1369-1451): one workgroup per block; the
+// channel sums of each pol are reduced in a fixed order.
+__global__ __launch_bounds__(kSkThreads) void k_sk_fscr(const float* __restrict__ sums,
+                                                       const uint32_t* __restrict__ counts, uint32_t nchan,
+                                                       uint32_t npol, uint32_t M, uint32_t Nd, int omit,
+                                                       const float* __restrict__ symmetric,
+                                                       uint8_t* __restrict__ mask)
+{
+#pragma clang fp contract(off)
+  __shared__ double ra[kSkThreads], rm[kSkThreads], rq[kSkThreads], rn[kSkThreads];
+  __shared__ int zap;
+  const int tid = threadIdx.x;
+  const uint64_t part = blockIdx.x;
+  uint8_t* mrow = mask + part * nchan;
+  if (tid == 0) zap = 0;
+  for (uint32_t pol = 0; pol < npol; pol++) {
+    double sk_avg = 0, M_tot = 0, Msq_tot = 0, chan_count = 0;
+    for (uint32_t ch = tid; ch < nchan; ch += kSkThreads) {
+      if (!omit || mrow[ch] == 0) {
+        const float* sd = sums + ((part * nchan + ch) * npol + pol) * 2;
+        float S1 = sd[0], S2 = sd[1];
+        unsigned Mprime = counts[part * nchan + ch];
+        const float M_fac = float(Mprime * Nd + 1) / (Mprime - 1);
+        float V = M_fac * (Mprime * (S2 / (S1 * S1)) - 1);
+        sk_avg += V * Mprime;
+        M_tot += Mprime;
+        Msq_tot += Mprime * Mprime;
+        chan_count += 1;
+      }
+    }
+    ra[tid] = sk_avg;
+    rm[tid] = M_tot;
+    rq[tid] = Msq_tot;
+    rn[tid] = chan_count;
+    for (int s = kSkThreads / 2; s > 0; s >>= 1) {
+      __syncthreads();
+      if (tid < s) {
+        ra[tid] += ra[tid + s];
+        rm[tid] += rm[tid + s];
+        rq[tid] += rq[tid + s];
+        rn[tid] += rn[tid + s];
+      }
+    }
+    __syncthreads();
+    if (tid == 0 && rn[0] > 0) {
+      double avg = ra[0] / rm[0];
+      double M_mean = rm[0] / rn[0];
+      double Msq_mean = rq[0] / rn[0];
+      double Deff = Msq_mean / (M_mean * M_mean);
+      double eff = rn[0] / Deff;
+      float threshold = symmetric[min(unsigned(M_mean + 0.5), M)] / sqrt(eff);
+      float up = 1.0 + threshold;
+      float lo = 1.0 - threshold;
+      if (avg > up || avg < lo) zap = 1;
+    }
+    __syncthreads();
+  }
+  if (zap)
+    for (uint32_t ch = tid; ch < nchan; ch += kSkThreads) mrow[ch] = 1;
+}
+
+// detect_tscr (SpectralKurtosis.C:1033-1095): one thread per channel, blocks
+// summed in their order.  Totals above M use the closed form of
+// SKLimits.C:152-160 when >= 32768; totals in between are not tested.
+__global__ void k_sk_tscr(const float* __restrict__ sums, const uint32_t* __restrict__ counts, uint64_t npart,
+                          uint32_t nchan, uint32_t npol, uint32_t M, uint32_t Nd, const float* __restrict__ lower,
+                          const float* __restrict__ upper, double pcsd, uint8_t* __restrict__ mask)
+{
+#pragma clang fp contract(off)
+  uint32_t ch = blockIdx.x * blockDim.x + threadIdx.x;
+  if (ch >= nchan) return;
+  uint64_t count = 0;
+  for (uint64_t p = 0; p < npart; p++) count += counts[p * nchan + ch];
+  if (count == 0) return;
+  float lo, up;
+  if (count <= M) {
+    lo = lower[count];
+    up = upper[count];
+  } else if (count >= 32768) {
+    double factor = sqrt(4.0 / double(count)) * pcsd;
+    lo = float(1.0f - factor);
+    up = float(1.0f + factor);
+  } else {
+    return;
+  }
+  const float M_t = float(count);
+  const float M_fac = (M_t * Nd + 1) / (M_t - 1);
+  bool zap_chan = false;
+  for (uint32_t pol = 0; pol < npol; pol++) {
+    double S1 = 0.0, S2 = 0.0;
+    for (uint64_t p = 0; p < npart; p++) {
+      uint64_t idx = (p * nchan + ch) * npol + pol;
+      S1 += sums[idx * 2];
+      S2 += sums[idx * 2 + 1];
+    }
+    double SK = M_fac * (M_t * (S2 / (S1 * S1)) - 1);
+    if (SK > up || SK < lo) zap_chan = true;
+  }
+  if (zap_chan)
+    for (uint64_t p = 0; p < npart; p++) mask[p * nchan + ch] = 1;
+}
+
+// mask (SpectralKurtosis.C:1531-1554): grid (ceil(npart*M/256), nchan*npol).
+__global__ void k_sk_mask(const uint8_t* __restrict__ mask, const float* __restrict__ in, uint64_t ics, uint64_t ips,
+                          float* __restrict__ out, uint64_t ocs, uint64_t ops, uint32_t nchan, uint32_t npol,
+                          uint32_t M, uint64_t n)
+{
+  uint64_t t = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  uint32_t chan = blockIdx.y / npol, pol = blockIdx.y % npol;
+  float2 v = reinterpret_cast<const float2*>(in + chan * ics + pol * ips)[t];
+  if (mask[(t / M) * nchan + chan]) v = make_float2(0.f, 0.f);
+  reinterpret_cast<float2*>(out + chan * ocs + pol * ops)[t] = v;
+}
+
+}  // namespace
+
+}  // namespace dspx
+
+using namespace dspx;
+
+extern "C" {
+
+int dspx_sk_create(dspx_ctx* ctx, const dspx_sk_desc* desc, dspx_sk** out)
+{
+  return guard(ctx, [&] {
+    DSPX_REQUIRE(ctx && desc && out, DSPX_E_PARAM, "null argument");
+    const dspx_sk_desc& d = *desc;
+    DSPX_REQUIRE(d.nchan >= 1 && d.nchan <= 65535, DSPX_E_PARAM, "nchan=%u", d.nchan);
+    DSPX_REQUIRE(d.npol == 1 || d.npol == 2, DSPX_E_PARAM, "npol=%u", d.npol);
+    DSPX_REQUIRE(d.M >= 2, DSPX_E_PARAM, "M=%u < 2 (SKLimits::calc_limits)", d.M);
+    DSPX_REQUIRE(d.Nd >= 1, DSPX_E_PARAM, "Nd=%u", d.Nd);
+    auto* s = new dspx_sk;
+    s->ctx = ctx;
+    s->d = d;
+    *out = s;
+  });
+}
+
+void dspx_sk_destroy(dspx_sk* s)
+{
+  if (!s) return;
+  (void) hipSetDevice(s->ctx->device);
+  (void) hipStreamSynchronize(s->ctx->stream);
+  delete s;
+}
+
+int dspx_sk_set_limits(dspx_sk* s, const float* lower, const float* upper, const float* symmetric,
+                       double pc_std_devs)
+{
+  return guard(s ? s->ctx : nullptr, [&] {
+    DSPX_REQUIRE(s && lower && upper && symmetric, DSPX_E_PARAM, "null argument");
+    const size_t n = size_t(s->d.M) + 1;
+    std::vector<float> h(3 * n);
+    std::copy(lower, lower + n, h.begin());
+    std::copy(upper, upper + n, h.begin() + n);
+    std::copy(symmetric, symmetric + n, h.begin() + 2 * n);
+    s->lim.reserve(h.size() * sizeof(float));
+    DSPX_HIP(hipMemcpyAsync(s->lim.p, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice, s->ctx->stream));
+    DSPX_HIP(hipStreamSynchronize(s->ctx->stream));
+    s->pcsd = pc_std_devs;
+    s->have_limits = true;
+  });
+}
+
+int dspx_sk_compute(dspx_sk* s, const float* d_in, uint64_t cs, uint64_t ps, uint64_t npart, int zeroed_data,
+                    float* d_sums, uint32_t* d_counts)
+{
+  return guard(s ? s->ctx : nullptr, [&] {
+    DSPX_REQUIRE(s && d_in && d_sums && d_counts, DSPX_E_PARAM, "null argument");
+    DSPX_REQUIRE(npart < (1u << 31), DSPX_E_UNSUPPORTED, "npart=%llu", (unsigned long long) npart);
+    if (!npart) return;
+    dim3 grid(unsigned(npart), s->d.nchan);
+    k_sk_sums_fpt<<<grid, kSkThreads, 0, s->ctx->stream>>>(d_in, cs, ps, s->d.nchan, s->d.npol, s->d.M,
+                                                            zeroed_data, d_sums, d_counts);
+    DSPX_HIP(hipGetLastError());
+  });
+}
+
+int dspx_sk_compute_raw(dspx_sk* s, const void* d_raw, uint32_t nbit, uint32_t offset_binary, float scale,
+                        uint64_t t0, uint64_t npart, float* d_sums, uint32_t* d_counts)
+{
+  return guard(s ? s->ctx : nullptr, [&] {
+    DSPX_REQUIRE(s && d_raw && d_sums && d_counts, DSPX_E_PARAM, "null argument");
+    DSPX_REQUIRE(npart < (1u << 31), DSPX_E_UNSUPPORTED, "npart=%llu", (unsigned long long) npart);
+    if (!npart) return;
+    const uint32_t nchan = s->d.nchan, npol = s->d.npol;
+    dim3 grid(unsigned(npart), (nchan + 31) / 32);
+    if (nbit == 8) {
+      LoadI8 ld{static_cast<const uint8_t*>(d_raw), uint64_t(nchan) * npol * 2, uint64_t(npol) * 2, npol,
+                offset_binary, scale};
+      k_sk_sums_raw<<<grid, kSkThreads, 0, s->ctx->stream>>>(ld, t0, nchan, s->d.M, d_sums, d_counts);
+    } else if (nbit == 16) {
+      LoadI16 ld{static_cast<const int16_t*>(d_raw), uint64_t(nchan) * npol * 2, uint64_t(npol) * 2, npol,
+                 offset_binary};
+      k_sk_sums_raw<<<grid, kSkThreads, 0, s->ctx->stream>>>(ld, t0, nchan, s->d.M, d_sums, d_counts);
+    } else {
+      fail(DSPX_E_PARAM, "nbit=%u (8 or 16 supported)", nbit);
+    }
+    DSPX_HIP(hipGetLastError());
+  });
+}
+
+int dspx_sk_detect(dspx_sk* s, float* d_sums, uint32_t* d_counts, uint64_t npart, uint32_t flags, uint8_t* d_mask)
+{
+  return guard(s ? s->ctx : nullptr, [&] {
+    DSPX_REQUIRE(s && d_sums && d_counts && d_mask, DSPX_E_PARAM, "null argument");
+    DSPX_REQUIRE(s->have_limits, DSPX_E_STATE, "thresholds not set (dspx_sk_set_limits)");
+    DSPX_REQUIRE((flags & ~7u) == 0, DSPX_E_PARAM, "flags=%u", flags);
+    DSPX_REQUIRE(npart < (1u << 31), DSPX_E_UNSUPPORTED, "npart=%llu", (unsigned long long) npart);
+    const dspx_sk_desc& d = s->d;
+    hipStream_t st = s->ctx->stream;
+    const uint64_t n = npart * d.nchan;
+    if (!n) return;
+    const float* lower = s->lim.as<float>();
+    const float* upper = lower + d.M + 1;
+    const float* sym = upper + d.M + 1;
+    // reset_mask (a fill kernel rather than a memset node, so that the
+    // sequence can be captured into a graph)
+    k_fill_u8<<<unsigned((n + 255) / 256), 256, 0, st>>>(d_mask, n, 0);
+    if (flags & DSPX_SK_SKFB)
+      k_sk_skfb<<<unsigned((n + 255) / 256), 256, 0, st>>>(d_sums, d_counts, n, d.npol, d.M, d.Nd,
+                                                          d.omit_outliers, lower, upper, d_mask);
+    if (flags & DSPX_SK_FSCR)
+      k_sk_fscr<<<unsigned(npart), kSkThreads, 0, st>>>(d_sums, d_counts, d.nchan, d.npol, d.M, d.Nd,
+                                                        d.omit_outliers, sym, d_mask);
+    if (flags & DSPX_SK_TSCR)
+      k_sk_tscr<<<(d.nchan + 255) / 256, 256, 0, st>>>(d_sums, d_counts, npart, d.nchan, d.npol, d.M, d.Nd, lower,
+                                                        upper, s->pcsd, d_mask);
+    DSPX_HIP(hipGetLastError());
+  });
+}
+
+int dspx_sk_mask(dspx_sk* s, const uint8_t* d_mask, const float* d_in, uint64_t ics, uint64_t ips, float* d_out,
+                 uint64_t ocs, uint64_t ops, uint64_t npart)
+{
+  return guard(s ? s->ctx : nullptr, [&] {
+    DSPX_REQUIRE(s && d_mask && d_in && d_out, DSPX_E_PARAM, "null argument");
+    const uint64_t n = npart * s->d.M;
+    if (!n) return;
+    DSPX_REQUIRE((n + 255) / 256 < (1ull << 31), DSPX_E_UNSUPPORTED, "block too long");
+    dim3 grid(unsigned((n + 255) / 256), s->d.nchan * s->d.npol);
+    DSPX_REQUIRE(grid.y <= 65535, DSPX_E_UNSUPPORTED, "nchan*npol=%u", grid.y);
+    k_sk_mask<<<grid, 256, 0, s->ctx->stream>>>(d_mask, d_in, ics, ips, d_out, ocs, ops, s->d.nchan, s->d.npol,
+                                                s->d.M, n);
+    DSPX_HIP(hipGetLastError());
+  });
+}
+
+}  // extern "C"

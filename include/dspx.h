@@ -187,6 +187,56 @@ int dspx_fold_execute(dspx_fold* fold, const float* d_in, uint64_t in_chan_strid
                       uint64_t in_pol_stride, uint32_t nchan, uint32_t npol, uint32_t ndim,
                       float* d_prof);
 
+/* ------------------------------------------------------- spectral kurtosis */
+/* One resolution of dsp::SpectralKurtosis (noverlap 1, every channel
+   included).  Sums [npart][nchan][npol][2] float (S1, S2), counts
+   [npart][nchan] uint32, mask [npart][nchan] bytes (1 = zapped), as the
+   reference's sums WeightedTimeSeries and zapmask BitSeries. */
+typedef struct dspx_sk dspx_sk;
+
+typedef struct {
+  uint32_t nchan;
+  uint32_t npol;          /* pols of the sums (1 or 2) */
+  uint32_t M;             /* samples per block */
+  uint32_t Nd;            /* integrated_Nd: 1 for complex (Analytic) input */
+  uint32_t omit_outliers; /* SpectralKurtosis::set_omit_outliers */
+  uint32_t reserved;
+} dspx_sk_desc;
+
+enum { DSPX_SK_SKFB = 1, DSPX_SK_FSCR = 2, DSPX_SK_TSCR = 4 };
+
+int dspx_sk_create(dspx_ctx* ctx, const dspx_sk_desc* desc, dspx_sk** out);
+void dspx_sk_destroy(dspx_sk* sk);
+/* Thresholds by block count 0..M (host arrays of M+1 floats: the SKLimits
+   lower / upper / symmetric thresholds of each count, as the reference's
+   SpectralKurtosis::get_limits caches them, SpectralKurtosis.C:989-1009) and
+   the per-comparison sigma of the closed form that detect_tscr uses for
+   totals >= 32768 (SKLimits.C:152-160).  Synchronous. */
+int dspx_sk_set_limits(dspx_sk* sk, const float* lower, const float* upper, const float* symmetric,
+                       double pc_std_devs);
+/* Engine::compute (SpectralKurtosis.C:699-703, 674-798) on an FPT complex
+   float TimeSeries: blocks [0, npart) of M samples.  zeroed_data: zero
+   samples are not counted (input->get_zeroed_data()). */
+int dspx_sk_compute(dspx_sk* sk, const float* d_in, uint64_t in_chan_stride, uint64_t in_pol_stride,
+                    uint64_t npart, int zeroed_data, float* d_sums, uint32_t* d_counts);
+/* The same from raw TFP 8/16-bit samples (unpacked as dspx_unpack does)
+   starting at input sample t0: the zero-DM input of a convolution whose
+   output sample 0 is input sample t0 = nfilt_pos (Convolution.C:792-799). */
+int dspx_sk_compute_raw(dspx_sk* sk, const void* d_raw, uint32_t nbit, uint32_t offset_binary, float scale,
+                        uint64_t t0, uint64_t npart, float* d_sums, uint32_t* d_counts);
+/* SpectralKurtosis::detect (SpectralKurtosis.C:830-909): the mask is reset,
+   then detect_skfb (DSPX_SK_SKFB, :1105-1222), detect_fscr (DSPX_SK_FSCR,
+   :1312-1462) and detect_tscr (DSPX_SK_TSCR, :1014-1103) as flagged.  With
+   omit_outliers, detect_skfb zeroes the sums and counts of the blocks it
+   zaps.  detect_tscr does not test totals between M and 32768. */
+int dspx_sk_detect(dspx_sk* sk, float* d_sums, uint32_t* d_counts, uint64_t npart, uint32_t flags,
+                   uint8_t* d_mask);
+/* Engine::mask (SpectralKurtosis.C:1518-1554): FPT out = in with block b of
+   channel c zeroed in every pol where mask[b][c]; in may equal out. */
+int dspx_sk_mask(dspx_sk* sk, const uint8_t* d_mask, const float* d_in, uint64_t in_chan_stride,
+                 uint64_t in_pol_stride, float* d_out, uint64_t out_chan_stride, uint64_t out_pol_stride,
+                 uint64_t npart);
+
 /* ------------------------------------------------------ fused LoadToFold */
 
 typedef struct dspx_ltf dspx_ltf;

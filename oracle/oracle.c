@@ -702,3 +702,167 @@ uint64_t ox_sk_detect_skfb(const float* sums, const uint32_t* counts, uint64_t n
     }
   return nzap;
 }
+
+/* SKLimits.C:152-160: thresholds of a count M >= 32768,
+   1 -/+ sqrt(4/M) * per_comparison_std_devs (pcsd), stored as float. */
+static void ox_sk_limits_large(uint64_t M, double pcsd, float* lower, float* upper)
+{
+  double one_std_dev = sqrt(4.0 / (double) M);
+  double factor = one_std_dev * pcsd;
+  *lower = (float) (1.0f - factor);
+  *upper = (float) (1.0f + factor);
+}
+
+/* SpectralKurtosis::detect (SpectralKurtosis.C:830-909) with one resolution
+   (noverlap 1, every channel included): the mask is reset, then
+     flags & 1: detect_skfb  (SpectralKurtosis.C:1105-1222)
+     flags & 2: detect_fscr  (SpectralKurtosis.C:1312-1462)
+     flags & 4: detect_tscr  (SpectralKurtosis.C:1014-1103)
+   lower/upper/symmetric: SKLimits thresholds by count (tables of M+1).
+   detect_tscr: totals above M use the closed form of SKLimits.C:152-160 with
+   pcsd (per-comparison sigma) when >= 32768; totals between M and 32768 are
+   not tested.  With omit_outliers, skfb zeroes the sums and counts of the
+   blocks it zaps (:1189-1199) and fscr skips them (:1382).  Unsigned
+   arithmetic as in the reference (Mprime - 1 wraps for Mprime = 0).
+   Returns the number of mask bytes set. */
+uint64_t ox_sk_detect(float* sums, uint32_t* counts, uint64_t npart, unsigned nchan, unsigned npol,
+                      unsigned M, unsigned Nd, int omit_outliers, unsigned flags, const float* lower,
+                      const float* upper, const float* symmetric, double pcsd, uint8_t* mask)
+{
+  uint64_t ipart, i, nset = 0;
+  unsigned ichan, ipol;
+  memset(mask, 0, npart * nchan);
+  if (flags & 1)
+    for (ipart = 0; ipart < npart; ipart++)
+      for (ichan = 0; ichan < nchan; ichan++)
+      {
+        uint32_t* wt = counts + ipart * nchan + ichan;
+        float* sd = sums + (ipart * nchan + ichan) * npol * 2;
+        unsigned Mprime = *wt;
+        int zap = 1;
+        if (Mprime == 0)
+          continue;
+        if (Mprime > 2)
+        {
+          const float M_fac = (float) (Mprime * Nd + 1) / (Mprime - 1);
+          zap = 0;
+          for (ipol = 0; ipol < npol; ipol++)
+          {
+            float S1_sum = sd[2 * ipol], S2_sum = sd[2 * ipol + 1];
+            float V = M_fac * (Mprime * (S2_sum / (S1_sum * S1_sum)) - 1);
+            if (V > upper[Mprime] || V < lower[Mprime])
+              zap = 1;
+          }
+        }
+        if (zap)
+        {
+          if (omit_outliers)
+          {
+            *wt = 0;
+            for (ipol = 0; ipol < npol; ipol++)
+              sd[2 * ipol] = sd[2 * ipol + 1] = 0.0f;
+          }
+          mask[ipart * nchan + ichan] = 1;
+        }
+      }
+  if (flags & 2)
+    for (ipart = 0; ipart < npart; ipart++)
+    {
+      int zap_ipart = 0;
+      for (ipol = 0; ipol < npol; ipol++)
+      {
+        double sk_avg = 0, M_tot = 0, Msq_tot = 0, chan_count = 0;
+        for (ichan = 0; ichan < nchan; ichan++)
+          if (!omit_outliers || mask[ipart * nchan + ichan] == 0)
+          {
+            const float* sd = sums + ((ipart * nchan + ichan) * npol + ipol) * 2;
+            float S1_sum = sd[0], S2_sum = sd[1];
+            unsigned Mprime = counts[ipart * nchan + ichan];
+            const float M_fac = (float) (Mprime * Nd + 1) / (Mprime - 1);
+            float V = M_fac * (Mprime * (S2_sum / (S1_sum * S1_sum)) - 1);
+            sk_avg += V * Mprime;
+            M_tot += Mprime;
+            Msq_tot += Mprime * Mprime;
+            chan_count++;
+          }
+        if (chan_count > 0)
+        {
+          double M_mean, Msq_mean, Deff, effective_count;
+          float threshold, up, lo;
+          sk_avg /= M_tot;
+          M_mean = M_tot / chan_count;
+          Msq_mean = Msq_tot / chan_count;
+          Deff = Msq_mean / (M_mean * M_mean);
+          effective_count = chan_count / Deff;
+          threshold = symmetric[(unsigned) (M_mean + 0.5)] / sqrt(effective_count);
+          up = 1.0 + threshold;
+          lo = 1.0 - threshold;
+          if (sk_avg > up || sk_avg < lo)
+            zap_ipart = 1;
+        }
+      }
+      if (zap_ipart)
+        for (ichan = 0; ichan < nchan; ichan++)
+          mask[ipart * nchan + ichan] = 1;
+    }
+  if (flags & 4)
+    for (ichan = 0; ichan < nchan; ichan++)
+    {
+      uint64_t count = 0;
+      float lo, up, M_t, M_fac;
+      int zap_chan = 0;
+      for (ipart = 0; ipart < npart; ipart++)
+        count += counts[ipart * nchan + ichan];
+      if (count == 0)
+        continue;
+      if (count <= M)
+      {
+        lo = lower[count];
+        up = upper[count];
+      }
+      else if (count >= 32768)
+        ox_sk_limits_large(count, pcsd, &lo, &up);
+      else
+        continue;
+      M_t = (float) count;
+      M_fac = (M_t * Nd + 1) / (M_t - 1);
+      for (ipol = 0; ipol < npol; ipol++)
+      {
+        double S1 = 0.0, S2 = 0.0, SK;
+        for (ipart = 0; ipart < npart; ipart++)
+        {
+          uint64_t idx = (ipart * nchan + ichan) * npol + ipol;
+          S1 += sums[idx * 2];
+          S2 += sums[idx * 2 + 1];
+        }
+        SK = M_fac * (M_t * (S2 / (S1 * S1)) - 1);
+        if (SK > up || SK < lo)
+          zap_chan = 1;
+      }
+      if (zap_chan)
+        for (ipart = 0; ipart < npart; ipart++)
+          mask[ipart * nchan + ichan] = 1;
+    }
+  for (i = 0; i < npart * nchan; i++)
+    nset += mask[i];
+  return nset;
+}
+
+/* SpectralKurtosis::mask (SpectralKurtosis.C:1531-1554), FPT complex input,
+   nskip = 0, noverlap 1: block ipart of channel ichan is zeroed in every pol
+   when mask[ipart][ichan] is set, copied otherwise. */
+void ox_sk_mask(const float* in, unsigned nchan, unsigned npol, uint64_t ndat, unsigned M,
+                const uint8_t* mask, uint64_t npart, float* out)
+{
+  unsigned ichan, ipol;
+  uint64_t ipart, j;
+  for (ichan = 0; ichan < nchan; ichan++)
+    for (ipol = 0; ipol < npol; ipol++)
+    {
+      const float* x = in + ((uint64_t) ichan * npol + ipol) * ndat * 2;
+      float* y = out + ((uint64_t) ichan * npol + ipol) * ndat * 2;
+      for (ipart = 0; ipart < npart; ipart++)
+        for (j = 0; j < 2 * (uint64_t) M; j++)
+          y[ipart * M * 2 + j] = mask[ipart * nchan + ichan] ? 0.0f : x[ipart * M * 2 + j];
+    }
+}

@@ -81,6 +81,10 @@ def lib():
     L.ox_sk_compute.argtypes = [f32p, u, u, U, u, i, f32p, u32p]
     L.ox_sk_detect_skfb.restype = U
     L.ox_sk_detect_skfb.argtypes = [f32p, u32p, U, u, u, u, f32p, f32p, u8p]
+    L.ox_sk_detect.restype = U
+    L.ox_sk_detect.argtypes = [f32p, u32p, U, u, u, u, u, i, u, f32p, f32p, f32p, d, u8p]
+    L.ox_sk_mask.restype = None
+    L.ox_sk_mask.argtypes = [f32p, u, u, U, u, u8p, U, f32p]
     _lib = L
     return L
 
@@ -254,3 +258,32 @@ def sk_detect_skfb(sums, counts, lower, upper, Nd=1):
                                 nchan, npol, Nd, np.ascontiguousarray(lower, np.float32),
                                 np.ascontiguousarray(upper, np.float32), mask)
     return mask.reshape(npart, nchan), int(n)
+
+
+SKFB, FSCR, TSCR = 1, 2, 4
+
+
+def sk_detect(sums, counts, M, lower, upper, symmetric, pcsd=3.0, flags=SKFB, Nd=1, omit_outliers=False):
+    """SpectralKurtosis::detect with one resolution.  Returns the mask
+    [npart][nchan] and the (possibly modified, with omit_outliers) sums and
+    counts."""
+    sums = np.array(sums, np.float32, copy=True)
+    counts = np.array(counts, np.uint32, copy=True)
+    npart, nchan, npol, _ = sums.shape
+    mask = np.zeros(npart * nchan, np.uint8)
+    lib().ox_sk_detect(sums.reshape(-1), counts.reshape(-1), npart, nchan, npol, M, Nd, int(omit_outliers),
+                       flags, np.ascontiguousarray(lower, np.float32), np.ascontiguousarray(upper, np.float32),
+                       np.ascontiguousarray(symmetric, np.float32), pcsd, mask)
+    return mask.reshape(npart, nchan), sums, counts
+
+
+def sk_mask(x, M, mask):
+    """x: [nchan][npol][ndat] complex64; blocks of M samples zeroed where
+    mask[ipart][ichan]; samples past npart*M are left zero."""
+    x = np.ascontiguousarray(x, np.complex64)
+    nchan, npol, ndat = x.shape
+    npart = mask.shape[0]
+    out = np.zeros_like(x)
+    lib().ox_sk_mask(x.view(np.float32).reshape(-1), nchan, npol, ndat, M,
+                     np.ascontiguousarray(mask, np.uint8).reshape(-1), npart, out.view(np.float32).reshape(-1))
+    return out

@@ -1,0 +1,23 @@
+"""Writes sk_limits.npz: the SK threshold tables the GPU tests use (counts
+0..32 in full, and count 64 of M = 64), from dspsr-kanyixia_amd/sklimits.py,
+so that GPU test processes do not evaluate the Pearson integrals themselves.
+
+    python tests/golden/make_sk_limits.py
+"""
+import importlib
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
+S = importlib.import_module("dspsr-kanyixia_amd.sklimits")
+
+out = {}
+for M, counts in ((32, None), (64, [64])):
+    lo, up, sym, pcsd = S.limit_tables(M, 3.0, npol=2, counts=counts)
+    out["lower%d" % M], out["upper%d" % M], out["sym%d" % M] = lo, up, sym
+    out["pcsd%d" % M] = np.float64(pcsd)
+np.savez(os.path.join(HERE, "sk_limits.npz"), **out)
+print("wrote", os.path.join(HERE, "sk_limits.npz"))

@@ -126,3 +126,23 @@ def oracle_load_to_fold(oracle, raw, nbit, nchan, npol, ndat, cfreq, bw, dm, geo
     for (i0, n, phi, isub) in chunks:
         oracle.fold(det, nd, phi, pps, nbin, idat_start=i0, ndat_fold=n, prof=prof[isub], hits=hits[isub])
     return fb, prof, hits[:, 0]
+
+
+def sk_rfi_input(nchan, npol, nblk, M, seed, tone_chan=None, impulse_blk=None, zero_blk=None):
+    """Complex Gaussian noise [nchan][npol][nblk*M] (sigma 10 per component)
+    with optional RFI: a constant-amplitude tone filling channel tone_chan (SK
+    near 0), a broadband burst filling block impulse_blk in every channel
+    (large, non-Gaussian power), and zeroed samples in block zero_blk of
+    channel 0 (every other sample), for the zeroed-data counts."""
+    rng = np.random.default_rng(seed)
+    n = nblk * M
+    x = (rng.standard_normal((nchan, npol, n)) + 1j * rng.standard_normal((nchan, npol, n))) * 10
+    if tone_chan is not None:
+        t = np.arange(n)
+        x[tone_chan] += 80 * np.exp(2j * np.pi * 0.123 * t)
+    if impulse_blk is not None:
+        s = slice(impulse_blk * M, impulse_blk * M + M // 8)
+        x[:, :, s] *= 30
+    if zero_blk is not None:
+        x[0, :, zero_blk * M:(zero_blk + 1) * M:2] = 0
+    return x.astype(np.complex64)
