@@ -98,6 +98,13 @@ _SIGS = {
     "dspx_sk_compute_raw": (I32, [P, P, U32, U32, C.c_float, U64, U64, P, P]),
     "dspx_sk_detect": (I32, [P, P, P, U64, U32, P]),
     "dspx_sk_mask": (I32, [P, P, P, U64, U64, P, U64, U64, U64]),
+    "dspx_batch_create": (I32, [P, C.POINTER(LtfDesc), U32, U32, C.POINTER(P)]),
+    "dspx_batch_destroy": (None, [P]),
+    "dspx_batch_beam": (I32, [P, U32, C.POINTER(LtfDesc), C.POINTER(U64), C.POINTER(U64)]),
+    "dspx_batch_set_fold": (I32, [P, U32, D, C.POINTER(FoldChunk), U32]),
+    "dspx_batch_execute": (I32, [P, P, P, P]),
+    "dspx_batch_capture": (I32, [P, P, P, P]),
+    "dspx_batch_replay": (I32, [P]),
     "dspx_ltf_create": (I32, [P, C.POINTER(LtfDesc), C.POINTER(P)]),
     "dspx_ltf_destroy": (None, [P]),
     "dspx_ltf_geometry": (I32, [P, C.POINTER(U64), C.POINTER(U64)]),
@@ -421,3 +428,67 @@ def fold_chunks(ndat_out, rate, period, nfilt_pos, nsub=1, t0=0.0, reference_pha
         t = t0 + (nfilt_pos + i0 + 0.5) / rate
         chunks.append((i0, i1 - i0, fold_phi(t, period, reference_phase), k))
     return chunks
+
+
+BATCH_AUTOTUNE = 1
+
+
+def ltf_desc(nbit, nchan, npol, ndat, centre_frequency, bandwidth, dm, nbin, state=INTENSITY, ndim=1,
+             chirp_mode=CHIRP_PARITY, form=FORM_FUSED, nfft=0, nfilt_pos=0, nfilt_neg=0, scale=1.0,
+             offset_binary=False):
+    """A dspx_ltf_desc (nfft = 0 leaves the length to the batch)."""
+    return LtfDesc(nbit, int(offset_binary), scale, nchan, npol, 0, ndat,
+                   _dedisp(centre_frequency, bandwidth, dm, nchan, False), nfft, nfilt_pos, nfilt_neg,
+                   chirp_mode, state, ndim, nbin, form)
+
+
+class Batch:
+    """dspx_batch: beams of uneven shape through the fused path, executed in
+    order or replayed from a captured HIP graph."""
+
+    def __init__(self, ctx, descs, autotune=False):
+        self.ctx = ctx
+        self.h = None
+        self._lib = lib()
+        arr = (LtfDesc * len(descs))(*descs)
+        h = P()
+        ctx.check(lib().dspx_batch_create(ctx.h, arr, len(descs), BATCH_AUTOTUNE if autotune else 0, C.byref(h)))
+        self.h = h
+        _register(self, False)
+        self.nbeam = len(descs)
+        self.beams = []
+        for i in range(self.nbeam):
+            d, npart, nout = LtfDesc(), U64(), U64()
+            ctx.check(lib().dspx_batch_beam(h, i, C.byref(d), C.byref(npart), C.byref(nout)))
+            self.beams.append({"nfft": d.nfft, "nfilt_pos": d.nfilt_pos, "nfilt_neg": d.nfilt_neg,
+                               "npart": npart.value, "ndat_out": nout.value,
+                               "nval": 1 if d.state == INTENSITY else (d.npol if d.state == PPQQ else 4)})
+
+    def close(self):
+        if self.h:
+            self._lib.dspx_batch_destroy(self.h)
+            self.h = None
+
+    __del__ = close
+
+    def set_fold(self, i, pps, chunks):
+        arr = (FoldChunk * len(chunks))(*[FoldChunk(c[0], c[1], c[2], c[3], 0) for c in chunks])
+        self.ctx.check(lib().dspx_batch_set_fold(self.h, i, pps, arr, len(chunks)))
+
+    def _ptrs(self, raws, profs, hits):
+        self._keep = (raws, profs, hits)
+        r = (P * self.nbeam)(*[ptr(x) for x in raws])
+        p = (P * self.nbeam)(*[ptr(x) for x in profs])
+        h = (P * self.nbeam)(*[ptr(x) for x in hits]) if hits is not None else None
+        return r, p, h
+
+    def execute(self, raws, profs, hits=None):
+        r, p, h = self._ptrs(raws, profs, hits)
+        self.ctx.check(lib().dspx_batch_execute(self.h, r, p, h))
+
+    def capture(self, raws, profs, hits=None):
+        r, p, h = self._ptrs(raws, profs, hits)
+        self.ctx.check(lib().dspx_batch_capture(self.h, r, p, h))
+
+    def replay(self):
+        self.ctx.check(lib().dspx_batch_replay(self.h))

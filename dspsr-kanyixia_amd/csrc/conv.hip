@@ -8,6 +8,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdlib>
+#include <memory>
 
 #include "dspx_internal.hpp"
 #include "fold_impl.hpp"
@@ -44,7 +45,8 @@ struct ConvCore {
   int h = 0;
   DevBuf chan_par; // coeff[nchan], cf[nchan]
   DevBuf table;    // [nchan][N1][N2]
-  DevBuf work;     // [nchan][npart][npol][N]
+  DevBuf work;     // [nchan][npart][npol][N], unless work_ext is set
+  DevBuf* work_ext = nullptr;  // shared work buffer (plans of a batch)
   uint32_t work_npart = 0;
   double binwidth = 0, half_cw = 0;
   float snorm = 0;
@@ -119,16 +121,19 @@ struct ConvCore {
     have_table = true;
   }
 
+  DevBuf& wbuf() { return work_ext ? *work_ext : work; }
+  const DevBuf& wbuf() const { return work_ext ? *work_ext : work; }
+
   void reserve_work(uint32_t npart)
   {
-    work.reserve(size_t(nchan) * npart * npol * N * sizeof(float2));
+    wbuf().reserve(size_t(nchan) * npart * npol * N * sizeof(float2));
     work_npart = npart;
   }
 
   ConvShape shape(uint32_t npart) const
   {
     ConvShape s;
-    s.work = work.as<float2>();
+    s.work = wbuf().as<float2>();
     s.N1 = N1;
     s.N2 = N2;
     s.log2N = m;
@@ -212,9 +217,14 @@ struct ConvCore {
     switch (N1) {
 #define LAUNCH_C(n, nv)                                                                         \
   do {                                                                                          \
-    if (shm > 64 * 1024)                                                                        \
+    /* the whole LDS allowed once per kernel (not a stream operation, so \
+       also outside any graph capture that follows) */                                          \
+    static bool lds_attr = false;                                                               \
+    if (!lds_attr) {                                                                            \
       DSPX_HIP(hipFuncSetAttribute((const void*) k_stageC<n, MODE, nv>,                         \
-                                   hipFuncAttributeMaxDynamicSharedMemorySize, int(shm)));      \
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));    \
+      lds_attr = true;                                                                          \
+    }                                                                                           \
     k_stageC<n, MODE, nv><<<unsigned(nb), kColThreads, shm, ctx->stream>>>(s, fb, fa, twc);     \
   } while (0)
 #define CASE_C(n)                                    \
@@ -419,6 +429,15 @@ int dspx_conv_perform(dspx_conv* c, const float* d_in, uint64_t ics, uint64_t ip
 
 // ------------------------------------------------------------ fused LoadToFold
 
+// Large per-block scratch of the fused form: owned by one plan, or shared by
+// the plans of a batch, which run one after another on one stream.
+struct LtfScratch {
+  DevBuf work;      // ConvCore work buffer
+  DevBuf ftp;       // channel-major copy of the raw block (nchan > 1)
+  DevBuf partial;   // [nchan][npart][G][nslot][nval*nbin]
+  DevBuf partial2;  // [nchan][npart][nslot][nval*nbin]
+};
+
 struct dspx_ltf {
   dspx_ctx* ctx = nullptr;
   dspx_ltf_desc d{};
@@ -432,10 +451,9 @@ struct dspx_ltf {
   // fused fold partials
   uint32_t G = 1, nslot = 1;
   DevBuf sub_first;
-  DevBuf partial;   // [nchan][npart][G][nslot][nval*nbin]
-  DevBuf partial2;  // [nchan][npart][nslot][nval*nbin]
-  // channel-major copy of the raw block (nchan > 1)
-  DevBuf ftp;
+  LtfScratch own;
+  LtfScratch* ext = nullptr;
+  LtfScratch& sc() { return ext ? *ext : own; }
   // plain form intermediates
   DevBuf unpacked, filtered, detected;
   // timing
@@ -464,9 +482,9 @@ template <class F> void with_loader(dspx_ltf* L, const void* raw, F&& f)
   uint64_t tstride = uint64_t(d.nchan) * d.npol * 2, cstride = uint64_t(d.npol) * 2;  // in elements
   const void* src = raw;
   if (d.nchan > 1) {
-    L->ftp.reserve(d.ndat * d.nchan * wbytes);
-    tfp_to_ftp(L->ctx, raw, d.ndat, d.nchan, uint32_t(wbytes), L->ftp.p);
-    src = L->ftp.p;
+    L->sc().ftp.reserve(d.ndat * d.nchan * wbytes);
+    tfp_to_ftp(L->ctx, raw, d.ndat, d.nchan, uint32_t(wbytes), L->sc().ftp.p);
+    src = L->sc().ftp.p;
     tstride = uint64_t(d.npol) * 2;
     cstride = d.ndat * d.npol * 2;
     ltf_record(L, 1);
@@ -485,10 +503,16 @@ template <class F> void with_loader(dspx_ltf* L, const void* raw, F&& f)
 
 extern "C" {
 
-int dspx_ltf_create(dspx_ctx* ctx, const dspx_ltf_desc* d, dspx_ltf** out)
+}  // extern "C"
+
+namespace {
+
+// Builds a plan; with ext, the large scratch is the caller's (shared by the
+// plans of a batch) and grows to this plan's needs.
+dspx_ltf* ltf_create(dspx_ctx* ctx, const dspx_ltf_desc* d, LtfScratch* ext)
 {
-  return guard(ctx, [&] {
-    DSPX_REQUIRE(ctx && d && out, DSPX_E_PARAM, "null argument");
+  dspx_ltf* result = nullptr;
+  {
     DSPX_REQUIRE(d->nbit == 8 || d->nbit == 16, DSPX_E_PARAM, "nbit=%u", d->nbit);
     DSPX_REQUIRE(d->npol == 1 || d->npol == 2, DSPX_E_PARAM, "npol=%u", d->npol);
     DSPX_REQUIRE(d->state <= DSPX_STOKES, DSPX_E_PARAM, "state=%u", d->state);
@@ -502,6 +526,8 @@ int dspx_ltf_create(dspx_ctx* ctx, const dspx_ltf_desc* d, dspx_ltf** out)
     auto* L = new dspx_ltf;
     L->ctx = ctx;
     L->d = *d;
+    L->ext = ext;
+    if (ext) L->core.work_ext = &ext->work;
     try {
       uint32_t nov = d->nfilt_pos + d->nfilt_neg;
       DSPX_REQUIRE(nov < d->nfft, DSPX_E_STATE, "error nfft=%u < nfilt=%u", d->nfft, nov);
@@ -525,7 +551,31 @@ int dspx_ltf_create(dspx_ctx* ctx, const dspx_ltf_desc* d, dspx_ltf** out)
       delete L;
       throw;
     }
-    *out = L;
+    result = L;
+  }
+  return result;
+}
+
+// Reserves the scratch an execute of L needs (so that an execute captured
+// into a graph allocates nothing).
+void ltf_reserve(dspx_ltf* L)
+{
+  const auto& d = L->d;
+  if (d.form != DSPX_FORM_FUSED || !L->npart) return;
+  uint64_t wbytes = uint64_t(d.npol) * 2 * d.nbit / 8;
+  if (d.nchan > 1) L->sc().ftp.reserve(d.ndat * d.nchan * wbytes);
+  if (L->have_fold) L->sc().partial2.reserve(uint64_t(d.nchan) * L->npart * L->nslot * L->nval * d.nbin * sizeof(float));
+}
+
+}  // namespace
+
+extern "C" {
+
+int dspx_ltf_create(dspx_ctx* ctx, const dspx_ltf_desc* d, dspx_ltf** out)
+{
+  return guard(ctx, [&] {
+    DSPX_REQUIRE(ctx && d && out, DSPX_E_PARAM, "null argument");
+    *out = ltf_create(ctx, d, nullptr);
   });
 }
 
@@ -577,7 +627,7 @@ int dspx_ltf_set_fold(dspx_ltf* L, double pps, const dspx_fold_chunk* chunks, ui
       L->G = G;
       L->sub_first.reserve(sf.size() * sizeof(uint32_t));
       DSPX_HIP(hipMemcpy(L->sub_first.p, sf.data(), sf.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
-      L->partial.reserve(units * G * hist_bytes);
+      L->sc().partial.reserve(units * G * hist_bytes);
     }
   });
 }
@@ -605,7 +655,7 @@ int dspx_ltf_execute(dspx_ltf* L, const void* d_raw, float* d_prof, uint32_t* d_
       L->stage_name[L->nstage++] = "stageB_rowfft_chirp";
       FoldArgs fa;
       fa.binplan = L->fold.plan.as<uint32_t>();
-      fa.partial = L->partial.as<float>();
+      fa.partial = L->sc().partial.as<float>();
       fa.sub_first = L->sub_first.as<uint32_t>();
       fa.nbin = d.nbin;
       fa.nval = L->nval;
@@ -620,12 +670,12 @@ int dspx_ltf_execute(dspx_ltf* L, const void* d_raw, float* d_prof, uint32_t* d_
       L->stage_name[L->nstage++] = "stageC_colifft_detect_fold";
       uint32_t hl = L->nval * d.nbin;
       uint64_t n2 = uint64_t(d.nchan) * np * L->nslot * hl;
-      L->partial2.reserve(n2 * sizeof(float));
-      k_fold_reduce_g<<<unsigned((n2 + 255) / 256), 256, 0, ctx->stream>>>(L->partial.as<float>(), L->G,
-                                                                          L->nslot * hl, n2, L->partial2.as<float>());
+      L->sc().partial2.reserve(n2 * sizeof(float));
+      k_fold_reduce_g<<<unsigned((n2 + 255) / 256), 256, 0, ctx->stream>>>(
+          L->sc().partial.as<float>(), L->G, L->nslot * hl, n2, L->sc().partial2.as<float>());
       DSPX_HIP(hipGetLastError());
       dim3 grid((hl + 255) / 256, d.nchan, L->fold.nsub);
-      k_fold_reduce_p<<<grid, 256, 0, ctx->stream>>>(L->partial2.as<float>(), L->sub_first.as<uint32_t>(), np,
+      k_fold_reduce_p<<<grid, 256, 0, ctx->stream>>>(L->sc().partial2.as<float>(), L->sub_first.as<uint32_t>(), np,
                                                      L->nslot, hl, d.nchan, d_prof);
       DSPX_HIP(hipGetLastError());
       if (d_hits) L->fold.hits(d_hits);
@@ -709,6 +759,240 @@ int dspx_ltf_stage_times(dspx_ltf* L, float* ms, const char** names, int cap, in
       if (names) names[i] = L->stage_name[i];
     }
     *n = L->nstage;
+  });
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------- batch
+// Many beams of uneven shape (channels, DM, FFT length, 8/16-bit input,
+// fold bins) through the fused path: one LoadToFold plan per beam, one set of
+// large scratch buffers shared by all of them (sized for the largest beam;
+// the plans run one after another on the context's stream), the sequence
+// captured once into a HIP graph and replayed.  The FFT length of a beam can
+// be chosen by timing candidate lengths (autotune_nfft) instead of the
+// reference's cost model (optimize_fft.c:64-135).
+
+struct dspx_batch {
+  dspx_ctx* ctx = nullptr;
+  LtfScratch sc;
+  std::vector<dspx_ltf*> beams;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  DevBuf tune_raw, tune_prof, tune_hits;
+
+  ~dspx_batch()
+  {
+    if (exec) (void) hipGraphExecDestroy(exec);
+    if (graph) (void) hipGraphDestroy(graph);
+    for (auto* L : beams) delete L;
+  }
+};
+
+namespace {
+
+// The descriptor with its overlap filled in for FFT length nfft (0: the
+// reference's rule, Response.C:553-588, raised to at least 4096).
+dspx_ltf_desc with_geometry(const dspx_ltf_desc& d0, uint32_t nfft)
+{
+  dspx_dedisp_desc dd = d0.dedisp;
+  dd.nfft = nfft;
+  dspx_geometry g{};
+  int rc = geometry(&dd, &g);
+  DSPX_REQUIRE(rc == DSPX_OK, rc, "geometry of a beam (nfft=%u) failed", nfft);
+  if (nfft == 0 && g.nfft < 4096) {
+    // below the fused form's shortest transform: the next supported length
+    // (as a user-set length would be, Response::set_frequency_resolution,
+    // Response.C:74-80, 242-250)
+    dd.nfft = 4096;
+    rc = geometry(&dd, &g);
+    DSPX_REQUIRE(rc == DSPX_OK, rc, "geometry of a beam (nfft=4096) failed");
+  }
+  dspx_ltf_desc d = d0;
+  d.nfft = g.nfft;
+  d.nfilt_pos = g.impulse_pos;
+  d.nfilt_neg = g.impulse_neg;
+  return d;
+}
+
+// Seconds per output sample of the fused LoadToFold at FFT length nfft, from
+// three timed executions on a zero-filled block of a few parts (the kernels'
+// run time does not depend on the sample values).
+double time_per_sample(dspx_batch* B, const dspx_ltf_desc& dn)
+{
+  dspx_ctx* ctx = B->ctx;
+  const uint32_t nov = dn.nfilt_pos + dn.nfilt_neg;
+  dspx_ltf_desc d = dn;
+  const uint64_t step = d.nfft - nov;
+  d.ndat = std::min<uint64_t>(dn.ndat, nov + 4 * step);
+  if (d.ndat < d.nfft) return 1e30;
+  std::unique_ptr<dspx_ltf> L(ltf_create(ctx, &d, &B->sc));
+  if (!L->npart) return 1e30;
+  dspx_fold_chunk ch{0, L->ndat_out, 0.0, 0, 0};
+  L->fold.build(ctx, L->ndat_out, 1.0 / 97.0e3, d.nbin, &ch, 1);
+  L->have_fold = true;
+  // single sub-integration: the per-part slots and partials of set_fold
+  L->nslot = 1;
+  std::vector<uint32_t> sf(L->npart, 0);
+  L->sub_first.reserve(sf.size() * sizeof(uint32_t));
+  DSPX_HIP(hipMemcpy(L->sub_first.p, sf.data(), sf.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+  uint64_t units = uint64_t(d.nchan) * L->npart;
+  L->G = uint32_t(std::min<uint64_t>(L->core.ncb(), std::max<uint64_t>(1, (2048 + units - 1) / units)));
+  L->sc().partial.reserve(units * L->G * L->nval * d.nbin * sizeof(float));
+  ltf_reserve(L.get());
+  const uint64_t raw_bytes = d.ndat * d.nchan * d.npol * 2 * (d.nbit / 8);
+  B->tune_raw.reserve(raw_bytes);
+  DSPX_HIP(hipMemsetAsync(B->tune_raw.p, 0, raw_bytes, ctx->stream));
+  B->tune_prof.reserve(size_t(d.nchan) * L->nval * d.nbin * sizeof(float));
+  B->tune_hits.reserve(size_t(d.nbin) * sizeof(uint32_t));
+  auto run = [&] {
+    int rc = dspx_ltf_execute(L.get(), B->tune_raw.p, B->tune_prof.as<float>(), B->tune_hits.as<uint32_t>());
+    DSPX_REQUIRE(rc == DSPX_OK, rc, "autotune execute: %s", ctx->last_error.c_str());
+  };
+  run();
+  DSPX_HIP(hipEventRecord(ctx->ev[0], ctx->stream));
+  for (int i = 0; i < 3; i++) run();
+  DSPX_HIP(hipEventRecord(ctx->ev[1], ctx->stream));
+  DSPX_HIP(hipEventSynchronize(ctx->ev[1]));
+  float ms = 0;
+  DSPX_HIP(hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]));
+  return ms * 1e-3 / 3.0 / double(L->ndat_out);
+}
+
+// Candidate lengths: the reference's choice and the next two powers of two
+// (longer transforms waste less on the overlap, shorter ones keep the column
+// and row transforms in their fastest shapes), within the fused form's range.
+uint32_t autotune_nfft(dspx_batch* B, const dspx_ltf_desc& d0)
+{
+  dspx_ltf_desc ref = with_geometry(d0, 0);
+  uint32_t best = ref.nfft;
+  double best_t = 1e30;
+  for (uint32_t n = ref.nfft; n <= (1u << 20) && n <= 4 * ref.nfft; n *= 2) {
+    if (n < 4096) continue;
+    dspx_ltf_desc d = with_geometry(d0, n);
+    if (d.nfilt_pos + d.nfilt_neg >= n || d.ndat < n) continue;
+    double t = time_per_sample(B, d);
+    if (t < best_t) {
+      best_t = t;
+      best = n;
+    }
+  }
+  return best;
+}
+
+}  // namespace
+
+extern "C" {
+
+int dspx_batch_create(dspx_ctx* ctx, const dspx_ltf_desc* beams, uint32_t nbeam, uint32_t flags, dspx_batch** out)
+{
+  return guard(ctx, [&] {
+    DSPX_REQUIRE(ctx && beams && out && nbeam, DSPX_E_PARAM, "null argument");
+    DSPX_REQUIRE((flags & ~uint32_t(DSPX_BATCH_AUTOTUNE)) == 0, DSPX_E_PARAM, "flags=%u", flags);
+    DSPX_HIP(hipSetDevice(ctx->device));
+    std::unique_ptr<dspx_batch> B(new dspx_batch);
+    B->ctx = ctx;
+    for (uint32_t i = 0; i < nbeam; i++) {
+      const dspx_ltf_desc& d0 = beams[i];
+      DSPX_REQUIRE(d0.form == DSPX_FORM_FUSED, DSPX_E_PARAM, "beam %u: the batch runs the fused form", i);
+      dspx_ltf_desc d = d0;
+      if (d0.nfft == 0) {
+        uint32_t n = (flags & DSPX_BATCH_AUTOTUNE) ? autotune_nfft(B.get(), d0) : 0;
+        d = with_geometry(d0, n);
+      }
+      B->beams.push_back(ltf_create(ctx, &d, &B->sc));
+    }
+    B->tune_raw.release();
+    B->tune_prof.release();
+    B->tune_hits.release();
+    *out = B.release();
+  });
+}
+
+void dspx_batch_destroy(dspx_batch* B)
+{
+  if (!B) return;
+  (void) hipSetDevice(B->ctx->device);
+  (void) hipStreamSynchronize(B->ctx->stream);
+  delete B;
+}
+
+int dspx_batch_beam(const dspx_batch* B, uint32_t ibeam, dspx_ltf_desc* desc, uint64_t* npart, uint64_t* ndat_out)
+{
+  if (!B || ibeam >= B->beams.size()) return DSPX_E_PARAM;
+  const dspx_ltf* L = B->beams[ibeam];
+  if (desc) *desc = L->d;
+  if (npart) *npart = L->npart;
+  if (ndat_out) *ndat_out = L->ndat_out;
+  return DSPX_OK;
+}
+
+int dspx_batch_set_fold(dspx_batch* B, uint32_t ibeam, double pps, const dspx_fold_chunk* chunks, uint32_t nchunk)
+{
+  dspx_ctx* ctx = B ? B->ctx : nullptr;
+  return guard(ctx, [&] {
+    DSPX_REQUIRE(B && ibeam < B->beams.size(), DSPX_E_PARAM, "beam %u", ibeam);
+    DSPX_REQUIRE(!B->exec, DSPX_E_STATE, "fold plans are fixed once the batch is captured");
+    int rc = dspx_ltf_set_fold(B->beams[ibeam], pps, chunks, nchunk);
+    DSPX_REQUIRE(rc == DSPX_OK, rc, "%s", ctx->last_error.c_str());
+    ltf_reserve(B->beams[ibeam]);
+  });
+}
+
+int dspx_batch_execute(dspx_batch* B, const void* const* d_raw, float* const* d_prof, uint32_t* const* d_hits)
+{
+  dspx_ctx* ctx = B ? B->ctx : nullptr;
+  return guard(ctx, [&] {
+    DSPX_REQUIRE(B && d_raw && d_prof, DSPX_E_PARAM, "null argument");
+    for (size_t i = 0; i < B->beams.size(); i++) {
+      int rc = dspx_ltf_execute(B->beams[i], d_raw[i], d_prof[i], d_hits ? d_hits[i] : nullptr);
+      DSPX_REQUIRE(rc == DSPX_OK, rc, "beam %zu: %s", i, ctx->last_error.c_str());
+    }
+  });
+}
+
+int dspx_batch_capture(dspx_batch* B, const void* const* d_raw, float* const* d_prof, uint32_t* const* d_hits)
+{
+  dspx_ctx* ctx = B ? B->ctx : nullptr;
+  return guard(ctx, [&] {
+    DSPX_REQUIRE(B && d_raw && d_prof, DSPX_E_PARAM, "null argument");
+    for (size_t i = 0; i < B->beams.size(); i++) {
+      DSPX_REQUIRE(B->beams[i]->have_fold, DSPX_E_STATE, "beam %zu has no fold plan", i);
+      DSPX_REQUIRE(!B->beams[i]->timing, DSPX_E_STATE, "timing is not captured");
+      ltf_reserve(B->beams[i]);
+    }
+    if (B->exec) {
+      DSPX_HIP(hipGraphExecDestroy(B->exec));
+      B->exec = nullptr;
+    }
+    if (B->graph) {
+      DSPX_HIP(hipGraphDestroy(B->graph));
+      B->graph = nullptr;
+    }
+    DSPX_HIP(hipStreamSynchronize(ctx->stream));
+    DSPX_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    int rc = DSPX_OK;
+    std::string err;
+    for (size_t i = 0; i < B->beams.size() && rc == DSPX_OK; i++) {
+      rc = dspx_ltf_execute(B->beams[i], d_raw[i], d_prof[i], d_hits ? d_hits[i] : nullptr);
+      if (rc != DSPX_OK) err = ctx->last_error;
+    }
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+    DSPX_REQUIRE(rc == DSPX_OK, rc, "capture: %s", err.c_str());
+    DSPX_HIP(e);
+    B->graph = g;
+    DSPX_HIP(hipGraphInstantiate(&B->exec, B->graph, nullptr, nullptr, 0));
+  });
+}
+
+int dspx_batch_replay(dspx_batch* B)
+{
+  dspx_ctx* ctx = B ? B->ctx : nullptr;
+  return guard(ctx, [&] {
+    DSPX_REQUIRE(B, DSPX_E_PARAM, "null argument");
+    DSPX_REQUIRE(B->exec, DSPX_E_STATE, "nothing captured (dspx_batch_capture)");
+    DSPX_HIP(hipGraphLaunch(B->exec, ctx->stream));
   });
 }
 

@@ -277,6 +277,43 @@ int dspx_ltf_filterbank(dspx_ltf* ltf, const void* d_raw, float* d_out);
 int dspx_ltf_set_timing(dspx_ltf* ltf, int enable);
 int dspx_ltf_stage_times(dspx_ltf* ltf, float* ms, const char** names, int cap, int* n);
 
+/* ------------------------------------------------------------------- batch */
+/* Many beams of uneven shape (channel count, DM, band, FFT length, 8/16-bit
+   input, fold bins) through the fused LoadToFold path as one unit of work:
+   one plan per beam, one set of large scratch buffers shared by all (sized
+   for the largest beam), the whole sequence captured once into a HIP graph
+   and replayed.  No reference counterpart: dspsr runs one LoadToFold
+   pipeline per beam (LoadToFold1.C:109-258); the FFT-length choice replaces
+   the cost model of optimal_fft_length (optimize_fft.c:64-135, used by
+   Response::set_optimal_ndat, Response.C:553-588) when autotuned. */
+typedef struct dspx_batch dspx_batch;
+
+enum { DSPX_BATCH_AUTOTUNE = 1 };
+
+/* beams[i] as for dspx_ltf_create (fused form).  A beam with nfft = 0 gets
+   its FFT length and overlap from the reference's rule (raised to 4096, the
+   fused form's shortest transform, where the rule gives less), or, with
+   DSPX_BATCH_AUTOTUNE, the fastest per output sample of the reference's
+   length and the next two powers of two (timed on the device here). */
+int dspx_batch_create(dspx_ctx* ctx, const dspx_ltf_desc* beams, uint32_t nbeam, uint32_t flags,
+                      dspx_batch** out);
+void dspx_batch_destroy(dspx_batch* batch);
+/* The plan of beam ibeam: its descriptor with the chosen nfft / overlap,
+   FFT parts and output samples per block. */
+int dspx_batch_beam(const dspx_batch* batch, uint32_t ibeam, dspx_ltf_desc* desc, uint64_t* npart,
+                    uint64_t* ndat_out);
+/* As dspx_ltf_set_fold for beam ibeam (before capture). */
+int dspx_batch_set_fold(dspx_batch* batch, uint32_t ibeam, double phase_per_sample, const dspx_fold_chunk* chunks,
+                        uint32_t nchunk);
+/* Every beam once, in order: d_raw[i] -> d_prof[i], d_hits[i] (d_hits may be
+   NULL), as dspx_ltf_execute. */
+int dspx_batch_execute(dspx_batch* batch, const void* const* d_raw, float* const* d_prof, uint32_t* const* d_hits);
+/* Captures dspx_batch_execute on these buffers into a graph (the buffers are
+   baked in; the context's stream must not be used by other work meanwhile). */
+int dspx_batch_capture(dspx_batch* batch, const void* const* d_raw, float* const* d_prof, uint32_t* const* d_hits);
+/* Replays the captured graph on the context's stream. */
+int dspx_batch_replay(dspx_batch* batch);
+
 #ifdef __cplusplus
 }
 #endif
