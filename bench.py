@@ -5,7 +5,7 @@ One step = one block of the 128-channel workload through the whole path
 bins x 8 sub-integrations), raw int8 input resident in HBM before timing.
 Prints one JSON line (see the keys below).
 
-  python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload multichan|single]
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload multichan|single|uneven]
   python bench.py --copy-probe MIB   # a plain device copy of MIB MiB, for
                                      # calibrating the HBM byte counters
 
@@ -116,6 +116,135 @@ def cpu_baseline(w, geom, raw_np, budget_s=20.0):
                                                                      t_total)}
 
 
+def draw_uneven(dspx, nbeam, seed=3, ndat_max_log2=22.0):
+    """The uneven-batch workload: beams with nchan from {32, 64, 128, 256,
+    512}, DM log-uniform 5-1500, centre frequency uniform 300-1500 MHz, band
+    min(0.25*fc, 400) MHz, 2^U(18, 22) samples per channel (multiple of 64),
+    8- or 16-bit, period log-uniform 5-500 ms, duty 1-5%, 512/1024/2048 bins.
+    Draws whose overlap exceeds a quarter of the block, whose smearing the
+    reference rule rejects, or whose reference FFT length exceeds 2^20 (the
+    fused form's longest) are drawn again."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    beams = []
+    while len(beams) < nbeam:
+        nchan = int(rng.choice([32, 64, 128, 256, 512]))
+        dm = float(np.exp(rng.uniform(np.log(5.0), np.log(1500.0))))
+        fc = float(rng.uniform(300.0, 1500.0))
+        bw = min(0.25 * fc, 400.0)
+        ndat = int(2 ** rng.uniform(18.0, ndat_max_log2)) // 64 * 64
+        nbit = int(rng.choice([8, 16]))
+        period = float(np.exp(rng.uniform(np.log(0.005), np.log(0.5))))
+        duty = float(rng.uniform(0.01, 0.05))
+        nbin = int(rng.choice([512, 1024, 2048]))
+        try:
+            g = dspx.geometry(fc, bw, nchan, dm)
+        except dspx.DspxError:
+            continue
+        if g["noverlap"] > ndat / 4 or g["nfft"] > (1 << 20):
+            continue
+        beams.append(dict(nchan=nchan, npol=2, ndat=ndat, cfreq=fc, bw=bw, dm=dm, nbit=nbit, period=period,
+                          duty=duty, nbin=nbin, geom=g))
+    return beams
+
+
+def run_uneven(args, dspx, synth, ctx):
+    """Fused batch (autotuned FFT lengths, replayed from a captured graph)
+    against the plain FFT-library form run beam after beam, on the same
+    synthetic beams.  Full Stokes, one sub-integration per beam."""
+    import numpy as np
+    import torch
+    beams = draw_uneven(dspx, args.nbeam, ndat_max_log2=args.uneven_log2)
+    nsamp = sum(b["nchan"] * b["npol"] * b["ndat"] for b in beams)
+    print("[bench] %d beams, %.3g samples, raw %.1f GB" % (
+        len(beams), nsamp, sum(b["nchan"] * b["npol"] * b["ndat"] * 2 * b["nbit"] // 8 for b in beams) / 1e9),
+        file=sys.stderr, flush=True)
+    raws = []
+    for i, b in enumerate(beams):
+        raws.append(synth.make_baseband_fast_torch(b["nchan"], b["npol"], b["ndat"], b["cfreq"], b["bw"], b["dm"],
+                                                   b["period"], b["duty"], 0.2,
+                                                   sigma=10.0 if b["nbit"] == 8 else 300.0, nbit=b["nbit"],
+                                                   seed=100 + i))
+    torch.cuda.synchronize()
+    print("[bench] input generated", file=sys.stderr, flush=True)
+    descs = [dspx.ltf_desc(b["nbit"], b["nchan"], b["npol"], b["ndat"], b["cfreq"], b["bw"], b["dm"], b["nbin"],
+                           state=dspx.STOKES, ndim=4) for b in beams]
+    t0 = time.perf_counter()
+    B = dspx.Batch(ctx, descs, autotune=True)
+    tune_s = time.perf_counter() - t0
+    profs, hits = [], []
+    for i, b in enumerate(beams):
+        rate = abs(b["bw"]) / b["nchan"] * 1e6
+        info = B.beams[i]
+        B.set_fold(i, (1.0 / rate) / b["period"], dspx.fold_chunks(info["ndat_out"], rate, b["period"],
+                                                                  info["nfilt_pos"]))
+        profs.append(torch.zeros((1, b["nchan"], 4 * b["nbin"]), dtype=torch.float32, device="cuda"))
+        hits.append(torch.zeros((1, b["nbin"]), dtype=torch.int32, device="cuda"))
+    B.execute(raws, profs, hits)
+    B.capture(raws, profs, hits)
+    for _ in range(args.warmup):
+        B.replay()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        B.replay()
+    torch.cuda.synchronize()
+    t_f = (time.perf_counter() - t0) / args.steps
+    print("[bench] fused batch %.3f ms/step (autotune %.1f s)" % (t_f * 1e3, tune_s), file=sys.stderr, flush=True)
+    # algorithmic bytes of the path as run: every FFT window's raw samples
+    # plus the three 8-byte passes (A write, B read+write, C read)
+    abytes = 0
+    for i, b in enumerate(beams):
+        info = B.beams[i]
+        n = b["nchan"] * b["npol"] * info["npart"] * info["nfft"]
+        abytes += n * (2 * b["nbit"] / 8 + 32)
+    nfft_tuned = [(B.beams[i]["nfft"], b["geom"]["nfft"]) for i, b in enumerate(beams)]
+    B.close()
+    del profs, hits
+    torch.cuda.empty_cache()
+    t_p = None
+    if not args.no_plain:
+        t_p = 0.0
+        for i, b in enumerate(beams):
+            L = dspx.LoadToFold(ctx, b["nbit"], b["nchan"], b["npol"], b["ndat"], b["cfreq"], b["bw"], b["dm"],
+                                b["nbin"], state=dspx.STOKES, ndim=4, form=dspx.FORM_PLAIN, geom=b["geom"])
+            rate = abs(b["bw"]) / b["nchan"] * 1e6
+            L.set_fold((1.0 / rate) / b["period"], dspx.fold_chunks(L.ndat_out, rate, b["period"],
+                                                                   b["geom"]["impulse_pos"]))
+            prof = torch.zeros((1, b["nchan"], 4 * b["nbin"]), dtype=torch.float32, device="cuda")
+            hit = torch.zeros((1, b["nbin"]), dtype=torch.int32, device="cuda")
+            L.execute(raws[i], prof, hit)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            L.execute(raws[i], prof, hit)
+            torch.cuda.synchronize()
+            t_p += time.perf_counter() - t0
+            L.close()
+            del prof, hit
+            torch.cuda.empty_cache()
+        print("[bench] plain beam by beam %.3f ms" % (t_p * 1e3), file=sys.stderr, flush=True)
+    value = nsamp / t_f / 1e6
+    out = {"metric": METRIC, "value": round(value, 2), "unit": "Msamples/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": round(t_f * 1e3, 3), "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f32",
+           "data": "synthetic (noise plus a pulse dispersed between channels, not within them)",
+           "config": {"workload": "uneven: %d beams, nchan %s, 2^18-2^%g samples per channel, int8/int16, "
+                                  "full Stokes, 512-2048 bins" % (len(beams), sorted(set(b["nchan"] for b in beams)),
+                                                                  args.uneven_log2),
+                      "samples_per_step": nsamp, "form": "fused batch, autotuned nfft, graph replay",
+                      "nfft_tuned_vs_reference": nfft_tuned},
+           "roofline": {"bound": "hbm", "kernel": "whole batch (graph replay)",
+                        "achieved": round(abytes / t_f / 1e9, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": round(abytes / t_f / 1e9 / HBM_PEAK_GBS, 4), "traffic": None,
+                        "algorithmic_bytes_per_step": int(abytes)},
+           "autotune_seconds": round(tune_s, 2)}
+    if t_p is not None:
+        out["plain_fp32"] = {"value": round(nsamp / t_p / 1e6, 2), "ms_per_step": round(t_p * 1e3, 3),
+                             "form": "FFT-library form, beam after beam, reference FFT lengths"}
+        out["fused_vs_plain"] = round(t_p / t_f, 3)
+    print(json.dumps(out))
+
+
 def load_traffic(workload, kernel):
     """HBM bytes per launch of ``kernel`` from the committed counter summary."""
     path = os.path.join(ROOT, "profiles", "traffic_%s.json" % workload)
@@ -146,7 +275,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="multichan", choices=list(WORKLOADS))
+    ap.add_argument("--workload", default="multichan", choices=list(WORKLOADS) + ["uneven"])
+    ap.add_argument("--nbeam", type=int, default=48)
+    ap.add_argument("--uneven-log2", type=float, default=22.0, help="largest log2 samples per channel (uneven)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
@@ -164,13 +295,17 @@ def main():
     import torch
     dspx = importlib.import_module("dspsr-kanyixia_amd")
     synth = importlib.import_module("dspsr-kanyixia_amd.synth")
-    w = WORKLOADS[args.workload]
     torch.cuda.init()
     # torch's fills and the library's kernels on one stream (the default
     # stream's handle is 0, for which the library would create its own)
     stream = torch.cuda.Stream()
     torch.cuda.set_stream(stream)
     ctx = dspx.Context(0, stream.cuda_stream)
+    if args.workload == "uneven":
+        run_uneven(args, dspx, synth, ctx)
+        ctx.close()
+        return
+    w = WORKLOADS[args.workload]
     geom = dspx.geometry(w["cfreq"], w["bw"], w["nchan"], w["dm"])
     print("[bench] workload %s geometry %s" % (args.workload, geom), file=sys.stderr, flush=True)
     raw = synth.make_baseband_torch(w["nchan"], w["npol"], w["ndat"], w["cfreq"], w["bw"], w["dm"], w["period"],
