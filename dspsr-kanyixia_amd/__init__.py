@@ -105,6 +105,17 @@ _SIGS = {
     "dspx_batch_execute": (I32, [P, P, P, P]),
     "dspx_batch_capture": (I32, [P, P, P, P]),
     "dspx_batch_replay": (I32, [P]),
+    "dspx_rfinet_nparam": (U32, []),
+    "dspx_rfinet_create": (I32, [P, U64, C.POINTER(P)]),
+    "dspx_rfinet_destroy": (None, [P]),
+    "dspx_rfinet_get_params": (I32, [P, P, I32]),
+    "dspx_rfinet_set_params": (I32, [P, P]),
+    "dspx_rfi_features": (I32, [P, P, P, U64, U32, U32, U32, P, P]),
+    "dspx_rfinet_forward": (I32, [P, P, U64, U32, P]),
+    "dspx_rfinet_backward": (I32, [P, P, P, U64, U32, C.c_float, P]),
+    "dspx_rfinet_grad": (I32, [P, P]),
+    "dspx_rfinet_adam": (I32, [P, C.c_float, C.c_float, C.c_float, C.c_float, U32]),
+    "dspx_rfinet_propose": (I32, [P, P, U64, C.c_float, P]),
     "dspx_ltf_create": (I32, [P, C.POINTER(LtfDesc), C.POINTER(P)]),
     "dspx_ltf_destroy": (None, [P]),
     "dspx_ltf_geometry": (I32, [P, C.POINTER(U64), C.POINTER(U64)]),
@@ -492,3 +503,65 @@ class Batch:
 
     def replay(self):
         self.ctx.check(lib().dspx_batch_replay(self.h))
+
+
+def rfi_features(ctx, sums, counts, nblk, nchan, npol, out, Nd=1):
+    """Features [nblk][4][nchan] (device tensor out) from SK sums and counts."""
+    import torch
+    scratch = torch.empty(3 * nblk * nchan + nchan, dtype=torch.float32, device="cuda")
+    ctx.check(lib().dspx_rfi_features(ctx.h, ptr(sums), ptr(counts), nblk, nchan, npol, Nd, ptr(scratch), ptr(out)))
+    return out
+
+
+class RfiNet:
+    """dspx_rfinet: the RFI proposal network (forward, backward, Adam)."""
+
+    def __init__(self, ctx, seed=1):
+        self.ctx = ctx
+        self.h = None
+        self._lib = lib()
+        h = P()
+        ctx.check(lib().dspx_rfinet_create(ctx.h, seed, C.byref(h)))
+        self.h = h
+        _register(self, False)
+        self.nparam = int(lib().dspx_rfinet_nparam())
+        self.t = 0
+
+    def close(self):
+        if self.h:
+            self._lib.dspx_rfinet_destroy(self.h)
+            self.h = None
+
+    __del__ = close
+
+    def params(self, rounded=False):
+        import numpy as np
+        out = np.zeros(self.nparam, np.float32)
+        self.ctx.check(lib().dspx_rfinet_get_params(self.h, out.ctypes.data, int(rounded)))
+        return out
+
+    def set_params(self, w):
+        import numpy as np
+        w = np.ascontiguousarray(w, np.float32)
+        assert w.size == self.nparam
+        self.ctx.check(lib().dspx_rfinet_set_params(self.h, w.ctypes.data))
+
+    def forward(self, feat, nblk, nchan, logits):
+        self.ctx.check(lib().dspx_rfinet_forward(self.h, ptr(feat), nblk, nchan, ptr(logits)))
+
+    def backward(self, feat, labels, nblk, nchan, pos_weight=1.0, loss=None):
+        self.ctx.check(lib().dspx_rfinet_backward(self.h, ptr(feat), ptr(labels), nblk, nchan, pos_weight,
+                                                  ptr(loss) if loss is not None else None))
+
+    def grad(self):
+        import numpy as np
+        out = np.zeros(self.nparam, np.float32)
+        self.ctx.check(lib().dspx_rfinet_grad(self.h, out.ctypes.data))
+        return out
+
+    def adam(self, lr=1e-3, beta1=0.9, beta2=0.999, eps=1e-8):
+        self.t += 1
+        self.ctx.check(lib().dspx_rfinet_adam(self.h, lr, beta1, beta2, eps, self.t))
+
+    def propose(self, logits, n, mask, threshold=0.0):
+        self.ctx.check(lib().dspx_rfinet_propose(self.ctx.h, ptr(logits), n, threshold, ptr(mask)))

@@ -150,3 +150,63 @@ def make_baseband_fast_torch(nchan, npol, ndat, cfreq, bw, dm, period, duty, pea
         v += torch.randn((T, nchan, npol, 2), generator=g, device=device) * env[:, :, None, None]
         out[t0:t0 + T] = torch.clamp(torch.round(v), -lim, lim).to(dt)
     return out
+
+
+def make_rfi_baseband_torch(nchan, npol, ndat, M, seed, sigma=10.0, bw=256.0, tone_frac=0.05,
+                            impulse_frac=0.01, am_frac=0.01, device="cuda", chunk=1 << 15):
+    """Complex Gaussian noise (sigma per component, quantised to int8, TFP
+    [ndat][nchan][npol][2]) with injected interference: constant-amplitude
+    tones (2-20 sigma) filling tone_frac of the channels, broadband impulses
+    (5-50 sigma, 1-64 samples) in impulse_frac of the M-sample blocks, and
+    50 Hz amplitude modulation (depth 0.5-1) in am_frac of the channels; the
+    channel sample rate is bw/nchan MHz.  Returns (raw, truth) with truth
+    [nblk][nchan] bool marking the blocks that hold injected interference."""
+    import torch
+    rng = np.random.default_rng(seed)
+    g = torch.Generator(device=device)
+    g.manual_seed(seed)
+    nblk = ndat // M
+    rate = bw / nchan * 1e6
+    chans = rng.permutation(nchan)
+    ntone = max(1, int(round(tone_frac * nchan)))
+    nam = max(1, int(round(am_frac * nchan)))
+    tone_ch, am_ch = chans[:ntone], chans[ntone:ntone + nam]
+    tone_amp = rng.uniform(2, 20, ntone) * sigma
+    tone_f = rng.uniform(-0.5, 0.5, ntone)
+    tone_ph = rng.uniform(0, 2 * np.pi, (ntone, npol))
+    am_depth = rng.uniform(0.5, 1.0, nam)
+    am_ph = rng.uniform(0, 2 * np.pi, nam)
+    nimp = max(1, int(round(impulse_frac * nblk)))
+    imp_blk = np.sort(rng.choice(nblk, nimp, replace=False))
+    imp_len = rng.integers(1, 65, nimp)
+    imp_amp = rng.uniform(5, 50, nimp) * sigma
+    imp_t0 = imp_blk * M + rng.integers(0, np.maximum(M - imp_len, 0) + 1)
+    truth = np.zeros((nblk, nchan), bool)
+    truth[:, tone_ch] = True
+    truth[:, am_ch] = True
+    truth[imp_blk, :] = True
+    out = torch.empty((ndat, nchan, npol, 2), dtype=torch.int8, device=device)
+    tch = torch.as_tensor(tone_ch, device=device)
+    ach = torch.as_tensor(am_ch, device=device)
+    tamp = torch.as_tensor(tone_amp, dtype=torch.float32, device=device)
+    tf = torch.as_tensor(tone_f, dtype=torch.float64, device=device)
+    tph = torch.as_tensor(tone_ph, dtype=torch.float64, device=device)
+    adep = torch.as_tensor(am_depth, dtype=torch.float32, device=device)
+    aph = torch.as_tensor(am_ph, dtype=torch.float64, device=device)
+    for t0 in range(0, ndat, chunk):
+        T = min(chunk, ndat - t0)
+        t = torch.arange(t0, t0 + T, device=device, dtype=torch.float64)
+        v = torch.randn((T, nchan, npol, 2), generator=g, device=device) * sigma
+        mod = 1 + adep[None, :] * torch.sin(2 * np.pi * 50.0 * t[:, None] / rate + aph[None, :]).float()
+        v[:, ach] *= mod[:, :, None, None]
+        ph = 2 * np.pi * tf[None, :, None] * t[:, None, None] + tph[None, :, :]  # [T][ntone][npol]
+        v[:, tch, :, 0] += tamp[None, :, None] * torch.cos(ph).float()
+        v[:, tch, :, 1] += tamp[None, :, None] * torch.sin(ph).float()
+        for j in range(nimp):
+            a, b = int(imp_t0[j]), int(imp_t0[j] + imp_len[j])
+            lo, hi = max(a, t0), min(b, t0 + T)
+            if lo < hi:
+                v[lo - t0:hi - t0] += torch.randn((hi - lo, nchan, npol, 2), generator=g, device=device) * float(
+                    imp_amp[j] / np.sqrt(2.0))
+        out[t0:t0 + T] = torch.clamp(torch.round(v), -127, 127).to(torch.int8)
+    return out, truth

@@ -314,6 +314,44 @@ int dspx_batch_capture(dspx_batch* batch, const void* const* d_raw, float* const
 /* Replays the captured graph on the context's stream. */
 int dspx_batch_replay(dspx_batch* batch);
 
+/* ------------------------------------------------------- RFI proposal net */
+/* No reference counterpart: a learned proposal of RFI-affected blocks ahead
+   of the spectral kurtosis detector (which stays the judge and the
+   fallback).  Features per (block, channel) from the SK sums; a 4-layer 1-D
+   convolutional classifier across frequency channels (4 -> 32 -> 32 -> 32
+   -> 1, 19 taps, ReLU), BF16-rounded weights with FP32 accumulation and an
+   FP32 master copy; class-weighted logistic loss; Adam.  Features and
+   logits are [nblk][nfeat][nchan] and [nblk][nchan] floats. */
+typedef struct dspx_rfinet dspx_rfinet;
+
+/* Number of parameters (weights W[co][ci][k] then biases, layer by layer). */
+uint32_t dspx_rfinet_nparam(void);
+/* He-uniform initial weights from seed, zero biases. */
+int dspx_rfinet_create(dspx_ctx* ctx, uint64_t seed, dspx_rfinet** out);
+void dspx_rfinet_destroy(dspx_rfinet* net);
+/* Host copies of the FP32 master weights (rounded = 0) or of the BF16-rounded
+   weights the passes use (rounded = 1).  Synchronous. */
+int dspx_rfinet_get_params(dspx_rfinet* net, float* host, int rounded);
+int dspx_rfinet_set_params(dspx_rfinet* net, const float* host);
+/* Features [nblk][4][nchan] from SK sums [nblk][nchan][npol][2] and counts:
+   log mean power minus its channel mean over the blocks, the mean and the
+   largest |SK - 1| of the pols, the change of log power since the previous
+   block.  d_scratch: 3*nblk*nchan + nchan floats. */
+int dspx_rfi_features(dspx_ctx* ctx, const float* d_sums, const uint32_t* d_counts, uint64_t nblk, uint32_t nchan,
+                      uint32_t npol, uint32_t Nd, float* d_scratch, float* d_feat);
+int dspx_rfinet_forward(dspx_rfinet* net, const float* d_feat, uint64_t nblk, uint32_t nchan, float* d_logits);
+/* Forward, loss (written to d_loss if not NULL) and the gradient of the mean
+   class-weighted logistic loss against labels [nblk][nchan] (bytes, 1 =
+   flagged) with respect to every parameter (kept on the device). */
+int dspx_rfinet_backward(dspx_rfinet* net, const float* d_feat, const uint8_t* d_label, uint64_t nblk,
+                         uint32_t nchan, float pos_weight, float* d_loss);
+/* Host copy of the last gradient.  Synchronous. */
+int dspx_rfinet_grad(dspx_rfinet* net, float* host);
+/* One Adam step (step number t >= 1) with the last gradient. */
+int dspx_rfinet_adam(dspx_rfinet* net, float lr, float beta1, float beta2, float eps, uint32_t t);
+/* mask[i] = logits[i] > threshold */
+int dspx_rfinet_propose(dspx_ctx* ctx, const float* d_logits, uint64_t n, float threshold, uint8_t* d_mask);
+
 #ifdef __cplusplus
 }
 #endif
