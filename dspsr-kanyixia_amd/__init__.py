@@ -60,6 +60,11 @@ class LtfDesc(C.Structure):
                 ("form", C.c_uint32)]
 
 
+class FbDesc(C.Structure):
+    _fields_ = [("nchan", C.c_uint32), ("npol", C.c_uint32), ("nsub", C.c_uint32), ("fres", C.c_uint32),
+                ("nfilt_pos", C.c_uint32), ("nfilt_neg", C.c_uint32), ("reserved", C.c_uint32 * 2)]
+
+
 class SkDesc(C.Structure):
     _fields_ = [("nchan", C.c_uint32), ("npol", C.c_uint32), ("M", C.c_uint32), ("Nd", C.c_uint32),
                 ("omit_outliers", C.c_uint32), ("reserved", C.c_uint32)]
@@ -83,6 +88,11 @@ _SIGS = {
     "dspx_conv_destroy": (None, [P]),
     "dspx_conv_set_response": (I32, [P, P]),
     "dspx_conv_perform": (I32, [P, P, U64, U64, U64, P, U64, U64, P, U32]),
+    "dspx_fb_create": (I32, [P, C.POINTER(FbDesc), C.POINTER(P)]),
+    "dspx_fb_destroy": (None, [P]),
+    "dspx_fb_set_response": (I32, [P, P]),
+    "dspx_fb_geometry": (I32, [P, U64, C.POINTER(U64), C.POINTER(U64)]),
+    "dspx_fb_perform": (I32, [P, P, U64, U64, U64, P, U64, U64]),
     "dspx_detect": (I32, [P, U32, U32, P, U64, U64, U32, U32, U64, P, U64, U64]),
     "dspx_fold_create": (I32, [P, U64, D, U32, C.POINTER(FoldChunk), U32, C.POINTER(P)]),
     "dspx_fold_destroy": (None, [P]),
@@ -122,6 +132,10 @@ _SIGS = {
     "dspx_ltf_set_fold": (I32, [P, D, C.POINTER(FoldChunk), U32]),
     "dspx_ltf_execute": (I32, [P, P, P, P]),
     "dspx_ltf_filterbank": (I32, [P, P, P]),
+    "dspx_ltf_stream_reset": (I32, [P]),
+    "dspx_ltf_stream_next": (I32, [P, C.POINTER(U64), C.POINTER(U64), C.POINTER(U64)]),
+    "dspx_ltf_stream_execute": (I32, [P, P, P, P]),
+    "dspx_ltf_stream_filterbank": (I32, [P, P, P]),
     "dspx_ltf_set_timing": (I32, [P, I32]),
     "dspx_ltf_stage_times": (I32, [P, P, P, I32, C.POINTER(I32)]),
 }
@@ -318,6 +332,42 @@ class Fold:
                                                ndim, ptr(prof)))
 
 
+class Filterbank:
+    """dspx_fb: the convolving filterbank engine (FFT-library form)."""
+
+    def __init__(self, ctx, nchan, npol, nsub, fres, nfilt_pos=0, nfilt_neg=0):
+        self.ctx = ctx
+        self.h = None
+        self._lib = lib()
+        h = P()
+        ctx.check(lib().dspx_fb_create(ctx.h, C.byref(FbDesc(nchan, npol, nsub, fres, nfilt_pos, nfilt_neg)),
+                                       C.byref(h)))
+        self.h = h
+        _register(self, False)
+        self.nchan, self.npol, self.nsub, self.fres = nchan, npol, nsub, fres
+
+    def close(self):
+        if self.h:
+            self._lib.dspx_fb_destroy(self.h)
+            self.h = None
+
+    __del__ = close
+
+    def set_response(self, resp):
+        import numpy as np
+        r = np.ascontiguousarray(resp, np.complex64)
+        assert r.size == self.nchan * self.nsub * self.fres
+        self.ctx.check(lib().dspx_fb_set_response(self.h, r.ctypes.data))
+
+    def geometry(self, ndat):
+        npart, nout = U64(), U64()
+        self.ctx.check(lib().dspx_fb_geometry(self.h, ndat, C.byref(npart), C.byref(nout)))
+        return npart.value, nout.value
+
+    def perform(self, x, ics, ips, ndat, out, ocs, ops):
+        self.ctx.check(lib().dspx_fb_perform(self.h, ptr(x), ics, ips, ndat, ptr(out), ocs, ops))
+
+
 class SpectralKurtosis:
     """dspx_sk: one resolution of SpectralKurtosis (compute / detect / mask).
     Tensors are device buffers: sums [npart][nchan][npol][2] float32, counts
@@ -406,6 +456,23 @@ class LoadToFold:
 
     def filterbank(self, raw, out):
         self.ctx.check(lib().dspx_ltf_filterbank(self.h, ptr(raw), ptr(out)))
+
+    def stream_reset(self):
+        self.ctx.check(lib().dspx_ltf_stream_reset(self.h))
+
+    def stream_next(self):
+        """(npart, ndat_out, t_first) of the next stream call; sets
+        self.npart / self.ndat_out for it."""
+        npart, nout, t = U64(), U64(), U64()
+        self.ctx.check(lib().dspx_ltf_stream_next(self.h, C.byref(npart), C.byref(nout), C.byref(t)))
+        self.npart, self.ndat_out = npart.value, nout.value
+        return npart.value, nout.value, t.value
+
+    def stream_execute(self, raw, prof, hits=None):
+        self.ctx.check(lib().dspx_ltf_stream_execute(self.h, ptr(raw), ptr(prof), ptr(hits)))
+
+    def stream_filterbank(self, raw, out):
+        self.ctx.check(lib().dspx_ltf_stream_filterbank(self.h, ptr(raw), ptr(out)))
 
     def set_timing(self, on=True):
         self.ctx.check(lib().dspx_ltf_set_timing(self.h, int(on)))

@@ -251,7 +251,7 @@ struct ConvCore {
 // input and of the output are both read / written as contiguous runs.
 template <class W>
 __global__ __launch_bounds__(256) void k_tfp_to_ftp(const W* __restrict__ in, uint64_t ndat, uint32_t nchan,
-                                                    W* __restrict__ out)
+                                                    W* __restrict__ out, uint64_t out_cstride, uint64_t out_off)
 {
   __shared__ W tile[64][65];
   const uint64_t t0 = uint64_t(blockIdx.x) * 64;
@@ -270,17 +270,52 @@ __global__ __launch_bounds__(256) void k_tfp_to_ftp(const W* __restrict__ in, ui
     const uint32_t i = ty + 4 * k;
     uint32_t c = c0 + i;
     uint64_t t = t0 + tx;
-    if (t < ndat && c < nchan) out[uint64_t(c) * ndat + t] = tile[tx][i];
+    if (t < ndat && c < nchan) out[uint64_t(c) * out_cstride + out_off + t] = tile[tx][i];
   }
 }
 
-void tfp_to_ftp(dspx_ctx* ctx, const void* in, uint64_t ndat, uint32_t nchan, uint32_t wbytes, void* out)
+// TFP words -> channel-major words: out[c * out_cstride + out_off + t] (in
+// words of wbytes, one sample of one channel, all pols).
+void tfp_to_ftp(dspx_ctx* ctx, const void* in, uint64_t ndat, uint32_t nchan, uint32_t wbytes, void* out,
+                uint64_t out_cstride, uint64_t out_off)
 {
+  DSPX_REQUIRE((ndat + 63) / 64 < (1ull << 31), DSPX_E_UNSUPPORTED, "ndat=%llu", (unsigned long long) ndat);
   dim3 grid(unsigned((ndat + 63) / 64), (nchan + 63) / 64);
   switch (wbytes) {
-    case 2: k_tfp_to_ftp<uint16_t><<<grid, 256, 0, ctx->stream>>>((const uint16_t*) in, ndat, nchan, (uint16_t*) out); break;
-    case 4: k_tfp_to_ftp<uint32_t><<<grid, 256, 0, ctx->stream>>>((const uint32_t*) in, ndat, nchan, (uint32_t*) out); break;
-    case 8: k_tfp_to_ftp<uint64_t><<<grid, 256, 0, ctx->stream>>>((const uint64_t*) in, ndat, nchan, (uint64_t*) out); break;
+#define CASE_W(b, T)                                                                                           \
+  case b:                                                                                                      \
+    k_tfp_to_ftp<T><<<grid, 256, 0, ctx->stream>>>((const T*) in, ndat, nchan, (T*) out, out_cstride, out_off); \
+    break;
+    CASE_W(2, uint16_t) CASE_W(4, uint32_t) CASE_W(8, uint64_t)
+#undef CASE_W
+    default: fail(DSPX_E_INTERNAL, "word size %u", wbytes);
+  }
+  DSPX_HIP(hipGetLastError());
+}
+
+// Per-channel copy of n words: dst[c * dcs + doff + t] = src[c * scs + soff + t].
+template <class W>
+__global__ void k_copy_chan(const W* __restrict__ src, uint64_t scs, uint64_t soff, W* __restrict__ dst, uint64_t dcs,
+                            uint64_t doff, uint64_t n)
+{
+  uint64_t t = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  uint32_t c = blockIdx.y;
+  if (t < n) dst[uint64_t(c) * dcs + doff + t] = src[uint64_t(c) * scs + soff + t];
+}
+
+void copy_chan(dspx_ctx* ctx, const void* src, uint64_t scs, uint64_t soff, void* dst, uint64_t dcs, uint64_t doff,
+               uint64_t n, uint32_t nchan, uint32_t wbytes)
+{
+  if (!n) return;
+  DSPX_REQUIRE((n + 255) / 256 < (1ull << 31) && nchan <= 65535, DSPX_E_UNSUPPORTED, "copy too large");
+  dim3 grid(unsigned((n + 255) / 256), nchan);
+  switch (wbytes) {
+#define CASE_W(b, T)                                                                                               \
+  case b:                                                                                                          \
+    k_copy_chan<T><<<grid, 256, 0, ctx->stream>>>((const T*) src, scs, soff, (T*) dst, dcs, doff, n); \
+    break;
+    CASE_W(2, uint16_t) CASE_W(4, uint32_t) CASE_W(8, uint64_t)
+#undef CASE_W
     default: fail(DSPX_E_INTERNAL, "word size %u", wbytes);
   }
   DSPX_HIP(hipGetLastError());
@@ -454,6 +489,13 @@ struct dspx_ltf {
   LtfScratch own;
   LtfScratch* ext = nullptr;
   LtfScratch& sc() { return ext ? *ext : own; }
+  // stream mode (InputBuffering): the unprocessed tail of the previous call
+  // (channel-major raw words) is prepended to the next block
+  bool streaming = false;
+  uint64_t ncarry = 0, consumed = 0;
+  uint64_t cap = 0;        // samples per channel row of the assembled input
+  uint64_t carry_cap = 0;  // samples per channel row of the carry buffer
+  DevBuf carry;
   // plain form intermediates
   DevBuf unpacked, filtered, detected;
   // timing
@@ -483,7 +525,7 @@ template <class F> void with_loader(dspx_ltf* L, const void* raw, F&& f)
   const void* src = raw;
   if (d.nchan > 1) {
     L->sc().ftp.reserve(d.ndat * d.nchan * wbytes);
-    tfp_to_ftp(L->ctx, raw, d.ndat, d.nchan, uint32_t(wbytes), L->sc().ftp.p);
+    tfp_to_ftp(L->ctx, raw, d.ndat, d.nchan, uint32_t(wbytes), L->sc().ftp.p, d.ndat, 0);
     src = L->sc().ftp.p;
     tstride = uint64_t(d.npol) * 2;
     cstride = d.ndat * d.npol * 2;
@@ -567,6 +609,46 @@ void ltf_reserve(dspx_ltf* L)
   if (L->have_fold) L->sc().partial2.reserve(uint64_t(d.nchan) * L->npart * L->nslot * L->nval * d.nbin * sizeof(float));
 }
 
+
+// Stages B and C, the profile reduction and the hits of a fold execute
+// (stage A has run on np parts).
+void fused_fold_after_a(dspx_ltf* L, uint32_t np, float* d_prof, uint32_t* d_hits, int& ev)
+{
+  dspx_ctx* ctx = L->ctx;
+  const auto& d = L->d;
+  ConvCore& k = L->core;
+  k.stage_b(np);
+  ltf_record(L, ev++);
+  L->stage_name[L->nstage++] = "stageB_rowfft_chirp";
+  FoldArgs fa;
+  fa.binplan = L->fold.plan.as<uint32_t>();
+  fa.partial = L->sc().partial.as<float>();
+  fa.sub_first = L->sub_first.as<uint32_t>();
+  fa.nbin = d.nbin;
+  fa.nval = L->nval;
+  fa.ndim = L->ndim_out;
+  fa.nslot = L->nslot;
+  fa.G = L->G;
+  fa.state = d.state;
+  fa.npol_out = L->npol_out;
+  for (uint32_t i = 0; i < 4; i++) fa.voff[i] = (i / L->ndim_out) * d.nbin * L->ndim_out + (i % L->ndim_out);
+  k.stage_c<kStageCFold>(np, FilterbankArgs{}, fa);
+  ltf_record(L, ev++);
+  L->stage_name[L->nstage++] = "stageC_colifft_detect_fold";
+  uint32_t hl = L->nval * d.nbin;
+  uint64_t n2 = uint64_t(d.nchan) * np * L->nslot * hl;
+  L->sc().partial2.reserve(n2 * sizeof(float));
+  k_fold_reduce_g<<<unsigned((n2 + 255) / 256), 256, 0, ctx->stream>>>(
+      L->sc().partial.as<float>(), L->G, L->nslot * hl, n2, L->sc().partial2.as<float>());
+  DSPX_HIP(hipGetLastError());
+  dim3 grid((hl + 255) / 256, d.nchan, L->fold.nsub);
+  k_fold_reduce_p<<<grid, 256, 0, ctx->stream>>>(L->sc().partial2.as<float>(), L->sub_first.as<uint32_t>(), np,
+                                                 L->nslot, hl, d.nchan, d_prof);
+  DSPX_HIP(hipGetLastError());
+  if (d_hits) L->fold.hits(d_hits);
+  ltf_record(L, ev++);
+  L->stage_name[L->nstage++] = "fold_reduce_hits";
+}
 }  // namespace
 
 extern "C" {
@@ -650,37 +732,7 @@ int dspx_ltf_execute(dspx_ltf* L, const void* d_raw, float* d_prof, uint32_t* d_
       with_loader(L, d_raw, [&](auto& ld) { k.stage_a(ld, np); });
       ltf_record(L, ev++);
       L->stage_name[L->nstage++] = "stageA_unpack_colfft";
-      k.stage_b(np);
-      ltf_record(L, ev++);
-      L->stage_name[L->nstage++] = "stageB_rowfft_chirp";
-      FoldArgs fa;
-      fa.binplan = L->fold.plan.as<uint32_t>();
-      fa.partial = L->sc().partial.as<float>();
-      fa.sub_first = L->sub_first.as<uint32_t>();
-      fa.nbin = d.nbin;
-      fa.nval = L->nval;
-      fa.ndim = L->ndim_out;
-      fa.nslot = L->nslot;
-      fa.G = L->G;
-      fa.state = d.state;
-      fa.npol_out = L->npol_out;
-      for (uint32_t i = 0; i < 4; i++) fa.voff[i] = (i / L->ndim_out) * d.nbin * L->ndim_out + (i % L->ndim_out);
-      k.stage_c<kStageCFold>(np, FilterbankArgs{}, fa);
-      ltf_record(L, ev++);
-      L->stage_name[L->nstage++] = "stageC_colifft_detect_fold";
-      uint32_t hl = L->nval * d.nbin;
-      uint64_t n2 = uint64_t(d.nchan) * np * L->nslot * hl;
-      L->sc().partial2.reserve(n2 * sizeof(float));
-      k_fold_reduce_g<<<unsigned((n2 + 255) / 256), 256, 0, ctx->stream>>>(
-          L->sc().partial.as<float>(), L->G, L->nslot * hl, n2, L->sc().partial2.as<float>());
-      DSPX_HIP(hipGetLastError());
-      dim3 grid((hl + 255) / 256, d.nchan, L->fold.nsub);
-      k_fold_reduce_p<<<grid, 256, 0, ctx->stream>>>(L->sc().partial2.as<float>(), L->sub_first.as<uint32_t>(), np,
-                                                     L->nslot, hl, d.nchan, d_prof);
-      DSPX_HIP(hipGetLastError());
-      if (d_hits) L->fold.hits(d_hits);
-      ltf_record(L, ev++);
-      L->stage_name[L->nstage++] = "fold_reduce_hits";
+      fused_fold_after_a(L, np, d_prof, d_hits, ev);
     } else {
       // plain: unpack -> hipFFT convolution -> detect -> fold, each a pass
       uint64_t nin = d.ndat, nout = L->ndat_out;
@@ -734,6 +786,144 @@ int dspx_ltf_filterbank(dspx_ltf* L, const void* d_raw, float* d_out)
       L->plain.perform(L->unpacked.as<float>(), uint64_t(d.npol) * nin * 2, nin * 2, d_out,
                        uint64_t(d.npol) * nout * 2, nout * 2, uint32_t(L->npart));
     }
+  });
+}
+
+// ------------------------------------------------------------ stream mode
+// InputBuffering (InputBuffering.C:66-180, TimeSeries::prepend,
+// TimeSeries.C:514-533): each call's input is the carried tail of the
+// previous call followed by d.ndat new samples; the parts that fit are
+// processed and the rest (the overlap and any remainder, fewer than
+// overlap + step samples) is carried on.  The input is assembled
+// channel-major in the transpose buffer (the carry copied to the head of each
+// channel's row, the new block transposed after it), so stage A reads one
+// contiguous row per channel as in a single block.
+
+int dspx_ltf_stream_reset(dspx_ltf* L)
+{
+  dspx_ctx* ctx = L ? L->ctx : nullptr;
+  return guard(ctx, [&] {
+    DSPX_REQUIRE(L, DSPX_E_PARAM, "null argument");
+    const auto& d = L->d;
+    DSPX_REQUIRE(d.form == DSPX_FORM_FUSED, DSPX_E_UNSUPPORTED, "stream mode runs the fused form");
+    DSPX_REQUIRE(d.ndat >= d.nfft, DSPX_E_PARAM, "stream blocks of %llu samples shorter than nfft=%u",
+                 (unsigned long long) d.ndat, d.nfft);
+    const uint64_t nov = uint64_t(d.nfilt_pos) + d.nfilt_neg;
+    L->carry_cap = nov + L->step;
+    L->cap = d.ndat + L->carry_cap;
+    const uint64_t npmax = (L->cap - nov) / L->step;
+    DSPX_REQUIRE(npmax < (1u << 20), DSPX_E_UNSUPPORTED, "npart too large");
+    L->core.reserve_work(uint32_t(npmax));
+    const uint64_t wbytes = uint64_t(d.npol) * 2 * d.nbit / 8;
+    L->sc().ftp.reserve(L->cap * d.nchan * wbytes);
+    L->carry.reserve(L->carry_cap * d.nchan * wbytes);
+    L->streaming = true;
+    L->ncarry = 0;
+    L->consumed = 0;
+    L->npart = 0;
+    L->ndat_out = 0;
+    L->have_fold = false;
+  });
+}
+
+int dspx_ltf_stream_next(dspx_ltf* L, uint64_t* npart, uint64_t* ndat_out, uint64_t* t_first)
+{
+  dspx_ctx* ctx = L ? L->ctx : nullptr;
+  return guard(ctx, [&] {
+    DSPX_REQUIRE(L && npart && ndat_out && t_first, DSPX_E_PARAM, "null argument");
+    DSPX_REQUIRE(L->streaming, DSPX_E_STATE, "not streaming (dspx_ltf_stream_reset)");
+    const auto& d = L->d;
+    const uint64_t nov = uint64_t(d.nfilt_pos) + d.nfilt_neg;
+    const uint64_t ntot = L->ncarry + d.ndat;
+    L->npart = ntot >= d.nfft ? (ntot - nov) / L->step : 0;
+    L->ndat_out = L->npart * L->step;
+    L->have_fold = false;  // the fold plan is per call
+    *npart = L->npart;
+    *ndat_out = L->ndat_out;
+    *t_first = L->consumed + d.nfilt_pos;  // Convolution.C:611
+  });
+}
+
+}  // extern "C"
+
+namespace {
+
+// Assembles carry + block, runs f(loader) for stage A, then (after the caller's
+// remaining stages, run inside f's scope by the caller) updates the carry.
+template <class F> void stream_call(dspx_ltf* L, const void* raw, F&& f)
+{
+  dspx_ctx* ctx = L->ctx;
+  const auto& d = L->d;
+  DSPX_REQUIRE(L->streaming, DSPX_E_STATE, "not streaming (dspx_ltf_stream_reset)");
+  const uint64_t nov = uint64_t(d.nfilt_pos) + d.nfilt_neg;
+  const uint64_t ntot = L->ncarry + d.ndat;
+  const uint64_t np = ntot >= d.nfft ? (ntot - nov) / L->step : 0;
+  DSPX_REQUIRE(np == L->npart, DSPX_E_STATE, "call dspx_ltf_stream_next before each stream call");
+  const uint32_t wbytes = uint32_t(uint64_t(d.npol) * 2 * d.nbit / 8);
+  void* ftp = L->sc().ftp.p;
+  copy_chan(ctx, L->carry.p, L->carry_cap, 0, ftp, L->cap, 0, L->ncarry, d.nchan, wbytes);
+  if (d.nchan > 1) {
+    tfp_to_ftp(ctx, raw, d.ndat, d.nchan, wbytes, ftp, L->cap, L->ncarry);
+  } else {
+    DSPX_HIP(hipMemcpyAsync(static_cast<char*>(ftp) + L->ncarry * wbytes, raw, d.ndat * wbytes,
+                            hipMemcpyDeviceToDevice, ctx->stream));
+  }
+  const uint64_t tstride = uint64_t(d.npol) * 2, cstride = L->cap * d.npol * 2;  // in elements
+  if (d.nbit == 8) {
+    LoadI8 ld{static_cast<const uint8_t*>(ftp), tstride, cstride, d.npol, d.offset_binary, d.scale};
+    f(ld, uint32_t(np));
+  } else {
+    LoadI16 ld{static_cast<const int16_t*>(ftp), tstride, cstride, d.npol, d.offset_binary};
+    f(ld, uint32_t(np));
+  }
+  const uint64_t done = np * L->step;
+  const uint64_t rem = ntot - done;
+  DSPX_REQUIRE(rem <= L->carry_cap, DSPX_E_INTERNAL, "carry of %llu samples", (unsigned long long) rem);
+  copy_chan(ctx, ftp, L->cap, done, L->carry.p, L->carry_cap, 0, rem, d.nchan, wbytes);
+  L->ncarry = rem;
+  L->consumed += done;
+}
+
+}  // namespace
+
+extern "C" {
+
+int dspx_ltf_stream_execute(dspx_ltf* L, const void* d_raw, float* d_prof, uint32_t* d_hits)
+{
+  dspx_ctx* ctx = L ? L->ctx : nullptr;
+  return guard(ctx, [&] {
+    DSPX_REQUIRE(L && d_raw && d_prof, DSPX_E_PARAM, "null argument");
+    DSPX_REQUIRE(L->npart == 0 || L->have_fold, DSPX_E_STATE, "no fold plan for this call: dspx_ltf_set_fold");
+    L->nstage = 0;
+    L->nstage_open = false;
+    int ev = 0;
+    stream_call(L, d_raw, [&](auto& ld, uint32_t np) {
+      if (!np) return;
+      ltf_record(L, ev++);
+      L->core.stage_a(ld, np);
+      ltf_record(L, ev++);
+      L->stage_name[L->nstage++] = "stageA_unpack_colfft";
+      fused_fold_after_a(L, np, d_prof, d_hits, ev);
+    });
+  });
+}
+
+int dspx_ltf_stream_filterbank(dspx_ltf* L, const void* d_raw, float* d_out)
+{
+  dspx_ctx* ctx = L ? L->ctx : nullptr;
+  return guard(ctx, [&] {
+    DSPX_REQUIRE(L && d_raw && d_out, DSPX_E_PARAM, "null argument");
+    const auto& d = L->d;
+    L->nstage = 0;
+    L->nstage_open = false;
+    stream_call(L, d_raw, [&](auto& ld, uint32_t np) {
+      if (!np) return;
+      const uint64_t nout = uint64_t(np) * L->step;
+      L->core.stage_a(ld, np);
+      L->core.stage_b(np);
+      FilterbankArgs fb{d_out, uint64_t(d.npol) * nout * 2, nout * 2};
+      L->core.stage_c<kStageCFilterbank>(np, fb, FoldArgs{});
+    });
   });
 }
 

@@ -1,0 +1,199 @@
+// fb.hip -- convolving filterbank engine, FFT-library form.
+//
+// Reference: Filterbank::prepare / resize_output (Filterbank.C:75-200,
+// 405-460) and FilterbankEngineCPU::perform (FilterbankEngineCPU.C:184-370):
+// per input channel, pol and part, the nsub*fres-point forward transform of
+// the window at ipart*in_step, the response product, the fres-point backward
+// transform of each sub-band's contiguous slice, and the kept samples
+// [nfilt_pos, nfilt_pos + nkeep) written to output channel ichan*nsub + isub.
+// The windows are batched into library transforms; the response product and
+// the overlap discard are kernels of their own.
+#include <hipfft/hipfft.h>
+
+#include <algorithm>
+
+#include "dspx_internal.hpp"
+
+namespace dspx {
+
+#define DSPX_FB_FFT(call)                                                                   \
+  do {                                                                                      \
+    hipfftResult r_ = (call);                                                               \
+    if (r_ != HIPFFT_SUCCESS) ::dspx::fail(DSPX_E_HIP, "%s failed: hipfft %d", #call, int(r_)); \
+  } while (0)
+
+// Batch unit u = (chan * npol + pol) * npart + part; scratch row b = u - u0.
+__global__ void k_fb_windows(const float* __restrict__ in, uint64_t ics, uint64_t ips, uint32_t npol,
+                             uint64_t nfft, uint64_t in_step, uint32_t npart, uint64_t u0,
+                             float2* __restrict__ scratch)
+{
+  uint64_t k = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (k >= nfft) return;
+  uint64_t b = blockIdx.y, u = u0 + b;
+  uint64_t p = u % npart, cp = u / npart;
+  uint64_t chan = cp / npol, pol = cp % npol;
+  const float* x = in + chan * ics + pol * ips + 2 * (p * in_step + k);
+  scratch[b * nfft + k] = *reinterpret_cast<const float2*>(x);
+}
+
+// Response::operate (Response.C:653-726): d * f per bin, the response of the
+// unit's input channel.
+__global__ void k_fb_multiply(float2* __restrict__ scratch, const float2* __restrict__ resp, uint64_t nfft,
+                              uint32_t npol, uint32_t npart, uint64_t u0)
+{
+  uint64_t k = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (k >= nfft) return;
+  uint64_t b = blockIdx.y;
+  uint64_t chan = ((u0 + b) / npart) / npol;
+  float2 d = scratch[b * nfft + k];
+  float2 f = resp[chan * nfft + k];
+  scratch[b * nfft + k] = make_float2(f.x * d.x - f.y * d.y, f.y * d.x + f.x * d.y);
+}
+
+// grid (ceil(nkeep/256), nb * nsub)
+__global__ void k_fb_discard(const float2* __restrict__ scratch, uint64_t nfft, uint32_t fres, uint32_t nsub,
+                             uint32_t npol, uint32_t npart, uint64_t u0, uint32_t nkeep, uint32_t nfilt_pos,
+                             float* __restrict__ out, uint64_t ocs, uint64_t ops)
+{
+  uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= nkeep) return;
+  uint64_t b = blockIdx.y / nsub;
+  uint32_t isub = blockIdx.y % nsub;
+  uint64_t u = u0 + b;
+  uint64_t p = u % npart, cp = u / npart;
+  uint64_t chan = cp / npol, pol = cp % npol;
+  float* o = out + (chan * nsub + isub) * ocs + pol * ops + 2 * (p * nkeep + m);
+  *reinterpret_cast<float2*>(o) = scratch[b * nfft + uint64_t(isub) * fres + nfilt_pos + m];
+}
+
+}  // namespace dspx
+
+using namespace dspx;
+
+struct dspx_fb {
+  dspx_ctx* ctx = nullptr;
+  dspx_fb_desc d{};
+  uint64_t nfft = 0, in_step = 0;
+  uint32_t nkeep = 0;
+  DevBuf resp, scratch;
+  bool have_resp = false;
+  hipfftHandle fwd = 0, bwd = 0;
+  bool have_plan = false;
+  uint64_t plan_batch = 0;
+
+  ~dspx_fb()
+  {
+    if (have_plan) {
+      hipfftDestroy(fwd);
+      hipfftDestroy(bwd);
+    }
+  }
+};
+
+extern "C" {
+
+int dspx_fb_create(dspx_ctx* ctx, const dspx_fb_desc* desc, dspx_fb** out)
+{
+  return guard(ctx, [&] {
+    DSPX_REQUIRE(ctx && desc && out, DSPX_E_PARAM, "null argument");
+    const dspx_fb_desc& d = *desc;
+    DSPX_REQUIRE(d.nchan >= 1 && d.npol >= 1 && d.npol <= 2 && d.nsub >= 1 && d.fres >= 1, DSPX_E_PARAM,
+                 "invalid shape");
+    DSPX_REQUIRE(uint64_t(d.nfilt_pos) + d.nfilt_neg < d.fres, DSPX_E_STATE, "nfilt=%u >= freq_res=%u",
+                 d.nfilt_pos + d.nfilt_neg, d.fres);  // Filterbank.C:425-428 analogue
+    DSPX_REQUIRE(uint64_t(d.nchan) * d.nsub <= 65535 * 64, DSPX_E_UNSUPPORTED, "too many output channels");
+    DSPX_HIP(hipSetDevice(ctx->device));
+    auto* F = new dspx_fb;
+    F->ctx = ctx;
+    F->d = d;
+    F->nfft = uint64_t(d.nsub) * d.fres;
+    F->nkeep = d.fres - d.nfilt_pos - d.nfilt_neg;
+    F->in_step = uint64_t(d.nsub) * F->nkeep;  // nsamp_fft - nsamp_overlap (Filterbank.C:154-192)
+    *out = F;
+  });
+}
+
+void dspx_fb_destroy(dspx_fb* F)
+{
+  if (!F) return;
+  (void) hipSetDevice(F->ctx->device);
+  (void) hipStreamSynchronize(F->ctx->stream);
+  delete F;
+}
+
+int dspx_fb_set_response(dspx_fb* F, const float* host)
+{
+  return guard(F ? F->ctx : nullptr, [&] {
+    DSPX_REQUIRE(F && host, DSPX_E_PARAM, "null argument");
+    const size_t bytes = size_t(F->d.nchan) * F->nfft * sizeof(float2);
+    F->resp.reserve(bytes);
+    DSPX_HIP(hipMemcpy(F->resp.p, host, bytes, hipMemcpyHostToDevice));
+    F->have_resp = true;
+  });
+}
+
+int dspx_fb_geometry(const dspx_fb* F, uint64_t ndat, uint64_t* npart, uint64_t* ndat_out)
+{
+  if (!F || !npart || !ndat_out) return DSPX_E_PARAM;
+  const uint64_t novl = uint64_t(F->d.nfilt_pos + F->d.nfilt_neg) * F->d.nsub;
+  *npart = ndat > novl ? (ndat - novl) / F->in_step : 0;  // Filterbank.C:423-424
+  if (*npart && (*npart - 1) * F->in_step + F->nfft > ndat) (*npart)--;
+  *ndat_out = *npart * F->nkeep;
+  return DSPX_OK;
+}
+
+int dspx_fb_perform(dspx_fb* F, const float* d_in, uint64_t ics, uint64_t ips, uint64_t ndat, float* d_out,
+                    uint64_t ocs, uint64_t ops)
+{
+  return guard(F ? F->ctx : nullptr, [&] {
+    DSPX_REQUIRE(F && d_in && d_out, DSPX_E_PARAM, "null argument");
+    DSPX_REQUIRE(F->have_resp, DSPX_E_STATE, "no response: dspx_fb_set_response");
+    uint64_t npart, nout;
+    dspx_fb_geometry(F, ndat, &npart, &nout);
+    if (!npart) return;
+    DSPX_REQUIRE(npart < (1u << 31), DSPX_E_UNSUPPORTED, "npart");
+    const dspx_fb_desc& d = F->d;
+    const uint64_t nunit = uint64_t(d.nchan) * d.npol * npart;
+    const uint64_t per = F->nfft * sizeof(float2);
+    uint64_t batch = std::max<uint64_t>(1, std::min<uint64_t>(nunit, (1ull << 30) / per));
+    batch = std::min<uint64_t>(batch, 65535 / d.nsub);  // grid.y of the discard
+    DSPX_REQUIRE(batch >= 1, DSPX_E_UNSUPPORTED, "nsub=%u too large", d.nsub);
+    if (!F->have_plan || F->plan_batch != batch) {
+      if (F->have_plan) {
+        hipfftDestroy(F->fwd);
+        hipfftDestroy(F->bwd);
+        F->have_plan = false;
+      }
+      int n = int(F->nfft), m = int(d.fres);
+      DSPX_FB_FFT(hipfftPlanMany(&F->fwd, 1, &n, nullptr, 1, n, nullptr, 1, n, HIPFFT_C2C, int(batch)));
+      DSPX_FB_FFT(hipfftPlanMany(&F->bwd, 1, &m, nullptr, 1, m, nullptr, 1, m, HIPFFT_C2C, int(batch * d.nsub)));
+      DSPX_FB_FFT(hipfftSetStream(F->fwd, F->ctx->stream));
+      DSPX_FB_FFT(hipfftSetStream(F->bwd, F->ctx->stream));
+      F->have_plan = true;
+      F->plan_batch = batch;
+    }
+    F->scratch.reserve(batch * per);
+    float2* s = F->scratch.as<float2>();
+    hipStream_t st = F->ctx->stream;
+    for (uint64_t u0 = 0; u0 < nunit; u0 += batch) {
+      const uint64_t nb = std::min(batch, nunit - u0);
+      if (nb != batch)  // the unused rows of the last batch stay finite
+        DSPX_HIP(hipMemsetAsync(s + nb * F->nfft, 0, (batch - nb) * per, st));
+      dim3 g1(unsigned((F->nfft + 255) / 256), unsigned(nb));
+      k_fb_windows<<<g1, 256, 0, st>>>(d_in, ics, ips, d.npol, F->nfft, F->in_step, uint32_t(npart), u0, s);
+      DSPX_HIP(hipGetLastError());
+      DSPX_FB_FFT(hipfftExecC2C(F->fwd, reinterpret_cast<hipfftComplex*>(s), reinterpret_cast<hipfftComplex*>(s),
+                                HIPFFT_FORWARD));
+      k_fb_multiply<<<g1, 256, 0, st>>>(s, F->resp.as<float2>(), F->nfft, d.npol, uint32_t(npart), u0);
+      DSPX_HIP(hipGetLastError());
+      DSPX_FB_FFT(hipfftExecC2C(F->bwd, reinterpret_cast<hipfftComplex*>(s), reinterpret_cast<hipfftComplex*>(s),
+                                HIPFFT_BACKWARD));
+      dim3 g2((F->nkeep + 255) / 256, unsigned(nb * d.nsub));
+      k_fb_discard<<<g2, 256, 0, st>>>(s, F->nfft, d.fres, d.nsub, d.npol, uint32_t(npart), u0, F->nkeep,
+                                       d.nfilt_pos, d_out, ocs, ops);
+      DSPX_HIP(hipGetLastError());
+    }
+  });
+}
+
+}  // extern "C"

@@ -135,6 +135,41 @@ int dspx_conv_perform(dspx_conv* conv, const float* d_in, uint64_t in_chan_strid
                       uint64_t out_chan_stride, uint64_t out_pol_stride, float* d_zero_dm,
                       uint32_t npart);
 
+/* --------------------------------------------------------------- filterbank */
+/* Filterbank::Engine (Filterbank.C:245-250, 463-474, 549-561;
+   FilterbankEngineCPU.C:184-370): nchan input channels each split into nsub
+   sub-bands of fres frequency bins, convolved with the response on the way.
+   FFT-library form: batched nsub*fres-point forward transforms of the windows
+   at ipart*nsub*nkeep, the response product, batched fres-point backward
+   transforms of each sub-band's slice, the kept samples [nfilt_pos,
+   nfilt_pos + nkeep) (nkeep = fres - nfilt_pos - nfilt_neg). */
+typedef struct dspx_fb dspx_fb;
+
+typedef struct {
+  uint32_t nchan;     /* input channels */
+  uint32_t npol;
+  uint32_t nsub;      /* output channels per input channel (nchan_subband) */
+  uint32_t fres;      /* frequency resolution: bins per output channel */
+  uint32_t nfilt_pos; /* the response's impulse_pos / impulse_neg */
+  uint32_t nfilt_neg;
+  uint32_t reserved[2];
+} dspx_fb_desc;
+
+int dspx_fb_create(dspx_ctx* ctx, const dspx_fb_desc* desc, dspx_fb** out);
+void dspx_fb_destroy(dspx_fb* fb);
+/* The response [nchan][nsub*fres] complex float, in the order the
+   reference's Response holds it after Response::swap_as_needed
+   (Response.C:325-400) and with the normaliser multiplied in
+   (Filterbank.C:176-185).  Synchronous. */
+int dspx_fb_set_response(dspx_fb* fb, const float* host_response);
+/* Parts and output samples per output channel for ndat input samples
+   (Filterbank::resize_output, Filterbank.C:410-440). */
+int dspx_fb_geometry(const dspx_fb* fb, uint64_t ndat, uint64_t* npart, uint64_t* ndat_out);
+/* FPT complex input [nchan][npol][ndat] (strides in floats) -> FPT complex
+   output [nchan*nsub][npol][npart*nkeep]. */
+int dspx_fb_perform(dspx_fb* fb, const float* d_in, uint64_t in_chan_stride, uint64_t in_pol_stride,
+                    uint64_t ndat, float* d_out, uint64_t out_chan_stride, uint64_t out_pol_stride);
+
 /* --------------------------------------------------------------- detection */
 
 enum { DSPX_INTENSITY = 0, DSPX_PPQQ = 1, DSPX_COHERENCE = 2, DSPX_STOKES = 3 };
@@ -272,6 +307,22 @@ int dspx_ltf_execute(dspx_ltf* ltf, const void* d_raw, float* d_prof, uint32_t* 
 /* Raw device input -> dedispersed filterbank out[ichan][ipol][npart*step]
    complex (what the Convolution operation outputs). */
 int dspx_ltf_filterbank(dspx_ltf* ltf, const void* d_raw, float* d_out);
+/* Stream mode (InputBuffering::set_next_start / pre_transformation,
+   InputBuffering.C:66-180, with TimeSeries::prepend, TimeSeries.C:514-533):
+   consecutive calls each bring d.ndat (>= nfft) new raw samples; the
+   unprocessed tail of the previous call (the overlap and any remainder) is
+   kept on the device and prepended, so the FFT windows tile the stream as
+   one long block would.  Fused form only.  reset starts a new stream. */
+int dspx_ltf_stream_reset(dspx_ltf* ltf);
+/* The next call's FFT parts and output samples, and the stream index of the
+   input sample under its first output sample (nfilt_pos after the first
+   unprocessed sample, Convolution.C:611).  Call before each stream call, and
+   set that call's fold plan (dspx_ltf_set_fold) after it. */
+int dspx_ltf_stream_next(dspx_ltf* ltf, uint64_t* npart, uint64_t* ndat_out, uint64_t* t_first);
+/* One stream call: profiles / hits as dspx_ltf_execute, or the call's
+   filterbank out[ichan][ipol][ndat_out] as dspx_ltf_filterbank. */
+int dspx_ltf_stream_execute(dspx_ltf* ltf, const void* d_raw, float* d_prof, uint32_t* d_hits);
+int dspx_ltf_stream_filterbank(dspx_ltf* ltf, const void* d_raw, float* d_out);
 /* Per-kernel durations (ms) of the last execute when timing is enabled;
    names[i] points at static strings.  Returns the count via *n. */
 int dspx_ltf_set_timing(dspx_ltf* ltf, int enable);

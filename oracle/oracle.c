@@ -866,3 +866,66 @@ void ox_sk_mask(const float* in, unsigned nchan, unsigned npol, uint64_t ndat, u
           y[ipart * M * 2 + j] = mask[ipart * nchan + ichan] ? 0.0f : x[ipart * M * 2 + j];
     }
 }
+
+/* FilterbankEngineCPU::perform (FilterbankEngineCPU.C:184-370), complex
+   input, no apodization, passband or zero-DM output: for each input channel,
+   FFT part and pol, the nsub*fres-point forward transform of the window at
+   ipart*in_step, times the response (resp[ichan_in][nsub*fres] complex, in
+   the order the reference's Response holds it after its swaps), then for
+   each of the nsub sub-bands the fres-point backward transform of its
+   contiguous slice, of which samples [nfilt_pos, nfilt_pos + nkeep) are
+   kept (nkeep = fres - nfilt_pos - nfilt_neg).
+     in  [nchan][npol][ndat] complex, out [nchan*nsub][npol][npart*nkeep].
+   in_step = nsub*nkeep (Filterbank.C:154-192), npart = (ndat - nsub*nfilt)
+   / in_step.  Unnormalized transforms; the response carries the scale.
+   Returns npart. */
+#define OX_DEFINE_FB(T, NAME, FFT)                                             \
+  uint64_t NAME(const T* in, unsigned nchan, unsigned npol, uint64_t ndat,     \
+                unsigned nsub, unsigned fres, unsigned nfilt_pos,              \
+                unsigned nfilt_neg, const T* resp, T* out)                     \
+  {                                                                            \
+    const uint64_t nfft = (uint64_t) nsub * fres;                              \
+    const unsigned nkeep = fres - nfilt_pos - nfilt_neg;                       \
+    const uint64_t novl = (uint64_t) (nfilt_pos + nfilt_neg) * nsub;           \
+    const uint64_t in_step = nfft - novl;                                      \
+    const uint64_t npart = ndat > novl ? (ndat - novl) / in_step : 0;          \
+    const uint64_t nout = npart * nkeep;                                       \
+    T* buf = (T*) malloc(sizeof(T) * 2 * nfft);                                \
+    unsigned ichan, ipol, isub, m;                                             \
+    uint64_t ipart, k;                                                         \
+    for (ichan = 0; ichan < nchan; ichan++)                                    \
+      for (ipart = 0; ipart < npart; ipart++)                                  \
+        for (ipol = 0; ipol < npol; ipol++)                                    \
+        {                                                                      \
+          const T* x = in + (((uint64_t) ichan * npol + ipol) * ndat           \
+                             + ipart * in_step) * 2;                           \
+          const T* r = resp + (uint64_t) ichan * nfft * 2;                     \
+          if (ipart * in_step + nfft > ndat) continue;                         \
+          memcpy(buf, x, sizeof(T) * 2 * nfft);                                \
+          FFT(nfft, buf, -1);                                                  \
+          for (k = 0; k < nfft; k++)                                           \
+          {                                                                    \
+            T dr = buf[2 * k], di = buf[2 * k + 1];                            \
+            T fr_ = r[2 * k], fi = r[2 * k + 1];                               \
+            buf[2 * k] = fr_ * dr - fi * di;                                   \
+            buf[2 * k + 1] = fi * dr + fr_ * di;                               \
+          }                                                                    \
+          for (isub = 0; isub < nsub; isub++)                                  \
+          {                                                                    \
+            T* s = buf + (uint64_t) isub * fres * 2;                           \
+            T* o = out + ((((uint64_t) ichan * nsub + isub) * npol + ipol)     \
+                          * nout + ipart * nkeep) * 2;                         \
+            FFT(fres, s, +1);                                                  \
+            for (m = 0; m < nkeep; m++)                                        \
+            {                                                                  \
+              o[2 * m] = s[2 * (nfilt_pos + m)];                               \
+              o[2 * m + 1] = s[2 * (nfilt_pos + m) + 1];                       \
+            }                                                                  \
+          }                                                                    \
+        }                                                                      \
+    free(buf);                                                                 \
+    return npart;                                                              \
+  }
+
+OX_DEFINE_FB(float, ox_filterbank_f32, ox_fft_f32)
+OX_DEFINE_FB(double, ox_filterbank_f64, ox_fft_f64)

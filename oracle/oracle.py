@@ -83,6 +83,10 @@ def lib():
     L.ox_sk_detect_skfb.argtypes = [f32p, u32p, U, u, u, u, f32p, f32p, u8p]
     L.ox_sk_detect.restype = U
     L.ox_sk_detect.argtypes = [f32p, u32p, U, u, u, u, u, i, u, f32p, f32p, f32p, d, u8p]
+    for name, p in (("ox_filterbank_f32", f32p), ("ox_filterbank_f64", f64p)):
+        fn = getattr(L, name)
+        fn.restype = U
+        fn.argtypes = [p, u, u, U, u, u, u, u, p, p]
     L.ox_sk_mask.restype = None
     L.ox_sk_mask.argtypes = [f32p, u, u, U, u, u8p, U, f32p]
     _lib = L
@@ -287,3 +291,37 @@ def sk_mask(x, M, mask):
     lib().ox_sk_mask(x.view(np.float32).reshape(-1), nchan, npol, ndat, M,
                      np.ascontiguousarray(mask, np.uint8).reshape(-1), npart, out.view(np.float32).reshape(-1))
     return out
+
+
+def filterbank(x, nsub, fres, nfilt_pos, nfilt_neg, resp, f64=False):
+    """FilterbankEngineCPU::perform: x [nchan][npol][ndat] complex, resp
+    [nchan][nsub*fres] complex in application order -> out
+    [nchan*nsub][npol][npart*nkeep] complex."""
+    dt, ct = (np.float64, np.complex128) if f64 else (np.float32, np.complex64)
+    x = np.ascontiguousarray(x, ct)
+    nchan, npol, ndat = x.shape
+    nfft = nsub * fres
+    nkeep = fres - nfilt_pos - nfilt_neg
+    novl = (nfilt_pos + nfilt_neg) * nsub
+    npart = (ndat - novl) // (nfft - novl) if ndat > novl else 0
+    out = np.zeros((nchan * nsub, npol, npart * nkeep), ct)
+    fn = lib().ox_filterbank_f64 if f64 else lib().ox_filterbank_f32
+    n = fn(x.view(dt).reshape(-1), nchan, npol, ndat, nsub, fres, nfilt_pos, nfilt_neg,
+           np.ascontiguousarray(resp, ct).view(dt).reshape(-1), out.view(dt).reshape(-1))
+    assert n == npart
+    return out
+
+
+def filterbank_response(cfreq, bw, nchan_in, nsub, dm, fres, mode=0):
+    """The dedispersion response of a filterbank of nchan_in*nsub output
+    channels of fres bins, in the order Response::swap_as_needed leaves it
+    for single-channel complex input (Response.C:335-345: built per channel
+    in frequency order, then the whole buffer's halves swapped), with the
+    normaliser 1/sqrt(nsub*fres*fres) (Filterbank.C:176-185).  nchan_in must
+    be 1 (the whole swap)."""
+    assert nchan_in == 1
+    nout = nchan_in * nsub
+    r = response(cfreq, bw, nout, dm, fres, mode).reshape(nout, fres)  # per-channel swapped, 1/sqrt(fres)
+    nat = np.roll(r, fres // 2, axis=1)  # undo the per-channel swap: frequency order within each channel
+    flat = nat.reshape(-1) * np.float32(1.0 / np.sqrt(nsub * fres))
+    return np.roll(flat, (nsub * fres) // 2).reshape(nchan_in, nsub * fres).astype(np.complex64)
