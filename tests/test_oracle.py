@@ -416,3 +416,37 @@ def test_sk_mask_zeroes_flagged_blocks(oracle):
             blk = y[c, :, b * M:(b + 1) * M]
             assert (blk == 0).all() if mask[b, c] else np.array_equal(blk, x[c, :, b * M:(b + 1) * M])
     assert (y[:, :, nblk * M:] == 0).all()
+
+
+def test_filterbank_single_subband_is_the_convolution(oracle):
+    """With one sub-band (nsub = 1) the convolving filterbank is the
+    overlap-save convolution (FilterbankEngineCPU.C:184-370 against
+    Convolution.C:646-813)."""
+    rng = np.random.default_rng(12)
+    nfft, pos, neg = 1024, 70, 90
+    x = (rng.standard_normal((2, 2, 5000)) + 1j * rng.standard_normal((2, 2, 5000))).astype(np.complex64)
+    resp = oracle.response(1400.0, 10.0, 2, 30.0, nfft)
+    fb = oracle.filterbank(x, 1, nfft, pos, neg, resp)
+    cv = oracle.convolve(x, resp, pos, neg)
+    assert fb.shape == cv.shape
+    assert np.array_equal(fb, cv)
+
+
+def test_filterbank_tone_lands_in_its_subband(oracle):
+    """An identity response and no overlap: a tone at big-FFT bin f comes out
+    of sub-band f // fres only, as nsub*fres * exp(2 pi i k0 m / fres) (the
+    unnormalized forward transform puts nsub*fres in bin f; the backward
+    transform of a single bin adds no factor)."""
+    nsub, fres = 8, 64
+    nfft = nsub * fres
+    f = 3 * fres + 5
+    n = np.arange(4 * nfft)
+    x = np.exp(2j * np.pi * f * n / nfft).astype(np.complex64)[None, None, :]
+    resp = np.ones((1, nfft), np.complex64)
+    out = oracle.filterbank(x, nsub, fres, 0, 0, resp, f64=True)
+    mag = np.abs(out[:, 0, :])
+    k0 = f - 3 * fres
+    assert np.all(mag[3] > 0.999 * nfft)
+    assert np.delete(mag, 3, axis=0).max() < 1e-6 * nfft
+    m = np.arange(fres)
+    assert np.allclose(out[3, 0, :fres], nfft * np.exp(2j * np.pi * k0 * m / fres), rtol=1e-6, atol=1e-4)
