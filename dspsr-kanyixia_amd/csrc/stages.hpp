@@ -298,15 +298,32 @@ __global__ __launch_bounds__(N2 / E) void k_stageB(ConvShape s, ChirpArgs c, con
 #pragma unroll
       for (int r = 0; r < RF; r++) pf[q * RF + r] = row[tid + q * NT + r * (N2 / RF)];
   }
+  // 4096-point rows: the workgroup's occupancy is set by its LDS (two per
+  // CU), so registers are spent on keeping the row loop's twiddles (the
+  // inter-stage twiddles here, the pass twiddles by the compiler's hoisting).
+  // Shorter rows recompute them per row, keeping registers for occupancy.
+  constexpr bool HOIST = N2 >= 4096;
+  float2 w[E];
+  if constexpr (HOIST) {
+#pragma unroll
+    for (int q = 0; q < Q; q++)
+#pragma unroll
+      for (int r = 0; r < RF; r++) w[q * RF + r] = cmul(base[q], step[r]);
+  }
   for (uint32_t i = 0; i < nrow; i++) {
     float2* row = row_ptr(i);
-    const float2* tw = opaque(twr);
+    const float2* tw = HOIST ? twr : opaque(twr);
     float2 v[E];
+    if constexpr (HOIST) {
 #pragma unroll
-    for (int q = 0; q < Q; q++) {
-      float2 b = opaque_v(base[q]);
+      for (int e = 0; e < E; e++) v[e] = cmul(pf[e], w[e]);
+    } else {
 #pragma unroll
-      for (int r = 0; r < RF; r++) v[q * RF + r] = cmul(pf[q * RF + r], cmul(b, step[r]));
+      for (int q = 0; q < Q; q++) {
+        float2 b = opaque_v(base[q]);
+#pragma unroll
+        for (int r = 0; r < RF; r++) v[q * RF + r] = cmul(pf[q * RF + r], cmul(b, step[r]));
+      }
     }
     if (i + 1 < nrow) {
       const float2* nxt = row_ptr(i + 1);
@@ -318,12 +335,13 @@ __global__ __launch_bounds__(N2 / E) void k_stageB(ConvShape s, ChirpArgs c, con
     fft_regs<LAY, N2, 1, NT, -1>(lds, v, tw, tid);
 #pragma unroll
     for (int r = 0; r < RI; r++) v[r] = cmul(v[r], hlds[tid + r * (N2 / RI)]);
-    fft_regs<LAY, N2, 1, NT, +1>(lds, v, opaque(tw), tid);
+    fft_regs<LAY, N2, 1, NT, +1>(lds, v, HOIST ? tw : opaque(tw), tid);
 #pragma unroll
     for (int q = 0; q < Q; q++) {
-      float2 b = opaque_v(base[q]);
+      float2 b = HOIST ? base[q] : opaque_v(base[q]);
 #pragma unroll
-      for (int r = 0; r < RL; r++) row[tid + q * NT + r * (N2 / RL)] = cmulc(v[q * RL + r], cmul(b, step[r]));
+      for (int r = 0; r < RL; r++)
+        row[tid + q * NT + r * (N2 / RL)] = cmulc(v[q * RL + r], HOIST ? w[q * RL + r] : cmul(b, step[r]));
     }
   }
 }
