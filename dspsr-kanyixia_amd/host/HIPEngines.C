@@ -110,6 +110,103 @@ void ConvolutionEngine::perform(const TimeSeries* in, TimeSeries* out, TimeSerie
              "dsp::HIP::ConvolutionEngine::perform");
 }
 
+// ------------------------------------------------------- FilterbankEngine
+
+FilterbankEngine::~FilterbankEngine() { dspx_fb_destroy(fb); }
+
+void FilterbankEngine::setup(Filterbank* filterbank)
+{
+  parent = filterbank;
+  const TimeSeries* in = filterbank->get_input();
+  dspx_fb_destroy(fb);
+  fb = nullptr;
+  desc = dspx_fb_desc{};
+  desc.nchan = in->get_nchan();
+  desc.npol = in->get_npol();
+  desc.nsub = filterbank->get_nchan_subband();
+  desc.fres = filterbank->get_freq_res();
+  desc.nfilt_pos = filterbank->get_nfilt_pos();
+  desc.nfilt_neg = filterbank->get_nfilt_neg();
+  ctx->check(dspx_fb_create(ctx->get(), &desc, &fb), "dsp::HIP::FilterbankEngine::setup");
+  ctx->check(dspx_fb_set_response(fb, filterbank->get_response_table().data()), "dsp::HIP::FilterbankEngine::setup");
+  filterbank->set_response_table_changed(false);
+}
+
+void FilterbankEngine::perform(const TimeSeries* in, TimeSeries* out, uint64_t npart, uint64_t in_step,
+                               uint64_t out_step)
+{
+  perform(in, out, nullptr, npart, in_step, out_step);
+}
+
+void FilterbankEngine::perform(const TimeSeries* in, TimeSeries* out, TimeSeries* zdm, uint64_t npart,
+                               uint64_t in_step, uint64_t out_step)
+{
+  const char* where = "dsp::HIP::FilterbankEngine::perform";
+  if (zdm) throw Error(InvalidState, where, "zero-DM output is not on this path");
+  require_device(in->get_datptr(), in->get_memory()->on_device, where);
+  require_device(out->get_datptr(), out->get_memory()->on_device, where);
+  const uint64_t nkeep = desc.fres - desc.nfilt_pos - desc.nfilt_neg;
+  if (in_step != 2 * uint64_t(desc.nsub) * nkeep || out_step != 2 * nkeep)
+    throw Error(InvalidParam, where, "in_step=%llu out_step=%llu do not match nsub=%u nkeep=%llu",
+                (unsigned long long) in_step, (unsigned long long) out_step, desc.nsub, (unsigned long long) nkeep);
+  if (parent && parent->response_table_changed()) {
+    ctx->check(dspx_fb_set_response(fb, parent->get_response_table().data()), where);
+    parent->set_response_table_changed(false);
+  }
+  // the library derives the part count from the samples it is given: hand it
+  // exactly the span of npart parts
+  const uint64_t nfft = uint64_t(desc.nsub) * desc.fres;
+  const uint64_t span = (npart - 1) * (in_step / 2) + nfft;
+  if (span > in->get_ndat()) throw Error(InvalidParam, where, "npart=%llu exceeds the input", (unsigned long long) npart);
+  ctx->check(dspx_fb_perform(fb, in->get_datptr(), in->chan_span(), in->get_pol_span(), span, out->get_datptr(),
+                             out->chan_span(), out->get_pol_span()),
+             where);
+}
+
+// ------------------------------------------------- SpectralKurtosisEngine
+
+SpectralKurtosisEngine::~SpectralKurtosisEngine() { dspx_sk_destroy(sk); }
+
+void SpectralKurtosisEngine::setup(SpectralKurtosis* s)
+{
+  const TimeSeries* in = s->get_input();
+  dspx_sk_destroy(sk);
+  sk = nullptr;
+  nchan = in->get_nchan();
+  dspx_sk_desc d{nchan, in->get_npol(), s->get_M(), 1u, s->get_omit_outliers() ? 1u : 0u, 0u};
+  ctx->check(dspx_sk_create(ctx->get(), &d, &sk), "dsp::HIP::SpectralKurtosisEngine::setup");
+  ctx->check(dspx_sk_set_limits(sk, s->get_lower().data(), s->get_upper().data(), s->get_symmetric().data(),
+                                s->get_pc_std_devs()),
+             "dsp::HIP::SpectralKurtosisEngine::setup");
+}
+
+void SpectralKurtosisEngine::compute(const TimeSeries* in, float* sums, uint32_t* counts, uint64_t npart)
+{
+  require_device(in->get_datptr(), in->get_memory()->on_device, "dsp::HIP::SpectralKurtosisEngine::compute");
+  ctx->check(dspx_sk_compute(sk, in->get_datptr(), in->chan_span(), in->get_pol_span(), npart,
+                             in->get_zeroed_data() ? 1 : 0, sums, counts),
+             "dsp::HIP::SpectralKurtosisEngine::compute");
+}
+
+void SpectralKurtosisEngine::reset_mask(uint8_t* mask, uint64_t npart)
+{
+  hipStream_t s = static_cast<hipStream_t>(dspx_ctx_stream(ctx->get()));
+  Memory::device()->zero(mask, size_t(npart) * nchan, s);
+}
+
+void SpectralKurtosisEngine::detect(float* sums, uint32_t* counts, uint64_t npart, unsigned flags, uint8_t* mask)
+{
+  ctx->check(dspx_sk_detect(sk, sums, counts, npart, flags, mask), "dsp::HIP::SpectralKurtosisEngine::detect");
+}
+
+void SpectralKurtosisEngine::mask(const uint8_t* m, const TimeSeries* in, TimeSeries* out, uint64_t npart)
+{
+  require_device(out->get_datptr(), out->get_memory()->on_device, "dsp::HIP::SpectralKurtosisEngine::mask");
+  ctx->check(dspx_sk_mask(sk, m, in->get_datptr(), in->chan_span(), in->get_pol_span(), out->get_datptr(),
+                          out->chan_span(), out->get_pol_span(), npart),
+             "dsp::HIP::SpectralKurtosisEngine::mask");
+}
+
 // -------------------------------------------------------- DetectionEngine
 
 void DetectionEngine::square_law(const TimeSeries* in, TimeSeries* out)

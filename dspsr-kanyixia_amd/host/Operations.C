@@ -104,6 +104,146 @@ void Dedispersion::prepare(const Observation* input, unsigned channels)
   ndat = g.nfft;
 }
 
+void TableResponse::set_table(unsigned nchan_, unsigned ndat_, unsigned impulse_pos_, unsigned impulse_neg_,
+                              std::vector<float> t)
+{
+  if (t.size() != size_t(nchan_) * ndat_ * 2)
+    throw Error(InvalidParam, "dsp::TableResponse::set_table", "table of %zu floats for nchan=%u ndat=%u", t.size(),
+                nchan_, ndat_);
+  nchan = nchan_;
+  ndat = ndat_;
+  impulse_pos = impulse_pos_;
+  impulse_neg = impulse_neg_;
+  data = std::move(t);
+  changed = true;
+}
+
+// ------------------------------------------------------------- Filterbank
+
+void Filterbank::prepare()
+{
+  if (prepared) return;
+  if (!engine) throw Error(InvalidState, "dsp::Filterbank::prepare", "no engine (the CPU branch is the oracle)");
+  if (input->get_state() != Signal::Analytic)
+    throw Error(InvalidState, "dsp::Filterbank::prepare", "only Analytic (complex) input is on this path");
+  const unsigned in_nchan = input->get_nchan();
+  if (nchan < in_nchan)
+    throw Error(InvalidState, "dsp::Filterbank::prepare", "output nchan=%d < input nchan=%d", nchan, in_nchan);
+  if (nchan % in_nchan)
+    throw Error(InvalidState, "dsp::Filterbank::prepare", "output nchan=%d not a multiple of input nchan=%d", nchan,
+                in_nchan);
+  nchan_subband = nchan / in_nchan;
+  nfilt_pos = nfilt_neg = 0;
+  if (response) {
+    response->prepare(input, nchan);
+    if (response->get_nchan() != nchan)
+      throw Error(InvalidState, "dsp::Filterbank::prepare", "response nchan=%d != output nchan=%d",
+                  response->get_nchan(), nchan);
+    nfilt_pos = response->get_impulse_pos();
+    nfilt_neg = response->get_impulse_neg();
+    freq_res = response->get_ndat();
+  }
+  if (freq_res == 0) throw Error(InvalidState, "dsp::Filterbank::prepare", "Response.ndat = 0");
+  if (nfilt_pos + nfilt_neg >= freq_res)
+    throw Error(InvalidState, "dsp::Filterbank::prepare", "nfilt=%u >= freq_res=%u", nfilt_pos + nfilt_neg,
+                freq_res);
+  nsamp_fft = nchan_subband * freq_res;
+  nsamp_overlap = (nfilt_pos + nfilt_neg) * nchan_subband;
+  nsamp_step = nsamp_fft - nsamp_overlap;
+  build_table();
+  engine->setup(this);
+  prepared = true;
+}
+
+void Filterbank::build_table()
+{
+  // ScalarFilter 1/sqrt(nsamp_fft*freq_res) appended to the response for the
+  // unnormalised transforms (Filterbank.C:176-185, Convolution.C:300-313)
+  const float norm = float(1.0 / std::sqrt(double(nsamp_fft) * double(freq_res)));
+  const size_t n = size_t(nchan) * freq_res;  // complex values
+  table.assign(2 * n, 0.f);
+  if (response) {
+    const auto& d = response->get_data();
+    if (d.size() != 2 * n)
+      throw Error(InvalidState, "dsp::Filterbank::prepare", "response has no host table (%zu floats, need %zu)",
+                  d.size(), 2 * n);
+    for (size_t i = 0; i < 2 * n; i++) table[i] = d[i] * norm;
+    response->set_changed(false);
+  } else {
+    for (size_t i = 0; i < n; i++) table[2 * i] = norm;
+  }
+  table_changed = true;
+}
+
+void Filterbank::operate()
+{
+  if (!prepared) prepare();
+  if (response && response->has_changed()) build_table();
+  const uint64_t ndat = input->get_ndat();
+  npart = ndat > nsamp_overlap ? (ndat - nsamp_overlap) / nsamp_step : 0;  // Filterbank.C:418-424
+  const unsigned nkeep = freq_res - nfilt_pos - nfilt_neg;
+  output->copy_configuration(input);
+  output->set_nchan(nchan);
+  output->set_state(Signal::Analytic);
+  output->set_ndim(2);
+  output->set_rate(input->get_rate() / nchan_subband);  // Filterbank.C:356
+  output->resize(npart * nkeep);
+  output->change_start_time(nfilt_pos);  // Filterbank.C:387
+  if (!npart) return;
+  engine->perform(input, output, npart, uint64_t(nsamp_step) * 2, uint64_t(nkeep) * 2);
+  engine->finish();
+}
+
+// ----------------------------------------------------- SpectralKurtosis
+
+void SpectralKurtosis::set_limits(std::vector<float> lo, std::vector<float> hi, std::vector<float> sym, double pcsd)
+{
+  if (lo.size() != size_t(M) + 1 || hi.size() != size_t(M) + 1 || sym.size() != size_t(M) + 1)
+    throw Error(InvalidParam, "dsp::SpectralKurtosis::set_limits", "tables need M+1=%u entries", M + 1);
+  lower = std::move(lo);
+  upper = std::move(hi);
+  symmetric = std::move(sym);
+  pc_std_devs = pcsd;
+  prepared = false;
+}
+
+SpectralKurtosis::~SpectralKurtosis()
+{
+  Memory::device()->free(sums);
+  Memory::device()->free(counts);
+  Memory::device()->free(zapmask);
+}
+
+void SpectralKurtosis::operate()
+{
+  if (!engine) throw Error(InvalidState, "dsp::SpectralKurtosis::prepare", "no engine");
+  if (input->get_state() != Signal::Analytic)
+    throw Error(InvalidState, "dsp::SpectralKurtosis::prepare", "only Analytic (complex) input is on this path");
+  if (lower.empty()) throw Error(InvalidState, "dsp::SpectralKurtosis::prepare", "thresholds not set");
+  if (!prepared) {
+    engine->setup(this);
+    prepared = true;
+  }
+  npart = input->get_ndat() / M;  // Resolution::prepare (SpectralKurtosis.C:263-281), noverlap 1
+  if (npart > capacity) {
+    Memory* dev = Memory::device();
+    dev->free(sums);
+    dev->free(counts);
+    dev->free(zapmask);
+    const size_t nc = size_t(npart) * input->get_nchan();
+    sums = static_cast<float*>(dev->allocate(nc * input->get_npol() * 2 * sizeof(float)));
+    counts = static_cast<uint32_t*>(dev->allocate(nc * sizeof(uint32_t)));
+    zapmask = static_cast<uint8_t*>(dev->allocate(nc));
+    capacity = npart;
+  }
+  output->copy_configuration(input);
+  output->resize(npart * M);  // output_ndat = Nblock * M
+  if (!npart) return;
+  engine->compute(input, sums, counts, npart);
+  engine->detect(sums, counts, npart, detectors, zapmask);
+  engine->mask(zapmask, input, output, npart);
+}
+
 // ------------------------------------------------------------ Convolution
 
 void Convolution::prepare()

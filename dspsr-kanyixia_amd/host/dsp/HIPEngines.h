@@ -12,6 +12,10 @@
 //     (LoadToFold1.C:843-852)
 //   fold->set_engine (new dsp::HIP::FoldEngine (ctx));
 //     (LoadToFold1.C:969-978)
+//   filterbank->set_engine (new dsp::HIP::FilterbankEngine (ctx));
+//     (FilterbankConfig.C:115-133)
+//   skestimator->set_engine (new dsp::HIP::SpectralKurtosisEngine (ctx));
+//     (LoadToFold1.C:341-349)
 #pragma once
 #include "dsp/Operations.h"
 #include "dspx.h"
@@ -59,6 +63,40 @@ class ConvolutionEngine : public Convolution::Engine {
   unsigned form;
   dspx_conv* conv = nullptr;
   Convolution* parent = nullptr;
+};
+
+//! Filterbank::Engine over dspx_fb_* (FFT-library form)
+class FilterbankEngine : public Filterbank::Engine {
+ public:
+  explicit FilterbankEngine(Context* c) : ctx(c) {}
+  ~FilterbankEngine() override;
+  void setup(Filterbank* filterbank) override;
+  void perform(const TimeSeries* in, TimeSeries* out, uint64_t npart, uint64_t in_step, uint64_t out_step) override;
+  void perform(const TimeSeries* in, TimeSeries* out, TimeSeries* zero_DM_out, uint64_t npart, uint64_t in_step,
+               uint64_t out_step) override;
+
+ private:
+  Context* ctx;
+  Filterbank* parent = nullptr;
+  dspx_fb* fb = nullptr;
+  dspx_fb_desc desc{};
+};
+
+//! SpectralKurtosis::Engine over dspx_sk_* (one resolution, noverlap 1)
+class SpectralKurtosisEngine : public SpectralKurtosis::Engine {
+ public:
+  explicit SpectralKurtosisEngine(Context* c) : ctx(c) {}
+  ~SpectralKurtosisEngine() override;
+  void setup(SpectralKurtosis* sk) override;
+  void compute(const TimeSeries* input, float* sums, uint32_t* counts, uint64_t npart) override;
+  void reset_mask(uint8_t* mask, uint64_t npart) override;
+  void detect(float* sums, uint32_t* counts, uint64_t npart, unsigned flags, uint8_t* mask) override;
+  void mask(const uint8_t* mask, const TimeSeries* input, TimeSeries* output, uint64_t npart) override;
+
+ private:
+  Context* ctx;
+  dspx_sk* sk = nullptr;
+  unsigned nchan = 0;
 };
 
 class DetectionEngine : public Detection::Engine {

@@ -3,6 +3,10 @@
 //   Convolution + Convolution::Engine  Signal/General/Convolution.C:316-813
 //     (engine calls :446-451, :656-667, :690-697; Engine methods as used by
 //      ConvolutionCUDA.cu:174-216, 219, 567-595)
+//   Filterbank + Filterbank::Engine    Signal/General/Filterbank.C:75-566
+//     (engine calls :249, :520-566; FilterbankEngineCPU.C:44-386)
+//   SpectralKurtosis + its Engine      Signal/General/SpectralKurtosis.C:263-281,
+//     674-909, 1014-1566 (engine calls :438, 701, 1133, 1228, 1338, 1523)
 //   Detection + Detection::Engine      Signal/General/Detection.C:225-573
 //     (engine calls :237-243, :483-490)
 //   Fold + Fold::Engine                Signal/Pulsar/Fold.C:546-990,
@@ -61,6 +65,15 @@ class Dedispersion : public Response {
   unsigned user_ndat = 0;
 };
 
+//! A response whose table is built elsewhere (a ResponseProduct, a filter
+//! designed by the caller): [nchan][ndat] complex float as Response holds it
+class TableResponse : public Response {
+ public:
+  void set_table(unsigned nchan_, unsigned ndat_, unsigned impulse_pos_, unsigned impulse_neg_,
+                 std::vector<float> table);
+  void prepare(const Observation*, unsigned) override {}
+};
+
 class Convolution {
  public:
   class Engine {
@@ -104,6 +117,130 @@ class Convolution {
   bool prepared = false, zero_DM = false;
   unsigned nsamp_fft = 0, nsamp_overlap = 0, nsamp_step = 0, nfilt_pos = 0, nfilt_neg = 0;
   uint64_t npart = 0;
+};
+
+//! Convolving filterbank, Analytic input (Signal/General/Filterbank.C:75-200,
+//! 405-566).  Each input channel becomes nchan/input_nchan output channels of
+//! freq_res bins; the response, when given, is applied on the way.
+class Filterbank {
+ public:
+  //! The interface FilterbankEngineCPU implements (FilterbankEngineCPU.C:44,
+  //! 150, 177, 184-191, 379-386) and Filterbank::filterbank calls
+  //! (Filterbank.C:520-566)
+  class Engine {
+   public:
+    virtual ~Engine() = default;
+    virtual void setup(Filterbank* filterbank) = 0;
+    virtual void set_scratch(float*) {}
+    //! in_step / out_step are in floats: nsamp_step*ndim and nkeep*2
+    virtual void perform(const TimeSeries* in, TimeSeries* out, uint64_t npart, uint64_t in_step,
+                         uint64_t out_step) = 0;
+    virtual void perform(const TimeSeries* in, TimeSeries* out, TimeSeries* zero_DM_out, uint64_t npart,
+                         uint64_t in_step, uint64_t out_step) = 0;
+    virtual void finish() {}
+  };
+
+  void set_input(const TimeSeries* in) { input = in; }
+  void set_output(TimeSeries* out) { output = out; }
+  //! total output channels (a multiple of the input's)
+  void set_nchan(unsigned n) { nchan = n; }
+  //! bins per output channel when no response sets it
+  void set_frequency_resolution(unsigned n) { freq_res = n; }
+  void set_response(Response* r) { response = r; }
+  void set_engine(Engine* e) { engine.reset(e); }
+  const TimeSeries* get_input() const { return input; }
+  TimeSeries* get_output() const { return output; }
+
+  //! Filterbank::do_prepare (Filterbank.C:75-200)
+  void prepare();
+  //! Filterbank::transformation + filterbank (Filterbank.C:460-566)
+  void operate();
+
+  unsigned get_nchan_subband() const { return nchan_subband; }
+  unsigned get_freq_res() const { return freq_res; }
+  unsigned get_nfilt_pos() const { return nfilt_pos; }
+  unsigned get_nfilt_neg() const { return nfilt_neg; }
+  unsigned get_minimum_samples() const { return nsamp_fft; }
+  uint64_t get_npart() const { return npart; }
+  //! [input nchan][nchan_subband*freq_res] complex float: the response
+  //! (unity without one) times the normaliser 1/sqrt(nsamp_fft*freq_res)
+  //! that Filterbank.C:176-185 appends to it
+  const std::vector<float>& get_response_table() const { return table; }
+  bool response_table_changed() const { return table_changed; }
+  void set_response_table_changed(bool c) { table_changed = c; }
+
+ protected:
+  void build_table();
+  const TimeSeries* input = nullptr;
+  TimeSeries* output = nullptr;
+  Response* response = nullptr;
+  std::unique_ptr<Engine> engine;
+  std::vector<float> table;
+  bool prepared = false, table_changed = true;
+  unsigned nchan = 0, nchan_subband = 0, freq_res = 0, nfilt_pos = 0, nfilt_neg = 0;
+  unsigned nsamp_fft = 0, nsamp_overlap = 0, nsamp_step = 0;
+  uint64_t npart = 0;
+};
+
+//! Spectral kurtosis RFI excision at one resolution (noverlap 1): block sums,
+//! the detectors of SpectralKurtosis::detect and the mask
+//! (Signal/General/SpectralKurtosis.C:674-909, 1014-1566).  The thresholds
+//! come from SKLimits in the Transformation (SpectralKurtosis.C:800-822,
+//! 989-1009) and are handed to the engine as tables by block count.
+class SpectralKurtosis {
+ public:
+  //! The engine calls SpectralKurtosis.C makes (:438, 701, 1133, 1228, 1338,
+  //! 1523), with the detectors taking the transformation's thresholds
+  class Engine {
+   public:
+    virtual ~Engine() = default;
+    virtual void setup(SpectralKurtosis* sk) = 0;
+    //! sums [npart][nchan][npol][2] (S1, S2) and counts [npart][nchan]
+    virtual void compute(const TimeSeries* input, float* sums, uint32_t* counts, uint64_t npart) = 0;
+    virtual void reset_mask(uint8_t* mask, uint64_t npart) = 0;
+    //! detect_ft / detect_fscr / detect_tscr as the flags select, after reset_mask
+    virtual void detect(float* sums, uint32_t* counts, uint64_t npart, unsigned flags, uint8_t* mask) = 0;
+    virtual void mask(const uint8_t* mask, const TimeSeries* input, TimeSeries* output, uint64_t npart) = 0;
+  };
+
+  void set_input(const TimeSeries* in) { input = in; }
+  void set_output(TimeSeries* out) { output = out; }
+  void set_engine(Engine* e) { engine.reset(e); }
+  void set_M(unsigned m) { M = m; }
+  unsigned get_M() const { return M; }
+  void set_omit_outliers(bool o) { omit_outliers = o; }
+  bool get_omit_outliers() const { return omit_outliers; }
+  //! which detectors run (DSPX_SK_SKFB | DSPX_SK_FSCR | DSPX_SK_TSCR)
+  void set_detectors(unsigned f) { detectors = f; }
+  //! SKLimits thresholds by block count 0..M and the closed-form sigma of
+  //! detect_tscr (SKLimits.C:152-160)
+  void set_limits(std::vector<float> lower, std::vector<float> upper, std::vector<float> symmetric, double pcsd);
+  const std::vector<float>& get_lower() const { return lower; }
+  const std::vector<float>& get_upper() const { return upper; }
+  const std::vector<float>& get_symmetric() const { return symmetric; }
+  double get_pc_std_devs() const { return pc_std_devs; }
+  const TimeSeries* get_input() const { return input; }
+  //! compute, detect and mask the input blocks (SpectralKurtosis::transformation)
+  void operate();
+  uint64_t get_npart() const { return npart; }
+  //! device arrays of the last operate
+  const float* get_sums() const { return sums; }
+  const uint32_t* get_counts() const { return counts; }
+  const uint8_t* get_zapmask() const { return zapmask; }
+  ~SpectralKurtosis();
+
+ protected:
+  const TimeSeries* input = nullptr;
+  TimeSeries* output = nullptr;
+  std::unique_ptr<Engine> engine;
+  unsigned M = 128, detectors = 1;
+  bool omit_outliers = false, prepared = false;
+  std::vector<float> lower, upper, symmetric;
+  double pc_std_devs = 0;
+  uint64_t npart = 0, capacity = 0;
+  float* sums = nullptr;
+  uint32_t* counts = nullptr;
+  uint8_t* zapmask = nullptr;
 };
 
 class Detection {

@@ -1,6 +1,7 @@
 // dspx_pipeline -- a LoadToFold-style driver over the dsp::HIP engines.
 //
 // Reads one block of raw TFP samples from a file, runs
+//   filterbank : Unpacker -> Filterbank[HIP::FilterbankEngine] (writes the channels)
 //   chain : Unpacker -> Convolution[HIP::ConvolutionEngine] ->
 //           Detection[HIP::DetectionEngine] -> Fold[HIP::FoldEngine] per sub-integration
 //   fused : HIP::LoadToFoldEngine (three fused kernels)
@@ -51,11 +52,13 @@ try {
   unsigned nbit = std::stoul(need("nbit")), nchan = std::stoul(need("nchan")), npol = std::stoul(need("npol"));
   uint64_t ndat = std::stoull(need("ndat"));
   double cfreq = std::stod(need("cfreq")), bw = std::stod(need("bw")), dm = std::stod(need("dm"));
-  double period = std::stod(need("period"));
-  unsigned nbin = std::stoul(need("nbin")), nsub = std::stoul(need("nsub"));
-  Signal::State state = parse_state(need("state"));
-  unsigned ndim = std::stoul(a.count("ndim") ? a["ndim"] : "1");
   std::string mode = need("mode");
+  // folding options (not used by the filterbank mode)
+  auto fold_opt = [&](const char* k, const char* dflt) { return mode == "filterbank" && !a.count(k) ? dflt : need(k); };
+  double period = std::stod(fold_opt("period", "1"));
+  unsigned nbin = std::stoul(fold_opt("nbin", "1")), nsub = std::stoul(fold_opt("nsub", "1"));
+  Signal::State state = parse_state(fold_opt("state", "Intensity"));
+  unsigned ndim = std::stoul(a.count("ndim") ? a["ndim"] : "1");
 
   HIP::Context ctx(0);
   BitSeries raw(Memory::device());
@@ -78,6 +81,46 @@ try {
 
   std::vector<float> prof;
   std::vector<uint32_t> hits;
+  if (mode == "filterbank") {
+    // Unpacker -> Filterbank[HIP::FilterbankEngine] (FilterbankConfig.C:115-133);
+    // the output is the FPT time series [nchan_out][npol][ndat_out][2]
+    TimeSeries unpacked(Memory::device()), channels(Memory::device());
+    HIP::Unpacker unpacker(&ctx);
+    unpacker.unpack(&raw, &unpacked);
+    Filterbank filterbank;
+    filterbank.set_input(&unpacked);
+    filterbank.set_output(&channels);
+    unsigned nchan_out = std::stoul(need("fb_nchan"));
+    filterbank.set_nchan(nchan_out);
+    TableResponse table;
+    if (a.count("response")) {
+      // [nchan_out][fres] complex float in the Response's order
+      unsigned fres = std::stoul(need("fres"));
+      std::vector<float> t(size_t(nchan_out) * fres * 2);
+      std::ifstream f(a["response"], std::ios::binary);
+      if (!f.read(reinterpret_cast<char*>(t.data()), std::streamsize(t.size() * sizeof(float))))
+        throw Error(FailedCall, "dspx_pipeline", "short read of the response");
+      table.set_table(nchan_out, fres, std::stoul(need("nfilt_pos")), std::stoul(need("nfilt_neg")), std::move(t));
+      filterbank.set_response(&table);
+    } else {
+      filterbank.set_frequency_resolution(std::stoul(need("fres")));
+    }
+    filterbank.set_engine(new HIP::FilterbankEngine(&ctx));
+    filterbank.operate();
+    ctx.synchronize();
+    const size_t per = size_t(channels.get_ndat()) * 2;
+    std::vector<float> h(size_t(nchan_out) * npol * per);
+    for (unsigned c = 0; c < nchan_out; c++)
+      for (unsigned p = 0; p < npol; p++)
+        if (per && hipMemcpy(&h[(size_t(c) * npol + p) * per], channels.get_datptr(c, p), per * sizeof(float),
+                             hipMemcpyDeviceToHost) != hipSuccess)
+          throw Error(FailedCall, "dspx_pipeline", "hipMemcpy");
+    std::ofstream o(need("out"), std::ios::binary);
+    o.write(reinterpret_cast<const char*>(h.data()), std::streamsize(h.size() * sizeof(float)));
+    std::cout << "dspx_pipeline filterbank: " << nchan_out << " channels x " << channels.get_ndat() << " samples"
+              << std::endl;
+    return 0;
+  }
   if (mode == "chain") {
     TimeSeries unpacked(Memory::device()), filtered(Memory::device()), detected(Memory::device());
     HIP::Unpacker unpacker(&ctx);

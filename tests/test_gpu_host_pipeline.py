@@ -48,3 +48,43 @@ def test_host_pipeline(dspx, oracle, tmp_path, case, mode):
     print("profile rel rms %.3g" % e)
     assert e < 1e-5
     assert np.array_equal(hits, hits_o)
+
+
+@pytest.mark.parametrize("with_response", [False, True])
+def test_host_filterbank(dspx, oracle, tmp_path, with_response):
+    """Unpacker -> Filterbank[HIP::FilterbankEngine] through the C++ host
+    layer, against the oracle's FilterbankEngineCPU restatement on the same
+    unpacked samples: 2 input channels x 16 sub-bands of 128 bins, with no
+    response (the normaliser alone) and with a response table."""
+    assert os.path.exists(EXE), "dspx_pipeline not built"
+    rng = np.random.default_rng(17)
+    nchan, npol, nsub, fres = 2, 2, 16, 128
+    pos, neg = (10, 12) if with_response else (0, 0)
+    nfft = nsub * fres
+    ndat = 6 * nfft + 300
+    raw = rng.integers(-60, 61, (ndat, nchan, npol, 2)).astype(np.int8)
+    rawf = tmp_path / "raw.bin"
+    raw.tofile(rawf)
+    outf = tmp_path / "fb.bin"
+    cmd = [EXE, "--raw", str(rawf), "--nbit", "8", "--nchan", str(nchan), "--npol", str(npol), "--ndat", str(ndat),
+           "--cfreq", "1400.0", "--bw", "64.0", "--dm", "0.0", "--mode", "filterbank", "--fb_nchan",
+           str(nchan * nsub), "--fres", str(fres), "--out", str(outf)]
+    norm = np.float32(1.0 / np.sqrt(float(nfft) * fres))
+    if with_response:
+        table = np.exp(1j * rng.uniform(0, 2 * np.pi, (nchan * nsub, fres))).astype(np.complex64)
+        respf = tmp_path / "resp.bin"
+        table.tofile(respf)
+        cmd += ["--response", str(respf), "--nfilt_pos", str(pos), "--nfilt_neg", str(neg)]
+        resp = (table.view(np.float32) * norm).view(np.complex64).reshape(nchan, nfft)
+    else:
+        resp = np.full((nchan, nfft), norm, np.complex64)
+    print("host filterbank response=%s" % with_response, flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=300)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0
+    x = oracle.unpack8(raw.reshape(-1), nchan, npol, ndat).view(np.complex64)[..., 0]
+    expect = oracle.filterbank(x, nsub, fres, pos, neg, resp)
+    got = np.fromfile(outf, np.float32).view(np.complex64).reshape(expect.shape)
+    e = rel_rms(got, expect)
+    print("filterbank rel rms %.3g" % e)
+    assert e < 1e-5
