@@ -247,15 +247,43 @@ struct ConvCore {
 
 // Raw TFP [ndat][nchan] words (one word = both pols of one sample) to
 // channel-major [nchan][ndat] words, so that stage A reads each channel's
-// time samples contiguously.  64 x 64 word tiles through LDS; rows of the
-// input and of the output are both read / written as contiguous runs.
+// time samples contiguously.  64 x 64 word tiles through LDS.  Full tiles
+// with 16-byte aligned rows move 16-byte vectors: a 16-lane group reads 256
+// contiguous bytes of one time row and writes 256 contiguous bytes of one
+// channel row.  Edge tiles and unaligned layouts move single words.
 template <class W>
 __global__ __launch_bounds__(256) void k_tfp_to_ftp(const W* __restrict__ in, uint64_t ndat, uint32_t nchan,
-                                                    W* __restrict__ out, uint64_t out_cstride, uint64_t out_off)
+                                                    W* __restrict__ out, uint64_t out_cstride, uint64_t out_off,
+                                                    int vec)
 {
   __shared__ W tile[64][65];
   const uint64_t t0 = uint64_t(blockIdx.x) * 64;
   const uint32_t c0 = blockIdx.y * 64;
+  constexpr int VW = 16 / sizeof(W);  // words per vector
+  constexpr int VPR = 64 / VW;        // vectors per tile row
+  constexpr int RPI = 256 / VPR;      // tile rows per pass
+  if (vec && t0 + 64 <= ndat && c0 + 64 <= nchan) {
+    const uint32_t x = threadIdx.x % VPR, y = threadIdx.x / VPR;
+#pragma unroll
+    for (int k = 0; k < 64 / RPI; k++) {
+      const uint32_t i = y + RPI * k;
+      const uint4 v = *reinterpret_cast<const uint4*>(in + (t0 + i) * nchan + c0 + x * VW);
+      const W* w = reinterpret_cast<const W*>(&v);
+#pragma unroll
+      for (int j = 0; j < VW; j++) tile[i][x * VW + j] = w[j];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 64 / RPI; k++) {
+      const uint32_t i = y + RPI * k;
+      uint4 v;
+      W* w = reinterpret_cast<W*>(&v);
+#pragma unroll
+      for (int j = 0; j < VW; j++) w[j] = tile[x * VW + j][i];
+      *reinterpret_cast<uint4*>(out + uint64_t(c0 + i) * out_cstride + out_off + t0 + x * VW) = v;
+    }
+    return;
+  }
   const uint32_t tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
 #pragma unroll
   for (uint32_t k = 0; k < 16; k++) {
@@ -281,10 +309,16 @@ void tfp_to_ftp(dspx_ctx* ctx, const void* in, uint64_t ndat, uint32_t nchan, ui
 {
   DSPX_REQUIRE((ndat + 63) / 64 < (1ull << 31), DSPX_E_UNSUPPORTED, "ndat=%llu", (unsigned long long) ndat);
   dim3 grid(unsigned((ndat + 63) / 64), (nchan + 63) / 64);
+  // vectors need every row start (input times, output channels) 16-byte aligned
+  const uint64_t vw = 16 / wbytes;
+  const int vec = (reinterpret_cast<uintptr_t>(in) % 16 == 0 && reinterpret_cast<uintptr_t>(out) % 16 == 0 &&
+                   nchan % vw == 0 && out_cstride % vw == 0 && out_off % vw == 0)
+                      ? 1
+                      : 0;
   switch (wbytes) {
-#define CASE_W(b, T)                                                                                           \
-  case b:                                                                                                      \
-    k_tfp_to_ftp<T><<<grid, 256, 0, ctx->stream>>>((const T*) in, ndat, nchan, (T*) out, out_cstride, out_off); \
+#define CASE_W(b, T)                                                                                                \
+  case b:                                                                                                           \
+    k_tfp_to_ftp<T><<<grid, 256, 0, ctx->stream>>>((const T*) in, ndat, nchan, (T*) out, out_cstride, out_off, vec); \
     break;
     CASE_W(2, uint16_t) CASE_W(4, uint32_t) CASE_W(8, uint64_t)
 #undef CASE_W
@@ -486,6 +520,7 @@ struct dspx_ltf {
   // fused fold partials
   uint32_t G = 1, nslot = 1;
   DevBuf sub_first;
+  DevBuf keys;  // [npart][nfft] histogram key of each transform output, ~0 for discarded or unfolded
   LtfScratch own;
   LtfScratch* ext = nullptr;
   LtfScratch& sc() { return ext ? *ext : own; }
@@ -610,6 +645,39 @@ void ltf_reserve(dspx_ltf* L)
 }
 
 
+// The fold plan in the order stage C meets it: for each part p and each
+// output t of its transform, the LDS histogram key ((isub - sub_first[p]) <<
+// 16) | ibin of output sample p*step + t - nfilt_pos, or ~0 where the sample
+// is discarded (the overlap) or not folded.  Indexed like the work array, a
+// thread's consecutive columns are consecutive keys (one vector load).
+__global__ void k_part_keys(const uint32_t* __restrict__ plan, const uint32_t* __restrict__ sub_first,
+                            uint32_t npart, uint32_t nfft, uint32_t step, uint32_t nfilt_pos,
+                            uint32_t* __restrict__ keys)
+{
+  const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= uint64_t(npart) * nfft) return;
+  const uint32_t p = uint32_t(i / nfft), t = uint32_t(i % nfft);
+  uint32_t key = 0xffffffffu;
+  if (t >= nfilt_pos && t - nfilt_pos < step) {
+    const uint32_t pl = plan[uint64_t(p) * step + (t - nfilt_pos)];
+    if (pl != 0xffffffffu) key = (((pl >> 16) - sub_first[p]) << 16) | (pl & 0xffffu);
+  }
+  keys[i] = key;
+}
+
+// After the sample plan and sub_first of the fused fold are in place.
+void build_keys(dspx_ltf* L)
+{
+  const uint64_t n = L->npart * L->d.nfft;
+  if (!n) return;
+  DSPX_REQUIRE((n + 255) / 256 < (1ull << 31), DSPX_E_UNSUPPORTED, "key plan too long");
+  L->keys.reserve(n * sizeof(uint32_t));
+  k_part_keys<<<unsigned((n + 255) / 256), 256, 0, L->ctx->stream>>>(
+      L->fold.plan.as<uint32_t>(), L->sub_first.as<uint32_t>(), uint32_t(L->npart), L->d.nfft, L->step,
+      L->d.nfilt_pos, L->keys.as<uint32_t>());
+  DSPX_HIP(hipGetLastError());
+}
+
 // Stages B and C, the profile reduction and the hits of a fold execute
 // (stage A has run on np parts).
 void fused_fold_after_a(dspx_ltf* L, uint32_t np, float* d_prof, uint32_t* d_hits, int& ev)
@@ -621,9 +689,8 @@ void fused_fold_after_a(dspx_ltf* L, uint32_t np, float* d_prof, uint32_t* d_hit
   ltf_record(L, ev++);
   L->stage_name[L->nstage++] = "stageB_rowfft_chirp";
   FoldArgs fa;
-  fa.binplan = L->fold.plan.as<uint32_t>();
+  fa.keys = L->keys.as<uint32_t>();
   fa.partial = L->sc().partial.as<float>();
-  fa.sub_first = L->sub_first.as<uint32_t>();
   fa.nbin = d.nbin;
   fa.nval = L->nval;
   fa.ndim = L->ndim_out;
@@ -709,6 +776,7 @@ int dspx_ltf_set_fold(dspx_ltf* L, double pps, const dspx_fold_chunk* chunks, ui
       L->G = G;
       L->sub_first.reserve(sf.size() * sizeof(uint32_t));
       DSPX_HIP(hipMemcpy(L->sub_first.p, sf.data(), sf.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+      build_keys(L);
       L->sc().partial.reserve(units * G * hist_bytes);
     }
   });
@@ -1026,6 +1094,7 @@ double time_per_sample(dspx_batch* B, const dspx_ltf_desc& dn)
   std::vector<uint32_t> sf(L->npart, 0);
   L->sub_first.reserve(sf.size() * sizeof(uint32_t));
   DSPX_HIP(hipMemcpy(L->sub_first.p, sf.data(), sf.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+  build_keys(L.get());
   uint64_t units = uint64_t(d.nchan) * L->npart;
   L->G = uint32_t(std::min<uint64_t>(L->core.ncb(), std::max<uint64_t>(1, (2048 + units - 1) / units)));
   L->sc().partial.reserve(units * L->G * L->nval * d.nbin * sizeof(float));

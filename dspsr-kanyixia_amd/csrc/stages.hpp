@@ -132,9 +132,8 @@ struct ChirpArgs {
 };
 
 struct FoldArgs {
-  const uint32_t* binplan;   // per output sample: (isub << 16) | bin, ~0 skip
+  const uint32_t* keys;      // [npart][nfft]: (slot << 16) | bin of each transform output, ~0 skip
   float* partial;            // [nchan][npart][G][nslot][nval*nbin]
-  const uint32_t* sub_first; // per part: first sub-integration its outputs touch
   uint32_t nbin, nval, ndim, nslot, G;
   uint32_t state, npol_out;
   // profile offset of detected value i of bin b: voff[i] + b * ndim
@@ -423,49 +422,26 @@ __device__ __forceinline__ void detect_nv(bool stokes, uint32_t npol, float2 a, 
   }
 }
 
-// Adds the NV values of each lane into the LDS histogram at key = (slot <<
-// 16) | bin, summing equal keys across the wavefront first.  The common case
-// (every lane of the wavefront in one bin) is a butterfly reduction and one
-// atomic per value; otherwise a segmented scan over runs of equal keys.
-// Every lane must call it.
-template <int NV>
-__device__ __forceinline__ void wave_fold(uint32_t key, float* d, float* hist, const FoldArgs& fa)
+// QN consecutive keys starting at a multiple of QN (vector loads)
+template <int QN>
+__device__ __forceinline__ void load_keys(const uint32_t* __restrict__ k, uint32_t* out)
 {
-  const uint32_t k0 = __builtin_amdgcn_readfirstlane(key);
-  if (__all(key == k0)) {
-    if (k0 == 0xffffffffu) return;
+  if constexpr (QN == 1) {
+    out[0] = k[0];
+  } else if constexpr (QN == 2) {
+    const uint2 v = *reinterpret_cast<const uint2*>(k);
+    out[0] = v.x;
+    out[1] = v.y;
+  } else {
+    static_assert(QN % 4 == 0, "key vectors of 1, 2 or 4k words");
 #pragma unroll
-    for (int i = 0; i < NV; i++) d[i] = wave_sum_lane63(d[i]);
-    if ((threadIdx.x & 63u) == 63) {
-      float* hs = hist + (k0 >> 16) * (NV * fa.nbin) + (k0 & 0xffffu) * fa.ndim;
-#pragma unroll
-      for (int i = 0; i < NV; i++) atomicAdd(hs + fa.voff[i], d[i]);
+    for (int i = 0; i < QN; i += 4) {
+      const uint4 v = *reinterpret_cast<const uint4*>(k + i);
+      out[i] = v.x;
+      out[i + 1] = v.y;
+      out[i + 2] = v.z;
+      out[i + 3] = v.w;
     }
-    return;
-  }
-  const int lane = int(threadIdx.x & 63u);
-  uint32_t kprev = __shfl_up(key, 1);
-  int head = (lane == 0 || kprev != key) ? lane : 0;
-#pragma unroll
-  for (int off = 1; off < 64; off <<= 1) {
-    int h = __shfl_up(head, off);
-    if (lane >= off) head = max(head, h);
-  }
-#pragma unroll
-  for (int i = 0; i < NV; i++) {
-    float v = d[i];
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-      float u = __shfl_up(v, off);
-      if (lane - off >= head) v += u;
-    }
-    d[i] = v;
-  }
-  uint32_t knext = __shfl_down(key, 1);
-  if ((lane == 63 || knext != key) && key != 0xffffffffu) {
-    float* hs = hist + (key >> 16) * (NV * fa.nbin) + (key & 0xffffu) * fa.ndim;
-#pragma unroll
-    for (int i = 0; i < NV; i++) atomicAdd(hs + fa.voff[i], d[i]);
   }
 }
 
@@ -543,7 +519,8 @@ __global__ __launch_bounds__(kColThreads) void k_stageC(ConvShape s, FilterbankA
   if (MODE == kStageCFold) {
     for (uint32_t i = tid; i < hist_len; i += kColThreads) hist[i] = 0.f;
   }
-  const uint32_t sub0 = MODE == kStageCFold ? fa.sub_first[p] : 0;
+  // this part's keys, indexed like the transform outputs
+  const uint32_t* kp = MODE == kStageCFold ? fa.keys + uint64_t(p) * s.N1 * s.N2 : nullptr;
 
   const uint64_t plane = uint64_t(s.N1) * s.N2;
   const float2* wbase = s.work + (uint64_t(chan) * s.npart + p) * s.npol * plane;
@@ -639,13 +616,11 @@ __global__ __launch_bounds__(kColThreads) void k_stageC(ConvShape s, FilterbankA
         for (int i = 0; i < NV; i++) acc[i] = 0.f;
         uint32_t lane_key = 0xffffffffu;
         bool divergent = false;
+        uint32_t kq[QN];
+        load_keys<QN>(kp + t0, kq);
 #pragma unroll
         for (int q = 0; q < QN; q++) {
-          const uint32_t t = t0 + q;
-          const bool kept = t >= s.nfilt_pos && t < s.nfilt_pos + s.step;
-          const uint64_t idat = uint64_t(p) * s.step + (t - s.nfilt_pos);
-          const uint32_t plan = kept ? fa.binplan[idat] : 0xffffffffu;
-          const uint32_t key = plan == 0xffffffffu ? 0xffffffffu : (((plan >> 16) - sub0) << 16) | (plan & 0xffffu);
+          const uint32_t key = kq[q];
           float d[NV];
           detect_nv<NV>(fa.state == DSPX_STOKES, s.npol, va[q * RL + r], vb[q * RL + r], d);
           if (q == 0) lane_key = key;
