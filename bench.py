@@ -1,8 +1,9 @@
 """Throughput of the coherent-dedispersion + fold hot path on one MI355X.
 
-One step = one block of the 128-channel workload through the whole path
-(unpack -> overlap-save convolution -> full-Stokes detection -> fold into 1024
-bins x 8 sub-integrations), raw int8 input resident in HBM before timing.
+One step = one block of the 128-channel workload through the whole path: the
+block's fold plan (bin of every output sample, sub-integration slots, hits),
+then unpack -> overlap-save convolution -> full-Stokes detection -> fold into
+1024 bins x 8 sub-integrations, raw int8 input resident in HBM before timing.
 Prints one JSON line (see the keys below).
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload multichan|single|uneven]
@@ -52,28 +53,43 @@ def algorithmic_bytes(w, geom, npart):
 
 
 def time_path(dspx, ctx, w, geom, raw, form, steps, warmup, timing_steps=3):
+    """One step = the block's fold plan (phases -> bin plan, the per-block
+    set-up of Fold::transformation) and the block through the whole path."""
     import torch
     L = dspx.LoadToFold(ctx, w["nbit"], w["nchan"], w["npol"], w["ndat"], w["cfreq"], w["bw"], w["dm"],
                         w["nbin"], state=w["state"], ndim=w["ndim"], form=form, geom=geom)
     rate = abs(w["bw"]) / w["nchan"] * 1e6
     chunks = dspx.fold_chunks(L.ndat_out, rate, w["period"], geom["impulse_pos"], nsub=w["nsub"])
-    L.set_fold((1.0 / rate) / w["period"], chunks)
+    pps = (1.0 / rate) / w["period"]
     nval = L.nval
     prof = torch.zeros((w["nsub"], w["nchan"], nval * w["nbin"]), dtype=torch.float32, device="cuda")
     hits = torch.zeros((w["nsub"], w["nbin"]), dtype=torch.int32, device="cuda")
-    for _ in range(warmup):
+
+    def step():
+        L.set_fold(pps, chunks)
         L.execute(raw, prof, hits)
+
+    for _ in range(warmup):
+        step()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(steps):
-        L.execute(raw, prof, hits)
+        step()
     torch.cuda.synchronize()
     t = (time.perf_counter() - t0) / steps
-    # per-kernel durations from HIP events on the library's stream
+    # per-kernel durations from HIP events on the library's stream (torch's
+    # current stream here), the fold plan's set-up included
     L.set_timing(True)
     stage = {}
+    cur = torch.cuda.current_stream()
     for _ in range(timing_steps):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(cur)
+        L.set_fold(pps, chunks)
+        e1.record(cur)
         L.execute(raw, prof, hits)
+        torch.cuda.synchronize()
+        stage.setdefault("fold_plan", []).append(e0.elapsed_time(e1))
         for name, ms in L.stage_times():
             stage.setdefault(name, []).append(ms)
     L.set_timing(False)

@@ -221,8 +221,7 @@ struct ConvCore {
        also outside any graph capture that follows) */                                          \
     static bool lds_attr = false;                                                               \
     if (!lds_attr) {                                                                            \
-      DSPX_HIP(hipFuncSetAttribute((const void*) k_stageC<n, MODE, nv>,                         \
-                                   hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));    \
+      allow_full_lds((const void*) k_stageC<n, MODE, nv>);                                      \
       lds_attr = true;                                                                          \
     }                                                                                           \
     k_stageC<n, MODE, nv><<<unsigned(nb), kColThreads, shm, ctx->stream>>>(s, fb, fa, twc);     \
@@ -373,7 +372,7 @@ __global__ void k_zero_dm(const float* __restrict__ in, uint64_t ics, uint64_t i
 // Sum the fused stage C partial histograms into the profiles, in a fixed
 // order (the result does not depend on scheduling):
 //   k_fold_reduce_g: part[chan][p][slot][e] = sum over groups g
-//   k_fold_reduce_p: prof[isub][chan][e] += sum over parts p touching isub
+//   k_fold_reduce_p: prof[isub][chan][e] += sum over parts p whose slots hold isub
 __global__ void k_fold_reduce_g(const float* __restrict__ partial, uint32_t G, uint32_t nslot_hl, uint64_t n,
                                 float* __restrict__ out)
 {
@@ -382,26 +381,169 @@ __global__ void k_fold_reduce_g(const float* __restrict__ partial, uint32_t G, u
   uint64_t cp = i / nslot_hl, r = i % nslot_hl;
   const float* b = partial + cp * G * nslot_hl + r;
   float acc = 0.f;
+#pragma unroll 8
   for (uint32_t g = 0; g < G; g++) acc += b[uint64_t(g) * nslot_hl];
   out[i] = acc;
 }
 
-__global__ void k_fold_reduce_p(const float* __restrict__ part, const uint32_t* __restrict__ sub_first,
-                                uint32_t npart, uint32_t nslot, uint32_t hl, uint32_t nchan, float* __restrict__ prof)
+__global__ void k_fold_reduce_p(const float* __restrict__ part, const uint32_t* __restrict__ meta, uint32_t meta_off,
+                                uint32_t meta_parts, uint32_t npart, uint32_t nslot, uint32_t hl, uint32_t nchan,
+                                float* __restrict__ prof)
 {
   uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
   uint32_t chan = blockIdx.y;
   uint32_t isub = blockIdx.z;
   if (e >= hl) return;
+  const uint32_t* sub_first = meta;
+  const uint32_t k0 = meta[meta_off + isub], k1 = meta[meta_off + isub + 1];
+  if (k0 == k1) return;
   float acc = 0.f;
-  bool any = false;
-  for (uint32_t p = 0; p < npart; p++) {
-    uint32_t s0 = sub_first[p];
-    if (isub < s0 || isub >= s0 + nslot) continue;
-    acc += part[((uint64_t(chan) * npart + p) * nslot + (isub - s0)) * hl + e];
-    any = true;
+#pragma unroll 8
+  for (uint32_t k = k0; k < k1; k++) {
+    const uint32_t p = meta[meta_parts + k];
+    acc += part[((uint64_t(chan) * npart + p) * nslot + (isub - sub_first[p])) * hl + e];
   }
-  if (any) prof[(uint64_t(isub) * nchan + chan) * hl + e] += acc;
+  prof[(uint64_t(isub) * nchan + chan) * hl + e] += acc;
+}
+
+// The fold plan in the order stage C meets it, evaluated from the plan's
+// segments: for each part p and each output t of its transform, the LDS
+// histogram key ((isub - sub_first[p]) << 16) | ibin of output sample
+// p*step + t - nfilt_pos, or ~0 where the sample is discarded (the overlap)
+// or not folded.  Indexed like the work array, so a thread of stage C reads
+// its consecutive columns' keys with one vector load.  The same pass counts
+// the plan's hits (Fold.C:827-836): each thread takes kKeyRun consecutive
+// outputs, adds runs of equal keys into an LDS histogram of the part's slots,
+// and the workgroup adds its non-zero bins into hits[isub][ibin].  Within a
+// segment the bin never decreases, so a thread whose outputs lie in one
+// segment and whose first and last keys agree evaluates two samples, not 16.
+constexpr int kKeyRun = 16;
+constexpr int kKeyChunk = 256 * kKeyRun;  // outputs per workgroup, inside one part
+__global__ __launch_bounds__(256) void k_fused_keys(const FoldSegment* __restrict__ seg, uint32_t nseg, double nbin_d,
+                                                    uint32_t nbin, const uint32_t* __restrict__ sub_first,
+                                                    uint32_t nslot, uint32_t nfft, uint32_t step, uint32_t nfilt_pos,
+                                                    uint64_t n, uint32_t* __restrict__ keys,
+                                                    uint32_t* __restrict__ hits)
+{
+  extern __shared__ uint32_t h[];  // nslot * nbin
+  // the segments from the one holding the workgroup's first kept output:
+  // kSegCache of them in LDS (a 4096-output chunk spans a few)
+  constexpr uint32_t kSegCache = 64;
+  __shared__ FoldSegment sseg[kSegCache];
+  __shared__ uint32_t s_si0;
+  __shared__ uint64_t s_jend;  // first sample past the cached segments
+  const uint64_t c0 = uint64_t(blockIdx.x) * kKeyChunk;
+  const uint32_t p = uint32_t(c0 / nfft);
+  const uint32_t s0 = sub_first[p];
+  const uint32_t hn = nslot * nbin;
+  for (uint32_t k = threadIdx.x; k < hn; k += blockDim.x) h[k] = 0;
+  if (threadIdx.x == 0) {
+    const uint32_t tc = uint32_t(c0 - uint64_t(p) * nfft);
+    const uint32_t tf = max(tc, nfilt_pos);
+    const uint64_t idat0 = uint64_t(p) * step + (tf < nfilt_pos + step ? tf - nfilt_pos : 0);
+    s_si0 = seg_find(seg, nseg, idat0);
+  }
+  __syncthreads();
+  const uint32_t si0 = s_si0;
+  const uint32_t ns = min(kSegCache, nseg - si0);
+  if (threadIdx.x < ns) sseg[threadIdx.x] = seg[si0 + threadIdx.x];
+  if (threadIdx.x == 0) s_jend = si0 + ns < nseg ? seg[si0 + ns].j0 : ~uint64_t(0);
+  __syncthreads();
+  const uint64_t jbeg = sseg[0].j0, jend = s_jend;
+  const uint64_t i0 = c0 + uint64_t(threadIdx.x) * kKeyRun;
+  uint32_t kv[kKeyRun];
+  // the thread's current segment, held in registers: its outputs are
+  // consecutive samples and nearly always lie in one segment
+  FoldSegment sg{};
+  uint64_t jlo = 1, jnext = 0;  // sg covers [jlo, jnext); empty at first
+  auto locate = [&](uint64_t idat) {
+    if (idat >= jlo && idat < jnext) return;
+    if (idat >= jbeg && idat < jend) {
+      uint32_t lo = 0, hi = ns;
+      while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (sseg[mid].j0 <= idat)
+          lo = mid;
+        else
+          hi = mid;
+      }
+      sg = sseg[lo];
+      jnext = lo + 1 < ns ? sseg[lo + 1].j0 : jend;
+    } else {
+      const uint32_t g = seg_find(seg, nseg, idat);
+      sg = seg[g];
+      jnext = g + 1 < nseg ? seg[g + 1].j0 : ~uint64_t(0);
+    }
+    jlo = sg.j0;
+  };
+  auto slot_key = [&](uint32_t pl) {
+    return pl == 0xffffffffu ? 0xffffffffu : (((pl >> 16) - s0) << 16) | (pl & 0xffffu);
+  };
+  // Common case: every output of the thread kept and in one segment, where
+  // the bin is non-decreasing, so equal keys at both ends mean one key.
+  bool uniform = false;
+  const uint32_t tt0 = uint32_t(i0 - uint64_t(p) * nfft);
+  if (i0 + kKeyRun <= n && tt0 >= nfilt_pos && tt0 - nfilt_pos + (kKeyRun - 1) < step) {
+    const uint64_t ia = uint64_t(p) * step + (tt0 - nfilt_pos), ib = ia + (kKeyRun - 1);
+    locate(ia);
+    if (ib < jnext) {
+      const uint32_t ka = seg_key(sg, ia, nbin_d);
+      if (ka == seg_key(sg, ib, nbin_d)) {
+        const uint32_t key = slot_key(ka);
+#pragma unroll
+        for (int k = 0; k < kKeyRun; k++) kv[k] = key;
+        if (key != 0xffffffffu && (key >> 16) < nslot)
+          atomicAdd(&h[(key >> 16) * nbin + (key & 0xffffu)], uint32_t(kKeyRun));
+        uniform = true;
+      }
+    }
+  }
+  if (!uniform) {
+    uint32_t cur = 0xffffffffu, cnt = 0;
+#pragma unroll
+    for (int k = 0; k < kKeyRun; k++) {
+      const uint64_t i = i0 + k;
+      const uint32_t t = uint32_t(i - uint64_t(p) * nfft);
+      uint32_t key = 0xffffffffu;
+      if (i < n && t >= nfilt_pos && t - nfilt_pos < step) {
+        const uint64_t idat = uint64_t(p) * step + (t - nfilt_pos);
+        locate(idat);
+        key = slot_key(seg_key(sg, idat, nbin_d));
+      }
+      kv[k] = key;
+      if (key != cur) {
+        if (cur != 0xffffffffu && (cur >> 16) < nslot) atomicAdd(&h[(cur >> 16) * nbin + (cur & 0xffffu)], cnt);
+        cur = key;
+        cnt = 0;
+      }
+      cnt++;
+    }
+    if (cur != 0xffffffffu && (cur >> 16) < nslot) atomicAdd(&h[(cur >> 16) * nbin + (cur & 0xffffu)], cnt);
+  }
+  if (i0 + kKeyRun <= n) {
+#pragma unroll
+    for (int q = 0; q < kKeyRun / 4; q++)
+      *reinterpret_cast<uint4*>(keys + i0 + 4 * q) = make_uint4(kv[4 * q], kv[4 * q + 1], kv[4 * q + 2], kv[4 * q + 3]);
+  } else {
+#pragma unroll
+    for (int k = 0; k < kKeyRun; k++)
+      if (i0 + k < n) keys[i0 + k] = kv[k];
+  }
+  __syncthreads();
+  for (uint32_t k = threadIdx.x; k < hn; k += blockDim.x)
+    if (h[k]) atomicAdd(&hits[uint64_t(s0 + k / nbin) * nbin + (k % nbin)], h[k]);
+}
+
+__global__ void k_fill_u32(uint32_t* __restrict__ p, uint64_t n, uint32_t v)
+{
+  uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+__global__ void k_add_u32(const uint32_t* __restrict__ src, uint64_t n, uint32_t* __restrict__ dst)
+{
+  uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] += src[i];
 }
 
 }  // namespace dspx
@@ -517,10 +659,16 @@ struct dspx_ltf {
   PlainConv plain;
   FoldPlan fold;
   bool have_fold = false;
-  // fused fold partials
+  // fused fold: partial-histogram geometry, the per-part key plan and the
+  // per-block metadata [sub_first: npart][off: nsub+1][parts: off[nsub]]
+  // (the parts whose slots hold sub-integration isub are parts[off[isub] ..
+  // off[isub+1]), in increasing order)
   uint32_t G = 1, nslot = 1;
-  DevBuf sub_first;
-  DevBuf keys;  // [npart][nfft] histogram key of each transform output, ~0 for discarded or unfolded
+  DevBuf keys;       // [npart][nfft] histogram key of each transform output, ~0 for discarded or unfolded
+  DevBuf foldmeta;
+  uint32_t meta_off = 0, meta_parts = 0;  // word offsets of off[] and parts[] in foldmeta
+  DevBuf plan_hits;  // [nsub][nbin] hits of the current fold plan
+  HostStage meta_stage;
   LtfScratch own;
   LtfScratch* ext = nullptr;
   LtfScratch& sc() { return ext ? *ext : own; }
@@ -645,36 +793,81 @@ void ltf_reserve(dspx_ltf* L)
 }
 
 
-// The fold plan in the order stage C meets it: for each part p and each
-// output t of its transform, the LDS histogram key ((isub - sub_first[p]) <<
-// 16) | ibin of output sample p*step + t - nfilt_pos, or ~0 where the sample
-// is discarded (the overlap) or not folded.  Indexed like the work array, a
-// thread's consecutive columns are consecutive keys (one vector load).
-__global__ void k_part_keys(const uint32_t* __restrict__ plan, const uint32_t* __restrict__ sub_first,
-                            uint32_t npart, uint32_t nfft, uint32_t step, uint32_t nfilt_pos,
-                            uint32_t* __restrict__ keys)
+// Per-block set-up of the fused fold, after L->fold holds this block's
+// segments: the sub-integration slots of each part, the partial-histogram
+// geometry, the metadata upload, the key plan and the plan's hits.  Nothing
+// here waits for the stream.
+void fused_fold_setup(dspx_ltf* L)
 {
-  const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i >= uint64_t(npart) * nfft) return;
-  const uint32_t p = uint32_t(i / nfft), t = uint32_t(i % nfft);
-  uint32_t key = 0xffffffffu;
-  if (t >= nfilt_pos && t - nfilt_pos < step) {
-    const uint32_t pl = plan[uint64_t(p) * step + (t - nfilt_pos)];
-    if (pl != 0xffffffffu) key = (((pl >> 16) - sub_first[p]) << 16) | (pl & 0xffffu);
+  dspx_ctx* ctx = L->ctx;
+  const uint64_t np = L->npart;
+  // sub-integration slots per part: the kept outputs of part p are samples
+  // [p*step, (p+1)*step)
+  std::vector<uint32_t> sf(np);
+  uint32_t nslot = 1;
+  for (uint64_t p = 0; p < np; p++) {
+    uint32_t first, last;
+    L->fold.sub_range(p * L->step, (p + 1) * L->step, first, last);
+    if (first == 0xffffffffu) {
+      first = 0;
+      last = 0;
+    }
+    sf[p] = first;
+    nslot = std::max(nslot, last - first + 1);
   }
-  keys[i] = key;
-}
+  L->nslot = nslot;
+  const size_t hist_bytes = size_t(nslot) * L->nval * L->d.nbin * sizeof(float);
+  DSPX_REQUIRE(kColTile * sizeof(float2) + hist_bytes <= 160 * 1024, DSPX_E_UNSUPPORTED,
+               "fold histogram of %zu bytes does not fit in LDS", hist_bytes);
+  const uint32_t nsub = L->fold.nsub;
+  // parts whose slots hold isub: sub_first[p] <= isub < sub_first[p] + nslot
+  std::vector<uint32_t> off(nsub + 1, 0), parts;
+  for (uint32_t isub = 0; isub < nsub; isub++) {
+    for (uint64_t p = 0; p < np; p++)
+      if (isub >= sf[p] && isub < sf[p] + nslot) parts.push_back(uint32_t(p));
+    off[isub + 1] = uint32_t(parts.size());
+  }
+  L->meta_off = uint32_t(np);
+  L->meta_parts = uint32_t(np + nsub + 1);
+  const size_t words = np + nsub + 1 + parts.size();
+  L->foldmeta.reserve(words * sizeof(uint32_t));
+  int slot;
+  auto* h = static_cast<uint32_t*>(L->meta_stage.acquire(words * sizeof(uint32_t), slot));
+  std::copy(sf.begin(), sf.end(), h);
+  std::copy(off.begin(), off.end(), h + L->meta_off);
+  std::copy(parts.begin(), parts.end(), h + L->meta_parts);
+  L->meta_stage.upload(slot, L->foldmeta.p, words * sizeof(uint32_t), ctx->stream);
 
-// After the sample plan and sub_first of the fused fold are in place.
-void build_keys(dspx_ltf* L)
-{
-  const uint64_t n = L->npart * L->d.nfft;
-  if (!n) return;
-  DSPX_REQUIRE((n + 255) / 256 < (1ull << 31), DSPX_E_UNSUPPORTED, "key plan too long");
+  // Column groups per (channel, part) in stage C.  Every stage C workgroup
+  // zeroes, writes and has reduced one histogram: with large histograms the
+  // grid is one round of the workgroups the device holds at once (LDS-bound),
+  // with small ones four rounds, for balance.
+  const uint32_t ncb = L->core.ncb();
+  const uint64_t units = uint64_t(L->d.nchan) * np;
+  const uint64_t per_cu = std::max<uint64_t>(1, (160 * 1024) / (kColTile * sizeof(float2) + hist_bytes));
+  const uint64_t slots = uint64_t(std::max(ctx->ncu, 1)) * per_cu;
+  const uint64_t target = hist_bytes > 16384 ? slots : 4 * slots;
+  L->G = uint32_t(std::min<uint64_t>(ncb, std::max<uint64_t>(1, (target + units - 1) / units)));
+  L->sc().partial.reserve(units * L->G * hist_bytes);
+
+  // key plan and hits
+  const uint64_t n = np * L->d.nfft;
+  const uint64_t nh = uint64_t(nsub) * L->d.nbin;
+  DSPX_REQUIRE(L->d.nfft % kKeyChunk == 0, DSPX_E_INTERNAL, "nfft=%u", L->d.nfft);
+  DSPX_REQUIRE(n / kKeyChunk < (1ull << 31), DSPX_E_UNSUPPORTED, "key plan too long");
   L->keys.reserve(n * sizeof(uint32_t));
-  k_part_keys<<<unsigned((n + 255) / 256), 256, 0, L->ctx->stream>>>(
-      L->fold.plan.as<uint32_t>(), L->sub_first.as<uint32_t>(), uint32_t(L->npart), L->d.nfft, L->step,
-      L->d.nfilt_pos, L->keys.as<uint32_t>());
+  L->plan_hits.reserve(std::max<uint64_t>(nh, 1) * sizeof(uint32_t));
+  k_fill_u32<<<unsigned((nh + 255) / 256), 256, 0, ctx->stream>>>(L->plan_hits.as<uint32_t>(), nh, 0u);
+  DSPX_HIP(hipGetLastError());
+  const size_t shm = size_t(nslot) * L->d.nbin * sizeof(uint32_t);
+  static bool lds_attr = false;
+  if (!lds_attr) {
+    allow_full_lds((const void*) k_fused_keys);
+    lds_attr = true;
+  }
+  k_fused_keys<<<unsigned(n / kKeyChunk), 256, shm, ctx->stream>>>(
+      L->fold.seg.as<FoldSegment>(), L->fold.nseg, double(L->d.nbin), L->d.nbin, L->foldmeta.as<uint32_t>(), nslot,
+      L->d.nfft, L->step, L->d.nfilt_pos, n, L->keys.as<uint32_t>(), L->plan_hits.as<uint32_t>());
   DSPX_HIP(hipGetLastError());
 }
 
@@ -704,15 +897,24 @@ void fused_fold_after_a(dspx_ltf* L, uint32_t np, float* d_prof, uint32_t* d_hit
   L->stage_name[L->nstage++] = "stageC_colifft_detect_fold";
   uint32_t hl = L->nval * d.nbin;
   uint64_t n2 = uint64_t(d.nchan) * np * L->nslot * hl;
-  L->sc().partial2.reserve(n2 * sizeof(float));
-  k_fold_reduce_g<<<unsigned((n2 + 255) / 256), 256, 0, ctx->stream>>>(
-      L->sc().partial.as<float>(), L->G, L->nslot * hl, n2, L->sc().partial2.as<float>());
-  DSPX_HIP(hipGetLastError());
+  // with one group the partials already are [chan][p][slot][e]
+  const float* per_part = L->sc().partial.as<float>();
+  if (L->G > 1) {
+    L->sc().partial2.reserve(n2 * sizeof(float));
+    k_fold_reduce_g<<<unsigned((n2 + 255) / 256), 256, 0, ctx->stream>>>(
+        L->sc().partial.as<float>(), L->G, L->nslot * hl, n2, L->sc().partial2.as<float>());
+    DSPX_HIP(hipGetLastError());
+    per_part = L->sc().partial2.as<float>();
+  }
   dim3 grid((hl + 255) / 256, d.nchan, L->fold.nsub);
-  k_fold_reduce_p<<<grid, 256, 0, ctx->stream>>>(L->sc().partial2.as<float>(), L->sub_first.as<uint32_t>(), np,
-                                                 L->nslot, hl, d.nchan, d_prof);
+  k_fold_reduce_p<<<grid, 256, 0, ctx->stream>>>(per_part, L->foldmeta.as<uint32_t>(),
+                                                 L->meta_off, L->meta_parts, np, L->nslot, hl, d.nchan, d_prof);
   DSPX_HIP(hipGetLastError());
-  if (d_hits) L->fold.hits(d_hits);
+  if (d_hits) {
+    const uint64_t nh = uint64_t(L->fold.nsub) * d.nbin;
+    k_add_u32<<<unsigned((nh + 255) / 256), 256, 0, ctx->stream>>>(L->plan_hits.as<uint32_t>(), nh, d_hits);
+    DSPX_HIP(hipGetLastError());
+  }
   ltf_record(L, ev++);
   L->stage_name[L->nstage++] = "fold_reduce_hits";
 }
@@ -749,36 +951,10 @@ int dspx_ltf_set_fold(dspx_ltf* L, double pps, const dspx_fold_chunk* chunks, ui
   dspx_ctx* ctx = L ? L->ctx : nullptr;
   return guard(ctx, [&] {
     DSPX_REQUIRE(L && chunks, DSPX_E_PARAM, "null argument");
-    L->fold.build(ctx, L->ndat_out, pps, L->d.nbin, chunks, nchunk);
+    const bool fused = L->d.form == DSPX_FORM_FUSED;
+    L->fold.build(ctx, L->ndat_out, pps, L->d.nbin, chunks, nchunk, !fused);
     L->have_fold = true;
-    if (L->d.form == DSPX_FORM_FUSED && L->npart) {
-      // sub-integration slots per part: the kept outputs of part p are
-      // samples [p*step, (p+1)*step)
-      std::vector<uint32_t> sf(L->npart);
-      uint32_t nslot = 1;
-      for (uint64_t p = 0; p < L->npart; p++) {
-        uint32_t first, last;
-        L->fold.sub_range(p * L->step, (p + 1) * L->step, first, last);
-        if (first == 0xffffffffu) {
-          first = 0;
-          last = 0;
-        }
-        sf[p] = first;
-        nslot = std::max(nslot, last - first + 1);
-      }
-      L->nslot = nslot;
-      size_t hist_bytes = size_t(nslot) * L->nval * L->d.nbin * sizeof(float);
-      DSPX_REQUIRE(kColTile * sizeof(float2) + hist_bytes <= 160 * 1024, DSPX_E_UNSUPPORTED,
-                   "fold histogram of %zu bytes does not fit in LDS", hist_bytes);
-      uint32_t ncb = L->core.ncb();
-      uint64_t units = uint64_t(L->d.nchan) * L->npart;
-      uint32_t G = uint32_t(std::min<uint64_t>(ncb, std::max<uint64_t>(1, (2048 + units - 1) / units)));
-      L->G = G;
-      L->sub_first.reserve(sf.size() * sizeof(uint32_t));
-      DSPX_HIP(hipMemcpy(L->sub_first.p, sf.data(), sf.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
-      build_keys(L);
-      L->sc().partial.reserve(units * G * hist_bytes);
-    }
+    if (fused && L->npart) fused_fold_setup(L);
   });
 }
 
@@ -1087,17 +1263,9 @@ double time_per_sample(dspx_batch* B, const dspx_ltf_desc& dn)
   std::unique_ptr<dspx_ltf> L(ltf_create(ctx, &d, &B->sc));
   if (!L->npart) return 1e30;
   dspx_fold_chunk ch{0, L->ndat_out, 0.0, 0, 0};
-  L->fold.build(ctx, L->ndat_out, 1.0 / 97.0e3, d.nbin, &ch, 1);
+  L->fold.build(ctx, L->ndat_out, 1.0 / 97.0e3, d.nbin, &ch, 1, false);
   L->have_fold = true;
-  // single sub-integration: the per-part slots and partials of set_fold
-  L->nslot = 1;
-  std::vector<uint32_t> sf(L->npart, 0);
-  L->sub_first.reserve(sf.size() * sizeof(uint32_t));
-  DSPX_HIP(hipMemcpy(L->sub_first.p, sf.data(), sf.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
-  build_keys(L.get());
-  uint64_t units = uint64_t(d.nchan) * L->npart;
-  L->G = uint32_t(std::min<uint64_t>(L->core.ncb(), std::max<uint64_t>(1, (2048 + units - 1) / units)));
-  L->sc().partial.reserve(units * L->G * L->nval * d.nbin * sizeof(float));
+  fused_fold_setup(L.get());
   ltf_reserve(L.get());
   const uint64_t raw_bytes = d.ndat * d.nchan * d.npol * 2 * (d.nbit / 8);
   B->tune_raw.reserve(raw_bytes);

@@ -156,6 +156,7 @@ int dspx_ctx_create(int device, void* stream, dspx_ctx** out)
     auto* c = new dspx_ctx;
     c->device = device;
     try {
+      DSPX_HIP(hipDeviceGetAttribute(&c->ncu, hipDeviceAttributeMultiprocessorCount, device));
       if (stream) {
         c->stream = static_cast<hipStream_t>(stream);
       } else {

@@ -12,6 +12,7 @@
 
 struct dspx_ctx {
   int device = 0;
+  int ncu = 0;  // compute units of the device
   hipStream_t stream = nullptr;
   bool own_stream = false;
   std::string last_error;
@@ -31,12 +32,17 @@ struct Error {
 
 [[noreturn]] void fail(int code, const char* fmt, ...);
 
+// A failed call's error is reported through the status and also cleared
+// from the thread's last-error slot, so that the caller's next
+// hipGetLastError (a launch check of its own) does not see it.
 #define DSPX_HIP(call)                                                                   \
   do {                                                                                   \
     hipError_t e_ = (call);                                                              \
-    if (e_ != hipSuccess)                                                                \
+    if (e_ != hipSuccess) {                                                              \
+      (void) hipGetLastError();                                                          \
       ::dspx::fail(DSPX_E_HIP, "%s failed: %s (%s:%d)", #call, hipGetErrorString(e_),   \
                    __FILE__, __LINE__);                                                  \
+    }                                                                                    \
   } while (0)
 
 #define DSPX_REQUIRE(cond, code, ...)            \
@@ -61,6 +67,16 @@ template <class F> int guard(dspx_ctx* ctx, F&& f)
 }
 
 inline const float2* twiddles(const dspx_ctx* ctx, int L) { return ctx->d_tw + L; }
+
+// Lets a kernel request, as dynamic LDS, the whole LDS of a workgroup
+// (160 KB on gfx950) less its static LDS; requests above 64 KB need this.
+inline void allow_full_lds(const void* kernel)
+{
+  hipFuncAttributes a{};
+  DSPX_HIP(hipFuncGetAttributes(&a, kernel));
+  DSPX_HIP(hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               int(160 * 1024 - a.sharedSizeBytes)));
+}
 
 // Two-level table of W_N^m = exp(-2 pi i m / N): lo[m & (2^h - 1)] * hi[m >> h].
 struct Tw2 {
@@ -90,6 +106,50 @@ struct DevBuf {
   }
   ~DevBuf() { release(); }
   template <class T> T* as() const { return static_cast<T*>(p); }
+};
+
+// Pinned host staging for small asynchronous uploads made every block (fold
+// plans): two slots used in turn, a slot reused only after the copy that last
+// read it has completed, so that the caller never waits for the stream.
+struct HostStage {
+  void* slot[2] = {};
+  size_t cap[2] = {};
+  hipEvent_t done[2] = {};
+  bool pending[2] = {};
+  int next = 0;
+  // a pinned buffer of n bytes for the caller to fill before upload(s, ...)
+  void* acquire(size_t n, int& s)
+  {
+    s = next;
+    next ^= 1;
+    if (pending[s]) {
+      DSPX_HIP(hipEventSynchronize(done[s]));
+      pending[s] = false;
+    }
+    if (cap[s] < n) {
+      if (slot[s]) (void) hipHostFree(slot[s]);
+      slot[s] = nullptr;
+      cap[s] = 0;
+      DSPX_HIP(hipHostMalloc(&slot[s], n, hipHostMallocDefault));
+      cap[s] = n;
+    }
+    if (!done[s]) DSPX_HIP(hipEventCreateWithFlags(&done[s], hipEventDisableTiming));
+    return slot[s];
+  }
+  void upload(int s, void* dst, size_t n, hipStream_t st)
+  {
+    DSPX_HIP(hipMemcpyAsync(dst, slot[s], n, hipMemcpyHostToDevice, st));
+    DSPX_HIP(hipEventRecord(done[s], st));
+    pending[s] = true;
+  }
+  ~HostStage()
+  {
+    for (int s = 0; s < 2; s++) {
+      if (pending[s]) (void) hipEventSynchronize(done[s]);
+      if (done[s]) (void) hipEventDestroy(done[s]);
+      if (slot[s]) (void) hipHostFree(slot[s]);
+    }
+  }
 };
 
 // ---- engines shared between entry points (ctx.hip) ----

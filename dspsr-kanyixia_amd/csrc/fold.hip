@@ -6,6 +6,7 @@
 // walks binade runs (binplan.cpp) and the device evaluates every sample.
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 
 #include "dspx_internal.hpp"
 #include "fft_core.hpp"
@@ -18,23 +19,7 @@ __global__ void k_binplan(const FoldSegment* __restrict__ seg, uint32_t nseg, ui
 {
   uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= ndat) return;
-  uint32_t lo = 0, hi = nseg;
-  while (hi - lo > 1) {
-    uint32_t mid = (lo + hi) >> 1;
-    if (seg[mid].j0 <= i)
-      lo = mid;
-    else
-      hi = mid;
-  }
-  FoldSegment sg = seg[lo];
-  if (sg.isub == 0xffffffffu) {
-    out[i] = 0xffffffffu;
-    return;
-  }
-  uint64_t m = sg.M + (i - sg.j0) * sg.D;
-  double x = ldexp(double(m), sg.e - 52);
-  uint32_t bin = uint32_t(x * nbin_d);
-  out[i] = (sg.isub << 16) | bin;
+  out[i] = seg_key(seg[seg_find(seg, nseg, i)], i, nbin_d);
 }
 
 // hits[isub][bin] += count (Fold.C:827-836 for unzapped data).  A workgroup
@@ -152,7 +137,7 @@ __global__ __launch_bounds__(256) void k_fold_fpt(const float* __restrict__ in, 
 }
 
 void FoldPlan::build(dspx_ctx* c, uint64_t ndat_, double pps_, uint32_t nbin_,
-                     const dspx_fold_chunk* chunks, uint32_t nchunk)
+                     const dspx_fold_chunk* chunks, uint32_t nchunk, bool sample_plan)
 {
   ctx = c;
   ndat = ndat_;
@@ -168,7 +153,6 @@ void FoldPlan::build(dspx_ctx* c, uint64_t ndat_, double pps_, uint32_t nbin_,
   std::vector<FoldSegment> segs;
   uint64_t cursor = 0;
   nsub = 0;
-  sub_of_sample_first.clear();
   for (auto& ch : cs) {
     DSPX_REQUIRE(ch.idat_start >= cursor, DSPX_E_PARAM, "fold chunks overlap at sample %llu",
                  (unsigned long long) ch.idat_start);
@@ -184,10 +168,13 @@ void FoldPlan::build(dspx_ctx* c, uint64_t ndat_, double pps_, uint32_t nbin_,
   if (cursor < ndat || segs.empty()) segs.push_back(FoldSegment{cursor, 0, 0, 0, 0xffffffffu});
   nseg = uint32_t(segs.size());
   seg.reserve(segs.size() * sizeof(FoldSegment));
-  plan.reserve(std::max<uint64_t>(ndat, 1) * sizeof(uint32_t));
-  DSPX_HIP(hipMemcpyAsync(seg.p, segs.data(), segs.size() * sizeof(FoldSegment),
-                          hipMemcpyHostToDevice, ctx->stream));
-  if (ndat) {
+  int slot;
+  void* h = stage.acquire(segs.size() * sizeof(FoldSegment), slot);
+  std::memcpy(h, segs.data(), segs.size() * sizeof(FoldSegment));
+  stage.upload(slot, seg.p, segs.size() * sizeof(FoldSegment), ctx->stream);
+  have_sample_plan = sample_plan;
+  if (sample_plan && ndat) {
+    plan.reserve(ndat * sizeof(uint32_t));
     unsigned nb = unsigned((ndat + 255) / 256);
     k_binplan<<<nb, 256, 0, ctx->stream>>>(seg.as<FoldSegment>(), nseg, ndat, double(nbin),
                                            plan.as<uint32_t>());
@@ -195,12 +182,12 @@ void FoldPlan::build(dspx_ctx* c, uint64_t ndat_, double pps_, uint32_t nbin_,
   }
   // host copy of the chunk table for the fused path's sub-integration slots
   chunk_table = cs;
-  DSPX_HIP(hipStreamSynchronize(ctx->stream));  // segs (host vector) outlives the copy
 }
 
 void FoldPlan::hits(uint32_t* d_hits)
 {
   if (!ndat) return;
+  DSPX_REQUIRE(have_sample_plan, DSPX_E_INTERNAL, "hits without a sample plan");
   unsigned nb = unsigned((ndat + kHitsChunk - 1) / kHitsChunk);
   k_hits<<<nb, 256, 2 * nbin * sizeof(uint32_t), ctx->stream>>>(plan.as<uint32_t>(), ndat, nbin,
                                                                    d_hits);
@@ -211,6 +198,7 @@ void FoldPlan::fold_fpt(const float* d_in, uint64_t chan_stride, uint64_t pol_st
                         uint32_t npol, uint32_t ndim, float* d_prof)
 {
   if (!ndat) return;
+  DSPX_REQUIRE(have_sample_plan, DSPX_E_INTERNAL, "fold without a sample plan");
   DSPX_REQUIRE(ndim >= 1 && ndim <= 4, DSPX_E_PARAM, "ndim=%u", ndim);
   size_t shm = size_t(2) * nbin * ndim * sizeof(float);
   DSPX_REQUIRE(shm <= 64 * 1024, DSPX_E_UNSUPPORTED, "nbin*ndim=%u too large for LDS", nbin * ndim);

@@ -15,12 +15,15 @@ struct FoldPlan {
   uint32_t nsub = 0;
   uint32_t nseg = 0;
   DevBuf seg;   // FoldSegment[nseg]
-  DevBuf plan;  // uint32[ndat]: (isub << 16) | bin, ~0 = not folded
+  DevBuf plan;  // uint32[ndat]: (isub << 16) | bin, ~0 = not folded (with a sample plan)
+  bool have_sample_plan = false;
+  HostStage stage;
   std::vector<dspx_fold_chunk> chunk_table;
-  std::vector<uint32_t> sub_of_sample_first;
 
+  // Segments of the chunks (uploaded without waiting for the stream) and,
+  // with sample_plan, the per-sample plan evaluated from them.
   void build(dspx_ctx* c, uint64_t ndat, double pps, uint32_t nbin, const dspx_fold_chunk* chunks,
-             uint32_t nchunk);
+             uint32_t nchunk, bool sample_plan = true);
   void hits(uint32_t* d_hits);
   void fold_fpt(const float* d_in, uint64_t chan_stride, uint64_t pol_stride, uint32_t nchan,
                 uint32_t npol, uint32_t ndim, float* d_prof);
@@ -38,5 +41,31 @@ struct FoldPlan {
     }
   }
 };
+
+// Bin-plan entry of output sample i in segment sg (bit-identical wherever the
+// plan is evaluated): (isub << 16) | bin, or ~0 for samples not folded.
+__device__ __forceinline__ uint32_t seg_key(const FoldSegment& sg, uint64_t i, double nbin_d)
+{
+  if (sg.isub == 0xffffffffu) return 0xffffffffu;
+  uint64_t m = sg.M + (i - sg.j0) * sg.D;
+  double x = ldexp(double(m), sg.e - 52);
+  uint32_t bin = uint32_t(x * nbin_d);
+  return (sg.isub << 16) | bin;
+}
+
+// Index of the segment holding output sample i (segments sorted by j0, the
+// first at j0 = 0).
+__device__ __forceinline__ uint32_t seg_find(const FoldSegment* __restrict__ seg, uint32_t nseg, uint64_t i)
+{
+  uint32_t lo = 0, hi = nseg;
+  while (hi - lo > 1) {
+    uint32_t mid = (lo + hi) >> 1;
+    if (seg[mid].j0 <= i)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  return lo;
+}
 
 }  // namespace dspx
