@@ -31,7 +31,7 @@ class DspxError(RuntimeError):
 class DedispDesc(C.Structure):
     _fields_ = [("centre_frequency", C.c_double), ("bandwidth", C.c_double),
                 ("dispersion_measure", C.c_double), ("nchan", C.c_uint32),
-                ("dc_centred", C.c_uint32), ("nfft", C.c_uint32), ("reserved", C.c_uint32)]
+                ("dc_centred", C.c_uint32), ("nfft", C.c_uint32), ("interchan", C.c_uint32)]
 
 
 class Geometry(C.Structure):
@@ -83,6 +83,10 @@ _SIGS = {
     "dspx_ctx_synchronize": (I32, [P]),
     "dspx_ctx_stream": (P, [P]),
     "dspx_dedisp_geometry": (I32, [C.POINTER(DedispDesc), C.POINTER(Geometry)]),
+    "dspx_sample_delays": (I32, [C.POINTER(DedispDesc), P, P, P, C.POINTER(C.c_int64), C.POINTER(U64)]),
+    "dspx_delay_create": (I32, [P, U32, U32, P, C.POINTER(P)]),
+    "dspx_delay_destroy": (None, [P]),
+    "dspx_delay_retard": (I32, [P, P, U64, U64, U32, U64, P, U64, U64]),
     "dspx_unpack": (I32, [P, P, U32, U32, U32, U32, U64, C.c_float, P, U64, U64]),
     "dspx_conv_create": (I32, [P, C.POINTER(ConvDesc), C.POINTER(P)]),
     "dspx_conv_destroy": (None, [P]),
@@ -262,19 +266,65 @@ class Context:
                                      nchan, npol, ndat, ptr(out), out_chan_stride, out_pol_stride))
 
 
-def _dedisp(centre_frequency, bandwidth, dm, nchan, dc_centred=False):
-    return DedispDesc(centre_frequency, bandwidth, dm, nchan, int(dc_centred), 0, 0)
+def _dedisp(centre_frequency, bandwidth, dm, nchan, dc_centred=False, interchan=False):
+    return DedispDesc(centre_frequency, bandwidth, dm, nchan, int(dc_centred), 0, int(interchan))
+
+
+def sample_delays(centre_frequency, bandwidth, nchan, dm, dc_centred=False):
+    """SampleDelay with the dispersion delay function (host only): per channel
+    the integer and fractional delays relative to the band centre and the
+    applied input offset; the zero delay and the total delay."""
+    import numpy as np
+    d = _dedisp(centre_frequency, bandwidth, dm, nchan, dc_centred, True)
+    delay = np.zeros(nchan, np.int64)
+    frac = np.zeros(nchan, np.float64)
+    applied = np.zeros(nchan, np.int64)
+    zd, tot = C.c_int64(0), U64(0)
+    rc = lib().dspx_sample_delays(C.byref(d), delay.ctypes.data, frac.ctypes.data, applied.ctypes.data,
+                                  C.byref(zd), C.byref(tot))
+    if rc != OK:
+        raise DspxError(rc, "sample_delays: invalid band or a negative offset")
+    return {"delay": delay, "frac": frac, "applied": applied, "zero_delay": zd.value, "total": tot.value}
+
+
+class SampleDelay:
+    """SampleDelay::Engine: fixed per-(channel, pol) input offsets."""
+
+    def __init__(self, ctx, nchan, npol, applied):
+        import numpy as np
+        self.ctx = ctx
+        self.h = None
+        self._lib = lib()
+        a = np.ascontiguousarray(np.asarray(applied, np.int64).reshape(nchan, -1))
+        if a.shape[1] == 1 and npol > 1:
+            a = np.ascontiguousarray(np.repeat(a, npol, axis=1))
+        h = P()
+        ctx.check(lib().dspx_delay_create(ctx.h, nchan, npol, a.ctypes.data, C.byref(h)))
+        self.h = h
+        _register(self, False)
+
+    def close(self):
+        if self.h:
+            self._lib.dspx_delay_destroy(self.h)
+            self.h = None
+
+    __del__ = close
+
+    def retard(self, x, in_chan_stride, in_pol_stride, ndim, ndat_out, out, out_chan_stride, out_pol_stride):
+        self.ctx.check(lib().dspx_delay_retard(self.h, ptr(x), in_chan_stride, in_pol_stride, ndim, ndat_out,
+                                               ptr(out), out_chan_stride, out_pol_stride))
 
 
 class Convolution:
     """Convolution::Engine (fused or plain form)."""
 
     def __init__(self, ctx, nchan, npol, nfft, nfilt_pos, nfilt_neg, chirp_mode=CHIRP_PARITY,
-                 form=FORM_FUSED, centre_frequency=0.0, bandwidth=0.0, dm=0.0, dc_centred=False):
+                 form=FORM_FUSED, centre_frequency=0.0, bandwidth=0.0, dm=0.0, dc_centred=False,
+                 interchan=False):
         self.ctx = ctx
         self.h = None
         self._lib = lib()
-        d = ConvDesc(_dedisp(centre_frequency, bandwidth, dm, nchan, dc_centred), nchan, npol, nfft,
+        d = ConvDesc(_dedisp(centre_frequency, bandwidth, dm, nchan, dc_centred, interchan), nchan, npol, nfft,
                      nfilt_pos, nfilt_neg, chirp_mode, form, 0)
         h = P()
         ctx.check(lib().dspx_conv_create(ctx.h, C.byref(d), C.byref(h)))

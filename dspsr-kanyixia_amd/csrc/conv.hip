@@ -43,7 +43,7 @@ struct ConvCore {
   dspx_dedisp_desc dd{};
   DevBuf tw2;      // lo[2^h] then hi[N >> h]
   int h = 0;
-  DevBuf chan_par; // coeff[nchan], cf[nchan]
+  DevBuf chan_par; // coeff[nchan], cf[nchan], fdelay[nchan]
   DevBuf table;    // [nchan][N1][N2]
   DevBuf work;     // [nchan][npart][npol][N], unless work_ext is set
   DevBuf* work_ext = nullptr;  // shared work buffer (plans of a batch)
@@ -96,11 +96,14 @@ struct ConvCore {
       double chanwidth = d.bandwidth / double(nchan);
       binwidth = chanwidth / double(N);
       half_cw = 0.5 * chanwidth;
-      std::vector<double> pc(2 * nchan);
+      std::vector<double> pc(3 * nchan, 0.0);
       for (uint32_t ic = 0; ic < nchan; ic++) {
         double cf = channel_centre(d.centre_frequency, d.bandwidth, nchan, d.dc_centred, ic);
         pc[ic] = chirp_coeff(d.bandwidth, d.dispersion_measure, cf);
         pc[nchan + ic] = cf;
+        if (d.interchan)
+          pc[2 * nchan + ic] =
+              interchan_fractional_delay(d.centre_frequency, d.bandwidth, nchan, d.dispersion_measure, cf);
       }
       chan_par.reserve(pc.size() * sizeof(double));
       DSPX_HIP(hipMemcpy(chan_par.p, pc.data(), pc.size() * sizeof(double), hipMemcpyHostToDevice));
@@ -157,6 +160,7 @@ struct ConvCore {
     c.half_cw = half_cw;
     c.coeff = chan_par.as<double>();
     c.cf = chan_par.as<double>() ? chan_par.as<double>() + nchan : nullptr;
+    c.fdelay = chan_par.as<double>() ? chan_par.as<double>() + 2 * nchan : nullptr;
     c.table = table.as<float2>();
     return c;
   }

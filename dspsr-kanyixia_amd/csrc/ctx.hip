@@ -214,6 +214,12 @@ void* dspx_ctx_stream(dspx_ctx* c) { return c ? static_cast<void*>(c->stream) : 
 
 int dspx_dedisp_geometry(const dspx_dedisp_desc* d, dspx_geometry* g) { return geometry(d, g); }
 
+int dspx_sample_delays(const dspx_dedisp_desc* d, int64_t* delay, double* frac, int64_t* applied,
+                       int64_t* zero_delay, uint64_t* total_delay)
+{
+  return sample_delays(d, delay, frac, applied, zero_delay, total_delay);
+}
+
 int dspx_unpack(dspx_ctx* ctx, const void* d_raw, uint32_t nbit, uint32_t offset_binary, uint32_t nchan,
                 uint32_t npol, uint64_t ndat, float scale, float* d_out, uint64_t cs, uint64_t ps)
 {

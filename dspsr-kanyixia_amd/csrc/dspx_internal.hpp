@@ -163,6 +163,10 @@ void detect(dspx_ctx* ctx, uint32_t state, uint32_t ndim, const float* in, uint6
 int geometry(const dspx_dedisp_desc* d, dspx_geometry* g);
 double channel_centre(double cfreq, double bw, unsigned nchan, int dc_centred, unsigned ichan);
 double chirp_coeff(double bw, double dm, double chan_freq);
+void sample_delay(double cfreq, double bw, unsigned nchan, double dm, double freq, int64_t& idelay, double& frac);
+double interchan_fractional_delay(double cfreq, double bw, unsigned nchan, double dm, double chan_freq);
+int sample_delays(const dspx_dedisp_desc* d, int64_t* delay, double* frac, int64_t* applied, int64_t* zero_delay,
+                  uint64_t* total_delay);
 
 // Bin-plan segments (binplan.cpp): x_j = (M + (j - j0) * D) * 2^(e - 52)
 struct FoldSegment {

@@ -12,7 +12,9 @@
 // Every expression keeps the reference's operation order and is compiled
 // without floating-point contraction, so the doubles it produces are the
 // ones the reference computes.
+#include <algorithm>
 #include <cmath>
+#include <vector>
 
 #include "dspx_internal.hpp"
 
@@ -119,6 +121,71 @@ double chirp_coeff(double bw, double dm, double chan_freq)
   double sign = bw / std::fabs(bw);
   double dispersion_per_MHz = 1e6 * dm / kDmDispersion;
   return -sign * 2 * M_PI * dispersion_per_MHz / (chan_freq * chan_freq);
+}
+
+// Dedispersion::SampleDelay::get_sample_delay (DedispersionSampleDelay.C:
+// 82-97) with the channel sample rate |bw|/nchan MHz that Dedispersion::build
+// configures it with (Dedispersion.C:86-97): the dispersion delay of
+// frequency freq relative to the band centre, in samples, split into the
+// nearest integer and the remainder.
+void sample_delay(double cfreq, double bw, unsigned nchan, double dm, double freq, int64_t& idelay, double& frac)
+{
+  double dispersion = dm / kDmDispersion;
+  double chanwidth = std::fabs(bw) / double(nchan);
+  double rate = chanwidth * 1e6;
+  double time_delay = dispersion * (1.0 / (cfreq * cfreq) - 1.0 / (freq * freq));
+  double sd = time_delay * rate;
+  idelay = int64_t(std::floor(sd + 0.5));
+  frac = sd - double(idelay);
+}
+
+// Dedispersion::build_setup (Dedispersion.C:142-164): the fractional sample
+// delay of channel frequency chan_freq as a time in microseconds (Doppler 1).
+double interchan_fractional_delay(double cfreq, double bw, unsigned nchan, double dm, double chan_freq)
+{
+  int64_t idelay;
+  double frac;
+  sample_delay(cfreq, bw, nchan, dm, chan_freq, idelay, frac);
+  double chanwidth = bw / double(nchan);
+  return frac / chanwidth;
+}
+
+// SampleDelay::build with one span (SampleDelay.C:56-160) and the applied
+// offsets of SampleDelayEngine::set_delays (SampleDelayCUDA.cu:92-107).
+int sample_delays(const dspx_dedisp_desc* d, int64_t* delay, double* frac, int64_t* applied, int64_t* zero_delay,
+                  uint64_t* total_delay)
+{
+  if (!d || d->nchan == 0 || !(d->bandwidth != 0.0) || !(d->centre_frequency > 0.0)) return DSPX_E_PARAM;
+  const unsigned nchan = d->nchan;
+  std::vector<int64_t> dl(nchan);
+  std::vector<double> fr(nchan), freqs(nchan);
+  for (unsigned c = 0; c < nchan; c++) {
+    freqs[c] = channel_centre(d->centre_frequency, d->bandwidth, nchan, int(d->dc_centred), c);
+    sample_delay(d->centre_frequency, d->bandwidth, nchan, d->dispersion_measure, freqs[c], dl[c], fr[c]);
+  }
+  // get_delay_range(0, nchan - 1) (DedispersionSampleDelay.C:122-135)
+  const double dispersion = d->dispersion_measure / kDmDispersion;
+  const double rate = (std::fabs(d->bandwidth) / double(nchan)) * 1e6;
+  const double fmid = (freqs[0] + freqs[nchan - 1]) / 2;
+  const double dmid = dispersion * (1.0 / (d->centre_frequency * d->centre_frequency) - 1.0 / (fmid * fmid));
+  const int64_t centre = int64_t(std::floor(dmid * rate + 0.5));
+  int64_t zd = 0;
+  for (unsigned c = 0; c < nchan; c++) zd = std::max(zd, dl[c] - centre);
+  uint64_t total = 0;
+  for (unsigned c = 0; c < nchan; c++) {
+    const int64_t zc = centre + zd;  // zero_delays[c]
+    if (zc - dl[c] > int64_t(total)) total = uint64_t(zc - dl[c]);
+    const int64_t a = zc ? zc - dl[c] : dl[c];
+    if (a < 0) return DSPX_E_STATE;  // SampleDelayCUDA.cu:100-104
+    // every read stays inside the input (ndat_out = ndat - total)
+    if (uint64_t(a) > total) total = uint64_t(a);
+    if (applied) applied[c] = a;
+    if (delay) delay[c] = dl[c];
+    if (frac) frac[c] = fr[c];
+  }
+  if (zero_delay) *zero_delay = zd;
+  if (total_delay) *total_delay = total;
+  return DSPX_OK;
 }
 
 }  // namespace dspx

@@ -79,11 +79,14 @@ void PlainConv::init(dspx_ctx* c, uint32_t nchan_, uint32_t npol_, uint32_t N_, 
   if (mode != DSPX_CHIRP_TABLE) {
     DSPX_REQUIRE(d.nchan == nchan, DSPX_E_PARAM, "dedisp.nchan=%u != nchan=%u", d.nchan, nchan);
     double chanwidth = d.bandwidth / double(nchan);
-    std::vector<double> pc(2 * nchan);
+    std::vector<double> pc(3 * nchan, 0.0);
     for (uint32_t ic = 0; ic < nchan; ic++) {
       double cf = channel_centre(d.centre_frequency, d.bandwidth, nchan, d.dc_centred, ic);
       pc[ic] = chirp_coeff(d.bandwidth, d.dispersion_measure, cf);
       pc[nchan + ic] = cf;
+      if (d.interchan)
+        pc[2 * nchan + ic] =
+            interchan_fractional_delay(d.centre_frequency, d.bandwidth, nchan, d.dispersion_measure, cf);
     }
     DevBuf par;
     par.reserve(pc.size() * sizeof(double));
@@ -95,6 +98,7 @@ void PlainConv::init(dspx_ctx* c, uint32_t nchan_, uint32_t npol_, uint32_t N_, 
     ca.half_cw = 0.5 * chanwidth;
     ca.coeff = par.as<double>();
     ca.cf = par.as<double>() + nchan;
+    ca.fdelay = par.as<double>() + 2 * nchan;
     ca.table = nullptr;
     dim3 grid((N + 255) / 256, nchan);
     k_build_resp<<<grid, 256, 0, ctx->stream>>>(ca, N, resp.as<float2>());

@@ -128,6 +128,7 @@ struct ChirpArgs {
   double half_cw;      // 0.5 * chanwidth
   const double* coeff; // per channel
   const double* cf;    // per channel centre frequency (MHz)
+  const double* fdelay; // per channel fractional inter-channel delay (us), 0 without -K
   const float2* table; // DSPX_CHIRP_TABLE: [nchan][N1][N2] (transposed bins)
 };
 
@@ -219,7 +220,9 @@ __device__ __forceinline__ float2 chirp_onchip(const ChirpArgs& c, uint32_t chan
   uint32_t ipt = (k + (N >> 1)) & (N - 1);
   double freq = double(ipt) * c.binwidth - c.half_cw;
   double cf = c.cf[chan];
-  double phase = c.coeff[chan] * (freq * freq) / (cf + freq);
+  // coeff*sqr(freq)/(chan_freq+freq) + delay_phase, delay_phase = -2 pi freq delay
+  double delay_phase = -6.283185307179586 * freq * c.fdelay[chan];
+  double phase = c.coeff[chan] * (freq * freq) / (cf + freq) + delay_phase;
   double th = c.mode == DSPX_CHIRP_PARITY ? double(float(phase)) : phase;
   double n = rint(th * 0.15915494309189535);          // 1 / (2 pi)
   double r = fma(-n, 6.283185307179586, th);           // 2 pi, high part

@@ -77,7 +77,8 @@ void ConvolutionEngine::create(const Response* response)
     // the chirp is evaluated on the device from the band and DM
     d.chirp_mode = DSPX_CHIRP_PARITY;
     d.dedisp = dspx_dedisp_desc{dd->get_centre_frequency(), dd->get_bandwidth(), dd->get_dispersion_measure(),
-                                dd->get_nchan(), dd->get_dc_centred() ? 1u : 0u, d.nfft, 0};
+                                dd->get_nchan(), dd->get_dc_centred() ? 1u : 0u, d.nfft,
+                                dd->get_interchan() ? 1u : 0u};
   } else {
     d.chirp_mode = DSPX_CHIRP_TABLE;
   }
@@ -205,6 +206,32 @@ void SpectralKurtosisEngine::mask(const uint8_t* m, const TimeSeries* in, TimeSe
   ctx->check(dspx_sk_mask(sk, m, in->get_datptr(), in->chan_span(), in->get_pol_span(), out->get_datptr(),
                           out->chan_span(), out->get_pol_span(), npart),
              "dsp::HIP::SpectralKurtosisEngine::mask");
+}
+
+// ------------------------------------------------------ SampleDelayEngine
+
+SampleDelayEngine::~SampleDelayEngine() { dspx_delay_destroy(delay); }
+
+void SampleDelayEngine::set_delays(unsigned npol, unsigned nchan, const std::vector<int64_t>& applied)
+{
+  dspx_delay_destroy(delay);
+  delay = nullptr;
+  if (applied.size() != size_t(nchan) * npol)
+    throw Error(InvalidParam, "dsp::HIP::SampleDelayEngine::set_delays", "%zu delays for %u x %u", applied.size(),
+                nchan, npol);
+  ctx->check(dspx_delay_create(ctx->get(), nchan, npol, applied.data(), &delay),
+             "dsp::HIP::SampleDelayEngine::set_delays");
+}
+
+void SampleDelayEngine::retard(const TimeSeries* in, TimeSeries* out, uint64_t output_ndat)
+{
+  const char* where = "dsp::HIP::SampleDelayEngine::retard";
+  if (!delay) throw Error(InvalidState, where, "no delays");
+  require_device(in->get_datptr(), in->get_memory()->on_device, where);
+  require_device(out->get_datptr(), out->get_memory()->on_device, where);
+  ctx->check(dspx_delay_retard(delay, in->get_datptr(), in->chan_span(), in->get_pol_span(), in->get_ndim(),
+                               output_ndat, out->get_datptr(), out->chan_span(), out->get_pol_span()),
+             where);
 }
 
 // -------------------------------------------------------- DetectionEngine

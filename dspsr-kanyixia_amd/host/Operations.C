@@ -244,6 +244,46 @@ void SpectralKurtosis::operate()
   engine->mask(zapmask, input, output, npart);
 }
 
+// ------------------------------------------------------------ SampleDelay
+
+void SampleDelay::build()
+{
+  const unsigned nchan = input->get_nchan();
+  dspx_dedisp_desc d{input->get_centre_frequency(), input->get_bandwidth(), input->get_dispersion_measure(),
+                     nchan, input->get_dc_centred() ? 1u : 0u, 0, 1};
+  std::vector<int64_t> a(nchan);
+  int rc = dspx_sample_delays(&d, nullptr, nullptr, a.data(), &zero_delay, &total_delay);
+  if (rc == DSPX_E_STATE)
+    throw Error(InvalidState, "dsp::SampleDelay::build", "a channel's delay is negative");
+  if (rc != DSPX_OK) throw Error(InvalidParam, "dsp::Dedispersion::SampleDelay::match", "invalid input");
+  const unsigned npol = input->get_npol();
+  applied.assign(size_t(nchan) * npol, 0);
+  for (unsigned c = 0; c < nchan; c++)
+    for (unsigned p = 0; p < npol; p++) applied[size_t(c) * npol + p] = a[c];
+  engine->set_delays(npol, nchan, applied);
+  built_cfreq = input->get_centre_frequency();
+  built_bw = input->get_bandwidth();
+  built_dm = input->get_dispersion_measure();
+  built_rate = input->get_rate();
+  built_nchan = nchan;
+  built_npol = npol;
+}
+
+void SampleDelay::operate()
+{
+  if (!engine) throw Error(InvalidState, "dsp::SampleDelay::transformation", "no engine");
+  if (built_dm != input->get_dispersion_measure() || built_cfreq != input->get_centre_frequency() ||
+      built_bw != input->get_bandwidth() || built_rate != input->get_rate() || built_nchan != input->get_nchan() ||
+      built_npol != input->get_npol())
+    build();  // Dedispersion::SampleDelay::match (DedispersionSampleDelay.C:25-80)
+  const uint64_t ndat = input->get_ndat();
+  const uint64_t output_ndat = ndat < total_delay ? 0 : ndat - total_delay;  // SampleDelay.C:168-182
+  output->copy_configuration(input);
+  output->resize(output_ndat);
+  output->change_start_time(zero_delay);  // SampleDelay.C:221
+  if (output_ndat) engine->retard(input, output, output_ndat);
+}
+
 // ------------------------------------------------------------ Convolution
 
 void Convolution::prepare()

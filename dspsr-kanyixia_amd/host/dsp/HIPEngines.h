@@ -16,6 +16,8 @@
 //     (FilterbankConfig.C:115-133)
 //   skestimator->set_engine (new dsp::HIP::SpectralKurtosisEngine (ctx));
 //     (LoadToFold1.C:341-349)
+//   sample_delay->set_engine (new dsp::HIP::SampleDelayEngine (ctx));
+//     (DedispersionPipe.C:435-472)
 #pragma once
 #include "dsp/Operations.h"
 #include "dspx.h"
@@ -97,6 +99,19 @@ class SpectralKurtosisEngine : public SpectralKurtosis::Engine {
   Context* ctx;
   dspx_sk* sk = nullptr;
   unsigned nchan = 0;
+};
+
+//! SampleDelay::Engine over dspx_delay_*
+class SampleDelayEngine : public SampleDelay::Engine {
+ public:
+  explicit SampleDelayEngine(Context* c) : ctx(c) {}
+  ~SampleDelayEngine() override;
+  void set_delays(unsigned npol, unsigned nchan, const std::vector<int64_t>& applied) override;
+  void retard(const TimeSeries* in, TimeSeries* out, uint64_t output_ndat) override;
+
+ private:
+  Context* ctx;
+  dspx_delay* delay = nullptr;
 };
 
 class DetectionEngine : public Detection::Engine {

@@ -58,10 +58,19 @@ class Dedispersion : public Response {
   double get_centre_frequency() const { return centre_frequency; }
   double get_bandwidth() const { return bandwidth; }
   bool get_dc_centred() const { return dc_centred; }
+  //! Dedispersion::set_sample_delay (Dedispersion.C:61-67): with inter-channel
+  //! delays removed by a SampleDelay (-K), each channel's fractional delay
+  //! goes into the chirp (Dedispersion.C:142-178)
+  void set_interchan(bool on)
+  {
+    if (on != interchan) changed = true;
+    interchan = on;
+  }
+  bool get_interchan() const { return interchan; }
 
  protected:
   double dispersion_measure = 0, centre_frequency = 0, bandwidth = 0;
-  bool dc_centred = false;
+  bool dc_centred = false, interchan = false;
   unsigned user_ndat = 0;
 };
 
@@ -241,6 +250,41 @@ class SpectralKurtosis {
   float* sums = nullptr;
   uint32_t* counts = nullptr;
   uint8_t* zapmask = nullptr;
+};
+
+//! SampleDelay with the dispersion delay function (Signal/General/
+//! SampleDelay.C:56-300, DedispersionSampleDelay.C:25-135; one span, every
+//! pol alike): removes the integer inter-channel dispersion delays.
+class SampleDelay {
+ public:
+  //! SampleDelay::Engine (SampleDelayCUDA.cu:58-160), given the applied
+  //! offsets that set_delays derives from zero_delays and the function
+  class Engine {
+   public:
+    virtual ~Engine() = default;
+    virtual void set_delays(unsigned npol, unsigned nchan, const std::vector<int64_t>& applied) = 0;
+    virtual void retard(const TimeSeries* in, TimeSeries* out, uint64_t output_ndat) = 0;
+  };
+  void set_input(const TimeSeries* in) { input = in; }
+  void set_output(TimeSeries* out) { output = out; }
+  void set_engine(Engine* e) { engine.reset(e); }
+  //! SampleDelay::transformation (SampleDelay.C:150-300)
+  void operate();
+  int64_t get_zero_delay() const { return zero_delay; }
+  uint64_t get_total_delay() const { return total_delay; }
+  const std::vector<int64_t>& get_applied() const { return applied; }
+
+ protected:
+  void build();
+  const TimeSeries* input = nullptr;
+  TimeSeries* output = nullptr;
+  std::unique_ptr<Engine> engine;
+  std::vector<int64_t> applied;
+  int64_t zero_delay = 0;
+  uint64_t total_delay = 0;
+  // the observation the delays were built for (Dedispersion::SampleDelay::match)
+  double built_cfreq = 0, built_bw = 0, built_dm = -1, built_rate = 0;
+  unsigned built_nchan = 0, built_npol = 0;
 };
 
 class Detection {

@@ -3,6 +3,7 @@
 // Reads one block of raw TFP samples from a file, runs
 //   filterbank : Unpacker -> Filterbank[HIP::FilterbankEngine] (writes the channels)
 //   chain : Unpacker -> Convolution[HIP::ConvolutionEngine] ->
+//           (--interchan 1: SampleDelay[HIP::SampleDelayEngine] ->)
 //           Detection[HIP::DetectionEngine] -> Fold[HIP::FoldEngine] per sub-integration
 //   fused : HIP::LoadToFoldEngine (three fused kernels)
 //   plain : HIP::LoadToFoldEngine, plain FP32 form
@@ -126,8 +127,13 @@ try {
     HIP::Unpacker unpacker(&ctx);
     unpacker.unpack(&raw, &unpacked);
 
+    // --interchan 1: inter-channel delays removed as dspsr -K does
+    // (DedispersionPipe.C:435-472): fractional part in the chirp, integer
+    // part by a SampleDelay after the convolution
+    const bool interchan = a.count("interchan") && a["interchan"] != "0";
     Dedispersion kernel;
     kernel.set_dispersion_measure(dm);
+    kernel.set_interchan(interchan);
     Convolution convolution;
     convolution.set_input(&unpacked);
     convolution.set_output(&filtered);
@@ -135,8 +141,19 @@ try {
     convolution.set_engine(new HIP::ConvolutionEngine(&ctx));
     convolution.operate();
 
+    TimeSeries delayed(Memory::device());
+    TimeSeries* to_detect = &filtered;
+    if (interchan) {
+      SampleDelay sample_delay;
+      sample_delay.set_input(&filtered);
+      sample_delay.set_output(&delayed);
+      sample_delay.set_engine(new HIP::SampleDelayEngine(&ctx));
+      sample_delay.operate();
+      to_detect = &delayed;
+    }
+
     Detection detect;
-    detect.set_input(&filtered);
+    detect.set_input(to_detect);
     detect.set_output(&detected);
     detect.set_output_state(state);
     detect.set_output_ndim(ndim);

@@ -68,7 +68,10 @@ typedef struct {
   uint32_t nchan;
   uint32_t dc_centred;       /* Observation::get_dc_centred */
   uint32_t nfft;             /* 0: reference rule; else user-set resolution */
-  uint32_t reserved;
+  uint32_t interchan;        /* 1: inter-channel delays removed (dspsr -K): the chirp
+                                carries each channel's fractional sample delay
+                                (Dedispersion.C:142-178) and dspx_delay_* applies the
+                                integer delays */
 } dspx_dedisp_desc;
 
 typedef struct {
@@ -82,6 +85,34 @@ typedef struct {
 
 /* Host-only.  Complex (Analytic) input. */
 int dspx_dedisp_geometry(const dspx_dedisp_desc* desc, dspx_geometry* out);
+
+/* Host-only.  The inter-channel dispersion delays of SampleDelay with the
+   Dedispersion::SampleDelay function (SampleDelay.C:56-160,
+   DedispersionSampleDelay.C:25-129; one span, every pol alike) at the
+   channel sample rate |bw|/nchan MHz:
+     delay[c]   = floor(s_c + 0.5), frac[c] = s_c - delay[c], with
+     s_c = (DM / 2.41e-4) * (1/f0^2 - 1/f_c^2) * rate   (f0 band centre,
+     f_c channel centre),
+     applied[c] = zero_delay + centre - delay[c] (input sample i + applied[c]
+     becomes output sample i), total = max_c applied[c];
+   zero_delay shifts the output start time (SampleDelay.C:221).  Any output
+   pointer may be null. */
+int dspx_sample_delays(const dspx_dedisp_desc* desc, int64_t* delay, double* frac, int64_t* applied,
+                       int64_t* zero_delay, uint64_t* total_delay);
+
+/* ------------------------------------------------------------- SampleDelay */
+
+/* SampleDelay::Engine (SampleDelayCUDA.cu:24-120): set_delays once, then
+   retard per block. */
+typedef struct dspx_delay dspx_delay;
+/* applied: host [nchan * npol] sample offsets (ichan * npol + ipol), >= 0. */
+int dspx_delay_create(dspx_ctx* ctx, uint32_t nchan, uint32_t npol, const int64_t* applied, dspx_delay** out);
+void dspx_delay_destroy(dspx_delay* delay);
+/* out[c][p][i] = in[c][p][i + applied[c*npol + p]] for i < ndat_out, ndim
+   floats per sample (FPT; strides in floats).  in != out. */
+int dspx_delay_retard(dspx_delay* delay, const float* d_in, uint64_t in_chan_stride, uint64_t in_pol_stride,
+                      uint32_t ndim, uint64_t ndat_out, float* d_out, uint64_t out_chan_stride,
+                      uint64_t out_pol_stride);
 
 /* ------------------------------------------------------------------ unpack */
 

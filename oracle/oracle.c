@@ -302,15 +302,88 @@ double ox_channel_centre(double cfreq, double bw, unsigned nchan, int dc_centred
   return base + (double) ichan * bw / (double) nchan;
 }
 
-/* Dedispersion.C:166-186 (build_compute, no fractional inter-channel delay).
-   freq is the offset (MHz) from the channel centre chan_freq. */
-double ox_chirp_phase(double bw, double dm, double chan_freq, double freq)
+/* Dedispersion.C:166-186 (build_compute).  freq is the offset (MHz) from
+   the channel centre chan_freq; delay_us the channel's fractional
+   inter-channel delay (Dedispersion.C:142-164; 0 without -K). */
+double ox_chirp_phase_delay(double bw, double dm, double chan_freq, double freq, double delay_us)
 {
   double sign = bw / fabs(bw);
   double dispersion_per_MHz = 1e6 * dm / ox_dm_dispersion;
   double coeff = -sign * 2 * M_PI * dispersion_per_MHz / (chan_freq * chan_freq);
-  double delay_phase = -2.0 * M_PI * freq * 0.0;
+  double delay_phase = -2.0 * M_PI * freq * delay_us;
   return coeff * (freq * freq) / (chan_freq + freq) + delay_phase;
+}
+
+double ox_chirp_phase(double bw, double dm, double chan_freq, double freq)
+{
+  return ox_chirp_phase_delay(bw, dm, chan_freq, freq, 0.0);
+}
+
+/* Dedispersion::SampleDelay::get_sample_delay (DedispersionSampleDelay.C:
+   82-97), configured by Dedispersion::build with the band's centre frequency,
+   DM and the channel rate |bw|/nchan MHz (Dedispersion.C:86-97): the delay of
+   frequency freq relative to the band centre in samples, as the nearest
+   integer and the remainder. */
+void ox_sample_delay(double cfreq, double bw, unsigned nchan, double dm, double freq, int64_t* idelay,
+                     double* frac)
+{
+  double dispersion = dm / ox_dm_dispersion;
+  double chanwidth = fabs(bw) / (double) nchan;
+  double rate = chanwidth * 1e6;
+  double time_delay = dispersion * (1.0 / (cfreq * cfreq) - 1.0 / (freq * freq));
+  double sd = time_delay * rate;
+  *idelay = (int64_t) floor(sd + 0.5);
+  *frac = sd - (double) *idelay;
+}
+
+/* Dedispersion::build_setup (Dedispersion.C:142-164): the fractional delay
+   as a time in microseconds (Doppler 1). */
+double ox_interchan_delay(double cfreq, double bw, unsigned nchan, double dm, double chan_freq)
+{
+  int64_t idelay;
+  double frac, chanwidth = bw / (double) nchan;
+  ox_sample_delay(cfreq, bw, nchan, dm, chan_freq, &idelay, &frac);
+  return frac / chanwidth;
+}
+
+/* SampleDelay::build with one span (SampleDelay.C:56-160) and the offsets
+   SampleDelayEngine::set_delays applies (SampleDelayCUDA.cu:92-107):
+   applied[c] = zero_delays[c] - delay[c] (delay[c] where zero_delays[c] is
+   0); total = the largest applied offset, so that output sample i of every
+   channel reads input sample i + applied[c] < ndat.  Returns -1 for a
+   negative offset (the reference throws). */
+int ox_sample_delays(double cfreq, double bw, unsigned nchan, int dc_centred, double dm, int64_t* delay,
+                     double* frac, int64_t* applied, int64_t* zero_delay, uint64_t* total)
+{
+  unsigned c;
+  double dispersion = dm / ox_dm_dispersion;
+  double rate = (fabs(bw) / (double) nchan) * 1e6;
+  double f0 = ox_channel_centre(cfreq, bw, nchan, dc_centred, 0);
+  double f1 = ox_channel_centre(cfreq, bw, nchan, dc_centred, nchan - 1);
+  double fmid = (f0 + f1) / 2;
+  double dmid = dispersion * (1.0 / (cfreq * cfreq) - 1.0 / (fmid * fmid));
+  int64_t centre = (int64_t) floor(dmid * rate + 0.5), zd = 0;
+  uint64_t tot = 0;
+  for (c = 0; c < nchan; c++)
+  {
+    ox_sample_delay(cfreq, bw, nchan, dm, ox_channel_centre(cfreq, bw, nchan, dc_centred, c), &delay[c], &frac[c]);
+    if (delay[c] - centre > zd)
+      zd = delay[c] - centre;
+  }
+  for (c = 0; c < nchan; c++)
+  {
+    int64_t zc = centre + zd;
+    if (zc - delay[c] > (int64_t) tot)
+      tot = (uint64_t) (zc - delay[c]);
+    applied[c] = zc ? zc - delay[c] : delay[c];
+    if (applied[c] < 0)
+      return -1;
+    if ((uint64_t) applied[c] > tot)
+      tot = (uint64_t) applied[c];
+  }
+  *zero_delay = zd;
+  *total = tot;
+  return 0;
 }
 
 /* Offset frequency of point ipt of an npt-point channel response, lowest
@@ -332,7 +405,7 @@ double ox_chirp_freq(double chanwidth, unsigned npt, unsigned ipt)
    phase, evaluated in double and rounded to float at the end).
    resp: [nchan][nfft][2] float. */
 void ox_dedisp_response_f32(double cfreq, double bw, unsigned nchan, int dc_centred,
-                            double dm, unsigned nfft, int mode, float* resp)
+                            double dm, unsigned nfft, int mode, int interchan, float* resp)
 {
   unsigned ichan, k;
   double chanwidth = bw / (double) nchan;
@@ -340,10 +413,11 @@ void ox_dedisp_response_f32(double cfreq, double bw, unsigned nchan, int dc_cent
   for (ichan = 0; ichan < nchan; ichan++)
   {
     double cf = ox_channel_centre(cfreq, bw, nchan, dc_centred, ichan);
+    double dly = interchan ? ox_interchan_delay(cfreq, bw, nchan, dm, cf) : 0.0;
     for (k = 0; k < nfft; k++)
     {
       unsigned ipt = (k + nfft / 2) % nfft;
-      double ph = ox_chirp_phase(bw, dm, cf, ox_chirp_freq(chanwidth, nfft, ipt));
+      double ph = ox_chirp_phase_delay(bw, dm, cf, ox_chirp_freq(chanwidth, nfft, ipt), dly);
       float c, sn;
       if (mode == 0)
       {
@@ -365,7 +439,7 @@ void ox_dedisp_response_f32(double cfreq, double bw, unsigned nchan, int dc_cent
 
 /* FP64 truth response: exp(i * phase) / sqrt(nfft), all in double. */
 void ox_dedisp_response_f64(double cfreq, double bw, unsigned nchan, int dc_centred,
-                            double dm, unsigned nfft, double* resp)
+                            double dm, unsigned nfft, int interchan, double* resp)
 {
   unsigned ichan, k;
   double chanwidth = bw / (double) nchan;
@@ -373,10 +447,11 @@ void ox_dedisp_response_f64(double cfreq, double bw, unsigned nchan, int dc_cent
   for (ichan = 0; ichan < nchan; ichan++)
   {
     double cf = ox_channel_centre(cfreq, bw, nchan, dc_centred, ichan);
+    double dly = interchan ? ox_interchan_delay(cfreq, bw, nchan, dm, cf) : 0.0;
     for (k = 0; k < nfft; k++)
     {
       unsigned ipt = (k + nfft / 2) % nfft;
-      double ph = ox_chirp_phase(bw, dm, cf, ox_chirp_freq(chanwidth, nfft, ipt));
+      double ph = ox_chirp_phase_delay(bw, dm, cf, ox_chirp_freq(chanwidth, nfft, ipt), dly);
       resp[2 * ((uint64_t) ichan * nfft + k)] = s * cos(ph);
       resp[2 * ((uint64_t) ichan * nfft + k) + 1] = s * sin(ph);
     }

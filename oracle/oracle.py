@@ -16,6 +16,7 @@ _ref = None
 
 f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
 f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+i64p = np.ctypeslib.ndpointer(np.int64, flags="C_CONTIGUOUS")
 u8p = np.ctypeslib.ndpointer(np.uint8, flags="C_CONTIGUOUS")
 i16p = np.ctypeslib.ndpointer(np.int16, flags="C_CONTIGUOUS")
 u16p = np.ctypeslib.ndpointer(np.uint16, flags="C_CONTIGUOUS")
@@ -62,10 +63,16 @@ def lib():
     L.ox_chirp_phase.argtypes = [d, d, d, d]
     L.ox_chirp_freq.restype = d
     L.ox_chirp_freq.argtypes = [d, u, u]
+    L.ox_chirp_phase_delay.restype = d
+    L.ox_chirp_phase_delay.argtypes = [d, d, d, d, d]
     L.ox_dedisp_response_f32.restype = None
-    L.ox_dedisp_response_f32.argtypes = [d, d, u, i, d, u, i, f32p]
+    L.ox_dedisp_response_f32.argtypes = [d, d, u, i, d, u, i, i, f32p]
     L.ox_dedisp_response_f64.restype = None
-    L.ox_dedisp_response_f64.argtypes = [d, d, u, i, d, u, f64p]
+    L.ox_dedisp_response_f64.argtypes = [d, d, u, i, d, u, i, f64p]
+    L.ox_interchan_delay.restype = d
+    L.ox_interchan_delay.argtypes = [d, d, u, d, d]
+    L.ox_sample_delays.restype = i
+    L.ox_sample_delays.argtypes = [d, d, u, i, d, i64p, f64p, i64p, C.POINTER(C.c_int64), C.POINTER(U)]
     for name, p in (("ox_convolve_f32", f32p), ("ox_convolve_f64", f64p)):
         fn = getattr(L, name)
         fn.restype = U
@@ -153,17 +160,41 @@ def fft(x, sign=-1):
     return buf.view(np.complex128)
 
 
-def response(cfreq, bw, nchan, dm, nfft, mode=0, dc_centred=False):
+def response(cfreq, bw, nchan, dm, nfft, mode=0, dc_centred=False, interchan=False):
     """[nchan][nfft] complex response in FFT-bin order (normaliser included).
     mode 0: float phase (reference parity); 1: FP64 phase rounded to float;
-    'f64': FP64 truth in double."""
+    'f64': FP64 truth in double.  interchan: each channel's fractional
+    inter-channel delay in the phase (dspsr -K)."""
     if mode == "f64":
         out = np.empty(nchan * nfft * 2, np.float64)
-        lib().ox_dedisp_response_f64(cfreq, bw, nchan, int(dc_centred), dm, nfft, out)
+        lib().ox_dedisp_response_f64(cfreq, bw, nchan, int(dc_centred), dm, nfft, int(interchan), out)
         return out.view(np.complex128).reshape(nchan, nfft)
     out = np.empty(nchan * nfft * 2, np.float32)
-    lib().ox_dedisp_response_f32(cfreq, bw, nchan, int(dc_centred), dm, nfft, mode, out)
+    lib().ox_dedisp_response_f32(cfreq, bw, nchan, int(dc_centred), dm, nfft, mode, int(interchan), out)
     return out.view(np.complex64).reshape(nchan, nfft)
+
+
+def sample_delays(cfreq, bw, nchan, dm, dc_centred=False):
+    """SampleDelay with the dispersion delay function (one span): per channel
+    the integer and fractional delay in samples relative to the band centre,
+    the applied input offset, the zero delay and the total delay."""
+    delay = np.zeros(nchan, np.int64)
+    frac = np.zeros(nchan, np.float64)
+    applied = np.zeros(nchan, np.int64)
+    zd, tot = C.c_int64(0), C.c_uint64(0)
+    rc = lib().ox_sample_delays(cfreq, bw, nchan, int(dc_centred), dm, delay, frac, applied, C.byref(zd), C.byref(tot))
+    if rc != 0:
+        raise ValueError("negative applied delay")
+    return {"delay": delay, "frac": frac, "applied": applied, "zero_delay": zd.value, "total": tot.value}
+
+
+def retard(x, applied, ndat_out):
+    """SampleDelay::transformation (SampleDelay.C:262-297): output sample i of
+    channel c is input sample i + applied[c] (every pol).  x: [nchan][npol][ndat]."""
+    out = np.empty(x.shape[:2] + (ndat_out,) + x.shape[3:], x.dtype)
+    for c in range(x.shape[0]):
+        out[c] = x[c, :, applied[c]:applied[c] + ndat_out]
+    return out
 
 
 def convolve(x, resp, nfilt_pos, nfilt_neg, zero_dm=False):

@@ -88,3 +88,48 @@ def test_host_filterbank(dspx, oracle, tmp_path, with_response):
     e = rel_rms(got, expect)
     print("filterbank rel rms %.3g" % e)
     assert e < 1e-5
+
+
+def test_host_pipeline_interchan(dspx, oracle, tmp_path):
+    """The chain with inter-channel delays removed (dspsr -K):
+    Convolution[fractional-delay chirp] -> SampleDelay -> Detection -> Fold
+    through the C++ host layer, against the oracle's chain."""
+    assert os.path.exists(EXE), "dspx_pipeline not built"
+    case = "multichan_stokes_subints"
+    nbit, nchan, npol, ndat, cfreq, bw, dm, state, ndim, nbin, period, nsub = CASES[case]
+    raw = _make(dspx, case)
+    rawf = tmp_path / "raw.bin"
+    raw.tofile(rawf)
+    outf = tmp_path / "prof.bin"
+    cmd = [EXE, "--raw", str(rawf), "--nbit", str(nbit), "--nchan", str(nchan), "--npol", str(npol),
+           "--ndat", str(ndat), "--cfreq", repr(cfreq), "--bw", repr(bw), "--dm", repr(dm), "--period",
+           repr(period), "--nbin", str(nbin), "--nsub", str(nsub), "--state", STATES[state], "--ndim",
+           str(ndim), "--mode", "chain", "--interchan", "1", "--out", str(outf)]
+    print("host pipeline interchan", flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=300)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0
+    geom = oracle.geometry(cfreq, bw, nchan, dm)
+    x = oracle.unpack8(raw, nchan, npol, ndat).view(np.complex64).reshape(nchan, npol, ndat)
+    resp = oracle.response(cfreq, bw, nchan, dm, geom["nfft"], 0, interchan=True)
+    fb = oracle.convolve(x, resp, geom["impulse_pos"], geom["impulse_neg"])
+    sd = oracle.sample_delays(cfreq, bw, nchan, dm)
+    nret = fb.shape[2] - sd["total"]
+    assert nret > 0
+    det = oracle.detect(oracle.retard(fb, sd["applied"], nret), state, ndim)
+    rate = abs(bw) / nchan * 1e6
+    pps = (1.0 / rate) / period
+    chunks = dspx.fold_chunks(nret, rate, period, geom["impulse_pos"] + sd["zero_delay"], nsub=nsub)
+    nd = det.shape[2] // nret
+    prof_o = np.zeros((nsub, nchan, det.shape[1], nbin * nd), np.float32)
+    hits_o = np.zeros((nsub, 1, nbin), np.uint32)
+    for (i0, n, phi, isub) in chunks:
+        oracle.fold(det, nd, phi, pps, nbin, idat_start=i0, ndat_fold=n, prof=prof_o[isub], hits=hits_o[isub])
+    data = np.fromfile(outf, np.uint8)
+    nprof = prof_o.size
+    prof = data[:nprof * 4].view(np.float32).reshape(prof_o.shape)
+    hits = data[nprof * 4:].view(np.uint32).reshape(nsub, nbin)
+    e = rel_rms(prof, prof_o)
+    print("interchan profile rel rms %.3g" % e)
+    assert e < 1e-5
+    assert np.array_equal(hits, hits_o[:, 0])
