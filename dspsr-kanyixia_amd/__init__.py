@@ -34,6 +34,15 @@ class DedispDesc(C.Structure):
                 ("dc_centred", C.c_uint32), ("nfft", C.c_uint32), ("interchan", C.c_uint32)]
 
 
+class DadaInfo(C.Structure):
+    _fields_ = [("centre_frequency", C.c_double), ("bandwidth", C.c_double), ("tsamp", C.c_double),
+                ("hdr_size", C.c_uint64), ("obs_offset", C.c_uint64), ("ndat", C.c_uint64),
+                ("picoseconds", C.c_int64), ("nchan", C.c_uint32), ("npol", C.c_uint32), ("nbit", C.c_uint32),
+                ("ndim", C.c_uint32), ("resolution", C.c_uint32), ("reserved", C.c_uint32),
+                ("utc_start", C.c_char * 32), ("source", C.c_char * 64), ("telescope", C.c_char * 64),
+                ("instrument", C.c_char * 64)]
+
+
 class Geometry(C.Structure):
     _fields_ = [("impulse_neg", C.c_uint32), ("impulse_pos", C.c_uint32), ("nfft", C.c_uint32),
                 ("nsamp_overlap", C.c_uint32), ("nsamp_step", C.c_uint32),
@@ -84,6 +93,8 @@ _SIGS = {
     "dspx_ctx_stream": (P, [P]),
     "dspx_dedisp_geometry": (I32, [C.POINTER(DedispDesc), C.POINTER(Geometry)]),
     "dspx_sample_delays": (I32, [C.POINTER(DedispDesc), P, P, P, C.POINTER(C.c_int64), C.POINTER(U64)]),
+    "dspx_ascii_header_get": (I32, [C.c_char_p, C.c_char_p, C.c_char_p, U32]),
+    "dspx_dada_parse": (I32, [C.c_char_p, C.POINTER(DadaInfo), C.c_char_p]),
     "dspx_delay_create": (I32, [P, U32, U32, P, C.POINTER(P)]),
     "dspx_delay_destroy": (None, [P]),
     "dspx_delay_retard": (I32, [P, P, U64, U64, U32, U64, P, U64, U64]),
@@ -285,6 +296,61 @@ def sample_delays(centre_frequency, bandwidth, nchan, dm, dc_centred=False):
     if rc != OK:
         raise DspxError(rc, "sample_delays: invalid band or a negative offset")
     return {"delay": delay, "frac": frac, "applied": applied, "zero_delay": zd.value, "total": tot.value}
+
+
+def ascii_header_get(header, key):
+    """The value token of key in a DADA ASCII header, or None."""
+    buf = C.create_string_buffer(4096)
+    h = header.encode() if isinstance(header, str) else bytes(header)
+    rc = lib().dspx_ascii_header_get(h, key.encode(), buf, len(buf))
+    return buf.value.decode() if rc == 1 else None
+
+
+def dada_parse(header):
+    """DADA header fields (ASCIIObservation::load) as a dict; raises
+    DspxError naming a missing required key."""
+    h = header.encode() if isinstance(header, str) else bytes(header)
+    info = DadaInfo()
+    missing = C.create_string_buffer(32)
+    rc = lib().dspx_dada_parse(h, C.byref(info), missing)
+    if rc != OK:
+        raise DspxError(rc, "DADA header: %s" % (("missing or invalid " + missing.value.decode())
+                                                 if missing.value else "invalid NDIM"))
+    out = {k: getattr(info, k) for k, _ in DadaInfo._fields_ if k != "reserved"}
+    for k in ("utc_start", "source", "telescope", "instrument"):
+        out[k] = out[k].decode()
+    return out
+
+
+def read_dada(path, default_header_size=4096):
+    """DADAFile: the header (DADAHeader::load_header: HDR_SIZE bytes, re-read
+    when HDR_SIZE exceeds the default) and the raw samples after it as a
+    read-only memory map [ndat][nchan][npol][ndim * nbit / 8] of bytes
+    (TFP), ndat limited by NDAT when the header gives it."""
+    import numpy as np
+    size = default_header_size
+    while True:
+        with open(path, "rb") as f:
+            head = f.read(size)
+        if len(head) < size:
+            raise DspxError(E_PARAM, "%s: shorter than its %d-byte header" % (path, size))
+        text = head.split(b"\0", 1)[0]
+        hs = ascii_header_get(text, "HDR_SIZE")
+        hdr = int(hs) if hs else 0
+        if hdr <= size:
+            break
+        size = hdr
+    if hdr == 0:
+        raise DspxError(E_PARAM, "%s: HDR_SIZE undefined" % path)
+    info = dada_parse(text)
+    bps = info["nchan"] * info["npol"] * info["ndim"] * info["nbit"] // 8
+    nbytes = os.path.getsize(path) - hdr
+    ndat = nbytes // bps
+    if info["ndat"]:
+        ndat = min(ndat, info["ndat"])
+    raw = np.memmap(path, dtype=np.uint8, mode="r", offset=hdr,
+                    shape=(ndat, info["nchan"], info["npol"], info["ndim"] * info["nbit"] // 8))
+    return info, raw
 
 
 class SampleDelay:

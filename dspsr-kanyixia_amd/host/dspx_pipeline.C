@@ -1,6 +1,7 @@
 // dspx_pipeline -- a LoadToFold-style driver over the dsp::HIP engines.
 //
-// Reads one block of raw TFP samples from a file, runs
+// Reads one block of raw TFP samples from a file (--raw with the shape given,
+// or --dada with the shape, band and rate from the DADA header), runs
 //   filterbank : Unpacker -> Filterbank[HIP::FilterbankEngine] (writes the channels)
 //   chain : Unpacker -> Convolution[HIP::ConvolutionEngine] ->
 //           (--interchan 1: SampleDelay[HIP::SampleDelayEngine] ->)
@@ -8,7 +9,8 @@
 //   fused : HIP::LoadToFoldEngine (three fused kernels)
 //   plain : HIP::LoadToFoldEngine, plain FP32 form
 // and writes the folded profiles [nsub][nchan][npol][nbin*ndim] (float) and
-// hits [nsub][nbin] (uint32) to the output file.  The construction mirrors
+// hits [nsub][nbin] (uint32) to the output file (and, with --npz FILE, as a
+// NumPy archive with the fold's parameters).  The construction mirrors
 // LoadToFold::construct (LoadToFold1.C:109-258) with engines attached as
 // DedispersionPipe.C:413-427 and LoadToFold1.C:843-852, 969-978 attach them.
 #include <hip/hip_runtime_api.h>
@@ -20,7 +22,9 @@
 #include <iostream>
 #include <map>
 
+#include "dsp/DADAFile.h"
 #include "dsp/HIPEngines.h"
+#include "dsp/ProfileWriter.h"
 
 using namespace dsp;
 
@@ -50,9 +54,18 @@ try {
     if (!a.count(k)) throw Error(InvalidParam, "dspx_pipeline", "missing --%s", k);
     return a[k];
   };
-  unsigned nbit = std::stoul(need("nbit")), nchan = std::stoul(need("nchan")), npol = std::stoul(need("npol"));
-  uint64_t ndat = std::stoull(need("ndat"));
-  double cfreq = std::stod(need("cfreq")), bw = std::stod(need("bw")), dm = std::stod(need("dm"));
+  // --dada FILE: shape, band and sample rate from the file's header (DADAFile);
+  // otherwise --raw FILE with the shape and band given
+  std::unique_ptr<DADAFile> dada;
+  if (a.count("dada")) dada.reset(new DADAFile(a["dada"]));
+  const dspx_dada_info* hi = dada ? &dada->get_info() : nullptr;
+  unsigned nbit = hi ? hi->nbit : std::stoul(need("nbit"));
+  unsigned nchan = hi ? hi->nchan : std::stoul(need("nchan"));
+  unsigned npol = hi ? hi->npol : std::stoul(need("npol"));
+  uint64_t ndat = a.count("ndat") || !dada ? std::stoull(need("ndat")) : dada->get_ndat();
+  double cfreq = hi ? hi->centre_frequency : std::stod(need("cfreq"));
+  double bw = hi ? hi->bandwidth : std::stod(need("bw"));
+  double dm = std::stod(need("dm"));
   std::string mode = need("mode");
   // folding options (not used by the filterbank mode)
   auto fold_opt = [&](const char* k, const char* dflt) { return mode == "filterbank" && !a.count(k) ? dflt : need(k); };
@@ -63,15 +76,20 @@ try {
 
   HIP::Context ctx(0);
   BitSeries raw(Memory::device());
-  raw.set_nchan(nchan);
-  raw.set_npol(npol);
-  raw.set_centre_frequency(cfreq);
-  raw.set_bandwidth(bw);
-  raw.set_dispersion_measure(dm);
-  raw.set_rate(std::fabs(bw) / nchan * 1e6);
-  raw.set_start_time(0.0);
-  raw.resize(ndat, nbit);
-  {
+  if (dada) {
+    dada->load(&raw, 0, ndat);
+    raw.set_dispersion_measure(dm);
+  } else {
+    raw.set_nchan(nchan);
+    raw.set_npol(npol);
+    raw.set_centre_frequency(cfreq);
+    raw.set_bandwidth(bw);
+    raw.set_dispersion_measure(dm);
+    raw.set_rate(std::fabs(bw) / nchan * 1e6);
+    raw.set_start_time(0.0);
+    raw.resize(ndat, nbit);
+  }
+  if (!dada) {
     size_t bytes = size_t(ndat) * nchan * npol * 2 * nbit / 8;
     std::vector<char> h(bytes);
     std::ifstream f(need("raw"), std::ios::binary);
@@ -198,6 +216,21 @@ try {
     ltf.get_result(prof, hits);
   } else {
     throw Error(InvalidParam, "dspx_pipeline", "unknown mode %s", mode.c_str());
+  }
+  if (a.count("npz")) {
+    // Archiver stand-in: the profiles, hits and fold parameters as named arrays
+    const unsigned ndim_out = (state == Signal::Intensity || state == Signal::PPQQ) ? 1 : ndim;
+    const unsigned npol_out = state == Signal::Intensity ? 1 : state == Signal::PPQQ ? npol : 4 / ndim;
+    ProfileWriter w;
+    w.add("profiles", prof.data(), {nsub, nchan, npol_out, uint64_t(nbin) * ndim_out});
+    w.add("hits", hits.data(), {nsub, nbin});
+    w.add("folding_period", period);
+    w.add("dispersion_measure", dm);
+    w.add("centre_frequency", cfreq);
+    w.add("bandwidth", bw);
+    w.add("nbin", double(nbin));
+    w.add("state", double(state));
+    w.write(a["npz"]);
   }
   std::ofstream o(need("out"), std::ios::binary);
   o.write(reinterpret_cast<const char*>(prof.data()), std::streamsize(prof.size() * sizeof(float)));

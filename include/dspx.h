@@ -100,6 +100,43 @@ int dspx_dedisp_geometry(const dspx_dedisp_desc* desc, dspx_geometry* out);
 int dspx_sample_delays(const dspx_dedisp_desc* desc, int64_t* delay, double* frac, int64_t* applied,
                        int64_t* zero_delay, uint64_t* total_delay);
 
+/* ------------------------------------------------------------- DADA source */
+
+/* Host-only.  ascii_header_get (Kernel/Classes/ascii_header.c): keyword key
+   counts where it starts the header, or follows a newline (or a backslash)
+   and is followed by a space or tab; its value is the first
+   whitespace-delimited token after it (sscanf "%s").  Copies the token,
+   NUL-terminated and truncated to cap bytes, and returns 1; -1 when the key
+   is absent or has no value. */
+int dspx_ascii_header_get(const char* header, const char* key, char* value, uint32_t cap);
+
+typedef struct {
+  double centre_frequency; /* FREQ, MHz */
+  double bandwidth;        /* BW, MHz (negative: inverted band) */
+  double tsamp;            /* TSAMP, microseconds */
+  uint64_t hdr_size;       /* HDR_SIZE, bytes of header before the data; 0 if absent */
+  uint64_t obs_offset;     /* OBS_OFFSET (or OFFSET), bytes from the start of the observation */
+  uint64_t ndat;           /* NDAT; 0 if absent */
+  int64_t picoseconds;     /* PICOSECONDS; 0 if absent */
+  uint32_t nchan;          /* NCHAN; 1 if absent */
+  uint32_t npol;           /* NPOL */
+  uint32_t nbit;           /* NBIT */
+  uint32_t ndim;           /* NDIM; 1 if absent (1 Nyquist, 2 Analytic, 4 Coherence) */
+  uint32_t resolution;     /* RESOLUTION in bytes; 1 if absent */
+  uint32_t reserved;
+  char utc_start[32];      /* UTC_START as written (YYYY-MM-DD-hh:mm:ss) */
+  char source[64];
+  char telescope[64];
+  char instrument[64];     /* empty if absent */
+} dspx_dada_info;
+
+/* Host-only.  The header fields that ASCIIObservation::load reads for this
+   path (ASCIIObservation.C:24-45, 100-420): returns DSPX_E_PARAM when a
+   required key (TELESCOPE, SOURCE, FREQ, BW, NPOL, NBIT, TSAMP, UTC_START,
+   OBS_OFFSET) is absent or does not parse, naming it in missing (>= 32 bytes,
+   may be null), and DSPX_E_STATE for NDIM other than 1, 2 or 4. */
+int dspx_dada_parse(const char* header, dspx_dada_info* out, char* missing);
+
 /* ------------------------------------------------------------- SampleDelay */
 
 /* SampleDelay::Engine (SampleDelayCUDA.cu:24-120): set_delays once, then

@@ -133,3 +133,51 @@ def test_host_pipeline_interchan(dspx, oracle, tmp_path):
     print("interchan profile rel rms %.3g" % e)
     assert e < 1e-5
     assert np.array_equal(hits, hits_o[:, 0])
+
+
+@pytest.mark.parametrize("mode", ["chain", "fused"])
+def test_host_pipeline_dada(dspx, oracle, tmp_path, mode):
+    """The same block read from a DADA file (shape, band and sample rate from
+    its header) gives the raw run's profiles and hits (bit-identical in the
+    fused form), which the oracle chain checks; the .npz archive holds them."""
+    from test_dada import write_dada
+    assert os.path.exists(EXE), "dspx_pipeline not built"
+    case = "multichan_stokes_subints"
+    nbit, nchan, npol, ndat, cfreq, bw, dm, state, ndim, nbin, period, nsub = CASES[case]
+    raw = _make(dspx, case)
+    tsamp = nchan / abs(bw)  # microseconds per complex sample of a channel
+    assert 1e6 / tsamp == abs(bw) / nchan * 1e6
+    dada = tmp_path / "block.dada"
+    write_dada(dada, raw.reshape(ndat, nchan, npol, 2), cfreq, bw, tsamp, nbit)
+    rawf = tmp_path / "raw.bin"
+    raw.tofile(rawf)
+    common = ["--dm", repr(dm), "--period", repr(period), "--nbin", str(nbin), "--nsub", str(nsub), "--state",
+              STATES[state], "--ndim", str(ndim), "--mode", mode]
+    out_dada, out_raw = tmp_path / "dada.bin", tmp_path / "raw_out.bin"
+    print("host pipeline dada", mode, flush=True)
+    npz = tmp_path / "fold.npz"
+    r = subprocess.run([EXE, "--dada", str(dada), "--out", str(out_dada), "--npz", str(npz)] + common,
+                       capture_output=True, text=True, timeout=300)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0
+    r = subprocess.run([EXE, "--raw", str(rawf), "--nbit", str(nbit), "--nchan", str(nchan), "--npol", str(npol),
+                        "--ndat", str(ndat), "--cfreq", repr(cfreq), "--bw", repr(bw), "--out", str(out_raw)] + common,
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0
+    a, b = np.fromfile(out_dada, np.uint8), np.fromfile(out_raw, np.uint8)
+    if mode == "fused":
+        assert np.array_equal(a, b)  # fixed-order reductions
+    else:
+        # the Fold engine adds with atomics (as the reference's does): the
+        # order of the float additions, and so the last bits, vary run to run
+        nb = nsub * nbin * 4
+        assert np.array_equal(a[-nb:], b[-nb:])
+        assert rel_rms(a[:-nb].view(np.float32), b[:-nb].view(np.float32)) < 1e-6
+    # the archive holds the same profiles and hits, with the fold's parameters
+    z = np.load(npz, allow_pickle=False)
+    data = np.fromfile(out_dada, np.uint8)
+    nprof = z["profiles"].size
+    assert z["profiles"].shape == (nsub, nchan, 4 // ndim, nbin * ndim)
+    assert np.array_equal(z["profiles"].ravel(), data[:nprof * 4].view(np.float32))
+    assert np.array_equal(z["hits"].ravel(), data[nprof * 4:].view(np.uint32))
+    assert float(z["folding_period"]) == period and float(z["dispersion_measure"]) == dm
