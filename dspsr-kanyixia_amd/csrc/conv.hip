@@ -956,9 +956,10 @@ int dspx_ltf_set_fold(dspx_ltf* L, double pps, const dspx_fold_chunk* chunks, ui
   return guard(ctx, [&] {
     DSPX_REQUIRE(L && chunks, DSPX_E_PARAM, "null argument");
     const bool fused = L->d.form == DSPX_FORM_FUSED;
+    L->have_fold = false;  // until the whole plan is in place
     L->fold.build(ctx, L->ndat_out, pps, L->d.nbin, chunks, nchunk, !fused);
-    L->have_fold = true;
     if (fused && L->npart) fused_fold_setup(L);
+    L->have_fold = true;
   });
 }
 
