@@ -141,6 +141,16 @@ template <int L, int B, int RS, int NT> struct ColLayout {
   __device__ static __forceinline__ int idx(int n, int b) { return n * RS + b; }
 };
 
+// The same with the two low column bits XOR-ed by bits 4-5 of the column:
+// a permutation within each row that keeps consecutive columns on distinct
+// banks and also spreads columns read with stride 4 (b = 4k + q, k < 16)
+// over 16 distinct bank pairs.
+template <int L, int B, int RS, int NT> struct ColLayoutSwz {
+  static_assert(B % 4 == 0, "swizzle permutes aligned groups of 4 columns");
+  static constexpr int E = L * B / NT;
+  __device__ static __forceinline__ int idx(int n, int b) { return n * RS + (b ^ ((b >> 4) & 3)); }
+};
+
 // Row layout (B = 1): element n at n + (n >> 4).
 template <int L, int NT> struct RowLayout {
   static constexpr int E = L / NT;

@@ -392,6 +392,50 @@ __device__ __forceinline__ int detect_values(uint32_t state, uint32_t npol, floa
   return 4;
 }
 
+// Products of one sample accumulated with fused multiply-adds, for the
+// fold: NV = 1: c0 += |p|^2 (+ |q|^2 with two pols); NV = 2: c0 += |p|^2,
+// c1 += |q|^2; NV = 4: also c2 += Re(p q*) = p.re q.re + p.im q.im and
+// c3 += Im(p q*) = p.re q.im - p.im q.re (cross_detect.ic, stokes_detect.ic).
+// The detected values of a sum of samples are formed from the sums by
+// detect_finish: the fold adds them anyway, in an order of its own.
+template <int NV>
+__device__ __forceinline__ void detect_acc(uint32_t npol, float2 a, float2 b, float* c)
+{
+  c[0] = fmaf(a.x, a.x, c[0]);
+  c[0] = fmaf(a.y, a.y, c[0]);
+  if constexpr (NV == 1) {
+    if (npol == 2) {
+      c[0] = fmaf(b.x, b.x, c[0]);
+      c[0] = fmaf(b.y, b.y, c[0]);
+    }
+  } else {
+    c[1] = fmaf(b.x, b.x, c[1]);
+    c[1] = fmaf(b.y, b.y, c[1]);
+    if constexpr (NV == 4) {
+      c[2] = fmaf(a.x, b.x, c[2]);
+      c[2] = fmaf(a.y, b.y, c[2]);
+      c[3] = fmaf(a.x, b.y, c[3]);
+      c[3] = fmaf(-a.y, b.x, c[3]);
+    }
+  }
+}
+
+// Stokes (I = PP + QQ, Q = PP - QQ, U = 2 Re, V = 2 Im) or the coherence
+// products themselves, from accumulated products.
+template <int NV>
+__device__ __forceinline__ void detect_finish(bool stokes, const float* c, float* d)
+{
+  if constexpr (NV == 4) {
+    d[0] = stokes ? c[0] + c[1] : c[0];
+    d[1] = stokes ? c[0] - c[1] : c[1];
+    d[2] = stokes ? 2.0f * c[2] : c[2];
+    d[3] = stokes ? 2.0f * c[3] : c[3];
+  } else {
+#pragma unroll
+    for (int i = 0; i < NV; i++) d[i] = c[i];
+  }
+}
+
 // Detection specialised on the number of detected values: NV = 1 Intensity,
 // 2 PPQQ, 4 Coherence (stokes = false) or Stokes.
 template <int NV>
@@ -496,7 +540,8 @@ __global__ __launch_bounds__(kColThreads) void k_stageC(ConvShape s, FilterbankA
   constexpr int E = kColTile / kColThreads;
   constexpr int RI = FirstRadix<N1, 16, +1>::v;
   constexpr int RL = LastRadix<N1, 16, +1>::v;
-  using LAY = ColLayout<N1, C, C, kColThreads>;
+  // swizzled: the last inverse pass reads columns with stride E/RL
+  using LAY = ColLayoutSwz<N1, C, C, kColThreads>;
   extern __shared__ float2 smem[];
   float2* lds = smem;
   float* hist = reinterpret_cast<float*>(smem + kColTile);
@@ -614,30 +659,29 @@ __global__ __launch_bounds__(kColThreads) void k_stageC(ConvShape s, FilterbankA
       for (int r = 0; r < RL; r++) {
         const uint32_t n1 = jr + r * (N1 / RL);
         const uint32_t t0 = s.N2 * n1 + c0 + bq0;
-        float acc[NV];
+        float cacc[NV];
 #pragma unroll
-        for (int i = 0; i < NV; i++) acc[i] = 0.f;
+        for (int i = 0; i < NV; i++) cacc[i] = 0.f;
         uint32_t lane_key = 0xffffffffu;
-        bool divergent = false;
         uint32_t kq[QN];
         load_keys<QN>(kp + t0, kq);
 #pragma unroll
         for (int q = 0; q < QN; q++) {
           const uint32_t key = kq[q];
-          float d[NV];
-          detect_nv<NV>(fa.state == DSPX_STOKES, s.npol, va[q * RL + r], vb[q * RL + r], d);
           if (q == 0) lane_key = key;
           if (key == lane_key) {
-#pragma unroll
-            for (int i = 0; i < NV; i++) acc[i] += d[i];
+            detect_acc<NV>(s.npol, va[q * RL + r], vb[q * RL + r], cacc);
           } else if (key != 0xffffffffu) {
-            divergent = true;  // rare: a bin edge inside this lane's samples
+            // rare: a bin edge inside this lane's samples
+            float d[NV];
+            detect_nv<NV>(fa.state == DSPX_STOKES, s.npol, va[q * RL + r], vb[q * RL + r], d);
             float* hs = hist + (key >> 16) * (NV * fa.nbin) + (key & 0xffffu) * fa.ndim;
 #pragma unroll
             for (int i = 0; i < NV; i++) atomicAdd(hs + fa.voff[i], d[i]);
           }
         }
-        (void) divergent;
+        float acc[NV];
+        detect_finish<NV>(fa.state == DSPX_STOKES, cacc, acc);
         // is the whole 16-lane row in one bin?
         const uint32_t k15 = __shfl(lane_key, lane | 15);
         const uint64_t same = __ballot(lane_key == k15);
