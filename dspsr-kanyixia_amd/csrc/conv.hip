@@ -51,8 +51,9 @@ struct ConvCore {
   double binwidth = 0, half_cw = 0;
   float snorm = 0;
   bool have_table = false;
-  // complex values per thread in the row transform (16 or 8); a tuning
-  // parameter (environment DSPX_ROW_E overrides the default)
+  // complex values per thread in the row transform (16 or 8; the faster
+  // of the two measured for each row length); a tuning parameter
+  // (environment DSPX_ROW_E overrides the default)
   int row_e = 16;
 
   void init(dspx_ctx* c, uint32_t nchan_, uint32_t npol_, uint32_t N_, uint32_t pos, uint32_t neg,
@@ -88,6 +89,7 @@ struct ConvCore {
     tw2.reserve(t.size() * sizeof(float2));
     DSPX_HIP(hipMemcpy(tw2.p, t.data(), t.size() * sizeof(float2), hipMemcpyHostToDevice));
     snorm = float(1.0 / std::sqrt(double(N)));  // Convolution.C:401-414
+    row_e = N2 == 512 || N2 == 1024 ? 8 : 16;
     if (const char* e = std::getenv("DSPX_ROW_E")) row_e = std::atoi(e) == 8 ? 8 : 16;
     if (N2 < 256 * 2 && row_e == 8) row_e = 16;
     if (mode != DSPX_CHIRP_TABLE) {
@@ -189,16 +191,21 @@ struct ConvCore {
                  "no response: call dspx_conv_set_response first");
     ConvShape s = shape(npart);
     ChirpArgs c = chirp();
-    // parts per workgroup: as many as keep >= 2048 workgroups in flight
+    // parts per workgroup: as many as keep >= 2048 workgroups in flight,
+    // and at least enough to give each of the rw rows in flight a row
+    const uint32_t rw = uint32_t(stage_b_rows(int(N2), row_e));
     uint64_t rows = uint64_t(N1) * nchan;
     uint32_t npg = rows >= 2048 ? 1u : uint32_t(std::min<uint64_t>(npart, (2048 + rows - 1) / rows));
-    uint32_t ppg = (npart + npg - 1) / npg;
+    uint32_t ppg = std::max((npart + npg - 1) / npg, std::min(npart, (rw + npol - 1) / npol));
     npg = (npart + ppg - 1) / ppg;
     uint64_t nb = rows * npg;
     DSPX_REQUIRE(nb < (1ull << 31), DSPX_E_UNSUPPORTED, "grid too large");
     const float2* twr = twiddles(ctx, N2);
-#define CASE_B(n, e) \
-  case n * 100 + e: k_stageB<n, e><<<unsigned(nb), n / e, 0, ctx->stream>>>(s, c, twr, ppg); break;
+#define CASE_B(n, e)                                                                      \
+  case n * 100 + e: {                                                                     \
+    constexpr int RW = stage_b_rows(n, e);                                                \
+    k_stageB<n, e, RW><<<unsigned(nb), RW * (n / e), 0, ctx->stream>>>(s, c, twr, ppg);   \
+  } break;
     switch (N2 * 100 + row_e) {
       CASE_B(256, 16) CASE_B(512, 16) CASE_B(1024, 16) CASE_B(2048, 16) CASE_B(4096, 16)
       CASE_B(512, 8) CASE_B(1024, 8) CASE_B(2048, 8) CASE_B(4096, 8)

@@ -232,21 +232,37 @@ __device__ __forceinline__ float2 chirp_onchip(const ChirpArgs& c, uint32_t chan
   return make_float2(c.snorm * cs, c.snorm * sn);
 }
 
-// grid.x = N1 * npg * nchan, block = N2/E.  A workgroup owns row k1 of
+// Rows in flight per stage B workgroup for N2-point rows with E values per
+// thread: the fastest of 1, 2, 4, ... measured on a 64-channel beam
+// (256 points: 3.4x faster than one row per 16-thread workgroup; 512: 1.9x
+// with E = 8; 2048: 1.25x; 1024 and 4096 points: one row).
+constexpr int stage_b_rows(int n2, int e)
+{
+  return e == 16 ? (n2 == 256 ? 16 : n2 == 512 || n2 == 2048 ? 2 : 1) : (n2 == 512 ? 4 : 1);
+}
+
+// grid.x = N1 * npg * nchan, block = RW * N2/E.  A workgroup owns row k1 of
 // channel chan for the parts [pg*ppg, min(npart, (pg+1)*ppg)): the chirp of
 // the row is generated once and applied to every part and polarisation.
-template <int N2, int E>
-__global__ __launch_bounds__(N2 / E) void k_stageB(ConvShape s, ChirpArgs c, const float2* __restrict__ twr,
-                                                   uint32_t ppg)
+// Rows shorter than 1024 points fill less than a wavefront with N2/E
+// threads, so RW of the workgroup's rows are in flight at once, one per
+// group of N2/E threads, each with its own LDS exchange buffer; the chirp
+// and the inter-stage twiddles (functions of k1 only) are shared.
+template <int N2, int E, int RW>
+__global__ __launch_bounds__(RW * N2 / E) void k_stageB(ConvShape s, ChirpArgs c, const float2* __restrict__ twr,
+                                                        uint32_t ppg)
 {
   constexpr int NT = N2 / E;
+  constexpr int LROW = N2 + N2 / 16;
   using LAY = RowLayout<N2, NT>;
-  __shared__ float2 lds[N2 + N2 / 16];
+  __shared__ float2 lds_all[RW * LROW];
   constexpr int RF = FirstRadix<N2, E, -1>::v;   // forward first pass
   constexpr int RI = FirstRadix<N2, E, +1>::v;   // inverse first pass (= forward last)
   constexpr int RL = LastRadix<N2, E, +1>::v;    // inverse last pass
 
-  const int tid = threadIdx.x;
+  const int tid = threadIdx.x % NT;
+  const int sub = RW > 1 ? int(threadIdx.x) / NT : 0;
+  float2* lds = lds_all + sub * LROW;
   const uint32_t npg = (s.npart + ppg - 1) / ppg;
   uint32_t bid = blockIdx.x;
   const uint32_t k1 = bid % s.N1;
@@ -273,11 +289,9 @@ __global__ __launch_bounds__(N2 / E) void k_stageB(ConvShape s, ChirpArgs c, con
   static_assert(RF == RL, "first forward and last inverse radix must agree");
   __shared__ float2 hlds[N2];
 #pragma unroll 1
-  for (int r = 0; r < RI; r++) {
-    uint32_t k2 = tid + r * (N2 / RI);
+  for (int k2 = threadIdx.x; k2 < N2; k2 += RW * NT)
     hlds[k2] = c.mode == DSPX_CHIRP_TABLE ? c.table[(uint64_t(chan) * s.N1 + k1) * N2 + k2]
                                           : chirp_onchip(c, chan, k1 + s.N1 * k2, N);
-  }
   constexpr int Q = E / RF;
   float2 base[Q];
 #pragma unroll
@@ -287,14 +301,18 @@ __global__ __launch_bounds__(N2 / E) void k_stageB(ConvShape s, ChirpArgs c, con
   for (int r = 0; r < RF; r++) step[r] = uniform(tw_N(s.tw, uint32_t(r * (N2 / RF)) * k1));
   __syncthreads();
 
+  // Row i = it * RW + sub.  Every group runs the same number of iterations
+  // (the exchanges inside the transforms are workgroup barriers); a group
+  // past the last row transforms stale registers and stores nothing.
   const uint32_t nrow = (p_end - p_begin) * s.npol;
+  const uint32_t niter = (nrow + RW - 1) / RW;
   auto row_ptr = [&](uint32_t i) {
     uint32_t p = p_begin + i / s.npol, pol = i % s.npol;
     return s.work + ((uint64_t(chan) * s.npart + p) * s.npol + pol) * plane + uint64_t(k1) * N2;
   };
-  float2 pf[E];
-  {
-    const float2* row = row_ptr(0);
+  float2 pf[E] = {};
+  if (uint32_t(sub) < nrow) {
+    const float2* row = row_ptr(sub);
 #pragma unroll
     for (int q = 0; q < Q; q++)
 #pragma unroll
@@ -312,9 +330,8 @@ __global__ __launch_bounds__(N2 / E) void k_stageB(ConvShape s, ChirpArgs c, con
 #pragma unroll
       for (int r = 0; r < RF; r++) w[q * RF + r] = cmul(base[q], step[r]);
   }
-  for (uint32_t i = 0; i < nrow; i++) {
-    float2* row = row_ptr(i);
-    const float2* tw = HOIST ? twr : opaque(twr);
+  for (uint32_t it = 0; it < niter; it++) {
+    const uint32_t i = it * RW + sub;
     float2 v[E];
     if constexpr (HOIST) {
 #pragma unroll
@@ -327,23 +344,27 @@ __global__ __launch_bounds__(N2 / E) void k_stageB(ConvShape s, ChirpArgs c, con
         for (int r = 0; r < RF; r++) v[q * RF + r] = cmul(pf[q * RF + r], cmul(b, step[r]));
       }
     }
-    if (i + 1 < nrow) {
-      const float2* nxt = row_ptr(i + 1);
+    if (i + RW < nrow) {
+      const float2* nxt = row_ptr(i + RW);
 #pragma unroll
       for (int q = 0; q < Q; q++)
 #pragma unroll
         for (int r = 0; r < RF; r++) pf[q * RF + r] = nxt[tid + q * NT + r * (N2 / RF)];
     }
+    const float2* tw = HOIST ? twr : opaque(twr);
     fft_regs<LAY, N2, 1, NT, -1>(lds, v, tw, tid);
 #pragma unroll
     for (int r = 0; r < RI; r++) v[r] = cmul(v[r], hlds[tid + r * (N2 / RI)]);
     fft_regs<LAY, N2, 1, NT, +1>(lds, v, HOIST ? tw : opaque(tw), tid);
+    if (i < nrow) {
+      float2* row = row_ptr(i);
 #pragma unroll
-    for (int q = 0; q < Q; q++) {
-      float2 b = HOIST ? base[q] : opaque_v(base[q]);
+      for (int q = 0; q < Q; q++) {
+        float2 b = HOIST ? base[q] : opaque_v(base[q]);
 #pragma unroll
-      for (int r = 0; r < RL; r++)
-        row[tid + q * NT + r * (N2 / RL)] = cmulc(v[q * RL + r], HOIST ? w[q * RL + r] : cmul(b, step[r]));
+        for (int r = 0; r < RL; r++)
+          row[tid + q * NT + r * (N2 / RL)] = cmulc(v[q * RL + r], HOIST ? w[q * RL + r] : cmul(b, step[r]));
+      }
     }
   }
 }
