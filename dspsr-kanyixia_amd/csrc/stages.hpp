@@ -513,22 +513,44 @@ __device__ __forceinline__ void load_keys(const uint32_t* __restrict__ k, uint32
   }
 }
 
-// Inverse column transform whose last pass hands each thread QN = E/RL
-// consecutive columns b = (tid % (C/QN))*QN + q of the rows
-// j + r*L/RL, j = tid / (C/QN).  On entry v holds the first pass's inputs in
-// the standard ownership; on exit v[q*RL + r] holds y(row j + r*L/RL,
-// column (tid % (C/QN))*QN + q).
+// The fold's ownership of an L-point inverse column transform's outputs:
+// each thread holds QN consecutive columns of R rows.  A multi-pass
+// transform's last pass delivers it directly (QN = E / last radix); a
+// single-pass transform (L = 16) would leave one column per thread, so its
+// outputs are regrouped through LDS into QN = 4 (a 16-lane row then spans
+// 64 consecutive samples and the fold's row sums serve 4 samples per lane).
+template <int L> struct FoldOwn {
+  static constexpr int E = kColTile / kColThreads;
+  static constexpr int RL = LastRadix<L, 16, +1>::v;
+  static constexpr bool REGROUP = RadixPlan<L, 16>::n == 1;
+  static constexpr int QN = REGROUP ? 4 : E / RL;
+  static constexpr int R = E / QN;
+};
+
+// Inverse column transform ending in the fold's ownership (FoldOwn): on
+// entry v holds the first pass's inputs in the standard ownership; on exit
+// v[q*R + r] holds y(row j + r*L/R, column (tid % (C/QN))*QN + q) with
+// j = tid / (C/QN).
 template <class LAY, int L, int C>
 __device__ __forceinline__ void last_pass_cols(float2* lds, float2* v, const float2* __restrict__ tw, int tid)
 {
+  using OWN = FoldOwn<L>;
   constexpr int NPASS = RadixPlan<L, 16>::n;
-  constexpr int E = kColTile / kColThreads;
-  constexpr int RL = LastRadix<L, 16, +1>::v;
-  constexpr int QN = E / RL;
-  if constexpr (NPASS == 1) {
-    // single pass: the standard ownership already is one column per thread
-    static_assert(QN == 1, "single-pass column transform owns one column");
+  constexpr int RL = OWN::RL;
+  constexpr int QN = OWN::QN;
+  const int b0 = (tid % (C / QN)) * QN;
+  const int j = tid / (C / QN);
+  if constexpr (OWN::REGROUP) {
+    // single pass in the standard ownership (one column per thread, the
+    // caller has synchronised), then one exchange
     stockham_compute<L, C, kColThreads, RL, 1, +1>(v, tw, tid);
+    lds_store_pass<LAY, L, C, kColThreads, RL, 1>(lds, v, tid);
+    __syncthreads();
+    constexpr int R = OWN::R;
+#pragma unroll
+    for (int q = 0; q < QN; q++)
+#pragma unroll
+      for (int r = 0; r < R; r++) v[q * R + r] = lds[LAY::idx(j + r * (L / R), b0 + q)];
   } else {
     run_passes<LAY, L, C, kColThreads, +1, 0, NPASS - 1>(lds, v, tw, tid);
     constexpr int RP = InvRadix<L, 16, NPASS - 2>::v;
@@ -537,8 +559,6 @@ __device__ __forceinline__ void last_pass_cols(float2* lds, float2* v, const flo
     __syncthreads();
     lds_store_pass<LAY, L, C, kColThreads, RP, NSP>(lds, v, tid);
     __syncthreads();
-    const int b0 = (tid % (C / QN)) * QN;
-    const int j = tid / (C / QN);
     const int k = j % NS;
 #pragma unroll
     for (int q = 0; q < QN; q++) {
@@ -672,13 +692,14 @@ __global__ __launch_bounds__(kColThreads) void k_stageC(ConvShape s, FilterbankA
         __syncthreads();
         last_pass_cols<LAY, N1, C>(lds, vb, opaque(tw), tid);
       }
-      constexpr int QN = E / RL;
+      constexpr int QN = FoldOwn<N1>::QN;
+      constexpr int RF = FoldOwn<N1>::R;
       const uint32_t bq0 = (tid % (C / QN)) * QN;
       const uint32_t jr = tid / (C / QN);
       const int lane = int(threadIdx.x & 63u);
 #pragma unroll
-      for (int r = 0; r < RL; r++) {
-        const uint32_t n1 = jr + r * (N1 / RL);
+      for (int r = 0; r < RF; r++) {
+        const uint32_t n1 = jr + r * (N1 / RF);
         const uint32_t t0 = s.N2 * n1 + c0 + bq0;
         float cacc[NV];
 #pragma unroll
@@ -691,11 +712,11 @@ __global__ __launch_bounds__(kColThreads) void k_stageC(ConvShape s, FilterbankA
           const uint32_t key = kq[q];
           if (q == 0) lane_key = key;
           if (key == lane_key) {
-            detect_acc<NV>(s.npol, va[q * RL + r], vb[q * RL + r], cacc);
+            detect_acc<NV>(s.npol, va[q * RF + r], vb[q * RF + r], cacc);
           } else if (key != 0xffffffffu) {
             // rare: a bin edge inside this lane's samples
             float d[NV];
-            detect_nv<NV>(fa.state == DSPX_STOKES, s.npol, va[q * RL + r], vb[q * RL + r], d);
+            detect_nv<NV>(fa.state == DSPX_STOKES, s.npol, va[q * RF + r], vb[q * RF + r], d);
             float* hs = hist + (key >> 16) * (NV * fa.nbin) + (key & 0xffffu) * fa.ndim;
 #pragma unroll
             for (int i = 0; i < NV; i++) atomicAdd(hs + fa.voff[i], d[i]);
