@@ -225,29 +225,46 @@ struct ConvCore {
     if (MODE == kStageCFold) shm += size_t(fa.nslot) * fa.nval * fa.nbin * sizeof(float);
     DSPX_REQUIRE(shm <= 160 * 1024, DSPX_E_UNSUPPORTED, "LDS request %zu too large", shm);
     const float2* twc = twiddles(ctx, N1);
+    // The next tile's prefetch (64 registers) where it costs no occupancy:
+    // always for the single-pass 16-point columns; at 32 and 64 points only
+    // where the histogram already holds the CU to two workgroups (without
+    // it those kernels run three waves per SIMD)
+    const bool pf = MODE == kStageCFold && (N1 <= 16 || (N1 <= 64 && 3 * shm > 160 * 1024));
     switch (N1) {
-#define LAUNCH_C(n, nv)                                                                         \
+#define LAUNCH_C(n, nv, p)                                                                      \
   do {                                                                                          \
     /* the whole LDS allowed once per kernel (not a stream operation, so \
        also outside any graph capture that follows) */                                          \
     static bool lds_attr = false;                                                               \
     if (!lds_attr) {                                                                            \
-      allow_full_lds((const void*) k_stageC<n, MODE, nv>);                                      \
+      allow_full_lds((const void*) k_stageC<n, MODE, nv, p>);                                   \
       lds_attr = true;                                                                          \
     }                                                                                           \
-    k_stageC<n, MODE, nv><<<unsigned(nb), kColThreads, shm, ctx->stream>>>(s, fb, fa, twc);     \
+    k_stageC<n, MODE, nv, p><<<unsigned(nb), kColThreads, shm, ctx->stream>>>(s, fb, fa, twc);  \
+  } while (0)
+#define LAUNCH_C2(n, nv)                                     \
+  do {                                                       \
+    if constexpr (MODE == kStageCFold && n <= 64) {          \
+      if (pf)                                                \
+        LAUNCH_C(n, nv, true);                               \
+      else                                                   \
+        LAUNCH_C(n, nv, false);                              \
+    } else {                                                 \
+      LAUNCH_C(n, nv, false);                                \
+    }                                                        \
   } while (0)
 #define CASE_C(n)                                    \
-  case n:                                            \
+  case n: {                                          \
     if (MODE == kStageCFilterbank || fa.nval == 1)   \
-      LAUNCH_C(n, 1);                                \
+      LAUNCH_C2(n, 1);                               \
     else if (fa.nval == 2)                           \
-      LAUNCH_C(n, 2);                                \
+      LAUNCH_C2(n, 2);                               \
     else                                             \
-      LAUNCH_C(n, 4);                                \
-    break;
+      LAUNCH_C2(n, 4);                               \
+  } break;
       CASE_C(16) CASE_C(32) CASE_C(64) CASE_C(128) CASE_C(256)
 #undef CASE_C
+#undef LAUNCH_C2
 #undef LAUNCH_C
       default: fail(DSPX_E_UNSUPPORTED, "N1=%u", N1);
     }

@@ -572,8 +572,10 @@ __device__ __forceinline__ void last_pass_cols(float2* lds, float2* v, const flo
 }
 
 // grid.x = G * npart * nchan (fold) or ncb * npart * nchan (filterbank).
-// NV: detected values per sample in fold mode (1, 2 or 4).
-template <int N1, int MODE, int NV = 1>
+// NV: detected values per sample in fold mode (1, 2 or 4).  PFT: the fold
+// loads each tile's inputs while the previous tile is transformed (64 more
+// registers; the launch chooses it where occupancy is set by the LDS).
+template <int N1, int MODE, int NV = 1, bool PFT = false>
 __global__ __launch_bounds__(kColThreads) void k_stageC(ConvShape s, FilterbankArgs fb, FoldArgs fa,
                                                        const float2* __restrict__ twc)
 {
@@ -630,9 +632,7 @@ __global__ __launch_bounds__(kColThreads) void k_stageC(ConvShape s, FilterbankA
       }
     }
   };
-  // (prefetch only where its 64 registers do not cost occupancy: long
-  // columns already need more registers for the column transform)
-  constexpr bool PF = MODE == kStageCFold && N1 <= 64;
+  constexpr bool PF = MODE == kStageCFold && PFT;
   float2 pa[E], pb[E];
   if (PF && cb_begin < cb_end) load_tile(cb_begin, pa, pb);
   for (uint32_t cb = cb_begin; cb < cb_end; cb++) {
