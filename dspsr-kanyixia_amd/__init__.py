@@ -250,6 +250,11 @@ class Context:
 
     def close(self):
         if self.h:
+            # engines made on this context hold its stream and device: they
+            # are destroyed first, never after the context is gone
+            for o in list(_live_engines):
+                if getattr(o, "ctx", None) is self:
+                    o.close()
             self._lib.dspx_ctx_destroy(self.h)
             self.h = None
 
