@@ -465,18 +465,26 @@ __global__ __launch_bounds__(256) void k_fused_keys(const FoldSegment* __restric
   const uint32_t s0 = sub_first[p];
   const uint32_t hn = nslot * nbin;
   for (uint32_t k = threadIdx.x; k < hn; k += blockDim.x) h[k] = 0;
-  if (threadIdx.x == 0) {
-    const uint32_t tc = uint32_t(c0 - uint64_t(p) * nfft);
-    const uint32_t tf = max(tc, nfilt_pos);
-    const uint64_t idat0 = uint64_t(p) * step + (tf < nfilt_pos + step ? tf - nfilt_pos : 0);
-    s_si0 = seg_find(seg, nseg, idat0);
+  uint32_t si0 = 0;
+  if (nseg <= kSegCache) {
+    // every segment fits: one load round, no search
+    if (threadIdx.x < nseg) sseg[threadIdx.x] = seg[threadIdx.x];
+    if (threadIdx.x == 0) s_jend = ~uint64_t(0);
+  } else {
+    if (threadIdx.x == 0) {
+      const uint32_t tc = uint32_t(c0 - uint64_t(p) * nfft);
+      const uint32_t tf = max(tc, nfilt_pos);
+      const uint64_t idat0 = uint64_t(p) * step + (tf < nfilt_pos + step ? tf - nfilt_pos : 0);
+      s_si0 = seg_find(seg, nseg, idat0);
+    }
+    __syncthreads();
+    si0 = s_si0;
+    const uint32_t nc = min(kSegCache, nseg - si0);
+    if (threadIdx.x < nc) sseg[threadIdx.x] = seg[si0 + threadIdx.x];
+    if (threadIdx.x == 0) s_jend = si0 + nc < nseg ? seg[si0 + nc].j0 : ~uint64_t(0);
   }
   __syncthreads();
-  const uint32_t si0 = s_si0;
   const uint32_t ns = min(kSegCache, nseg - si0);
-  if (threadIdx.x < ns) sseg[threadIdx.x] = seg[si0 + threadIdx.x];
-  if (threadIdx.x == 0) s_jend = si0 + ns < nseg ? seg[si0 + ns].j0 : ~uint64_t(0);
-  __syncthreads();
   const uint64_t jbeg = sseg[0].j0, jend = s_jend;
   const uint64_t i0 = c0 + uint64_t(threadIdx.x) * kKeyRun;
   uint32_t kv[kKeyRun];

@@ -616,50 +616,36 @@ __global__ __launch_bounds__(kColThreads) void k_stageC(ConvShape s, FilterbankA
   const uint64_t plane = uint64_t(s.N1) * s.N2;
   const float2* wbase = s.work + (uint64_t(chan) * s.npart + p) * s.npol * plane;
 
-  // the tile's first-pass inputs, loaded one tile ahead of the transform
+  // The tile's first-pass inputs.  Each address is a workgroup-uniform row
+  // base (scalar registers: tile column, row r of the butterfly) plus a
+  // per-thread 32-bit byte offset that does not change from tile to tile,
+  // so the loads need no per-element 64-bit address arithmetic.
+  const char* wbytes = reinterpret_cast<const char*>(wbase);
+  const uint64_t plane_bytes = plane * sizeof(float2);
+  uint32_t voff[E / RI];
+#pragma unroll
+  for (int q = 0; q < E / RI; q++) {
+    const int gg = tid + q * kColThreads;
+    voff[q] = (uint32_t(gg / C) * s.N2 + uint32_t(gg % C)) * uint32_t(sizeof(float2));
+  }
   auto load_tile = [&](uint32_t cb, float2* xa, float2* xb) {
-    const uint32_t c0 = cb * C;
+    const char* t0 = wbytes + uint64_t(cb) * C * sizeof(float2);
 #pragma unroll
     for (int q = 0; q < E / RI; q++) {
-      int gg = tid + q * kColThreads;
-      int b = gg % C, j = gg / C;
 #pragma unroll
       for (int r = 0; r < RI; r++) {
-        uint32_t k1 = j + r * (N1 / RI);
-        uint64_t off = uint64_t(k1) * s.N2 + c0 + b;
-        xa[q * RI + r] = wbase[off];
-        xb[q * RI + r] = s.npol == 2 ? wbase[plane + off] : make_float2(0.f, 0.f);
+        const char* rowp = t0 + uint64_t(r * (N1 / RI)) * s.N2 * sizeof(float2);
+        xa[q * RI + r] = *reinterpret_cast<const float2*>(rowp + voff[q]);
+        xb[q * RI + r] =
+            s.npol == 2 ? *reinterpret_cast<const float2*>(rowp + plane_bytes + voff[q]) : make_float2(0.f, 0.f);
       }
     }
   };
-  constexpr bool PF = MODE == kStageCFold && PFT;
-  float2 pa[E], pb[E];
-  if (PF && cb_begin < cb_end) load_tile(cb_begin, pa, pb);
-  for (uint32_t cb = cb_begin; cb < cb_end; cb++) {
+  // One tile: column transform, then the filterbank store or the fold.
+  // va, vb hold the tile's inputs and are overwritten.
+  auto tile = [&](uint32_t cb, float2* va, float2* vb) {
     const uint32_t c0 = cb * C;
     const float2* tw = opaque(twc);
-    float2 va[E], vb[E];
-    if constexpr (PF) {
-#pragma unroll
-      for (int e = 0; e < E; e++) {
-        va[e] = pa[e];
-        vb[e] = pb[e];
-      }
-      if (cb + 1 < cb_end) load_tile(cb + 1, pa, pb);
-    } else {
-#pragma unroll
-      for (int q = 0; q < E / RI; q++) {
-        int gg = tid + q * kColThreads;
-        int b = gg % C, j = gg / C;
-#pragma unroll
-        for (int r = 0; r < RI; r++) {
-          uint32_t k1 = j + r * (N1 / RI);
-          uint64_t off = uint64_t(k1) * s.N2 + c0 + b;
-          va[q * RI + r] = wbase[off];
-          vb[q * RI + r] = s.npol == 2 ? wbase[plane + off] : make_float2(0.f, 0.f);
-        }
-      }
-    }
     if constexpr (MODE == kStageCFilterbank) {
       __syncthreads();  // previous tile's LDS reads are complete
       fft_regs<LAY, N1, C, kColThreads, +1>(lds, va, tw, tid);
@@ -697,16 +683,17 @@ __global__ __launch_bounds__(kColThreads) void k_stageC(ConvShape s, FilterbankA
       const uint32_t bq0 = (tid % (C / QN)) * QN;
       const uint32_t jr = tid / (C / QN);
       const int lane = int(threadIdx.x & 63u);
+      // key address: uniform row base plus a per-thread 32-bit byte offset
+      const uint32_t kvo = (s.N2 * jr + bq0) * uint32_t(sizeof(uint32_t));
 #pragma unroll
       for (int r = 0; r < RF; r++) {
-        const uint32_t n1 = jr + r * (N1 / RF);
-        const uint32_t t0 = s.N2 * n1 + c0 + bq0;
         float cacc[NV];
 #pragma unroll
         for (int i = 0; i < NV; i++) cacc[i] = 0.f;
         uint32_t lane_key = 0xffffffffu;
         uint32_t kq[QN];
-        load_keys<QN>(kp + t0, kq);
+        const char* krow = reinterpret_cast<const char*>(kp + c0 + uint64_t(r * (N1 / RF)) * s.N2);
+        load_keys<QN>(reinterpret_cast<const uint32_t*>(krow + kvo), kq);
 #pragma unroll
         for (int q = 0; q < QN; q++) {
           const uint32_t key = kq[q];
@@ -743,6 +730,30 @@ __global__ __launch_bounds__(kColThreads) void k_stageC(ConvShape s, FilterbankA
           for (int i = 0; i < NV; i++) atomicAdd(hs + fa.voff[i], acc[i]);
         }
       }
+    }
+  };
+  // With PF the next tile's inputs are loaded while the current one is
+  // transformed (two register sets used in turn, without the copy, measured
+  // 5% slower on the 128-channel fold: 0.545 against 0.517 ms).
+  constexpr bool PF = MODE == kStageCFold && PFT;
+  if constexpr (PF) {
+    float2 pa[E], pb[E];
+    if (cb_begin < cb_end) load_tile(cb_begin, pa, pb);
+    for (uint32_t cb = cb_begin; cb < cb_end; cb++) {
+      float2 va[E], vb[E];
+#pragma unroll
+      for (int e = 0; e < E; e++) {
+        va[e] = pa[e];
+        vb[e] = pb[e];
+      }
+      if (cb + 1 < cb_end) load_tile(cb + 1, pa, pb);
+      tile(cb, va, vb);
+    }
+  } else {
+    for (uint32_t cb = cb_begin; cb < cb_end; cb++) {
+      float2 va[E], vb[E];
+      load_tile(cb, va, vb);
+      tile(cb, va, vb);
     }
   }
 
