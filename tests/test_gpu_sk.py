@@ -10,14 +10,7 @@ import pytest
 
 from helpers import from_dev, sk_rfi_input, to_dev
 
-# The first GPU runs of this module passed every test, but the test process
-# then ended with a segmentation fault during interpreter exit, after pytest
-# had finished (no traceback).  Its cause is not established (the process
-# also held a second OpenBLAS/libgfortran, loaded by the threshold quadrature,
-# which the tables below now avoid), so the module stays skipped on the GPU.
-pytestmark = [pytest.mark.gpu,
-              pytest.mark.skip(reason="exit-time segmentation fault of the test process after these tests "
-                                      "passed; cause not established")]
+pytestmark = pytest.mark.gpu
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "sk_limits.npz")
 
