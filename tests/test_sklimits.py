@@ -71,6 +71,9 @@ def test_committed_tables_match_the_restatement():
     z = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "sk_limits.npz"))
     lo, up, sym, pcsd = S.limit_tables(64, 3.0, npol=2, counts=[64])
     assert np.array_equal(z["lower64"], lo) and np.array_equal(z["upper64"], up) and z["pcsd64"] == pcsd
+    lo, up, sym, pcsd = S.limit_tables(256, 3.0, npol=2, counts=[256])
+    assert np.array_equal(z["lower256"], lo) and np.array_equal(z["upper256"], up) and z["pcsd256"] == pcsd
+    assert np.array_equal(z["sym256"], sym)
     lo, up, sym, pcsd = S.limit_tables(32, 3.0, npol=2, counts=[3, 6, 17, 32])
     for c in (3, 6, 17, 32):
         assert z["lower32"][c] == lo[c] and z["upper32"][c] == up[c] and z["sym32"][c] == sym[c]
