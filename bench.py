@@ -261,6 +261,156 @@ def run_uneven(args, dspx, synth, ctx):
     print(json.dumps(out))
 
 
+ENGINE_WORKLOADS = ("sk", "rfinet", "filterbank", "delay")
+
+
+def _event_ms(fn, reps):
+    """Average milliseconds of ``fn`` over ``reps`` calls, by HIP events on
+    the current (library) stream, after one untimed call."""
+    import torch
+    fn()
+    s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    s.record()
+    for _ in range(reps):
+        fn()
+    e.record()
+    torch.cuda.synchronize()
+    return s.elapsed_time(e) / reps
+
+
+def run_engine(args, dspx, synth, ctx):
+    """The engines beside the dedispersion path, each on device-resident
+    synthetic input of the shape it meets in the pipeline, timed by HIP
+    events: one JSON line with the throughput and the bound of its main
+    kernel (algorithmic bytes, or flops for the network, per call).
+
+      sk:         SpectralKurtosis on 512 channels x 2 pols x 2^20 int8
+                  samples with injected RFI, M = 1024: block sums from the
+                  raw bytes, the three detectors, the mask of the unpacked
+                  float samples (the mask is the HBM-heavy step).
+      rfinet:     the proposal network's inference (features, 4 conv layers,
+                  threshold) on that data's 1024 x 512 blocks.
+      filterbank: the convolving filterbank engine, 128 channels x 2 pols x
+                  2^20 float samples into 16 sub-bands of 128 bins.
+      delay:      SampleDelay on 128 channels x 2 pols x 2^20 float samples
+                  (the -K inter-channel delays of the 128-channel band at
+                  DM 100; at DM 300 the band's delay exceeds the block)."""
+    import numpy as np
+    import torch
+    w = args.workload
+    reps = max(3, args.steps)
+    out = {"metric": None, "value": None, "unit": None, "n_gpus": 1, "steps": reps, "warmup": 1,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+           "data": "synthetic", "cpu_baseline": None}
+    if w in ("sk", "rfinet"):
+        nchan, npol, ndat, M = 512, 2, 1 << 20, 1024
+        nblk = ndat // M
+        raw, truth = synth.make_rfi_baseband_torch(nchan, npol, ndat, M, 11)
+        torch.cuda.synchronize()
+        S = importlib.import_module("dspsr-kanyixia_amd.sklimits")
+        lo, up, sym, pcsd = S.limit_tables(M, 3.0, npol=npol, counts=[M])
+        sk = dspx.SpectralKurtosis(ctx, nchan, npol, M, omit_outliers=True)
+        sk.set_limits(lo, up, sym, pcsd)
+        sums = torch.zeros((nblk, nchan, npol, 2), dtype=torch.float32, device="cuda")
+        counts = torch.zeros((nblk, nchan), dtype=torch.int32, device="cuda")
+        mask = torch.zeros((nblk, nchan), dtype=torch.uint8, device="cuda")
+        nsamp = nchan * npol * ndat
+        if w == "sk":
+            x = torch.empty((nchan, npol, ndat, 2), dtype=torch.float32, device="cuda")
+            x.copy_(raw.permute(1, 2, 0, 3).float())
+            y = torch.empty_like(x)
+            t_c = _event_ms(lambda: sk.compute_raw(raw, 8, 0, nblk, sums, counts), reps)
+            s2, c2 = sums.clone(), counts.clone()
+
+            def det():
+                s2.copy_(sums)
+                c2.copy_(counts)
+                sk.detect(s2, c2, nblk, 7, mask)
+            t_d = _event_ms(det, reps)
+            t_m = _event_ms(lambda: sk.mask(mask, x, npol * ndat * 2, ndat * 2, y, npol * ndat * 2, ndat * 2, nblk),
+                            reps)
+            t = t_c + t_d + t_m
+            mbytes = 2 * nsamp * 8 + nblk * nchan
+            frac_flag = float(mask.float().mean())
+            out.update({"metric": "Msamples/s spectral kurtosis (sums from raw, detect, mask)",
+                        "value": round(nsamp / (t * 1e-3) / 1e6, 2), "unit": "Msamples/s",
+                        "ms_per_step": round(t, 4),
+                        "config": {"workload": "sk: 512 chan x 2 pol x 2^20 complex int8 with injected RFI, "
+                                               "M 1024, 3 sigma, skfb+fscr+tscr with omit_outliers",
+                                   "flagged_fraction": round(frac_flag, 4)},
+                        "stages_ms": {"compute_raw": round(t_c, 4), "detect": round(t_d, 4), "mask": round(t_m, 4)},
+                        "roofline": {"bound": "hbm", "kernel": "mask", "achieved": round(mbytes / t_m / 1e6, 1),
+                                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                     "frac": round(mbytes / t_m / 1e6 / HBM_PEAK_GBS, 4), "traffic": None,
+                                     "compute_raw_GBps": round(nsamp * 2 / t_c / 1e6, 1)}})
+        else:
+            sk.compute_raw(raw, 8, 0, nblk, sums, counts)
+            feat = torch.zeros((nblk, 4, nchan), dtype=torch.float32, device="cuda")
+            logits = torch.zeros((nblk, nchan), dtype=torch.float32, device="cuda")
+            net = dspx.RfiNet(ctx, seed=7)
+
+            def infer():
+                dspx.rfi_features(ctx, sums, counts, nblk, nchan, npol, feat)
+                net.forward(feat, nblk, nchan, logits)
+                net.propose(logits, nblk * nchan, mask)
+            t = _event_ms(infer, reps)
+            t_f = _event_ms(lambda: net.forward(feat, nblk, nchan, logits), reps)
+            K, Wd, F = 19, 32, 4
+            flops = 2 * nblk * nchan * K * (F * Wd + 2 * Wd * Wd + Wd)
+            net.close()
+            out.update({"metric": "block-channel positions/s, RFI proposal inference (features, network, threshold)",
+                        "value": round(nblk * nchan / (t * 1e-3), 1), "unit": "positions/s",
+                        "ms_per_step": round(t, 4),
+                        "config": {"workload": "rfinet: SK sums of 512 chan x 2^20 samples, M 1024 -> 1024 x 512 "
+                                               "positions, 4 conv layers (4-32-32-32-1, kernel 19 across channels)"},
+                        "roofline": {"bound": "valu", "kernel": "forward (4 conv layers)",
+                                     "achieved": round(flops / t_f / 1e9, 2), "peak": 78.6, "unit": "TFLOP/s",
+                                     "frac": round(flops / t_f / 1e9 / 78.6, 4), "traffic": None,
+                                     "note": "FP32 VALU peak without packed FP32 instructions"}})
+        sk.close()
+    elif w in ("filterbank", "delay"):
+        nchan, npol, ndat = 128, 2, 1 << 20
+        g = torch.Generator(device="cuda")
+        g.manual_seed(5)
+        x = torch.randn((nchan, npol, ndat, 2), generator=g, device="cuda") * 10
+        nsamp = nchan * npol * ndat
+        if w == "filterbank":
+            nsub, fres = 16, 128
+            fb = dspx.Filterbank(ctx, nchan, npol, nsub, fres)
+            resp = np.exp(1j * np.random.default_rng(1).uniform(0, 2 * np.pi, (nchan * nsub, fres)))
+            fb.set_response(resp.astype(np.complex64))
+            npart, nout = fb.geometry(ndat)  # nout samples per output channel
+            y = torch.empty((nchan * nsub, npol, nout, 2), dtype=torch.float32, device="cuda")
+            t = _event_ms(lambda: fb.perform(x, npol * ndat * 2, ndat * 2, ndat, y, npol * nout * 2, nout * 2), reps)
+            fb.close()
+            abytes = nsamp * 8 + nchan * nsub * npol * nout * 8
+            out.update({"metric": "Msamples/s convolving filterbank (FFT-library form)",
+                        "value": round(nsamp / (t * 1e-3) / 1e6, 2), "unit": "Msamples/s", "ms_per_step": round(t, 4),
+                        "config": {"workload": "filterbank: 128 chan x 2 pol x 2^20 complex float -> 16 sub-bands "
+                                               "x 128 bins per channel", "npart": npart, "ndat_out": nout},
+                        "roofline": {"bound": "hbm", "kernel": "whole engine (unpack-free float in, float out)",
+                                     "achieved": round(abytes / t / 1e6, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                     "frac": round(abytes / t / 1e6 / HBM_PEAK_GBS, 4), "traffic": None}})
+        else:
+            sd = dspx.sample_delays(1400.0, 400.0, nchan, 100.0)
+            dl = dspx.SampleDelay(ctx, nchan, npol, sd["applied"])
+            nret = ndat - int(sd["total"])
+            y = torch.empty((nchan, npol, nret, 2), dtype=torch.float32, device="cuda")
+            t = _event_ms(lambda: dl.retard(x, npol * ndat * 2, ndat * 2, 2, nret, y, npol * nret * 2, nret * 2), reps)
+            dl.close()
+            abytes = 2 * nchan * npol * nret * 8
+            out.update({"metric": "Msamples/s SampleDelay (inter-channel delays, -K)",
+                        "value": round(nchan * npol * nret / (t * 1e-3) / 1e6, 2), "unit": "Msamples/s",
+                        "ms_per_step": round(t, 4),
+                        "config": {"workload": "delay: 128 chan x 2 pol x 2^20 complex float, 400 MHz at 1400 MHz, "
+                                               "DM 100", "total_delay": int(sd["total"])},
+                        "roofline": {"bound": "hbm", "kernel": "k_retard", "achieved": round(abytes / t / 1e6, 1),
+                                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                     "frac": round(abytes / t / 1e6 / HBM_PEAK_GBS, 4), "traffic": None}})
+    print(json.dumps(out))
+
+
 def load_traffic(workload, kernel):
     """HBM bytes per launch of ``kernel`` from the committed counter summary."""
     path = os.path.join(ROOT, "profiles", "traffic_%s.json" % workload)
@@ -291,7 +441,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="multichan", choices=list(WORKLOADS) + ["uneven"])
+    ap.add_argument("--workload", default="multichan", choices=list(WORKLOADS) + ["uneven"] + list(ENGINE_WORKLOADS))
     ap.add_argument("--nbeam", type=int, default=48)
     ap.add_argument("--uneven-log2", type=float, default=22.0, help="largest log2 samples per channel (uneven)")
     ap.add_argument("--no-cpu", action="store_true")
@@ -319,6 +469,10 @@ def main():
     ctx = dspx.Context(0, stream.cuda_stream)
     if args.workload == "uneven":
         run_uneven(args, dspx, synth, ctx)
+        ctx.close()
+        return
+    if args.workload in ENGINE_WORKLOADS:
+        run_engine(args, dspx, synth, ctx)
         ctx.close()
         return
     w = WORKLOADS[args.workload]
