@@ -12,17 +12,28 @@ namespace dspx {
 // grid (ceil(nfloat / (256 * 4)), nchan * npol); nfloat = ndat_out * ndim
 __global__ __launch_bounds__(256) void k_retard(const float* __restrict__ in, uint64_t ics, uint64_t ips,
                                                 float* __restrict__ out, uint64_t ocs, uint64_t ops, uint32_t npol,
-                                                uint32_t ndim, const int64_t* __restrict__ applied, uint64_t nfloat,
-                                                int vec)
+                                                uint32_t ndim, const int64_t* __restrict__ applied, uint64_t nfloat)
 {
   const uint32_t cp = blockIdx.y;
   const uint32_t chan = cp / npol, pol = cp % npol;
   const float* x = in + chan * ics + pol * ips + uint64_t(applied[cp]) * ndim;
   float* y = out + chan * ocs + pol * ops;
   const uint64_t i = (uint64_t(blockIdx.x) * blockDim.x + threadIdx.x) * 4;
-  if (vec && i + 4 <= nfloat && (reinterpret_cast<uintptr_t>(x) & 15) == 0) {
-    *reinterpret_cast<float4*>(y + i) = *reinterpret_cast<const float4*>(x + i);
-    return;
+  // 16-byte moves where both rows allow them, else 8-byte ones (complex
+  // rows start 8-byte aligned whatever the delay and the row length); the
+  // alignment is uniform over the workgroup
+  const uintptr_t al = reinterpret_cast<uintptr_t>(x) | reinterpret_cast<uintptr_t>(y);
+  if (i + 4 <= nfloat) {
+    if ((al & 15) == 0) {
+      *reinterpret_cast<float4*>(y + i) = *reinterpret_cast<const float4*>(x + i);
+      return;
+    }
+    if ((al & 7) == 0) {
+      const float2 a = *reinterpret_cast<const float2*>(x + i), b = *reinterpret_cast<const float2*>(x + i + 2);
+      *reinterpret_cast<float2*>(y + i) = a;
+      *reinterpret_cast<float2*>(y + i + 2) = b;
+      return;
+    }
   }
   for (uint64_t k = i; k < i + 4 && k < nfloat; k++) y[k] = x[k];
 }
@@ -82,15 +93,9 @@ int dspx_delay_retard(dspx_delay* D, const float* d_in, uint64_t ics, uint64_t i
     if (!nfloat) return;
     const uint64_t nb = (nfloat + 1023) / 1024;
     DSPX_REQUIRE(nb < (1ull << 31), DSPX_E_UNSUPPORTED, "block too long");
-    // whole float4 moves need 16-byte aligned output rows; the input offset
-    // is checked per row in the kernel
-    const int vec = (reinterpret_cast<uintptr_t>(d_out) % 16 == 0 && ocs % 4 == 0 && ops % 4 == 0 &&
-                     reinterpret_cast<uintptr_t>(d_in) % 16 == 0 && ics % 4 == 0 && ips % 4 == 0)
-                        ? 1
-                        : 0;
     dim3 grid(unsigned(nb), D->nchan * D->npol);
     k_retard<<<grid, 256, 0, D->ctx->stream>>>(d_in, ics, ips, d_out, ocs, ops, D->npol, ndim,
-                                                D->applied.as<int64_t>(), nfloat, vec);
+                                                D->applied.as<int64_t>(), nfloat);
     DSPX_HIP(hipGetLastError());
   });
 }

@@ -229,48 +229,84 @@ __global__ __launch_bounds__(kSkThreads) void k_sk_fscr(const float* __restrict_
     for (uint32_t ch = tid; ch < nchan; ch += kSkThreads) mrow[ch] = 1;
 }
 
-// detect_tscr (SpectralKurtosis.C:1033-1095): one thread per channel, blocks
-// summed in their order.  Totals above M use the closed form of
+// detect_tscr (SpectralKurtosis.C:1033-1095): one workgroup per channel.
+// The reference adds the blocks' sums in block order; here they are added in
+// a fixed tree, in double precision as there, which can move the statistic
+// only in the last bits of a double.  Totals above M use the closed form of
 // SKLimits.C:152-160 when >= 32768; totals in between are not tested.
-__global__ void k_sk_tscr(const float* __restrict__ sums, const uint32_t* __restrict__ counts, uint64_t npart,
-                          uint32_t nchan, uint32_t npol, uint32_t M, uint32_t Nd, const float* __restrict__ lower,
-                          const float* __restrict__ upper, double pcsd, uint8_t* __restrict__ mask)
+__global__ __launch_bounds__(kSkThreads) void k_sk_tscr(const float* __restrict__ sums,
+                                                       const uint32_t* __restrict__ counts, uint64_t npart,
+                                                       uint32_t nchan, uint32_t npol, uint32_t M, uint32_t Nd,
+                                                       const float* __restrict__ lower,
+                                                       const float* __restrict__ upper, double pcsd,
+                                                       uint8_t* __restrict__ mask)
 {
 #pragma clang fp contract(off)
-  uint32_t ch = blockIdx.x * blockDim.x + threadIdx.x;
-  if (ch >= nchan) return;
+  // one workgroup per channel: the count and the double sums over the
+  // blocks, each thread taking every kSkThreads-th block, then a fixed-order
+  // tree in LDS
+  __shared__ double r1[2][kSkThreads], r2[2][kSkThreads];
+  __shared__ uint64_t rc[kSkThreads];
+  __shared__ int zap;
+  const uint32_t ch = blockIdx.x;
+  const int tid = threadIdx.x;
   uint64_t count = 0;
-  for (uint64_t p = 0; p < npart; p++) count += counts[p * nchan + ch];
-  if (count == 0) return;
-  float lo, up;
-  if (count <= M) {
-    lo = lower[count];
-    up = upper[count];
-  } else if (count >= 32768) {
-    double factor = sqrt(4.0 / double(count)) * pcsd;
-    lo = float(1.0f - factor);
-    up = float(1.0f + factor);
-  } else {
-    return;
-  }
-  const float M_t = float(count);
-  const float M_fac = (M_t * Nd + 1) / (M_t - 1);
-  bool zap_chan = false;
-  for (uint32_t pol = 0; pol < npol; pol++) {
-    double S1 = 0.0, S2 = 0.0;
-    for (uint64_t p = 0; p < npart; p++) {
-      uint64_t idx = (p * nchan + ch) * npol + pol;
-      S1 += sums[idx * 2];
-      S2 += sums[idx * 2 + 1];
+  double s1[2] = {0.0, 0.0}, s2[2] = {0.0, 0.0};
+  for (uint64_t p = tid; p < npart; p += kSkThreads) {
+    count += counts[p * nchan + ch];
+    for (uint32_t pol = 0; pol < npol; pol++) {
+      const uint64_t idx = (p * nchan + ch) * npol + pol;
+      s1[pol] += sums[idx * 2];
+      s2[pol] += sums[idx * 2 + 1];
     }
-    double SK = M_fac * (M_t * (S2 / (S1 * S1)) - 1);
-    if (SK > up || SK < lo) zap_chan = true;
   }
-  if (zap_chan)
-    for (uint64_t p = 0; p < npart; p++) mask[p * nchan + ch] = 1;
+  rc[tid] = count;
+  for (int pol = 0; pol < 2; pol++) {
+    r1[pol][tid] = s1[pol];
+    r2[pol][tid] = s2[pol];
+  }
+  for (int st = kSkThreads / 2; st > 0; st >>= 1) {
+    __syncthreads();
+    if (tid < st) {
+      rc[tid] += rc[tid + st];
+      for (int pol = 0; pol < 2; pol++) {
+        r1[pol][tid] += r1[pol][tid + st];
+        r2[pol][tid] += r2[pol][tid + st];
+      }
+    }
+  }
+  if (tid == 0) {
+    zap = 0;
+    const uint64_t total = rc[0];
+    float lo = 0.f, up = 0.f;
+    bool test = true;
+    if (total == 0) {
+      test = false;
+    } else if (total <= M) {
+      lo = lower[total];
+      up = upper[total];
+    } else if (total >= 32768) {
+      double factor = sqrt(4.0 / double(total)) * pcsd;
+      lo = float(1.0f - factor);
+      up = float(1.0f + factor);
+    } else {
+      test = false;  // totals between M and 32768 are not tested
+    }
+    if (test) {
+      const float M_t = float(total);
+      const float M_fac = (M_t * Nd + 1) / (M_t - 1);
+      for (uint32_t pol = 0; pol < npol; pol++) {
+        const double S1 = r1[pol][0], S2 = r2[pol][0];
+        const double SK = M_fac * (M_t * (S2 / (S1 * S1)) - 1);
+        if (SK > up || SK < lo) zap = 1;
+      }
+    }
+  }
+  __syncthreads();
+  if (zap)
+    for (uint64_t p = tid; p < npart; p += kSkThreads) mask[p * nchan + ch] = 1;
 }
 
-// mask (SpectralKurtosis.C:1531-1554): grid (ceil(npart*M/256), nchan*npol).
 __global__ void k_sk_mask(const uint8_t* __restrict__ mask, const float* __restrict__ in, uint64_t ics, uint64_t ips,
                           float* __restrict__ out, uint64_t ocs, uint64_t ops, uint32_t nchan, uint32_t npol,
                           uint32_t M, uint64_t n)
@@ -395,7 +431,7 @@ int dspx_sk_detect(dspx_sk* s, float* d_sums, uint32_t* d_counts, uint64_t npart
       k_sk_fscr<<<unsigned(npart), kSkThreads, 0, st>>>(d_sums, d_counts, d.nchan, d.npol, d.M, d.Nd,
                                                         d.omit_outliers, sym, d_mask);
     if (flags & DSPX_SK_TSCR)
-      k_sk_tscr<<<(d.nchan + 255) / 256, 256, 0, st>>>(d_sums, d_counts, npart, d.nchan, d.npol, d.M, d.Nd, lower,
+      k_sk_tscr<<<d.nchan, kSkThreads, 0, st>>>(d_sums, d_counts, npart, d.nchan, d.npol, d.M, d.Nd, lower,
                                                         upper, s->pcsd, d_mask);
     DSPX_HIP(hipGetLastError());
   });
