@@ -8,6 +8,8 @@
 // [nfilt_pos, nfilt_pos + nkeep) written to output channel ichan*nsub + isub.
 // The windows are batched into library transforms; the response product and
 // the overlap discard are kernels of their own.
+#include <cstdlib>
+
 #include <hipfft/hipfft.h>
 
 #include <algorithm>
@@ -50,15 +52,16 @@ __global__ void k_fb_multiply(float2* __restrict__ scratch, const float2* __rest
   scratch[b * nfft + k] = make_float2(f.x * d.x - f.y * d.y, f.y * d.x + f.x * d.y);
 }
 
-// grid (ceil(nkeep/256), nb * nsub)
+// grid (ceil(nkeep/256) * nsub, nb): x = (sub-band, block of 256 samples)
 __global__ void k_fb_discard(const float2* __restrict__ scratch, uint64_t nfft, uint32_t fres, uint32_t nsub,
                              uint32_t npol, uint32_t npart, uint64_t u0, uint32_t nkeep, uint32_t nfilt_pos,
                              float* __restrict__ out, uint64_t ocs, uint64_t ops)
 {
-  uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t nmb = (nkeep + blockDim.x - 1) / blockDim.x;
+  const uint32_t isub = blockIdx.x / nmb;
+  const uint32_t m = (blockIdx.x % nmb) * blockDim.x + threadIdx.x;
   if (m >= nkeep) return;
-  uint64_t b = blockIdx.y / nsub;
-  uint32_t isub = blockIdx.y % nsub;
+  const uint64_t b = blockIdx.y;
   uint64_t u = u0 + b;
   uint64_t p = u % npart, cp = u / npart;
   uint64_t chan = cp / npol, pol = cp % npol;
@@ -155,9 +158,18 @@ int dspx_fb_perform(dspx_fb* F, const float* d_in, uint64_t ics, uint64_t ips, u
     const dspx_fb_desc& d = F->d;
     const uint64_t nunit = uint64_t(d.nchan) * d.npol * npart;
     const uint64_t per = F->nfft * sizeof(float2);
-    uint64_t batch = std::max<uint64_t>(1, std::min<uint64_t>(nunit, (1ull << 30) / per));
-    batch = std::min<uint64_t>(batch, 65535 / d.nsub);  // grid.y of the discard
-    DSPX_REQUIRE(batch >= 1, DSPX_E_UNSUPPORTED, "nsub=%u too large", d.nsub);
+    // units per batch: scratch that stays in the 256 MB last-level cache
+    // between the passes (128 MB measured fastest of 16..256 MB on 128
+    // channels x 2^20 samples in 2048-point windows; environment
+    // DSPX_FB_SCRATCH_MB overrides), at most 65535 (grid.y), spread evenly
+    // over the batches so that the last one is not mostly padding
+    uint64_t cap = 128ull << 20;
+    if (const char* e = std::getenv("DSPX_FB_SCRATCH_MB")) cap = std::max<uint64_t>(1, std::strtoull(e, nullptr, 10)) << 20;
+    const uint64_t maxb = std::max<uint64_t>(1, std::min<uint64_t>({nunit, cap / per, 65535}));
+    const uint64_t nbat = (nunit + maxb - 1) / maxb;
+    const uint64_t batch = (nunit + nbat - 1) / nbat;
+    const uint64_t nmb = (F->nkeep + 255) / 256;
+    DSPX_REQUIRE(nmb * d.nsub < (1ull << 31), DSPX_E_UNSUPPORTED, "nsub=%u too large", d.nsub);
     if (!F->have_plan || F->plan_batch != batch) {
       if (F->have_plan) {
         hipfftDestroy(F->fwd);
@@ -188,7 +200,7 @@ int dspx_fb_perform(dspx_fb* F, const float* d_in, uint64_t ics, uint64_t ips, u
       DSPX_HIP(hipGetLastError());
       DSPX_FB_FFT(hipfftExecC2C(F->bwd, reinterpret_cast<hipfftComplex*>(s), reinterpret_cast<hipfftComplex*>(s),
                                 HIPFFT_BACKWARD));
-      dim3 g2((F->nkeep + 255) / 256, unsigned(nb * d.nsub));
+      dim3 g2(unsigned(nmb * d.nsub), unsigned(nb));
       k_fb_discard<<<g2, 256, 0, st>>>(s, F->nfft, d.fres, d.nsub, d.npol, uint32_t(npart), u0, F->nkeep,
                                        d.nfilt_pos, d_out, ocs, ops);
       DSPX_HIP(hipGetLastError());
