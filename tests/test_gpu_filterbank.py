@@ -52,3 +52,22 @@ def test_filterbank_multichannel_table(ctx, dspx, oracle):
     print("filterbank multichannel", flush=True)
     got = _run(dspx, ctx, x, nsub, fres, pos, neg, resp)
     assert rel_rms(got, expect) < 1e-5
+
+
+def test_filterbank_several_batches(ctx, dspx, oracle, monkeypatch):
+    """Windows spread over several library batches, the last one shorter
+    (a 1 MB scratch holds 64 windows of 2048 points: 246 windows in batches
+    of 62, 62, 62 and 60), against the oracle."""
+    rng = np.random.default_rng(6)
+    nchan, npol, nsub, fres, pos, neg = 3, 2, 16, 128, 10, 12
+    nfft = nsub * fres
+    resp = (np.exp(1j * rng.uniform(0, 2 * np.pi, (nchan, nfft))) / np.sqrt(nfft * fres)).astype(np.complex64)
+    in_step = nsub * (fres - pos - neg)
+    ndat = 40 * in_step + nfft + 5
+    x = (rng.standard_normal((nchan, npol, ndat)) + 1j * rng.standard_normal((nchan, npol, ndat))).astype(np.complex64)
+    expect = oracle.filterbank(x, nsub, fres, pos, neg, resp)
+    monkeypatch.setenv("DSPX_FB_SCRATCH_MB", "1")
+    print("filterbank several batches", flush=True)
+    got = _run(dspx, ctx, x, nsub, fres, pos, neg, resp)
+    assert got.shape == expect.shape and expect.shape[2] == 41 * (fres - pos - neg)
+    assert rel_rms(got, expect) < 1e-5
