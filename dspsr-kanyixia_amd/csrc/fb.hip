@@ -25,17 +25,27 @@ namespace dspx {
   } while (0)
 
 // Batch unit u = (chan * npol + pol) * npart + part; scratch row b = u - u0.
+// grid (ceil(nfft/512), nb): two complex samples per thread, moved as 16
+// bytes where the window and the scratch row both allow it (uniform over
+// the workgroup), else as two 8-byte moves.
 __global__ void k_fb_windows(const float* __restrict__ in, uint64_t ics, uint64_t ips, uint32_t npol,
                              uint64_t nfft, uint64_t in_step, uint32_t npart, uint64_t u0,
                              float2* __restrict__ scratch)
 {
-  uint64_t k = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  const uint64_t k = (uint64_t(blockIdx.x) * blockDim.x + threadIdx.x) * 2;
   if (k >= nfft) return;
-  uint64_t b = blockIdx.y, u = u0 + b;
-  uint64_t p = u % npart, cp = u / npart;
-  uint64_t chan = cp / npol, pol = cp % npol;
-  const float* x = in + chan * ics + pol * ips + 2 * (p * in_step + k);
-  scratch[b * nfft + k] = *reinterpret_cast<const float2*>(x);
+  const uint64_t b = blockIdx.y, u = u0 + b;
+  const uint64_t p = u % npart, cp = u / npart;
+  const uint64_t chan = cp / npol, pol = cp % npol;
+  const float* x = in + chan * ics + pol * ips + 2 * p * in_step;
+  float2* y = scratch + b * nfft;
+  const uintptr_t al = reinterpret_cast<uintptr_t>(x) | reinterpret_cast<uintptr_t>(y);
+  if (k + 2 <= nfft && (al & 15) == 0) {
+    *reinterpret_cast<float4*>(y + k) = *reinterpret_cast<const float4*>(x + 2 * k);
+  } else {
+    y[k] = *reinterpret_cast<const float2*>(x + 2 * k);
+    if (k + 1 < nfft) y[k + 1] = *reinterpret_cast<const float2*>(x + 2 * k + 2);
+  }
 }
 
 // Response::operate (Response.C:653-726): d * f per bin, the response of the
@@ -52,15 +62,15 @@ __global__ void k_fb_multiply(float2* __restrict__ scratch, const float2* __rest
   scratch[b * nfft + k] = make_float2(f.x * d.x - f.y * d.y, f.y * d.x + f.x * d.y);
 }
 
-// grid (ceil(nkeep/256) * nsub, nb): x = (sub-band, block of 256 samples)
+// grid (ceil(nsub * nkeep / 256), nb): x runs over (sub-band, kept sample)
+// of one unit, so that short sub-bands still fill the workgroups
 __global__ void k_fb_discard(const float2* __restrict__ scratch, uint64_t nfft, uint32_t fres, uint32_t nsub,
                              uint32_t npol, uint32_t npart, uint64_t u0, uint32_t nkeep, uint32_t nfilt_pos,
                              float* __restrict__ out, uint64_t ocs, uint64_t ops)
 {
-  const uint32_t nmb = (nkeep + blockDim.x - 1) / blockDim.x;
-  const uint32_t isub = blockIdx.x / nmb;
-  const uint32_t m = (blockIdx.x % nmb) * blockDim.x + threadIdx.x;
-  if (m >= nkeep) return;
+  const uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= nsub * nkeep) return;
+  const uint32_t isub = idx / nkeep, m = idx % nkeep;
   const uint64_t b = blockIdx.y;
   uint64_t u = u0 + b;
   uint64_t p = u % npart, cp = u / npart;
@@ -168,8 +178,9 @@ int dspx_fb_perform(dspx_fb* F, const float* d_in, uint64_t ics, uint64_t ips, u
     const uint64_t maxb = std::max<uint64_t>(1, std::min<uint64_t>({nunit, cap / per, 65535}));
     const uint64_t nbat = (nunit + maxb - 1) / maxb;
     const uint64_t batch = (nunit + nbat - 1) / nbat;
-    const uint64_t nmb = (F->nkeep + 255) / 256;
-    DSPX_REQUIRE(nmb * d.nsub < (1ull << 31), DSPX_E_UNSUPPORTED, "nsub=%u too large", d.nsub);
+    const uint64_t nkb = (uint64_t(d.nsub) * F->nkeep + 255) / 256;
+    DSPX_REQUIRE(uint64_t(d.nsub) * F->nkeep < (1ull << 32) && nkb < (1ull << 31), DSPX_E_UNSUPPORTED,
+                 "nsub=%u too large", d.nsub);
     if (!F->have_plan || F->plan_batch != batch) {
       if (F->have_plan) {
         hipfftDestroy(F->fwd);
@@ -191,8 +202,9 @@ int dspx_fb_perform(dspx_fb* F, const float* d_in, uint64_t ics, uint64_t ips, u
       const uint64_t nb = std::min(batch, nunit - u0);
       if (nb != batch)  // the unused rows of the last batch stay finite
         DSPX_HIP(hipMemsetAsync(s + nb * F->nfft, 0, (batch - nb) * per, st));
+      dim3 g0(unsigned((F->nfft + 511) / 512), unsigned(nb));
+      k_fb_windows<<<g0, 256, 0, st>>>(d_in, ics, ips, d.npol, F->nfft, F->in_step, uint32_t(npart), u0, s);
       dim3 g1(unsigned((F->nfft + 255) / 256), unsigned(nb));
-      k_fb_windows<<<g1, 256, 0, st>>>(d_in, ics, ips, d.npol, F->nfft, F->in_step, uint32_t(npart), u0, s);
       DSPX_HIP(hipGetLastError());
       DSPX_FB_FFT(hipfftExecC2C(F->fwd, reinterpret_cast<hipfftComplex*>(s), reinterpret_cast<hipfftComplex*>(s),
                                 HIPFFT_FORWARD));
@@ -200,7 +212,7 @@ int dspx_fb_perform(dspx_fb* F, const float* d_in, uint64_t ics, uint64_t ips, u
       DSPX_HIP(hipGetLastError());
       DSPX_FB_FFT(hipfftExecC2C(F->bwd, reinterpret_cast<hipfftComplex*>(s), reinterpret_cast<hipfftComplex*>(s),
                                 HIPFFT_BACKWARD));
-      dim3 g2(unsigned(nmb * d.nsub), unsigned(nb));
+      dim3 g2(static_cast<unsigned>(nkb), static_cast<unsigned>(nb));
       k_fb_discard<<<g2, 256, 0, st>>>(s, F->nfft, d.fres, d.nsub, d.npol, uint32_t(npart), u0, F->nkeep,
                                        d.nfilt_pos, d_out, ocs, ops);
       DSPX_HIP(hipGetLastError());

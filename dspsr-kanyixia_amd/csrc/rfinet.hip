@@ -68,15 +68,26 @@ __global__ void k_rfi_stats(const float* __restrict__ sums, const uint32_t* __re
   vmax[i] = vm;
 }
 
-// per-channel mean of the log power over the blocks (sequential, one thread
-// per channel)
-__global__ void k_rfi_chan_mean(const float* __restrict__ lp, uint64_t nblk, uint32_t nchan, float* __restrict__ mean)
+// per-channel mean of the log power over the blocks: a workgroup per 32
+// channels, whose 8 row groups each add every 8th block (rows of 32
+// channels read together), then the 8 partials added in a fixed order
+__global__ __launch_bounds__(256) void k_rfi_chan_mean(const float* __restrict__ lp, uint64_t nblk, uint32_t nchan,
+                                                       float* __restrict__ mean)
 {
-  uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= nchan) return;
+  constexpr int CH = 32, RG = 8;
+  __shared__ double part[RG][CH];
+  const int cl = threadIdx.x % CH, rg = threadIdx.x / CH;
+  const uint32_t c = blockIdx.x * CH + cl;
   double s = 0;
-  for (uint64_t b = 0; b < nblk; b++) s += lp[b * nchan + c];
-  mean[c] = float(s / double(nblk));
+  if (c < nchan)
+    for (uint64_t b = rg; b < nblk; b += RG) s += lp[b * nchan + c];
+  part[rg][cl] = s;
+  __syncthreads();
+  if (rg == 0 && c < nchan) {
+    double t = 0;
+    for (int g = 0; g < RG; g++) t += part[g][cl];
+    mean[c] = float(t / double(nblk));
+  }
 }
 
 // feat[b][f][c]: f0 = log power - channel mean, f1 = mean SK deviation,
@@ -469,7 +480,7 @@ int dspx_rfi_features(dspx_ctx* ctx, const float* d_sums, const uint32_t* d_coun
     float* mean = vm + n;
     hipStream_t st = ctx->stream;
     k_rfi_stats<<<unsigned((n + 255) / 256), 256, 0, st>>>(d_sums, d_counts, n, npol, Nd, lp, v1, vm);
-    k_rfi_chan_mean<<<(nchan + 255) / 256, 256, 0, st>>>(lp, nblk, nchan, mean);
+    k_rfi_chan_mean<<<(nchan + 31) / 32, 256, 0, st>>>(lp, nblk, nchan, mean);
     k_rfi_features<<<unsigned((n + 255) / 256), 256, 0, st>>>(lp, v1, vm, mean, nblk, nchan, d_feat);
     DSPX_HIP(hipGetLastError());
   });
