@@ -9,6 +9,7 @@
 // layer, an FP32 master copy and the BF16-rounded copy the forward and
 // backward passes use (values widened to FP32, FP32 accumulation).
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 
@@ -151,6 +152,105 @@ __global__ __launch_bounds__(kTileX) void k_conv(const float* __restrict__ A, co
     if (MASK) v = z[uint64_t(co) * nchan] > 0.f ? v : 0.f;
     y[uint64_t(co) * nchan] = v;
   }
+}
+
+// ------------------------------------------------- 32 -> 32 layers on MFMA
+// The same convolution as an implicit GEMM on the matrix cores:
+// Y[co][x] = bias[co] + sum_k W'[co][k] * B[k][x], k = tap * 32 + ci,
+// B[k][x] = A[ci][x + tap - P], with 16x16x16 BF16 products accumulated in
+// FP32.  The weights are BF16 already (the network's weights are kept
+// rounded); each activation is split into BF16 hi = bf16(a) and lo =
+// bf16(a - hi), and both halves are multiplied, so the products carry
+// about 16 mantissa bits of the activation (relative error near 2^-17)
+// instead of 8.
+//
+// Lane layout of v_mfma_f32_16x16x16bf16_1k (lane l, r = l % 16, q = l / 16):
+// A operand A[r][4q + j], B operand B[4q + j][r], j < 4 (4 BF16 each);
+// result D[4q + j][r].  A workgroup of 4 waves computes 32 output channels
+// (two 16-row tiles) at 64 positions (one 16-column tile per wave); k runs
+// in 38 steps of 16 (tap s / 2, input channels 16 (s % 2) + 4q + j).
+constexpr int kMT = 64;              // positions per workgroup
+constexpr int kMXP = kMT + kK - 1;   // activation tile width
+constexpr int kMRS = kW + 4;         // LDS row stride (BF16 units): rows 72 bytes apart
+typedef short bf16x4_t __attribute__((ext_vector_type(4)));
+typedef float f32x4_t __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ uint16_t bf16_bits_rne(float f)
+{
+  uint32_t u = __float_as_uint(f);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return uint16_t(u >> 16);
+}
+
+// BF16 weights of a 32 -> 32 layer as [tap][co][ci] (exact: the upper half
+// of each BF16-rounded weight)
+__global__ void k_conv_w_bf16(const float* __restrict__ W, uint16_t* __restrict__ Wb)
+{
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= uint32_t(kK * kW * kW)) return;
+  const uint32_t ci = i % kW, co = (i / kW) % kW, tap = i / (kW * kW);
+  Wb[i] = uint16_t(__float_as_uint(W[(co * kW + ci) * kK + tap]) >> 16);
+}
+
+// grid (ceil(nchan / kMT), nblk), block 256
+template <bool RELU>
+__global__ __launch_bounds__(256) void k_conv_mfma(const float* __restrict__ A, const uint16_t* __restrict__ Wb,
+                                                   const float* __restrict__ bias, uint32_t nchan,
+                                                   float* __restrict__ Y)
+{
+  __shared__ __align__(16) uint16_t wl[kK * kW * kMRS];
+  __shared__ __align__(16) uint16_t ahi[kMXP * kMRS], alo[kMXP * kMRS];
+  const int tid = threadIdx.x;
+  const uint32_t x0 = blockIdx.x * kMT;
+  const uint64_t b = blockIdx.y;
+  // weights: rows of 32 BF16 (64 bytes) into rows of kMRS, 8 bytes a move
+  const uint2* wg = reinterpret_cast<const uint2*>(Wb);
+  for (int i = tid; i < kK * kW * kW / 4; i += 256) {
+    const int row = i / (kW / 4), part = i % (kW / 4);
+    *reinterpret_cast<uint2*>(&wl[row * kMRS + part * 4]) = wg[i];
+  }
+  // activations [ci][x0 - P + j] -> hi/lo at [j][ci]
+  const float* a = A + b * kW * nchan;
+  for (int i = tid; i < kMXP * kW; i += 256) {
+    const int ci = i / kMXP, j = i % kMXP;
+    const int64_t x = int64_t(x0) + j - kP;
+    const float v = (x >= 0 && x < int64_t(nchan)) ? a[uint64_t(ci) * nchan + uint64_t(x)] : 0.f;
+    const uint16_t h = bf16_bits_rne(v);
+    ahi[j * kMRS + ci] = h;
+    alo[j * kMRS + ci] = bf16_bits_rne(v - __uint_as_float(uint32_t(h) << 16));
+  }
+  __syncthreads();
+  const int lane = tid & 63, n = tid >> 6;
+  const int r = lane & 15, q = lane >> 4;
+  f32x4_t acc[2];
+#pragma unroll
+  for (int m = 0; m < 2; m++)
+#pragma unroll
+    for (int j = 0; j < 4; j++) acc[m][j] = bias[16 * m + 4 * q + j];
+#pragma unroll 2
+  for (int s = 0; s < 2 * kK; s++) {
+    const int tap = s >> 1, ci = 16 * (s & 1) + 4 * q;
+    const int jj = 16 * n + r + tap;
+    const bf16x4_t bh = *reinterpret_cast<const bf16x4_t*>(&ahi[jj * kMRS + ci]);
+    const bf16x4_t bl = *reinterpret_cast<const bf16x4_t*>(&alo[jj * kMRS + ci]);
+#pragma unroll
+    for (int m = 0; m < 2; m++) {
+      const bf16x4_t aw = *reinterpret_cast<const bf16x4_t*>(&wl[(tap * kW + 16 * m + r) * kMRS + ci]);
+      acc[m] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(aw, bh, acc[m], 0, 0, 0);
+      acc[m] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(aw, bl, acc[m], 0, 0, 0);
+    }
+  }
+  const uint32_t x = x0 + 16 * n + r;
+  if (x >= nchan) return;
+  float* y = Y + b * kW * nchan + x;
+#pragma unroll
+  for (int m = 0; m < 2; m++)
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      float v = acc[m][j];
+      if (RELU) v = fmaxf(v, 0.f);
+      y[uint64_t(16 * m + 4 * q + j) * nchan] = v;
+    }
 }
 
 // Weights of the layer transposed and flipped for the data gradient:
@@ -328,6 +428,7 @@ using namespace dspx;
 struct dspx_rfinet {
   dspx_ctx* ctx = nullptr;
   DevBuf w, wq, m, v, grad, wt;  // wt: transposed weights of one layer
+  DevBuf wbf;                    // BF16 [tap][co][ci] weights of the two 32 -> 32 layers
   DevBuf act;                    // A1..A3, logits of the last forward
   DevBuf gbuf[2];                // gradient ping-pong
   DevBuf part, lpart, loss;
@@ -369,8 +470,27 @@ void forward(dspx_rfinet* N, const float* feat, uint64_t nblk, uint32_t nchan, f
   float* a3 = act_of(N, 2, npos);
   float* z = logits_out ? logits_out : act_of(N, 3, npos);
   launch_conv<kF, kW, true, false>(st, feat, wq + off_of(0), nblk, nchan, a1);
-  launch_conv<kW, kW, true, false>(st, a1, wq + off_of(1), nblk, nchan, a2);
-  launch_conv<kW, kW, true, false>(st, a2, wq + off_of(2), nblk, nchan, a3);
+  static const bool valu = [] {
+    const char* e = std::getenv("DSPX_RFINET_VALU");
+    return e && std::atoi(e) != 0;
+  }();
+  if (valu) {
+    launch_conv<kW, kW, true, false>(st, a1, wq + off_of(1), nblk, nchan, a2);
+    launch_conv<kW, kW, true, false>(st, a2, wq + off_of(2), nblk, nchan, a3);
+  } else {
+    // the two 32 -> 32 layers on the matrix cores (DSPX_RFINET_VALU=1: on the VALU)
+    uint16_t* wb = N->wbf.as<uint16_t>();
+    constexpr int nw = kK * kW * kW;
+    for (int l = 1; l <= 2; l++) {
+      k_conv_w_bf16<<<(nw + 255) / 256, 256, 0, st>>>(wq + off_of(l), wb + (l - 1) * nw);
+      DSPX_HIP(hipGetLastError());
+    }
+    DSPX_REQUIRE(nblk <= 65535, DSPX_E_UNSUPPORTED, "nblk=%llu", (unsigned long long) nblk);
+    dim3 grid((nchan + kMT - 1) / kMT, unsigned(nblk));
+    k_conv_mfma<true><<<grid, 256, 0, st>>>(a1, wb, wq + off_of(1) + kW * kW * kK, nchan, a2);
+    k_conv_mfma<true><<<grid, 256, 0, st>>>(a2, wb + nw, wq + off_of(2) + kW * kW * kK, nchan, a3);
+    DSPX_HIP(hipGetLastError());
+  }
   launch_conv<kW, 1, false, false>(st, a3, wq + off_of(3), nblk, nchan, z);
 }
 
@@ -413,6 +533,7 @@ int dspx_rfinet_create(dspx_ctx* ctx, uint64_t seed, dspx_rfinet** out)
     const size_t bytes = size_t(kNParam) * sizeof(float);
     for (DevBuf* b : {&N->w, &N->wq, &N->m, &N->v, &N->grad}) b->reserve(bytes);
     N->wt.reserve(size_t(kW * kW * kK + kW) * sizeof(float));
+    N->wbf.reserve(size_t(2 * kK * kW * kW) * sizeof(uint16_t));
     // He-uniform weights (bound sqrt(6 / fan_in)), zero biases
     std::vector<float> h(kNParam, 0.f);
     uint64_t s = seed;
