@@ -167,9 +167,9 @@ __global__ __launch_bounds__(kTileX) void k_conv(const float* __restrict__ A, co
 // Lane layout of v_mfma_f32_16x16x16bf16_1k (lane l, r = l % 16, q = l / 16):
 // A operand A[r][4q + j], B operand B[4q + j][r], j < 4 (4 BF16 each);
 // result D[4q + j][r].  A workgroup of 4 waves computes 32 output channels
-// (two 16-row tiles) at 64 positions (one 16-column tile per wave); k runs
+// (two 16-row tiles) at kMT positions (kMT / 64 16-column tiles per wave); k runs
 // in 38 steps of 16 (tap s / 2, input channels 16 (s % 2) + 4q + j).
-constexpr int kMT = 64;              // positions per workgroup
+constexpr int kMT = 128;             // positions per workgroup (a multiple of 64)
 constexpr int kMXP = kMT + kK - 1;   // activation tile width
 constexpr int kMRS = kW + 4;         // LDS row stride (BF16 units): rows 72 bytes apart
 typedef short bf16x4_t __attribute__((ext_vector_type(4)));
@@ -192,23 +192,19 @@ __global__ void k_conv_w_bf16(const float* __restrict__ W, uint16_t* __restrict_
   Wb[i] = uint16_t(__float_as_uint(W[(co * kW + ci) * kK + tap]) >> 16);
 }
 
-// grid (ceil(nchan / kMT), nblk), block 256
+// grid (ceil(nchan / kMT), nblk), block 256.  The weight fragments are read
+// from global memory (every workgroup reads the same 39 KB, which stay in
+// the caches); each wave takes NT 16-position column tiles.
 template <bool RELU>
 __global__ __launch_bounds__(256) void k_conv_mfma(const float* __restrict__ A, const uint16_t* __restrict__ Wb,
                                                    const float* __restrict__ bias, uint32_t nchan,
                                                    float* __restrict__ Y)
 {
-  __shared__ __align__(16) uint16_t wl[kK * kW * kMRS];
+  constexpr int NT = kMT / 64;   // column tiles per wave
   __shared__ __align__(16) uint16_t ahi[kMXP * kMRS], alo[kMXP * kMRS];
   const int tid = threadIdx.x;
   const uint32_t x0 = blockIdx.x * kMT;
   const uint64_t b = blockIdx.y;
-  // weights: rows of 32 BF16 (64 bytes) into rows of kMRS, 8 bytes a move
-  const uint2* wg = reinterpret_cast<const uint2*>(Wb);
-  for (int i = tid; i < kK * kW * kW / 4; i += 256) {
-    const int row = i / (kW / 4), part = i % (kW / 4);
-    *reinterpret_cast<uint2*>(&wl[row * kMRS + part * 4]) = wg[i];
-  }
   // activations [ci][x0 - P + j] -> hi/lo at [j][ci]
   const float* a = A + b * kW * nchan;
   for (int i = tid; i < kMXP * kW; i += 256) {
@@ -220,37 +216,48 @@ __global__ __launch_bounds__(256) void k_conv_mfma(const float* __restrict__ A, 
     alo[j * kMRS + ci] = bf16_bits_rne(v - __uint_as_float(uint32_t(h) << 16));
   }
   __syncthreads();
-  const int lane = tid & 63, n = tid >> 6;
+  const int lane = tid & 63, wv = tid >> 6;
   const int r = lane & 15, q = lane >> 4;
-  f32x4_t acc[2];
+  f32x4_t acc[2][NT];
 #pragma unroll
   for (int m = 0; m < 2; m++)
 #pragma unroll
-    for (int j = 0; j < 4; j++) acc[m][j] = bias[16 * m + 4 * q + j];
+    for (int t = 0; t < NT; t++)
+#pragma unroll
+      for (int j = 0; j < 4; j++) acc[m][t][j] = bias[16 * m + 4 * q + j];
+  const bf16x4_t* wg = reinterpret_cast<const bf16x4_t*>(Wb);
 #pragma unroll 2
   for (int s = 0; s < 2 * kK; s++) {
     const int tap = s >> 1, ci = 16 * (s & 1) + 4 * q;
-    const int jj = 16 * n + r + tap;
-    const bf16x4_t bh = *reinterpret_cast<const bf16x4_t*>(&ahi[jj * kMRS + ci]);
-    const bf16x4_t bl = *reinterpret_cast<const bf16x4_t*>(&alo[jj * kMRS + ci]);
+    bf16x4_t aw[2];
 #pragma unroll
-    for (int m = 0; m < 2; m++) {
-      const bf16x4_t aw = *reinterpret_cast<const bf16x4_t*>(&wl[(tap * kW + 16 * m + r) * kMRS + ci]);
-      acc[m] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(aw, bh, acc[m], 0, 0, 0);
-      acc[m] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(aw, bl, acc[m], 0, 0, 0);
+    for (int m = 0; m < 2; m++) aw[m] = wg[((tap * kW + 16 * m + r) * kW + ci) / 4];
+#pragma unroll
+    for (int t = 0; t < NT; t++) {
+      const int jj = 16 * (wv * NT + t) + r + tap;
+      const bf16x4_t bh = *reinterpret_cast<const bf16x4_t*>(&ahi[jj * kMRS + ci]);
+      const bf16x4_t bl = *reinterpret_cast<const bf16x4_t*>(&alo[jj * kMRS + ci]);
+#pragma unroll
+      for (int m = 0; m < 2; m++) {
+        acc[m][t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(aw[m], bh, acc[m][t], 0, 0, 0);
+        acc[m][t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(aw[m], bl, acc[m][t], 0, 0, 0);
+      }
     }
   }
-  const uint32_t x = x0 + 16 * n + r;
-  if (x >= nchan) return;
-  float* y = Y + b * kW * nchan + x;
 #pragma unroll
-  for (int m = 0; m < 2; m++)
+  for (int t = 0; t < NT; t++) {
+    const uint32_t x = x0 + 16 * (wv * NT + t) + r;
+    if (x >= nchan) continue;
+    float* y = Y + b * kW * nchan + x;
 #pragma unroll
-    for (int j = 0; j < 4; j++) {
-      float v = acc[m][j];
-      if (RELU) v = fmaxf(v, 0.f);
-      y[uint64_t(16 * m + 4 * q + j) * nchan] = v;
-    }
+    for (int m = 0; m < 2; m++)
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        float v = acc[m][t][j];
+        if (RELU) v = fmaxf(v, 0.f);
+        y[uint64_t(16 * m + 4 * q + j) * nchan] = v;
+      }
+  }
 }
 
 // Weights of the layer transposed and flipped for the data gradient:
