@@ -49,10 +49,12 @@ def test_sk_compute_fpt(dspx, ctx, oracle, tables, zeroed):
     assert u.max() <= 1 and (u > 0).mean() < 0.01
 
 
-@pytest.mark.parametrize("nbit", [8, 16])
-def test_sk_compute_raw(dspx, ctx, oracle, tables, nbit):
+@pytest.mark.parametrize("nbit,nchan", [(8, 37), (16, 37), (8, 36)])
+def test_sk_compute_raw(dspx, ctx, oracle, tables, nbit, nchan):
+    """(8-bit samples of two pols with a multiple of 4 channels take the
+    kernel that loads four channels' words at once.)"""
     import torch
-    M, nchan, npol, nblk, t0 = 32, 37, 2, 20, 13
+    M, npol, nblk, t0 = 32, 2, 20, 13
     ndat = t0 + nblk * M + 5
     rng = np.random.default_rng(nbit)
     if nbit == 8:
