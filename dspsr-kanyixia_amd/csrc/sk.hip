@@ -366,16 +366,30 @@ __global__ __launch_bounds__(kSkThreads) void k_sk_tscr(const float* __restrict_
     for (uint64_t p = tid; p < npart; p += kSkThreads) mask[p * nchan + ch] = 1;
 }
 
+// grid (ceil(n / 512), nchan * npol): two complex samples per thread, one
+// 16-byte move where both rows allow it and M is even (the pair then lies
+// in one block), else two 8-byte moves
 __global__ void k_sk_mask(const uint8_t* __restrict__ mask, const float* __restrict__ in, uint64_t ics, uint64_t ips,
                           float* __restrict__ out, uint64_t ocs, uint64_t ops, uint32_t nchan, uint32_t npol,
                           uint32_t M, uint64_t n)
 {
-  uint64_t t = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  const uint64_t t = (uint64_t(blockIdx.x) * blockDim.x + threadIdx.x) * 2;
   if (t >= n) return;
-  uint32_t chan = blockIdx.y / npol, pol = blockIdx.y % npol;
-  float2 v = reinterpret_cast<const float2*>(in + chan * ics + pol * ips)[t];
-  if (mask[(t / M) * nchan + chan]) v = make_float2(0.f, 0.f);
-  reinterpret_cast<float2*>(out + chan * ocs + pol * ops)[t] = v;
+  const uint32_t chan = blockIdx.y / npol, pol = blockIdx.y % npol;
+  const float2* x = reinterpret_cast<const float2*>(in + chan * ics + pol * ips);
+  float2* y = reinterpret_cast<float2*>(out + chan * ocs + pol * ops);
+  const uintptr_t al = reinterpret_cast<uintptr_t>(x) | reinterpret_cast<uintptr_t>(y);
+  if (t + 2 <= n && (al & 15) == 0 && (M & 1) == 0) {
+    float4 v = *reinterpret_cast<const float4*>(x + t);
+    if (mask[(t / M) * nchan + chan]) v = make_float4(0.f, 0.f, 0.f, 0.f);
+    *reinterpret_cast<float4*>(y + t) = v;
+    return;
+  }
+  for (uint64_t k = t; k < t + 2 && k < n; k++) {
+    float2 v = x[k];
+    if (mask[(k / M) * nchan + chan]) v = make_float2(0.f, 0.f);
+    y[k] = v;
+  }
 }
 
 }  // namespace
@@ -509,8 +523,8 @@ int dspx_sk_mask(dspx_sk* s, const uint8_t* d_mask, const float* d_in, uint64_t 
     DSPX_REQUIRE(s && d_mask && d_in && d_out, DSPX_E_PARAM, "null argument");
     const uint64_t n = npart * s->d.M;
     if (!n) return;
-    DSPX_REQUIRE((n + 255) / 256 < (1ull << 31), DSPX_E_UNSUPPORTED, "block too long");
-    dim3 grid(unsigned((n + 255) / 256), s->d.nchan * s->d.npol);
+    DSPX_REQUIRE((n + 511) / 512 < (1ull << 31), DSPX_E_UNSUPPORTED, "block too long");
+    dim3 grid(unsigned((n + 511) / 512), s->d.nchan * s->d.npol);
     DSPX_REQUIRE(grid.y <= 65535, DSPX_E_UNSUPPORTED, "nchan*npol=%u", grid.y);
     k_sk_mask<<<grid, 256, 0, s->ctx->stream>>>(d_mask, d_in, ics, ips, d_out, ocs, ops, s->d.nchan, s->d.npol,
                                                 s->d.M, n);
