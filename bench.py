@@ -488,7 +488,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this path has no multi-GPU form here "
-              "(channels and beams shard without exchange; see DESIGN.md)", file=sys.stderr)
+              "(channels and beams shard without exchange)", file=sys.stderr)
         sys.exit(2)
 
     if args.copy_probe:

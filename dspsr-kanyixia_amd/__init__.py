@@ -188,7 +188,7 @@ def lib():
     if _lib is not None:
         return _lib
     if not os.path.exists(LIB_PATH):
-        raise RuntimeError("libdspx.so not built (run __graft_entry__.build() or make -C %s)" % _HERE)
+        raise RuntimeError("libdspx.so not built (run make -C %s)" % _HERE)
     L = C.CDLL(LIB_PATH)
     for name, (res, args) in _SIGS.items():
         fn = getattr(L, name)

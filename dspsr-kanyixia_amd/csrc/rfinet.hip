@@ -2,7 +2,7 @@
 // kurtosis sums, and a 4-layer 1-D convolutional classifier across frequency
 // channels (forward, backward, loss and Adam) trained on the SK detector's
 // masks.  No counterpart in the reference: it proposes blocks ahead of the
-// SK detector (SpectralKurtosis.C:830-909), which stays the judge.
+// SK detector (SpectralKurtosis.C:830-909), which keeps the final decision.
 //
 // Layout: activations [nblk][C][nchan] float (the channel axis is the
 // convolution's axis, contiguous); weights W[co][ci][k] then bias[co] per

@@ -1,4 +1,4 @@
-// dspx_pipeline -- a LoadToFold-style driver over the dsp::HIP engines.
+// dspx_pipeline -- a LoadToFold-style command over the dsp::HIP engines.
 //
 // Reads one block of raw TFP samples from a file (--raw with the shape given,
 // or --dada with the shape, band and rate from the DADA header), runs

@@ -4,7 +4,7 @@ dspsr's SpectralKurtosis transformation evaluates its thresholds on the host
 (SpectralKurtosis::get_limits, SpectralKurtosis.C:989-1009) and hands them to
 its engine; the HIP engine here takes them as tables indexed by the block
 count M' (dspx_sk_set_limits).  This module computes those tables for callers
-that are not dspsr itself (the tests, bench.py, the pipeline driver).
+that are not dspsr itself (the tests, bench.py, the dspx_pipeline command).
 
 It restates SKLimits::calc_limits (Signal/Statistics/SKLimits.C:136-229): the
 SK estimator's distribution is approximated by a Pearson type IV density

@@ -435,8 +435,8 @@ int dspx_batch_replay(dspx_batch* batch);
 
 /* ------------------------------------------------------- RFI proposal net */
 /* No reference counterpart: a learned proposal of RFI-affected blocks ahead
-   of the spectral kurtosis detector (which stays the judge and the
-   fallback).  Features per (block, channel) from the SK sums; a 4-layer 1-D
+   of the spectral kurtosis detector (which keeps the final decision and
+   stays the fallback).  Features per (block, channel) from the SK sums; a 4-layer 1-D
    convolutional classifier across frequency channels (4 -> 32 -> 32 -> 32
    -> 1, 19 taps, ReLU), BF16-rounded weights with FP32 accumulation and an
    FP32 master copy; class-weighted logistic loss; Adam.  Features and

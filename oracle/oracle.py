@@ -1,9 +1,9 @@
 """ctypes wrapper of the CPU oracle (oracle.c) and of the reference's own C
 files compiled into oracle/_ref/libref.so.
 
-TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
-cpu_baseline leg of bench.py, as the checker.  The product library never
-imports this module.
+TEST INFRASTRUCTURE ONLY: imported by tests/, the smoke test of the build
+entry point and the CPU-baseline leg of bench.py, as the reference results
+are checked against.  The product library never imports this module.
 """
 import ctypes as C
 import os

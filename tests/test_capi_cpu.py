@@ -57,7 +57,7 @@ def test_binplan_host_matches_serial_recurrence(dspx, oracle, seed):
     assert np.array_equal(dspx.binplan_host(phi, pps, nbin, n), plan)
 
 
-def test_binplan_host_baseline_phase_rate(dspx, oracle):
+def test_binplan_host_single_channel_phase_rate(dspx, oracle):
     """The single-channel workload's rate: 62.5 ns samples folded at 89.33 ms."""
     pps = 62.5e-9 / 0.08933
     for phi in (0.0, 0.25, 0.9999999999):

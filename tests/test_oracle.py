@@ -62,7 +62,7 @@ def test_smearing_closed_form(oracle, cfreq, bw, nchan, dm):
     assert n & (n - 1) == 0 and n > g["noverlap"]
 
 
-def test_geometry_baseline_configs(oracle):
+def test_geometry_measured_workloads(oracle):
     # the values of the single-channel and 128-channel workloads
     g1 = oracle.geometry(1405.0, 16.0, 1, 56.77)
     assert (g1["impulse_neg"], g1["impulse_pos"], g1["nfft"], g1["step"]) == (24331, 23514, 524288, 476443)
