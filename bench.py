@@ -103,8 +103,8 @@ def time_path(dspx, ctx, w, geom, raw, form, steps, warmup, timing_steps=3):
 
 def cpu_baseline(w, geom, raw_np, budget_s=20.0, threads=16):
     """The oracle chain (unpack, FP32 overlap-save convolution with its own
-    FFT, detection, fold) on the host cores, up to ``threads`` threads (the
-    GPU box's CPU share is 16), as the reference's MultiThread pipelines
+    FFT, detection, fold) on the host cores, up to ``threads`` threads (16
+    by default: a host CPU share per GPU), as the reference's MultiThread pipelines
     share a block's work.  The threads run the oracle's C functions, which
     release the interpreter lock.  With at least as many channels as
     threads, each thread takes whole channels, in rounds until the time
