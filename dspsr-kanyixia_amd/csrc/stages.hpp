@@ -694,6 +694,22 @@ __global__ __launch_bounds__(kColThreads) void k_stageC(ConvShape s, FilterbankA
         uint32_t kq[QN];
         const char* krow = reinterpret_cast<const char*>(kp + c0 + uint64_t(r * (N1 / RF)) * s.N2);
         load_keys<QN>(reinterpret_cast<const uint32_t*>(krow + kvo), kq);
+        // Samples after a bin edge inside this lane's samples: accumulated
+        // per run of equal keys and added once per run (bins a few samples
+        // wide, short periods with many bins, put an edge in most lanes)
+        uint32_t run_key = 0xffffffffu;
+        float racc[NV];
+#pragma unroll
+        for (int i = 0; i < NV; i++) racc[i] = 0.f;
+        auto flush_run = [&]() {
+          if (run_key != 0xffffffffu) {
+            float d[NV];
+            detect_finish<NV>(fa.state == DSPX_STOKES, racc, d);
+            float* hs = hist + (run_key >> 16) * (NV * fa.nbin) + (run_key & 0xffffu) * fa.ndim;
+#pragma unroll
+            for (int i = 0; i < NV; i++) atomicAdd(hs + fa.voff[i], d[i]);
+          }
+        };
 #pragma unroll
         for (int q = 0; q < QN; q++) {
           const uint32_t key = kq[q];
@@ -701,14 +717,16 @@ __global__ __launch_bounds__(kColThreads) void k_stageC(ConvShape s, FilterbankA
           if (key == lane_key) {
             detect_acc<NV>(s.npol, va[q * RF + r], vb[q * RF + r], cacc);
           } else if (key != 0xffffffffu) {
-            // rare: a bin edge inside this lane's samples
-            float d[NV];
-            detect_nv<NV>(fa.state == DSPX_STOKES, s.npol, va[q * RF + r], vb[q * RF + r], d);
-            float* hs = hist + (key >> 16) * (NV * fa.nbin) + (key & 0xffffu) * fa.ndim;
+            if (key != run_key) {
+              flush_run();
+              run_key = key;
 #pragma unroll
-            for (int i = 0; i < NV; i++) atomicAdd(hs + fa.voff[i], d[i]);
+              for (int i = 0; i < NV; i++) racc[i] = 0.f;
+            }
+            detect_acc<NV>(s.npol, va[q * RF + r], vb[q * RF + r], racc);
           }
         }
+        flush_run();
         float acc[NV];
         detect_finish<NV>(fa.state == DSPX_STOKES, cacc, acc);
         // is the whole 16-lane row in one bin?
