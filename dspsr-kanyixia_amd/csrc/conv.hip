@@ -440,11 +440,14 @@ __global__ void k_fold_reduce_p(const float* __restrict__ part, const uint32_t* 
 // p*step + t - nfilt_pos, or ~0 where the sample is discarded (the overlap)
 // or not folded.  Indexed like the work array, so a thread of stage C reads
 // its consecutive columns' keys with one vector load.  The same pass counts
-// the plan's hits (Fold.C:827-836): each thread takes kKeyRun consecutive
-// outputs, adds runs of equal keys into an LDS histogram of the part's slots,
-// and the workgroup adds its non-zero bins into hits[isub][ibin].  Within a
-// segment the bin never decreases, so a thread whose outputs lie in one
-// segment and whose first and last keys agree evaluates two samples, not 16.
+// the plan's hits (Fold.C:827-836): each thread takes kKeyRun/4 groups of 4
+// consecutive outputs, group q of thread t at offset 4*(q*256 + t) of the
+// workgroup's chunk, so that each vector store of a wave writes 1 KiB in one
+// piece.  Runs of equal keys are added into an LDS histogram of the part's
+// slots, once per wave where the wave's groups share one key, and the
+// workgroup adds its non-zero bins into hits[isub][ibin].  Within a segment
+// the bin never decreases, so a group whose outputs lie in one segment and
+// whose first and last keys agree evaluates two samples, not 4.
 constexpr int kKeyRun = 16;
 constexpr int kKeyChunk = 256 * kKeyRun;  // outputs per workgroup, inside one part
 __global__ __launch_bounds__(256) void k_fused_keys(const FoldSegment* __restrict__ seg, uint32_t nseg, double nbin_d,
@@ -486,8 +489,6 @@ __global__ __launch_bounds__(256) void k_fused_keys(const FoldSegment* __restric
   __syncthreads();
   const uint32_t ns = min(kSegCache, nseg - si0);
   const uint64_t jbeg = sseg[0].j0, jend = s_jend;
-  const uint64_t i0 = c0 + uint64_t(threadIdx.x) * kKeyRun;
-  uint32_t kv[kKeyRun];
   // the thread's current segment, held in registers: its outputs are
   // consecutive samples and nearly always lie in one segment
   FoldSegment sg{};
@@ -515,55 +516,72 @@ __global__ __launch_bounds__(256) void k_fused_keys(const FoldSegment* __restric
   auto slot_key = [&](uint32_t pl) {
     return pl == 0xffffffffu ? 0xffffffffu : (((pl >> 16) - s0) << 16) | (pl & 0xffffu);
   };
-  // Common case: every output of the thread kept and in one segment, where
-  // the bin is non-decreasing, so equal keys at both ends mean one key.
-  bool uniform = false;
-  const uint32_t tt0 = uint32_t(i0 - uint64_t(p) * nfft);
-  if (i0 + kKeyRun <= n && tt0 >= nfilt_pos && tt0 - nfilt_pos + (kKeyRun - 1) < step) {
-    const uint64_t ia = uint64_t(p) * step + (tt0 - nfilt_pos), ib = ia + (kKeyRun - 1);
-    locate(ia);
-    if (ib < jnext) {
-      const uint32_t ka = seg_key(sg, ia, nbin_d);
-      if (ka == seg_key(sg, ib, nbin_d)) {
-        const uint32_t key = slot_key(ka);
+  constexpr uint32_t kNone = 0xffffffffu;
+  auto add_hits = [&](uint32_t key, uint32_t cnt) {
+    if (key != kNone && (key >> 16) < nslot) atomicAdd(&h[(key >> 16) * nbin + (key & 0xffffu)], cnt);
+  };
+  const uint32_t lane = threadIdx.x & 63;
 #pragma unroll
-        for (int k = 0; k < kKeyRun; k++) kv[k] = key;
-        if (key != 0xffffffffu && (key >> 16) < nslot)
-          atomicAdd(&h[(key >> 16) * nbin + (key & 0xffffu)], uint32_t(kKeyRun));
-        uniform = true;
+  for (int q = 0; q < kKeyRun / 4; q++) {
+    const uint64_t i0 = c0 + 4 * (uint64_t(q) * 256 + threadIdx.x);
+    uint32_t kv[4];
+    // Common case: the group's 4 outputs kept and in one segment, where the
+    // bin is non-decreasing, so equal keys at both ends mean one key.
+    uint32_t ukey = kNone;  // the group's one key, else kNone
+    bool uniform = false;
+    const uint32_t tt0 = uint32_t(i0 - uint64_t(p) * nfft);
+    if (i0 + 4 <= n && tt0 >= nfilt_pos && tt0 - nfilt_pos + 3 < step) {
+      const uint64_t ia = uint64_t(p) * step + (tt0 - nfilt_pos), ib = ia + 3;
+      locate(ia);
+      if (ib < jnext) {
+        const uint32_t ka = seg_key(sg, ia, nbin_d);
+        if (ka == seg_key(sg, ib, nbin_d)) {
+          ukey = slot_key(ka);
+          uniform = true;
+        }
       }
     }
-  }
-  if (!uniform) {
-    uint32_t cur = 0xffffffffu, cnt = 0;
+    if (uniform) {
 #pragma unroll
-    for (int k = 0; k < kKeyRun; k++) {
-      const uint64_t i = i0 + k;
-      const uint32_t t = uint32_t(i - uint64_t(p) * nfft);
-      uint32_t key = 0xffffffffu;
-      if (i < n && t >= nfilt_pos && t - nfilt_pos < step) {
-        const uint64_t idat = uint64_t(p) * step + (t - nfilt_pos);
-        locate(idat);
-        key = slot_key(seg_key(sg, idat, nbin_d));
+      for (int k = 0; k < 4; k++) kv[k] = ukey;
+    } else {
+      uint32_t cur = kNone, cnt = 0;
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        const uint64_t i = i0 + k;
+        const uint32_t t = uint32_t(i - uint64_t(p) * nfft);
+        uint32_t key = kNone;
+        if (i < n && t >= nfilt_pos && t - nfilt_pos < step) {
+          const uint64_t idat = uint64_t(p) * step + (t - nfilt_pos);
+          locate(idat);
+          key = slot_key(seg_key(sg, idat, nbin_d));
+        }
+        kv[k] = key;
+        if (key != cur) {
+          add_hits(cur, cnt);
+          cur = key;
+          cnt = 0;
+        }
+        cnt++;
       }
-      kv[k] = key;
-      if (key != cur) {
-        if (cur != 0xffffffffu && (cur >> 16) < nslot) atomicAdd(&h[(cur >> 16) * nbin + (cur & 0xffffu)], cnt);
-        cur = key;
-        cnt = 0;
-      }
-      cnt++;
+      add_hits(cur, cnt);
     }
-    if (cur != 0xffffffffu && (cur >> 16) < nslot) atomicAdd(&h[(cur >> 16) * nbin + (cur & 0xffffu)], cnt);
-  }
-  if (i0 + kKeyRun <= n) {
+    // the uniform groups' hits: the lanes whose key is lane 0's add it once
+    const uint32_t lead = __builtin_amdgcn_readfirstlane(ukey);
+    const bool same = uniform && ukey == lead;
+    const uint64_t m = __ballot(same);
+    if (same) {
+      if (lane == uint32_t(__ffsll((long long) m) - 1)) add_hits(lead, 4u * uint32_t(__popcll(m)));
+    } else if (uniform) {
+      add_hits(ukey, 4u);
+    }
+    if (i0 + 4 <= n) {
+      *reinterpret_cast<uint4*>(keys + i0) = make_uint4(kv[0], kv[1], kv[2], kv[3]);
+    } else {
 #pragma unroll
-    for (int q = 0; q < kKeyRun / 4; q++)
-      *reinterpret_cast<uint4*>(keys + i0 + 4 * q) = make_uint4(kv[4 * q], kv[4 * q + 1], kv[4 * q + 2], kv[4 * q + 3]);
-  } else {
-#pragma unroll
-    for (int k = 0; k < kKeyRun; k++)
-      if (i0 + k < n) keys[i0 + k] = kv[k];
+      for (int k = 0; k < 4; k++)
+        if (i0 + k < n) keys[i0 + k] = kv[k];
+    }
   }
   __syncthreads();
   for (uint32_t k = threadIdx.x; k < hn; k += blockDim.x)
