@@ -895,13 +895,18 @@ void fused_fold_setup(dspx_ltf* L)
   // Column groups per (channel, part) in stage C.  Every stage C workgroup
   // zeroes, writes and has reduced one histogram: with large histograms the
   // grid is one round of the workgroups the device holds at once (LDS-bound),
-  // with small ones four rounds, for balance.
+  // with small ones four rounds, for balance.  The count is then lowered to
+  // give every group the same number of column tiles (128 tiles over 118
+  // groups would leave most groups one tile and a few two, each group with
+  // its own histogram).
   const uint32_t ncb = L->core.ncb();
   const uint64_t units = uint64_t(L->d.nchan) * np;
   const uint64_t per_cu = std::max<uint64_t>(1, (160 * 1024) / (kColTile * sizeof(float2) + hist_bytes));
   const uint64_t slots = uint64_t(std::max(ctx->ncu, 1)) * per_cu;
   const uint64_t target = hist_bytes > 16384 ? slots : 4 * slots;
-  L->G = uint32_t(std::min<uint64_t>(ncb, std::max<uint64_t>(1, (target + units - 1) / units)));
+  const uint64_t g0 = std::min<uint64_t>(ncb, std::max<uint64_t>(1, (target + units - 1) / units));
+  const uint64_t tiles_per_group = (ncb + g0 - 1) / g0;
+  L->G = uint32_t((ncb + tiles_per_group - 1) / tiles_per_group);
   L->sc().partial.reserve(units * L->G * hist_bytes);
 
   // key plan and hits
