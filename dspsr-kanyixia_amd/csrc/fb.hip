@@ -6,15 +6,18 @@
 // the window at ipart*in_step, the response product, the fres-point backward
 // transform of each sub-band's contiguous slice, and the kept samples
 // [nfilt_pos, nfilt_pos + nkeep) written to output channel ichan*nsub + isub.
-// The windows are batched into library transforms; the response product and
-// the overlap discard are kernels of their own.
+// The common window lengths run as one fused kernel per window; the others
+// are batched into library transforms, with the response product and the
+// overlap discard as kernels of their own.
 #include <cstdlib>
 
 #include <hipfft/hipfft.h>
 
 #include <algorithm>
+#include <string>
 
 #include "dspx_internal.hpp"
+#include "fft_core.hpp"
 
 namespace dspx {
 
@@ -77,6 +80,70 @@ __global__ void k_fb_discard(const float2* __restrict__ scratch, uint64_t nfft, 
   uint64_t chan = cp / npol, pol = cp % npol;
   float* o = out + (chan * nsub + isub) * ocs + pol * ops + 2 * (p * nkeep + m);
   *reinterpret_cast<float2*>(o) = scratch[b * nfft + uint64_t(isub) * fres + nfilt_pos + m];
+}
+
+// The whole filterbank of one unit in one workgroup of NT = NFFT/16 threads,
+// 16 complex values per thread: the window loaded in the forward plan's
+// first-pass order, the NFFT-point forward transform in registers and LDS,
+// the response product on the way into the batched-column layout (bin k =
+// s*FRES + m at column s, row m), NSUB = NFFT/FRES inverse FRES-point
+// transforms of those columns, and the kept rows [nfilt_pos, nfilt_pos +
+// nkeep) of every column written with consecutive lanes on consecutive
+// output samples.  Same transforms and product as the library form above,
+// without its four HBM round trips of the window.
+template <int NFFT, int FRES>
+__global__ __launch_bounds__(NFFT / 16) void k_fb_fused(const float* __restrict__ in, uint64_t ics, uint64_t ips,
+                                                        uint32_t npol, uint64_t in_step, uint32_t npart,
+                                                        const float2* __restrict__ resp,
+                                                        const float2* __restrict__ tw_f,
+                                                        const float2* __restrict__ tw_b, uint32_t nkeep,
+                                                        uint32_t nfilt_pos, float* __restrict__ out, uint64_t ocs,
+                                                        uint64_t ops)
+{
+  constexpr int NT = NFFT / 16, E = 16, NSUB = NFFT / FRES, RS = NSUB + 1;
+  using ROW = RowLayout<NFFT, NT>;
+  using COL = ColLayout<FRES, NSUB, RS, NT>;
+  constexpr int LROW = NFFT + NFFT / 16, LCOL = FRES * RS;
+  __shared__ float2 lds[LROW > LCOL ? LROW : LCOL];
+  const int tid = threadIdx.x;
+  const uint64_t u = blockIdx.x;
+  const uint64_t p = u % npart, cp = u / npart;
+  const uint64_t chan = cp / npol, pol = cp % npol;
+  const float2* x = reinterpret_cast<const float2*>(in + chan * ics + pol * ips + 2 * p * in_step);
+
+  float2 v[E];
+  constexpr int R0 = FirstRadix<NFFT, E, -1>::v;
+#pragma unroll
+  for (int q = 0; q < E / R0; q++)
+#pragma unroll
+    for (int r = 0; r < R0; r++) v[q * R0 + r] = x[tid + q * NT + r * (NFFT / R0)];
+  fft_regs<ROW, NFFT, 1, NT, -1>(lds, v, tw_f, tid);
+
+  // the forward transform's last pass leaves bin g + r*NFFT/RL in v[q*RL + r]
+  constexpr int RL = LastRadix<NFFT, E, -1>::v;
+  const float2* f = resp + chan * NFFT;
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < E / RL; q++)
+#pragma unroll
+    for (int r = 0; r < RL; r++) {
+      const int k = tid + q * NT + r * (NFFT / RL);
+      const float2 d = v[q * RL + r], w = f[k];
+      lds[COL::idx(k % FRES, k / FRES)] = make_float2(w.x * d.x - w.y * d.y, w.y * d.x + w.x * d.y);
+    }
+  __syncthreads();
+  lds_load_pass<COL, FRES, NSUB, NT, FirstRadix<FRES, E, +1>::v>(lds, v, tid);
+  fft_regs<COL, FRES, NSUB, NT, +1>(lds, v, tw_b, tid);
+  constexpr int NP = RadixPlan<FRES, E>::n;
+  __syncthreads();
+  lds_store_pass<COL, FRES, NSUB, NT, InvRadix<FRES, E, NP - 1>::v, InvNs<FRES, E, NP - 1>::v>(lds, v, tid);
+  __syncthreads();
+
+  float* o = out + chan * NSUB * ocs + pol * ops + 2 * p * uint64_t(nkeep);
+  for (uint32_t e = tid; e < NSUB * nkeep; e += NT) {
+    const uint32_t s = e / nkeep, m = e - s * nkeep;
+    *reinterpret_cast<float2*>(o + s * ocs + 2 * m) = lds[COL::idx(int(nfilt_pos + m), int(s))];
+  }
 }
 
 }  // namespace dspx
@@ -167,6 +234,28 @@ int dspx_fb_perform(dspx_fb* F, const float* d_in, uint64_t ics, uint64_t ips, u
     DSPX_REQUIRE(npart < (1u << 31), DSPX_E_UNSUPPORTED, "npart");
     const dspx_fb_desc& d = F->d;
     const uint64_t nunit = uint64_t(d.nchan) * d.npol * npart;
+    DSPX_REQUIRE(nunit < (1ull << 31), DSPX_E_UNSUPPORTED, "too many windows: %llu", (unsigned long long) nunit);
+    hipStream_t st = F->ctx->stream;
+
+    // One fused kernel for the window lengths it is built for (1024-4096
+    // points in sub-bands of 64-256), the library form otherwise or where
+    // the environment asks for it (DSPX_FB_FORM=library).
+    const char* form = std::getenv("DSPX_FB_FORM");
+    const bool library = form && std::string(form) == "library";
+#define FB_FUSED(n, m)                                                                                   \
+  if (!library && F->nfft == n && d.fres == m) {                                                         \
+    k_fb_fused<n, m><<<unsigned(nunit), n / 16, 0, st>>>(d_in, ics, ips, d.npol, F->in_step,             \
+                                                         uint32_t(npart), F->resp.as<float2>(),          \
+                                                         twiddles(F->ctx, n), twiddles(F->ctx, m),       \
+                                                         F->nkeep, d.nfilt_pos, d_out, ocs, ops);        \
+    DSPX_HIP(hipGetLastError());                                                                         \
+    return;                                                                                              \
+  }
+    FB_FUSED(1024, 64) FB_FUSED(1024, 128) FB_FUSED(1024, 256)
+    FB_FUSED(2048, 64) FB_FUSED(2048, 128) FB_FUSED(2048, 256)
+    FB_FUSED(4096, 64) FB_FUSED(4096, 128) FB_FUSED(4096, 256)
+#undef FB_FUSED
+
     const uint64_t per = F->nfft * sizeof(float2);
     // units per batch: scratch that stays in the 256 MB last-level cache
     // between the passes (128 MB measured fastest of 16..256 MB on 128
@@ -197,7 +286,6 @@ int dspx_fb_perform(dspx_fb* F, const float* d_in, uint64_t ics, uint64_t ips, u
     }
     F->scratch.reserve(batch * per);
     float2* s = F->scratch.as<float2>();
-    hipStream_t st = F->ctx->stream;
     for (uint64_t u0 = 0; u0 < nunit; u0 += batch) {
       const uint64_t nb = std::min(batch, nunit - u0);
       if (nb != batch)  // the unused rows of the last batch stay finite

@@ -71,3 +71,27 @@ def test_filterbank_several_batches(ctx, dspx, oracle, monkeypatch):
     got = _run(dspx, ctx, x, nsub, fres, pos, neg, resp)
     assert got.shape == expect.shape and expect.shape[2] == 41 * (fres - pos - neg)
     assert rel_rms(got, expect) < 1e-5
+
+
+@pytest.mark.parametrize("form", ["fused", "library"])
+@pytest.mark.parametrize("nsub,fres", [(16, 64), (8, 128), (8, 256), (32, 128), (16, 256), (64, 64)])
+def test_filterbank_window_lengths(ctx, dspx, oracle, monkeypatch, form, nsub, fres):
+    # 1024-4096-point windows: the one-kernel form and the library form
+    # (DSPX_FB_FORM=library) against the oracle, 2 channels x 2 pols
+    if form == "library":
+        monkeypatch.setenv("DSPX_FB_FORM", "library")
+    else:
+        monkeypatch.delenv("DSPX_FB_FORM", raising=False)
+    rng = np.random.default_rng(nsub * 1000 + fres)
+    nchan, npol, pos, neg = 2, 2, 9, 14
+    nfft = nsub * fres
+    resp = (np.exp(1j * rng.uniform(0, 2 * np.pi, (nchan, nfft))) / np.sqrt(nfft * fres)).astype(np.complex64)
+    ndat = 4 * nfft + 33
+    x = (rng.standard_normal((nchan, npol, ndat)) + 1j * rng.standard_normal((nchan, npol, ndat))).astype(np.complex64)
+    expect = oracle.filterbank(x, nsub, fres, pos, neg, resp)
+    print("filterbank", form, nsub, fres, flush=True)
+    got = _run(dspx, ctx, x, nsub, fres, pos, neg, resp)
+    assert got.shape == expect.shape
+    e = rel_rms(got, expect)
+    print("rel rms", e, flush=True)
+    assert e < 1e-5
