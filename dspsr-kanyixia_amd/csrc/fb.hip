@@ -106,10 +106,10 @@ __global__ __launch_bounds__(NFFT / 16) void k_fb_fused(const float* __restrict_
   constexpr int LROW = NFFT + NFFT / 16, LCOL = FRES * RS;
   __shared__ float2 lds[LROW > LCOL ? LROW : LCOL];
   const int tid = threadIdx.x;
-  const uint64_t u = blockIdx.x;
-  const uint64_t p = u % npart, cp = u / npart;
-  const uint64_t chan = cp / npol, pol = cp % npol;
-  const float2* x = reinterpret_cast<const float2*>(in + chan * ics + pol * ips + 2 * p * in_step);
+  const uint32_t u = blockIdx.x;  // fewer than 2^31 units
+  const uint32_t p = u % npart, cp = u / npart;
+  const uint32_t chan = cp / npol, pol = cp % npol;
+  const float2* x = reinterpret_cast<const float2*>(in + chan * ics + pol * ips + 2 * uint64_t(p) * in_step);
 
   float2 v[E];
   constexpr int R0 = FirstRadix<NFFT, E, -1>::v;
@@ -121,7 +121,7 @@ __global__ __launch_bounds__(NFFT / 16) void k_fb_fused(const float* __restrict_
 
   // the forward transform's last pass leaves bin g + r*NFFT/RL in v[q*RL + r]
   constexpr int RL = LastRadix<NFFT, E, -1>::v;
-  const float2* f = resp + chan * NFFT;
+  const float2* f = resp + uint64_t(chan) * NFFT;
   __syncthreads();
 #pragma unroll
   for (int q = 0; q < E / RL; q++)
@@ -139,11 +139,11 @@ __global__ __launch_bounds__(NFFT / 16) void k_fb_fused(const float* __restrict_
   lds_store_pass<COL, FRES, NSUB, NT, InvRadix<FRES, E, NP - 1>::v, InvNs<FRES, E, NP - 1>::v>(lds, v, tid);
   __syncthreads();
 
-  float* o = out + chan * NSUB * ocs + pol * ops + 2 * p * uint64_t(nkeep);
-  for (uint32_t e = tid; e < NSUB * nkeep; e += NT) {
-    const uint32_t s = e / nkeep, m = e - s * nkeep;
-    *reinterpret_cast<float2*>(o + s * ocs + 2 * m) = lds[COL::idx(int(nfilt_pos + m), int(s))];
-  }
+  float* o = out + uint64_t(chan) * NSUB * ocs + pol * ops + 2 * uint64_t(p) * nkeep;
+#pragma unroll
+  for (int s = 0; s < NSUB; s++)
+    for (uint32_t m = tid; m < nkeep; m += NT)
+      *reinterpret_cast<float2*>(o + s * ocs + 2 * m) = lds[COL::idx(int(nfilt_pos + m), s)];
 }
 
 }  // namespace dspx
