@@ -238,7 +238,7 @@ int dspx_fb_perform(dspx_fb* F, const float* d_in, uint64_t ics, uint64_t ips, u
     hipStream_t st = F->ctx->stream;
 
     // One fused kernel for the window lengths it is built for (1024-4096
-    // points in sub-bands of 64-256), the library form otherwise or where
+    // points in sub-bands of 32-512), the library form otherwise or where
     // the environment asks for it (DSPX_FB_FORM=library).
     const char* form = std::getenv("DSPX_FB_FORM");
     const bool library = form && std::string(form) == "library";
@@ -251,9 +251,9 @@ int dspx_fb_perform(dspx_fb* F, const float* d_in, uint64_t ics, uint64_t ips, u
     DSPX_HIP(hipGetLastError());                                                                         \
     return;                                                                                              \
   }
-    FB_FUSED(1024, 64) FB_FUSED(1024, 128) FB_FUSED(1024, 256)
-    FB_FUSED(2048, 64) FB_FUSED(2048, 128) FB_FUSED(2048, 256)
-    FB_FUSED(4096, 64) FB_FUSED(4096, 128) FB_FUSED(4096, 256)
+    FB_FUSED(1024, 32) FB_FUSED(1024, 64) FB_FUSED(1024, 128) FB_FUSED(1024, 256) FB_FUSED(1024, 512)
+    FB_FUSED(2048, 32) FB_FUSED(2048, 64) FB_FUSED(2048, 128) FB_FUSED(2048, 256) FB_FUSED(2048, 512)
+    FB_FUSED(4096, 32) FB_FUSED(4096, 64) FB_FUSED(4096, 128) FB_FUSED(4096, 256) FB_FUSED(4096, 512)
 #undef FB_FUSED
 
     const uint64_t per = F->nfft * sizeof(float2);

@@ -74,7 +74,8 @@ def test_filterbank_several_batches(ctx, dspx, oracle, monkeypatch):
 
 
 @pytest.mark.parametrize("form", ["fused", "library"])
-@pytest.mark.parametrize("nsub,fres", [(16, 64), (8, 128), (8, 256), (32, 128), (16, 256), (64, 64)])
+@pytest.mark.parametrize("nsub,fres", [(16, 64), (8, 128), (8, 256), (32, 128), (16, 256), (64, 64), (32, 32),
+                                       (64, 32), (128, 32), (2, 512), (4, 512), (8, 512)])
 def test_filterbank_window_lengths(ctx, dspx, oracle, monkeypatch, form, nsub, fres):
     # 1024-4096-point windows: the one-kernel form and the library form
     # (DSPX_FB_FORM=library) against the oracle, 2 channels x 2 pols
