@@ -441,7 +441,8 @@ def run_engine(args, dspx, synth, ctx):
                                                "x 128 bins per channel", "npart": npart, "ndat_out": nout},
                         "roofline": {"bound": "hbm", "kernel": "k_fb_fused<2048, 128> (float in, float out)",
                                      "achieved": round(abytes / t / 1e6, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                                     "frac": round(abytes / t / 1e6 / HBM_PEAK_GBS, 4), "traffic": None}})
+                                     "frac": round(abytes / t / 1e6 / HBM_PEAK_GBS, 4),
+                                     "traffic": load_traffic("filterbank", "k_fb_fused")}})
         else:
             sd = dspx.sample_delays(1400.0, 400.0, nchan, 100.0)
             dl = dspx.SampleDelay(ctx, nchan, npol, sd["applied"])
