@@ -69,6 +69,11 @@ class LtfDesc(C.Structure):
                 ("form", C.c_uint32)]
 
 
+class FoldOpts(C.Structure):
+    _fields_ = [("weights", C.c_void_p), ("nweights", C.c_uint64), ("ndatperweight", C.c_uint32),
+                ("weight_idat", C.c_uint32), ("zeroed_samples", C.c_uint32), ("reserved", C.c_uint32)]
+
+
 class FbDesc(C.Structure):
     _fields_ = [("nchan", C.c_uint32), ("npol", C.c_uint32), ("nsub", C.c_uint32), ("fres", C.c_uint32),
                 ("nfilt_pos", C.c_uint32), ("nfilt_neg", C.c_uint32), ("reserved", C.c_uint32 * 2)]
@@ -108,6 +113,7 @@ _SIGS = {
     "dspx_fb_set_response": (I32, [P, P]),
     "dspx_fb_geometry": (I32, [P, U64, C.POINTER(U64), C.POINTER(U64)]),
     "dspx_fb_perform": (I32, [P, P, U64, U64, U64, P, U64, U64]),
+    "dspx_fb_batches": (I32, [P, C.POINTER(U32)]),
     "dspx_detect": (I32, [P, U32, U32, P, U64, U64, U32, U32, U64, P, U64, U64]),
     "dspx_fold_create": (I32, [P, U64, D, U32, C.POINTER(FoldChunk), U32, C.POINTER(P)]),
     "dspx_fold_destroy": (None, [P]),
@@ -116,6 +122,10 @@ _SIGS = {
     "dspx_fold_nsegments_host": (I32, [D, D, U64, P]),
     "dspx_fold_hits": (I32, [P, P]),
     "dspx_fold_execute": (I32, [P, P, U64, U64, U32, U32, U32, P]),
+    "dspx_fold_create_opts": (I32, [P, U64, D, U32, C.POINTER(FoldChunk), U32, C.POINTER(FoldOpts), C.POINTER(P)]),
+    "dspx_fold_replan": (I32, [P, U64, D, U32, C.POINTER(FoldChunk), U32, C.POINTER(FoldOpts)]),
+    "dspx_fold_ndat_folded": (I32, [P, C.POINTER(U64)]),
+    "dspx_fold_execute_zeroed": (I32, [P, P, U64, U64, U32, U32, U32, P, P, U32, P]),
     "dspx_sk_create": (I32, [P, C.POINTER(SkDesc), C.POINTER(P)]),
     "dspx_sk_destroy": (None, [P]),
     "dspx_sk_set_limits": (I32, [P, P, P, P, D]),
@@ -423,17 +433,38 @@ class Convolution:
 
 
 class Fold:
-    """Fold::Engine: device bin plan + accumulation."""
+    """Fold::Engine: device bin plan + accumulation.  weights (uint16 array,
+    with ndatperweight and weight_idat) and zeroed follow Fold::fold's
+    WeightedTimeSeries and zeroed-data options (Fold.C:700-733, 893-930)."""
 
-    def __init__(self, ctx, ndat, pps, nbin, chunks):
+    def __init__(self, ctx, ndat, pps, nbin, chunks, weights=None, ndatperweight=0, weight_idat=0,
+                 zeroed=False):
         self.ctx = ctx
         self.h = None
         self._lib = lib()
-        arr = (FoldChunk * len(chunks))(*[FoldChunk(c[0], c[1], c[2], c[3], 0) for c in chunks])
+        arr, opts, keep = self._args(chunks, weights, ndatperweight, weight_idat, zeroed)
         h = P()
-        ctx.check(lib().dspx_fold_create(ctx.h, ndat, pps, nbin, arr, len(chunks), C.byref(h)))
+        ctx.check(lib().dspx_fold_create_opts(ctx.h, ndat, pps, nbin, arr, len(chunks), C.byref(opts), C.byref(h)))
         self.h = h
         _register(self, False)
+
+    @staticmethod
+    def _args(chunks, weights, ndatperweight, weight_idat, zeroed):
+        import numpy as np
+        arr = (FoldChunk * len(chunks))(*[FoldChunk(c[0], c[1], c[2], c[3], 0) for c in chunks])
+        w = None if weights is None else np.ascontiguousarray(weights, np.uint16)
+        opts = FoldOpts(w.ctypes.data if w is not None else None, 0 if w is None else w.size,
+                        ndatperweight if w is not None else 0, weight_idat, int(bool(zeroed)), 0)
+        return arr, opts, w
+
+    def replan(self, ndat, pps, nbin, chunks, weights=None, ndatperweight=0, weight_idat=0, zeroed=False):
+        arr, opts, keep = self._args(chunks, weights, ndatperweight, weight_idat, zeroed)
+        self.ctx.check(lib().dspx_fold_replan(self.h, ndat, pps, nbin, arr, len(chunks), C.byref(opts)))
+
+    def ndat_folded(self):
+        n = U64()
+        _check(lib().dspx_fold_ndat_folded(self.h, C.byref(n)))
+        return n.value
 
     def close(self):
         if self.h:
@@ -452,9 +483,18 @@ class Fold:
         self.ctx.check(lib().dspx_fold_execute(self.h, ptr(x), chan_stride, pol_stride, nchan, npol,
                                                ndim, ptr(prof)))
 
+    def execute_zeroed(self, x, chan_stride, pol_stride, nchan, npol, ndim, prof, hits, hits_nchan, nfolded):
+        """hits: [nsub][hits_nchan][nbin] int32; nfolded: one int64 on the device."""
+        self.ctx.check(lib().dspx_fold_execute_zeroed(self.h, ptr(x), chan_stride, pol_stride, nchan, npol, ndim,
+                                                      ptr(prof), ptr(hits), hits_nchan, ptr(nfolded)))
+
 
 class Filterbank:
-    """dspx_fb: the convolving filterbank engine (FFT-library form)."""
+    """dspx_fb: the convolving filterbank engine.  Windows of 1024-4096
+    points (sub-bands of 32-512 bins) run as one kernel per window; other
+    lengths, or any length with the environment DSPX_FB_FORM=library, run as
+    batches of FFT-library transforms with kernels for the response product
+    and the overlap discard."""
 
     def __init__(self, ctx, nchan, npol, nsub, fres, nfilt_pos=0, nfilt_neg=0):
         self.ctx = ctx
@@ -487,6 +527,12 @@ class Filterbank:
 
     def perform(self, x, ics, ips, ndat, out, ocs, ops):
         self.ctx.check(lib().dspx_fb_perform(self.h, ptr(x), ics, ips, ndat, ptr(out), ocs, ops))
+
+    def batches(self):
+        """Library-form batches of the last perform (0: the one-kernel form)."""
+        n = U32()
+        _check(lib().dspx_fb_batches(self.h, C.byref(n)))
+        return n.value
 
 
 class SpectralKurtosis:

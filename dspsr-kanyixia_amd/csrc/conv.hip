@@ -607,6 +607,7 @@ using namespace dspx;
 // ------------------------------------------------------- Convolution engine
 
 struct dspx_conv {
+  CtxRef hold;
   dspx_ctx* ctx = nullptr;
   dspx_conv_desc d{};
   ConvCore core;
@@ -621,6 +622,7 @@ int dspx_conv_create(dspx_ctx* ctx, const dspx_conv_desc* d, dspx_conv** out)
     DSPX_REQUIRE(ctx && d && out, DSPX_E_PARAM, "null argument");
     DSPX_HIP(hipSetDevice(ctx->device));
     auto* c = new dspx_conv;
+    c->hold.reset(ctx);
     c->ctx = ctx;
     c->d = *d;
     try {
@@ -704,6 +706,7 @@ struct LtfScratch {
 };
 
 struct dspx_ltf {
+  CtxRef hold;
   dspx_ctx* ctx = nullptr;
   dspx_ltf_desc d{};
   uint64_t npart = 0, ndat_out = 0;
@@ -803,6 +806,7 @@ dspx_ltf* ltf_create(dspx_ctx* ctx, const dspx_ltf_desc* d, LtfScratch* ext)
     DSPX_REQUIRE(d->nbin >= 1 && d->nbin <= 65535, DSPX_E_PARAM, "nbin=%u", d->nbin);
     DSPX_HIP(hipSetDevice(ctx->device));
     auto* L = new dspx_ltf;
+    L->hold.reset(ctx);
     L->ctx = ctx;
     L->d = *d;
     L->ext = ext;
@@ -1268,6 +1272,7 @@ int dspx_ltf_stage_times(dspx_ltf* L, float* ms, const char** names, int cap, in
 // reference's cost model (optimize_fft.c:64-135).
 
 struct dspx_batch {
+  CtxRef hold;
   dspx_ctx* ctx = nullptr;
   LtfScratch sc;
   std::vector<dspx_ltf*> beams;
@@ -1378,6 +1383,7 @@ int dspx_batch_create(dspx_ctx* ctx, const dspx_ltf_desc* beams, uint32_t nbeam,
     DSPX_REQUIRE((flags & ~uint32_t(DSPX_BATCH_AUTOTUNE)) == 0, DSPX_E_PARAM, "flags=%u", flags);
     DSPX_HIP(hipSetDevice(ctx->device));
     std::unique_ptr<dspx_batch> B(new dspx_batch);
+    B->hold.reset(ctx);
     B->ctx = ctx;
     for (uint32_t i = 0; i < nbeam; i++) {
       const dspx_ltf_desc& d0 = beams[i];

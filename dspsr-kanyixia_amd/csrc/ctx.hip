@@ -140,6 +140,17 @@ void detect(dspx_ctx* ctx, uint32_t state, uint32_t ndim, const float* in, uint6
   DSPX_HIP(hipGetLastError());
 }
 
+void ctx_free(dspx_ctx* c)
+{
+  (void) hipSetDevice(c->device);
+  if (c->stream) (void) hipStreamSynchronize(c->stream);
+  for (auto& e : c->ev)
+    if (e) (void) hipEventDestroy(e);
+  if (c->d_tw) (void) hipFree(c->d_tw);
+  if (c->own_stream && c->stream) (void) hipStreamDestroy(c->stream);
+  delete c;
+}
+
 }  // namespace dspx
 
 using namespace dspx;
@@ -181,7 +192,7 @@ int dspx_ctx_create(int device, void* stream, dspx_ctx** out)
       DSPX_HIP(hipMemcpy(c->d_tw, h.data(), total * sizeof(float2), hipMemcpyHostToDevice));
       for (auto& e : c->ev) DSPX_HIP(hipEventCreate(&e));
     } catch (...) {
-      dspx_ctx_destroy(c);
+      ctx_free(c);
       throw;
     }
     *out = c;
@@ -191,13 +202,11 @@ int dspx_ctx_create(int device, void* stream, dspx_ctx** out)
 void dspx_ctx_destroy(dspx_ctx* c)
 {
   if (!c) return;
+  // the work queued so far completes before the call returns; the context
+  // itself lives on while engines made on it hold references
   (void) hipSetDevice(c->device);
   if (c->stream) (void) hipStreamSynchronize(c->stream);
-  for (auto& e : c->ev)
-    if (e) (void) hipEventDestroy(e);
-  if (c->d_tw) (void) hipFree(c->d_tw);
-  if (c->own_stream && c->stream) (void) hipStreamDestroy(c->stream);
-  delete c;
+  CtxRef::drop(c);
 }
 
 const char* dspx_last_error(const dspx_ctx* c) { return c ? c->last_error.c_str() : "null context"; }

@@ -43,6 +43,7 @@ __global__ __launch_bounds__(256) void k_retard(const float* __restrict__ in, ui
 using namespace dspx;
 
 struct dspx_delay {
+  CtxRef hold;
   dspx_ctx* ctx = nullptr;
   uint32_t nchan = 0, npol = 0;
   DevBuf applied;  // int64 [nchan * npol]
@@ -61,6 +62,7 @@ int dspx_delay_create(dspx_ctx* ctx, uint32_t nchan, uint32_t npol, const int64_
                    (unsigned long long) (i / npol));  // SampleDelayCUDA.cu:100-104
     DSPX_HIP(hipSetDevice(ctx->device));
     auto* D = new dspx_delay;
+    D->hold.reset(ctx);
     D->ctx = ctx;
     D->nchan = nchan;
     D->npol = npol;

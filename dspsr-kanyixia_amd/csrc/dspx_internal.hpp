@@ -9,8 +9,11 @@
 #include <vector>
 
 #include "dspx.h"
+#include "refcount.hpp"
 
 struct dspx_ctx {
+  // the caller's handle and every engine made on the context (refcount.hpp)
+  dspx::RefCount refs;
   int device = 0;
   int ncu = 0;  // compute units of the device
   hipStream_t stream = nullptr;
@@ -24,6 +27,12 @@ struct dspx_ctx {
 };
 
 namespace dspx {
+
+// Frees a context whose last reference was dropped (ctx.hip).
+void ctx_free(dspx_ctx* c);
+// An engine's reference on its context: declared as the engine's first
+// member, so it is dropped after the engine's own buffers are released.
+using CtxRef = Ref<dspx_ctx, ctx_free>;
 
 struct Error {
   int code;

@@ -1,4 +1,5 @@
-// fb.hip -- convolving filterbank engine, FFT-library form.
+// fb.hip -- convolving filterbank engine: one kernel per window for
+// 1024-4096-point windows, the FFT-library form otherwise.
 //
 // Reference: Filterbank::prepare / resize_output (Filterbank.C:75-200,
 // 405-460) and FilterbankEngineCPU::perform (FilterbankEngineCPU.C:184-370):
@@ -151,6 +152,7 @@ __global__ __launch_bounds__(NFFT / 16) void k_fb_fused(const float* __restrict_
 using namespace dspx;
 
 struct dspx_fb {
+  CtxRef hold;
   dspx_ctx* ctx = nullptr;
   dspx_fb_desc d{};
   uint64_t nfft = 0, in_step = 0;
@@ -160,6 +162,7 @@ struct dspx_fb {
   hipfftHandle fwd = 0, bwd = 0;
   bool have_plan = false;
   uint64_t plan_batch = 0;
+  uint32_t last_batches = 0;  // library-form batches of the last perform (0: one kernel)
 
   ~dspx_fb()
   {
@@ -184,6 +187,7 @@ int dspx_fb_create(dspx_ctx* ctx, const dspx_fb_desc* desc, dspx_fb** out)
     DSPX_REQUIRE(uint64_t(d.nchan) * d.nsub <= 65535 * 64, DSPX_E_UNSUPPORTED, "too many output channels");
     DSPX_HIP(hipSetDevice(ctx->device));
     auto* F = new dspx_fb;
+    F->hold.reset(ctx);
     F->ctx = ctx;
     F->d = d;
     F->nfft = uint64_t(d.nsub) * d.fres;
@@ -236,6 +240,7 @@ int dspx_fb_perform(dspx_fb* F, const float* d_in, uint64_t ics, uint64_t ips, u
     const uint64_t nunit = uint64_t(d.nchan) * d.npol * npart;
     DSPX_REQUIRE(nunit < (1ull << 31), DSPX_E_UNSUPPORTED, "too many windows: %llu", (unsigned long long) nunit);
     hipStream_t st = F->ctx->stream;
+    F->last_batches = 0;
 
     // One fused kernel for the window lengths it is built for (1024-4096
     // points in sub-bands of 32-512), the library form otherwise or where
@@ -285,6 +290,7 @@ int dspx_fb_perform(dspx_fb* F, const float* d_in, uint64_t ics, uint64_t ips, u
       F->plan_batch = batch;
     }
     F->scratch.reserve(batch * per);
+    F->last_batches = uint32_t(nbat);
     float2* s = F->scratch.as<float2>();
     for (uint64_t u0 = 0; u0 < nunit; u0 += batch) {
       const uint64_t nb = std::min(batch, nunit - u0);
@@ -306,6 +312,13 @@ int dspx_fb_perform(dspx_fb* F, const float* d_in, uint64_t ics, uint64_t ips, u
       DSPX_HIP(hipGetLastError());
     }
   });
+}
+
+int dspx_fb_batches(const dspx_fb* F, uint32_t* nbatch)
+{
+  if (!F || !nbatch) return DSPX_E_PARAM;
+  *nbatch = F->last_batches;
+  return DSPX_OK;
 }
 
 }  // extern "C"

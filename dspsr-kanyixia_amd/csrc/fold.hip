@@ -1,9 +1,9 @@
 // fold.hip -- Fold::Engine: device bin plan, hits and phase-bin accumulation.
 //
-// Reference: Fold::fold (Signal/Pulsar/Fold.C:662-949), CUDA::FoldEngine
+// Reference: Fold::fold (Signal/Pulsar/Fold.C:662-973), CUDA::FoldEngine
 // (Signal/Pulsar/FoldCUDA.cu:25-704).  The reference GPU engine builds its bin
 // plan serially on the host and copies it every block; here the host only
-// walks binade runs (binplan.cpp) and the device evaluates every sample.
+// walks binade runs (segments.hpp) and the device evaluates every sample.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -22,31 +22,32 @@ __global__ void k_binplan(const FoldSegment* __restrict__ seg, uint32_t nseg, ui
   out[i] = seg_key(seg[seg_find(seg, nseg, i)], i, nbin_d);
 }
 
-// hits[isub][bin] += count (Fold.C:827-836 for unzapped data).  A workgroup
-// covers 4096 consecutive samples.
+// First sub-integration folded in samples [i0, i1), or ~0.
+template <class KEYS> __device__ uint32_t first_sub(const KEYS& keys, uint64_t i0, uint64_t i1)
+{
+  for (uint64_t i = i0; i < i1; i++) {
+    const uint32_t k = keys(i);
+    if (k != 0xffffffffu) return k >> 16;
+  }
+  return 0xffffffffu;
+}
+
+// hits[isub][bin] += count (Fold.C:827-836 without zeroed data).  A
+// workgroup covers 4096 consecutive samples.
 constexpr int kHitsChunk = 4096;
-__global__ __launch_bounds__(256) void k_hits(const uint32_t* __restrict__ plan, uint64_t ndat,
-                                              uint32_t nbin, uint32_t* __restrict__ hits)
+template <class KEYS>
+__global__ __launch_bounds__(256) void k_hits(KEYS keys, uint64_t ndat, uint32_t nbin, uint32_t* __restrict__ hits)
 {
   extern __shared__ uint32_t h[];  // 2 * nbin
   uint64_t i0 = uint64_t(blockIdx.x) * kHitsChunk;
   uint64_t i1 = min(ndat, i0 + kHitsChunk);
   for (uint32_t k = threadIdx.x; k < 2 * nbin; k += blockDim.x) h[k] = 0;
   __shared__ uint32_t sub0;
-  if (threadIdx.x == 0) {
-    uint32_t s = 0xffffffffu;
-    for (uint64_t i = i0; i < i1; i++)
-      if (plan[i] != 0xffffffffu) {
-        s = plan[i] >> 16;
-        break;
-      }
-    sub0 = s;
-  }
+  if (threadIdx.x == 0) sub0 = first_sub(keys, i0, i1);
   __syncthreads();
   for (uint64_t base = i0; base < i1; base += blockDim.x) {
     uint64_t i = base + threadIdx.x;
-    uint32_t key = i < i1 ? plan[i] : 0xffffffffu;
-    // count as a float sum: exact for counts below 2^24
+    uint32_t key = i < i1 ? keys(i) : 0xffffffffu;
     const uint32_t k0 = __builtin_amdgcn_readfirstlane(key);
     if (__all(key == k0)) {
       if (k0 == 0xffffffffu) continue;
@@ -75,15 +76,19 @@ __global__ __launch_bounds__(256) void k_hits(const uint32_t* __restrict__ plan,
 // folds a contiguous run of samples of one (chan, pol) into an LDS copy of
 // the (at most two) sub-integrations the run meets, summing each wavefront's
 // samples first when they share a bin, then adds the non-zero bins into the
-// profile.
+// profile.  With zhits (zeroed data, Fold.C:893-930) the pol-0 workgroups
+// also count, per bin, the samples whose value 0 is non-zero into
+// zhits[isub][hchan][bin] (hchan = chan when hits_nchan = nchan, else 0) and
+// add their number into *nfolded.
 constexpr int kFoldChunk = 8192;
+template <class KEYS>
 __global__ __launch_bounds__(256) void k_fold_fpt(const float* __restrict__ in, uint64_t chan_stride,
-                                                  uint64_t pol_stride, uint32_t npol, uint32_t ndim,
-                                                  const uint32_t* __restrict__ plan, uint64_t ndat,
-                                                  uint32_t nbin, uint32_t nchan, uint32_t nchunk,
-                                                  float* __restrict__ prof)
+                                                  uint64_t pol_stride, uint32_t npol, uint32_t ndim, KEYS keys,
+                                                  uint64_t ndat, uint32_t nbin, uint32_t nchan, uint32_t nchunk,
+                                                  float* __restrict__ prof, uint32_t* __restrict__ zhits,
+                                                  uint32_t hits_nchan, unsigned long long* __restrict__ nfolded)
 {
-  extern __shared__ float hf[];  // 2 * nbin * ndim
+  extern __shared__ float hf[];  // 2 * nbin * ndim floats, then (zhits) 2 * nbin counts
   uint32_t bid = blockIdx.x;
   uint32_t chunk = bid % nchunk;
   bid /= nchunk;
@@ -92,31 +97,55 @@ __global__ __launch_bounds__(256) void k_fold_fpt(const float* __restrict__ in, 
   uint64_t i0 = uint64_t(chunk) * kFoldChunk;
   uint64_t i1 = min(ndat, i0 + kFoldChunk);
   const uint32_t hl = nbin * ndim;
+  const bool count = zhits && pol == 0;
+  uint32_t* hz = reinterpret_cast<uint32_t*>(hf + 2 * hl);
   for (uint32_t k = threadIdx.x; k < 2 * hl; k += blockDim.x) hf[k] = 0.f;
+  if (count)
+    for (uint32_t k = threadIdx.x; k < 2 * nbin; k += blockDim.x) hz[k] = 0;
   __shared__ uint32_t sub0;
+  __shared__ uint32_t nz;
   if (threadIdx.x == 0) {
-    uint32_t s = 0xffffffffu;
-    for (uint64_t i = i0; i < i1; i++)
-      if (plan[i] != 0xffffffffu) {
-        s = plan[i] >> 16;
-        break;
-      }
-    sub0 = s;
+    sub0 = first_sub(keys, i0, i1);
+    nz = 0;
   }
   __syncthreads();
   const float* x = in + uint64_t(chan) * chan_stride + uint64_t(pol) * pol_stride;
   const uint64_t prof_sub = uint64_t(nchan) * npol * hl;
   float* pc = prof + (uint64_t(chan) * npol + pol) * hl;
+  const uint64_t hz_sub = uint64_t(hits_nchan) * nbin;
+  uint32_t* hzg = count ? zhits + uint64_t(hits_nchan == nchan ? chan : 0) * nbin : nullptr;
+  uint32_t my_nz = 0;
   for (uint64_t base = i0; base < i1; base += blockDim.x) {
     uint64_t i = base + threadIdx.x;
-    uint32_t key = i < i1 ? plan[i] : 0xffffffffu;
+    uint32_t key = i < i1 ? keys(i) : 0xffffffffu;
     const uint32_t k0 = __builtin_amdgcn_readfirstlane(key);
     const bool uniform = __all(key == k0);
+    if (uniform && k0 == 0xffffffffu) continue;
     for (uint32_t d = 0; d < ndim; d++) {
       float v = (key != 0xffffffffu) ? x[i * ndim + d] : 0.f;
+      if (count && d == 0) {
+        // samples zapped to zero are folded but not counted (Fold.C:911-920)
+        const bool hit = key != 0xffffffffu && v != 0.f;
+        my_nz += hit ? 1u : 0u;
+        if (uniform) {
+          const uint32_t nh = uint32_t(__popcll(__ballot(hit)));
+          if ((threadIdx.x & 63u) == 63 && nh) {
+            uint32_t sub = k0 >> 16, bin = k0 & 0xffffu;
+            if (sub - sub0 < 2)
+              atomicAdd(&hz[(sub - sub0) * nbin + bin], nh);
+            else
+              atomicAdd(&hzg[uint64_t(sub) * hz_sub + bin], nh);
+          }
+        } else if (hit) {
+          uint32_t sub = key >> 16, bin = key & 0xffffu;
+          if (sub - sub0 < 2)
+            atomicAdd(&hz[(sub - sub0) * nbin + bin], 1u);
+          else
+            atomicAdd(&hzg[uint64_t(sub) * hz_sub + bin], 1u);
+        }
+      }
       uint32_t k = key;
       if (uniform) {
-        if (k0 == 0xffffffffu) break;
         v = wave_sum_lane63(v);
         if ((threadIdx.x & 63u) != 63) continue;
         k = k0;
@@ -130,14 +159,20 @@ __global__ __launch_bounds__(256) void k_fold_fpt(const float* __restrict__ in, 
         atomicAdd(&pc[uint64_t(sub) * prof_sub + bin * ndim + d], v);
     }
   }
+  if (count && my_nz) atomicAdd(&nz, my_nz);
   __syncthreads();
   if (sub0 == 0xffffffffu) return;
   for (uint32_t k = threadIdx.x; k < 2 * hl; k += blockDim.x)
     if (hf[k] != 0.f) atomicAdd(&pc[uint64_t(sub0 + k / hl) * prof_sub + (k % hl)], hf[k]);
+  if (count) {
+    for (uint32_t k = threadIdx.x; k < 2 * nbin; k += blockDim.x)
+      if (hz[k]) atomicAdd(&hzg[uint64_t(sub0 + k / nbin) * hz_sub + (k % nbin)], hz[k]);
+    if (threadIdx.x == 0 && nz && nfolded) atomicAdd(nfolded, (unsigned long long) nz);
+  }
 }
 
 void FoldPlan::build(dspx_ctx* c, uint64_t ndat_, double pps_, uint32_t nbin_,
-                     const dspx_fold_chunk* chunks, uint32_t nchunk, bool sample_plan)
+                     const dspx_fold_chunk* chunks, uint32_t nchunk, bool sample_plan, const FoldWeights* weights)
 {
   ctx = c;
   ndat = ndat_;
@@ -150,22 +185,16 @@ void FoldPlan::build(dspx_ctx* c, uint64_t ndat_, double pps_, uint32_t nbin_,
   std::vector<dspx_fold_chunk> cs(chunks, chunks + nchunk);
   std::sort(cs.begin(), cs.end(),
             [](const dspx_fold_chunk& a, const dspx_fold_chunk& b) { return a.idat_start < b.idat_start; });
-  std::vector<FoldSegment> segs;
-  uint64_t cursor = 0;
-  nsub = 0;
-  for (auto& ch : cs) {
-    DSPX_REQUIRE(ch.idat_start >= cursor, DSPX_E_PARAM, "fold chunks overlap at sample %llu",
-                 (unsigned long long) ch.idat_start);
-    DSPX_REQUIRE(ch.idat_start + ch.ndat <= ndat, DSPX_E_PARAM,
-                 "fold chunk [%llu, +%llu) beyond ndat=%llu", (unsigned long long) ch.idat_start,
-                 (unsigned long long) ch.ndat, (unsigned long long) ndat);
-    DSPX_REQUIRE(ch.isub < 0xffff, DSPX_E_PARAM, "isub=%u too large", ch.isub);
-    if (ch.idat_start > cursor) segs.push_back(FoldSegment{cursor, 0, 0, 0, 0xffffffffu});
-    if (ch.ndat) fold_segments(ch.phi, pps, ch.idat_start, ch.ndat, ch.isub, segs);
-    cursor = ch.idat_start + ch.ndat;
-    nsub = std::max(nsub, ch.isub + 1);
+  WeightView wv;
+  if (weights && weights->ndatperweight) {
+    wv.w = weights->w.data();
+    wv.nweights = weights->w.size();
+    wv.ndatperweight = weights->ndatperweight;
+    wv.weight_idat = weights->weight_idat;
   }
-  if (cursor < ndat || segs.empty()) segs.push_back(FoldSegment{cursor, 0, 0, 0, 0xffffffffu});
+  std::vector<FoldSegment> segs;
+  plan_segments(ndat, pps, cs.data(), uint32_t(cs.size()), wv, [&](const FoldSegment& s) { segs.push_back(s); },
+                nsub, nfold);
   nseg = uint32_t(segs.size());
   seg.reserve(segs.size() * sizeof(FoldSegment));
   int slot;
@@ -184,31 +213,48 @@ void FoldPlan::build(dspx_ctx* c, uint64_t ndat_, double pps_, uint32_t nbin_,
   chunk_table = cs;
 }
 
-void FoldPlan::hits(uint32_t* d_hits)
+// Launches of the two fold kernels over any key source (shared by the
+// planned engine and the form without a per-sample plan).
+template <class KEYS> void launch_hits(hipStream_t st, const KEYS& keys, uint64_t ndat, uint32_t nbin, uint32_t* d_hits)
 {
   if (!ndat) return;
-  DSPX_REQUIRE(have_sample_plan, DSPX_E_INTERNAL, "hits without a sample plan");
   unsigned nb = unsigned((ndat + kHitsChunk - 1) / kHitsChunk);
-  k_hits<<<nb, 256, 2 * nbin * sizeof(uint32_t), ctx->stream>>>(plan.as<uint32_t>(), ndat, nbin,
-                                                                   d_hits);
+  k_hits<KEYS><<<nb, 256, 2 * nbin * sizeof(uint32_t), st>>>(keys, ndat, nbin, d_hits);
   DSPX_HIP(hipGetLastError());
 }
 
-void FoldPlan::fold_fpt(const float* d_in, uint64_t chan_stride, uint64_t pol_stride, uint32_t nchan,
-                        uint32_t npol, uint32_t ndim, float* d_prof)
+template <class KEYS>
+void launch_fold(hipStream_t st, const KEYS& keys, uint64_t ndat, uint32_t nbin, const float* d_in, uint64_t chan_stride,
+                 uint64_t pol_stride, uint32_t nchan, uint32_t npol, uint32_t ndim, float* d_prof, uint32_t* zhits,
+                 uint32_t hits_nchan, unsigned long long* nfolded)
 {
   if (!ndat) return;
-  DSPX_REQUIRE(have_sample_plan, DSPX_E_INTERNAL, "fold without a sample plan");
   DSPX_REQUIRE(ndim >= 1 && ndim <= 4, DSPX_E_PARAM, "ndim=%u", ndim);
-  size_t shm = size_t(2) * nbin * ndim * sizeof(float);
+  DSPX_REQUIRE(!zhits || hits_nchan == 1 || hits_nchan == nchan, DSPX_E_PARAM,
+               "hits_nchan=%u must be 1 or nchan=%u", hits_nchan, nchan);
+  size_t shm = size_t(2) * nbin * ndim * sizeof(float) + (zhits ? size_t(2) * nbin * sizeof(uint32_t) : 0);
   DSPX_REQUIRE(shm <= 64 * 1024, DSPX_E_UNSUPPORTED, "nbin*ndim=%u too large for LDS", nbin * ndim);
   uint32_t nchunk = uint32_t((ndat + kFoldChunk - 1) / kFoldChunk);
   uint64_t nb = uint64_t(nchunk) * npol * nchan;
   DSPX_REQUIRE(nb < (1ull << 31), DSPX_E_UNSUPPORTED, "grid too large");
-  k_fold_fpt<<<unsigned(nb), 256, shm, ctx->stream>>>(d_in, chan_stride, pol_stride, npol, ndim,
-                                                      plan.as<uint32_t>(), ndat, nbin, nchan, nchunk,
-                                                      d_prof);
+  k_fold_fpt<KEYS><<<unsigned(nb), 256, shm, st>>>(d_in, chan_stride, pol_stride, npol, ndim, keys, ndat, nbin,
+                                                   nchan, nchunk, d_prof, zhits, hits_nchan, nfolded);
   DSPX_HIP(hipGetLastError());
+}
+
+void FoldPlan::hits(uint32_t* d_hits)
+{
+  DSPX_REQUIRE(have_sample_plan, DSPX_E_INTERNAL, "hits without a sample plan");
+  launch_hits(ctx->stream, PlanKeys{plan.as<uint32_t>()}, ndat, nbin, d_hits);
+}
+
+void FoldPlan::fold_fpt(const float* d_in, uint64_t chan_stride, uint64_t pol_stride, uint32_t nchan,
+                        uint32_t npol, uint32_t ndim, float* d_prof, uint32_t* zhits, uint32_t hits_nchan,
+                        unsigned long long* nfolded)
+{
+  DSPX_REQUIRE(have_sample_plan, DSPX_E_INTERNAL, "fold without a sample plan");
+  launch_fold(ctx->stream, PlanKeys{plan.as<uint32_t>()}, ndat, nbin, d_in, chan_stride, pol_stride, nchan, npol,
+              ndim, d_prof, zhits, hits_nchan, nfolded);
 }
 
 }  // namespace dspx
@@ -216,20 +262,52 @@ void FoldPlan::fold_fpt(const float* d_in, uint64_t chan_stride, uint64_t pol_st
 using namespace dspx;
 
 struct dspx_fold {
+  CtxRef hold;
   FoldPlan plan;
+  FoldWeights weights;
+  bool zeroed = false;
 };
+
+namespace {
+
+void fold_plan(dspx_fold* f, dspx_ctx* ctx, uint64_t ndat, double pps, uint32_t nbin, const dspx_fold_chunk* chunks,
+               uint32_t nchunk, const dspx_fold_opts* opts)
+{
+  f->zeroed = opts && opts->zeroed_samples;
+  f->weights.w.clear();
+  f->weights.ndatperweight = 0;
+  f->weights.weight_idat = 0;
+  // with zeroed data the weights mark nothing (Fold.C:728, 802)
+  if (opts && opts->ndatperweight && !f->zeroed) {
+    DSPX_REQUIRE(opts->weights && opts->nweights, DSPX_E_PARAM, "ndatperweight=%u without weights",
+                 opts->ndatperweight);
+    f->weights.w.assign(opts->weights, opts->weights + opts->nweights);
+    f->weights.ndatperweight = opts->ndatperweight;
+    f->weights.weight_idat = opts->weight_idat;
+  }
+  f->plan.build(ctx, ndat, pps, nbin, chunks, nchunk, true, &f->weights);
+}
+
+}  // namespace
 
 extern "C" {
 
 int dspx_fold_create(dspx_ctx* ctx, uint64_t ndat, double pps, uint32_t nbin,
                      const dspx_fold_chunk* chunks, uint32_t nchunk, dspx_fold** out)
 {
+  return dspx_fold_create_opts(ctx, ndat, pps, nbin, chunks, nchunk, nullptr, out);
+}
+
+int dspx_fold_create_opts(dspx_ctx* ctx, uint64_t ndat, double pps, uint32_t nbin, const dspx_fold_chunk* chunks,
+                          uint32_t nchunk, const dspx_fold_opts* opts, dspx_fold** out)
+{
   return guard(ctx, [&] {
     DSPX_REQUIRE(ctx && out && chunks, DSPX_E_PARAM, "null argument");
     DSPX_HIP(hipSetDevice(ctx->device));
     auto* f = new dspx_fold;
+    f->hold.reset(ctx);
     try {
-      f->plan.build(ctx, ndat, pps, nbin, chunks, nchunk);
+      fold_plan(f, ctx, ndat, pps, nbin, chunks, nchunk, opts);
     } catch (...) {
       delete f;
       throw;
@@ -238,7 +316,30 @@ int dspx_fold_create(dspx_ctx* ctx, uint64_t ndat, double pps, uint32_t nbin,
   });
 }
 
-void dspx_fold_destroy(dspx_fold* f) { delete f; }
+int dspx_fold_replan(dspx_fold* f, uint64_t ndat, double pps, uint32_t nbin, const dspx_fold_chunk* chunks,
+                     uint32_t nchunk, const dspx_fold_opts* opts)
+{
+  dspx_ctx* ctx = f ? f->plan.ctx : nullptr;
+  return guard(ctx, [&] {
+    DSPX_REQUIRE(f && chunks, DSPX_E_PARAM, "null argument");
+    fold_plan(f, ctx, ndat, pps, nbin, chunks, nchunk, opts);
+  });
+}
+
+void dspx_fold_destroy(dspx_fold* f)
+{
+  if (!f) return;
+  (void) hipSetDevice(f->plan.ctx->device);
+  (void) hipStreamSynchronize(f->plan.ctx->stream);
+  delete f;
+}
+
+int dspx_fold_ndat_folded(const dspx_fold* f, uint64_t* n)
+{
+  if (!f || !n) return DSPX_E_PARAM;
+  *n = f->zeroed ? 0 : f->plan.nfold;
+  return DSPX_OK;
+}
 
 int dspx_fold_binplan(dspx_fold* f, uint32_t* d_out)
 {
@@ -256,6 +357,7 @@ int dspx_fold_hits(dspx_fold* f, uint32_t* d_hits)
   dspx_ctx* ctx = f ? f->plan.ctx : nullptr;
   return guard(ctx, [&] {
     DSPX_REQUIRE(f && d_hits, DSPX_E_PARAM, "null argument");
+    DSPX_REQUIRE(!f->zeroed, DSPX_E_STATE, "zeroed data: hits are counted by dspx_fold_execute_zeroed");
     f->plan.hits(d_hits);
   });
 }
@@ -267,6 +369,18 @@ int dspx_fold_execute(dspx_fold* f, const float* d_in, uint64_t in_chan_stride, 
   return guard(ctx, [&] {
     DSPX_REQUIRE(f && d_in && d_prof, DSPX_E_PARAM, "null argument");
     f->plan.fold_fpt(d_in, in_chan_stride, in_pol_stride, nchan, npol, ndim, d_prof);
+  });
+}
+
+int dspx_fold_execute_zeroed(dspx_fold* f, const float* d_in, uint64_t in_chan_stride, uint64_t in_pol_stride,
+                             uint32_t nchan, uint32_t npol, uint32_t ndim, float* d_prof, uint32_t* d_hits,
+                             uint32_t hits_nchan, unsigned long long* d_ndat_folded)
+{
+  dspx_ctx* ctx = f ? f->plan.ctx : nullptr;
+  return guard(ctx, [&] {
+    DSPX_REQUIRE(f && d_in && d_prof && d_hits, DSPX_E_PARAM, "null argument");
+    f->plan.fold_fpt(d_in, in_chan_stride, in_pol_stride, nchan, npol, ndim, d_prof, d_hits, hits_nchan,
+                     d_ndat_folded);
   });
 }
 

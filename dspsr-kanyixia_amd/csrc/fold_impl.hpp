@@ -4,8 +4,17 @@
 #include <vector>
 
 #include "dspx_internal.hpp"
+#include "segments.hpp"
 
 namespace dspx {
+
+// WeightedTimeSeries weights of a block (Fold.C:700-733, 792-810): samples of
+// a zero weight are not folded unless the data are zeroed instead.
+struct FoldWeights {
+  std::vector<uint16_t> w;
+  uint32_t ndatperweight = 0;
+  uint32_t weight_idat = 0;
+};
 
 struct FoldPlan {
   dspx_ctx* ctx = nullptr;
@@ -14,6 +23,7 @@ struct FoldPlan {
   uint32_t nbin = 0;
   uint32_t nsub = 0;
   uint32_t nseg = 0;
+  uint64_t nfold = 0;  // samples the plan folds (Fold::fold's ndat_folded without zeroed data)
   DevBuf seg;   // FoldSegment[nseg]
   DevBuf plan;  // uint32[ndat]: (isub << 16) | bin, ~0 = not folded (with a sample plan)
   bool have_sample_plan = false;
@@ -23,10 +33,13 @@ struct FoldPlan {
   // Segments of the chunks (uploaded without waiting for the stream) and,
   // with sample_plan, the per-sample plan evaluated from them.
   void build(dspx_ctx* c, uint64_t ndat, double pps, uint32_t nbin, const dspx_fold_chunk* chunks,
-             uint32_t nchunk, bool sample_plan = true);
+             uint32_t nchunk, bool sample_plan = true, const FoldWeights* weights = nullptr);
   void hits(uint32_t* d_hits);
+  // zhits (zeroed data, Fold.C:911-930): hits[isub][hits_nchan][nbin] counted
+  // where value 0 of pol 0 is non-zero, the count added into *nfolded
   void fold_fpt(const float* d_in, uint64_t chan_stride, uint64_t pol_stride, uint32_t nchan,
-                uint32_t npol, uint32_t ndim, float* d_prof);
+                uint32_t npol, uint32_t ndim, float* d_prof, uint32_t* zhits = nullptr,
+                uint32_t hits_nchan = 1, unsigned long long* nfolded = nullptr);
   // first and last sub-integration touched by samples [i0, i1)
   void sub_range(uint64_t i0, uint64_t i1, uint32_t& first, uint32_t& last) const
   {
@@ -67,5 +80,21 @@ __device__ __forceinline__ uint32_t seg_find(const FoldSegment* __restrict__ seg
   }
   return lo;
 }
+
+// Where the fold kernels read each sample's key: the per-sample plan, or
+// the segments themselves (no per-sample buffer).
+struct PlanKeys {
+  const uint32_t* plan;
+  __device__ __forceinline__ uint32_t operator()(uint64_t i) const { return plan[i]; }
+};
+struct SegKeys {
+  const FoldSegment* seg;
+  uint32_t nseg;
+  double nbin_d;
+  __device__ __forceinline__ uint32_t operator()(uint64_t i) const
+  {
+    return seg_key(seg[seg_find(seg, nseg, i)], i, nbin_d);
+  }
+};
 
 }  // namespace dspx

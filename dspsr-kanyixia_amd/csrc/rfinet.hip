@@ -433,6 +433,7 @@ uint64_t splitmix(uint64_t& s)
 using namespace dspx;
 
 struct dspx_rfinet {
+  CtxRef hold;
   dspx_ctx* ctx = nullptr;
   DevBuf w, wq, m, v, grad, wt;  // wt: transposed weights of one layer
   DevBuf wbf;                    // BF16 [tap][co][ci] weights of the two 32 -> 32 layers
@@ -536,6 +537,7 @@ int dspx_rfinet_create(dspx_ctx* ctx, uint64_t seed, dspx_rfinet** out)
     DSPX_REQUIRE(ctx && out, DSPX_E_PARAM, "null argument");
     DSPX_HIP(hipSetDevice(ctx->device));
     std::unique_ptr<dspx_rfinet> N(new dspx_rfinet);
+    N->hold.reset(ctx);
     N->ctx = ctx;
     const size_t bytes = size_t(kNParam) * sizeof(float);
     for (DevBuf* b : {&N->w, &N->wq, &N->m, &N->v, &N->grad}) b->reserve(bytes);

@@ -12,6 +12,7 @@
 #include "stages.hpp"
 
 struct dspx_sk {
+  dspx::CtxRef hold;
   dspx_ctx* ctx = nullptr;
   dspx_sk_desc d{};
   dspx::DevBuf lim;  // lower[M+1], upper[M+1], symmetric[M+1]
@@ -410,6 +411,7 @@ int dspx_sk_create(dspx_ctx* ctx, const dspx_sk_desc* desc, dspx_sk** out)
     DSPX_REQUIRE(d.M >= 2, DSPX_E_PARAM, "M=%u < 2 (SKLimits::calc_limits)", d.M);
     DSPX_REQUIRE(d.Nd >= 1, DSPX_E_PARAM, "Nd=%u", d.Nd);
     auto* s = new dspx_sk;
+    s->hold.reset(ctx);
     s->ctx = ctx;
     s->d = d;
     *out = s;

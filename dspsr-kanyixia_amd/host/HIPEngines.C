@@ -260,17 +260,27 @@ FoldEngine::~FoldEngine()
   dspx_fold_destroy(plan);
   if (d_prof) (void) hipFree(d_prof);
   if (d_hits) (void) hipFree(d_hits);
+  if (d_nfolded) (void) hipFree(d_nfolded);
 }
 
 void FoldEngine::set_nbin(unsigned n)
 {
   if (n == nbin) return;
   nbin = n;
+  hits_rows = 0;  // reallocated for the new bin count by the next set_bins
+}
+
+void FoldEngine::reserve_hits(unsigned rows)
+{
+  if (rows == hits_rows && d_hits) return;
   if (d_hits) (void) hipFree(d_hits);
   d_hits = nullptr;
-  if (hipMalloc(&d_hits, nbin * sizeof(uint32_t)) != hipSuccess)
-    throw Error(FailedCall, "dsp::HIP::FoldEngine::set_nbin", "hipMalloc");
-  hits_host.assign(nbin, 0);
+  if (hipMalloc(&d_hits, size_t(rows) * nbin * sizeof(uint32_t)) != hipSuccess)
+    throw Error(FailedCall, "dsp::HIP::FoldEngine::set_bins", "hipMalloc");
+  hits_rows = rows;
+  hipStream_t s = static_cast<hipStream_t>(dspx_ctx_stream(ctx->get()));
+  if (hipMemsetAsync(d_hits, 0, size_t(rows) * nbin * sizeof(uint32_t), s) != hipSuccess)
+    throw Error(FailedCall, "dsp::HIP::FoldEngine::set_bins", "hipMemsetAsync");
 }
 
 void FoldEngine::set_bin(uint64_t, double, double)
@@ -278,29 +288,48 @@ void FoldEngine::set_bin(uint64_t, double, double)
   throw Error(InvalidState, "dsp::HIP::FoldEngine::set_bin", "this engine builds its plan with set_bins");
 }
 
+void FoldEngine::set_weights(const uint16_t* w, uint64_t nweights, unsigned ndpw, unsigned widat)
+{
+  if (w && ndpw)
+    weights.assign(w, w + nweights);
+  else
+    weights.clear();
+  ndatperweight = w ? ndpw : 0;
+  weight_idat = widat;
+}
+
 uint64_t FoldEngine::set_bins(double phi, double pps, uint64_t ndat, uint64_t idat_start)
 {
-  dspx_fold_destroy(plan);
-  plan = nullptr;
   // the plan covers the input block; only [idat_start, idat_start + ndat) folds
+  zeroed = in->get_zeroed_data();
+  reserve_hits(zeroed ? in->get_nchan() : 1);  // PhaseSeries::mixable (PhaseSeries.C:402-404)
   uint64_t block = in->get_ndat();
   dspx_fold_chunk chunk{idat_start, ndat, phi, 0, 0};
-  ctx->check(dspx_fold_create(ctx->get(), block, pps, nbin, &chunk, 1, &plan), "dsp::HIP::FoldEngine::set_bins");
-  plan_ndat = block;
-  // hits of this call, for Fold::fold's hits[ibin] += get_bin_hits(ibin)
-  hipStream_t s = static_cast<hipStream_t>(dspx_ctx_stream(ctx->get()));
-  if (hipMemsetAsync(d_hits, 0, nbin * sizeof(uint32_t), s) != hipSuccess)
-    throw Error(FailedCall, "dsp::HIP::FoldEngine::set_bins", "hipMemsetAsync");
-  ctx->check(dspx_fold_hits(plan, d_hits), "dsp::HIP::FoldEngine::set_bins");
-  if (hipMemcpyAsync(hits_host.data(), d_hits, nbin * sizeof(uint32_t), hipMemcpyDeviceToHost, s) != hipSuccess)
-    throw Error(FailedCall, "dsp::HIP::FoldEngine::set_bins", "hipMemcpyAsync");
-  ctx->synchronize();
-  ndat_folded = 0;
-  for (auto h : hits_host) ndat_folded += h;
+  dspx_fold_opts opts{weights.empty() ? nullptr : weights.data(), weights.size(), ndatperweight, weight_idat,
+                      zeroed ? 1u : 0u, 0};
+  if (!plan)
+    ctx->check(dspx_fold_create_opts(ctx->get(), block, pps, nbin, &chunk, 1, &opts, &plan),
+               "dsp::HIP::FoldEngine::set_bins");
+  else
+    ctx->check(dspx_fold_replan(plan, block, pps, nbin, &chunk, 1, &opts), "dsp::HIP::FoldEngine::set_bins");
+  if (!zeroed) ctx->check(dspx_fold_hits(plan, d_hits), "dsp::HIP::FoldEngine::set_bins");
+  ctx->check(dspx_fold_ndat_folded(plan, &ndat_folded), "dsp::HIP::FoldEngine::set_bins");
   return ndat_folded;
 }
 
-uint64_t FoldEngine::get_bin_hits(int ibin) { return hits_host.at(size_t(ibin)); }
+uint64_t FoldEngine::get_bin_hits(int) { return 0; }
+
+uint64_t FoldEngine::get_ndat_folded() const
+{
+  if (!zeroed || !d_nfolded) return ndat_folded;
+  ctx->synchronize();
+  unsigned long long total = 0;
+  if (hipMemcpy(&total, d_nfolded, sizeof(total), hipMemcpyDeviceToHost) != hipSuccess)
+    throw Error(FailedCall, "dsp::HIP::FoldEngine::get_ndat_folded", "hipMemcpy");
+  uint64_t n = total - nfolded_taken;
+  nfolded_taken = total;
+  return n;
+}
 
 void FoldEngine::fold()
 {
@@ -311,34 +340,62 @@ void FoldEngine::fold()
     if (hipMalloc(&d_prof, need * sizeof(float)) != hipSuccess)
       throw Error(FailedCall, "dsp::HIP::FoldEngine::fold", "hipMalloc");
     prof_floats = need;
-    zero();
+    hipStream_t s = static_cast<hipStream_t>(dspx_ctx_stream(ctx->get()));
+    if (hipMemsetAsync(d_prof, 0, prof_floats * sizeof(float), s) != hipSuccess)
+      throw Error(FailedCall, "dsp::HIP::FoldEngine::fold", "hipMemsetAsync");
   }
-  ctx->check(dspx_fold_execute(plan, in->get_datptr(), in->chan_span(), in->get_pol_span(), in->get_nchan(),
-                               in->get_npol(), in->get_ndim(), d_prof),
-             "dsp::HIP::FoldEngine::fold");
+  if (zeroed) {
+    if (!d_nfolded) {
+      if (hipMalloc(&d_nfolded, sizeof(unsigned long long)) != hipSuccess)
+        throw Error(FailedCall, "dsp::HIP::FoldEngine::fold", "hipMalloc");
+      hipStream_t s = static_cast<hipStream_t>(dspx_ctx_stream(ctx->get()));
+      if (hipMemsetAsync(d_nfolded, 0, sizeof(unsigned long long), s) != hipSuccess)
+        throw Error(FailedCall, "dsp::HIP::FoldEngine::fold", "hipMemsetAsync");
+      nfolded_taken = 0;
+    }
+    ctx->check(dspx_fold_execute_zeroed(plan, in->get_datptr(), in->chan_span(), in->get_pol_span(),
+                                        in->get_nchan(), in->get_npol(), in->get_ndim(), d_prof, d_hits, hits_rows,
+                                        d_nfolded),
+               "dsp::HIP::FoldEngine::fold");
+  } else {
+    ctx->check(dspx_fold_execute(plan, in->get_datptr(), in->chan_span(), in->get_pol_span(), in->get_nchan(),
+                                 in->get_npol(), in->get_ndim(), d_prof),
+               "dsp::HIP::FoldEngine::fold");
+  }
 }
 
 void FoldEngine::zero()
 {
-  if (!d_prof) return;
   hipStream_t s = static_cast<hipStream_t>(dspx_ctx_stream(ctx->get()));
-  if (hipMemsetAsync(d_prof, 0, prof_floats * sizeof(float), s) != hipSuccess)
+  if (d_prof && hipMemsetAsync(d_prof, 0, prof_floats * sizeof(float), s) != hipSuccess)
+    throw Error(FailedCall, "dsp::HIP::FoldEngine::zero", "hipMemsetAsync");
+  if (d_hits && hipMemsetAsync(d_hits, 0, size_t(hits_rows) * nbin * sizeof(uint32_t), s) != hipSuccess)
     throw Error(FailedCall, "dsp::HIP::FoldEngine::zero", "hipMemsetAsync");
 }
 
 void FoldEngine::synch(PhaseSeries* output)
 {
   // TransferPhaseSeriesCUDA D2H (FoldCUDA.cu:121-146); host profiles are FPT
-  if (!d_prof) return;
   ctx->synchronize();
-  std::vector<float> h(prof_floats);
-  if (hipMemcpy(h.data(), d_prof, prof_floats * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess)
-    throw Error(FailedCall, "dsp::HIP::FoldEngine::synch", "hipMemcpy");
-  size_t per = size_t(nbin) * output->get_ndim();
-  for (unsigned c = 0; c < output->get_nchan(); c++)
-    for (unsigned p = 0; p < output->get_npol(); p++)
-      std::copy(h.begin() + (size_t(c) * output->get_npol() + p) * per,
-                h.begin() + (size_t(c) * output->get_npol() + p + 1) * per, output->get_datptr(c, p));
+  if (d_prof) {
+    std::vector<float> h(prof_floats);
+    if (hipMemcpy(h.data(), d_prof, prof_floats * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess)
+      throw Error(FailedCall, "dsp::HIP::FoldEngine::synch", "hipMemcpy");
+    size_t per = size_t(nbin) * output->get_ndim();
+    for (unsigned c = 0; c < output->get_nchan(); c++)
+      for (unsigned p = 0; p < output->get_npol(); p++)
+        std::copy(h.begin() + (size_t(c) * output->get_npol() + p) * per,
+                  h.begin() + (size_t(c) * output->get_npol() + p + 1) * per, output->get_datptr(c, p));
+  }
+  if (d_hits) {
+    if (output->get_hits_nchan() != hits_rows) {
+      output->set_hits_nchan(hits_rows);
+      output->resize_hits();
+    }
+    if (hipMemcpy(output->get_hits(), d_hits, size_t(hits_rows) * nbin * sizeof(uint32_t), hipMemcpyDeviceToHost) !=
+        hipSuccess)
+      throw Error(FailedCall, "dsp::HIP::FoldEngine::synch", "hipMemcpy");
+  }
 }
 
 // ------------------------------------------------------- LoadToFoldEngine

@@ -238,6 +238,7 @@ void SpectralKurtosis::operate()
   }
   output->copy_configuration(input);
   output->resize(npart * M);  // output_ndat = Nblock * M
+  output->set_zeroed_data(true);  // zapped samples are zeroed (SpectralKurtosis.C:1468)
   if (!npart) return;
   engine->compute(input, sums, counts, npart);
   engine->detect(sums, counts, npart, detectors, zapmask);
@@ -404,7 +405,8 @@ void Fold::operate()
     output->copy_configuration(input);
     output->resize(folding_nbin);
     output->zero();
-    output->set_hits_nchan(1);
+    // zeroed data: hits per channel (PhaseSeries::mixable, PhaseSeries.C:402-404)
+    output->set_hits_nchan(input->get_zeroed_data() ? input->get_nchan() : 1);
     output->resize_hits();
     engine->zero();
   }
@@ -419,15 +421,29 @@ void Fold::operate()
   uint64_t ndat_folded = engine->set_bins(phi, pps, ndat_fold, idat_start);
   unsigned* hits = output->get_hits();
   for (unsigned ibin = 0; ibin < folding_nbin; ibin++) hits[ibin] += unsigned(engine->get_bin_hits(int(ibin)));
+  // with zeroed data ndat_folded is 0 here; the samples the engine counts
+  // are added by get_result (Fold.C:862-887, 931-942)
   output->integration_length += double(ndat_folded) / input->get_rate();
   output->ndat_total += ndat_fold;
   output->folding_period = folding_period;
   engine->fold();
+  if (input->get_zeroed_data()) {
+    zeroed_rate = input->get_rate();
+    zeroed_nchan = input->get_nchan();
+  }
 }
 
 PhaseSeries* Fold::get_result()
 {
-  if (engine) engine->synch(output);
+  if (engine) {
+    engine->synch(output);
+    // zeroed data: every channel's non-zero samples were counted, so the time
+    // folded is their number over nchan (Fold.C:931-942)
+    if (zeroed_nchan) {
+      output->integration_length += double(engine->get_ndat_folded()) / (zeroed_rate * zeroed_nchan);
+      zeroed_nchan = 0;
+    }
+  }
   return output;
 }
 

@@ -135,23 +135,39 @@ class FoldEngine : public Fold::Engine {
   ~FoldEngine() override;
   void set_nbin(unsigned nbin) override;
   void set_bin(uint64_t idat, double ibin, double bins_per_samp) override;
+  //! Replans the engine's fold for this block (no allocation once the
+  //! buffers fit, no wait for the stream) and adds the plan's hits into the
+  //! device totals; returns Fold::fold's ndat_folded (0 with zeroed data).
   uint64_t set_bins(double phi, double phase_per_sample, uint64_t ndat, uint64_t idat_start) override;
+  //! 0: the hits are accumulated on the device and handed over by synch.
   uint64_t get_bin_hits(int ibin) override;
-  uint64_t get_ndat_folded() const override { return ndat_folded; }
+  //! With zeroed data: the samples counted on the device since the last
+  //! call (waits for the stream); otherwise the last set_bins count.
+  uint64_t get_ndat_folded() const override;
   PhaseSeries* get_profiles() override { return out; }
   void fold() override;
+  //! Copies the profiles and the accumulated hits ([hits_nchan][nbin]) into
+  //! the output (FoldCUDA.cu:121-146).
   void synch(PhaseSeries* output) override;
   void zero() override;
+  //! Weights of the next set_bins' block (WeightedTimeSeries), or none.
+  void set_weights(const uint16_t* w, uint64_t nweights, unsigned ndatperweight, unsigned weight_idat);
 
  private:
+  void reserve_hits(unsigned hits_nchan);
   Context* ctx;
   unsigned nbin = 0;
   dspx_fold* plan = nullptr;
-  uint64_t ndat_folded = 0, plan_ndat = 0;
-  std::vector<uint32_t> hits_host;
+  uint64_t ndat_folded = 0;
+  bool zeroed = false;
+  std::vector<uint16_t> weights;
+  unsigned ndatperweight = 0, weight_idat = 0;
   float* d_prof = nullptr;  // [nchan][npol][nbin*ndim] accumulated on the device
   size_t prof_floats = 0;
-  uint32_t* d_hits = nullptr;
+  uint32_t* d_hits = nullptr;  // [hits_nchan][nbin] accumulated on the device
+  unsigned hits_rows = 0;
+  unsigned long long* d_nfolded = nullptr;  // zeroed data: samples counted by the device
+  mutable unsigned long long nfolded_taken = 0;
 };
 
 //! The LoadToFold chain Unpack -> Convolution -> Detection -> Fold fused

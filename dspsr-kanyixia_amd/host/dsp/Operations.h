@@ -369,6 +369,8 @@ class Fold {
   unsigned folding_nbin = 0;
   uint64_t idat_start = 0, ndat_fold = 0;
   bool limits_set = false;
+  double zeroed_rate = 0;
+  unsigned zeroed_nchan = 0;
 };
 
 }  // namespace dsp

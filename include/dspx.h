@@ -237,6 +237,9 @@ int dspx_fb_geometry(const dspx_fb* fb, uint64_t ndat, uint64_t* npart, uint64_t
    output [nchan*nsub][npol][npart*nkeep]. */
 int dspx_fb_perform(dspx_fb* fb, const float* d_in, uint64_t in_chan_stride, uint64_t in_pol_stride,
                     uint64_t ndat, float* d_out, uint64_t out_chan_stride, uint64_t out_pol_stride);
+/* The number of FFT-library batches the last dspx_fb_perform ran (0 when it
+   ran the one-kernel form); for checking the batch loop. */
+int dspx_fb_batches(const dspx_fb* fb, uint32_t* nbatch);
 
 /* --------------------------------------------------------------- detection */
 
@@ -272,6 +275,33 @@ typedef struct dspx_fold dspx_fold;
 int dspx_fold_create(dspx_ctx* ctx, uint64_t ndat, double phase_per_sample, uint32_t nbin,
                      const dspx_fold_chunk* chunks, uint32_t nchunk, dspx_fold** out);
 void dspx_fold_destroy(dspx_fold* fold);
+/* Options of Fold::fold beyond the phase model (Fold.C:662-740):
+   WeightedTimeSeries weights (sample idat of the block has weight
+   weights[(idat + weight_idat) / ndatperweight]; a zero weight marks the
+   sample not folded, Fold.C:792-810, 826-827), and zeroed data
+   (input->get_zeroed_data(): the weights mark nothing and the hits are
+   counted by dspx_fold_execute_zeroed, Fold.C:893-930).  weights is read
+   during the call only. */
+typedef struct {
+  const uint16_t* weights;   /* host; NULL with ndatperweight = 0 */
+  uint64_t nweights;
+  uint32_t ndatperweight;
+  uint32_t weight_idat;
+  uint32_t zeroed_samples;
+  uint32_t reserved;
+} dspx_fold_opts;
+int dspx_fold_create_opts(dspx_ctx* ctx, uint64_t ndat, double phase_per_sample, uint32_t nbin,
+                          const dspx_fold_chunk* chunks, uint32_t nchunk, const dspx_fold_opts* opts,
+                          dspx_fold** out);
+/* Rebuilds the plan of an existing fold for the next block (the per-block
+   Fold::fold call), reusing its device buffers; nothing waits for the
+   stream. */
+int dspx_fold_replan(dspx_fold* fold, uint64_t ndat, double phase_per_sample, uint32_t nbin,
+                     const dspx_fold_chunk* chunks, uint32_t nchunk, const dspx_fold_opts* opts);
+/* Host-side count of the samples the plan folds (Fold::fold's ndat_folded,
+   Fold.C:831-836): the chunks' samples less those of zero weight; 0 with
+   zeroed data, whose count is made on the device by dspx_fold_execute_zeroed. */
+int dspx_fold_ndat_folded(const dspx_fold* fold, uint64_t* ndat_folded);
 /* Per-sample plan: (isub << 16) | ibin, or 0xffffffff for samples outside
    every chunk.  d_out holds ndat uint32. */
 int dspx_fold_binplan(dspx_fold* fold, uint32_t* d_out);
@@ -289,6 +319,14 @@ int dspx_fold_nsegments_host(double phi, double phase_per_sample, uint64_t ndat,
 int dspx_fold_execute(dspx_fold* fold, const float* d_in, uint64_t in_chan_stride,
                       uint64_t in_pol_stride, uint32_t nchan, uint32_t npol, uint32_t ndim,
                       float* d_prof);
+/* Engine::fold on zeroed data (Fold.C:893-930; FoldCUDA.cu:424-583): as
+   dspx_fold_execute, and hits[isub][hits_nchan][nbin] += the samples of pol 0
+   whose value 0 is non-zero (hits_nchan = nchan: per channel; 1: all
+   channels into one row), their number added into *d_ndat_folded (device). */
+int dspx_fold_execute_zeroed(dspx_fold* fold, const float* d_in, uint64_t in_chan_stride,
+                             uint64_t in_pol_stride, uint32_t nchan, uint32_t npol, uint32_t ndim,
+                             float* d_prof, uint32_t* d_hits, uint32_t hits_nchan,
+                             unsigned long long* d_ndat_folded);
 
 /* ------------------------------------------------------- spectral kurtosis */
 /* One resolution of dsp::SpectralKurtosis (noverlap 1, every channel

@@ -166,3 +166,33 @@ def test_convolution_truth_mode_tracks_fp64(ctx, dspx, oracle):
     err = rel_rms(res, truth)
     print("truth-mode rel rms vs FP64 oracle %.3g" % err)
     assert err < 1e-5
+
+
+@pytest.mark.parametrize("form", ["fused", "plain"])
+def test_convolution_zero_dm_output(ctx, dspx, oracle, form):
+    """Engine::perform with a zero-DM output (Convolution.C:792-799,
+    ConvolutionCUDA.cu:662-673; test_Convolution.cpp:151-186): the zero-DM
+    series is the input's kept samples bit for bit, and the dedispersed one
+    matches the oracle."""
+    import torch
+    rng = np.random.default_rng(21)
+    nchan, npol, nfft, pos, neg = 2, 2, 4096, 300, 250
+    ndat = 5 * (nfft - pos - neg) + pos + neg + 17
+    x = (rng.standard_normal((nchan, npol, ndat)) + 1j * rng.standard_normal((nchan, npol, ndat))).astype(np.complex64)
+    resp = oracle.response(1400.0, 8.0, nchan, 10.0, nfft, 0)
+    expect, zdm_o = oracle.convolve(x, resp, pos, neg, zero_dm=True)
+    step = nfft - pos - neg
+    npart = (ndat - pos - neg) // step
+    nout = npart * step
+    f = dspx.FORM_FUSED if form == "fused" else dspx.FORM_PLAIN
+    conv = dspx.Convolution(ctx, nchan, npol, nfft, pos, neg, chirp_mode=dspx.CHIRP_TABLE, form=f)
+    conv.set_response(resp)
+    out = torch.zeros((nchan, npol, nout, 2), dtype=torch.float32, device="cuda")
+    zdm = torch.zeros((nchan, npol, nout, 2), dtype=torch.float32, device="cuda")
+    print("zero-DM", form, flush=True)
+    conv.perform(to_dev(x.view(np.float32)), npol * ndat * 2, ndat * 2, ndat, out, npol * nout * 2, nout * 2, npart,
+                 zero_dm=zdm)
+    got_zdm = from_dev(zdm).view(np.complex64)[..., 0]
+    assert np.array_equal(got_zdm, zdm_o)
+    assert rel_rms(from_dev(out).view(np.complex64)[..., 0], expect) < 1e-5
+    conv.close()

@@ -10,7 +10,7 @@ from helpers import from_dev, rel_rms, to_dev
 pytestmark = pytest.mark.gpu
 
 
-def _run(dspx, ctx, x, nsub, fres, pos, neg, resp):
+def _run(dspx, ctx, x, nsub, fres, pos, neg, resp, batches=None):
     import torch
     nchan, npol, ndat = x.shape
     F = dspx.Filterbank(ctx, nchan, npol, nsub, fres, pos, neg)
@@ -19,6 +19,8 @@ def _run(dspx, ctx, x, nsub, fres, pos, neg, resp):
     out = torch.zeros((nchan * nsub, npol, nout, 2), dtype=torch.float32, device="cuda")
     F.perform(to_dev(x.view(np.float32)), npol * ndat * 2, ndat * 2, ndat, out, npol * nout * 2, nout * 2)
     got = from_dev(out).view(np.complex64)[..., 0]
+    if batches is not None:
+        batches.append(F.batches())
     F.close()
     return got
 
@@ -67,15 +69,18 @@ def test_filterbank_several_batches(ctx, dspx, oracle, monkeypatch):
     x = (rng.standard_normal((nchan, npol, ndat)) + 1j * rng.standard_normal((nchan, npol, ndat))).astype(np.complex64)
     expect = oracle.filterbank(x, nsub, fres, pos, neg, resp)
     monkeypatch.setenv("DSPX_FB_SCRATCH_MB", "1")
+    monkeypatch.setenv("DSPX_FB_FORM", "library")  # 2048-point windows would otherwise run as one kernel
     print("filterbank several batches", flush=True)
-    got = _run(dspx, ctx, x, nsub, fres, pos, neg, resp)
+    nb = []
+    got = _run(dspx, ctx, x, nsub, fres, pos, neg, resp, nb)
+    assert nb == [4]
     assert got.shape == expect.shape and expect.shape[2] == 41 * (fres - pos - neg)
     assert rel_rms(got, expect) < 1e-5
 
 
 @pytest.mark.parametrize("form", ["fused", "library"])
 @pytest.mark.parametrize("nsub,fres", [(16, 64), (8, 128), (8, 256), (32, 128), (16, 256), (64, 64), (32, 32),
-                                       (64, 32), (128, 32), (2, 512), (4, 512), (8, 512)])
+                                       (64, 32), (128, 32), (2, 512), (4, 512), (8, 512), (4, 256), (32, 64)])
 def test_filterbank_window_lengths(ctx, dspx, oracle, monkeypatch, form, nsub, fres):
     # 1024-4096-point windows: the one-kernel form and the library form
     # (DSPX_FB_FORM=library) against the oracle, 2 channels x 2 pols
@@ -91,7 +96,9 @@ def test_filterbank_window_lengths(ctx, dspx, oracle, monkeypatch, form, nsub, f
     x = (rng.standard_normal((nchan, npol, ndat)) + 1j * rng.standard_normal((nchan, npol, ndat))).astype(np.complex64)
     expect = oracle.filterbank(x, nsub, fres, pos, neg, resp)
     print("filterbank", form, nsub, fres, flush=True)
-    got = _run(dspx, ctx, x, nsub, fres, pos, neg, resp)
+    nb = []
+    got = _run(dspx, ctx, x, nsub, fres, pos, neg, resp, nb)
+    assert (nb[0] == 0) == (form == "fused")
     assert got.shape == expect.shape
     e = rel_rms(got, expect)
     print("rel rms", e, flush=True)
