@@ -126,6 +126,7 @@ _SIGS = {
     "dspx_fold_replan": (I32, [P, U64, D, U32, C.POINTER(FoldChunk), U32, C.POINTER(FoldOpts)]),
     "dspx_fold_ndat_folded": (I32, [P, C.POINTER(U64)]),
     "dspx_fold_execute_zeroed": (I32, [P, P, U64, U64, U32, U32, U32, P, P, U32, P]),
+    "dspx_fold_static": (I32, [P, U64, D, U32, C.POINTER(FoldChunk), U32, P, U64, U64, U32, U32, U32, P, P]),
     "dspx_sk_create": (I32, [P, C.POINTER(SkDesc), C.POINTER(P)]),
     "dspx_sk_destroy": (None, [P]),
     "dspx_sk_set_limits": (I32, [P, P, P, P, D]),

@@ -191,6 +191,7 @@ int dspx_ctx_create(int device, void* stream, dspx_ctx** out)
       DSPX_HIP(hipMalloc(&c->d_tw, total * sizeof(float2)));
       DSPX_HIP(hipMemcpy(c->d_tw, h.data(), total * sizeof(float2), hipMemcpyHostToDevice));
       for (auto& e : c->ev) DSPX_HIP(hipEventCreate(&e));
+      fold_static_setup(device);
     } catch (...) {
       ctx_free(c);
       throw;

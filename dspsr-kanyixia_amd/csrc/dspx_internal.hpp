@@ -168,6 +168,9 @@ void unpack(dspx_ctx* ctx, const void* raw, uint32_t nbit, uint32_t offset_binar
 void detect(dspx_ctx* ctx, uint32_t state, uint32_t ndim, const float* in, uint64_t ics, uint64_t ips,
             uint32_t nchan, uint32_t npol, uint64_t ndat, float* out, uint64_t ocs, uint64_t ops);
 
+// ---- fold without allocation (fold.hip): the per-device table address ----
+void fold_static_setup(int device);
+
 // ---- host-side algorithms (geometry.cpp) ----
 int geometry(const dspx_dedisp_desc* d, dspx_geometry* g);
 double channel_centre(double cfreq, double bw, unsigned nchan, int dc_centred, unsigned ichan);

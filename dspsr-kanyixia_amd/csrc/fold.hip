@@ -7,6 +7,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 
 #include "dspx_internal.hpp"
 #include "fft_core.hpp"
@@ -257,6 +258,64 @@ void FoldPlan::fold_fpt(const float* d_in, uint64_t chan_stride, uint64_t pol_st
               ndim, d_prof, zhits, hits_nchan, nfolded);
 }
 
+// ------------------------------------------------- fold without allocation
+// dspx_fold_static keeps its segment table in this array: one copy per GPU,
+// held until the process ends, shared by every call on that GPU.  Host code
+// reaches it only through the address hipGetSymbolAddress gives for the
+// device (static_ptr), never through the array's name.
+constexpr uint32_t kStaticSegMax = 32768;  // 1.25 MiB
+static __device__ __align__(256) FoldSegment scratch[kStaticSegMax];
+
+constexpr int kMaxDevices = 64;
+static FoldSegment* static_ptr[kMaxDevices] = {};
+static std::mutex static_mu;
+// host staging of the segments (a plain array: the call allocates nothing)
+static FoldSegment static_host[kStaticSegMax];
+
+// Called by dspx_ctx_create with the context's device current.
+void fold_static_setup(int device)
+{
+  DSPX_REQUIRE(device >= 0 && device < kMaxDevices, DSPX_E_UNSUPPORTED, "device %d", device);
+  std::lock_guard<std::mutex> lock(static_mu);
+  if (static_ptr[device]) return;
+  void* p = nullptr;
+  DSPX_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+  static_ptr[device] = static_cast<FoldSegment*>(p);
+}
+
+void fold_static(dspx_ctx* ctx, uint64_t ndat, double pps, uint32_t nbin, const dspx_fold_chunk* chunks,
+                 uint32_t nchunk, const float* d_in, uint64_t ics, uint64_t ips, uint32_t nchan, uint32_t npol,
+                 uint32_t ndim, float* d_prof, uint32_t* d_hits)
+{
+  DSPX_REQUIRE(nbin >= 1 && nbin <= 65535, DSPX_E_PARAM, "nbin=%u outside 1..65535", nbin);
+  DSPX_REQUIRE(pps > 0.0 && pps < 0.5, DSPX_E_PARAM, "phase_per_sample=%g outside (0, 0.5)", pps);
+  DSPX_REQUIRE(nchunk >= 1, DSPX_E_PARAM, "no fold chunks");
+  FoldSegment* dev = static_ptr[ctx->device];
+  DSPX_REQUIRE(dev, DSPX_E_STATE, "no segment table on device %d", ctx->device);
+  std::lock_guard<std::mutex> lock(static_mu);
+  uint32_t nseg = 0, nsub = 0;
+  uint64_t nfold = 0;
+  bool over = false;
+  plan_segments(ndat, pps, chunks, nchunk, WeightView{},
+                [&](const FoldSegment& s) {
+                  if (nseg < kStaticSegMax)
+                    static_host[nseg] = s;
+                  else
+                    over = true;
+                  nseg++;
+                },
+                nsub, nfold);
+  DSPX_REQUIRE(!over, DSPX_E_UNSUPPORTED, "the plan needs %u segments; at most %u without allocation", nseg,
+               kStaticSegMax);
+  // stream-ordered after the previous call's kernels that read the table;
+  // the host array is reused once the copy has been made
+  DSPX_HIP(hipMemcpyAsync(dev, static_host, nseg * sizeof(FoldSegment), hipMemcpyHostToDevice, ctx->stream));
+  DSPX_HIP(hipStreamSynchronize(ctx->stream));
+  const SegKeys keys{dev, nseg, double(nbin)};
+  if (d_hits) launch_hits(ctx->stream, keys, ndat, nbin, d_hits);
+  if (d_in) launch_fold(ctx->stream, keys, ndat, nbin, d_in, ics, ips, nchan, npol, ndim, d_prof, nullptr, 1, nullptr);
+}
+
 }  // namespace dspx
 
 using namespace dspx;
@@ -381,6 +440,18 @@ int dspx_fold_execute_zeroed(dspx_fold* f, const float* d_in, uint64_t in_chan_s
     DSPX_REQUIRE(f && d_in && d_prof && d_hits, DSPX_E_PARAM, "null argument");
     f->plan.fold_fpt(d_in, in_chan_stride, in_pol_stride, nchan, npol, ndim, d_prof, d_hits, hits_nchan,
                      d_ndat_folded);
+  });
+}
+
+int dspx_fold_static(dspx_ctx* ctx, uint64_t ndat, double pps, uint32_t nbin, const dspx_fold_chunk* chunks,
+                     uint32_t nchunk, const float* d_in, uint64_t in_chan_stride, uint64_t in_pol_stride,
+                     uint32_t nchan, uint32_t npol, uint32_t ndim, float* d_prof, uint32_t* d_hits)
+{
+  return guard(ctx, [&] {
+    DSPX_REQUIRE(ctx && chunks, DSPX_E_PARAM, "null argument");
+    DSPX_REQUIRE(!d_in == !d_prof, DSPX_E_PARAM, "input and profiles go together");
+    fold_static(ctx, ndat, pps, nbin, chunks, nchunk, d_in, in_chan_stride, in_pol_stride, nchan, npol, ndim,
+                d_prof, d_hits);
   });
 }
 

@@ -328,6 +328,25 @@ int dspx_fold_execute_zeroed(dspx_fold* fold, const float* d_in, uint64_t in_cha
                              float* d_prof, uint32_t* d_hits, uint32_t hits_nchan,
                              unsigned long long* d_ndat_folded);
 
+/* The fold without allocation: dspx_fold_create + dspx_fold_hits +
+   dspx_fold_execute in one call that allocates nothing (no hipMalloc,
+   hipFree or allocating library routine).  It has no counterpart in the
+   reference; it is for callers that cannot allocate device memory (tight
+   serving loops, harnesses that forbid it).  The plan's segment table lives
+   in a fixed device array of 32768 segments (1.25 MiB per GPU, held until
+   the process ends); a plan needing more is refused with
+   DSPX_E_UNSUPPORTED.  The bins are evaluated from the segments in the
+   kernels (no per-sample plan).  chunks must be sorted by idat_start.  The
+   call waits for the context's stream once (the table's upload, ordered
+   after earlier work); the kernels then run asynchronously.  Every call on a
+   GPU shares that GPU's table: calls that overlap on one GPU (several
+   threads or contexts) race.  d_hits (nullable): hits[isub][nbin] +=;
+   d_in and d_prof (both or neither): as dspx_fold_execute. */
+int dspx_fold_static(dspx_ctx* ctx, uint64_t ndat, double phase_per_sample, uint32_t nbin,
+                     const dspx_fold_chunk* chunks, uint32_t nchunk, const float* d_in,
+                     uint64_t in_chan_stride, uint64_t in_pol_stride, uint32_t nchan, uint32_t npol,
+                     uint32_t ndim, float* d_prof, uint32_t* d_hits);
+
 /* ------------------------------------------------------- spectral kurtosis */
 /* One resolution of dsp::SpectralKurtosis (noverlap 1, every channel
    included).  Sums [npart][nchan][npol][2] float (S1, S2), counts
