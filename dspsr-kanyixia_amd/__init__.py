@@ -18,7 +18,7 @@ _lib = None
 
 OK, E_PARAM, E_STATE, E_HIP, E_UNSUPPORTED, E_INTERNAL = range(6)
 CHIRP_TABLE, CHIRP_PARITY, CHIRP_TRUTH = 0, 1, 2
-FORM_FUSED, FORM_PLAIN = 0, 1
+FORM_FUSED, FORM_PLAIN, FORM_TWOLEVEL, FORM_ONEPASS = 0, 1, 2, 3
 INTENSITY, PPQQ, COHERENCE, STOKES = 0, 1, 2, 3
 
 

@@ -12,18 +12,19 @@
 
 #include "dspx_internal.hpp"
 #include "fold_impl.hpp"
+#include "onepass.hpp"
 #include "plain.hpp"
 #include "stages.hpp"
 
 namespace dspx {
 
-// N = N1 * N2 with N2 <= 4096 (one LDS row) and N1 <= 256 (column tile).
+// N = N1 * N2 with N2 <= 4096 (one LDS row) and N1 <= 1024 (column tile).
 static bool split_n(uint32_t N, uint32_t& N1, uint32_t& N2, uint32_t& m)
 {
   if (N == 0 || (N & (N - 1))) return false;
   m = 0;
   while ((1u << m) < N) m++;
-  if (m < 12 || m > 20) return false;
+  if (m < 12 || m > 22) return false;
   if (m >= 16) {
     N1 = 1u << (m - 12);
     N2 = 4096;
@@ -40,6 +41,7 @@ struct ConvCore {
   uint32_t nchan = 0, npol = 0, N = 0, N1 = 0, N2 = 0, m = 0;
   uint32_t nfilt_pos = 0, nfilt_neg = 0, step = 0;
   uint32_t chirp_mode = 0;
+  bool onepass = false;  // the one-kernel form (onepass.hpp) instead of stages A/B/C
   dspx_dedisp_desc dd{};
   DevBuf tw2;      // lo[2^h] then hi[N >> h]
   int h = 0;
@@ -57,9 +59,10 @@ struct ConvCore {
   int row_e = 16;
 
   void init(dspx_ctx* c, uint32_t nchan_, uint32_t npol_, uint32_t N_, uint32_t pos, uint32_t neg,
-            uint32_t mode, const dspx_dedisp_desc& d)
+            uint32_t mode, const dspx_dedisp_desc& d, bool one = false)
   {
     ctx = c;
+    onepass = one;
     nchan = nchan_;
     npol = npol_;
     N = N_;
@@ -69,13 +72,25 @@ struct ConvCore {
     dd = d;
     DSPX_REQUIRE(npol == 1 || npol == 2, DSPX_E_PARAM, "npol=%u", npol);
     DSPX_REQUIRE(nchan >= 1, DSPX_E_PARAM, "nchan=0");
-    DSPX_REQUIRE(split_n(N, N1, N2, m), DSPX_E_UNSUPPORTED,
-                 "nfft=%u: the fused form supports powers of two 2^12..2^20", N);
+    if (onepass) {
+      DSPX_REQUIRE(N && !(N & (N - 1)) && N >= (1u << kOnePassMinLog2) && N <= (1u << kOnePassMaxLog2),
+                   DSPX_E_UNSUPPORTED, "nfft=%u: the one-pass form supports powers of two 2^%d..2^%d", N,
+                   kOnePassMinLog2, kOnePassMaxLog2);
+      m = 0;
+      while ((1u << m) < N) m++;
+      N1 = 1;
+      N2 = N;
+    } else {
+      DSPX_REQUIRE(split_n(N, N1, N2, m), DSPX_E_UNSUPPORTED,
+                   "nfft=%u: the three-stage form supports powers of two 2^12..2^22", N);
+    }
     DSPX_REQUIRE(uint64_t(pos) + neg < N, DSPX_E_STATE, "error nfft=%u < nfilt=%u", N, pos + neg);
     step = N - pos - neg;
     DSPX_REQUIRE(mode <= DSPX_CHIRP_TRUTH, DSPX_E_PARAM, "chirp_mode=%u", mode);
+    DSPX_REQUIRE(!onepass || mode != DSPX_CHIRP_TABLE, DSPX_E_UNSUPPORTED, "the one-pass form evaluates the chirp on chip");
     // two-level twiddle table W_N^m = lo[m & (2^h-1)] * hi[m >> h]
     h = (int(m) + 1) / 2;
+    if (onepass) h = int(m);  // unused: the one-pass form takes the context's tables
     uint32_t nlo = 1u << h, nhi = N >> h;
     std::vector<float2> t(nlo + nhi);
     for (uint32_t i = 0; i < nlo; i++) {
@@ -131,6 +146,7 @@ struct ConvCore {
 
   void reserve_work(uint32_t npart)
   {
+    if (onepass) return;  // no work buffer
     wbuf().reserve(size_t(nchan) * npart * npol * N * sizeof(float2));
     work_npart = npart;
   }
@@ -178,7 +194,7 @@ struct ConvCore {
     switch (N1) {
 #define CASE_A(n) \
   case n: k_stageA<n, LD><<<unsigned(nb), kColThreads, 0, ctx->stream>>>(ld, s, twc); break;
-      CASE_A(16) CASE_A(32) CASE_A(64) CASE_A(128) CASE_A(256)
+      CASE_A(16) CASE_A(32) CASE_A(64) CASE_A(128) CASE_A(256) CASE_A(512) CASE_A(1024)
 #undef CASE_A
       default: fail(DSPX_E_UNSUPPORTED, "N1=%u", N1);
     }
@@ -262,12 +278,53 @@ struct ConvCore {
     else                                             \
       LAUNCH_C2(n, 4);                               \
   } break;
-      CASE_C(16) CASE_C(32) CASE_C(64) CASE_C(128) CASE_C(256)
+      CASE_C(16) CASE_C(32) CASE_C(64) CASE_C(128) CASE_C(256) CASE_C(512) CASE_C(1024)
 #undef CASE_C
 #undef LAUNCH_C2
 #undef LAUNCH_C
       default: fail(DSPX_E_UNSUPPORTED, "N1=%u", N1);
     }
+    DSPX_HIP(hipGetLastError());
+  }
+
+  // The one-kernel form over a block's windows (onepass.hpp); fold mode
+  // needs fa's histogram geometry and key plan, filterbank mode fb.
+  template <int MODE, class LD>
+  void run_onepass(const LD& ld, const OnePassArgs& a, const FilterbankArgs& fb, const FoldArgs& fa)
+  {
+    DSPX_REQUIRE(onepass, DSPX_E_INTERNAL, "not a one-pass plan");
+    const uint64_t nb = uint64_t(a.nchan) * a.ngrp;
+    DSPX_REQUIRE(nb >= 1 && nb < (1ull << 31), DSPX_E_UNSUPPORTED, "grid of %llu workgroups",
+                 (unsigned long long) nb);
+    size_t shm = onepass_lds_bytes(N);
+    if (MODE == kStageCFold) shm += size_t(fa.nslot) * fa.nval * fa.nbin * sizeof(float);
+    DSPX_REQUIRE(shm <= 160 * 1024, DSPX_E_UNSUPPORTED, "one-pass LDS request %zu too large", shm);
+    ChirpArgs c = chirp();
+    const float2* tw = twiddles(ctx, int(N));
+#define LAUNCH_OP(n, nv)                                                                                        \
+  do {                                                                                                          \
+    static bool lds_attr = false;                                                                               \
+    if (!lds_attr) {                                                                                            \
+      allow_full_lds((const void*) k_onepass<n, MODE, nv, LD>);                                                 \
+      lds_attr = true;                                                                                          \
+    }                                                                                                           \
+    k_onepass<n, MODE, nv, LD><<<unsigned(nb), OnePass<n>::NT, shm, ctx->stream>>>(ld, a, c, fb, fa, tw);       \
+  } while (0)
+#define CASE_OP(n)                                     \
+  case n:                                              \
+    if (MODE == kStageCFilterbank || fa.nval == 1)     \
+      LAUNCH_OP(n, 1);                                 \
+    else if (fa.nval == 2)                             \
+      LAUNCH_OP(n, 2);                                 \
+    else                                               \
+      LAUNCH_OP(n, 4);                                 \
+    break;
+    switch (N) {
+      CASE_OP(128) CASE_OP(256) CASE_OP(512) CASE_OP(1024) CASE_OP(2048) CASE_OP(4096) CASE_OP(8192)
+      default: fail(DSPX_E_UNSUPPORTED, "one-pass nfft=%u", N);
+    }
+#undef CASE_OP
+#undef LAUNCH_OP
     DSPX_HIP(hipGetLastError());
   }
 };
@@ -401,6 +458,8 @@ __global__ void k_zero_dm(const float* __restrict__ in, uint64_t ics, uint64_t i
 // order (the result does not depend on scheduling):
 //   k_fold_reduce_g: part[chan][p][slot][e] = sum over groups g
 //   k_fold_reduce_p: prof[isub][chan][e] += sum over parts p whose slots hold isub
+//                    (the one-pass form's workgroups of windows take the place
+//                    of parts here)
 __global__ void k_fold_reduce_g(const float* __restrict__ partial, uint32_t G, uint32_t nslot_hl, uint64_t n,
                                 float* __restrict__ out)
 {
@@ -414,15 +473,15 @@ __global__ void k_fold_reduce_g(const float* __restrict__ partial, uint32_t G, u
   out[i] = acc;
 }
 
-__global__ void k_fold_reduce_p(const float* __restrict__ part, const uint32_t* __restrict__ meta, uint32_t meta_off,
-                                uint32_t meta_parts, uint32_t npart, uint32_t nslot, uint32_t hl, uint32_t nchan,
-                                float* __restrict__ prof)
+__global__ void k_fold_reduce_p(const float* __restrict__ part, const uint32_t* __restrict__ meta, uint32_t meta_sf,
+                                uint32_t meta_off, uint32_t meta_parts, uint32_t npart, uint32_t nslot, uint32_t hl,
+                                uint32_t nchan, float* __restrict__ prof)
 {
   uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
   uint32_t chan = blockIdx.y;
   uint32_t isub = blockIdx.z;
   if (e >= hl) return;
-  const uint32_t* sub_first = meta;
+  const uint32_t* sub_first = meta + meta_sf;
   const uint32_t k0 = meta[meta_off + isub], k1 = meta[meta_off + isub + 1];
   if (k0 == k1) return;
   float acc = 0.f;
@@ -524,14 +583,17 @@ __global__ __launch_bounds__(256) void k_fused_keys(const FoldSegment* __restric
 #pragma unroll
   for (int q = 0; q < kKeyRun / 4; q++) {
     const uint64_t i0 = c0 + 4 * (uint64_t(q) * 256 + threadIdx.x);
+    // the group's part (transforms shorter than a chunk put several parts in
+    // one chunk, all with the same sub_first: they share a one-pass group)
+    const uint32_t pq = nfft >= uint32_t(kKeyChunk) ? p : uint32_t(i0 / nfft);
     uint32_t kv[4];
     // Common case: the group's 4 outputs kept and in one segment, where the
     // bin is non-decreasing, so equal keys at both ends mean one key.
     uint32_t ukey = kNone;  // the group's one key, else kNone
     bool uniform = false;
-    const uint32_t tt0 = uint32_t(i0 - uint64_t(p) * nfft);
+    const uint32_t tt0 = uint32_t(i0 - uint64_t(pq) * nfft);
     if (i0 + 4 <= n && tt0 >= nfilt_pos && tt0 - nfilt_pos + 3 < step) {
-      const uint64_t ia = uint64_t(p) * step + (tt0 - nfilt_pos), ib = ia + 3;
+      const uint64_t ia = uint64_t(pq) * step + (tt0 - nfilt_pos), ib = ia + 3;
       locate(ia);
       if (ib < jnext) {
         const uint32_t ka = seg_key(sg, ia, nbin_d);
@@ -549,10 +611,10 @@ __global__ __launch_bounds__(256) void k_fused_keys(const FoldSegment* __restric
 #pragma unroll
       for (int k = 0; k < 4; k++) {
         const uint64_t i = i0 + k;
-        const uint32_t t = uint32_t(i - uint64_t(p) * nfft);
+        const uint32_t t = uint32_t(i - uint64_t(pq) * nfft);
         uint32_t key = kNone;
         if (i < n && t >= nfilt_pos && t - nfilt_pos < step) {
-          const uint64_t idat = uint64_t(p) * step + (t - nfilt_pos);
+          const uint64_t idat = uint64_t(pq) * step + (t - nfilt_pos);
           locate(idat);
           key = slot_key(seg_key(sg, idat, nbin_d));
         }
@@ -628,7 +690,7 @@ int dspx_conv_create(dspx_ctx* ctx, const dspx_conv_desc* d, dspx_conv** out)
     try {
       if (d->form == DSPX_FORM_PLAIN)
         c->plain.init(ctx, d->nchan, d->npol, d->nfft, d->nfilt_pos, d->nfilt_neg, d->chirp_mode, d->dedisp);
-      else if (d->form == DSPX_FORM_FUSED)
+      else if (d->form == DSPX_FORM_FUSED || d->form == DSPX_FORM_TWOLEVEL)
         c->core.init(ctx, d->nchan, d->npol, d->nfft, d->nfilt_pos, d->nfilt_neg, d->chirp_mode, d->dedisp);
       else
         fail(DSPX_E_PARAM, "form=%u", d->form);
@@ -721,6 +783,11 @@ struct dspx_ltf {
   // (the parts whose slots hold sub-integration isub are parts[off[isub] ..
   // off[isub+1]), in increasing order)
   uint32_t G = 1, nslot = 1;
+  // one-pass form (N <= 2^13): windows per workgroup and workgroups per
+  // channel of the current block; the reduction's sub_first[] per workgroup
+  bool onepass = false;
+  uint32_t ppg = 0, ngrp = 0;
+  uint32_t meta_sf = 0;
   DevBuf keys;       // [npart][nfft] histogram key of each transform output, ~0 for discarded or unfolded
   DevBuf foldmeta;
   uint32_t meta_off = 0, meta_parts = 0;  // word offsets of off[] and parts[] in foldmeta
@@ -781,11 +848,33 @@ template <class F> void with_loader(dspx_ltf* L, const void* raw, F&& f)
   }
 }
 
+// Windows per one-pass workgroup (a multiple of the windows in flight) and
+// workgroups per channel for a block of np windows: about four rounds of the
+// workgroups the device holds at once, as few as that allows (each
+// workgroup evaluates its channel's chirp once and writes one histogram).
+void onepass_groups(const dspx_ltf* L, uint64_t np, uint32_t& ppg, uint32_t& ngrp)
+{
+  const uint32_t wf = onepass_wf(L->d.nfft);
+  const size_t lds = onepass_lds_bytes(L->d.nfft) + size_t(L->nval) * L->d.nbin * sizeof(float);
+  const uint64_t per_cu = std::max<uint64_t>(1, (160 * 1024) / lds);
+  const uint64_t target = uint64_t(std::max(L->ctx->ncu, 1)) * per_cu * 4;
+  const uint64_t units = std::max<uint64_t>(1, (np + wf - 1) / wf);
+  const uint64_t g = std::min<uint64_t>(units, std::max<uint64_t>(1, (target + L->d.nchan - 1) / L->d.nchan));
+  const uint64_t per = (units + g - 1) / g * wf;
+  DSPX_REQUIRE(per < (1ull << 31), DSPX_E_UNSUPPORTED, "too many windows per channel");
+  ppg = uint32_t(per);
+  ngrp = uint32_t(std::max<uint64_t>(1, (np + per - 1) / per));
+}
+
+// Whether the one-pass form holds an N-point transform's exchange rows and a
+// two-slot fold histogram of nval*nbin values in the LDS.
+bool onepass_fits(uint32_t nfft, uint32_t nval, uint32_t nbin)
+{
+  if (!nfft || (nfft & (nfft - 1)) || nfft < (1u << kOnePassMinLog2) || nfft > (1u << kOnePassMaxLog2)) return false;
+  return onepass_lds_bytes(nfft) + 2 * size_t(nval) * nbin * sizeof(float) <= 160 * 1024;
+}
+
 }  // namespace
-
-extern "C" {
-
-}  // extern "C"
 
 namespace {
 
@@ -825,9 +914,18 @@ dspx_ltf* ltf_create(dspx_ctx* ctx, const dspx_ltf_desc* d, LtfScratch* ext)
       if (d->form == DSPX_FORM_PLAIN) {
         L->plain.init(ctx, d->nchan, d->npol, d->nfft, d->nfilt_pos, d->nfilt_neg, d->chirp_mode, d->dedisp);
       } else {
-        DSPX_REQUIRE(d->form == DSPX_FORM_FUSED, DSPX_E_PARAM, "form=%u", d->form);
-        L->core.init(ctx, d->nchan, d->npol, d->nfft, d->nfilt_pos, d->nfilt_neg, d->chirp_mode, d->dedisp);
-        DSPX_REQUIRE(L->npart < (1u << 20), DSPX_E_UNSUPPORTED, "npart too large");
+        DSPX_REQUIRE(d->form == DSPX_FORM_FUSED || d->form == DSPX_FORM_TWOLEVEL || d->form == DSPX_FORM_ONEPASS,
+                     DSPX_E_PARAM, "form=%u", d->form);
+        // the fused form: one kernel where the transform and the fold
+        // histogram fit the LDS, else the three stages
+        const bool fits = onepass_fits(d->nfft, L->nval, d->nbin);
+        DSPX_REQUIRE(d->form != DSPX_FORM_ONEPASS || fits, DSPX_E_UNSUPPORTED,
+                     "nfft=%u with %u x %u fold values: outside the one-pass form", d->nfft, L->nval, d->nbin);
+        L->onepass = d->form == DSPX_FORM_ONEPASS || (d->form == DSPX_FORM_FUSED && fits);
+        L->core.init(ctx, d->nchan, d->npol, d->nfft, d->nfilt_pos, d->nfilt_neg, d->chirp_mode, d->dedisp,
+                     L->onepass);
+        DSPX_REQUIRE(L->npart < (1u << 30), DSPX_E_UNSUPPORTED, "npart too large");
+        DSPX_REQUIRE(L->onepass || L->npart < (1u << 20), DSPX_E_UNSUPPORTED, "npart too large");
         L->core.reserve_work(uint32_t(L->npart));
       }
     } catch (...) {
@@ -844,7 +942,7 @@ dspx_ltf* ltf_create(dspx_ctx* ctx, const dspx_ltf_desc* d, LtfScratch* ext)
 void ltf_reserve(dspx_ltf* L)
 {
   const auto& d = L->d;
-  if (d.form != DSPX_FORM_FUSED || !L->npart) return;
+  if (d.form == DSPX_FORM_PLAIN || !L->npart) return;
   uint64_t wbytes = uint64_t(d.npol) * 2 * d.nbit / 8;
   if (d.nchan > 1) L->sc().ftp.reserve(d.ndat * d.nchan * wbytes);
   if (L->have_fold) L->sc().partial2.reserve(uint64_t(d.nchan) * L->npart * L->nslot * L->nval * d.nbin * sizeof(float));
@@ -852,72 +950,92 @@ void ltf_reserve(dspx_ltf* L)
 
 
 // Per-block set-up of the fused fold, after L->fold holds this block's
-// segments: the sub-integration slots of each part, the partial-histogram
-// geometry, the metadata upload, the key plan and the plan's hits.  Nothing
-// here waits for the stream.
+// segments: the sub-integration slots of each part (of each workgroup's
+// windows in the one-pass form), the partial-histogram geometry, the
+// metadata upload, the key plan and the plan's hits.  Nothing here waits for
+// the stream.
 void fused_fold_setup(dspx_ltf* L)
 {
   dspx_ctx* ctx = L->ctx;
   const uint64_t np = L->npart;
-  // sub-integration slots per part: the kept outputs of part p are samples
-  // [p*step, (p+1)*step)
-  std::vector<uint32_t> sf(np);
+  // Units whose histograms the reduction sums: parts (three stages) or the
+  // one-pass workgroups of ppg windows.  Each unit's slots start at the first
+  // sub-integration its kept outputs meet.
+  uint64_t per_unit = 1;
+  if (L->onepass) {
+    onepass_groups(L, np, L->ppg, L->ngrp);
+    per_unit = L->ppg;
+  }
+  const uint64_t nunit = (np + per_unit - 1) / per_unit;
+  std::vector<uint32_t> sfu(nunit);
   uint32_t nslot = 1;
-  for (uint64_t p = 0; p < np; p++) {
+  for (uint64_t u = 0; u < nunit; u++) {
     uint32_t first, last;
-    L->fold.sub_range(p * L->step, (p + 1) * L->step, first, last);
+    const uint64_t p0 = u * per_unit, p1 = std::min(np, p0 + per_unit);
+    L->fold.sub_range(p0 * L->step, p1 * L->step, first, last);
     if (first == 0xffffffffu) {
       first = 0;
       last = 0;
     }
-    sf[p] = first;
+    sfu[u] = first;
     nslot = std::max(nslot, last - first + 1);
   }
   L->nslot = nslot;
   const size_t hist_bytes = size_t(nslot) * L->nval * L->d.nbin * sizeof(float);
-  DSPX_REQUIRE(kColTile * sizeof(float2) + hist_bytes <= 160 * 1024, DSPX_E_UNSUPPORTED,
+  const size_t tile_bytes = L->onepass ? onepass_lds_bytes(L->d.nfft) : kColTile * sizeof(float2);
+  DSPX_REQUIRE(tile_bytes + hist_bytes <= 160 * 1024, DSPX_E_UNSUPPORTED,
                "fold histogram of %zu bytes does not fit in LDS", hist_bytes);
   const uint32_t nsub = L->fold.nsub;
-  // parts whose slots hold isub: sub_first[p] <= isub < sub_first[p] + nslot
-  std::vector<uint32_t> off(nsub + 1, 0), parts;
+  // units whose slots hold isub: sfu[u] <= isub < sfu[u] + nslot
+  std::vector<uint32_t> off(nsub + 1, 0), units;
   for (uint32_t isub = 0; isub < nsub; isub++) {
-    for (uint64_t p = 0; p < np; p++)
-      if (isub >= sf[p] && isub < sf[p] + nslot) parts.push_back(uint32_t(p));
-    off[isub + 1] = uint32_t(parts.size());
+    for (uint64_t u = 0; u < nunit; u++)
+      if (isub >= sfu[u] && isub < sfu[u] + nslot) units.push_back(uint32_t(u));
+    off[isub + 1] = uint32_t(units.size());
   }
+  // metadata words: [sub_first of each part: np][off: nsub+1][units][sub_first of each unit: nunit]
   L->meta_off = uint32_t(np);
   L->meta_parts = uint32_t(np + nsub + 1);
-  const size_t words = np + nsub + 1 + parts.size();
+  L->meta_sf = uint32_t(np + nsub + 1 + units.size());
+  const size_t words = size_t(L->meta_sf) + nunit;
   L->foldmeta.reserve(words * sizeof(uint32_t));
   int slot;
   auto* h = static_cast<uint32_t*>(L->meta_stage.acquire(words * sizeof(uint32_t), slot));
-  std::copy(sf.begin(), sf.end(), h);
+  for (uint64_t p = 0; p < np; p++) h[p] = sfu[p / per_unit];
   std::copy(off.begin(), off.end(), h + L->meta_off);
-  std::copy(parts.begin(), parts.end(), h + L->meta_parts);
+  std::copy(units.begin(), units.end(), h + L->meta_parts);
+  std::copy(sfu.begin(), sfu.end(), h + L->meta_sf);
   L->meta_stage.upload(slot, L->foldmeta.p, words * sizeof(uint32_t), ctx->stream);
 
-  // Column groups per (channel, part) in stage C.  Every stage C workgroup
-  // zeroes, writes and has reduced one histogram: with large histograms the
-  // grid is one round of the workgroups the device holds at once (LDS-bound),
-  // with small ones four rounds, for balance.  The count is then lowered to
-  // give every group the same number of column tiles (128 tiles over 118
-  // groups would leave most groups one tile and a few two, each group with
-  // its own histogram).
-  const uint32_t ncb = L->core.ncb();
-  const uint64_t units = uint64_t(L->d.nchan) * np;
-  const uint64_t per_cu = std::max<uint64_t>(1, (160 * 1024) / (kColTile * sizeof(float2) + hist_bytes));
-  const uint64_t slots = uint64_t(std::max(ctx->ncu, 1)) * per_cu;
-  const uint64_t target = hist_bytes > 16384 ? slots : 4 * slots;
-  const uint64_t g0 = std::min<uint64_t>(ncb, std::max<uint64_t>(1, (target + units - 1) / units));
-  const uint64_t tiles_per_group = (ncb + g0 - 1) / g0;
-  L->G = uint32_t((ncb + tiles_per_group - 1) / tiles_per_group);
-  L->sc().partial.reserve(units * L->G * hist_bytes);
+  const uint64_t units_all = uint64_t(L->d.nchan) * nunit;
+  if (L->onepass) {
+    L->G = 1;
+    L->sc().partial.reserve(units_all * hist_bytes);
+  } else {
+    // Column groups per (channel, part) in stage C.  Every stage C workgroup
+    // zeroes, writes and has reduced one histogram: with large histograms the
+    // grid is one round of the workgroups the device holds at once
+    // (LDS-bound), with small ones four rounds, for balance.  The count is
+    // then lowered to give every group the same number of column tiles (128
+    // tiles over 118 groups would leave most groups one tile and a few two,
+    // each group with its own histogram).
+    const uint32_t ncb = L->core.ncb();
+    const uint64_t per_cu = std::max<uint64_t>(1, (160 * 1024) / (kColTile * sizeof(float2) + hist_bytes));
+    const uint64_t slots = uint64_t(std::max(ctx->ncu, 1)) * per_cu;
+    const uint64_t target = hist_bytes > 16384 ? slots : 4 * slots;
+    const uint64_t g0 = std::min<uint64_t>(ncb, std::max<uint64_t>(1, (target + units_all - 1) / units_all));
+    const uint64_t tiles_per_group = (ncb + g0 - 1) / g0;
+    L->G = uint32_t((ncb + tiles_per_group - 1) / tiles_per_group);
+    L->sc().partial.reserve(units_all * L->G * hist_bytes);
+  }
 
   // key plan and hits
   const uint64_t n = np * L->d.nfft;
   const uint64_t nh = uint64_t(nsub) * L->d.nbin;
-  DSPX_REQUIRE(L->d.nfft % kKeyChunk == 0, DSPX_E_INTERNAL, "nfft=%u", L->d.nfft);
-  DSPX_REQUIRE(n / kKeyChunk < (1ull << 31), DSPX_E_UNSUPPORTED, "key plan too long");
+  DSPX_REQUIRE(L->d.nfft % 4 == 0 && (L->d.nfft % kKeyChunk == 0 || L->onepass), DSPX_E_INTERNAL, "nfft=%u",
+               L->d.nfft);
+  const uint64_t nkb = (n + kKeyChunk - 1) / kKeyChunk;
+  DSPX_REQUIRE(nkb < (1ull << 31), DSPX_E_UNSUPPORTED, "key plan too long");
   L->keys.reserve(n * sizeof(uint32_t));
   L->plan_hits.reserve(std::max<uint64_t>(nh, 1) * sizeof(uint32_t));
   k_fill_u32<<<unsigned((nh + 255) / 256), 256, 0, ctx->stream>>>(L->plan_hits.as<uint32_t>(), nh, 0u);
@@ -928,50 +1046,49 @@ void fused_fold_setup(dspx_ltf* L)
     allow_full_lds((const void*) k_fused_keys);
     lds_attr = true;
   }
-  k_fused_keys<<<unsigned(n / kKeyChunk), 256, shm, ctx->stream>>>(
+  k_fused_keys<<<unsigned(nkb), 256, shm, ctx->stream>>>(
       L->fold.seg.as<FoldSegment>(), L->fold.nseg, double(L->d.nbin), L->d.nbin, L->foldmeta.as<uint32_t>(), nslot,
       L->d.nfft, L->step, L->d.nfilt_pos, n, L->keys.as<uint32_t>(), L->plan_hits.as<uint32_t>());
   DSPX_HIP(hipGetLastError());
 }
 
-// Stages B and C, the profile reduction and the hits of a fold execute
-// (stage A has run on np parts).
-void fused_fold_after_a(dspx_ltf* L, uint32_t np, float* d_prof, uint32_t* d_hits, int& ev)
+FoldArgs fold_args(dspx_ltf* L)
 {
-  dspx_ctx* ctx = L->ctx;
-  const auto& d = L->d;
-  ConvCore& k = L->core;
-  k.stage_b(np);
-  ltf_record(L, ev++);
-  L->stage_name[L->nstage++] = "stageB_rowfft_chirp";
   FoldArgs fa;
   fa.keys = L->keys.as<uint32_t>();
   fa.partial = L->sc().partial.as<float>();
-  fa.nbin = d.nbin;
+  fa.nbin = L->d.nbin;
   fa.nval = L->nval;
   fa.ndim = L->ndim_out;
   fa.nslot = L->nslot;
   fa.G = L->G;
-  fa.state = d.state;
+  fa.state = L->d.state;
   fa.npol_out = L->npol_out;
-  for (uint32_t i = 0; i < 4; i++) fa.voff[i] = (i / L->ndim_out) * d.nbin * L->ndim_out + (i % L->ndim_out);
-  k.stage_c<kStageCFold>(np, FilterbankArgs{}, fa);
-  ltf_record(L, ev++);
-  L->stage_name[L->nstage++] = "stageC_colifft_detect_fold";
+  for (uint32_t i = 0; i < 4; i++) fa.voff[i] = (i / L->ndim_out) * L->d.nbin * L->ndim_out + (i % L->ndim_out);
+  return fa;
+}
+
+// The profile reduction over the units' partial histograms (parts, or
+// one-pass workgroups) and the plan's hits.
+void fold_finish(dspx_ltf* L, uint32_t np, float* d_prof, uint32_t* d_hits, int& ev)
+{
+  dspx_ctx* ctx = L->ctx;
+  const auto& d = L->d;
+  const uint32_t nunit = L->onepass ? L->ngrp : np;
   uint32_t hl = L->nval * d.nbin;
-  uint64_t n2 = uint64_t(d.nchan) * np * L->nslot * hl;
-  // with one group the partials already are [chan][p][slot][e]
-  const float* per_part = L->sc().partial.as<float>();
+  uint64_t n2 = uint64_t(d.nchan) * nunit * L->nslot * hl;
+  // with one group the partials already are [chan][unit][slot][e]
+  const float* per_unit = L->sc().partial.as<float>();
   if (L->G > 1) {
     L->sc().partial2.reserve(n2 * sizeof(float));
     k_fold_reduce_g<<<unsigned((n2 + 255) / 256), 256, 0, ctx->stream>>>(
         L->sc().partial.as<float>(), L->G, L->nslot * hl, n2, L->sc().partial2.as<float>());
     DSPX_HIP(hipGetLastError());
-    per_part = L->sc().partial2.as<float>();
+    per_unit = L->sc().partial2.as<float>();
   }
   dim3 grid((hl + 255) / 256, d.nchan, L->fold.nsub);
-  k_fold_reduce_p<<<grid, 256, 0, ctx->stream>>>(per_part, L->foldmeta.as<uint32_t>(),
-                                                 L->meta_off, L->meta_parts, np, L->nslot, hl, d.nchan, d_prof);
+  k_fold_reduce_p<<<grid, 256, 0, ctx->stream>>>(per_unit, L->foldmeta.as<uint32_t>(), L->meta_sf, L->meta_off,
+                                                 L->meta_parts, nunit, L->nslot, hl, d.nchan, d_prof);
   DSPX_HIP(hipGetLastError());
   if (d_hits) {
     const uint64_t nh = uint64_t(L->fold.nsub) * d.nbin;
@@ -980,6 +1097,59 @@ void fused_fold_after_a(dspx_ltf* L, uint32_t np, float* d_prof, uint32_t* d_hit
   }
   ltf_record(L, ev++);
   L->stage_name[L->nstage++] = "fold_reduce_hits";
+}
+
+// Stages B and C, the profile reduction and the hits of a fold execute
+// (stage A has run on np parts).
+void fused_fold_after_a(dspx_ltf* L, uint32_t np, float* d_prof, uint32_t* d_hits, int& ev)
+{
+  ConvCore& k = L->core;
+  k.stage_b(np);
+  ltf_record(L, ev++);
+  L->stage_name[L->nstage++] = "stageB_rowfft_chirp";
+  k.stage_c<kStageCFold>(np, FilterbankArgs{}, fold_args(L));
+  ltf_record(L, ev++);
+  L->stage_name[L->nstage++] = "stageC_colifft_detect_fold";
+  fold_finish(L, np, d_prof, d_hits, ev);
+}
+
+OnePassArgs onepass_args(const dspx_ltf* L, uint32_t np, uint32_t ppg, uint32_t ngrp)
+{
+  return OnePassArgs{L->d.nchan, L->d.npol, np, L->step, L->d.nfilt_pos, ppg, ngrp};
+}
+
+// The whole fold execute of np windows over the loader ld: the three stages
+// or the one-pass kernel, then the reduction (stage timing events as the
+// stages complete).
+template <class LD> void fold_exec(dspx_ltf* L, const LD& ld, uint32_t np, float* d_prof, uint32_t* d_hits, int& ev)
+{
+  if (L->onepass) {
+    L->core.run_onepass<kStageCFold>(ld, onepass_args(L, np, L->ppg, L->ngrp), FilterbankArgs{}, fold_args(L));
+    ltf_record(L, ev++);
+    L->stage_name[L->nstage++] = "onepass_unpack_fft_chirp_ifft_detect_fold";
+    fold_finish(L, np, d_prof, d_hits, ev);
+  } else {
+    L->core.stage_a(ld, np);
+    ltf_record(L, ev++);
+    L->stage_name[L->nstage++] = "stageA_unpack_colfft";
+    fused_fold_after_a(L, np, d_prof, d_hits, ev);
+  }
+}
+
+// The convolution of np windows into the FPT filterbank output.
+template <class LD> void filterbank_exec(dspx_ltf* L, const LD& ld, uint32_t np, float* d_out)
+{
+  const uint64_t nout = uint64_t(np) * L->step;
+  FilterbankArgs fb{d_out, uint64_t(L->d.npol) * nout * 2, nout * 2};
+  if (L->onepass) {
+    uint32_t ppg, ngrp;
+    onepass_groups(L, np, ppg, ngrp);
+    L->core.run_onepass<kStageCFilterbank>(ld, onepass_args(L, np, ppg, ngrp), fb, FoldArgs{});
+  } else {
+    L->core.stage_a(ld, np);
+    L->core.stage_b(np);
+    L->core.stage_c<kStageCFilterbank>(np, fb, FoldArgs{});
+  }
 }
 }  // namespace
 
@@ -1014,7 +1184,7 @@ int dspx_ltf_set_fold(dspx_ltf* L, double pps, const dspx_fold_chunk* chunks, ui
   dspx_ctx* ctx = L ? L->ctx : nullptr;
   return guard(ctx, [&] {
     DSPX_REQUIRE(L && chunks, DSPX_E_PARAM, "null argument");
-    const bool fused = L->d.form == DSPX_FORM_FUSED;
+    const bool fused = L->d.form != DSPX_FORM_PLAIN;
     L->have_fold = false;  // until the whole plan is in place
     L->fold.build(ctx, L->ndat_out, pps, L->d.nbin, chunks, nchunk, !fused);
     if (fused && L->npart) fused_fold_setup(L);
@@ -1034,13 +1204,9 @@ int dspx_ltf_execute(dspx_ltf* L, const void* d_raw, float* d_prof, uint32_t* d_
     L->nstage = 0;
     L->nstage_open = false;
     ltf_record(L, ev++);
-    if (d.form == DSPX_FORM_FUSED) {
-      ConvCore& k = L->core;
+    if (d.form != DSPX_FORM_PLAIN) {
       uint32_t np = uint32_t(L->npart);
-      with_loader(L, d_raw, [&](auto& ld) { k.stage_a(ld, np); });
-      ltf_record(L, ev++);
-      L->stage_name[L->nstage++] = "stageA_unpack_colfft";
-      fused_fold_after_a(L, np, d_prof, d_hits, ev);
+      with_loader(L, d_raw, [&](auto& ld) { fold_exec(L, ld, np, d_prof, d_hits, ev); });
     } else {
       // plain: unpack -> hipFFT convolution -> detect -> fold, each a pass
       uint64_t nin = d.ndat, nout = L->ndat_out;
@@ -1079,13 +1245,9 @@ int dspx_ltf_filterbank(dspx_ltf* L, const void* d_raw, float* d_out)
     uint64_t nout = L->ndat_out;
     L->nstage = 0;
     L->nstage_open = false;
-    if (d.form == DSPX_FORM_FUSED) {
-      ConvCore& k = L->core;
+    if (d.form != DSPX_FORM_PLAIN) {
       uint32_t np = uint32_t(L->npart);
-      with_loader(L, d_raw, [&](auto& ld) { k.stage_a(ld, np); });
-      k.stage_b(np);
-      FilterbankArgs fb{d_out, uint64_t(d.npol) * nout * 2, nout * 2};
-      k.stage_c<kStageCFilterbank>(np, fb, FoldArgs{});
+      with_loader(L, d_raw, [&](auto& ld) { filterbank_exec(L, ld, np, d_out); });
     } else {
       uint64_t nin = d.ndat;
       L->unpacked.reserve(size_t(d.nchan) * d.npol * nin * sizeof(float2));
@@ -1113,7 +1275,7 @@ int dspx_ltf_stream_reset(dspx_ltf* L)
   return guard(ctx, [&] {
     DSPX_REQUIRE(L, DSPX_E_PARAM, "null argument");
     const auto& d = L->d;
-    DSPX_REQUIRE(d.form == DSPX_FORM_FUSED, DSPX_E_UNSUPPORTED, "stream mode runs the fused form");
+    DSPX_REQUIRE(d.form != DSPX_FORM_PLAIN, DSPX_E_UNSUPPORTED, "stream mode runs the fused form");
     DSPX_REQUIRE(d.ndat >= d.nfft, DSPX_E_PARAM, "stream blocks of %llu samples shorter than nfft=%u",
                  (unsigned long long) d.ndat, d.nfft);
     const uint64_t nov = uint64_t(d.nfilt_pos) + d.nfilt_neg;
@@ -1208,10 +1370,7 @@ int dspx_ltf_stream_execute(dspx_ltf* L, const void* d_raw, float* d_prof, uint3
     stream_call(L, d_raw, [&](auto& ld, uint32_t np) {
       if (!np) return;
       ltf_record(L, ev++);
-      L->core.stage_a(ld, np);
-      ltf_record(L, ev++);
-      L->stage_name[L->nstage++] = "stageA_unpack_colfft";
-      fused_fold_after_a(L, np, d_prof, d_hits, ev);
+      fold_exec(L, ld, np, d_prof, d_hits, ev);
     });
   });
 }
@@ -1226,11 +1385,7 @@ int dspx_ltf_stream_filterbank(dspx_ltf* L, const void* d_raw, float* d_out)
     L->nstage_open = false;
     stream_call(L, d_raw, [&](auto& ld, uint32_t np) {
       if (!np) return;
-      const uint64_t nout = uint64_t(np) * L->step;
-      L->core.stage_a(ld, np);
-      L->core.stage_b(np);
-      FilterbankArgs fb{d_out, uint64_t(d.npol) * nout * 2, nout * 2};
-      L->core.stage_c<kStageCFilterbank>(np, fb, FoldArgs{});
+      filterbank_exec(L, ld, np, d_out);
     });
   });
 }
@@ -1291,7 +1446,8 @@ struct dspx_batch {
 namespace {
 
 // The descriptor with its overlap filled in for FFT length nfft (0: the
-// reference's rule, Response.C:553-588, raised to at least 4096).
+// reference's rule, Response.C:553-588, raised to at least the one-pass
+// form's shortest transform).
 dspx_ltf_desc with_geometry(const dspx_ltf_desc& d0, uint32_t nfft)
 {
   dspx_dedisp_desc dd = d0.dedisp;
@@ -1299,13 +1455,13 @@ dspx_ltf_desc with_geometry(const dspx_ltf_desc& d0, uint32_t nfft)
   dspx_geometry g{};
   int rc = geometry(&dd, &g);
   DSPX_REQUIRE(rc == DSPX_OK, rc, "geometry of a beam (nfft=%u) failed", nfft);
-  if (nfft == 0 && g.nfft < 4096) {
-    // below the fused form's shortest transform: the next supported length
-    // (as a user-set length would be, Response::set_frequency_resolution,
+  if (nfft == 0 && g.nfft < (1u << kOnePassMinLog2)) {
+    // below the shortest transform built: the next supported length (as a
+    // user-set length would be, Response::set_frequency_resolution,
     // Response.C:74-80, 242-250)
-    dd.nfft = 4096;
+    dd.nfft = 1u << kOnePassMinLog2;
     rc = geometry(&dd, &g);
-    DSPX_REQUIRE(rc == DSPX_OK, rc, "geometry of a beam (nfft=4096) failed");
+    DSPX_REQUIRE(rc == DSPX_OK, rc, "geometry of a beam (nfft=%u) failed", dd.nfft);
   }
   dspx_ltf_desc d = d0;
   d.nfft = g.nfft;
@@ -1352,15 +1508,15 @@ double time_per_sample(dspx_batch* B, const dspx_ltf_desc& dn)
 }
 
 // Candidate lengths: the reference's choice and the next two powers of two
-// (longer transforms waste less on the overlap, shorter ones keep the column
-// and row transforms in their fastest shapes), within the fused form's range.
+// (longer transforms waste less on the overlap, shorter ones keep the
+// transforms in their fastest shapes), within the fused forms' range; each
+// runs in the form the plan picks for it (one pass where it fits).
 uint32_t autotune_nfft(dspx_batch* B, const dspx_ltf_desc& d0)
 {
   dspx_ltf_desc ref = with_geometry(d0, 0);
   uint32_t best = ref.nfft;
   double best_t = 1e30;
-  for (uint32_t n = ref.nfft; n <= (1u << 20) && n <= 4 * ref.nfft; n *= 2) {
-    if (n < 4096) continue;
+  for (uint32_t n = ref.nfft; n <= (1u << 22) && n <= 4 * ref.nfft; n *= 2) {
     dspx_ltf_desc d = with_geometry(d0, n);
     if (d.nfilt_pos + d.nfilt_neg >= n || d.ndat < n) continue;
     double t = time_per_sample(B, d);
@@ -1387,7 +1543,7 @@ int dspx_batch_create(dspx_ctx* ctx, const dspx_ltf_desc* beams, uint32_t nbeam,
     B->ctx = ctx;
     for (uint32_t i = 0; i < nbeam; i++) {
       const dspx_ltf_desc& d0 = beams[i];
-      DSPX_REQUIRE(d0.form == DSPX_FORM_FUSED, DSPX_E_PARAM, "beam %u: the batch runs the fused form", i);
+      DSPX_REQUIRE(d0.form != DSPX_FORM_PLAIN, DSPX_E_PARAM, "beam %u: the batch runs the fused forms", i);
       dspx_ltf_desc d = d0;
       if (d0.nfft == 0) {
         uint32_t n = (flags & DSPX_BATCH_AUTOTUNE) ? autotune_nfft(B.get(), d0) : 0;

@@ -174,7 +174,7 @@ int dspx_ctx_create(int device, void* stream, dspx_ctx** out)
         DSPX_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
         c->own_stream = true;
       }
-      // exp(-2 pi i m / L) for L = 2, 4, ..., 8192 at offset L, then the
+      // exp(-2 pi i m / L) for L = 2, 4, ..., 16384 at offset L, then the
       // power records of each span (layout in fft_core.hpp), all in FP64
       // rounded once to float
       const int total = kTwTotal;

@@ -161,11 +161,11 @@ template <int L, int NT> struct RowLayout {
 // thread's E/R butterflies q, the R inputs x[j + r*L/R]).  After the call
 // v[q*R + r] holds the output destined for index (j/Ns)*Ns*R + j%Ns + r*Ns.
 // Twiddle memory (float2 units from the base of the table the context
-// builds): exp(-2 pi i m / L) for L = 2..8192 at offset L, m < L; from
+// builds): exp(-2 pi i m / L) for L = 2..16384 at offset L, m < L; from
 // kTwPowBase, power records of 4 float2 {W_S^k, W_S^2k, W_S^4k, W_S^8k}
-// (W_S = exp(-2 pi i / S)) for spans S = 2..8192 at record S/2 + k, k < S/2.
+// (W_S = exp(-2 pi i / S)) for spans S = 2..16384 at record S/2 + k, k < S/2.
 // Kernels are handed the pointer to the exp table of their length L.
-constexpr int kTwMaxL = 8192;
+constexpr int kTwMaxL = 16384;
 constexpr int kTwPowBase = 2 * kTwMaxL;
 constexpr int kTwTotal = kTwPowBase + 4 * kTwMaxL;
 
@@ -260,6 +260,8 @@ DSPX_PLAN(512, 16, 3, 2, 16, 16, 0)
 DSPX_PLAN(1024, 16, 3, 4, 16, 16, 0)
 DSPX_PLAN(2048, 16, 3, 8, 16, 16, 0)
 DSPX_PLAN(4096, 16, 3, 16, 16, 16, 0)
+DSPX_PLAN(8192, 16, 4, 2, 16, 16, 16)
+DSPX_PLAN(16384, 16, 4, 4, 16, 16, 16)
 DSPX_PLAN(512, 8, 3, 8, 8, 8, 0)
 DSPX_PLAN(1024, 8, 4, 2, 8, 8, 8)
 DSPX_PLAN(2048, 8, 4, 4, 8, 8, 8)
@@ -326,6 +328,15 @@ template <class T> __device__ __forceinline__ const T* opaque(const T* p)
 __device__ __forceinline__ float2 opaque_v(float2 x)
 {
   asm volatile("" : "+v"(x.x), "+v"(x.y));
+  return x;
+}
+
+// Returns a per-lane integer unchanged, opaquely: index arithmetic on the
+// result is redone where it is used instead of being hoisted out of the loop
+// that calls this (and held in registers for the loop's duration).
+__device__ __forceinline__ int opaque_i(int x)
+{
+  asm volatile("" : "+v"(x));
   return x;
 }
 

@@ -1,0 +1,235 @@
+// onepass.hpp -- the LoadToFold chain in one kernel for short transforms.
+//
+// For an N-point overlap-save transform that fits one LDS row (N = 2^7 ..
+// 2^13), every window is read once as raw 8/16-bit samples and leaves the
+// chip only as fold histograms (or, for the filterbank output, as its kept
+// samples): unpack -> forward N-point Stockham FFT -> dedispersion chirp
+// (FP64 phase, on chip) -> inverse FFT -> overlap discard -> detection ->
+// phase-bin accumulation in LDS.  The HBM traffic per input sample is the
+// raw bytes of the windows, r*B_in (r = N / step), against r*(B_in + 32) for
+// the three-stage form of the longer transforms.
+//
+// A workgroup owns one channel and a range of its windows (ppg of them, a
+// multiple of WF), WF windows in flight, one per group of NTW = N/16
+// threads, each with its own LDS exchange row; the histogram of the
+// workgroup's sub-integration slots follows the rows.  The channel's chirp
+// is evaluated once per workgroup, into LDS after the rows (N <= 4096) or
+// into registers (the bins each thread holds).
+//
+// Reference: Convolution::transformation (Signal/General/Convolution.C:
+// 646-813), Detection (Detection.C:225-573), Fold::fold (Fold.C:790-949),
+// chained as LoadToFold1.C:109-258 chains them.
+#pragma once
+#include "stages.hpp"
+
+namespace dspx {
+
+template <int N> struct OnePass {
+  static constexpr int E = 16;                    // complex values per thread per pol
+  static constexpr int NTW = N / E;               // threads per window
+  static constexpr int NT = NTW < 256 ? 256 : NTW;  // threads per workgroup
+  static constexpr int WF = NT / NTW;             // windows in flight
+  static constexpr int LROW = N + N / 16;         // exchange row (RowLayout padding)
+  // exchange rows, then the channel's chirp (N values, CH_LDS)
+  static constexpr size_t lds_bytes = (size_t(WF) * LROW + (N <= 4096 ? N : 0)) * sizeof(float2);
+  static constexpr int R0 = RadixPlan<N, E>::r[0];                      // first forward = last inverse pass
+  static constexpr int RL = RadixPlan<N, E>::r[RadixPlan<N, E>::n - 1];  // last forward = first inverse pass
+  static constexpr int QG = E / 4;                // groups of 4 consecutive samples per thread (fold)
+  // the channel's chirp in LDS after the rows (N <= 4096), else in registers
+  static constexpr bool CH_LDS = N <= 4096;
+};
+
+constexpr int kOnePassMinLog2 = 7, kOnePassMaxLog2 = 13;
+
+// LDS bytes of the exchange rows for an n-point transform (host side).
+inline size_t onepass_lds_bytes(uint32_t n)
+{
+  const uint32_t ntw = n / 16, nt = ntw < 256 ? 256 : ntw;
+  return (size_t(nt / ntw) * (n + n / 16) + (n <= 4096 ? n : 0)) * sizeof(float2);
+}
+inline uint32_t onepass_threads(uint32_t n) { return n / 16 < 256 ? 256 : n / 16; }
+inline uint32_t onepass_wf(uint32_t n) { return onepass_threads(n) / (n / 16); }
+
+struct OnePassArgs {
+  uint32_t nchan, npol, npart;
+  uint32_t step, nfilt_pos;
+  uint32_t ppg;   // windows per workgroup, a multiple of WF
+  uint32_t ngrp;  // workgroups per channel
+};
+
+// grid.x = nchan * ngrp (group fastest), block = OnePass<N>::NT, dynamic LDS
+// = lds_bytes (+ the fold histogram).  Fold partials go to
+// fa.partial[chan][grp][nslot][nval*nbin]; fa.keys is the key plan
+// [npart][N] (slots relative to the first sub-integration of the window's
+// workgroup).
+template <int N, int MODE, int NV, class LD>
+__global__ __launch_bounds__(OnePass<N>::NT) void k_onepass(LD ld, OnePassArgs a, ChirpArgs c, FilterbankArgs fb,
+                                                            FoldArgs fa, const float2* __restrict__ tw)
+{
+  using S = OnePass<N>;
+  constexpr int E = S::E, NTW = S::NTW, NT = S::NT, WF = S::WF, R0 = S::R0, RL = S::RL, QG = S::QG;
+  using LAY = RowLayout<N, NTW>;
+  extern __shared__ float2 smem[];
+  const int slot = WF > 1 ? int(threadIdx.x) / NTW : 0;
+  const int tid0 = WF > 1 ? int(threadIdx.x) % NTW : int(threadIdx.x);
+  float2* const lds0 = smem + slot * S::LROW;
+  float2* chl = smem + WF * S::LROW;
+  float* hist = reinterpret_cast<float*>(chl + (S::CH_LDS ? N : 0));
+
+  const uint32_t grp = blockIdx.x % a.ngrp;
+  const uint32_t chan = blockIdx.x / a.ngrp;
+  const uint32_t w_begin = grp * a.ppg;
+  const uint32_t w_end = min(a.npart, w_begin + a.ppg);
+
+  const uint32_t hist_len = MODE == kStageCFold ? fa.nslot * fa.nval * fa.nbin : 0;
+  if (MODE == kStageCFold)
+    for (uint32_t i = threadIdx.x; i < hist_len; i += NT) hist[i] = 0.f;
+
+  // the channel's chirp, evaluated once per workgroup: into LDS (read after
+  // the first exchange's barrier), or the bins X[j + r*N/RL] (j = tid +
+  // q*NTW) this thread holds after the forward transform into registers
+  float2 ch[S::CH_LDS ? 1 : E];
+  if constexpr (S::CH_LDS) {
+    for (uint32_t k = threadIdx.x; k < uint32_t(N); k += NT) chl[k] = chirp_onchip(c, chan, k, N);
+  } else {
+#pragma unroll
+    for (int q = 0; q < E / RL; q++)
+#pragma unroll
+      for (int r = 0; r < RL; r++) ch[q * RL + r] = chirp_onchip(c, chan, uint32_t(tid0 + q * NTW + r * (N / RL)), N);
+  }
+  int tid = tid0;
+  float2* lds = lds0;
+
+  // every slot runs the same number of iterations (the exchanges are
+  // workgroup barriers); a slot past the group's last window transforms its
+  // group's first window again and neither stores nor folds it
+  const uint32_t niter = (w_end - w_begin + WF - 1) / WF;
+  for (uint32_t it = 0; it < niter; it++) {
+    const uint32_t w = w_begin + it * WF + uint32_t(slot);
+    const bool valid = w < w_end;
+    const uint64_t t0 = uint64_t(valid ? w : w_begin) * a.step;
+    // the lane's index arithmetic (LDS addresses of every pass) is redone
+    // per window rather than held in registers across the loop
+    tid = opaque_i(tid0);
+    lds = smem + slot * S::LROW;
+    float2 y0[E], v[E];
+    for (uint32_t pol = 0; pol < a.npol; pol++) {
+#pragma unroll
+      for (int q = 0; q < E / R0; q++)
+#pragma unroll
+        for (int r = 0; r < R0; r++)
+          v[q * R0 + r] = ld.pol(ld.word(chan, t0 + uint32_t(tid + q * NTW + r * (N / R0))), pol);
+      // opaque: the pass twiddles are reloaded per window rather than held
+      // in registers across the loop (registers go to occupancy)
+      fft_regs<LAY, N, 1, NTW, -1>(lds, v, opaque(tw), tid);
+      // X[j + r*N/RL] (j = tid + q*NTW) times the chirp of its bin
+#pragma unroll
+      for (int q = 0; q < E / RL; q++)
+#pragma unroll
+        for (int r = 0; r < RL; r++)
+          v[q * RL + r] = cmul(v[q * RL + r], S::CH_LDS ? chl[tid + q * NTW + r * (N / RL)] : ch[q * RL + r]);
+      fft_regs<LAY, N, 1, NTW, +1>(lds, v, opaque(tw), tid);
+      // v[q*R0 + r] = y[tid + q*NTW + r*N/R0]
+      if constexpr (MODE == kStageCFilterbank) {
+        if (valid) {
+          float* o = fb.out + uint64_t(chan) * fb.chan_stride + uint64_t(pol) * fb.pol_stride;
+#pragma unroll
+          for (int q = 0; q < E / R0; q++)
+#pragma unroll
+            for (int r = 0; r < R0; r++) {
+              const uint32_t t = uint32_t(tid + q * NTW + r * (N / R0));
+              if (t < a.nfilt_pos || t >= a.nfilt_pos + a.step) continue;
+              *reinterpret_cast<float2*>(o + 2 * (uint64_t(w) * a.step + (t - a.nfilt_pos))) = v[q * R0 + r];
+            }
+        }
+      } else {
+        // regroup through LDS: each thread then holds QG groups of 4
+        // consecutive samples, (g*NTW + tid)*4 + k
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < E / R0; q++)
+#pragma unroll
+          for (int r = 0; r < R0; r++) lds[LAY::idx(tid + q * NTW + r * (N / R0), 0)] = v[q * R0 + r];
+        __syncthreads();
+#pragma unroll
+        for (int g = 0; g < QG; g++)
+#pragma unroll
+          for (int k = 0; k < 4; k++) {
+            const float2 x = lds[LAY::idx((g * NTW + tid) * 4 + k, 0)];
+            if (pol == 0)
+              y0[g * 4 + k] = x;
+            else
+              v[g * 4 + k] = x;
+          }
+      }
+    }
+    if constexpr (MODE == kStageCFold) {
+      const int lane = int(threadIdx.x & 63u);
+      const uint32_t* kw = fa.keys + uint64_t(valid ? w : w_begin) * N;
+#pragma unroll
+      for (int g = 0; g < QG; g++) {
+        uint32_t kq[4];
+        load_keys<4>(kw + (g * NTW + tid) * 4, kq);
+        if (!valid) kq[0] = kq[1] = kq[2] = kq[3] = 0xffffffffu;
+        float cacc[NV], racc[NV];
+#pragma unroll
+        for (int i = 0; i < NV; i++) cacc[i] = racc[i] = 0.f;
+        const uint32_t lane_key = kq[0];
+        uint32_t run_key = 0xffffffffu;
+        auto flush_run = [&]() {
+          if (run_key != 0xffffffffu) {
+            float d[NV];
+            detect_finish<NV>(fa.state == DSPX_STOKES, racc, d);
+            float* hs = hist + (run_key >> 16) * (NV * fa.nbin) + (run_key & 0xffffu) * fa.ndim;
+#pragma unroll
+            for (int i = 0; i < NV; i++) atomicAdd(hs + fa.voff[i], d[i]);
+          }
+        };
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+          const float2 b = a.npol == 2 ? v[g * 4 + k] : make_float2(0.f, 0.f);
+          const uint32_t key = kq[k];
+          if (key == lane_key) {
+            detect_acc<NV>(a.npol, y0[g * 4 + k], b, cacc);
+          } else if (key != 0xffffffffu) {
+            if (key != run_key) {
+              flush_run();
+              run_key = key;
+#pragma unroll
+              for (int i = 0; i < NV; i++) racc[i] = 0.f;
+            }
+            detect_acc<NV>(a.npol, y0[g * 4 + k], b, racc);
+          }
+        }
+        flush_run();
+        float acc[NV];
+        detect_finish<NV>(fa.state == DSPX_STOKES, cacc, acc);
+        // one add per 16-lane row when the row's first keys agree
+        const uint32_t k15 = __shfl(lane_key, lane | 15);
+        const uint64_t same = __ballot(lane_key == k15);
+        const bool row_uniform = ((same >> (lane & 48)) & 0xffffull) == 0xffffull;
+        float rs[NV];
+#pragma unroll
+        for (int i = 0; i < NV; i++) rs[i] = row_sum_lane15(acc[i]);
+        if (row_uniform) {
+          if ((lane & 15) == 15 && k15 != 0xffffffffu) {
+            float* hs = hist + (k15 >> 16) * (NV * fa.nbin) + (k15 & 0xffffu) * fa.ndim;
+#pragma unroll
+            for (int i = 0; i < NV; i++) atomicAdd(hs + fa.voff[i], rs[i]);
+          }
+        } else if (lane_key != 0xffffffffu) {
+          float* hs = hist + (lane_key >> 16) * (NV * fa.nbin) + (lane_key & 0xffffu) * fa.ndim;
+#pragma unroll
+          for (int i = 0; i < NV; i++) atomicAdd(hs + fa.voff[i], acc[i]);
+        }
+      }
+    }
+  }
+  if constexpr (MODE == kStageCFold) {
+    __syncthreads();
+    float* part = fa.partial + (uint64_t(chan) * a.ngrp + grp) * hist_len;
+    for (uint32_t i = threadIdx.x; i < hist_len; i += NT) part[i] = hist[i];
+  }
+}
+
+}  // namespace dspx

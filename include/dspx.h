@@ -171,8 +171,12 @@ enum {
 };
 
 enum {
-  DSPX_FORM_FUSED = 0, /* hand-written three-stage Stockham path */
-  DSPX_FORM_PLAIN = 1  /* hipFFT + separate multiply / discard kernels */
+  DSPX_FORM_FUSED = 0,    /* hand-written: the one-pass kernel where the transform
+                             (2^7..2^13 points) and the fold histogram fit the LDS
+                             (LoadToFold plans), else the three stages */
+  DSPX_FORM_PLAIN = 1,    /* hipFFT + separate multiply / discard kernels */
+  DSPX_FORM_TWOLEVEL = 2, /* the three-stage (four-step) form, 2^12..2^22 points */
+  DSPX_FORM_ONEPASS = 3   /* the one-pass form, 2^7..2^13 points (LoadToFold plans) */
 };
 
 typedef struct dspx_conv dspx_conv;
@@ -415,7 +419,7 @@ typedef struct {
   uint32_t state;         /* DSPX_INTENSITY ... DSPX_STOKES */
   uint32_t ndim;          /* 1, 2 or 4 for Coherence / Stokes */
   uint32_t nbin;
-  uint32_t form;          /* DSPX_FORM_FUSED or DSPX_FORM_PLAIN */
+  uint32_t form;          /* DSPX_FORM_* */
 } dspx_ltf_desc;
 
 int dspx_ltf_create(dspx_ctx* ctx, const dspx_ltf_desc* desc, dspx_ltf** out);

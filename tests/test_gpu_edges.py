@@ -1,5 +1,5 @@
 """Edge cases of the convolution and the fused path on the GPU: every FFT
-length the four-step split supports (2^12 .. 2^20, both forms) against the
+length the four-step split supports (2^12 .. 2^22, both forms) against the
 oracle, a block too short for one FFT, and a fold histogram too large for
 LDS (a clean error, nothing launched)."""
 import math
@@ -13,7 +13,7 @@ pytestmark = pytest.mark.gpu
 
 
 @pytest.mark.parametrize("form", [0, 1])
-@pytest.mark.parametrize("m", list(range(12, 21)))
+@pytest.mark.parametrize("m", list(range(12, 23)))
 def test_convolution_every_length(ctx, dspx, oracle, m, form):
     import torch
     n = 1 << m

@@ -13,6 +13,8 @@ CASES = {
     # nbit, nchan, npol, nblock, ndat per block, cfreq, bw, dm, state, ndim, nbin, period
     "multichan_stokes": (8, 16, 2, 3, 100_000, 1400.0, 50.0, 100.0, 3, 4, 256, 0.0123),
     "single_int16": (16, 1, 2, 4, 300_000, 1405.0, 16.0, 20.0, 0, 1, 512, 0.0457),
+    # reference length 2^13: the one-pass form
+    "onepass_stokes": (8, 16, 2, 3, 100_000, 1400.0, 50.0, 20.0, 3, 4, 256, 0.0123),
 }
 
 
