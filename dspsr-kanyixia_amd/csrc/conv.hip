@@ -848,13 +848,16 @@ template <class F> void with_loader(dspx_ltf* L, const void* raw, F&& f)
   }
 }
 
-// Windows per one-pass workgroup (a multiple of the windows in flight) and
+// Windows per one-pass workgroup (a multiple of the windows in flight and of
+// the windows in a key-plan chunk) and
 // workgroups per channel for a block of np windows: about four rounds of the
 // workgroups the device holds at once, as few as that allows (each
 // workgroup evaluates its channel's chirp once and writes one histogram).
 void onepass_groups(const dspx_ltf* L, uint64_t np, uint32_t& ppg, uint32_t& ngrp)
 {
-  const uint32_t wf = onepass_wf(L->d.nfft);
+  // whole batches of windows in flight, and whole chunks of the key plan
+  // (whose workgroups assume one sub_first per chunk)
+  const uint32_t wf = std::max(onepass_wf(L->d.nfft), std::max<uint32_t>(1, uint32_t(kKeyChunk) / L->d.nfft));
   const size_t lds = onepass_lds_bytes(L->d.nfft) + size_t(L->nval) * L->d.nbin * sizeof(float);
   const uint64_t per_cu = std::max<uint64_t>(1, (160 * 1024) / lds);
   const uint64_t target = uint64_t(std::max(L->ctx->ncu, 1)) * per_cu * 4;
@@ -965,6 +968,8 @@ void fused_fold_setup(dspx_ltf* L)
   if (L->onepass) {
     onepass_groups(L, np, L->ppg, L->ngrp);
     per_unit = L->ppg;
+    DSPX_REQUIRE(L->d.nfft >= uint32_t(kKeyChunk) || (uint64_t(L->ppg) * L->d.nfft) % kKeyChunk == 0, DSPX_E_INTERNAL,
+                 "one-pass groups of %u windows split key-plan chunks", L->ppg);
   }
   const uint64_t nunit = (np + per_unit - 1) / per_unit;
   std::vector<uint32_t> sfu(nunit);
@@ -1471,15 +1476,18 @@ dspx_ltf_desc with_geometry(const dspx_ltf_desc& d0, uint32_t nfft)
 }
 
 // Seconds per output sample of the fused LoadToFold at FFT length nfft, from
-// three timed executions on a zero-filled block of a few parts (the kernels'
-// run time does not depend on the sample values).
+// three timed executions on a zero-filled block (the kernels' run time does
+// not depend on the sample values) of at least four parts and about 2^20
+// samples per channel where the beam has them, so that the short transforms'
+// grids fill the device as they do on the whole block.
 double time_per_sample(dspx_batch* B, const dspx_ltf_desc& dn)
 {
   dspx_ctx* ctx = B->ctx;
   const uint32_t nov = dn.nfilt_pos + dn.nfilt_neg;
   dspx_ltf_desc d = dn;
   const uint64_t step = d.nfft - nov;
-  d.ndat = std::min<uint64_t>(dn.ndat, nov + 4 * step);
+  const uint64_t nparts = std::max<uint64_t>(4, ((1u << 20) + step - 1) / step);
+  d.ndat = std::min<uint64_t>(dn.ndat, nov + nparts * step);
   if (d.ndat < d.nfft) return 1e30;
   std::unique_ptr<dspx_ltf> L(ltf_create(ctx, &d, &B->sc));
   if (!L->npart) return 1e30;

@@ -157,6 +157,22 @@ template <int L, int NT> struct RowLayout {
   __device__ static __forceinline__ int idx(int n, int /*b*/) { return n + (n >> 4); }
 };
 
+// The exchange barrier of a transform done by NT threads: a workgroup
+// barrier, or, when one wavefront (or part of one) owns the transform, a
+// wavefront-scope fence and scheduling barrier only (a wavefront's LDS
+// accesses complete in order, so its lanes see each other's writes without
+// waiting for the rest of the workgroup).
+template <int NT> __device__ __forceinline__ void xsync()
+{
+  if constexpr (NT <= 64) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  } else {
+    __syncthreads();
+  }
+}
+
 // Compute one Stockham pass on registers v (which hold, for each of the
 // thread's E/R butterflies q, the R inputs x[j + r*L/R]).  After the call
 // v[q*R + r] holds the output destined for index (j/Ns)*Ns*R + j%Ns + r*Ns.
@@ -262,6 +278,8 @@ DSPX_PLAN(2048, 16, 3, 8, 16, 16, 0)
 DSPX_PLAN(4096, 16, 3, 16, 16, 16, 0)
 DSPX_PLAN(8192, 16, 4, 2, 16, 16, 16)
 DSPX_PLAN(16384, 16, 4, 4, 16, 16, 16)
+DSPX_PLAN(128, 8, 3, 2, 8, 8, 0)
+DSPX_PLAN(256, 8, 3, 4, 8, 8, 0)
 DSPX_PLAN(512, 8, 3, 8, 8, 8, 0)
 DSPX_PLAN(1024, 8, 4, 2, 8, 8, 8)
 DSPX_PLAN(2048, 8, 4, 4, 8, 8, 8)
@@ -296,9 +314,9 @@ __device__ __forceinline__ void run_passes(float2* lds, float2* v, const float2*
       // previous pass left outputs in registers: scatter them, then gather
       constexpr int RP = DIR < 0 ? RadixPlan<L, E>::r[P - 1] : InvRadix<L, E, P - 1>::v;
       constexpr int NSP = DIR < 0 ? FwdNs<L, E, P - 1>::v : InvNs<L, E, P - 1>::v;
-      __syncthreads();
+      xsync<NT>();
       lds_store_pass<LAY, L, B, NT, RP, NSP>(lds, v, tid);
-      __syncthreads();
+      xsync<NT>();
       lds_load_pass<LAY, L, B, NT, R>(lds, v, tid);
     }
     stockham_compute<L, B, NT, R, NS, DIR>(v, tw, tid);

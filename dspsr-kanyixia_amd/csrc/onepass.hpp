@@ -25,7 +25,10 @@
 namespace dspx {
 
 template <int N> struct OnePass {
-  static constexpr int E = 16;                    // complex values per thread per pol
+  // complex values per thread per pol: 8 up to 512 points and 16 above, so
+  // that a transform of up to 1024 points is one wavefront's (or part of
+  // one), whose exchanges need no workgroup barrier
+  static constexpr int E = N <= 512 ? 8 : 16;
   static constexpr int NTW = N / E;               // threads per window
   static constexpr int NT = NTW < 256 ? 256 : NTW;  // threads per workgroup
   static constexpr int WF = NT / NTW;             // windows in flight
@@ -34,7 +37,7 @@ template <int N> struct OnePass {
   static constexpr size_t lds_bytes = (size_t(WF) * LROW + (N <= 4096 ? N : 0)) * sizeof(float2);
   static constexpr int R0 = RadixPlan<N, E>::r[0];                      // first forward = last inverse pass
   static constexpr int RL = RadixPlan<N, E>::r[RadixPlan<N, E>::n - 1];  // last forward = first inverse pass
-  static constexpr int QG = E / 4;                // groups of 4 consecutive samples per thread (fold)
+  static constexpr bool WAVE = NTW <= 64;         // windows within one wavefront: no workgroup barriers
   // the channel's chirp in LDS after the rows (N <= 4096), else in registers
   static constexpr bool CH_LDS = N <= 4096;
 };
@@ -42,13 +45,13 @@ template <int N> struct OnePass {
 constexpr int kOnePassMinLog2 = 7, kOnePassMaxLog2 = 13;
 
 // LDS bytes of the exchange rows for an n-point transform (host side).
+inline uint32_t onepass_ntw(uint32_t n) { return n / (n <= 512 ? 8 : 16); }
+inline uint32_t onepass_threads(uint32_t n) { return onepass_ntw(n) < 256 ? 256 : onepass_ntw(n); }
+inline uint32_t onepass_wf(uint32_t n) { return onepass_threads(n) / onepass_ntw(n); }
 inline size_t onepass_lds_bytes(uint32_t n)
 {
-  const uint32_t ntw = n / 16, nt = ntw < 256 ? 256 : ntw;
-  return (size_t(nt / ntw) * (n + n / 16) + (n <= 4096 ? n : 0)) * sizeof(float2);
+  return (size_t(onepass_wf(n)) * (n + n / 16) + (n <= 4096 ? n : 0)) * sizeof(float2);
 }
-inline uint32_t onepass_threads(uint32_t n) { return n / 16 < 256 ? 256 : n / 16; }
-inline uint32_t onepass_wf(uint32_t n) { return onepass_threads(n) / (n / 16); }
 
 struct OnePassArgs {
   uint32_t nchan, npol, npart;
@@ -67,7 +70,7 @@ __global__ __launch_bounds__(OnePass<N>::NT) void k_onepass(LD ld, OnePassArgs a
                                                             FoldArgs fa, const float2* __restrict__ tw)
 {
   using S = OnePass<N>;
-  constexpr int E = S::E, NTW = S::NTW, NT = S::NT, WF = S::WF, R0 = S::R0, RL = S::RL, QG = S::QG;
+  constexpr int E = S::E, NTW = S::NTW, NT = S::NT, WF = S::WF, R0 = S::R0, RL = S::RL;
   using LAY = RowLayout<N, NTW>;
   extern __shared__ float2 smem[];
   const int slot = WF > 1 ? int(threadIdx.x) / NTW : 0;
@@ -84,6 +87,7 @@ __global__ __launch_bounds__(OnePass<N>::NT) void k_onepass(LD ld, OnePassArgs a
   const uint32_t hist_len = MODE == kStageCFold ? fa.nslot * fa.nval * fa.nbin : 0;
   if (MODE == kStageCFold)
     for (uint32_t i = threadIdx.x; i < hist_len; i += NT) hist[i] = 0.f;
+  constexpr int QG = E / 4;
 
   // the channel's chirp, evaluated once per workgroup: into LDS (read after
   // the first exchange's barrier), or the bins X[j + r*N/RL] (j = tid +
@@ -97,6 +101,10 @@ __global__ __launch_bounds__(OnePass<N>::NT) void k_onepass(LD ld, OnePassArgs a
 #pragma unroll
       for (int r = 0; r < RL; r++) ch[q * RL + r] = chirp_onchip(c, chan, uint32_t(tid0 + q * NTW + r * (N / RL)), N);
   }
+  // the histogram's zeros and the chirp table are seen by every wavefront
+  // (the loop below has no workgroup barrier where windows are a
+  // wavefront's)
+  __syncthreads();
   int tid = tid0;
   float2* lds = lds0;
 
@@ -110,8 +118,13 @@ __global__ __launch_bounds__(OnePass<N>::NT) void k_onepass(LD ld, OnePassArgs a
     const uint64_t t0 = uint64_t(valid ? w : w_begin) * a.step;
     // the lane's index arithmetic (LDS addresses of every pass) is redone
     // per window rather than held in registers across the loop
-    tid = opaque_i(tid0);
-    lds = smem + slot * S::LROW;
+#ifndef DSPX_OP_REINDEX
+#define DSPX_OP_REINDEX 1
+#endif
+    if (DSPX_OP_REINDEX) {
+      tid = opaque_i(tid0);
+      lds = smem + slot * S::LROW;
+    }
     float2 y0[E], v[E];
     for (uint32_t pol = 0; pol < a.npol; pol++) {
 #pragma unroll
@@ -145,12 +158,12 @@ __global__ __launch_bounds__(OnePass<N>::NT) void k_onepass(LD ld, OnePassArgs a
       } else {
         // regroup through LDS: each thread then holds QG groups of 4
         // consecutive samples, (g*NTW + tid)*4 + k
-        __syncthreads();
+        xsync<NTW>();
 #pragma unroll
         for (int q = 0; q < E / R0; q++)
 #pragma unroll
           for (int r = 0; r < R0; r++) lds[LAY::idx(tid + q * NTW + r * (N / R0), 0)] = v[q * R0 + r];
-        __syncthreads();
+        xsync<NTW>();
 #pragma unroll
         for (int g = 0; g < QG; g++)
 #pragma unroll
