@@ -1,19 +1,22 @@
 """Throughput of the coherent-dedispersion + fold hot path on one MI355X.
 
-One step = one block of the 128-channel workload through the whole path: the
-block's fold plan (bin of every output sample, sub-integration slots, hits),
-then unpack -> overlap-save convolution -> full-Stokes detection -> fold into
-1024 bins x 8 sub-integrations, raw int8 input resident in HBM before timing.
-Prints one JSON line (see the keys below).
+Default workload (the configuration the metric is quoted on): the uneven
+batch, 48 beams of 32-512 channels, 2^18-2^22 samples per channel, int8 and
+int16, DM 5-1500 at 300-1500 MHz, full Stokes, 512-2048 bins.  One step is
+one graph replay of the whole batch (unpack -> overlap-save convolution ->
+detection -> fold for every beam), raw input resident in HBM.  The other
+workloads (--workload multichan|single, and the engines beside the path)
+are kept for comparison.  Prints one JSON line (see the keys below).
 
-  python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload multichan|single|uneven]
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload uneven|multichan|single|...]
   python bench.py --copy-probe MIB   # a plain device copy of MIB MiB, for
                                      # calibrating the HBM byte counters
 
-roofline.traffic is the HBM bytes per launch of the dominant kernel read by
-FETCH_SIZE / WRITE_SIZE counter runs (scaled by the same counters' reading of
-a copy kernel of known size); it is taken from profiles/traffic_<workload>.json
-when that file exists and is null otherwise.
+roofline.traffic is the HBM bytes per launch (per step, for the batch) read
+by FETCH_SIZE / WRITE_SIZE counter runs (scaled by the same counters'
+reading of a copy kernel of known size); it is taken from
+profiles/traffic_<workload>.json when that file exists and is null
+otherwise.
 """
 import argparse
 import importlib
@@ -171,9 +174,10 @@ def draw_uneven(dspx, nbeam, seed=3, ndat_max_log2=22.0):
     512}, DM log-uniform 5-1500, centre frequency uniform 300-1500 MHz, band
     min(0.25*fc, 400) MHz, 2^U(18, 22) samples per channel (multiple of 64),
     8- or 16-bit, period log-uniform 5-500 ms, duty 1-5%, 512/1024/2048 bins.
-    Draws whose overlap exceeds a quarter of the block, whose smearing the
-    reference rule rejects, or whose reference FFT length exceeds 2^20 (the
-    fused form's longest) are drawn again."""
+    Draws whose overlap exceeds a quarter of the block, whose reference FFT
+    length exceeds the block (the reference needs one FFT per block,
+    Convolution.C:485-487), or whose smearing the reference rule rejects, are
+    drawn again."""
     import numpy as np
     rng = np.random.default_rng(seed)
     beams = []
@@ -191,17 +195,98 @@ def draw_uneven(dspx, nbeam, seed=3, ndat_max_log2=22.0):
             g = dspx.geometry(fc, bw, nchan, dm)
         except dspx.DspxError:
             continue
-        if g["noverlap"] > ndat / 4 or g["nfft"] > (1 << 20):
+        if g["noverlap"] > ndat / 4 or g["nfft"] > ndat:
             continue
         beams.append(dict(nchan=nchan, npol=2, ndat=ndat, cfreq=fc, bw=bw, dm=dm, nbit=nbit, period=period,
                           duty=duty, nbin=nbin, geom=g))
     return beams
 
 
+def reference_bytes(b, g):
+    """Algorithmic HBM bytes of one beam on the reference-rule basis: FFT
+    length N, overlap and r = N/step from the reference's geometry rule;
+    B_in = 2*nbit/8 per complex sample; A = r*B_in per input sample where the
+    whole transform stays on chip (N <= 2^14), r*(B_in + 32) for the
+    two-level form (the window written and read back twice)."""
+    n = b["nchan"] * b["npol"] * b["ndat"]
+    r = g["nfft"] / g["step"]
+    b_in = 2 * b["nbit"] / 8
+    a = r * b_in if g["nfft"] <= (1 << 14) else r * (b_in + 32)
+    return n * a
+
+
+def cpu_baseline_uneven(beams, raws, dspx, budget_s=20.0, threads=16):
+    """The oracle chain (unpack, FP32 overlap-save convolution at the
+    reference FFT length with the oracle's own FFT, full-Stokes detection,
+    fold) on the host cores, whole channels of the batch's beams taken
+    round-robin over the beams (so that the sample follows the batch's mix
+    of lengths, channel counts and sample widths), on up to ``threads``
+    threads (16 by default: the host CPU share per GPU), until the time
+    budget is spent.  The threads run the oracle's C functions, which release
+    the interpreter lock."""
+    from concurrent.futures import ThreadPoolExecutor
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from oracle import oracle as O
+    from helpers import oracle_load_to_fold
+    O.lib()
+    threads = max(1, min(threads, os.cpu_count() or 1))
+    order = []
+    for c in range(max(b["nchan"] for b in beams)):
+        for i, b in enumerate(beams):
+            if c < b["nchan"]:
+                order.append((i, c))
+
+    def task(ic):
+        i, c = ic
+        b, g = beams[i], beams[i]["geom"]
+        raw = np.ascontiguousarray(raws[i][:, c:c + 1].cpu().numpy())
+        rate = b["bw"] / b["nchan"] * 1e6
+        npart = (b["ndat"] - g["noverlap"]) // g["step"]
+        chunks = dspx.fold_chunks(npart * g["step"], rate, b["period"], g["impulse_pos"])
+        t0 = time.perf_counter()
+        # channel c alone: a one-channel band of its centre and width has its chirp
+        cf = b["cfreq"] - 0.5 * b["bw"] + (c + 0.5) * b["bw"] / b["nchan"]
+        oracle_load_to_fold(O, raw, b["nbit"], 1, b["npol"], b["ndat"], cf, b["bw"] / b["nchan"], b["dm"],
+                            g, dspx.STOKES, 4, b["nbin"], (1.0 / rate) / b["period"], chunks, 1)
+        return b["npol"] * b["ndat"], time.perf_counter() - t0
+
+    # one warm channel alone: the single-core rate
+    n1, t1 = task(order[0])
+    done_samp, k = 0, 0
+    t_start = time.perf_counter()
+    with ThreadPoolExecutor(threads) as ex:
+        while k < len(order) and time.perf_counter() - t_start < budget_s:
+            res = list(ex.map(task, order[k:k + threads]))
+            done_samp += sum(r[0] for r in res)
+            k += len(res)
+    t_all = time.perf_counter() - t_start
+    nbeams = len(set(i for i, _ in order[:k]))
+    return {"value": round(done_samp / t_all / 1e6, 2), "unit": "Msamples/s", "cores": threads, "kind": "port",
+            "single_core_value": round(n1 / t1 / 1e6, 2), "nproc": os.cpu_count(), "cpu_model": _cpu_model(),
+            "sample": "%d channels (%d of the %d beams, taken round-robin over the beams), %.3g complex samples, "
+                      "each channel through the oracle chain (unpack, FP32 overlap-save convolution at the "
+                      "reference FFT length, Stokes detection, fold) on %d threads, %.1f s wall" % (
+                          k, nbeams, len(beams), done_samp, threads, t_all)}
+
+
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
 def run_uneven(args, dspx, synth, ctx):
-    """Fused batch (autotuned FFT lengths, replayed from a captured graph)
-    against the plain FFT-library form run beam after beam, on the same
-    synthetic beams.  Full Stokes, one sub-integration per beam."""
+    """The uneven batch (BASELINE's headline configuration): the fused batch
+    (autotuned FFT lengths, one-pass or three-stage form per beam, replayed
+    from a captured graph) against the plain FFT-library form run beam after
+    beam, on the same synthetic beams, with the oracle chain on the host's
+    cores beside them.  Full Stokes, one sub-integration per beam."""
     import numpy as np
     import torch
     beams = draw_uneven(dspx, args.nbeam, ndat_max_log2=args.uneven_log2)
@@ -234,20 +319,29 @@ def run_uneven(args, dspx, synth, ctx):
     B.capture(raws, profs, hits)
     for _ in range(args.warmup):
         B.replay()
+    stream = torch.cuda.current_stream()  # the library's stream (the context was given it)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
+    e0.record(stream)
     for _ in range(args.steps):
         B.replay()
+    e1.record(stream)
     torch.cuda.synchronize()
     t_f = (time.perf_counter() - t0) / args.steps
-    print("[bench] fused batch %.3f ms/step (autotune %.1f s)" % (t_f * 1e3, tune_s), file=sys.stderr, flush=True)
-    # algorithmic bytes of the path as run: every FFT window's raw samples
-    # plus the three 8-byte passes (A write, B read+write, C read)
-    abytes = 0
+    t_ev = e0.elapsed_time(e1) * 1e-3 / args.steps
+    print("[bench] fused batch %.3f ms/step (events %.3f ms; autotune %.1f s)" % (t_f * 1e3, t_ev * 1e3, tune_s),
+          file=sys.stderr, flush=True)
+    ref_bytes = sum(reference_bytes(b, b["geom"]) for b in beams)
+    run_bytes = 0  # the windows as run: the chosen lengths and forms
+    forms = {"onepass": 0, "twolevel": 0}
     for i, b in enumerate(beams):
         info = B.beams[i]
         n = b["nchan"] * b["npol"] * info["npart"] * info["nfft"]
-        abytes += n * (2 * b["nbit"] / 8 + 32)
+        b_in = 2 * b["nbit"] / 8
+        one = info["form"] == dspx.FORM_ONEPASS
+        run_bytes += n * (b_in if one else b_in + 32)
+        forms["onepass" if one else "twolevel"] += 1
     nfft_tuned = [(B.beams[i]["nfft"], b["geom"]["nfft"]) for i, b in enumerate(beams)]
     B.close()
     del profs, hits
@@ -274,24 +368,33 @@ def run_uneven(args, dspx, synth, ctx):
             torch.cuda.empty_cache()
         print("[bench] plain beam by beam %.3f ms" % (t_p * 1e3), file=sys.stderr, flush=True)
     value = nsamp / t_f / 1e6
+    traffic = load_traffic("uneven", "batch")
     out = {"metric": METRIC, "value": round(value, 2), "unit": "Msamples/s", "n_gpus": 1, "steps": args.steps,
            "warmup": args.warmup, "ms_per_step": round(t_f * 1e3, 3), "higher_is_better": True, "scaling": "weak",
            "vs_baseline": None, "dtype": "f32",
            "data": "synthetic (noise plus a pulse dispersed between channels, not within them)",
-           "config": {"workload": "uneven: %d beams, nchan %s, 2^18-2^%g samples per channel, int8/int16, "
-                                  "full Stokes, 512-2048 bins" % (len(beams), sorted(set(b["nchan"] for b in beams)),
-                                                                  args.uneven_log2),
+           "config": {"workload": "uneven batch: %d beams (seed 3), nchan %s, 2^18-2^%g samples per channel, "
+                                  "int8/int16, full Stokes, 512-2048 bins, one graph replay per step" % (
+                                      len(beams), sorted(set(b["nchan"] for b in beams)), args.uneven_log2),
                       "samples_per_step": nsamp, "form": "fused batch, autotuned nfft, graph replay",
-                      "nfft_tuned_vs_reference": nfft_tuned},
-           "roofline": {"bound": "hbm", "kernel": "whole batch (graph replay)",
-                        "achieved": round(abytes / t_f / 1e9, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                        "frac": round(abytes / t_f / 1e9 / HBM_PEAK_GBS, 4), "traffic": None,
-                        "algorithmic_bytes_per_step": int(abytes)},
+                      "forms": forms, "nfft_tuned_vs_reference": nfft_tuned},
+           "roofline": {"bound": "hbm", "kernel": "whole batch (graph replay of every beam's kernels)",
+                        "achieved": round(ref_bytes / t_ev / 1e9, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": round(ref_bytes / t_ev / 1e9 / HBM_PEAK_GBS, 4), "traffic": traffic,
+                        "basis": "reference-rule geometry per beam: A = r*B_in (N <= 2^14), r*(B_in + 32) above",
+                        "algorithmic_bytes_per_launch": int(ref_bytes), "ms_per_launch_events": round(t_ev * 1e3, 3),
+                        "as_run": {"algorithmic_bytes_per_launch": int(run_bytes),
+                                   "achieved": round(run_bytes / t_ev / 1e9, 1),
+                                   "frac": round(run_bytes / t_ev / 1e9 / HBM_PEAK_GBS, 4),
+                                   "basis": "the chosen lengths: raw bytes of every window (one-pass), plus 32 B "
+                                            "per window sample (three-stage)"}},
            "autotune_seconds": round(tune_s, 2)}
     if t_p is not None:
         out["plain_fp32"] = {"value": round(nsamp / t_p / 1e6, 2), "ms_per_step": round(t_p * 1e3, 3),
                              "form": "FFT-library form, beam after beam, reference FFT lengths"}
         out["fused_vs_plain"] = round(t_p / t_f, 3)
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline_uneven(beams, raws, dspx, args.cpu_budget)
     print(json.dumps(out))
 
 
@@ -492,7 +595,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="multichan", choices=list(WORKLOADS) + ["uneven"] + list(ENGINE_WORKLOADS))
+    ap.add_argument("--workload", default="uneven", choices=["uneven"] + list(WORKLOADS) + list(ENGINE_WORKLOADS))
     ap.add_argument("--nbeam", type=int, default=48)
     ap.add_argument("--uneven-log2", type=float, default=22.0, help="largest log2 samples per channel (uneven)")
     ap.add_argument("--no-cpu", action="store_true")

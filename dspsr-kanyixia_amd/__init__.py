@@ -707,7 +707,7 @@ class Batch:
             d, npart, nout = LtfDesc(), U64(), U64()
             ctx.check(lib().dspx_batch_beam(h, i, C.byref(d), C.byref(npart), C.byref(nout)))
             self.beams.append({"nfft": d.nfft, "nfilt_pos": d.nfilt_pos, "nfilt_neg": d.nfilt_neg,
-                               "npart": npart.value, "ndat_out": nout.value,
+                               "npart": npart.value, "ndat_out": nout.value, "form": d.form,
                                "nval": 1 if d.state == INTENSITY else (d.npol if d.state == PPQQ else 4)})
 
     def close(self):

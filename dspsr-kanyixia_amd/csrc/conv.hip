@@ -1578,7 +1578,10 @@ int dspx_batch_beam(const dspx_batch* B, uint32_t ibeam, dspx_ltf_desc* desc, ui
 {
   if (!B || ibeam >= B->beams.size()) return DSPX_E_PARAM;
   const dspx_ltf* L = B->beams[ibeam];
-  if (desc) *desc = L->d;
+  if (desc) {
+    *desc = L->d;
+    desc->form = L->onepass ? DSPX_FORM_ONEPASS : DSPX_FORM_TWOLEVEL;  // the form the plan runs
+  }
   if (npart) *npart = L->npart;
   if (ndat_out) *ndat_out = L->ndat_out;
   return DSPX_OK;

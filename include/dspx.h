@@ -478,8 +478,9 @@ enum { DSPX_BATCH_AUTOTUNE = 1 };
 int dspx_batch_create(dspx_ctx* ctx, const dspx_ltf_desc* beams, uint32_t nbeam, uint32_t flags,
                       dspx_batch** out);
 void dspx_batch_destroy(dspx_batch* batch);
-/* The plan of beam ibeam: its descriptor with the chosen nfft / overlap,
-   FFT parts and output samples per block. */
+/* The plan of beam ibeam: its descriptor with the chosen nfft / overlap and
+   the form it runs (DSPX_FORM_ONEPASS or DSPX_FORM_TWOLEVEL), FFT parts and
+   output samples per block. */
 int dspx_batch_beam(const dspx_batch* batch, uint32_t ibeam, dspx_ltf_desc* desc, uint64_t* npart,
                     uint64_t* ndat_out);
 /* As dspx_ltf_set_fold for beam ibeam (before capture). */
