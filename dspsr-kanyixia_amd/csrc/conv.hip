@@ -14,6 +14,7 @@
 #include "fold_impl.hpp"
 #include "onepass.hpp"
 #include "plain.hpp"
+#include "stageb_mfma.hpp"
 #include "stages.hpp"
 
 namespace dspx {
@@ -57,6 +58,11 @@ struct ConvCore {
   // of the two measured for each row length); a tuning parameter
   // (environment DSPX_ROW_E overrides the default)
   int row_e = 16;
+  // the matrix-core form of 4096-point rows (stageb_mfma.hpp): the largest
+  // magnitude of an unpacked input component (0: unknown, float input, the
+  // form is not used) and whether it is selected
+  float in_max = 0.f;
+  bool row_mfma = false;
 
   void init(dspx_ctx* c, uint32_t nchan_, uint32_t npol_, uint32_t N_, uint32_t pos, uint32_t neg,
             uint32_t mode, const dspx_dedisp_desc& d, bool one = false)
@@ -107,6 +113,7 @@ struct ConvCore {
     row_e = N2 == 512 || N2 == 1024 ? 8 : 16;
     if (const char* e = std::getenv("DSPX_ROW_E")) row_e = std::atoi(e) == 8 ? 8 : 16;
     if (N2 < 256 * 2 && row_e == 8) row_e = 16;
+    if (const char* e = std::getenv("DSPX_STAGEB")) row_mfma = std::string(e) == "mfma";
     if (mode != DSPX_CHIRP_TABLE) {
       DSPX_REQUIRE(d.nchan == nchan, DSPX_E_PARAM, "dedisp.nchan=%u != nchan=%u", d.nchan, nchan);
       DSPX_REQUIRE(d.bandwidth != 0.0 && d.centre_frequency > 0.0, DSPX_E_PARAM, "invalid band");
@@ -217,6 +224,18 @@ struct ConvCore {
     uint64_t nb = rows * npg;
     DSPX_REQUIRE(nb < (1ull << 31), DSPX_E_UNSUPPORTED, "grid too large");
     const float2* twr = twiddles(ctx, N2);
+    if (row_mfma && N2 == 4096 && in_max > 0.f) {
+      // scale on load so that no complex magnitude exceeds 46341 (its FP16
+      // components then stay below 65504 through every pass): stage A's
+      // N1-point column sums reach N1 * sqrt(2) * in_max at most
+      const double peak = double(N1) * std::sqrt(2.0) * double(in_max);
+      const int p = int(std::ceil(std::log2(peak / 46341.0)));
+      const int q = int(m) / 2;
+      MfmaScale sc{std::ldexp(1.0f, -p), std::ldexp(1.0f, q), std::ldexp(1.0f, p + 24 - q)};
+      k_stageB_mfma<<<unsigned(nb), 256, 0, ctx->stream>>>(s, c, twr, ppg, sc);
+      DSPX_HIP(hipGetLastError());
+      return;
+    }
 #define CASE_B(n, e)                                                                      \
   case n * 100 + e: {                                                                     \
     constexpr int RW = stage_b_rows(n, e);                                                \
@@ -927,6 +946,8 @@ dspx_ltf* ltf_create(dspx_ctx* ctx, const dspx_ltf_desc* d, LtfScratch* ext)
         L->onepass = d->form == DSPX_FORM_ONEPASS || (d->form == DSPX_FORM_FUSED && fits);
         L->core.init(ctx, d->nchan, d->npol, d->nfft, d->nfilt_pos, d->nfilt_neg, d->chirp_mode, d->dedisp,
                      L->onepass);
+        // largest unpacked component (LoadI8: |(v + 0.5) * scale| <= 127.5 |scale|; LoadI16: 32767.5)
+        L->core.in_max = d->nbit == 8 ? 127.5f * std::fabs(d->scale) : 32767.5f;
         DSPX_REQUIRE(L->npart < (1u << 30), DSPX_E_UNSUPPORTED, "npart too large");
         DSPX_REQUIRE(L->onepass || L->npart < (1u << 20), DSPX_E_UNSUPPORTED, "npart too large");
         L->core.reserve_work(uint32_t(L->npart));
