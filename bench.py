@@ -343,7 +343,37 @@ def run_uneven(args, dspx, synth, ctx):
         run_bytes += n * (b_in if one else b_in + 32)
         forms["onepass" if one else "twolevel"] += 1
     nfft_tuned = [(B.beams[i]["nfft"], b["geom"]["nfft"]) for i, b in enumerate(beams)]
+    tuned = [(B.beams[i]["nfft"], B.beams[i]["nfilt_pos"], B.beams[i]["nfilt_neg"]) for i in range(len(beams))]
     B.close()
+    t_m = None
+    if not args.no_mfma:
+        # the same lengths with stage B's 4096-point rows on the matrix cores
+        os.environ["DSPX_STAGEB"] = "mfma"
+        try:
+            d2 = [dspx.ltf_desc(b["nbit"], b["nchan"], b["npol"], b["ndat"], b["cfreq"], b["bw"], b["dm"], b["nbin"],
+                                state=dspx.STOKES, ndim=4, nfft=tuned[i][0], nfilt_pos=tuned[i][1],
+                                nfilt_neg=tuned[i][2]) for i, b in enumerate(beams)]
+            B2 = dspx.Batch(ctx, d2)
+            for i, b in enumerate(beams):
+                rate = abs(b["bw"]) / b["nchan"] * 1e6
+                info = B2.beams[i]
+                B2.set_fold(i, (1.0 / rate) / b["period"], dspx.fold_chunks(info["ndat_out"], rate, b["period"],
+                                                                           info["nfilt_pos"]))
+            B2.execute(raws, profs, hits)
+            B2.capture(raws, profs, hits)
+            B2.replay()
+            torch.cuda.synchronize()
+            e0.record(stream)
+            for _ in range(args.steps):
+                B2.replay()
+            e1.record(stream)
+            torch.cuda.synchronize()
+            t_m = e0.elapsed_time(e1) * 1e-3 / args.steps
+            B2.close()
+        finally:
+            del os.environ["DSPX_STAGEB"]
+        print("[bench] fused batch, stage B on the matrix cores %.3f ms/step" % (t_m * 1e3), file=sys.stderr,
+              flush=True)
     del profs, hits
     torch.cuda.empty_cache()
     t_p = None
@@ -393,6 +423,12 @@ def run_uneven(args, dspx, synth, ctx):
         out["plain_fp32"] = {"value": round(nsamp / t_p / 1e6, 2), "ms_per_step": round(t_p * 1e3, 3),
                              "form": "FFT-library form, beam after beam, reference FFT lengths"}
         out["fused_vs_plain"] = round(t_p / t_f, 3)
+    if t_m is not None:
+        out["fused_mfma"] = {"value": round(nsamp / t_m / 1e6, 2), "ms_per_step": round(t_m * 1e3, 3),
+                             "form": "the same batch and lengths with stage B's 4096-point rows as FP16 hi/lo-split "
+                                     "matrix-core products (FP32 accumulation); by HIP events"}
+        if t_p is not None:
+            out["fused_mfma_vs_plain"] = round(t_p / t_m, 3)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline_uneven(beams, raws, dspx, args.cpu_budget)
     print(json.dumps(out))
@@ -600,6 +636,7 @@ def main():
     ap.add_argument("--uneven-log2", type=float, default=22.0, help="largest log2 samples per channel (uneven)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-mfma", action="store_true", help="skip the matrix-core stage B timing (uneven)")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB")
     args = ap.parse_args()
