@@ -133,6 +133,9 @@ _SIGS = {
     "dspx_sk_compute": (I32, [P, P, U64, U64, U64, I32, P, P]),
     "dspx_sk_compute_raw": (I32, [P, P, U32, U32, C.c_float, U64, U64, P, P]),
     "dspx_sk_detect": (I32, [P, P, P, U64, U32, P]),
+    "dspx_sk_tscr_totals": (I32, [P, P, U64, P]),
+    "dspx_sk_set_tscr_limits": (I32, [P, U32, P, P, P]),
+    "dspx_sk_detect_tscr": (I32, [P, P, P, U64, P]),
     "dspx_sk_mask": (I32, [P, P, P, U64, U64, P, U64, U64, U64]),
     "dspx_batch_create": (I32, [P, C.POINTER(LtfDesc), U32, U32, C.POINTER(P)]),
     "dspx_batch_destroy": (None, [P]),
@@ -558,11 +561,31 @@ class SpectralKurtosis:
 
     __del__ = close
 
-    def set_limits(self, lower, upper, symmetric, pcsd):
+    def set_limits(self, lower, upper, symmetric, pcsd, std_devs=None, Nd=1):
+        """Thresholds by block count 0..M.  With std_devs, detect() also
+        evaluates SKLimits(total) for the channel totals above M that
+        detect_tscr meets (get_limits, SpectralKurtosis.C:989-1012, 1045)."""
         import numpy as np
         lo, up, sy = (np.ascontiguousarray(a, np.float32) for a in (lower, upper, symmetric))
         assert lo.size == up.size == sy.size == self.M + 1
         self.ctx.check(lib().dspx_sk_set_limits(self.h, lo.ctypes.data, up.ctypes.data, sy.ctypes.data, pcsd))
+        self.std_devs, self.Nd = std_devs, Nd
+        self._tscr = {}
+
+    def tscr_totals(self, counts, npart):
+        import numpy as np
+        t = np.zeros(self.nchan, np.uint64)
+        self.ctx.check(lib().dspx_sk_tscr_totals(self.h, ptr(counts), npart, t.ctypes.data))
+        return t
+
+    def set_tscr_limits(self, limits):
+        """limits: {total: (lower, upper)} for totals above M."""
+        import numpy as np
+        tot = np.array(sorted(limits), np.uint64)
+        lo = np.array([limits[int(t)][0] for t in tot], np.float32)
+        up = np.array([limits[int(t)][1] for t in tot], np.float32)
+        self.ctx.check(lib().dspx_sk_set_tscr_limits(self.h, tot.size, tot.ctypes.data, lo.ctypes.data,
+                                                     up.ctypes.data))
 
     def compute(self, x, chan_stride, pol_stride, npart, sums, counts, zeroed=False):
         self.ctx.check(lib().dspx_sk_compute(self.h, ptr(x), chan_stride, pol_stride, npart, int(zeroed),
@@ -573,7 +596,21 @@ class SpectralKurtosis:
                                                  ptr(sums), ptr(counts)))
 
     def detect(self, sums, counts, npart, flags, mask):
-        self.ctx.check(lib().dspx_sk_detect(self.h, ptr(sums), ptr(counts), npart, flags, ptr(mask)))
+        if not (flags & 4) or getattr(self, "std_devs", None) is None:
+            self.ctx.check(lib().dspx_sk_detect(self.h, ptr(sums), ptr(counts), npart, flags, ptr(mask)))
+            return
+        # detect_skfb / detect_fscr, then the totals they leave, SKLimits of
+        # the totals not seen before, then detect_tscr
+        self.ctx.check(lib().dspx_sk_detect(self.h, ptr(sums), ptr(counts), npart, flags & 3, ptr(mask)))
+        need = sorted({int(t) for t in self.tscr_totals(counts, npart)
+                       if self.M < t < 32768 and int(t) not in self._tscr})
+        if need:
+            from .sklimits import SKLimits
+            for t in need:
+                L = SKLimits(t, self.std_devs, self.Nd, self.npol)
+                self._tscr[t] = (L.lower, L.upper)
+            self.set_tscr_limits(self._tscr)
+        self.ctx.check(lib().dspx_sk_detect_tscr(self.h, ptr(sums), ptr(counts), npart, ptr(mask)))
 
     def mask(self, mask, x, ics, ips, out, ocs, ops, npart):
         self.ctx.check(lib().dspx_sk_mask(self.h, ptr(mask), ptr(x), ics, ips, ptr(out), ocs, ops, npart))

@@ -8,6 +8,9 @@
 // where a decision depends on it: float sq = re*re + im*im and sq*sq, both
 // summed in double; the float estimator of detect_skfb; double averages of
 // detect_fscr; sequential double sums of detect_tscr.
+#include <algorithm>
+#include <cstring>
+
 #include "dspx_internal.hpp"
 #include "stages.hpp"
 
@@ -16,6 +19,12 @@ struct dspx_sk {
   dspx_ctx* ctx = nullptr;
   dspx_sk_desc d{};
   dspx::DevBuf lim;  // lower[M+1], upper[M+1], symmetric[M+1]
+  // detect_tscr's thresholds of the totals above M: totals (ascending),
+  // then lower and upper, ntscr of each
+  dspx::DevBuf tscr;
+  uint32_t ntscr = 0;
+  dspx::DevBuf status;  // detect_tscr: [0] totals without thresholds, [1] the smallest of them
+  dspx::DevBuf tot;     // per-channel totals (dspx_sk_tscr_totals)
   double pcsd = 0;
   bool have_limits = false;
 };
@@ -289,16 +298,43 @@ __global__ __launch_bounds__(kSkThreads) void k_sk_fscr(const float* __restrict_
     for (uint32_t ch = tid; ch < nchan; ch += kSkThreads) mrow[ch] = 1;
 }
 
+// The per-channel totals of the counts (detect_tscr's M_t before its test,
+// SpectralKurtosis.C:1036-1040), one workgroup per channel.
+__global__ __launch_bounds__(kSkThreads) void k_sk_totals(const uint32_t* __restrict__ counts, uint64_t npart,
+                                                         uint32_t nchan, unsigned long long* __restrict__ totals)
+{
+  __shared__ unsigned long long rc[kSkThreads];
+  const uint32_t ch = blockIdx.x;
+  const int tid = threadIdx.x;
+  unsigned long long c = 0;
+  for (uint64_t p = tid; p < npart; p += kSkThreads) c += counts[p * nchan + ch];
+  rc[tid] = c;
+  for (int st = kSkThreads / 2; st > 0; st >>= 1) {
+    __syncthreads();
+    if (tid < st) rc[tid] += rc[tid + st];
+  }
+  if (tid == 0) totals[ch] = rc[0];
+}
+
 // detect_tscr (SpectralKurtosis.C:1033-1095): one workgroup per channel.
 // The reference adds the blocks' sums in block order; here they are added in
 // a fixed tree, in double precision as there, which can move the statistic
-// only in the last bits of a double.  Totals above M use the closed form of
-// SKLimits.C:152-160 when >= 32768; totals in between are not tested.
+// only in the last bits of a double.  Every total is tested with
+// SKLimits(total) (get_limits, SpectralKurtosis.C:989-1012, 1045): totals up
+// to M from the M+1 tables, larger ones from the table of totals (ttot,
+// ascending), else, from 32768 on, the closed form of SKLimits.C:152-160.
+// A total in (M, 32768) absent from the table is not tested and is counted
+// in status[0] (status[1]: the smallest such total), so that the caller
+// fails rather than return a mask that misses it.
 __global__ __launch_bounds__(kSkThreads) void k_sk_tscr(const float* __restrict__ sums,
                                                        const uint32_t* __restrict__ counts, uint64_t npart,
                                                        uint32_t nchan, uint32_t npol, uint32_t M, uint32_t Nd,
                                                        const float* __restrict__ lower,
                                                        const float* __restrict__ upper, double pcsd,
+                                                       const unsigned long long* __restrict__ ttot,
+                                                       const float* __restrict__ tlo,
+                                                       const float* __restrict__ tup, uint32_t ntscr,
+                                                       unsigned long long* __restrict__ status,
                                                        uint8_t* __restrict__ mask)
 {
 #pragma clang fp contract(off)
@@ -345,12 +381,28 @@ __global__ __launch_bounds__(kSkThreads) void k_sk_tscr(const float* __restrict_
     } else if (total <= M) {
       lo = lower[total];
       up = upper[total];
-    } else if (total >= 32768) {
-      double factor = sqrt(4.0 / double(total)) * pcsd;
-      lo = float(1.0f - factor);
-      up = float(1.0f + factor);
     } else {
-      test = false;  // totals between M and 32768 are not tested
+      // binary search of the table of totals
+      uint32_t a = 0, b = ntscr;
+      while (a < b) {
+        const uint32_t m = (a + b) / 2;
+        if (ttot[m] < total)
+          a = m + 1;
+        else
+          b = m;
+      }
+      if (a < ntscr && ttot[a] == total) {
+        lo = tlo[a];
+        up = tup[a];
+      } else if (total >= 32768) {
+        double factor = sqrt(4.0 / double(total)) * pcsd;
+        lo = float(1.0f - factor);
+        up = float(1.0f + factor);
+      } else {
+        test = false;
+        atomicAdd(status, 1ull);
+        atomicMin(status + 1, (unsigned long long) total);
+      }
     }
     if (test) {
       const float M_t = float(total);
@@ -391,6 +443,36 @@ __global__ void k_sk_mask(const uint8_t* __restrict__ mask, const float* __restr
     if (mask[(k / M) * nchan + chan]) v = make_float2(0.f, 0.f);
     y[k] = v;
   }
+}
+
+__global__ void k_sk_status_init(unsigned long long* status)
+{
+  status[0] = 0;
+  status[1] = ~0ull;
+}
+
+// detect_tscr over the blocks' sums, OR-ed into the mask; waits for the
+// stream and fails when a channel's total has no thresholds.
+void run_tscr(dspx_sk* s, const float* d_sums, const uint32_t* d_counts, uint64_t npart, uint8_t* d_mask)
+{
+  const dspx_sk_desc& d = s->d;
+  hipStream_t st = s->ctx->stream;
+  s->status.reserve(2 * sizeof(unsigned long long));
+  auto* status = s->status.as<unsigned long long>();
+  const float* lower = s->lim.as<float>();
+  const float* upper = lower + d.M + 1;
+  const auto* ttot = s->tscr.as<unsigned long long>();
+  const float* tlo = reinterpret_cast<const float*>(ttot + s->ntscr);
+  k_sk_status_init<<<1, 1, 0, st>>>(status);
+  k_sk_tscr<<<d.nchan, kSkThreads, 0, st>>>(d_sums, d_counts, npart, d.nchan, d.npol, d.M, d.Nd, lower, upper,
+                                            s->pcsd, ttot, tlo, tlo + s->ntscr, s->ntscr, status, d_mask);
+  DSPX_HIP(hipGetLastError());
+  unsigned long long h[2];
+  DSPX_HIP(hipMemcpyAsync(h, status, sizeof(h), hipMemcpyDeviceToHost, st));
+  DSPX_HIP(hipStreamSynchronize(st));
+  if (h[0])
+    fail(DSPX_E_STATE, "detect_tscr: %llu channel totals (the smallest %llu) have no thresholds "
+         "(dspx_sk_set_tscr_limits)", h[0], h[1]);
 }
 
 }  // namespace
@@ -511,10 +593,54 @@ int dspx_sk_detect(dspx_sk* s, float* d_sums, uint32_t* d_counts, uint64_t npart
     if (flags & DSPX_SK_FSCR)
       k_sk_fscr<<<unsigned(npart), kSkThreads, 0, st>>>(d_sums, d_counts, d.nchan, d.npol, d.M, d.Nd,
                                                         d.omit_outliers, sym, d_mask);
-    if (flags & DSPX_SK_TSCR)
-      k_sk_tscr<<<d.nchan, kSkThreads, 0, st>>>(d_sums, d_counts, npart, d.nchan, d.npol, d.M, d.Nd, lower,
-                                                        upper, s->pcsd, d_mask);
     DSPX_HIP(hipGetLastError());
+    if (flags & DSPX_SK_TSCR) run_tscr(s, d_sums, d_counts, npart, d_mask);
+  });
+}
+
+int dspx_sk_set_tscr_limits(dspx_sk* s, uint32_t n, const uint64_t* totals, const float* lower, const float* upper)
+{
+  return guard(s ? s->ctx : nullptr, [&] {
+    DSPX_REQUIRE(s && (n == 0 || (totals && lower && upper)), DSPX_E_PARAM, "null argument");
+    for (uint32_t i = 0; i < n; i++) {
+      DSPX_REQUIRE(totals[i] > s->d.M, DSPX_E_PARAM, "total %llu <= M=%u (the M+1 tables hold it)",
+                   (unsigned long long) totals[i], s->d.M);
+      DSPX_REQUIRE(i == 0 || totals[i] > totals[i - 1], DSPX_E_PARAM, "totals not ascending at %u", i);
+    }
+    const size_t bytes = size_t(n) * (sizeof(unsigned long long) + 2 * sizeof(float));
+    std::vector<unsigned char> h(std::max<size_t>(bytes, 1));
+    std::memcpy(h.data(), totals, size_t(n) * sizeof(uint64_t));
+    std::memcpy(h.data() + size_t(n) * 8, lower, size_t(n) * sizeof(float));
+    std::memcpy(h.data() + size_t(n) * 12, upper, size_t(n) * sizeof(float));
+    s->tscr.reserve(h.size());
+    DSPX_HIP(hipMemcpyAsync(s->tscr.p, h.data(), h.size(), hipMemcpyHostToDevice, s->ctx->stream));
+    DSPX_HIP(hipStreamSynchronize(s->ctx->stream));
+    s->ntscr = n;
+  });
+}
+
+int dspx_sk_tscr_totals(dspx_sk* s, const uint32_t* d_counts, uint64_t npart, uint64_t* h_totals)
+{
+  return guard(s ? s->ctx : nullptr, [&] {
+    DSPX_REQUIRE(s && d_counts && h_totals, DSPX_E_PARAM, "null argument");
+    const uint32_t nchan = s->d.nchan;
+    hipStream_t st = s->ctx->stream;
+    s->tot.reserve(size_t(nchan) * sizeof(unsigned long long));
+    k_sk_totals<<<nchan, kSkThreads, 0, st>>>(d_counts, npart, nchan, s->tot.as<unsigned long long>());
+    DSPX_HIP(hipGetLastError());
+    DSPX_HIP(hipMemcpyAsync(h_totals, s->tot.p, size_t(nchan) * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
+    DSPX_HIP(hipStreamSynchronize(st));
+  });
+}
+
+int dspx_sk_detect_tscr(dspx_sk* s, const float* d_sums, const uint32_t* d_counts, uint64_t npart, uint8_t* d_mask)
+{
+  return guard(s ? s->ctx : nullptr, [&] {
+    DSPX_REQUIRE(s && d_sums && d_counts && d_mask, DSPX_E_PARAM, "null argument");
+    DSPX_REQUIRE(s->have_limits, DSPX_E_STATE, "thresholds not set (dspx_sk_set_limits)");
+    DSPX_REQUIRE(npart < (1u << 31), DSPX_E_UNSUPPORTED, "npart=%llu", (unsigned long long) npart);
+    if (!npart) return;
+    run_tscr(s, d_sums, d_counts, npart, d_mask);
   });
 }
 

@@ -175,6 +175,8 @@ void SpectralKurtosisEngine::setup(SpectralKurtosis* s)
   sk = nullptr;
   nchan = in->get_nchan();
   dspx_sk_desc d{nchan, in->get_npol(), s->get_M(), 1u, s->get_omit_outliers() ? 1u : 0u, 0u};
+  setup_M = s->get_M();
+  tscr_valid = false;
   ctx->check(dspx_sk_create(ctx->get(), &d, &sk), "dsp::HIP::SpectralKurtosisEngine::setup");
   ctx->check(dspx_sk_set_limits(sk, s->get_lower().data(), s->get_upper().data(), s->get_symmetric().data(),
                                 s->get_pc_std_devs()),
@@ -198,6 +200,33 @@ void SpectralKurtosisEngine::reset_mask(uint8_t* mask, uint64_t npart)
 void SpectralKurtosisEngine::detect(float* sums, uint32_t* counts, uint64_t npart, unsigned flags, uint8_t* mask)
 {
   ctx->check(dspx_sk_detect(sk, sums, counts, npart, flags, mask), "dsp::HIP::SpectralKurtosisEngine::detect");
+}
+
+std::vector<uint64_t> SpectralKurtosisEngine::tscr_totals(const uint32_t* counts, uint64_t npart)
+{
+  std::vector<uint64_t> t(nchan);
+  ctx->check(dspx_sk_tscr_totals(sk, counts, npart, t.data()), "dsp::HIP::SpectralKurtosisEngine::tscr_totals");
+  return t;
+}
+
+void SpectralKurtosisEngine::detect_tscr(const float* sums, const uint32_t* counts, uint64_t npart, uint8_t* mask,
+                                         const std::map<uint64_t, std::pair<float, float>>& limits)
+{
+  const char* where = "dsp::HIP::SpectralKurtosisEngine::detect_tscr";
+  if (!tscr_valid || limits != tscr_uploaded) {
+    std::vector<uint64_t> tot;
+    std::vector<float> lo, up;
+    for (const auto& kv : limits) {
+      if (kv.first <= setup_M) continue;
+      tot.push_back(kv.first);
+      lo.push_back(kv.second.first);
+      up.push_back(kv.second.second);
+    }
+    ctx->check(dspx_sk_set_tscr_limits(sk, uint32_t(tot.size()), tot.data(), lo.data(), up.data()), where);
+    tscr_uploaded = limits;
+    tscr_valid = true;
+  }
+  ctx->check(dspx_sk_detect_tscr(sk, sums, counts, npart, mask), where);
 }
 
 void SpectralKurtosisEngine::mask(const uint8_t* m, const TimeSeries* in, TimeSeries* out, uint64_t npart)

@@ -241,7 +241,15 @@ void SpectralKurtosis::operate()
   output->set_zeroed_data(true);  // zapped samples are zeroed (SpectralKurtosis.C:1468)
   if (!npart) return;
   engine->compute(input, sums, counts, npart);
-  engine->detect(sums, counts, npart, detectors, zapmask);
+  engine->detect(sums, counts, npart, detectors & ~unsigned(DSPX_SK_TSCR), zapmask);
+  if (detectors & DSPX_SK_TSCR) {
+    // detect_tscr tests every total with SKLimits(total) (SpectralKurtosis.C:1045)
+    for (uint64_t t : engine->tscr_totals(counts, npart))
+      if (t > M && t < 32768 && !tscr_limits.count(t))
+        throw Error(InvalidState, "dsp::SpectralKurtosis::detect_tscr", "no thresholds for the total count %llu",
+                    (unsigned long long) t);
+    engine->detect_tscr(sums, counts, npart, zapmask, tscr_limits);
+  }
   engine->mask(zapmask, input, output, npart);
 }
 

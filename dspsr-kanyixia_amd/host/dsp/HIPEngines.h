@@ -93,11 +93,17 @@ class SpectralKurtosisEngine : public SpectralKurtosis::Engine {
   void compute(const TimeSeries* input, float* sums, uint32_t* counts, uint64_t npart) override;
   void reset_mask(uint8_t* mask, uint64_t npart) override;
   void detect(float* sums, uint32_t* counts, uint64_t npart, unsigned flags, uint8_t* mask) override;
+  std::vector<uint64_t> tscr_totals(const uint32_t* counts, uint64_t npart) override;
+  void detect_tscr(const float* sums, const uint32_t* counts, uint64_t npart, uint8_t* mask,
+                   const std::map<uint64_t, std::pair<float, float>>& limits) override;
   void mask(const uint8_t* mask, const TimeSeries* input, TimeSeries* output, uint64_t npart) override;
 
  private:
   Context* ctx;
   dspx_sk* sk = nullptr;
+  std::map<uint64_t, std::pair<float, float>> tscr_uploaded;  // the table last handed to the library
+  bool tscr_valid = false;
+  unsigned setup_M = 0;
   unsigned nchan = 0;
 };
 

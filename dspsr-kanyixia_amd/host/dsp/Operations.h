@@ -14,6 +14,9 @@
 // The CPU branches of these Transformations are not restated here: on this
 // tree they are the oracle (oracle/), and the product requires an engine.
 #pragma once
+#include <map>
+#include <utility>
+
 #include "dsp/Types.h"
 
 namespace dsp {
@@ -209,6 +212,12 @@ class SpectralKurtosis {
     virtual void reset_mask(uint8_t* mask, uint64_t npart) = 0;
     //! detect_ft / detect_fscr / detect_tscr as the flags select, after reset_mask
     virtual void detect(float* sums, uint32_t* counts, uint64_t npart, unsigned flags, uint8_t* mask) = 0;
+    //! the per-channel totals that detect_tscr tests (SpectralKurtosis.C:1036-1040)
+    virtual std::vector<uint64_t> tscr_totals(const uint32_t* counts, uint64_t npart) = 0;
+    //! detect_tscr alone, OR-ed into the mask, with the thresholds of the
+    //! totals above M given by total
+    virtual void detect_tscr(const float* sums, const uint32_t* counts, uint64_t npart, uint8_t* mask,
+                             const std::map<uint64_t, std::pair<float, float>>& limits) = 0;
     virtual void mask(const uint8_t* mask, const TimeSeries* input, TimeSeries* output, uint64_t npart) = 0;
   };
 
@@ -224,6 +233,10 @@ class SpectralKurtosis {
   //! SKLimits thresholds by block count 0..M and the closed-form sigma of
   //! detect_tscr (SKLimits.C:152-160)
   void set_limits(std::vector<float> lower, std::vector<float> upper, std::vector<float> symmetric, double pcsd);
+  //! detect_tscr's thresholds (lower, upper) of SKLimits(total) for channel
+  //! totals above M (get_limits, SpectralKurtosis.C:989-1012, 1045); operate
+  //! throws InvalidState, naming the total, when one in (M, 32768) is missing
+  void set_tscr_limits(std::map<uint64_t, std::pair<float, float>> l) { tscr_limits = std::move(l); }
   const std::vector<float>& get_lower() const { return lower; }
   const std::vector<float>& get_upper() const { return upper; }
   const std::vector<float>& get_symmetric() const { return symmetric; }
@@ -245,6 +258,7 @@ class SpectralKurtosis {
   unsigned M = 128, detectors = 1;
   bool omit_outliers = false, prepared = false;
   std::vector<float> lower, upper, symmetric;
+  std::map<uint64_t, std::pair<float, float>> tscr_limits;
   double pc_std_devs = 0;
   uint64_t npart = 0, capacity = 0;
   float* sums = nullptr;

@@ -392,9 +392,33 @@ int dspx_sk_compute_raw(dspx_sk* sk, const void* d_raw, uint32_t nbit, uint32_t 
    then detect_skfb (DSPX_SK_SKFB, :1105-1222), detect_fscr (DSPX_SK_FSCR,
    :1312-1462) and detect_tscr (DSPX_SK_TSCR, :1014-1103) as flagged.  With
    omit_outliers, detect_skfb zeroes the sums and counts of the blocks it
-   zaps.  detect_tscr does not test totals between M and 32768. */
+   zaps.  detect_tscr tests every channel's total count with the thresholds
+   of SKLimits(total) (get_limits, SpectralKurtosis.C:989-1012, 1045): those
+   of dspx_sk_set_limits up to M, those of dspx_sk_set_tscr_limits above it,
+   else from 32768 on the closed form of SKLimits.C:152-160.  With
+   DSPX_SK_TSCR the call waits for the stream and fails with DSPX_E_STATE
+   (the mask then misses those channels' tests) when a total in (M, 32768)
+   has no thresholds: dspx_sk_tscr_totals names the totals to evaluate. */
 int dspx_sk_detect(dspx_sk* sk, float* d_sums, uint32_t* d_counts, uint64_t npart, uint32_t flags,
                    uint8_t* d_mask);
+/* The per-channel totals of the counts that detect_tscr will test
+   (SpectralKurtosis.C:1036-1040): h_totals[nchan], host memory.  Called
+   after a dspx_sk_detect without DSPX_SK_TSCR, it sees the counts that
+   detect_skfb left (omit_outliers zeroes those of zapped blocks).
+   Synchronous. */
+int dspx_sk_tscr_totals(dspx_sk* sk, const uint32_t* d_counts, uint64_t npart, uint64_t* h_totals);
+/* detect_tscr's thresholds of totals above M: n ascending totals with the
+   lower and upper thresholds of SKLimits(total) (Nd and ntest = npol as in
+   get_limits, SpectralKurtosis.C:989-1009).  Replaces the previous table.
+   Synchronous. */
+int dspx_sk_set_tscr_limits(dspx_sk* sk, uint32_t n, const uint64_t* totals, const float* lower,
+                            const float* upper);
+/* detect_tscr alone (SpectralKurtosis.C:1014-1103), OR-ed into the mask of a
+   preceding dspx_sk_detect: the second half of a detect whose totals were
+   read with dspx_sk_tscr_totals.  Waits for the stream; DSPX_E_STATE as
+   dspx_sk_detect. */
+int dspx_sk_detect_tscr(dspx_sk* sk, const float* d_sums, const uint32_t* d_counts, uint64_t npart,
+                        uint8_t* d_mask);
 /* Engine::mask (SpectralKurtosis.C:1518-1554): FPT out = in with block b of
    channel c zeroed in every pol where mask[b][c]; in may equal out. */
 int dspx_sk_mask(dspx_sk* sk, const uint8_t* d_mask, const float* d_in, uint64_t in_chan_stride,

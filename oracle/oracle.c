@@ -794,15 +794,19 @@ static void ox_sk_limits_large(uint64_t M, double pcsd, float* lower, float* upp
      flags & 2: detect_fscr  (SpectralKurtosis.C:1312-1462)
      flags & 4: detect_tscr  (SpectralKurtosis.C:1014-1103)
    lower/upper/symmetric: SKLimits thresholds by count (tables of M+1).
-   detect_tscr: totals above M use the closed form of SKLimits.C:152-160 with
-   pcsd (per-comparison sigma) when >= 32768; totals between M and 32768 are
-   not tested.  With omit_outliers, skfb zeroes the sums and counts of the
-   blocks it zaps (:1189-1199) and fscr skips them (:1382).  Unsigned
-   arithmetic as in the reference (Mprime - 1 wraps for Mprime = 0).
-   Returns the number of mask bytes set. */
+   detect_tscr tests every total with SKLimits(total) (get_limits, :1045):
+   totals up to M from the tables, larger ones from tscr_totals (ascending,
+   ntscr of them, thresholds tscr_lower/tscr_upper), else, from 32768 on, the
+   closed form of SKLimits.C:152-160 with pcsd (per-comparison sigma).  A
+   total in (M, 32768) without thresholds is an error: returns UINT64_MAX.
+   With omit_outliers, skfb zeroes the sums and counts of the blocks it zaps
+   (:1189-1199) and fscr skips them (:1382).  Unsigned arithmetic as in the
+   reference (Mprime - 1 wraps for Mprime = 0).  Returns the number of mask
+   bytes set. */
 uint64_t ox_sk_detect(float* sums, uint32_t* counts, uint64_t npart, unsigned nchan, unsigned npol,
                       unsigned M, unsigned Nd, int omit_outliers, unsigned flags, const float* lower,
-                      const float* upper, const float* symmetric, double pcsd, uint8_t* mask)
+                      const float* upper, const float* symmetric, double pcsd, const uint64_t* tscr_totals,
+                      const float* tscr_lower, const float* tscr_upper, unsigned ntscr, uint8_t* mask)
 {
   uint64_t ipart, i, nset = 0;
   unsigned ichan, ipol;
@@ -895,10 +899,21 @@ uint64_t ox_sk_detect(float* sums, uint32_t* counts, uint64_t npart, unsigned nc
         lo = lower[count];
         up = upper[count];
       }
-      else if (count >= 32768)
-        ox_sk_limits_large(count, pcsd, &lo, &up);
       else
-        continue;
+      {
+        unsigned k;
+        for (k = 0; k < ntscr && tscr_totals[k] != count; k++)
+          ;
+        if (k < ntscr)
+        {
+          lo = tscr_lower[k];
+          up = tscr_upper[k];
+        }
+        else if (count >= 32768)
+          ox_sk_limits_large(count, pcsd, &lo, &up);
+        else
+          return UINT64_MAX;
+      }
       M_t = (float) count;
       M_fac = (M_t * Nd + 1) / (M_t - 1);
       for (ipol = 0; ipol < npol; ipol++)

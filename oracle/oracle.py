@@ -89,7 +89,8 @@ def lib():
     L.ox_sk_detect_skfb.restype = U
     L.ox_sk_detect_skfb.argtypes = [f32p, u32p, U, u, u, u, f32p, f32p, u8p]
     L.ox_sk_detect.restype = U
-    L.ox_sk_detect.argtypes = [f32p, u32p, U, u, u, u, u, i, u, f32p, f32p, f32p, d, u8p]
+    L.ox_sk_detect.argtypes = [f32p, u32p, U, u, u, u, u, i, u, f32p, f32p, f32p, d, C.c_void_p, C.c_void_p,
+                               C.c_void_p, u, u8p]
     for name, p in (("ox_filterbank_f32", f32p), ("ox_filterbank_f64", f64p)):
         fn = getattr(L, name)
         fn.restype = U
@@ -298,17 +299,26 @@ def sk_detect_skfb(sums, counts, lower, upper, Nd=1):
 SKFB, FSCR, TSCR = 1, 2, 4
 
 
-def sk_detect(sums, counts, M, lower, upper, symmetric, pcsd=3.0, flags=SKFB, Nd=1, omit_outliers=False):
-    """SpectralKurtosis::detect with one resolution.  Returns the mask
-    [npart][nchan] and the (possibly modified, with omit_outliers) sums and
-    counts."""
+def sk_detect(sums, counts, M, lower, upper, symmetric, pcsd=3.0, flags=SKFB, Nd=1, omit_outliers=False,
+              tscr_limits=None):
+    """SpectralKurtosis::detect with one resolution.  tscr_limits: {total:
+    (lower, upper)} for the channel totals above M that detect_tscr tests.
+    Returns the mask [npart][nchan] and the (possibly modified, with
+    omit_outliers) sums and counts."""
     sums = np.array(sums, np.float32, copy=True)
     counts = np.array(counts, np.uint32, copy=True)
     npart, nchan, npol, _ = sums.shape
     mask = np.zeros(npart * nchan, np.uint8)
-    lib().ox_sk_detect(sums.reshape(-1), counts.reshape(-1), npart, nchan, npol, M, Nd, int(omit_outliers),
-                       flags, np.ascontiguousarray(lower, np.float32), np.ascontiguousarray(upper, np.float32),
-                       np.ascontiguousarray(symmetric, np.float32), pcsd, mask)
+    lim = tscr_limits or {}
+    tt = np.array(sorted(lim), np.uint64)
+    tl = np.array([lim[int(t)][0] for t in tt], np.float32)
+    tu = np.array([lim[int(t)][1] for t in tt], np.float32)
+    n = lib().ox_sk_detect(sums.reshape(-1), counts.reshape(-1), npart, nchan, npol, M, Nd, int(omit_outliers),
+                           flags, np.ascontiguousarray(lower, np.float32), np.ascontiguousarray(upper, np.float32),
+                           np.ascontiguousarray(symmetric, np.float32), pcsd, tt.ctypes.data, tl.ctypes.data,
+                           tu.ctypes.data, tt.size, mask)
+    if n == 2 ** 64 - 1:
+        raise ValueError("detect_tscr: a channel total in (M, 32768) has no thresholds")
     return mask.reshape(npart, nchan), sums, counts
 
 

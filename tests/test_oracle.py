@@ -368,9 +368,22 @@ def test_sk_detectors_find_injected_rfi(oracle):
     assert np.delete(m1, 5, axis=1).mean() < 0.02
     m4, _, _ = oracle.sk_detect(sums, counts, M, lower, upper, sym, pcsd, flags=oracle.TSCR)
     assert np.array_equal(np.flatnonzero(m4.all(axis=0)), [5]) and m4.sum() == nblk
-    # totals between M and 32768 are not tested by tscr
-    m4s, _, _ = oracle.sk_detect(sums[:100], counts[:100], M, lower, upper, sym, pcsd, flags=oracle.TSCR)
-    assert m4s.sum() == 0
+    # a total between M and 32768 is tested with SKLimits(total)
+    # (get_limits, SpectralKurtosis.C:1045): without its thresholds the
+    # detector fails; with them the tone channel alone is zapped, and with
+    # thresholds that no Gaussian block meets every channel is
+    import importlib
+    S = importlib.import_module("dspsr-kanyixia_amd.sklimits")
+    with pytest.raises(ValueError):
+        oracle.sk_detect(sums[:100], counts[:100], M, lower, upper, sym, pcsd, flags=oracle.TSCR)
+    L = S.SKLimits(100 * M, 3.0, 1, 2)
+    assert 0.9 < L.lower < 1 < L.upper < 1.1
+    m4s, _, _ = oracle.sk_detect(sums[:100], counts[:100], M, lower, upper, sym, pcsd, flags=oracle.TSCR,
+                                 tscr_limits={100 * M: (L.lower, L.upper)})
+    assert np.array_equal(np.flatnonzero(m4s.all(axis=0)), [5]) and m4s.sum() == 100
+    m4t, _, _ = oracle.sk_detect(sums[:100], counts[:100], M, lower, upper, sym, pcsd, flags=oracle.TSCR,
+                                 tscr_limits={100 * M: (0.9999, 1.0001)})
+    assert m4t.all()
     # the tone channel (kept: omit_outliers is off) pulls the channel average
     # of every block ~2 sigma low, so fscr zaps a few percent of blocks
     m2, _, _ = oracle.sk_detect(sums, counts, M, lower, upper, sym, pcsd, flags=oracle.FSCR)
