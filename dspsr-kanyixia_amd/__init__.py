@@ -160,6 +160,7 @@ _SIGS = {
     "dspx_ltf_geometry": (I32, [P, C.POINTER(U64), C.POINTER(U64)]),
     "dspx_ltf_set_fold": (I32, [P, D, C.POINTER(FoldChunk), U32]),
     "dspx_ltf_execute": (I32, [P, P, P, P]),
+    "dspx_ltf_execute_masked": (I32, [P, P, P, U32, U64, P, P]),
     "dspx_ltf_filterbank": (I32, [P, P, P]),
     "dspx_ltf_stream_reset": (I32, [P]),
     "dspx_ltf_stream_next": (I32, [P, C.POINTER(U64), C.POINTER(U64), C.POINTER(U64)]),
@@ -658,6 +659,13 @@ class LoadToFold:
 
     def execute(self, raw, prof, hits=None):
         self.ctx.check(lib().dspx_ltf_execute(self.h, ptr(raw), ptr(prof), ptr(hits)))
+
+    def execute_masked(self, raw, mask, M, prof, hits=None):
+        """The SpectralKurtosis-excised fold: mask [nblk][nchan] uint8 over
+        the output samples in blocks of M; hits [nsub][nchan][nbin] counts
+        the samples whose first detected value is non-zero."""
+        self.ctx.check(lib().dspx_ltf_execute_masked(self.h, ptr(raw), ptr(mask), M, int(mask.shape[0]), ptr(prof),
+                                                     ptr(hits)))
 
     def filterbank(self, raw, out):
         self.ctx.check(lib().dspx_ltf_filterbank(self.h, ptr(raw), ptr(out)))

@@ -250,21 +250,23 @@ struct ConvCore {
     DSPX_HIP(hipGetLastError());
   }
 
-  template <int MODE> void stage_c(uint32_t npart, const FilterbankArgs& fb, const FoldArgs& fa)
+  // ZH: the fold applies the zap mask and counts per-channel hits (fold_lane)
+  template <int MODE, bool ZH = false> void stage_c(uint32_t npart, const FilterbankArgs& fb, const FoldArgs& fa)
   {
+    static_assert(!ZH || MODE == kStageCFold, "the zap mask is a fold option");
     ConvShape s = shape(npart);
     uint32_t G = MODE == kStageCFold ? fa.G : ncb();
     uint64_t nb = uint64_t(G) * npart * nchan;
     DSPX_REQUIRE(nb < (1ull << 31), DSPX_E_UNSUPPORTED, "grid too large");
     size_t shm = kColTile * sizeof(float2);
-    if (MODE == kStageCFold) shm += size_t(fa.nslot) * fa.nval * fa.nbin * sizeof(float);
+    if (MODE == kStageCFold) shm += size_t(fa.nslot) * fa.hrow * sizeof(float);
     DSPX_REQUIRE(shm <= 160 * 1024, DSPX_E_UNSUPPORTED, "LDS request %zu too large", shm);
     const float2* twc = twiddles(ctx, N1);
     // The next tile's prefetch (64 registers) where it costs no occupancy:
     // always for the single-pass 16-point columns; at 32 and 64 points only
     // where the histogram already holds the CU to two workgroups (without
     // it those kernels run three waves per SIMD)
-    const bool pf = MODE == kStageCFold && (N1 <= 16 || (N1 <= 64 && 3 * shm > 160 * 1024));
+    const bool pf = MODE == kStageCFold && !ZH && (N1 <= 16 || (N1 <= 64 && 3 * shm > 160 * 1024));
     switch (N1) {
 #define LAUNCH_C(n, nv, p)                                                                      \
   do {                                                                                          \
@@ -272,14 +274,14 @@ struct ConvCore {
        also outside any graph capture that follows) */                                          \
     static bool lds_attr = false;                                                               \
     if (!lds_attr) {                                                                            \
-      allow_full_lds((const void*) k_stageC<n, MODE, nv, p>);                                   \
+      allow_full_lds((const void*) k_stageC<n, MODE, nv, p, ZH>);                               \
       lds_attr = true;                                                                          \
     }                                                                                           \
-    k_stageC<n, MODE, nv, p><<<unsigned(nb), kColThreads, shm, ctx->stream>>>(s, fb, fa, twc);  \
+    k_stageC<n, MODE, nv, p, ZH><<<unsigned(nb), kColThreads, shm, ctx->stream>>>(s, fb, fa, twc);  \
   } while (0)
 #define LAUNCH_C2(n, nv)                                     \
   do {                                                       \
-    if constexpr (MODE == kStageCFold && n <= 64) {          \
+    if constexpr (MODE == kStageCFold && n <= 64 && !ZH) {   \
       if (pf)                                                \
         LAUNCH_C(n, nv, true);                               \
       else                                                   \
@@ -308,15 +310,16 @@ struct ConvCore {
 
   // The one-kernel form over a block's windows (onepass.hpp); fold mode
   // needs fa's histogram geometry and key plan, filterbank mode fb.
-  template <int MODE, class LD>
+  template <int MODE, class LD, bool ZH = false>
   void run_onepass(const LD& ld, const OnePassArgs& a, const FilterbankArgs& fb, const FoldArgs& fa)
   {
+    static_assert(!ZH || MODE == kStageCFold, "the zap mask is a fold option");
     DSPX_REQUIRE(onepass, DSPX_E_INTERNAL, "not a one-pass plan");
     const uint64_t nb = uint64_t(a.nchan) * a.ngrp;
     DSPX_REQUIRE(nb >= 1 && nb < (1ull << 31), DSPX_E_UNSUPPORTED, "grid of %llu workgroups",
                  (unsigned long long) nb);
     size_t shm = onepass_lds_bytes(N);
-    if (MODE == kStageCFold) shm += size_t(fa.nslot) * fa.nval * fa.nbin * sizeof(float);
+    if (MODE == kStageCFold) shm += size_t(fa.nslot) * fa.hrow * sizeof(float);
     DSPX_REQUIRE(shm <= 160 * 1024, DSPX_E_UNSUPPORTED, "one-pass LDS request %zu too large", shm);
     ChirpArgs c = chirp();
     const float2* tw = twiddles(ctx, int(N));
@@ -324,10 +327,10 @@ struct ConvCore {
   do {                                                                                                          \
     static bool lds_attr = false;                                                                               \
     if (!lds_attr) {                                                                                            \
-      allow_full_lds((const void*) k_onepass<n, MODE, nv, LD>);                                                 \
+      allow_full_lds((const void*) k_onepass<n, MODE, nv, LD, ZH>);                                             \
       lds_attr = true;                                                                                          \
     }                                                                                                           \
-    k_onepass<n, MODE, nv, LD><<<unsigned(nb), OnePass<n>::NT, shm, ctx->stream>>>(ld, a, c, fb, fa, tw);       \
+    k_onepass<n, MODE, nv, LD, ZH><<<unsigned(nb), OnePass<n>::NT, shm, ctx->stream>>>(ld, a, c, fb, fa, tw);   \
   } while (0)
 #define CASE_OP(n)                                     \
   case n:                                              \
@@ -492,9 +495,13 @@ __global__ void k_fold_reduce_g(const float* __restrict__ partial, uint32_t G, u
   out[i] = acc;
 }
 
+//   (ZH: the rows also hold nbin hit counts after the nvb profile values;
+//   each unit's count, an integer below 2^24 and so exact in a float, is
+//   added as an integer into hits[isub][chan][bin])
+template <bool ZH>
 __global__ void k_fold_reduce_p(const float* __restrict__ part, const uint32_t* __restrict__ meta, uint32_t meta_sf,
                                 uint32_t meta_off, uint32_t meta_parts, uint32_t npart, uint32_t nslot, uint32_t hl,
-                                uint32_t nchan, float* __restrict__ prof)
+                                uint32_t nvb, uint32_t nchan, float* __restrict__ prof, uint32_t* __restrict__ hits)
 {
   uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
   uint32_t chan = blockIdx.y;
@@ -503,13 +510,39 @@ __global__ void k_fold_reduce_p(const float* __restrict__ part, const uint32_t* 
   const uint32_t* sub_first = meta + meta_sf;
   const uint32_t k0 = meta[meta_off + isub], k1 = meta[meta_off + isub + 1];
   if (k0 == k1) return;
+  if (ZH && e >= nvb) {
+    if (!hits) return;
+    uint32_t acc = 0;
+    for (uint32_t k = k0; k < k1; k++) {
+      const uint32_t p = meta[meta_parts + k];
+      acc += uint32_t(part[((uint64_t(chan) * npart + p) * nslot + (isub - sub_first[p])) * hl + e]);
+    }
+    const uint32_t nbin = hl - nvb;
+    hits[(uint64_t(isub) * nchan + chan) * nbin + (e - nvb)] += acc;
+    return;
+  }
   float acc = 0.f;
 #pragma unroll 8
   for (uint32_t k = k0; k < k1; k++) {
     const uint32_t p = meta[meta_parts + k];
     acc += part[((uint64_t(chan) * npart + p) * nslot + (isub - sub_first[p])) * hl + e];
   }
-  prof[(uint64_t(isub) * nchan + chan) * hl + e] += acc;
+  prof[(uint64_t(isub) * nchan + chan) * nvb + e] += acc;
+}
+
+// The SpectralKurtosis mask on an FPT complex series of nout samples per
+// row (the plain form's convolution output): sample idat of channel c is
+// zeroed where mask[idat / M][c] is set or idat >= nblk*M.
+__global__ void k_zap_fpt(float* __restrict__ x, uint64_t chan_stride, uint64_t pol_stride, uint32_t npol,
+                          uint64_t nout, ZapArgs z)
+{
+  const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= nout) return;
+  const uint32_t chan = blockIdx.y;
+  const uint64_t k = i / z.M;
+  if (k < z.nblk && !z.mask[k * z.nchan + chan]) return;
+  for (uint32_t pol = 0; pol < npol; pol++)
+    *reinterpret_cast<float2*>(x + chan * chan_stride + pol * pol_stride + 2 * i) = make_float2(0.f, 0.f);
 }
 
 // The fold plan in the order stage C meets it, evaluated from the plan's
@@ -802,6 +835,11 @@ struct dspx_ltf {
   // (the parts whose slots hold sub-integration isub are parts[off[isub] ..
   // off[isub+1]), in increasing order)
   uint32_t G = 1, nslot = 1;
+  // the same for a fold with the zap mask and per-channel hits (a longer
+  // histogram row); zh_ok: it fits in LDS and every unit's hit counts stay
+  // below 2^24
+  uint32_t G_zh = 1;
+  bool zh_ok = false;
   // one-pass form (N <= 2^13): windows per workgroup and workgroups per
   // channel of the current block; the reduction's sub_first[] per workgroup
   bool onepass = false;
@@ -1008,9 +1046,11 @@ void fused_fold_setup(dspx_ltf* L)
   }
   L->nslot = nslot;
   const size_t hist_bytes = size_t(nslot) * L->nval * L->d.nbin * sizeof(float);
+  const size_t hist_bytes_zh = size_t(nslot) * (L->nval + 1) * L->d.nbin * sizeof(float);
   const size_t tile_bytes = L->onepass ? onepass_lds_bytes(L->d.nfft) : kColTile * sizeof(float2);
   DSPX_REQUIRE(tile_bytes + hist_bytes <= 160 * 1024, DSPX_E_UNSUPPORTED,
                "fold histogram of %zu bytes does not fit in LDS", hist_bytes);
+  L->zh_ok = tile_bytes + hist_bytes_zh <= 160 * 1024 && uint64_t(per_unit) * L->step < (1u << 24);
   const uint32_t nsub = L->fold.nsub;
   // units whose slots hold isub: sfu[u] <= isub < sfu[u] + nslot
   std::vector<uint32_t> off(nsub + 1, 0), units;
@@ -1035,8 +1075,8 @@ void fused_fold_setup(dspx_ltf* L)
 
   const uint64_t units_all = uint64_t(L->d.nchan) * nunit;
   if (L->onepass) {
-    L->G = 1;
-    L->sc().partial.reserve(units_all * hist_bytes);
+    L->G = L->G_zh = 1;
+    L->sc().partial.reserve(units_all * (L->zh_ok ? hist_bytes_zh : hist_bytes));
   } else {
     // Column groups per (channel, part) in stage C.  Every stage C workgroup
     // zeroes, writes and has reduced one histogram: with large histograms the
@@ -1046,13 +1086,17 @@ void fused_fold_setup(dspx_ltf* L)
     // tiles over 118 groups would leave most groups one tile and a few two,
     // each group with its own histogram).
     const uint32_t ncb = L->core.ncb();
-    const uint64_t per_cu = std::max<uint64_t>(1, (160 * 1024) / (kColTile * sizeof(float2) + hist_bytes));
-    const uint64_t slots = uint64_t(std::max(ctx->ncu, 1)) * per_cu;
-    const uint64_t target = hist_bytes > 16384 ? slots : 4 * slots;
-    const uint64_t g0 = std::min<uint64_t>(ncb, std::max<uint64_t>(1, (target + units_all - 1) / units_all));
-    const uint64_t tiles_per_group = (ncb + g0 - 1) / g0;
-    L->G = uint32_t((ncb + tiles_per_group - 1) / tiles_per_group);
-    L->sc().partial.reserve(units_all * L->G * hist_bytes);
+    auto groups = [&](size_t hb) {
+      const uint64_t per_cu = std::max<uint64_t>(1, (160 * 1024) / (kColTile * sizeof(float2) + hb));
+      const uint64_t slots = uint64_t(std::max(ctx->ncu, 1)) * per_cu;
+      const uint64_t target = hb > 16384 ? slots : 4 * slots;
+      const uint64_t g0 = std::min<uint64_t>(ncb, std::max<uint64_t>(1, (target + units_all - 1) / units_all));
+      const uint64_t tiles_per_group = (ncb + g0 - 1) / g0;
+      return uint32_t((ncb + tiles_per_group - 1) / tiles_per_group);
+    };
+    L->G = groups(hist_bytes);
+    L->G_zh = groups(hist_bytes_zh);
+    L->sc().partial.reserve(units_all * std::max(L->G * hist_bytes, L->zh_ok ? L->G_zh * hist_bytes_zh : 0));
   }
 
   // key plan and hits
@@ -1078,7 +1122,8 @@ void fused_fold_setup(dspx_ltf* L)
   DSPX_HIP(hipGetLastError());
 }
 
-FoldArgs fold_args(dspx_ltf* L)
+// zap: the SpectralKurtosis mask of a fold with per-channel hits, or null
+FoldArgs fold_args(dspx_ltf* L, const ZapArgs* zap = nullptr)
 {
   FoldArgs fa;
   fa.keys = L->keys.as<uint32_t>();
@@ -1087,36 +1132,47 @@ FoldArgs fold_args(dspx_ltf* L)
   fa.nval = L->nval;
   fa.ndim = L->ndim_out;
   fa.nslot = L->nslot;
-  fa.G = L->G;
+  fa.G = zap ? L->G_zh : L->G;
   fa.state = L->d.state;
   fa.npol_out = L->npol_out;
   for (uint32_t i = 0; i < 4; i++) fa.voff[i] = (i / L->ndim_out) * L->d.nbin * L->ndim_out + (i % L->ndim_out);
+  fa.hrow = (L->nval + (zap ? 1 : 0)) * L->d.nbin;
+  fa.zap = zap ? *zap : ZapArgs{};
   return fa;
 }
 
 // The profile reduction over the units' partial histograms (parts, or
-// one-pass workgroups) and the plan's hits.
-void fold_finish(dspx_ltf* L, uint32_t np, float* d_prof, uint32_t* d_hits, int& ev)
+// one-pass workgroups) and the hits: the plan's, or with zh the per-channel
+// counts the fold made.
+void fold_finish(dspx_ltf* L, uint32_t np, float* d_prof, uint32_t* d_hits, int& ev, bool zh = false)
 {
   dspx_ctx* ctx = L->ctx;
   const auto& d = L->d;
   const uint32_t nunit = L->onepass ? L->ngrp : np;
-  uint32_t hl = L->nval * d.nbin;
+  const uint32_t nvb = L->nval * d.nbin;
+  const uint32_t hl = zh ? nvb + d.nbin : nvb;
+  const uint32_t G = zh ? L->G_zh : L->G;
   uint64_t n2 = uint64_t(d.nchan) * nunit * L->nslot * hl;
   // with one group the partials already are [chan][unit][slot][e]
   const float* per_unit = L->sc().partial.as<float>();
-  if (L->G > 1) {
+  if (G > 1) {
     L->sc().partial2.reserve(n2 * sizeof(float));
     k_fold_reduce_g<<<unsigned((n2 + 255) / 256), 256, 0, ctx->stream>>>(
-        L->sc().partial.as<float>(), L->G, L->nslot * hl, n2, L->sc().partial2.as<float>());
+        L->sc().partial.as<float>(), G, L->nslot * hl, n2, L->sc().partial2.as<float>());
     DSPX_HIP(hipGetLastError());
     per_unit = L->sc().partial2.as<float>();
   }
   dim3 grid((hl + 255) / 256, d.nchan, L->fold.nsub);
-  k_fold_reduce_p<<<grid, 256, 0, ctx->stream>>>(per_unit, L->foldmeta.as<uint32_t>(), L->meta_sf, L->meta_off,
-                                                 L->meta_parts, nunit, L->nslot, hl, d.nchan, d_prof);
+  if (zh)
+    k_fold_reduce_p<true><<<grid, 256, 0, ctx->stream>>>(per_unit, L->foldmeta.as<uint32_t>(), L->meta_sf,
+                                                         L->meta_off, L->meta_parts, nunit, L->nslot, hl, nvb,
+                                                         d.nchan, d_prof, d_hits);
+  else
+    k_fold_reduce_p<false><<<grid, 256, 0, ctx->stream>>>(per_unit, L->foldmeta.as<uint32_t>(), L->meta_sf,
+                                                          L->meta_off, L->meta_parts, nunit, L->nslot, hl, nvb,
+                                                          d.nchan, d_prof, nullptr);
   DSPX_HIP(hipGetLastError());
-  if (d_hits) {
+  if (d_hits && !zh) {
     const uint64_t nh = uint64_t(L->fold.nsub) * d.nbin;
     k_add_u32<<<unsigned((nh + 255) / 256), 256, 0, ctx->stream>>>(L->plan_hits.as<uint32_t>(), nh, d_hits);
     DSPX_HIP(hipGetLastError());
@@ -1126,17 +1182,21 @@ void fold_finish(dspx_ltf* L, uint32_t np, float* d_prof, uint32_t* d_hits, int&
 }
 
 // Stages B and C, the profile reduction and the hits of a fold execute
-// (stage A has run on np parts).
-void fused_fold_after_a(dspx_ltf* L, uint32_t np, float* d_prof, uint32_t* d_hits, int& ev)
+// (stage A has run on np parts); zap: the SpectralKurtosis mask, or null.
+void fused_fold_after_a(dspx_ltf* L, uint32_t np, float* d_prof, uint32_t* d_hits, int& ev,
+                        const ZapArgs* zap = nullptr)
 {
   ConvCore& k = L->core;
   k.stage_b(np);
   ltf_record(L, ev++);
   L->stage_name[L->nstage++] = "stageB_rowfft_chirp";
-  k.stage_c<kStageCFold>(np, FilterbankArgs{}, fold_args(L));
+  if (zap)
+    k.stage_c<kStageCFold, true>(np, FilterbankArgs{}, fold_args(L, zap));
+  else
+    k.stage_c<kStageCFold>(np, FilterbankArgs{}, fold_args(L));
   ltf_record(L, ev++);
   L->stage_name[L->nstage++] = "stageC_colifft_detect_fold";
-  fold_finish(L, np, d_prof, d_hits, ev);
+  fold_finish(L, np, d_prof, d_hits, ev, zap != nullptr);
 }
 
 OnePassArgs onepass_args(const dspx_ltf* L, uint32_t np, uint32_t ppg, uint32_t ngrp)
@@ -1147,18 +1207,24 @@ OnePassArgs onepass_args(const dspx_ltf* L, uint32_t np, uint32_t ppg, uint32_t 
 // The whole fold execute of np windows over the loader ld: the three stages
 // or the one-pass kernel, then the reduction (stage timing events as the
 // stages complete).
-template <class LD> void fold_exec(dspx_ltf* L, const LD& ld, uint32_t np, float* d_prof, uint32_t* d_hits, int& ev)
+template <class LD>
+void fold_exec(dspx_ltf* L, const LD& ld, uint32_t np, float* d_prof, uint32_t* d_hits, int& ev,
+               const ZapArgs* zap = nullptr)
 {
   if (L->onepass) {
-    L->core.run_onepass<kStageCFold>(ld, onepass_args(L, np, L->ppg, L->ngrp), FilterbankArgs{}, fold_args(L));
+    const OnePassArgs oa = onepass_args(L, np, L->ppg, L->ngrp);
+    if (zap)
+      L->core.run_onepass<kStageCFold, LD, true>(ld, oa, FilterbankArgs{}, fold_args(L, zap));
+    else
+      L->core.run_onepass<kStageCFold>(ld, oa, FilterbankArgs{}, fold_args(L));
     ltf_record(L, ev++);
     L->stage_name[L->nstage++] = "onepass_unpack_fft_chirp_ifft_detect_fold";
-    fold_finish(L, np, d_prof, d_hits, ev);
+    fold_finish(L, np, d_prof, d_hits, ev, zap != nullptr);
   } else {
     L->core.stage_a(ld, np);
     ltf_record(L, ev++);
     L->stage_name[L->nstage++] = "stageA_unpack_colfft";
-    fused_fold_after_a(L, np, d_prof, d_hits, ev);
+    fused_fold_after_a(L, np, d_prof, d_hits, ev, zap);
   }
 }
 
@@ -1218,46 +1284,93 @@ int dspx_ltf_set_fold(dspx_ltf* L, double pps, const dspx_fold_chunk* chunks, ui
   });
 }
 
+}  // extern "C"
+
+namespace {
+
+// One block through the fold plan; zap: the SpectralKurtosis mask with
+// per-channel hits of zeroed data, or null.
+void ltf_execute(dspx_ltf* L, const void* d_raw, float* d_prof, uint32_t* d_hits, const ZapArgs* zap)
+{
+  dspx_ctx* ctx = L->ctx;
+  DSPX_REQUIRE(L->have_fold, DSPX_E_STATE, "no fold plan: call dspx_ltf_set_fold");
+  const auto& d = L->d;
+  if (!L->npart) return;
+  int ev = 0;
+  L->nstage = 0;
+  L->nstage_open = false;
+  ltf_record(L, ev++);
+  if (d.form != DSPX_FORM_PLAIN) {
+    uint32_t np = uint32_t(L->npart);
+    with_loader(L, d_raw, [&](auto& ld) { fold_exec(L, ld, np, d_prof, d_hits, ev, zap); });
+  } else {
+    // plain: unpack -> hipFFT convolution -> (mask) -> detect -> fold, each a pass
+    uint64_t nin = d.ndat, nout = L->ndat_out;
+    L->unpacked.reserve(size_t(d.nchan) * d.npol * nin * sizeof(float2));
+    L->filtered.reserve(size_t(d.nchan) * d.npol * nout * sizeof(float2));
+    L->detected.reserve(size_t(d.nchan) * L->nval * nout * sizeof(float));
+    unpack(ctx, d_raw, d.nbit, d.offset_binary, d.nchan, d.npol, nin, d.scale, L->unpacked.as<float>(),
+           uint64_t(d.npol) * nin * 2, nin * 2);
+    ltf_record(L, ev++);
+    L->stage_name[L->nstage++] = "unpack";
+    L->plain.perform(L->unpacked.as<float>(), uint64_t(d.npol) * nin * 2, nin * 2, L->filtered.as<float>(),
+                     uint64_t(d.npol) * nout * 2, nout * 2, uint32_t(L->npart));
+    ltf_record(L, ev++);
+    L->stage_name[L->nstage++] = "convolution_hipfft";
+    if (zap) {
+      dim3 grid(unsigned((nout + 255) / 256), d.nchan);
+      k_zap_fpt<<<grid, 256, 0, ctx->stream>>>(L->filtered.as<float>(), uint64_t(d.npol) * nout * 2, nout * 2,
+                                               d.npol, nout, *zap);
+      DSPX_HIP(hipGetLastError());
+    }
+    uint64_t ops = nout * L->ndim_out;  // floats per output pol
+    uint64_t ocs = uint64_t(L->npol_out) * ops;
+    detect(ctx, d.state, L->ndim_out, L->filtered.as<float>(), uint64_t(d.npol) * nout * 2, nout * 2, d.nchan,
+           d.npol, nout, L->detected.as<float>(), ocs, ops);
+    ltf_record(L, ev++);
+    L->stage_name[L->nstage++] = "detect";
+    if (zap) {
+      L->fold.fold_fpt(L->detected.as<float>(), ocs, ops, d.nchan, L->npol_out, L->ndim_out, d_prof, d_hits,
+                       d.nchan);
+    } else {
+      L->fold.fold_fpt(L->detected.as<float>(), ocs, ops, d.nchan, L->npol_out, L->ndim_out, d_prof);
+      if (d_hits) L->fold.hits(d_hits);
+    }
+    ltf_record(L, ev++);
+    L->stage_name[L->nstage++] = "fold_hits";
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
 int dspx_ltf_execute(dspx_ltf* L, const void* d_raw, float* d_prof, uint32_t* d_hits)
 {
   dspx_ctx* ctx = L ? L->ctx : nullptr;
   return guard(ctx, [&] {
     DSPX_REQUIRE(L && d_raw && d_prof, DSPX_E_PARAM, "null argument");
-    DSPX_REQUIRE(L->have_fold, DSPX_E_STATE, "no fold plan: call dspx_ltf_set_fold");
-    const auto& d = L->d;
-    if (!L->npart) return;
-    int ev = 0;
-    L->nstage = 0;
-    L->nstage_open = false;
-    ltf_record(L, ev++);
-    if (d.form != DSPX_FORM_PLAIN) {
-      uint32_t np = uint32_t(L->npart);
-      with_loader(L, d_raw, [&](auto& ld) { fold_exec(L, ld, np, d_prof, d_hits, ev); });
-    } else {
-      // plain: unpack -> hipFFT convolution -> detect -> fold, each a pass
-      uint64_t nin = d.ndat, nout = L->ndat_out;
-      L->unpacked.reserve(size_t(d.nchan) * d.npol * nin * sizeof(float2));
-      L->filtered.reserve(size_t(d.nchan) * d.npol * nout * sizeof(float2));
-      L->detected.reserve(size_t(d.nchan) * L->nval * nout * sizeof(float));
-      unpack(ctx, d_raw, d.nbit, d.offset_binary, d.nchan, d.npol, nin, d.scale, L->unpacked.as<float>(),
-             uint64_t(d.npol) * nin * 2, nin * 2);
-      ltf_record(L, ev++);
-      L->stage_name[L->nstage++] = "unpack";
-      L->plain.perform(L->unpacked.as<float>(), uint64_t(d.npol) * nin * 2, nin * 2, L->filtered.as<float>(),
-                       uint64_t(d.npol) * nout * 2, nout * 2, uint32_t(L->npart));
-      ltf_record(L, ev++);
-      L->stage_name[L->nstage++] = "convolution_hipfft";
-      uint64_t ops = nout * L->ndim_out;  // floats per output pol
-      uint64_t ocs = uint64_t(L->npol_out) * ops;
-      detect(ctx, d.state, L->ndim_out, L->filtered.as<float>(), uint64_t(d.npol) * nout * 2, nout * 2, d.nchan,
-             d.npol, nout, L->detected.as<float>(), ocs, ops);
-      ltf_record(L, ev++);
-      L->stage_name[L->nstage++] = "detect";
-      L->fold.fold_fpt(L->detected.as<float>(), ocs, ops, d.nchan, L->npol_out, L->ndim_out, d_prof);
-      if (d_hits) L->fold.hits(d_hits);
-      ltf_record(L, ev++);
-      L->stage_name[L->nstage++] = "fold_hits";
-    }
+    ltf_execute(L, d_raw, d_prof, d_hits, nullptr);
+  });
+}
+
+int dspx_ltf_execute_masked(dspx_ltf* L, const void* d_raw, const uint8_t* d_mask, uint32_t M, uint64_t nblk,
+                            float* d_prof, uint32_t* d_hits)
+{
+  dspx_ctx* ctx = L ? L->ctx : nullptr;
+  return guard(ctx, [&] {
+    DSPX_REQUIRE(L && d_raw && d_mask && d_prof, DSPX_E_PARAM, "null argument");
+    DSPX_REQUIRE(M >= 2, DSPX_E_PARAM, "M=%u < 2", M);
+    DSPX_REQUIRE(nblk * M <= L->ndat_out, DSPX_E_PARAM, "%llu blocks of %u exceed the %llu output samples",
+                 (unsigned long long) nblk, M, (unsigned long long) L->ndat_out);
+    DSPX_REQUIRE(L->ndat_out + L->d.nfft < (1ull << 31), DSPX_E_UNSUPPORTED,
+                 "block of %llu output samples too long for the zap mask", (unsigned long long) L->ndat_out);
+    if (L->d.form != DSPX_FORM_PLAIN && L->have_fold && L->npart)
+      DSPX_REQUIRE(L->zh_ok, DSPX_E_UNSUPPORTED,
+                   "fold histogram with per-channel hit counts does not fit in LDS (nbin=%u, %u slots)", L->d.nbin,
+                   L->nslot);
+    ZapArgs z{d_mask, M, uint32_t(nblk), L->d.nchan, ~0ull / M + 1};
+    ltf_execute(L, d_raw, d_prof, d_hits, &z);
   });
 }
 

@@ -62,10 +62,10 @@ struct OnePassArgs {
 
 // grid.x = nchan * ngrp (group fastest), block = OnePass<N>::NT, dynamic LDS
 // = lds_bytes (+ the fold histogram).  Fold partials go to
-// fa.partial[chan][grp][nslot][nval*nbin]; fa.keys is the key plan
-// [npart][N] (slots relative to the first sub-integration of the window's
-// workgroup).
-template <int N, int MODE, int NV, class LD>
+// fa.partial[chan][grp][nslot][hrow]; fa.keys is the key plan [npart][N]
+// (slots relative to the first sub-integration of the window's workgroup).
+// ZH: the zap mask and per-channel hits of zeroed data (fold_lane).
+template <int N, int MODE, int NV, class LD, bool ZH = false>
 __global__ __launch_bounds__(OnePass<N>::NT) void k_onepass(LD ld, OnePassArgs a, ChirpArgs c, FilterbankArgs fb,
                                                             FoldArgs fa, const float2* __restrict__ tw)
 {
@@ -84,7 +84,7 @@ __global__ __launch_bounds__(OnePass<N>::NT) void k_onepass(LD ld, OnePassArgs a
   const uint32_t w_begin = grp * a.ppg;
   const uint32_t w_end = min(a.npart, w_begin + a.ppg);
 
-  const uint32_t hist_len = MODE == kStageCFold ? fa.nslot * fa.nval * fa.nbin : 0;
+  const uint32_t hist_len = MODE == kStageCFold ? fa.nslot * fa.hrow : 0;
   if (MODE == kStageCFold)
     for (uint32_t i = threadIdx.x; i < hist_len; i += NT) hist[i] = 0.f;
   constexpr int QG = E / 4;
@@ -177,64 +177,18 @@ __global__ __launch_bounds__(OnePass<N>::NT) void k_onepass(LD ld, OnePassArgs a
       }
     }
     if constexpr (MODE == kStageCFold) {
-      const int lane = int(threadIdx.x & 63u);
       const uint32_t* kw = fa.keys + uint64_t(valid ? w : w_begin) * N;
 #pragma unroll
       for (int g = 0; g < QG; g++) {
         uint32_t kq[4];
         load_keys<4>(kw + (g * NTW + tid) * 4, kq);
         if (!valid) kq[0] = kq[1] = kq[2] = kq[3] = 0xffffffffu;
-        float cacc[NV], racc[NV];
+        float2 b[4];
 #pragma unroll
-        for (int i = 0; i < NV; i++) cacc[i] = racc[i] = 0.f;
-        const uint32_t lane_key = kq[0];
-        uint32_t run_key = 0xffffffffu;
-        auto flush_run = [&]() {
-          if (run_key != 0xffffffffu) {
-            float d[NV];
-            detect_finish<NV>(fa.state == DSPX_STOKES, racc, d);
-            float* hs = hist + (run_key >> 16) * (NV * fa.nbin) + (run_key & 0xffffu) * fa.ndim;
-#pragma unroll
-            for (int i = 0; i < NV; i++) atomicAdd(hs + fa.voff[i], d[i]);
-          }
-        };
-#pragma unroll
-        for (int k = 0; k < 4; k++) {
-          const float2 b = a.npol == 2 ? v[g * 4 + k] : make_float2(0.f, 0.f);
-          const uint32_t key = kq[k];
-          if (key == lane_key) {
-            detect_acc<NV>(a.npol, y0[g * 4 + k], b, cacc);
-          } else if (key != 0xffffffffu) {
-            if (key != run_key) {
-              flush_run();
-              run_key = key;
-#pragma unroll
-              for (int i = 0; i < NV; i++) racc[i] = 0.f;
-            }
-            detect_acc<NV>(a.npol, y0[g * 4 + k], b, racc);
-          }
-        }
-        flush_run();
-        float acc[NV];
-        detect_finish<NV>(fa.state == DSPX_STOKES, cacc, acc);
-        // one add per 16-lane row when the row's first keys agree
-        const uint32_t k15 = __shfl(lane_key, lane | 15);
-        const uint64_t same = __ballot(lane_key == k15);
-        const bool row_uniform = ((same >> (lane & 48)) & 0xffffull) == 0xffffull;
-        float rs[NV];
-#pragma unroll
-        for (int i = 0; i < NV; i++) rs[i] = row_sum_lane15(acc[i]);
-        if (row_uniform) {
-          if ((lane & 15) == 15 && k15 != 0xffffffffu) {
-            float* hs = hist + (k15 >> 16) * (NV * fa.nbin) + (k15 & 0xffffu) * fa.ndim;
-#pragma unroll
-            for (int i = 0; i < NV; i++) atomicAdd(hs + fa.voff[i], rs[i]);
-          }
-        } else if (lane_key != 0xffffffffu) {
-          float* hs = hist + (lane_key >> 16) * (NV * fa.nbin) + (lane_key & 0xffffu) * fa.ndim;
-#pragma unroll
-          for (int i = 0; i < NV; i++) atomicAdd(hs + fa.voff[i], acc[i]);
-        }
+        for (int k = 0; k < 4; k++) b[k] = a.npol == 2 ? v[g * 4 + k] : make_float2(0.f, 0.f);
+        // output index of the group's first sample
+        const int32_t i0 = int32_t(w * a.step + uint32_t(g * NTW + tid) * 4) - int32_t(a.nfilt_pos);
+        fold_lane<NV, 4, ZH>(hist, fa, a.npol, kq, y0 + g * 4, b, chan, i0);
       }
     }
   }

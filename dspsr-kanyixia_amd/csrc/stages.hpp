@@ -132,13 +132,28 @@ struct ChirpArgs {
   const float2* table; // DSPX_CHIRP_TABLE: [nchan][N1][N2] (transposed bins)
 };
 
+// The SpectralKurtosis zap mask a fused fold applies (LoadToFold1.C:
+// 265-377): output sample idat of channel chan is zeroed, so neither folded
+// nor counted, where its block of M samples is flagged, mask[idat / M][chan],
+// or lies past the nblk blocks the estimator formed (the cleaned series of
+// SpectralKurtosis::mask is nblk*M samples, SpectralKurtosis.C:1465-1566).
+struct ZapArgs {
+  const uint8_t* mask;  // [nblk][nchan]
+  uint32_t M, nblk, nchan;
+  uint64_t magic;       // ceil(2^64 / M): idat / M = umul64hi(idat, magic) for idat < 2^32
+};
+
 struct FoldArgs {
   const uint32_t* keys;      // [npart][nfft]: (slot << 16) | bin of each transform output, ~0 skip
-  float* partial;            // [nchan][npart][G][nslot][nval*nbin]
+  float* partial;            // [nchan][npart][G][nslot][hrow]
   uint32_t nbin, nval, ndim, nslot, G;
   uint32_t state, npol_out;
   // profile offset of detected value i of bin b: voff[i] + b * ndim
   uint32_t voff[4];
+  // histogram floats per slot: nval*nbin, then with per-channel hits (zeroed
+  // data) nbin hit counts
+  uint32_t hrow;
+  ZapArgs zap;
 };
 
 struct FilterbankArgs {
@@ -513,6 +528,111 @@ __device__ __forceinline__ void load_keys(const uint32_t* __restrict__ k, uint32
   }
 }
 
+// One lane's QN consecutive output samples of a fold row (a[q], b[q]: the
+// two pols; kq[q]: their histogram keys, ~0 where not folded) added into the
+// LDS histogram: samples of the lane's first key accumulated in registers
+// and added once per 16-lane row when the row's first keys agree, else once
+// per lane; samples after a bin edge accumulated per run of equal keys and
+// added once per run (bins a few samples wide, short periods with many
+// bins, put an edge in most lanes).
+// ZH, zeroed data with per-channel hits (Fold.C:911-930): samples the zap
+// mask flags are dropped, and each sample whose first detected value is
+// non-zero counts one hit, accumulated as one more value and added at
+// hist[slot][nval*nbin + bin].  i0: output sample index of q = 0 (negative
+// within the discarded overlap, where every key is ~0).
+template <int NV, int QN, bool ZH>
+__device__ __forceinline__ void fold_lane(float* hist, const FoldArgs& fa, uint32_t npol, uint32_t* kq,
+                                          const float2* a, const float2* b, uint32_t chan, int32_t i0)
+{
+  constexpr int NA = ZH ? NV + 1 : NV;
+  constexpr uint32_t NONE = 0xffffffffu;
+  const bool stokes = fa.state == DSPX_STOKES;
+  if constexpr (ZH) {
+    // block k and offset r of the first sample at or past output 0, then
+    // advanced sample by sample (M >= 1: at most one block edge per step)
+    const ZapArgs& z = fa.zap;
+    const uint32_t base = i0 < 0 ? 0u : uint32_t(i0);
+    uint32_t k = uint32_t(__umul64hi(uint64_t(base), z.magic));
+    uint32_t r = base - k * z.M;
+    bool zap = k >= z.nblk || z.mask[uint64_t(k) * z.nchan + chan] != 0;
+#pragma unroll
+    for (int q = 0; q < QN; q++) {
+      if (q > 0 && i0 + q > int32_t(base)) {
+        if (++r == z.M) {
+          r = 0;
+          k++;
+          zap = k >= z.nblk || z.mask[uint64_t(k) * z.nchan + chan] != 0;
+        }
+      }
+      if (zap) kq[q] = NONE;
+    }
+  }
+  auto acc_sample = [&](int q, float* c) {
+    detect_acc<NV>(npol, a[q], b[q], c);
+    if constexpr (ZH) {
+      // the first detected value (PP, PP + QQ or Stokes I) is a sum of
+      // squares: zero exactly when each square is
+      float x0 = a[q].x * a[q].x + a[q].y * a[q].y;
+      if ((NV == 1 && npol == 2) || (NV == 4 && stokes)) x0 += b[q].x * b[q].x + b[q].y * b[q].y;
+      c[NV] += x0 != 0.f ? 1.f : 0.f;
+    }
+  };
+  auto finish = [&](const float* c, float* d) {
+    detect_finish<NV>(stokes, c, d);
+    if constexpr (ZH) d[NV] = c[NV];
+  };
+  auto add = [&](uint32_t key, const float* d) {
+    float* hs = hist + (key >> 16) * fa.hrow;
+    const uint32_t bin = key & 0xffffu;
+#pragma unroll
+    for (int i = 0; i < NV; i++) atomicAdd(hs + bin * fa.ndim + fa.voff[i], d[i]);
+    if constexpr (ZH) atomicAdd(hs + NV * fa.nbin + bin, d[NV]);
+  };
+  float cacc[NA], racc[NA];
+#pragma unroll
+  for (int i = 0; i < NA; i++) cacc[i] = racc[i] = 0.f;
+  const uint32_t lane_key = kq[0];
+  uint32_t run_key = NONE;
+  auto flush_run = [&]() {
+    if (run_key != NONE) {
+      float d[NA];
+      finish(racc, d);
+      add(run_key, d);
+    }
+  };
+#pragma unroll
+  for (int q = 0; q < QN; q++) {
+    const uint32_t key = kq[q];
+    if (key == lane_key) {
+      acc_sample(q, cacc);
+    } else if (key != NONE) {
+      if (key != run_key) {
+        flush_run();
+        run_key = key;
+#pragma unroll
+        for (int i = 0; i < NA; i++) racc[i] = 0.f;
+      }
+      acc_sample(q, racc);
+    }
+  }
+  flush_run();
+  float acc[NA];
+  finish(cacc, acc);
+  // is the whole 16-lane row in one bin?
+  const int lane = int(threadIdx.x & 63u);
+  const uint32_t k15 = __shfl(lane_key, lane | 15);
+  const uint64_t same = __ballot(lane_key == k15);
+  const bool row_uniform = ((same >> (lane & 48)) & 0xffffull) == 0xffffull;
+  float rs[NA];
+#pragma unroll
+  for (int i = 0; i < NA; i++) rs[i] = row_sum_lane15(acc[i]);
+  if (row_uniform) {
+    if ((lane & 15) == 15 && k15 != NONE) add(k15, rs);
+  } else if (lane_key != NONE) {
+    add(lane_key, acc);
+  }
+}
+
 // The fold's ownership of an L-point inverse column transform's outputs:
 // each thread holds QN consecutive columns of R rows.  A multi-pass
 // transform's last pass delivers it directly (QN = E / last radix); a
@@ -575,7 +695,7 @@ __device__ __forceinline__ void last_pass_cols(float2* lds, float2* v, const flo
 // NV: detected values per sample in fold mode (1, 2 or 4).  PFT: the fold
 // loads each tile's inputs while the previous tile is transformed (64 more
 // registers; the launch chooses it where occupancy is set by the LDS).
-template <int N1, int MODE, int NV = 1, bool PFT = false>
+template <int N1, int MODE, int NV = 1, bool PFT = false, bool ZH = false>
 __global__ __launch_bounds__(kColThreads) void k_stageC(ConvShape s, FilterbankArgs fb, FoldArgs fa,
                                                        const float2* __restrict__ twc)
 {
@@ -606,7 +726,7 @@ __global__ __launch_bounds__(kColThreads) void k_stageC(ConvShape s, FilterbankA
     cb_end = g + 1;
   }
 
-  const uint32_t hist_len = MODE == kStageCFold ? fa.nslot * fa.nval * fa.nbin : 0;
+  const uint32_t hist_len = MODE == kStageCFold ? fa.nslot * fa.hrow : 0;
   if (MODE == kStageCFold) {
     for (uint32_t i = tid; i < hist_len; i += kColThreads) hist[i] = 0.f;
   }
@@ -682,71 +802,22 @@ __global__ __launch_bounds__(kColThreads) void k_stageC(ConvShape s, FilterbankA
       constexpr int RF = FoldOwn<N1>::R;
       const uint32_t bq0 = (tid % (C / QN)) * QN;
       const uint32_t jr = tid / (C / QN);
-      const int lane = int(threadIdx.x & 63u);
       // key address: uniform row base plus a per-thread 32-bit byte offset
       const uint32_t kvo = (s.N2 * jr + bq0) * uint32_t(sizeof(uint32_t));
 #pragma unroll
       for (int r = 0; r < RF; r++) {
-        float cacc[NV];
-#pragma unroll
-        for (int i = 0; i < NV; i++) cacc[i] = 0.f;
-        uint32_t lane_key = 0xffffffffu;
         uint32_t kq[QN];
         const char* krow = reinterpret_cast<const char*>(kp + c0 + uint64_t(r * (N1 / RF)) * s.N2);
         load_keys<QN>(reinterpret_cast<const uint32_t*>(krow + kvo), kq);
-        // Samples after a bin edge inside this lane's samples: accumulated
-        // per run of equal keys and added once per run (bins a few samples
-        // wide, short periods with many bins, put an edge in most lanes)
-        uint32_t run_key = 0xffffffffu;
-        float racc[NV];
-#pragma unroll
-        for (int i = 0; i < NV; i++) racc[i] = 0.f;
-        auto flush_run = [&]() {
-          if (run_key != 0xffffffffu) {
-            float d[NV];
-            detect_finish<NV>(fa.state == DSPX_STOKES, racc, d);
-            float* hs = hist + (run_key >> 16) * (NV * fa.nbin) + (run_key & 0xffffu) * fa.ndim;
-#pragma unroll
-            for (int i = 0; i < NV; i++) atomicAdd(hs + fa.voff[i], d[i]);
-          }
-        };
+        float2 xa[QN], xb[QN];
 #pragma unroll
         for (int q = 0; q < QN; q++) {
-          const uint32_t key = kq[q];
-          if (q == 0) lane_key = key;
-          if (key == lane_key) {
-            detect_acc<NV>(s.npol, va[q * RF + r], vb[q * RF + r], cacc);
-          } else if (key != 0xffffffffu) {
-            if (key != run_key) {
-              flush_run();
-              run_key = key;
-#pragma unroll
-              for (int i = 0; i < NV; i++) racc[i] = 0.f;
-            }
-            detect_acc<NV>(s.npol, va[q * RF + r], vb[q * RF + r], racc);
-          }
+          xa[q] = va[q * RF + r];
+          xb[q] = vb[q * RF + r];
         }
-        flush_run();
-        float acc[NV];
-        detect_finish<NV>(fa.state == DSPX_STOKES, cacc, acc);
-        // is the whole 16-lane row in one bin?
-        const uint32_t k15 = __shfl(lane_key, lane | 15);
-        const uint64_t same = __ballot(lane_key == k15);
-        const bool row_uniform = ((same >> (lane & 48)) & 0xffffull) == 0xffffull;
-        float rs[NV];
-#pragma unroll
-        for (int i = 0; i < NV; i++) rs[i] = row_sum_lane15(acc[i]);
-        if (row_uniform) {
-          if ((lane & 15) == 15 && k15 != 0xffffffffu) {
-            float* hs = hist + (k15 >> 16) * (NV * fa.nbin) + (k15 & 0xffffu) * fa.ndim;
-#pragma unroll
-            for (int i = 0; i < NV; i++) atomicAdd(hs + fa.voff[i], rs[i]);
-          }
-        } else if (lane_key != 0xffffffffu) {
-          float* hs = hist + (lane_key >> 16) * (NV * fa.nbin) + (lane_key & 0xffffu) * fa.ndim;
-#pragma unroll
-          for (int i = 0; i < NV; i++) atomicAdd(hs + fa.voff[i], acc[i]);
-        }
+        // output index of the lane's first sample (row n1, column c0 + bq0)
+        const int32_t i0 = int32_t(p * s.step + s.N2 * (jr + r * (N1 / RF)) + c0 + bq0) - int32_t(s.nfilt_pos);
+        fold_lane<NV, QN, ZH>(hist, fa, s.npol, kq, xa, xb, chan, i0);
       }
     }
   };

@@ -457,6 +457,21 @@ int dspx_ltf_set_fold(dspx_ltf* ltf, double phase_per_sample, const dspx_fold_ch
 /* Raw device input -> profiles prof[isub][nchan][npol_out][nbin*ndim_out]
    and hits[isub][nbin], both accumulated. */
 int dspx_ltf_execute(dspx_ltf* ltf, const void* d_raw, float* d_prof, uint32_t* d_hits);
+/* The SpectralKurtosis-excised chain (LoadToFold1.C:265-377 with
+   coherent dedispersion: SpectralKurtosis on the zero-DM copy, its mask on
+   the dedispersed samples, SpectralKurtosis.C:1465-1566, then Detection and
+   Fold of zeroed data, Fold.C:911-930).  d_mask [nblk][nchan] bytes is the
+   dspx_sk_detect mask over the block's output samples in blocks of M (the
+   zero-DM input is the raw block from input sample nfilt_pos,
+   Convolution.C:792-799: dspx_sk_compute_raw with t0 = nfilt_pos and
+   nblk <= ndat_out / M blocks).  Output samples of flagged blocks, and those
+   past nblk*M (not in the cleaned series), are zeroed: they add nothing to
+   the profiles and no hits.  Hits are per channel, hits[isub][nchan][nbin]
+   (hits_nchan = nchan), counting the samples whose first detected value is
+   non-zero, accumulated; their sum is Fold's ndat_folded.  M >= 2;
+   ndat_out + nfft < 2^31. */
+int dspx_ltf_execute_masked(dspx_ltf* ltf, const void* d_raw, const uint8_t* d_mask, uint32_t M, uint64_t nblk,
+                            float* d_prof, uint32_t* d_hits);
 /* Raw device input -> dedispersed filterbank out[ichan][ipol][npart*step]
    complex (what the Convolution operation outputs). */
 int dspx_ltf_filterbank(dspx_ltf* ltf, const void* d_raw, float* d_out);

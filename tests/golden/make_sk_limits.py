@@ -1,5 +1,6 @@
 """Writes sk_limits.npz: the SK threshold tables the GPU tests use (counts
-0..32 in full, count 64 of M = 64 and count 256 of M = 256), from
+0..32 in full, count 64 of M = 64, count 256 of M = 256 and count 1024 of
+M = 1024), from
 dspsr-kanyixia_amd/sklimits.py,
 so that GPU test processes do not evaluate the Pearson integrals themselves.
 
@@ -16,7 +17,7 @@ sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
 S = importlib.import_module("dspsr-kanyixia_amd.sklimits")
 
 out = {}
-for M, counts in ((32, None), (64, [64]), (256, [256])):
+for M, counts in ((32, None), (64, [64]), (256, [256]), (1024, [1024])):
     lo, up, sym, pcsd = S.limit_tables(M, 3.0, npol=2, counts=counts)
     out["lower%d" % M], out["upper%d" % M], out["sym%d" % M] = lo, up, sym
     out["pcsd%d" % M] = np.float64(pcsd)
