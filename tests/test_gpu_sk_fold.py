@@ -25,8 +25,8 @@ def _rfi_raw(nchan, npol, ndat, M, t0, seed, tone_frac=0.05, imp_frac=0.01, am_f
     tones (2-20 sigma) in tone_frac of the channels, broadband impulses
     (5-50 sigma, 1-64 samples) in imp_frac of the M-sample blocks counted
     from input sample t0, 50 Hz-like amplitude modulation in am_frac of the
-    channels; zero_chan: a channel of zeros (detected values exactly 0,
-    which zeroed data does not count as hits)."""
+    channels; zero_chan: a channel of zero samples (which unpack to
+    0.5 + 0.5i: constant power)."""
     rng = np.random.default_rng(seed)
     x = rng.standard_normal((ndat, nchan, npol, 2)).astype(np.float32) * 10
     t = np.arange(ndat)
@@ -48,10 +48,10 @@ def _rfi_raw(nchan, npol, ndat, M, t0, seed, tone_frac=0.05, imp_frac=0.01, am_f
 
 
 def _oracle_chain(oracle, raw, nchan, npol, ndat, g, cfreq, bw, dm, state, ndim, nbin, pps, chunks, nsub, M,
-                  mask):
+                  mask, scale=1.0):
     """Convolution -> SpectralKurtosis::mask -> Detection -> Fold (zeroed
     data, hits_nchan = nchan) on the oracle."""
-    x = oracle.unpack8(raw, nchan, npol, ndat)
+    x = oracle.unpack8(raw, nchan, npol, ndat, scale=scale)
     x = np.ascontiguousarray(x).view(np.complex64).reshape(nchan, npol, ndat)
     resp = oracle.response(cfreq, bw, nchan, dm, g["nfft"], 0)
     fb = oracle.convolve(x, resp, g["impulse_pos"], g["impulse_neg"])
@@ -138,6 +138,7 @@ def test_sk_fold_rfi_band(ctx, dspx, oracle):
     L.close()
     sk.close()
     assert np.array_equal(hits, hits_o)
+    # the channel of constant power is zapped throughout (SK = 0)
     assert hits.sum() == nfold and hits[:, 9].sum() == 0
     assert e < 1e-5
 
@@ -157,13 +158,12 @@ FORM_CASES = [
 def test_sk_fold_forms(ctx, dspx, oracle, form, nfft, state, ndim, nbin, nsub, M, drop):
     """Random masks over blocks of M output samples (M not a power of two;
     the last blocks left out, as past the estimator's blocks), sub-integration
-    edges inside windows, one channel of zeros: every form's fold against
-    the oracle chain."""
+    edges inside windows: every form's fold against the oracle chain."""
     nchan, npol = 12, 2
     cfreq, bw, dm, period = 1500.0, 24.0, 8.0, 0.0047
     g = dspx.geometry(cfreq, bw, nchan, dm, nfft=nfft)
     ndat = 40 * g["step"] + g["noverlap"] + 11
-    raw = _rfi_raw(nchan, npol, ndat, M, g["impulse_pos"], seed=M + nbin, zero_chan=4)
+    raw = _rfi_raw(nchan, npol, ndat, M, g["impulse_pos"], seed=M + nbin)
     L = dspx.LoadToFold(ctx, 8, nchan, npol, ndat, cfreq, bw, dm, nbin, state=state, ndim=ndim,
                         form=getattr(dspx, "FORM_" + form), geom=g)
     nblk = L.ndat_out // M - drop
@@ -182,8 +182,33 @@ def test_sk_fold_forms(ctx, dspx, oracle, form, nfft, state, ndim, nbin, nsub, M
     print("profile %.3g" % e, flush=True)
     L.close()
     assert np.array_equal(hits, hits_o)
-    assert hits[:, 4].sum() == 0 and hits[:, 7].sum() == 0 and hits.sum() == nfold
+    assert hits[:, 7].sum() == 0 and hits.sum() == nfold
     assert e < 1e-5
+
+
+@pytest.mark.parametrize("form", ["ONEPASS", "TWOLEVEL", "PLAIN"])
+def test_sk_fold_zero_values_not_counted(ctx, dspx, oracle, form):
+    """Unmasked samples whose detected value is exactly zero (a block
+    unpacked with scale 0) add no hits (Fold.C:911-930), in every form."""
+    nchan, npol, M, nbin = 4, 2, 64, 128
+    g = dspx.geometry(1500.0, 24.0, nchan, 8.0, nfft=1024 if form == "ONEPASS" else 1 << 13)
+    ndat = 6 * g["step"] + g["noverlap"]
+    raw = _rfi_raw(nchan, npol, ndat, M, g["impulse_pos"], seed=5)
+    L = dspx.LoadToFold(ctx, 8, nchan, npol, ndat, 1500.0, 24.0, 8.0, nbin, state=dspx.STOKES, ndim=4,
+                        form=getattr(dspx, "FORM_" + form), scale=0.0, geom=g)
+    mask = np.zeros((L.ndat_out // M, nchan), np.uint8)
+    mask[::3, 1] = 1
+    rate = 24.0 / nchan * 1e6
+    chunks = dspx.fold_chunks(L.ndat_out, rate, 0.0031, g["impulse_pos"], nsub=2)
+    pps = (1.0 / rate) / 0.0031
+    L.set_fold(pps, chunks)
+    _, prof_o, hits_o, nfold = _oracle_chain(oracle, raw, nchan, npol, ndat, g, 1500.0, 24.0, 8.0, dspx.STOKES, 4,
+                                            nbin, pps, chunks, 2, M, mask, scale=0.0)
+    assert nfold == 0 and not hits_o.any() and not prof_o.any()
+    print("sk fold zero values", form, flush=True)
+    prof, hits = _run(ctx, dspx, L, raw, to_dev(mask), M, prof_o.shape, hits_o.shape)
+    L.close()
+    assert not hits.any() and not prof.any()
 
 
 def test_sk_fold_rejects_bad_mask_shape(ctx, dspx, oracle):
