@@ -51,3 +51,38 @@ def test_features_closed_forms():
     assert np.allclose(f[:, 1], np.float32(M + 1) / (M - 1) * 0 - 1, atol=2e-6)
     assert np.allclose(f[:, 0].mean(axis=0), 0, atol=1e-6)
     assert np.allclose(f[1:, 3, 0], np.log(2), atol=1e-6) and np.allclose(f[:, 3, 1], 0)
+
+
+def test_forward_and_gradient_match_torch():
+    """The restated forward (four 'same' zero-padded 1-D convolutions with
+    ReLU between them) against torch.nn.functional.conv1d, and the
+    restated loss and gradient against torch's autograd of
+    binary_cross_entropy_with_logits with the positive-class weight, all in
+    float64 on the CPU."""
+    import torch
+    import torch.nn.functional as Fn
+    rng = np.random.default_rng(4)
+    theta = rng.standard_normal(R.nparam()) * 0.2
+    feat = rng.standard_normal((3, R.F, 37))
+    labels = (rng.random((3, 37)) < 0.25).astype(np.uint8)
+    z, _ = R.forward(theta, feat)
+    t = torch.tensor(theta, dtype=torch.float64, requires_grad=True)
+    params, o = [], 0
+    for co, ci in R.SHAPES:
+        w = t[o:o + co * ci * R.K].view(co, ci, R.K)
+        o += co * ci * R.K
+        params.append((w, t[o:o + co]))
+        o += co
+    a = torch.tensor(feat, dtype=torch.float64)
+    for l, (w, b) in enumerate(params):
+        a = Fn.conv1d(a, w, b, padding=R.P)
+        if l < R.L - 1:
+            a = torch.relu(a)
+    zt = a[:, 0]
+    assert np.allclose(z, zt.detach().numpy(), rtol=1e-12, atol=1e-12)
+    y = torch.tensor(labels, dtype=torch.float64)
+    loss_t = Fn.binary_cross_entropy_with_logits(zt, y, pos_weight=torch.tensor(3.0, dtype=torch.float64))
+    loss_t.backward()
+    loss, g = R.loss_grad(theta, feat, labels, pos_weight=3.0)
+    assert abs(loss - loss_t.item()) < 1e-12
+    assert np.allclose(g, t.grad.numpy(), rtol=1e-10, atol=1e-14)
