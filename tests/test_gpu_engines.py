@@ -118,7 +118,7 @@ def test_convolution_tukey_known_answer_plain(ctx, dspx, oracle):
 @pytest.mark.parametrize("form", [0, 1])
 @pytest.mark.parametrize("cfreq,bw,nchan,dm,ndat", [
     (1405.0, 16.0, 1, 56.77, 3 << 19),      # single channel, N = 2^19
-    (1400.0, 25.0, 8, 300.0, 1 << 19),      # 3.125 MHz channels, N = 2^18
+    (1400.0, 25.0, 8, 300.0, 1 << 19),      # 3.125 MHz channels at the band top, N = 2^17
     (1500.0, 64.0, 16, 10.0, 1 << 15),      # N = 2^12
     (1400.0, 50.0, 16, 20.0, 1 << 15),      # N = 2^13
 ])
