@@ -86,6 +86,19 @@ class SkDesc(C.Structure):
 
 SK_SKFB, SK_FSCR, SK_TSCR = 1, 2, 4
 
+
+class Mjd(C.Structure):
+    _fields_ = [("day", C.c_int64), ("sec", C.c_int64), ("frac", C.c_double)]
+
+
+class TimedivBounds(C.Structure):
+    _fields_ = [("idat_start", C.c_uint64), ("ndat", C.c_uint64), ("division", C.c_uint64),
+                ("division_ndat", C.c_uint64), ("new_division", C.c_int32), ("end_reached", C.c_int32),
+                ("in_next", C.c_int32), ("is_valid", C.c_int32)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
 P = C.c_void_p
 U32, U64, I32, D = C.c_uint32, C.c_uint64, C.c_int, C.c_double
 
@@ -159,6 +172,10 @@ _SIGS = {
     "dspx_ltf_destroy": (None, [P]),
     "dspx_ltf_geometry": (I32, [P, C.POINTER(U64), C.POINTER(U64)]),
     "dspx_ltf_set_fold": (I32, [P, D, C.POINTER(FoldChunk), U32]),
+    "dspx_timediv_create": (I32, [P, D, D, D, P, C.POINTER(P)]),
+    "dspx_timediv_destroy": (None, [P]),
+    "dspx_timediv_set_bounds": (I32, [P, P, D, U64, P]),
+    "dspx_timediv_block": (I32, [P, P, D, U64, P, U32, C.POINTER(U32)]),
     "dspx_ltf_execute": (I32, [P, P, P, P]),
     "dspx_ltf_execute_masked": (I32, [P, P, P, U32, U64, P, P]),
     "dspx_ltf_filterbank": (I32, [P, P, P]),
@@ -241,6 +258,53 @@ def geometry(centre_frequency, bandwidth, nchan, dm, nfft=0, dc_centred=False):
         raise DspxError(rc, "geometry: smearing exceeds the limit or nfft too small")
     return {"impulse_neg": g.impulse_neg, "impulse_pos": g.impulse_pos, "nfft": g.nfft,
             "noverlap": g.nsamp_overlap, "step": g.nsamp_step, "nunsupported": g.nunsupported}
+
+
+class TimeDivide:
+    """dspx_timediv: sub-integration bounds across blocks (TimeDivide,
+    TimeDivide.C:36-506) for divisions of `seconds`, or of `turns` of a
+    constant folding period.  Epochs are (day, second of the day, fraction
+    of a second) triples."""
+
+    def __init__(self, start, seconds=0.0, turns=0.0, period=0.0, reference_epoch=None):
+        self.h = None
+        h = P()
+        ref = C.byref(Mjd(*reference_epoch)) if reference_epoch is not None else None
+        _check(lib().dspx_timediv_create(C.byref(Mjd(*start)), seconds, turns, period, ref, C.byref(h)))
+        self.h = h
+        self._lib = lib()
+
+    def close(self):
+        if self.h:
+            self._lib.dspx_timediv_destroy(self.h)
+            self.h = None
+
+    __del__ = close
+
+    def set_bounds(self, start, rate, ndat):
+        b = TimedivBounds()
+        _check(lib().dspx_timediv_set_bounds(self.h, C.byref(Mjd(*start)), rate, ndat, C.byref(b)))
+        return b.as_dict()
+
+    def block(self, start, rate, ndat, max_div=4096):
+        """The set_bounds calls Subint::transformation makes for one block:
+        the valid bounds, in order."""
+        arr = (TimedivBounds * max_div)()
+        n = C.c_uint32()
+        _check(lib().dspx_timediv_block(self.h, C.byref(Mjd(*start)), rate, ndat, arr, max_div, C.byref(n)))
+        return [arr[i].as_dict() for i in range(n.value)]
+
+    def fold_chunks(self, start, rate, ndat, period, reference_phase=0.0, t_ref=(0, 0, 0.0)):
+        """The block's bounds as fold chunks [(idat_start, ndat, phi,
+        division)], phi the phase of each chunk's first sample (constant
+        period, zero phase at t_ref plus reference_phase)."""
+        out = []
+        for b in self.block(start, rate, ndat):
+            day, sec, frac = start
+            t = (float(day - t_ref[0]) * 86400.0 + float(sec - t_ref[1]) + (frac - t_ref[2])
+                 + (b["idat_start"] + 0.5) / rate)
+            out.append((b["idat_start"], b["ndat"], fold_phi(t, period, reference_phase), b["division"]))
+        return out
 
 
 def binplan_host(phi, pps, nbin, ndat):

@@ -351,6 +351,55 @@ int dspx_fold_static(dspx_ctx* ctx, uint64_t ndat, double phase_per_sample, uint
                      uint64_t in_chan_stride, uint64_t in_pol_stride, uint32_t nchan, uint32_t npol,
                      uint32_t ndim, float* d_prof, uint32_t* d_hits);
 
+/* ---------------------------------------------------- sub-integrations */
+
+/* dsp::TimeDivide (Signal/Pulsar/TimeDivide.C:36-506) with divisions of a
+   fixed number of seconds, or of turns of a constant folding period (the
+   branch Subint::prepare takes without a predictor, Subint.h:217-240):
+   which samples of each block belong to which sub-integration, continued
+   across blocks.  Host computation; no device work. */
+typedef struct {
+  int64_t day;  /* MJD: integer day */
+  int64_t sec;  /* seconds of the day */
+  double frac;  /* fraction of a second */
+} dspx_mjd;
+
+typedef struct dspx_timediv dspx_timediv;
+
+/* The samples [idat_start, idat_start + ndat) of a block in division
+   `division` (counted from the start time) of division_ndat samples, and
+   TimeDivide's flags: the division is new (bounds started a new one), its
+   end is in this block, the block has samples for a later division. */
+typedef struct {
+  uint64_t idat_start;
+  uint64_t ndat;
+  uint64_t division;
+  uint64_t division_ndat;
+  int32_t new_division;
+  int32_t end_reached;
+  int32_t in_next;
+  int32_t is_valid;
+} dspx_timediv_bounds;
+
+/* set_start_time / set_seconds / set_turns / set_period / set_reference_epoch
+   (TimeDivide.C:36-104): exactly one of seconds and turns positive; turns
+   need period (s).  Without a reference epoch, a whole number of seconds
+   per division rounds the start down to a multiple of it within the day
+   (TimeDivide.C:55-65).  reference_epoch may be NULL. */
+int dspx_timediv_create(const dspx_mjd* start_time, double seconds, double turns, double period,
+                        const dspx_mjd* reference_epoch, dspx_timediv** out);
+void dspx_timediv_destroy(dspx_timediv* td);
+/* One TimeDivide::set_bounds call (TimeDivide.C:124-338) for a block of
+   input_ndat samples at `rate` (Hz) starting at input_start; division edges
+   are rounded to the block's samples (set_boundaries, :467-506). */
+int dspx_timediv_set_bounds(dspx_timediv* td, const dspx_mjd* input_start, double rate, uint64_t input_ndat,
+                            dspx_timediv_bounds* out);
+/* The set_bounds calls Subint::transformation makes for one block (until
+   in_next is clear, Subint.h:253-330): the block's valid bounds, in order,
+   into out[max]; *n of them.  DSPX_E_PARAM when more than max. */
+int dspx_timediv_block(dspx_timediv* td, const dspx_mjd* input_start, double rate, uint64_t input_ndat,
+                       dspx_timediv_bounds* out, uint32_t max, uint32_t* n);
+
 /* ------------------------------------------------------- spectral kurtosis */
 /* One resolution of dsp::SpectralKurtosis (noverlap 1, every channel
    included).  Sums [npart][nchan][npol][2] float (S1, S2), counts
