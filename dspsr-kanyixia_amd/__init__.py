@@ -81,7 +81,7 @@ class FbDesc(C.Structure):
 
 class SkDesc(C.Structure):
     _fields_ = [("nchan", C.c_uint32), ("npol", C.c_uint32), ("M", C.c_uint32), ("Nd", C.c_uint32),
-                ("omit_outliers", C.c_uint32), ("reserved", C.c_uint32)]
+                ("omit_outliers", C.c_uint32), ("noverlap", C.c_uint32)]
 
 
 SK_SKFB, SK_FSCR, SK_TSCR = 1, 2, 4
@@ -147,6 +147,8 @@ _SIGS = {
     "dspx_sk_compute_raw": (I32, [P, P, U32, U32, C.c_float, U64, U64, P, P]),
     "dspx_sk_detect": (I32, [P, P, P, U64, U32, P]),
     "dspx_sk_tscr_totals": (I32, [P, P, U64, P]),
+    "dspx_sk_set_channels": (I32, [P, P]),
+    "dspx_sk_blocks": (I32, [P, U64, C.POINTER(U64), C.POINTER(U64)]),
     "dspx_sk_set_tscr_limits": (I32, [P, U32, P, P, P]),
     "dspx_sk_detect_tscr": (I32, [P, P, P, U64, P]),
     "dspx_sk_mask": (I32, [P, P, P, U64, U64, P, U64, U64, U64]),
@@ -609,13 +611,14 @@ class SpectralKurtosis:
     Tensors are device buffers: sums [npart][nchan][npol][2] float32, counts
     [npart][nchan] int32 (read as uint32), mask [npart][nchan] uint8."""
 
-    def __init__(self, ctx, nchan, npol, M, Nd=1, omit_outliers=False):
+    def __init__(self, ctx, nchan, npol, M, Nd=1, omit_outliers=False, noverlap=1):
         self.ctx = ctx
         self.h = None
         self._lib = lib()
-        self.nchan, self.npol, self.M = nchan, npol, M
+        self.nchan, self.npol, self.M, self.noverlap = nchan, npol, M, noverlap
         h = P()
-        ctx.check(lib().dspx_sk_create(ctx.h, C.byref(SkDesc(nchan, npol, M, Nd, int(omit_outliers), 0)), C.byref(h)))
+        ctx.check(lib().dspx_sk_create(ctx.h, C.byref(SkDesc(nchan, npol, M, Nd, int(omit_outliers), noverlap)),
+                                       C.byref(h)))
         self.h = h
         _register(self, False)
 
@@ -636,6 +639,23 @@ class SpectralKurtosis:
         self.ctx.check(lib().dspx_sk_set_limits(self.h, lo.ctypes.data, up.ctypes.data, sy.ctypes.data, pcsd))
         self.std_devs, self.Nd = std_devs, Nd
         self._tscr = {}
+
+    def blocks(self, ndat):
+        """(npart, ndat_out) of a series of ndat samples."""
+        npart, nout = U64(), U64()
+        _check(lib().dspx_sk_blocks(self.h, ndat, C.byref(npart), C.byref(nout)))
+        return npart.value, nout.value
+
+    def set_channels(self, include=None):
+        """include: per-channel flags (the averaging detectors' channels), or
+        None for every channel."""
+        import numpy as np
+        if include is None:
+            self.ctx.check(lib().dspx_sk_set_channels(self.h, None))
+            return
+        inc = np.ascontiguousarray(include, np.uint8)
+        assert inc.size == self.nchan
+        self.ctx.check(lib().dspx_sk_set_channels(self.h, inc.ctypes.data))
 
     def tscr_totals(self, counts, npart):
         import numpy as np

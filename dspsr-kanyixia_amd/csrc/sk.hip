@@ -1,6 +1,7 @@
 // sk.hip -- spectral kurtosis engine: block sums, the three detectors of
-// SpectralKurtosis::detect and the mask, for one resolution (noverlap 1,
-// every channel included).
+// SpectralKurtosis::detect and the mask, for one resolution (blocks of M
+// samples starting every M / noverlap samples; the averaging detectors
+// restricted to the included channels).
 //
 // Reference: SpectralKurtosis.C:674-798 (compute), 830-909 (detect),
 // 1105-1222 (detect_skfb), 1312-1462 (detect_fscr), 1014-1103 (detect_tscr),
@@ -25,6 +26,9 @@ struct dspx_sk {
   uint32_t ntscr = 0;
   dspx::DevBuf status;  // detect_tscr: [0] totals without thresholds, [1] the smallest of them
   dspx::DevBuf tot;     // per-channel totals (dspx_sk_tscr_totals)
+  dspx::DevBuf chans;   // channels the averaging detectors include (dspx_sk_set_channels), 1 byte each
+  bool have_chans = false;
+  uint32_t nov = 1, off = 0;  // noverlap; blocks start every off = M / noverlap samples
   double pcsd = 0;
   bool have_limits = false;
 };
@@ -54,7 +58,8 @@ __device__ __forceinline__ void block_sum(double* a, double* b, uint32_t* c, int
 // grid (npart, nchan): S1 = sum sq, S2 = sum sq^2 over block ipart of each pol
 // (SpectralKurtosis.C:752-766), counts of pol 0 (:781-787).
 __global__ __launch_bounds__(kSkThreads) void k_sk_sums_fpt(const float* __restrict__ in, uint64_t cs, uint64_t ps,
-                                                           uint32_t nchan, uint32_t npol, uint32_t M, int zeroed,
+                                                           uint32_t nchan, uint32_t npol, uint32_t M, uint32_t off,
+                                                           int zeroed,
                                                            float* __restrict__ sums, uint32_t* __restrict__ counts)
 {
 #pragma clang fp contract(off)
@@ -64,7 +69,7 @@ __global__ __launch_bounds__(kSkThreads) void k_sk_sums_fpt(const float* __restr
   const uint64_t part = blockIdx.x;
   const uint32_t chan = blockIdx.y;
   for (uint32_t pol = 0; pol < npol; pol++) {
-    const float2* x = reinterpret_cast<const float2*>(in + uint64_t(chan) * cs + uint64_t(pol) * ps) + part * M;
+    const float2* x = reinterpret_cast<const float2*>(in + uint64_t(chan) * cs + uint64_t(pol) * ps) + part * off;
     double s1 = 0, s2 = 0;
     uint32_t c = 0;
     for (uint32_t i = tid; i < M; i += kSkThreads) {
@@ -94,7 +99,8 @@ __global__ __launch_bounds__(kSkThreads) void k_sk_sums_fpt(const float* __restr
 // 8 row groups stride the block; the row groups are summed in order.
 template <class LD>
 __global__ __launch_bounds__(kSkThreads) void k_sk_sums_raw(LD ld, uint64_t t0, uint32_t nchan, uint32_t M,
-                                                           float* __restrict__ sums, uint32_t* __restrict__ counts)
+                                                           uint32_t off, float* __restrict__ sums,
+                                                           uint32_t* __restrict__ counts)
 {
 #pragma clang fp contract(off)
   constexpr int CH = 32, RG = kSkThreads / CH;
@@ -106,7 +112,7 @@ __global__ __launch_bounds__(kSkThreads) void k_sk_sums_raw(LD ld, uint64_t t0, 
   const uint32_t npol = ld.npol;
   double s1[2] = {0, 0}, s2[2] = {0, 0};
   if (chan < nchan) {
-    const uint64_t tb = t0 + part * M;
+    const uint64_t tb = t0 + part * off;
     for (uint32_t i = rg; i < M; i += RG) {
       typename LD::Word w = ld.word(chan, tb + i);
 #pragma unroll
@@ -146,7 +152,7 @@ __global__ __launch_bounds__(kSkThreads) void k_sk_sums_raw(LD ld, uint64_t t0, 
 // load of their words), a workgroup 128 channels.  The row groups and the
 // order of the additions are those of k_sk_sums_raw, so the sums agree.
 __global__ __launch_bounds__(kSkThreads) void k_sk_sums_raw8x4(LoadI8 ld, uint64_t t0, uint32_t nchan, uint32_t M,
-                                                              float* __restrict__ sums,
+                                                              uint32_t off, float* __restrict__ sums,
                                                               uint32_t* __restrict__ counts)
 {
 #pragma clang fp contract(off)
@@ -158,7 +164,7 @@ __global__ __launch_bounds__(kSkThreads) void k_sk_sums_raw8x4(LoadI8 ld, uint64
   const uint32_t c0 = blockIdx.y * (CL * CW) + cl * CW;
   double s1[CW][2] = {}, s2[CW][2] = {};
   if (c0 < nchan) {
-    const uint64_t tb = t0 + part * M;
+    const uint64_t tb = t0 + part * off;
     for (uint32_t i = rg; i < M; i += RG) {
       const uint4 w = *reinterpret_cast<const uint4*>(ld.raw + (tb + i) * ld.tstride + uint64_t(c0) * 4);
       const uint32_t wd[CW] = {w.x, w.y, w.z, w.w};
@@ -238,11 +244,13 @@ __global__ void k_sk_skfb(float* __restrict__ sums, uint32_t* __restrict__ count
 }
 
 // detect_fscr (SpectralKurtosis.C:1369-1451): one workgroup per block; the
-// channel sums of each pol are reduced in a fixed order.
+// channel sums of each pol are reduced in a fixed order, over the included
+// channels (chans, or every channel when null; Resolution::get_channels).
 __global__ __launch_bounds__(kSkThreads) void k_sk_fscr(const float* __restrict__ sums,
                                                        const uint32_t* __restrict__ counts, uint32_t nchan,
                                                        uint32_t npol, uint32_t M, uint32_t Nd, int omit,
                                                        const float* __restrict__ symmetric,
+                                                       const uint8_t* __restrict__ chans,
                                                        uint8_t* __restrict__ mask)
 {
 #pragma clang fp contract(off)
@@ -255,7 +263,7 @@ __global__ __launch_bounds__(kSkThreads) void k_sk_fscr(const float* __restrict_
   for (uint32_t pol = 0; pol < npol; pol++) {
     double sk_avg = 0, M_tot = 0, Msq_tot = 0, chan_count = 0;
     for (uint32_t ch = tid; ch < nchan; ch += kSkThreads) {
-      if (!omit || mrow[ch] == 0) {
+      if ((!chans || chans[ch]) && (!omit || mrow[ch] == 0)) {
         const float* sd = sums + ((part * nchan + ch) * npol + pol) * 2;
         float S1 = sd[0], S2 = sd[1];
         unsigned Mprime = counts[part * nchan + ch];
@@ -299,15 +307,17 @@ __global__ __launch_bounds__(kSkThreads) void k_sk_fscr(const float* __restrict_
 }
 
 // The per-channel totals of the counts (detect_tscr's M_t before its test,
-// SpectralKurtosis.C:1036-1040), one workgroup per channel.
+// SpectralKurtosis.C:1036-1040: every noverlap-th block), one workgroup per
+// channel.
 __global__ __launch_bounds__(kSkThreads) void k_sk_totals(const uint32_t* __restrict__ counts, uint64_t npart,
-                                                         uint32_t nchan, unsigned long long* __restrict__ totals)
+                                                         uint32_t nchan, uint32_t nov,
+                                                         unsigned long long* __restrict__ totals)
 {
   __shared__ unsigned long long rc[kSkThreads];
   const uint32_t ch = blockIdx.x;
   const int tid = threadIdx.x;
   unsigned long long c = 0;
-  for (uint64_t p = tid; p < npart; p += kSkThreads) c += counts[p * nchan + ch];
+  for (uint64_t p = uint64_t(tid) * nov; p < npart; p += uint64_t(kSkThreads) * nov) c += counts[p * nchan + ch];
   rc[tid] = c;
   for (int st = kSkThreads / 2; st > 0; st >>= 1) {
     __syncthreads();
@@ -335,6 +345,7 @@ __global__ __launch_bounds__(kSkThreads) void k_sk_tscr(const float* __restrict_
                                                        const float* __restrict__ tlo,
                                                        const float* __restrict__ tup, uint32_t ntscr,
                                                        unsigned long long* __restrict__ status,
+                                                       const uint8_t* __restrict__ chans, uint32_t nov,
                                                        uint8_t* __restrict__ mask)
 {
 #pragma clang fp contract(off)
@@ -345,10 +356,12 @@ __global__ __launch_bounds__(kSkThreads) void k_sk_tscr(const float* __restrict_
   __shared__ uint64_t rc[kSkThreads];
   __shared__ int zap;
   const uint32_t ch = blockIdx.x;
+  if (chans && !chans[ch]) return;  // not an included channel (SpectralKurtosis.C:1032-1033)
   const int tid = threadIdx.x;
   uint64_t count = 0;
   double s1[2] = {0.0, 0.0}, s2[2] = {0.0, 0.0};
-  for (uint64_t p = tid; p < npart; p += kSkThreads) {
+  // the non-overlapping blocks: every noverlap-th (:1037, 1067)
+  for (uint64_t p = uint64_t(tid) * nov; p < npart; p += uint64_t(kSkThreads) * nov) {
     count += counts[p * nchan + ch];
     for (uint32_t pol = 0; pol < npol; pol++) {
       const uint64_t idx = (p * nchan + ch) * npol + pol;
@@ -419,28 +432,38 @@ __global__ __launch_bounds__(kSkThreads) void k_sk_tscr(const float* __restrict_
     for (uint64_t p = tid; p < npart; p += kSkThreads) mask[p * nchan + ch] = 1;
 }
 
-// grid (ceil(n / 512), nchan * npol): two complex samples per thread, one
-// 16-byte move where both rows allow it and M is even (the pair then lies
-// in one block), else two 8-byte moves
+// SpectralKurtosis::mask (SpectralKurtosis.C:1465-1566), one resolution:
+// output sample t (< n) is input sample t + idat_start (idat_start = M - off)
+// of block min(t / off, npm - 1) + nskip's channel, zeroed in every pol when
+// that block is flagged.  (The reference writes the blocks in order, each
+// M samples from ipart * off, so the last block over a sample decides it.)
+// With noverlap 1: off = M, idat_start = nskip = 0, out[t] = in[t] masked by
+// block t / M.  grid (ceil(n / 512), nchan * npol): two complex samples per
+// thread, one 16-byte move where both rows allow it and off is even (the
+// pair then lies in one block), else two 8-byte moves.
+struct SkMaskShape {
+  uint32_t nchan, npol, off, nskip;
+  uint64_t npm, idat_start, n;
+};
 __global__ void k_sk_mask(const uint8_t* __restrict__ mask, const float* __restrict__ in, uint64_t ics, uint64_t ips,
-                          float* __restrict__ out, uint64_t ocs, uint64_t ops, uint32_t nchan, uint32_t npol,
-                          uint32_t M, uint64_t n)
+                          float* __restrict__ out, uint64_t ocs, uint64_t ops, SkMaskShape m)
 {
   const uint64_t t = (uint64_t(blockIdx.x) * blockDim.x + threadIdx.x) * 2;
-  if (t >= n) return;
-  const uint32_t chan = blockIdx.y / npol, pol = blockIdx.y % npol;
-  const float2* x = reinterpret_cast<const float2*>(in + chan * ics + pol * ips);
+  if (t >= m.n) return;
+  const uint32_t chan = blockIdx.y / m.npol, pol = blockIdx.y % m.npol;
+  const float2* x = reinterpret_cast<const float2*>(in + chan * ics + pol * ips) + m.idat_start;
   float2* y = reinterpret_cast<float2*>(out + chan * ocs + pol * ops);
+  auto flagged = [&](uint64_t k) { return mask[(min(k / m.off, m.npm - 1) + m.nskip) * m.nchan + chan] != 0; };
   const uintptr_t al = reinterpret_cast<uintptr_t>(x) | reinterpret_cast<uintptr_t>(y);
-  if (t + 2 <= n && (al & 15) == 0 && (M & 1) == 0) {
+  if (t + 2 <= m.n && (al & 15) == 0 && (m.off & 1) == 0) {
     float4 v = *reinterpret_cast<const float4*>(x + t);
-    if (mask[(t / M) * nchan + chan]) v = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (flagged(t)) v = make_float4(0.f, 0.f, 0.f, 0.f);
     *reinterpret_cast<float4*>(y + t) = v;
     return;
   }
-  for (uint64_t k = t; k < t + 2 && k < n; k++) {
+  for (uint64_t k = t; k < t + 2 && k < m.n; k++) {
     float2 v = x[k];
-    if (mask[(k / M) * nchan + chan]) v = make_float2(0.f, 0.f);
+    if (flagged(k)) v = make_float2(0.f, 0.f);
     y[k] = v;
   }
 }
@@ -465,7 +488,8 @@ void run_tscr(dspx_sk* s, const float* d_sums, const uint32_t* d_counts, uint64_
   const float* tlo = reinterpret_cast<const float*>(ttot + s->ntscr);
   k_sk_status_init<<<1, 1, 0, st>>>(status);
   k_sk_tscr<<<d.nchan, kSkThreads, 0, st>>>(d_sums, d_counts, npart, d.nchan, d.npol, d.M, d.Nd, lower, upper,
-                                            s->pcsd, ttot, tlo, tlo + s->ntscr, s->ntscr, status, d_mask);
+                                            s->pcsd, ttot, tlo, tlo + s->ntscr, s->ntscr, status,
+                                            s->have_chans ? s->chans.as<uint8_t>() : nullptr, s->nov, d_mask);
   DSPX_HIP(hipGetLastError());
   unsigned long long h[2];
   DSPX_HIP(hipMemcpyAsync(h, status, sizeof(h), hipMemcpyDeviceToHost, st));
@@ -492,10 +516,14 @@ int dspx_sk_create(dspx_ctx* ctx, const dspx_sk_desc* desc, dspx_sk** out)
     DSPX_REQUIRE(d.npol == 1 || d.npol == 2, DSPX_E_PARAM, "npol=%u", d.npol);
     DSPX_REQUIRE(d.M >= 2, DSPX_E_PARAM, "M=%u < 2 (SKLimits::calc_limits)", d.M);
     DSPX_REQUIRE(d.Nd >= 1, DSPX_E_PARAM, "Nd=%u", d.Nd);
+    const uint32_t nov = d.noverlap ? d.noverlap : 1;
+    DSPX_REQUIRE(d.M % nov == 0, DSPX_E_STATE, "noverlap=%u does not divide M=%u", nov, d.M);
     auto* s = new dspx_sk;
     s->hold.reset(ctx);
     s->ctx = ctx;
     s->d = d;
+    s->nov = nov;
+    s->off = d.M / nov;
     *out = s;
   });
 }
@@ -534,7 +562,7 @@ int dspx_sk_compute(dspx_sk* s, const float* d_in, uint64_t cs, uint64_t ps, uin
     DSPX_REQUIRE(npart < (1u << 31), DSPX_E_UNSUPPORTED, "npart=%llu", (unsigned long long) npart);
     if (!npart) return;
     dim3 grid(unsigned(npart), s->d.nchan);
-    k_sk_sums_fpt<<<grid, kSkThreads, 0, s->ctx->stream>>>(d_in, cs, ps, s->d.nchan, s->d.npol, s->d.M,
+    k_sk_sums_fpt<<<grid, kSkThreads, 0, s->ctx->stream>>>(d_in, cs, ps, s->d.nchan, s->d.npol, s->d.M, s->off,
                                                             zeroed_data, d_sums, d_counts);
     DSPX_HIP(hipGetLastError());
   });
@@ -555,14 +583,14 @@ int dspx_sk_compute_raw(dspx_sk* s, const void* d_raw, uint32_t nbit, uint32_t o
       if (npol == 2 && nchan % 4 == 0 && reinterpret_cast<uintptr_t>(d_raw) % 16 == 0) {
         // rows of 4*nchan bytes: 16-byte aligned with the base
         dim3 g4(unsigned(npart), (nchan + 127) / 128);
-        k_sk_sums_raw8x4<<<g4, kSkThreads, 0, s->ctx->stream>>>(ld, t0, nchan, s->d.M, d_sums, d_counts);
+        k_sk_sums_raw8x4<<<g4, kSkThreads, 0, s->ctx->stream>>>(ld, t0, nchan, s->d.M, s->off, d_sums, d_counts);
       } else {
-        k_sk_sums_raw<<<grid, kSkThreads, 0, s->ctx->stream>>>(ld, t0, nchan, s->d.M, d_sums, d_counts);
+        k_sk_sums_raw<<<grid, kSkThreads, 0, s->ctx->stream>>>(ld, t0, nchan, s->d.M, s->off, d_sums, d_counts);
       }
     } else if (nbit == 16) {
       LoadI16 ld{static_cast<const int16_t*>(d_raw), uint64_t(nchan) * npol * 2, uint64_t(npol) * 2, npol,
                  offset_binary};
-      k_sk_sums_raw<<<grid, kSkThreads, 0, s->ctx->stream>>>(ld, t0, nchan, s->d.M, d_sums, d_counts);
+      k_sk_sums_raw<<<grid, kSkThreads, 0, s->ctx->stream>>>(ld, t0, nchan, s->d.M, s->off, d_sums, d_counts);
     } else {
       fail(DSPX_E_PARAM, "nbit=%u (8 or 16 supported)", nbit);
     }
@@ -592,7 +620,8 @@ int dspx_sk_detect(dspx_sk* s, float* d_sums, uint32_t* d_counts, uint64_t npart
                                                           d.omit_outliers, lower, upper, d_mask);
     if (flags & DSPX_SK_FSCR)
       k_sk_fscr<<<unsigned(npart), kSkThreads, 0, st>>>(d_sums, d_counts, d.nchan, d.npol, d.M, d.Nd,
-                                                        d.omit_outliers, sym, d_mask);
+                                                        d.omit_outliers, sym,
+                                                        s->have_chans ? s->chans.as<uint8_t>() : nullptr, d_mask);
     DSPX_HIP(hipGetLastError());
     if (flags & DSPX_SK_TSCR) run_tscr(s, d_sums, d_counts, npart, d_mask);
   });
@@ -619,6 +648,36 @@ int dspx_sk_set_tscr_limits(dspx_sk* s, uint32_t n, const uint64_t* totals, cons
   });
 }
 
+int dspx_sk_set_channels(dspx_sk* s, const uint8_t* include)
+{
+  return guard(s ? s->ctx : nullptr, [&] {
+    DSPX_REQUIRE(s, DSPX_E_PARAM, "null argument");
+    if (!include) {
+      s->have_chans = false;
+      return;
+    }
+    s->chans.reserve(s->d.nchan);
+    DSPX_HIP(hipMemcpyAsync(s->chans.p, include, s->d.nchan, hipMemcpyHostToDevice, s->ctx->stream));
+    DSPX_HIP(hipStreamSynchronize(s->ctx->stream));
+    s->have_chans = true;
+  });
+}
+
+int dspx_sk_blocks(const dspx_sk* s, uint64_t ndat, uint64_t* npart, uint64_t* ndat_out)
+{
+  if (!s || !npart || !ndat_out) return DSPX_E_PARAM;
+  // Resolution::prepare (SpectralKurtosis.C:263-281)
+  const uint64_t off = s->off, idat_start = s->d.M - off;
+  *npart = 0;
+  *ndat_out = 0;
+  if (ndat < idat_start) return DSPX_OK;
+  const uint64_t nblock = (ndat - idat_start) / off;
+  if (nblock * off < idat_start) return DSPX_OK;
+  *npart = nblock;
+  *ndat_out = nblock * off - idat_start;
+  return DSPX_OK;
+}
+
 int dspx_sk_tscr_totals(dspx_sk* s, const uint32_t* d_counts, uint64_t npart, uint64_t* h_totals)
 {
   return guard(s ? s->ctx : nullptr, [&] {
@@ -626,7 +685,7 @@ int dspx_sk_tscr_totals(dspx_sk* s, const uint32_t* d_counts, uint64_t npart, ui
     const uint32_t nchan = s->d.nchan;
     hipStream_t st = s->ctx->stream;
     s->tot.reserve(size_t(nchan) * sizeof(unsigned long long));
-    k_sk_totals<<<nchan, kSkThreads, 0, st>>>(d_counts, npart, nchan, s->tot.as<unsigned long long>());
+    k_sk_totals<<<nchan, kSkThreads, 0, st>>>(d_counts, npart, nchan, s->nov, s->tot.as<unsigned long long>());
     DSPX_HIP(hipGetLastError());
     DSPX_HIP(hipMemcpyAsync(h_totals, s->tot.p, size_t(nchan) * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
     DSPX_HIP(hipStreamSynchronize(st));
@@ -649,13 +708,21 @@ int dspx_sk_mask(dspx_sk* s, const uint8_t* d_mask, const float* d_in, uint64_t 
 {
   return guard(s ? s->ctx : nullptr, [&] {
     DSPX_REQUIRE(s && d_mask && d_in && d_out, DSPX_E_PARAM, "null argument");
-    const uint64_t n = npart * s->d.M;
-    if (!n) return;
+    if (!npart) return;
+    // Resolution::prepare (SpectralKurtosis.C:263-281) and mask (:1480-1510)
+    // with one resolution: the output holds (npart + 1) * off - M samples
+    const uint64_t M = s->d.M, off = s->off;
+    DSPX_REQUIRE((npart + 1) * off >= 2 * M, DSPX_E_PARAM, "%llu blocks of %llu with noverlap %u: no output",
+                 (unsigned long long) npart, (unsigned long long) M, s->nov);
+    SkMaskShape m{s->d.nchan, s->d.npol, uint32_t(off), s->nov - 1, 0, M - off, (npart + 1) * off - M};
+    m.npm = (m.n - M) / off + 1;
+    DSPX_REQUIRE(m.npm + m.nskip <= npart, DSPX_E_INTERNAL, "mask blocks");
+    DSPX_REQUIRE(d_in != d_out || s->nov == 1, DSPX_E_PARAM, "in place with noverlap %u", s->nov);
+    const uint64_t n = m.n;
     DSPX_REQUIRE((n + 511) / 512 < (1ull << 31), DSPX_E_UNSUPPORTED, "block too long");
     dim3 grid(unsigned((n + 511) / 512), s->d.nchan * s->d.npol);
     DSPX_REQUIRE(grid.y <= 65535, DSPX_E_UNSUPPORTED, "nchan*npol=%u", grid.y);
-    k_sk_mask<<<grid, 256, 0, s->ctx->stream>>>(d_mask, d_in, ics, ips, d_out, ocs, ops, s->d.nchan, s->d.npol,
-                                                s->d.M, n);
+    k_sk_mask<<<grid, 256, 0, s->ctx->stream>>>(d_mask, d_in, ics, ips, d_out, ocs, ops, m);
     DSPX_HIP(hipGetLastError());
   });
 }

@@ -174,13 +174,15 @@ void SpectralKurtosisEngine::setup(SpectralKurtosis* s)
   dspx_sk_destroy(sk);
   sk = nullptr;
   nchan = in->get_nchan();
-  dspx_sk_desc d{nchan, in->get_npol(), s->get_M(), 1u, s->get_omit_outliers() ? 1u : 0u, 0u};
+  dspx_sk_desc d{nchan, in->get_npol(), s->get_M(), 1u, s->get_omit_outliers() ? 1u : 0u, s->get_noverlap()};
   setup_M = s->get_M();
   tscr_valid = false;
   ctx->check(dspx_sk_create(ctx->get(), &d, &sk), "dsp::HIP::SpectralKurtosisEngine::setup");
   ctx->check(dspx_sk_set_limits(sk, s->get_lower().data(), s->get_upper().data(), s->get_symmetric().data(),
                                 s->get_pc_std_devs()),
              "dsp::HIP::SpectralKurtosisEngine::setup");
+  const std::vector<uint8_t> chans = s->get_channels(nchan);
+  ctx->check(dspx_sk_set_channels(sk, chans.data()), "dsp::HIP::SpectralKurtosisEngine::setup");
 }
 
 void SpectralKurtosisEngine::compute(const TimeSeries* in, float* sums, uint32_t* counts, uint64_t npart)

@@ -207,6 +207,22 @@ void SpectralKurtosis::set_limits(std::vector<float> lo, std::vector<float> hi, 
   prepared = false;
 }
 
+std::vector<uint8_t> SpectralKurtosis::get_channels(unsigned nchan) const
+{
+  std::vector<uint8_t> c(nchan, include.empty() ? 1 : 0);
+  auto set = [&](const std::vector<std::pair<unsigned, unsigned>>& r, uint8_t v) {
+    for (const auto& p : r) {
+      if (p.second >= nchan || p.first > p.second)
+        throw Error(InvalidParam, "dsp::SpectralKurtosis::Resolution::get_channels", "range [%u, %u] of %u channels",
+                    p.first, p.second, nchan);
+      for (unsigned i = p.first; i <= p.second; i++) c[i] = v;
+    }
+  };
+  set(include, 1);
+  set(exclude, 0);
+  return c;
+}
+
 SpectralKurtosis::~SpectralKurtosis()
 {
   Memory::device()->free(sums);
@@ -224,7 +240,14 @@ void SpectralKurtosis::operate()
     engine->setup(this);
     prepared = true;
   }
-  npart = input->get_ndat() / M;  // Resolution::prepare (SpectralKurtosis.C:263-281), noverlap 1
+  // Resolution::prepare (SpectralKurtosis.C:263-281)
+  if (M % noverlap)
+    throw Error(InvalidState, "dsp::SpectralKurtosis::Resolution::prepare", "noverlap=%u does not divide M=%u",
+                noverlap, M);
+  const uint64_t off = M / noverlap, idat_start = M - off, ndat = input->get_ndat();
+  npart = ndat < idat_start ? 0 : (ndat - idat_start) / off;
+  if (npart * off < idat_start) npart = 0;
+  const uint64_t ndat_out = npart ? npart * off - idat_start : 0;
   if (npart > capacity) {
     Memory* dev = Memory::device();
     dev->free(sums);
@@ -237,7 +260,7 @@ void SpectralKurtosis::operate()
     capacity = npart;
   }
   output->copy_configuration(input);
-  output->resize(npart * M);  // output_ndat = Nblock * M
+  output->resize(ndat_out);  // output_ndat = Nblock * off - idat_start
   output->set_zeroed_data(true);  // zapped samples are zeroed (SpectralKurtosis.C:1468)
   if (!npart) return;
   engine->compute(input, sums, counts, npart);

@@ -194,7 +194,7 @@ class Filterbank {
   uint64_t npart = 0;
 };
 
-//! Spectral kurtosis RFI excision at one resolution (noverlap 1): block sums,
+//! Spectral kurtosis RFI excision at one resolution: block sums,
 //! the detectors of SpectralKurtosis::detect and the mask
 //! (Signal/General/SpectralKurtosis.C:674-909, 1014-1566).  The thresholds
 //! come from SKLimits in the Transformation (SpectralKurtosis.C:800-822,
@@ -226,6 +226,20 @@ class SpectralKurtosis {
   void set_engine(Engine* e) { engine.reset(e); }
   void set_M(unsigned m) { M = m; }
   unsigned get_M() const { return M; }
+  //! blocks of M samples every M / noverlap samples (Resolution::set_noverlap)
+  void set_noverlap(unsigned n)
+  {
+    noverlap = n ? n : 1;
+    prepared = false;
+  }
+  unsigned get_noverlap() const { return noverlap; }
+  //! the averaging detectors' channels [start, end) (set_channel_range,
+  //! SpectralKurtosis.C:848-851), and ranges [first, last] excluded from them
+  //! (Resolution::add_exclude)
+  void set_channel_range(unsigned start, unsigned end) { include.emplace_back(start, end - 1); prepared = false; }
+  void add_exclude(unsigned first, unsigned last) { exclude.emplace_back(first, last); prepared = false; }
+  //! Resolution::get_channels (SpectralKurtosis.C:342-360): 1 where included
+  std::vector<uint8_t> get_channels(unsigned nchan) const;
   void set_omit_outliers(bool o) { omit_outliers = o; }
   bool get_omit_outliers() const { return omit_outliers; }
   //! which detectors run (DSPX_SK_SKFB | DSPX_SK_FSCR | DSPX_SK_TSCR)
@@ -255,7 +269,8 @@ class SpectralKurtosis {
   const TimeSeries* input = nullptr;
   TimeSeries* output = nullptr;
   std::unique_ptr<Engine> engine;
-  unsigned M = 128, detectors = 1;
+  unsigned M = 128, detectors = 1, noverlap = 1;
+  std::vector<std::pair<unsigned, unsigned>> include, exclude;
   bool omit_outliers = false, prepared = false;
   std::vector<float> lower, upper, symmetric;
   std::map<uint64_t, std::pair<float, float>> tscr_limits;

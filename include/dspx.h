@@ -401,10 +401,15 @@ int dspx_timediv_block(dspx_timediv* td, const dspx_mjd* input_start, double rat
                        dspx_timediv_bounds* out, uint32_t max, uint32_t* n);
 
 /* ------------------------------------------------------- spectral kurtosis */
-/* One resolution of dsp::SpectralKurtosis (noverlap 1, every channel
-   included).  Sums [npart][nchan][npol][2] float (S1, S2), counts
-   [npart][nchan] uint32, mask [npart][nchan] bytes (1 = zapped), as the
-   reference's sums WeightedTimeSeries and zapmask BitSeries. */
+/* One resolution of dsp::SpectralKurtosis: blocks of M samples starting
+   every M / noverlap samples (Resolution::prepare, SpectralKurtosis.C:
+   263-281), the averaging detectors over the included channels
+   (set_channel_range, :848-851; Resolution::get_channels, :342-360).  Sums
+   [npart][nchan][npol][2] float (S1, S2), counts [npart][nchan] uint32, mask
+   [npart][nchan] bytes (1 = zapped), as the reference's sums
+   WeightedTimeSeries and zapmask BitSeries.  Several resolutions (adaptive
+   SK) are not offered: the reference's Engine path runs the first only
+   (SpectralKurtosis.C:1131-1135). */
 typedef struct dspx_sk dspx_sk;
 
 typedef struct {
@@ -413,7 +418,7 @@ typedef struct {
   uint32_t M;             /* samples per block */
   uint32_t Nd;            /* integrated_Nd: 1 for complex (Analytic) input */
   uint32_t omit_outliers; /* SpectralKurtosis::set_omit_outliers */
-  uint32_t reserved;
+  uint32_t noverlap;      /* SpectralKurtosis::set_noverlap; 0 or 1: blocks do not overlap */
 } dspx_sk_desc;
 
 enum { DSPX_SK_SKFB = 1, DSPX_SK_FSCR = 2, DSPX_SK_TSCR = 4 };
@@ -427,9 +432,20 @@ void dspx_sk_destroy(dspx_sk* sk);
    totals >= 32768 (SKLimits.C:152-160).  Synchronous. */
 int dspx_sk_set_limits(dspx_sk* sk, const float* lower, const float* upper, const float* symmetric,
                        double pc_std_devs);
+/* The blocks of a series of ndat samples (Resolution::prepare,
+   SpectralKurtosis.C:263-281): npart blocks, and the ndat_out samples of the
+   masked output (dspx_sk_mask).  Host only. */
+int dspx_sk_blocks(const dspx_sk* sk, uint64_t ndat, uint64_t* npart, uint64_t* ndat_out);
+/* The channels detect_fscr averages and detect_tscr tests: include[nchan]
+   host bytes, non-zero where included (the include / exclude ranges of
+   Resolution::get_channels resolved per channel); NULL includes every
+   channel.  detect_skfb tests every channel, as the reference's does.
+   Synchronous. */
+int dspx_sk_set_channels(dspx_sk* sk, const uint8_t* include);
 /* Engine::compute (SpectralKurtosis.C:699-703, 674-798) on an FPT complex
-   float TimeSeries: blocks [0, npart) of M samples.  zeroed_data: zero
-   samples are not counted (input->get_zeroed_data()). */
+   float TimeSeries: blocks [0, npart), block b the M samples from b * M /
+   noverlap.  zeroed_data: zero samples are not counted
+   (input->get_zeroed_data()). */
 int dspx_sk_compute(dspx_sk* sk, const float* d_in, uint64_t in_chan_stride, uint64_t in_pol_stride,
                     uint64_t npart, int zeroed_data, float* d_sums, uint32_t* d_counts);
 /* The same from raw TFP 8/16-bit samples (unpacked as dspx_unpack does)
@@ -468,8 +484,11 @@ int dspx_sk_set_tscr_limits(dspx_sk* sk, uint32_t n, const uint64_t* totals, con
    dspx_sk_detect. */
 int dspx_sk_detect_tscr(dspx_sk* sk, const float* d_sums, const uint32_t* d_counts, uint64_t npart,
                         uint8_t* d_mask);
-/* Engine::mask (SpectralKurtosis.C:1518-1554): FPT out = in with block b of
-   channel c zeroed in every pol where mask[b][c]; in may equal out. */
+/* SpectralKurtosis::mask (SpectralKurtosis.C:1465-1566) with the npart
+   blocks of the sums: the output's ndat_out samples (dspx_sk_blocks) are
+   input samples M - M/noverlap on, each zeroed in every pol where the last
+   block written over it is flagged (noverlap 1: out = in with block b of
+   channel c zeroed where mask[b][c]; in may then equal out). */
 int dspx_sk_mask(dspx_sk* sk, const uint8_t* d_mask, const float* d_in, uint64_t in_chan_stride,
                  uint64_t in_pol_stride, float* d_out, uint64_t out_chan_stride, uint64_t out_pol_stride,
                  uint64_t npart);

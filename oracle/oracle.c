@@ -699,22 +699,41 @@ uint64_t ox_fold(const float* in, unsigned nchan, unsigned npol, unsigned ndim,
 /* Spectral kurtosis (SpectralKurtosis.C, CPU path).                          */
 /* ------------------------------------------------------------------------ */
 
-/* SpectralKurtosis.C:674-798 (compute, FPT input, no engine).  Per block of
-   M samples: sqld = re*re + im*im (float), S1 += sqld, S2 += sqld*sqld (float
-   product added to a double accumulator), skipping zero samples when the
-   input carries zeroed data.  The sums are stored as float, TFP
-   [npart][nchan][npol][2], and the count of the first pol per
-   [npart][nchan].  Analytic input [nchan][npol][ndat][2].  Returns npart. */
-uint64_t ox_sk_compute(const float* in, unsigned nchan, unsigned npol, uint64_t ndat,
-                       unsigned M, int zeroed_data, float* sums, uint32_t* counts)
+/* Resolution::prepare (SpectralKurtosis.C:263-281): blocks of M samples
+   every M / noverlap samples; npart of them fit in ndat, and the masked
+   output holds *ndat_out samples. */
+uint64_t ox_sk_blocks(uint64_t ndat, unsigned M, unsigned noverlap, uint64_t* ndat_out)
 {
-  uint64_t npart = ndat / M, ipart, i;
+  uint64_t idat_off = M / noverlap, idat_start = M - idat_off, nblock, idat_end;
+  *ndat_out = 0;
+  if (ndat < idat_start)
+    return 0;
+  nblock = (ndat - idat_start) / idat_off;
+  idat_end = nblock * idat_off;
+  if (idat_end < idat_start)
+    return 0;
+  *ndat_out = idat_end - idat_start;
+  return nblock;
+}
+
+/* SpectralKurtosis.C:674-798 (compute, FPT input, no engine).  Per block of
+   M samples (block ipart from ipart * M / noverlap): sqld = re*re + im*im
+   (float), S1 += sqld, S2 += sqld*sqld (float product added to a double
+   accumulator), skipping zero samples when the input carries zeroed data.
+   The sums are stored as float, TFP [npart][nchan][npol][2], and the count
+   of the first pol per [npart][nchan].  Analytic input
+   [nchan][npol][ndat][2].  Returns npart. */
+uint64_t ox_sk_compute(const float* in, unsigned nchan, unsigned npol, uint64_t ndat,
+                       unsigned M, unsigned noverlap, int zeroed_data, float* sums, uint32_t* counts)
+{
+  uint64_t ndat_out, npart = ox_sk_blocks(ndat, M, noverlap, &ndat_out), ipart, i;
+  const uint64_t off = M / noverlap;
   unsigned ichan, ipol;
   for (ipart = 0; ipart < npart; ipart++)
     for (ichan = 0; ichan < nchan; ichan++)
       for (ipol = 0; ipol < npol; ipol++)
       {
-        const float* x = in + ((uint64_t) ichan * npol + ipol) * ndat * 2 + ipart * M * 2;
+        const float* x = in + ((uint64_t) ichan * npol + ipol) * ndat * 2 + ipart * off * 2;
         double S1_sum = 0, S2_sum = 0;
         unsigned count = 0;
         for (i = 0; i < M; i++)
@@ -806,7 +825,8 @@ static void ox_sk_limits_large(uint64_t M, double pcsd, float* lower, float* upp
 uint64_t ox_sk_detect(float* sums, uint32_t* counts, uint64_t npart, unsigned nchan, unsigned npol,
                       unsigned M, unsigned Nd, int omit_outliers, unsigned flags, const float* lower,
                       const float* upper, const float* symmetric, double pcsd, const uint64_t* tscr_totals,
-                      const float* tscr_lower, const float* tscr_upper, unsigned ntscr, uint8_t* mask)
+                      const float* tscr_lower, const float* tscr_upper, unsigned ntscr, unsigned noverlap,
+                      const uint8_t* channels, uint8_t* mask)
 {
   uint64_t ipart, i, nset = 0;
   unsigned ichan, ipol;
@@ -852,7 +872,7 @@ uint64_t ox_sk_detect(float* sums, uint32_t* counts, uint64_t npart, unsigned nc
       {
         double sk_avg = 0, M_tot = 0, Msq_tot = 0, chan_count = 0;
         for (ichan = 0; ichan < nchan; ichan++)
-          if (!omit_outliers || mask[ipart * nchan + ichan] == 0)
+          if ((!channels || channels[ichan]) && (!omit_outliers || mask[ipart * nchan + ichan] == 0))
           {
             const float* sd = sums + ((ipart * nchan + ichan) * npol + ipol) * 2;
             float S1_sum = sd[0], S2_sum = sd[1];
@@ -890,7 +910,9 @@ uint64_t ox_sk_detect(float* sums, uint32_t* counts, uint64_t npart, unsigned nc
       uint64_t count = 0;
       float lo, up, M_t, M_fac;
       int zap_chan = 0;
-      for (ipart = 0; ipart < npart; ipart++)
+      if (channels && !channels[ichan])
+        continue;
+      for (ipart = 0; ipart < npart; ipart += noverlap)
         count += counts[ipart * nchan + ichan];
       if (count == 0)
         continue;
@@ -919,7 +941,7 @@ uint64_t ox_sk_detect(float* sums, uint32_t* counts, uint64_t npart, unsigned nc
       for (ipol = 0; ipol < npol; ipol++)
       {
         double S1 = 0.0, S2 = 0.0, SK;
-        for (ipart = 0; ipart < npart; ipart++)
+        for (ipart = 0; ipart < npart; ipart += noverlap)
         {
           uint64_t idx = (ipart * nchan + ichan) * npol + ipol;
           S1 += sums[idx * 2];
@@ -938,23 +960,31 @@ uint64_t ox_sk_detect(float* sums, uint32_t* counts, uint64_t npart, unsigned nc
   return nset;
 }
 
-/* SpectralKurtosis::mask (SpectralKurtosis.C:1531-1554), FPT complex input,
-   nskip = 0, noverlap 1: block ipart of channel ichan is zeroed in every pol
-   when mask[ipart][ichan] is set, copied otherwise. */
-void ox_sk_mask(const float* in, unsigned nchan, unsigned npol, uint64_t ndat, unsigned M,
-                const uint8_t* mask, uint64_t npart, float* out)
+/* SpectralKurtosis::mask (SpectralKurtosis.C:1480-1554), one resolution,
+   FPT complex input of ndat samples per row, mask of the npart blocks
+   [npart][nchan]: the blocks written in order, block ipart (of npm) the M
+   samples from ipart * off of the output from input sample idat_start +
+   ipart * off, zeroed in every pol when mask[ipart + nskip][ichan] is set
+   (off = M / noverlap, idat_start = M - off, nskip = idat_start / off, npm =
+   (ndat_out - M) / off + 1).  out rows hold ndat samples (the output's
+   ndat_out first).  Returns ndat_out. */
+uint64_t ox_sk_mask(const float* in, unsigned nchan, unsigned npol, uint64_t ndat, unsigned M, unsigned noverlap,
+                    const uint8_t* mask, uint64_t npart, float* out)
 {
+  const uint64_t off = M / noverlap, idat_start = M - off, nskip = idat_start / off;
+  const uint64_t ndat_out = (npart + 1) * off - M, npm = (ndat_out - M) / off + 1;
   unsigned ichan, ipol;
   uint64_t ipart, j;
   for (ichan = 0; ichan < nchan; ichan++)
     for (ipol = 0; ipol < npol; ipol++)
     {
-      const float* x = in + ((uint64_t) ichan * npol + ipol) * ndat * 2;
+      const float* x = in + ((uint64_t) ichan * npol + ipol) * ndat * 2 + idat_start * 2;
       float* y = out + ((uint64_t) ichan * npol + ipol) * ndat * 2;
-      for (ipart = 0; ipart < npart; ipart++)
+      for (ipart = 0; ipart < npm; ipart++)
         for (j = 0; j < 2 * (uint64_t) M; j++)
-          y[ipart * M * 2 + j] = mask[ipart * nchan + ichan] ? 0.0f : x[ipart * M * 2 + j];
+          y[ipart * off * 2 + j] = mask[(ipart + nskip) * nchan + ichan] ? 0.0f : x[ipart * off * 2 + j];
     }
+  return ndat_out;
 }
 
 /* FilterbankEngineCPU::perform (FilterbankEngineCPU.C:184-370), complex

@@ -85,18 +85,20 @@ def lib():
     L.ox_fold.argtypes = [f32p, u, u, u, U, U, U, d, d, u, f32p, u32p, u, i,
                           C.c_void_p, U, u, u, C.c_void_p]
     L.ox_sk_compute.restype = U
-    L.ox_sk_compute.argtypes = [f32p, u, u, U, u, i, f32p, u32p]
+    L.ox_sk_compute.argtypes = [f32p, u, u, U, u, u, i, f32p, u32p]
+    L.ox_sk_blocks.restype = U
+    L.ox_sk_blocks.argtypes = [U, u, u, C.POINTER(U)]
     L.ox_sk_detect_skfb.restype = U
     L.ox_sk_detect_skfb.argtypes = [f32p, u32p, U, u, u, u, f32p, f32p, u8p]
     L.ox_sk_detect.restype = U
     L.ox_sk_detect.argtypes = [f32p, u32p, U, u, u, u, u, i, u, f32p, f32p, f32p, d, C.c_void_p, C.c_void_p,
-                               C.c_void_p, u, u8p]
+                               C.c_void_p, u, u, C.c_void_p, u8p]
     for name, p in (("ox_filterbank_f32", f32p), ("ox_filterbank_f64", f64p)):
         fn = getattr(L, name)
         fn.restype = U
         fn.argtypes = [p, u, u, U, u, u, u, u, p, p]
-    L.ox_sk_mask.restype = None
-    L.ox_sk_mask.argtypes = [f32p, u, u, U, u, u8p, U, f32p]
+    L.ox_sk_mask.restype = U
+    L.ox_sk_mask.argtypes = [f32p, u, u, U, u, u, u8p, U, f32p]
     _lib = L
     return L
 
@@ -273,16 +275,24 @@ def fold(data, ndim, phi, pps, nbin, idat_start=0, ndat_fold=None, prof=None,
     return prof, hits, nfolded
 
 
-def sk_compute(x, M, zeroed=False):
+def sk_blocks(ndat, M, noverlap=1):
+    """(npart, ndat_out) of Resolution::prepare."""
+    nout = C.c_uint64()
+    npart = lib().ox_sk_blocks(ndat, M, noverlap, C.byref(nout))
+    return npart, nout.value
+
+
+def sk_compute(x, M, zeroed=False, noverlap=1):
     """x: [nchan][npol][ndat] complex64 -> sums [npart][nchan][npol][2] float32,
-    counts [npart][nchan] uint32."""
+    counts [npart][nchan] uint32 (blocks of M every M / noverlap samples)."""
     x = np.ascontiguousarray(x, np.complex64)
     nchan, npol, ndat = x.shape
-    npart = ndat // M
-    sums = np.zeros(npart * nchan * npol * 2, np.float32)
-    counts = np.zeros(npart * nchan, np.uint32)
-    lib().ox_sk_compute(x.view(np.float32).ravel(), nchan, npol, ndat, M, int(zeroed),
+    npart, _ = sk_blocks(ndat, M, noverlap)
+    sums = np.zeros(max(npart, 1) * nchan * npol * 2, np.float32)
+    counts = np.zeros(max(npart, 1) * nchan, np.uint32)
+    lib().ox_sk_compute(x.view(np.float32).ravel(), nchan, npol, ndat, M, noverlap, int(zeroed),
                         sums, counts)
+    sums, counts = sums[:npart * nchan * npol * 2], counts[:npart * nchan]
     return sums.reshape(npart, nchan, npol, 2), counts.reshape(npart, nchan)
 
 
@@ -300,36 +310,39 @@ SKFB, FSCR, TSCR = 1, 2, 4
 
 
 def sk_detect(sums, counts, M, lower, upper, symmetric, pcsd=3.0, flags=SKFB, Nd=1, omit_outliers=False,
-              tscr_limits=None):
+              tscr_limits=None, noverlap=1, channels=None):
     """SpectralKurtosis::detect with one resolution.  tscr_limits: {total:
-    (lower, upper)} for the channel totals above M that detect_tscr tests.
-    Returns the mask [npart][nchan] and the (possibly modified, with
-    omit_outliers) sums and counts."""
+    (lower, upper)} for the channel totals above M that detect_tscr tests;
+    channels: per-channel include flags of the averaging detectors (None:
+    every channel).  Returns the mask [npart][nchan] and the (possibly
+    modified, with omit_outliers) sums and counts."""
     sums = np.array(sums, np.float32, copy=True)
     counts = np.array(counts, np.uint32, copy=True)
     npart, nchan, npol, _ = sums.shape
     mask = np.zeros(npart * nchan, np.uint8)
     lim = tscr_limits or {}
+    ch = None if channels is None else np.ascontiguousarray(channels, np.uint8)
     tt = np.array(sorted(lim), np.uint64)
     tl = np.array([lim[int(t)][0] for t in tt], np.float32)
     tu = np.array([lim[int(t)][1] for t in tt], np.float32)
     n = lib().ox_sk_detect(sums.reshape(-1), counts.reshape(-1), npart, nchan, npol, M, Nd, int(omit_outliers),
                            flags, np.ascontiguousarray(lower, np.float32), np.ascontiguousarray(upper, np.float32),
                            np.ascontiguousarray(symmetric, np.float32), pcsd, tt.ctypes.data, tl.ctypes.data,
-                           tu.ctypes.data, tt.size, mask)
+                           tu.ctypes.data, tt.size, noverlap, ch.ctypes.data if ch is not None else None, mask)
     if n == 2 ** 64 - 1:
         raise ValueError("detect_tscr: a channel total in (M, 32768) has no thresholds")
     return mask.reshape(npart, nchan), sums, counts
 
 
-def sk_mask(x, M, mask):
-    """x: [nchan][npol][ndat] complex64; blocks of M samples zeroed where
-    mask[ipart][ichan]; samples past npart*M are left zero."""
+def sk_mask(x, M, mask, noverlap=1):
+    """x: [nchan][npol][ndat] complex64, mask [npart][nchan] of the sums'
+    blocks -> the masked output, rows of ndat samples of which the first
+    ndat_out (= npart*M with noverlap 1) are written and the rest left zero."""
     x = np.ascontiguousarray(x, np.complex64)
     nchan, npol, ndat = x.shape
     npart = mask.shape[0]
     out = np.zeros_like(x)
-    lib().ox_sk_mask(x.view(np.float32).reshape(-1), nchan, npol, ndat, M,
+    lib().ox_sk_mask(x.view(np.float32).reshape(-1), nchan, npol, ndat, M, noverlap,
                      np.ascontiguousarray(mask, np.uint8).reshape(-1), npart, out.view(np.float32).reshape(-1))
     return out
 

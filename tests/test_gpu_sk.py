@@ -177,3 +177,80 @@ def test_sk_tscr_every_total(dspx, ctx, oracle, tables, flags, omit):
     assert np.array_equal(from_dev(mask), mask_o)
     assert np.array_equal(from_dev(sums), s2_o) and np.array_equal(from_dev(counts).view(np.uint32), c2_o)
     sk.close()
+
+
+@pytest.mark.parametrize("nov", [2, 4])
+def test_sk_noverlap(dspx, ctx, oracle, tables, nov):
+    """Overlapping blocks (noverlap 2 and 4): sums and counts from FPT float
+    and raw 8-bit input, all three detectors (detect_tscr over every
+    noverlap-th block) and the mask's output against the oracle."""
+    import torch
+    M, nchan, npol = 32, 6, 2
+    ndat = 32 * 70 + 9
+    x = sk_rfi_input(nchan, npol, 71, M, seed=30 + nov, tone_chan=2)[:, :, :ndat]
+    x = np.ascontiguousarray(x)
+    sk = dspx.SpectralKurtosis(ctx, nchan, npol, M, noverlap=nov)
+    sk.set_limits(*tables[M], std_devs=3.0)
+    npart, nout = sk.blocks(ndat)
+    assert (npart, nout) == oracle.sk_blocks(ndat, M, nov)
+    sums_o, counts_o = oracle.sk_compute(x, M, noverlap=nov)
+    sums = torch.zeros((npart, nchan, npol, 2), dtype=torch.float32, device="cuda")
+    counts = torch.zeros((npart, nchan), dtype=torch.int32, device="cuda")
+    print("sk noverlap", nov, "compute", flush=True)
+    sk.compute(to_dev(x.view(np.float32)), npol * ndat * 2, ndat * 2, npart, sums, counts)
+    assert np.array_equal(from_dev(counts).view(np.uint32), counts_o)
+    u = _ulps(from_dev(sums), sums_o)
+    assert u.max() <= 1
+    # raw 8-bit TFP input from sample t0
+    rng = np.random.default_rng(nov)
+    t0 = 5
+    raw = rng.integers(-40, 41, (ndat + t0, nchan, npol, 2)).astype(np.int8).view(np.uint8)
+    xr = oracle.unpack8(raw, nchan, npol, ndat + t0).view(np.complex64).reshape(nchan, npol, ndat + t0)
+    sr_o, cr_o = oracle.sk_compute(np.ascontiguousarray(xr[:, :, t0:]), M, noverlap=nov)
+    sk.compute_raw(to_dev(raw), 8, t0, npart, sums, counts)
+    assert np.array_equal(from_dev(counts).view(np.uint32), cr_o)
+    assert _ulps(from_dev(sums), sr_o).max() <= 1
+    # detectors on the oracle's sums (tscr's totals below 32768: SKLimits of each)
+    from importlib import import_module
+    S = import_module("dspsr-kanyixia_amd.sklimits")
+    lower, upper, sym, pcsd = tables[M]
+    tot = {int(t) for t in counts_o[::nov].astype(np.uint64).sum(axis=0)}
+    lim = {t: (S.SKLimits(t, 3.0, 1, 2).lower, S.SKLimits(t, 3.0, 1, 2).upper) for t in tot if M < t < 32768}
+    mask_o, _, _ = oracle.sk_detect(sums_o, counts_o, M, lower, upper, sym, pcsd, flags=7, tscr_limits=lim,
+                                    noverlap=nov)
+    assert mask_o[:, 2].all() and not mask_o.all()
+    mask = torch.zeros((npart, nchan), dtype=torch.uint8, device="cuda")
+    sk.detect(to_dev(sums_o), to_dev(counts_o.view(np.int32)), npart, 7, mask)
+    assert np.array_equal(from_dev(mask), mask_o)
+    # the mask's output: input samples M - M/noverlap on, block by block
+    out_o = oracle.sk_mask(x, M, mask_o, noverlap=nov)
+    out = torch.full((nchan, npol, ndat, 2), 3.0, dtype=torch.float32, device="cuda")
+    sk.mask(mask, to_dev(x.view(np.float32)), npol * ndat * 2, ndat * 2, out, npol * ndat * 2, ndat * 2, npart)
+    got = from_dev(out).view(np.complex64).reshape(nchan, npol, ndat)
+    assert np.array_equal(got[:, :, :nout], out_o[:, :, :nout])
+    sk.close()
+
+
+def test_sk_channel_range(dspx, ctx, oracle, tables):
+    """Excluded channels: out of detect_fscr's average and detect_tscr's
+    tests, still tested by detect_skfb."""
+    import torch
+    M, nchan, nblk = 64, 16, 512
+    lower, upper, sym, pcsd = tables[M]
+    x = sk_rfi_input(nchan, 2, nblk, M, seed=3, tone_chan=5)
+    sums_o, counts_o = oracle.sk_compute(x, M)
+    inc = np.ones(nchan, np.uint8)
+    inc[5] = 0
+    inc[11:14] = 0
+    mask_o, _, _ = oracle.sk_detect(sums_o, counts_o, M, lower, upper, sym, pcsd, flags=7, channels=inc)
+    sk = _sk(dspx, ctx, tables, nchan, M)
+    sk.set_channels(inc)
+    mask = torch.zeros((nblk, nchan), dtype=torch.uint8, device="cuda")
+    print("sk channel range", flush=True)
+    sk.detect(to_dev(sums_o), to_dev(counts_o.view(np.int32)), nblk, 7, mask)
+    assert np.array_equal(from_dev(mask), mask_o)
+    sk.set_channels(None)
+    mask_all, _, _ = oracle.sk_detect(sums_o, counts_o, M, lower, upper, sym, pcsd, flags=7)
+    sk.detect(to_dev(sums_o), to_dev(counts_o.view(np.int32)), nblk, 7, mask)
+    assert np.array_equal(from_dev(mask), mask_all) and not np.array_equal(mask_all, mask_o)
+    sk.close()

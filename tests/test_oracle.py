@@ -463,3 +463,60 @@ def test_filterbank_tone_lands_in_its_subband(oracle):
     assert np.delete(mag, 3, axis=0).max() < 1e-6 * nfft
     m = np.arange(fres)
     assert np.allclose(out[3, 0, :fres], nfft * np.exp(2j * np.pi * k0 * m / fres), rtol=1e-6, atol=1e-4)
+
+
+def test_sk_noverlap_blocks_and_mask(oracle):
+    """Overlapping blocks (Resolution::prepare, SpectralKurtosis.C:263-281):
+    block b of noverlap 4 holds the M samples from b*M/4, so every fourth
+    block is a block of the non-overlapping compute; the mask writes the
+    blocks in order from input sample M - M/4, each block's samples zeroed
+    by its own flag until the next block overwrites them
+    (SpectralKurtosis.C:1480-1554)."""
+    rng = np.random.default_rng(12)
+    M, nov, nchan, npol, ndat = 32, 4, 3, 2, 32 * 20 + 5
+    x = (rng.standard_normal((nchan, npol, ndat)) + 1j * rng.standard_normal((nchan, npol, ndat))).astype(np.complex64)
+    npart, nout = oracle.sk_blocks(ndat, M, nov)
+    off = M // nov
+    assert npart == (ndat - (M - off)) // off and nout == npart * off - (M - off)
+    s4, c4 = oracle.sk_compute(x, M, noverlap=nov)
+    s1, c1 = oracle.sk_compute(x, M)
+    assert s4.shape[0] == npart and np.array_equal(s4[::nov], s1[:len(s4[::nov])])
+    for b in (1, 2, 7, npart - 1):
+        blk = x[:, :, b * off:b * off + M]
+        sq = (blk.real * blk.real + blk.imag * blk.imag).astype(np.float32)
+        assert np.allclose(s4[b, :, :, 0], sq.astype(np.float64).sum(-1), rtol=1e-6)
+    mask = (rng.random((npart, nchan)) < 0.3).astype(np.uint8)
+    y = oracle.sk_mask(x, M, mask, noverlap=nov)
+    npm = (nout - M) // off + 1
+    for t in range(nout):
+        k = min(t // off, npm - 1) + (nov - 1)
+        for c in range(nchan):
+            exp = 0 if mask[k, c] else x[c, :, t + M - off]
+            assert np.array_equal(y[c, :, t], np.broadcast_to(exp, (npol,)).astype(np.complex64))
+    assert (y[:, :, nout:] == 0).all()
+    # noverlap 1 keeps the plain block semantics
+    m1 = (rng.random((s1.shape[0], nchan)) < 0.3).astype(np.uint8)
+    assert np.array_equal(oracle.sk_mask(x, M, m1), oracle.sk_mask(x, M, m1, noverlap=1))
+
+
+def test_sk_channel_range(oracle):
+    """Channels outside the included range take no part in detect_fscr's
+    average and are not tested by detect_tscr (Resolution::get_channels,
+    SpectralKurtosis.C:342-360, 1366, 1032); detect_skfb tests every
+    channel."""
+    from helpers import sk_rfi_input
+    M, nchan, nblk = 64, 16, 512
+    lower, upper, sym, pcsd = _sk_tables(M, counts=[M])
+    x = sk_rfi_input(nchan, 2, nblk, M, seed=3, tone_chan=5)
+    sums, counts = oracle.sk_compute(x, M)
+    inc = np.ones(nchan, np.uint8)
+    inc[5] = 0
+    m2_all, _, _ = oracle.sk_detect(sums, counts, M, lower, upper, sym, pcsd, flags=oracle.FSCR)
+    m2_inc, _, _ = oracle.sk_detect(sums, counts, M, lower, upper, sym, pcsd, flags=oracle.FSCR, channels=inc)
+    # the tone channel pulls the average low in a few percent of blocks; without it almost none
+    assert m2_inc.any(axis=1).sum() < m2_all.any(axis=1).sum()
+    m4, _, _ = oracle.sk_detect(sums, counts, M, lower, upper, sym, pcsd, flags=oracle.TSCR, channels=inc)
+    assert not m4[:, 5].any()
+    m1a, _, _ = oracle.sk_detect(sums, counts, M, lower, upper, sym, pcsd, flags=oracle.SKFB)
+    m1i, _, _ = oracle.sk_detect(sums, counts, M, lower, upper, sym, pcsd, flags=oracle.SKFB, channels=inc)
+    assert np.array_equal(m1a, m1i) and m1a[:, 5].mean() > 0.9
