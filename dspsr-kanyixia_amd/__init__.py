@@ -821,13 +821,16 @@ class Batch:
     """dspx_batch: beams of uneven shape through the fused path, executed in
     order or replayed from a captured HIP graph."""
 
-    def __init__(self, ctx, descs, autotune=False):
+    def __init__(self, ctx, descs, autotune=False, branches=1):
+        """branches: streams the beams are split over (1: all in order on
+        the context's stream)."""
         self.ctx = ctx
         self.h = None
         self._lib = lib()
         arr = (LtfDesc * len(descs))(*descs)
         h = P()
-        ctx.check(lib().dspx_batch_create(ctx.h, arr, len(descs), BATCH_AUTOTUNE if autotune else 0, C.byref(h)))
+        flags = (BATCH_AUTOTUNE if autotune else 0) | (int(branches) << 8)
+        ctx.check(lib().dspx_batch_create(ctx.h, arr, len(descs), flags, C.byref(h)))
         self.h = h
         _register(self, False)
         self.nbeam = len(descs)

@@ -1573,12 +1573,21 @@ struct dspx_batch {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   DevBuf tune_raw, tune_prof, tune_hits;
+  // Branches: beams of branch k > 0 run on a context (and stream) of their
+  // own with scratch of their own; an execute forks from the batch stream
+  // and joins it again (one graph with parallel branches when captured).
+  std::vector<dspx_ctx*> branch_ctx;  // [0] = ctx (not owned)
+  std::vector<std::unique_ptr<LtfScratch>> branch_sc;
+  std::vector<hipEvent_t> branch_ev;  // [0] fork, [k] join of branch k
 
   ~dspx_batch()
   {
     if (exec) (void) hipGraphExecDestroy(exec);
     if (graph) (void) hipGraphDestroy(graph);
     for (auto* L : beams) delete L;
+    for (auto e : branch_ev)
+      if (e) (void) hipEventDestroy(e);
+    for (size_t k = 1; k < branch_ctx.size(); k++) dspx_ctx_destroy(branch_ctx[k]);
   }
 };
 
@@ -1678,11 +1687,16 @@ int dspx_batch_create(dspx_ctx* ctx, const dspx_ltf_desc* beams, uint32_t nbeam,
 {
   return guard(ctx, [&] {
     DSPX_REQUIRE(ctx && beams && out && nbeam, DSPX_E_PARAM, "null argument");
-    DSPX_REQUIRE((flags & ~uint32_t(DSPX_BATCH_AUTOTUNE)) == 0, DSPX_E_PARAM, "flags=%u", flags);
+    DSPX_REQUIRE((flags & ~uint32_t(DSPX_BATCH_AUTOTUNE | DSPX_BATCH_BRANCH_MASK)) == 0, DSPX_E_PARAM, "flags=%u",
+                 flags);
     DSPX_HIP(hipSetDevice(ctx->device));
     std::unique_ptr<dspx_batch> B(new dspx_batch);
     B->hold.reset(ctx);
     B->ctx = ctx;
+    uint32_t nbranch = (flags & DSPX_BATCH_BRANCH_MASK) >> DSPX_BATCH_BRANCH_SHIFT;
+    if (!nbranch) nbranch = 1;
+    nbranch = std::min(nbranch, nbeam);
+    std::vector<dspx_ltf_desc> descs(nbeam);
     for (uint32_t i = 0; i < nbeam; i++) {
       const dspx_ltf_desc& d0 = beams[i];
       DSPX_REQUIRE(d0.form != DSPX_FORM_PLAIN, DSPX_E_PARAM, "beam %u: the batch runs the fused forms", i);
@@ -1691,7 +1705,41 @@ int dspx_batch_create(dspx_ctx* ctx, const dspx_ltf_desc* beams, uint32_t nbeam,
         uint32_t n = (flags & DSPX_BATCH_AUTOTUNE) ? autotune_nfft(B.get(), d0) : 0;
         d = with_geometry(d0, n);
       }
-      B->beams.push_back(ltf_create(ctx, &d, &B->sc));
+      descs[i] = d;
+    }
+    // branches: the main context, then contexts with streams of their own
+    B->branch_ctx.push_back(ctx);
+    B->branch_sc.emplace_back(nullptr);
+    for (uint32_t k = 1; k < nbranch; k++) {
+      dspx_ctx* c = nullptr;
+      int rc = dspx_ctx_create(ctx->device, nullptr, &c);
+      DSPX_REQUIRE(rc == DSPX_OK, rc, "branch context %u", k);
+      B->branch_ctx.push_back(c);
+      B->branch_sc.emplace_back(new LtfScratch);
+    }
+    for (uint32_t k = 0; k <= nbranch && nbranch > 1; k++) {
+      hipEvent_t e = nullptr;
+      DSPX_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+      B->branch_ev.push_back(e);
+    }
+    // beams to branches, longest first to the least loaded (a beam's cost
+    // taken as its window samples times log2 of its length)
+    std::vector<uint32_t> order(nbeam), branch(nbeam, 0);
+    std::vector<double> cost(nbeam), load(nbranch, 0.0);
+    for (uint32_t i = 0; i < nbeam; i++) {
+      order[i] = i;
+      cost[i] = double(descs[i].ndat) * descs[i].nchan * descs[i].npol * std::log2(double(descs[i].nfft));
+    }
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return cost[a] > cost[b]; });
+    for (uint32_t i : order) {
+      const uint32_t k = uint32_t(std::min_element(load.begin(), load.end()) - load.begin());
+      branch[i] = k;
+      load[k] += cost[i];
+    }
+    for (uint32_t i = 0; i < nbeam; i++) {
+      const uint32_t k = branch[i];
+      LtfScratch* sc = k ? B->branch_sc[k].get() : &B->sc;
+      B->beams.push_back(ltf_create(B->branch_ctx[k], &descs[i], sc));
     }
     B->tune_raw.release();
     B->tune_prof.release();
@@ -1704,7 +1752,7 @@ void dspx_batch_destroy(dspx_batch* B)
 {
   if (!B) return;
   (void) hipSetDevice(B->ctx->device);
-  (void) hipStreamSynchronize(B->ctx->stream);
+  for (auto* c : B->branch_ctx) (void) hipStreamSynchronize(c->stream);
   delete B;
 }
 
@@ -1728,20 +1776,52 @@ int dspx_batch_set_fold(dspx_batch* B, uint32_t ibeam, double pps, const dspx_fo
     DSPX_REQUIRE(B && ibeam < B->beams.size(), DSPX_E_PARAM, "beam %u", ibeam);
     DSPX_REQUIRE(!B->exec, DSPX_E_STATE, "fold plans are fixed once the batch is captured");
     int rc = dspx_ltf_set_fold(B->beams[ibeam], pps, chunks, nchunk);
-    DSPX_REQUIRE(rc == DSPX_OK, rc, "%s", ctx->last_error.c_str());
+    DSPX_REQUIRE(rc == DSPX_OK, rc, "%s", B->beams[ibeam]->ctx->last_error.c_str());
     ltf_reserve(B->beams[ibeam]);
   });
 }
+
+}  // extern "C"
+
+namespace {
+
+// Every beam's execute: on the batch stream, or forked into the branches'
+// streams after the work already on the batch stream and joined back.
+int batch_run(dspx_batch* B, const void* const* d_raw, float* const* d_prof, uint32_t* const* d_hits,
+              std::string& err)
+{
+  const size_t nb = B->branch_ctx.size();
+  if (nb > 1) {
+    DSPX_HIP(hipEventRecord(B->branch_ev[0], B->ctx->stream));
+    for (size_t k = 1; k < nb; k++) DSPX_HIP(hipStreamWaitEvent(B->branch_ctx[k]->stream, B->branch_ev[0], 0));
+  }
+  int rc = DSPX_OK;
+  for (size_t i = 0; i < B->beams.size() && rc == DSPX_OK; i++) {
+    rc = dspx_ltf_execute(B->beams[i], d_raw[i], d_prof[i], d_hits ? d_hits[i] : nullptr);
+    if (rc != DSPX_OK) err = "beam " + std::to_string(i) + ": " + B->beams[i]->ctx->last_error;
+  }
+  if (nb > 1) {
+    // joined even after a failure, so that a capture can end
+    for (size_t k = 1; k < nb; k++) {
+      DSPX_HIP(hipEventRecord(B->branch_ev[k], B->branch_ctx[k]->stream));
+      DSPX_HIP(hipStreamWaitEvent(B->ctx->stream, B->branch_ev[k], 0));
+    }
+  }
+  return rc;
+}
+
+}  // namespace
+
+extern "C" {
 
 int dspx_batch_execute(dspx_batch* B, const void* const* d_raw, float* const* d_prof, uint32_t* const* d_hits)
 {
   dspx_ctx* ctx = B ? B->ctx : nullptr;
   return guard(ctx, [&] {
     DSPX_REQUIRE(B && d_raw && d_prof, DSPX_E_PARAM, "null argument");
-    for (size_t i = 0; i < B->beams.size(); i++) {
-      int rc = dspx_ltf_execute(B->beams[i], d_raw[i], d_prof[i], d_hits ? d_hits[i] : nullptr);
-      DSPX_REQUIRE(rc == DSPX_OK, rc, "beam %zu: %s", i, ctx->last_error.c_str());
-    }
+    std::string err;
+    int rc = batch_run(B, d_raw, d_prof, d_hits, err);
+    DSPX_REQUIRE(rc == DSPX_OK, rc, "%s", err.c_str());
   });
 }
 
@@ -1763,14 +1843,10 @@ int dspx_batch_capture(dspx_batch* B, const void* const* d_raw, float* const* d_
       DSPX_HIP(hipGraphDestroy(B->graph));
       B->graph = nullptr;
     }
-    DSPX_HIP(hipStreamSynchronize(ctx->stream));
+    for (auto* c : B->branch_ctx) DSPX_HIP(hipStreamSynchronize(c->stream));
     DSPX_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-    int rc = DSPX_OK;
     std::string err;
-    for (size_t i = 0; i < B->beams.size() && rc == DSPX_OK; i++) {
-      rc = dspx_ltf_execute(B->beams[i], d_raw[i], d_prof[i], d_hits ? d_hits[i] : nullptr);
-      if (rc != DSPX_OK) err = ctx->last_error;
-    }
+    int rc = batch_run(B, d_raw, d_prof, d_hits, err);
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamEndCapture(ctx->stream, &g);
     DSPX_REQUIRE(rc == DSPX_OK, rc, "capture: %s", err.c_str());

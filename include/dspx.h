@@ -575,7 +575,15 @@ int dspx_ltf_stage_times(dspx_ltf* ltf, float* ms, const char** names, int cap, 
    Response::set_optimal_ndat, Response.C:553-588) when autotuned. */
 typedef struct dspx_batch dspx_batch;
 
-enum { DSPX_BATCH_AUTOTUNE = 1 };
+enum {
+  DSPX_BATCH_AUTOTUNE = 1,
+  /* bits 8-11: the number of branches (0 or 1: every beam on the context's
+     stream; k: beams split over k streams, the longest beams first to the
+     least loaded branch, each branch with scratch of its own; executes
+     fork from the context's stream and join it again) */
+  DSPX_BATCH_BRANCH_SHIFT = 8,
+  DSPX_BATCH_BRANCH_MASK = 0xf00
+};
 
 /* beams[i] as for dspx_ltf_create (fused form).  A beam with nfft = 0 gets
    its FFT length and overlap from the reference's rule (raised to 4096, the

@@ -18,11 +18,11 @@ BEAMS = [
 ]
 
 
-def _setup(dspx, ctx, autotune):
+def _setup(dspx, ctx, autotune, branches=1):
     from importlib import import_module
     synth = import_module("dspsr-kanyixia_amd.synth")
     descs = [dspx.ltf_desc(b[0], b[1], b[2], b[3], b[4], b[5], b[6], b[9], state=b[7], ndim=b[8]) for b in BEAMS]
-    B = dspx.Batch(ctx, descs, autotune=autotune)
+    B = dspx.Batch(ctx, descs, autotune=autotune, branches=branches)
     raws, folds = [], []
     for i, (nbit, nchan, npol, ndat, cfreq, bw, dm, state, ndim, nbin, period, nsub) in enumerate(BEAMS):
         raws.append(synth.make_baseband(nchan, npol, ndat, cfreq, bw, dm, period, 0.03, 2.0,
@@ -59,9 +59,11 @@ def _buffers(B):
     return profs, hits
 
 
-@pytest.mark.parametrize("autotune", [False, True])
-def test_batch_matches_oracle(ctx, dspx, oracle, autotune):
-    B, raws, folds = _setup(dspx, ctx, autotune)
+@pytest.mark.parametrize("autotune,branches", [(False, 1), (True, 1), (True, 3)])
+def test_batch_matches_oracle(ctx, dspx, oracle, autotune, branches):
+    """(branches 3: the beams split over three streams with scratch of their
+    own, forked from and joined to the context's stream.)"""
+    B, raws, folds = _setup(dspx, ctx, autotune, branches)
     print("batch beams", B.beams, flush=True)
     for i, b in enumerate(BEAMS):
         ref = max(128, oracle.geometry(b[4], b[5], b[1], b[6])["nfft"])
@@ -78,9 +80,11 @@ def test_batch_matches_oracle(ctx, dspx, oracle, autotune):
     B.close()
 
 
-def test_batch_graph_replays(ctx, dspx, oracle):
+@pytest.mark.parametrize("branches", [1, 2])
+def test_batch_graph_replays(ctx, dspx, oracle, branches):
+    """(branches 2: a graph with two parallel branches of beams.)"""
     import torch
-    B, raws, folds = _setup(dspx, ctx, False)
+    B, raws, folds = _setup(dspx, ctx, False, branches)
     expect = _oracle(oracle, B, raws, folds)
     d_raw = [to_dev(r) for r in raws]
     profs, hits = _buffers(B)
