@@ -25,9 +25,13 @@
  *                        (Signal/Pulsar/dsp/Fold.h:262-345; Fold.C:760-887)
  *   dspx_sk_*            SpectralKurtosis::Engine compute / detect_ft / mask
  *                        (SpectralKurtosis.C:699-703, 1130-1134, 1518-1524)
+ *                        and detect_tscr / get_limits (:989-1103)
+ *   dspx_timediv_*       TimeDivide::set_bounds / set_boundaries
+ *                        (Signal/Pulsar/TimeDivide.C:36-506, Subint.h:253-330)
  *   dspx_ltf_*           the LoadToFold operation chain Unpack -> Convolution
  *                        -> Detection -> Fold (LoadToFold1.C:109-258,
- *                        SingleThread.C:662-769) fused into three kernels.
+ *                        SingleThread.C:662-769) fused into one kernel or
+ *                        three, with SpectralKurtosis excision (:265-377).
  */
 #ifndef DSPX_H
 #define DSPX_H
