@@ -115,7 +115,7 @@ def test_training_on_sk_masks(ctx, dspx):
     nblk = ndat // M
     z = np.load(GOLDEN)
     sk = dspx.SpectralKurtosis(ctx, nchan, 2, M, omit_outliers=True)
-    sk.set_limits(z["lower256"], z["upper256"], z["sym256"], float(z["pcsd256"]))
+    sk.set_limits(z["lower256"], z["upper256"], z["sym256"], float(z["pcsd256"]), std_devs=3.0)
     print("rfinet data", flush=True)
     train = [_realisation(dspx, ctx, sk, 1000 + i, nchan, ndat, M) for i in range(16)]
     test = [_realisation(dspx, ctx, sk, 2000 + i, nchan, ndat, M) for i in range(4)]
