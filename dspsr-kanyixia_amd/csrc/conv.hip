@@ -11,6 +11,7 @@
 #include <memory>
 
 #include "dspx_internal.hpp"
+#include "fb4.hpp"
 #include "fold_impl.hpp"
 #include "onepass.hpp"
 #include "plain.hpp"
@@ -1867,3 +1868,71 @@ int dspx_batch_replay(dspx_batch* B)
 }
 
 }  // extern "C"
+
+namespace dspx {
+
+// The convolving filterbank's four-step form: stage A over the FPT float
+// input (windows in_step = nsub * nkeep apart), the rows' forward transform,
+// response product and sub-band backward transforms (k_stageB_fb), and stage
+// C's backward column transform into the kept samples of each sub-band.
+struct Fb4 {
+  ConvCore core;
+  uint32_t nsub = 0, nkeep = 0, F2 = 0;
+  uint64_t in_step = 0;
+};
+
+Fb4* fb4_create(dspx_ctx* ctx, const dspx_fb_desc& d)
+{
+  const uint64_t n = uint64_t(d.nsub) * d.fres;
+  uint32_t N1 = 0, N2 = 0, m = 0;
+  if (n > (1u << 22) || !split_n(uint32_t(n), N1, N2, m) || m < 13 || d.fres % N1) return nullptr;
+  if (!fb4_rows_ok(N2, d.fres / N1)) return nullptr;
+  std::unique_ptr<Fb4> f(new Fb4);
+  f->core.init(ctx, d.nchan, d.npol, uint32_t(n), 0, 0, DSPX_CHIRP_TABLE, dspx_dedisp_desc{});
+  f->nsub = d.nsub;
+  f->nkeep = d.fres - d.nfilt_pos - d.nfilt_neg;
+  f->F2 = d.fres / N1;
+  f->in_step = uint64_t(d.nsub) * f->nkeep;
+  DSPX_REQUIRE(f->in_step < (1ull << 32), DSPX_E_UNSUPPORTED, "window step");
+  f->core.step = uint32_t(f->in_step);  // stage A's window starts
+  f->core.nfilt_pos = d.nfilt_pos;      // stage C's first kept sample of a sub-band
+  return f.release();
+}
+
+void fb4_destroy(Fb4* f) { delete f; }
+
+void fb4_set_response(Fb4* f, const float* host) { f->core.set_table(host); }
+
+void fb4_perform(Fb4* f, const float* d_in, uint64_t ics, uint64_t ips, uint64_t npart, float* d_out, uint64_t ocs,
+                 uint64_t ops)
+{
+  ConvCore& k = f->core;
+  DSPX_REQUIRE(k.have_table, DSPX_E_STATE, "no response: call dspx_fb_set_response first");
+  // parts per pass: a work buffer of at most 2 GiB
+  const uint64_t per_part = uint64_t(k.nchan) * k.npol * k.N * sizeof(float2);
+  const uint64_t chunk = std::max<uint64_t>(1, std::min<uint64_t>(npart, (2ull << 30) / per_part));
+  for (uint64_t p0 = 0; p0 < npart; p0 += chunk) {
+    const uint32_t np = uint32_t(std::min(chunk, npart - p0));
+    k.reserve_work(np);
+    LoadF32 ld{d_in + 2 * p0 * f->in_step, ics, ips, k.npol};
+    k.stage_a(ld, np);
+    // rows per workgroup as stage B of the convolution: parts grouped so
+    // that >= 2048 workgroups are in flight where the rows alone are fewer
+    const uint64_t rows = uint64_t(k.N1) * k.nchan;
+    uint32_t npg = rows >= 2048 ? 1u : uint32_t(std::min<uint64_t>(np, (2048 + rows - 1) / rows));
+    const uint32_t ppg = (np + npg - 1) / npg;
+    npg = (np + ppg - 1) / ppg;
+    const uint64_t nb = rows * npg;
+    DSPX_REQUIRE(nb < (1ull << 31), DSPX_E_UNSUPPORTED, "grid too large");
+    fb4_stage_b(k.N2, f->F2, unsigned(nb), k.ctx->stream, k.shape(np), k.chirp(), twiddles(k.ctx, int(k.N2)),
+                twiddles(k.ctx, int(f->F2)), ppg, f->nsub);
+    FilterbankArgs fb{d_out + 2 * p0 * f->nkeep, ocs, ops};
+    fb.f2 = f->F2;
+    fb.lf2 = uint32_t(__builtin_ctz(f->F2));
+    fb.nsub = f->nsub;
+    fb.nkeep = f->nkeep;
+    k.stage_c<kStageCFilterbank>(np, fb, FoldArgs{});
+  }
+}
+
+}  // namespace dspx

@@ -191,4 +191,14 @@ struct FoldSegment {
 void fold_segments(double phi, double pps, uint64_t j_begin, uint64_t n, uint32_t isub,
                    std::vector<FoldSegment>& out);
 
+// The convolving filterbank's four-step form (conv.hip): windows of 2^13 to
+// 2^22 points whose sub-bands hold at least 8 * N1 bins; fb4_create returns
+// null for the other shapes.
+struct Fb4;
+Fb4* fb4_create(dspx_ctx* ctx, const dspx_fb_desc& d);
+void fb4_destroy(Fb4* f);
+void fb4_set_response(Fb4* f, const float* host);
+void fb4_perform(Fb4* f, const float* d_in, uint64_t ics, uint64_t ips, uint64_t npart, float* d_out, uint64_t ocs,
+                 uint64_t ops);
+
 }  // namespace dspx

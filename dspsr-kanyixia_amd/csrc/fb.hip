@@ -1,5 +1,7 @@
 // fb.hip -- convolving filterbank engine: one kernel per window for
-// 1024-4096-point windows, the FFT-library form otherwise.
+// 1024-4096-point windows, the four-step form (three kernels, conv.hip) for
+// 2^13-2^22-point windows whose sub-bands hold at least 8*N1 bins, the
+// FFT-library form otherwise.
 //
 // Reference: Filterbank::prepare / resize_output (Filterbank.C:75-200,
 // 405-460) and FilterbankEngineCPU::perform (FilterbankEngineCPU.C:184-370):
@@ -15,6 +17,7 @@
 #include <hipfft/hipfft.h>
 
 #include <algorithm>
+#include <memory>
 #include <string>
 
 #include "dspx_internal.hpp"
@@ -163,9 +166,11 @@ struct dspx_fb {
   bool have_plan = false;
   uint64_t plan_batch = 0;
   uint32_t last_batches = 0;  // library-form batches of the last perform (0: one kernel)
+  Fb4* fb4 = nullptr;         // the four-step form, where the window's shape has one
 
   ~dspx_fb()
   {
+    fb4_destroy(fb4);
     if (have_plan) {
       hipfftDestroy(fwd);
       hipfftDestroy(bwd);
@@ -193,7 +198,9 @@ int dspx_fb_create(dspx_ctx* ctx, const dspx_fb_desc* desc, dspx_fb** out)
     F->nfft = uint64_t(d.nsub) * d.fres;
     F->nkeep = d.fres - d.nfilt_pos - d.nfilt_neg;
     F->in_step = uint64_t(d.nsub) * F->nkeep;  // nsamp_fft - nsamp_overlap (Filterbank.C:154-192)
-    *out = F;
+    std::unique_ptr<dspx_fb> hold(F);
+    F->fb4 = fb4_create(ctx, d);
+    *out = hold.release();
   });
 }
 
@@ -212,6 +219,7 @@ int dspx_fb_set_response(dspx_fb* F, const float* host)
     const size_t bytes = size_t(F->d.nchan) * F->nfft * sizeof(float2);
     F->resp.reserve(bytes);
     DSPX_HIP(hipMemcpy(F->resp.p, host, bytes, hipMemcpyHostToDevice));
+    if (F->fb4) fb4_set_response(F->fb4, host);
     F->have_resp = true;
   });
 }
@@ -243,8 +251,9 @@ int dspx_fb_perform(dspx_fb* F, const float* d_in, uint64_t ics, uint64_t ips, u
     F->last_batches = 0;
 
     // One fused kernel for the window lengths it is built for (1024-4096
-    // points in sub-bands of 32-512), the library form otherwise or where
-    // the environment asks for it (DSPX_FB_FORM=library).
+    // points in sub-bands of 32-512), the four-step form for the longer
+    // windows it is built for, the library form otherwise or where the
+    // environment asks for it (DSPX_FB_FORM=library).
     const char* form = std::getenv("DSPX_FB_FORM");
     const bool library = form && std::string(form) == "library";
 #define FB_FUSED(n, m)                                                                                   \
@@ -260,6 +269,10 @@ int dspx_fb_perform(dspx_fb* F, const float* d_in, uint64_t ics, uint64_t ips, u
     FB_FUSED(2048, 32) FB_FUSED(2048, 64) FB_FUSED(2048, 128) FB_FUSED(2048, 256) FB_FUSED(2048, 512)
     FB_FUSED(4096, 32) FB_FUSED(4096, 64) FB_FUSED(4096, 128) FB_FUSED(4096, 256) FB_FUSED(4096, 512)
 #undef FB_FUSED
+    if (!library && F->fb4) {
+      fb4_perform(F->fb4, d_in, ics, ips, npart, d_out, ocs, ops);
+      return;
+    }
 
     const uint64_t per = F->nfft * sizeof(float2);
     // units per batch: scratch that stays in the 256 MB last-level cache

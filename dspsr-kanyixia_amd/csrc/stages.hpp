@@ -160,6 +160,12 @@ struct FilterbankArgs {
   float* out;            // FPT complex
   uint64_t chan_stride;  // floats
   uint64_t pol_stride;   // floats
+  // the convolving filterbank's four-step form (k_stageB_fb): stage B left
+  // sub-band s's backward transform at row positions s*f2 + ob, so output
+  // sample ob + f2*n1 of sub-band s goes to output channel chan*nsub + s,
+  // kept where it lies in [nfilt_pos, nfilt_pos + nkeep); f2 = 0: the
+  // convolution's discard
+  uint32_t f2 = 0, lf2 = 0, nsub = 1, nkeep = 0;
 };
 
 __device__ __forceinline__ float2 tw_N(const Tw2& t, uint32_t m)
@@ -380,6 +386,101 @@ __global__ __launch_bounds__(RW * N2 / E) void k_stageB(ConvShape s, ChirpArgs c
         for (int r = 0; r < RL; r++)
           row[tid + q * NT + r * (N2 / RL)] = cmulc(v[q * RL + r], HOIST ? w[q * RL + r] : cmul(b, step[r]));
       }
+    }
+  }
+}
+
+// --------------------------------- stage B of the convolving filterbank
+
+// The convolving filterbank (Filterbank.C:75-570, FilterbankEngineCPU.C:
+// 184-370) transforms an N = nsub*fres point window forward, multiplies the
+// response, and transforms each sub-band's fres bins back.  In the four-step
+// split (N = N1*N2, fres = N1*F2) the forward transform leaves bin
+// k1 + N1*k2 at row k1, position k2, and sub-band s owns bins s*fres + m
+// with m = k1 + N1*j (j < F2): the row's segment k2 = s*F2 + j.  Its
+// fres-point backward transform splits the same way,
+//   Y_s[ob + F2*a] = sum_k1 e^{+2 pi i k1 a / N1} conj(W_fres^{k1 ob})
+//                    sum_j Z[k1][s*F2 + j] e^{+2 pi i j ob / F2},
+// so stage B runs each row's NSUB = N2/F2 backward F2-point transforms
+// (batched columns in LDS), multiplies by conj(W_N^{nsub k1 ob}), and stores
+// Y at row position s*F2 + ob; stage C's backward column transform over k1
+// then yields output sample ob + F2*a of sub-band s (FilterbankArgs.f2).
+// The response is the transposed table [nchan][N1][N2] (ConvCore::set_table).
+// grid.x = N1 * npg * nchan, block = N2/16; one row in flight.
+template <int N2, int F2>
+__global__ __launch_bounds__(N2 / 16) void k_stageB_fb(ConvShape s, ChirpArgs c, const float2* __restrict__ twr,
+                                                       const float2* __restrict__ twf, uint32_t ppg, uint32_t nsub)
+{
+  constexpr int E = 16, NT = N2 / E, NSUB = N2 / F2;
+  constexpr int LROW = N2 + N2 / 16, LSEG = F2 * (NSUB + 1);
+  using LAY = RowLayout<N2, NT>;
+  using SEG = ColLayout<F2, NSUB, NSUB + 1, NT>;
+  __shared__ float2 lds[LROW > LSEG ? LROW : LSEG];
+  __shared__ float2 hlds[N2];
+  constexpr int RF = FirstRadix<N2, E, -1>::v;   // forward first pass
+  constexpr int RI = FirstRadix<N2, E, +1>::v;   // forward last pass
+  constexpr int RS0 = FirstRadix<F2, E, +1>::v;  // segment transform, first pass
+  constexpr int RSL = LastRadix<F2, E, +1>::v;   // segment transform, last pass
+  static_assert(E / RI == 1, "row plan must end with a pass of radix E");
+  constexpr int Q = E / RF;
+
+  const int tid = threadIdx.x;
+  const uint32_t npg = (s.npart + ppg - 1) / ppg;
+  uint32_t bid = blockIdx.x;
+  const uint32_t k1 = bid % s.N1;
+  bid /= s.N1;
+  const uint32_t pg = bid % npg;
+  const uint32_t chan = bid / npg;
+  const uint32_t p_begin = pg * ppg;
+  const uint32_t p_end = min(s.npart, p_begin + ppg);
+  const uint64_t plane = uint64_t(s.N1) * N2;
+
+  for (int k2 = tid; k2 < N2; k2 += NT) hlds[k2] = c.table[(uint64_t(chan) * s.N1 + k1) * N2 + k2];
+  // inter-stage twiddles W_N^{n2 k1} of the loaded elements (n2 = tid + q*NT
+  // + r*N2/RF), and the output twiddles conj(W_N^{nsub k1 ob}) of the stored
+  // ones (n2 = tid + e*NT, ob = n2 % F2), both fixed for the workgroup
+  float2 w[E], wo[E];
+#pragma unroll
+  for (int q = 0; q < Q; q++)
+#pragma unroll
+    for (int r = 0; r < RF; r++) w[q * RF + r] = tw_N(s.tw, uint32_t(tid + q * NT + r * (N2 / RF)) * k1);
+#pragma unroll
+  for (int e = 0; e < E; e++) wo[e] = tw_N(s.tw, nsub * k1 * (uint32_t(tid + e * NT) & uint32_t(F2 - 1)));
+  __syncthreads();
+
+  const uint32_t nrow = (p_end - p_begin) * s.npol;
+  for (uint32_t i = 0; i < nrow; i++) {
+    const uint32_t p = p_begin + i / s.npol, pol = i % s.npol;
+    float2* row = s.work + ((uint64_t(chan) * s.npart + p) * s.npol + pol) * plane + uint64_t(k1) * N2;
+    float2 v[E];
+#pragma unroll
+    for (int q = 0; q < Q; q++)
+#pragma unroll
+      for (int r = 0; r < RF; r++) v[q * RF + r] = cmul(row[tid + q * NT + r * (N2 / RF)], w[q * RF + r]);
+    fft_regs<LAY, N2, 1, NT, -1>(lds, v, twr, tid);
+    // bin k2 = tid + r*N2/RI times the response, into the segments' layout
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < RI; r++) {
+      const int k2 = tid + r * (N2 / RI);
+      lds[SEG::idx(k2 % F2, k2 / F2)] = cmul(v[r], hlds[k2]);
+    }
+    __syncthreads();
+    lds_load_pass<SEG, F2, NSUB, NT, RS0>(lds, v, tid);
+    fft_regs<SEG, F2, NSUB, NT, +1>(lds, v, twf, tid);
+    // v[q*RSL + r] = Y_b[j + r*F2/RSL] (g = tid + q*NT, b = g % NSUB, j = g / NSUB)
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < E / RSL; q++) {
+      const int g = tid + q * NT, b = g % NSUB, j = g / NSUB;
+#pragma unroll
+      for (int r = 0; r < RSL; r++) lds[LAY::idx(b * F2 + j + r * (F2 / RSL), 0)] = v[q * RSL + r];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < E; e++) {
+      const int n2 = tid + e * NT;
+      row[n2] = cmulc(lds[LAY::idx(n2, 0)], wo[e]);
     }
   }
 }
@@ -780,10 +881,19 @@ __global__ __launch_bounds__(kColThreads) void k_stageC(ConvShape s, FilterbankA
 #pragma unroll
         for (int r = 0; r < RL; r++) {
           uint32_t n1 = j + r * (N1 / RL);
-          uint32_t t = s.N2 * n1 + c0 + b;
-          if (t < s.nfilt_pos || t >= s.nfilt_pos + s.step) continue;
-          uint64_t idat = uint64_t(p) * s.step + (t - s.nfilt_pos);
-          float* o = fb.out + uint64_t(chan) * fb.chan_stride + 2 * idat;
+          float* o;
+          if (fb.f2) {
+            const uint32_t n2 = c0 + b, sb = n2 >> fb.lf2;
+            const uint32_t t = (n2 & (fb.f2 - 1)) + (n1 << fb.lf2);
+            if (t < s.nfilt_pos || t >= s.nfilt_pos + fb.nkeep) continue;
+            const uint64_t idat = uint64_t(p) * fb.nkeep + (t - s.nfilt_pos);
+            o = fb.out + (uint64_t(chan) * fb.nsub + sb) * fb.chan_stride + 2 * idat;
+          } else {
+            uint32_t t = s.N2 * n1 + c0 + b;
+            if (t < s.nfilt_pos || t >= s.nfilt_pos + s.step) continue;
+            uint64_t idat = uint64_t(p) * s.step + (t - s.nfilt_pos);
+            o = fb.out + uint64_t(chan) * fb.chan_stride + 2 * idat;
+          }
           *reinterpret_cast<float2*>(o) = va[q * RL + r];
           if (s.npol == 2) *reinterpret_cast<float2*>(o + fb.pol_stride) = vb[q * RL + r];
         }

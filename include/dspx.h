@@ -214,11 +214,17 @@ int dspx_conv_perform(dspx_conv* conv, const float* d_in, uint64_t in_chan_strid
 /* --------------------------------------------------------------- filterbank */
 /* Filterbank::Engine (Filterbank.C:245-250, 463-474, 549-561;
    FilterbankEngineCPU.C:184-370): nchan input channels each split into nsub
-   sub-bands of fres frequency bins, convolved with the response on the way.
-   FFT-library form: batched nsub*fres-point forward transforms of the windows
-   at ipart*nsub*nkeep, the response product, batched fres-point backward
-   transforms of each sub-band's slice, the kept samples [nfilt_pos,
-   nfilt_pos + nkeep) (nkeep = fres - nfilt_pos - nfilt_neg). */
+   sub-bands of fres frequency bins, convolved with the response on the way:
+   the nsub*fres-point forward transform of the window at ipart*nsub*nkeep,
+   the response product, the fres-point backward transform of each
+   sub-band's slice, the kept samples [nfilt_pos, nfilt_pos + nkeep) (nkeep =
+   fres - nfilt_pos - nfilt_neg).  Three forms, chosen by window shape:
+   one kernel per window (1024-4096 points, fres 32-512); the four-step form
+   (2^13-2^22 points with fres >= 8*N1, N1 = 16 below 2^16 points and
+   nfft/4096 from there: forward column transform, rows with their sub-band
+   backward transforms, backward column transform into the kept samples);
+   batched FFT-library transforms otherwise (or with the environment
+   DSPX_FB_FORM=library). */
 typedef struct dspx_fb dspx_fb;
 
 typedef struct {
@@ -246,7 +252,7 @@ int dspx_fb_geometry(const dspx_fb* fb, uint64_t ndat, uint64_t* npart, uint64_t
 int dspx_fb_perform(dspx_fb* fb, const float* d_in, uint64_t in_chan_stride, uint64_t in_pol_stride,
                     uint64_t ndat, float* d_out, uint64_t out_chan_stride, uint64_t out_pol_stride);
 /* The number of FFT-library batches the last dspx_fb_perform ran (0 when it
-   ran the one-kernel form); for checking the batch loop. */
+   ran the one-kernel or the four-step form); for checking the batch loop. */
 int dspx_fb_batches(const dspx_fb* fb, uint32_t* nbatch);
 
 /* --------------------------------------------------------------- detection */

@@ -103,3 +103,53 @@ def test_filterbank_window_lengths(ctx, dspx, oracle, monkeypatch, form, nsub, f
     e = rel_rms(got, expect)
     print("rel rms", e, flush=True)
     assert e < 1e-5
+
+
+@pytest.mark.parametrize("nsub,fres,nchan", [(16, 512, 2), (8, 2048, 1), (64, 512, 1), (16, 4096, 2), (8, 32768, 1),
+                                             (128, 2048, 1), (512, 512, 1), (2, 1 << 19, 1), (4, 1 << 20, 1)])
+def test_filterbank_four_step(ctx, dspx, oracle, monkeypatch, nsub, fres, nchan):
+    """Windows of 2^13 to 2^22 points in the four-step form (the rows'
+    sub-band transforms in stage B, the sub-band store in stage C), every row
+    length (512-4096) and segment lengths from 8 to 2048 points, against the
+    oracle and against the library form of the same engine."""
+    monkeypatch.delenv("DSPX_FB_FORM", raising=False)
+    rng = np.random.default_rng(nsub + fres)
+    npol, pos, neg = 2, 21, 17
+    nfft = nsub * fres
+    resp = (np.exp(1j * rng.uniform(0, 2 * np.pi, (nchan, nfft))) / np.sqrt(nfft * fres)).astype(np.complex64)
+    ndat = 3 * nfft + 101
+    x = (rng.standard_normal((nchan, npol, ndat)) + 1j * rng.standard_normal((nchan, npol, ndat))).astype(np.complex64)
+    expect = oracle.filterbank(x, nsub, fres, pos, neg, resp)
+    print("four-step filterbank", nsub, fres, nchan, flush=True)
+    nb = []
+    got = _run(dspx, ctx, x, nsub, fres, pos, neg, resp, nb)
+    assert nb == [0]  # not the library form
+    assert got.shape == expect.shape
+    e = rel_rms(got, expect)
+    monkeypatch.setenv("DSPX_FB_FORM", "library")
+    lib_out = _run(dspx, ctx, x, nsub, fres, pos, neg, resp)
+    e_lib = rel_rms(got, lib_out)
+    print("rel rms vs oracle %.3g, vs library form %.3g" % (e, e_lib), flush=True)
+    assert e < 1e-5 and e_lib < 1e-5
+
+
+def test_dedispersing_filterbank_long_windows(ctx, dspx, oracle):
+    """-F 16:D on one 16 MHz complex channel at DM 1000: the reference rule's
+    sub-band length gives 2^15-bin sub-bands in 2^19-point windows (the
+    four-step form), against the oracle with the dedispersion response."""
+    cfreq, bw, dm, nsub = 1405.0, 16.0, 1000.0, 16
+    g = oracle.geometry(cfreq, bw, nsub, dm)
+    fres, pos, neg = g["nfft"], g["impulse_pos"], g["impulse_neg"]
+    resp = oracle.filterbank_response(cfreq, bw, 1, nsub, dm, fres)
+    rng = np.random.default_rng(5)
+    ndat = 3 * nsub * fres
+    x = (rng.standard_normal((1, 2, ndat)) + 1j * rng.standard_normal((1, 2, ndat))).astype(np.complex64) * 10
+    expect = oracle.filterbank(x, nsub, fres, pos, neg, resp)
+    print("dedispersing filterbank", nsub, fres, pos, neg, flush=True)
+    nb = []
+    got = _run(dspx, ctx, x, nsub, fres, pos, neg, resp, nb)
+    assert nb == [0]
+    assert got.shape == expect.shape
+    e = rel_rms(got, expect)
+    print("rel rms", e, flush=True)
+    assert e < 1e-5
