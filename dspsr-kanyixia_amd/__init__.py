@@ -152,6 +152,13 @@ _SIGS = {
     "dspx_sk_set_tscr_limits": (I32, [P, U32, P, P, P]),
     "dspx_sk_detect_tscr": (I32, [P, P, P, U64, P]),
     "dspx_sk_mask": (I32, [P, P, P, U64, U64, P, U64, U64, U64]),
+    "dspx_sk_add_resolution": (I32, [P, U32, U32, P, P, P, P]),
+    "dspx_sk_clear_resolutions": (I32, [P]),
+    "dspx_sk_blocks_res": (I32, [P, U64, P, P]),
+    "dspx_sk_detect_res": (I32, [P, P, P, U64, U32, P]),
+    "dspx_sk_res_totals": (I32, [P, P]),
+    "dspx_sk_res_tscr": (I32, [P, P]),
+    "dspx_sk_mask_res": (I32, [P, P, P, U64, U64, P, U64, U64, U64]),
     "dspx_batch_create": (I32, [P, C.POINTER(LtfDesc), U32, U32, C.POINTER(P)]),
     "dspx_batch_destroy": (None, [P]),
     "dspx_batch_beam": (I32, [P, U32, C.POINTER(LtfDesc), C.POINTER(U64), C.POINTER(U64)]),
@@ -699,6 +706,59 @@ class SpectralKurtosis:
 
     def mask(self, mask, x, ics, ips, out, ocs, ops, npart):
         self.ctx.check(lib().dspx_sk_mask(self.h, ptr(mask), ptr(x), ics, ips, ptr(out), ocs, ops, npart))
+
+    # several resolutions (adaptive SK): the engine's M is the first
+    def add_resolution(self, M, noverlap, lower, upper, symmetric, include=None, std_devs=None):
+        """A coarser resolution (M a multiple of the previous one's, block
+        offset M/noverlap a multiple of the previous one's) with its
+        thresholds by count 0..M; std_devs: the level detect_res evaluates
+        SKLimits at for the totals above M its time-scrunched test meets (the
+        last resolution's)."""
+        import numpy as np
+        lo, up, sy = (np.ascontiguousarray(a, np.float32) for a in (lower, upper, symmetric))
+        assert lo.size == up.size == sy.size == M + 1
+        inc = None if include is None else np.ascontiguousarray(include, np.uint8)
+        self.ctx.check(lib().dspx_sk_add_resolution(self.h, M, noverlap, lo.ctypes.data, up.ctypes.data,
+                                                    sy.ctypes.data, None if inc is None else inc.ctypes.data))
+        self._res = getattr(self, "_res", []) + [(M, std_devs)]
+        self._tscr = {}
+
+    def clear_resolutions(self):
+        self.ctx.check(lib().dspx_sk_clear_resolutions(self.h))
+        self._res = []
+        self._tscr = {}
+
+    def blocks_res(self, ndat):
+        """(npart of the first resolution, ndat_out of the masked output)."""
+        npart, nout = U64(), U64()
+        _check(lib().dspx_sk_blocks_res(self.h, ndat, C.byref(npart), C.byref(nout)))
+        return npart.value, nout.value
+
+    def detect_res(self, sums, counts, ndat, flags, mask):
+        """Every resolution's detectors over the first resolution's sums of a
+        series of ndat samples; the tscr thresholds of totals above the last
+        M evaluated with SKLimits where needed."""
+        res = getattr(self, "_res", [])
+        sd = res[-1][1] if res else getattr(self, "std_devs", None)
+        Mlast = res[-1][0] if res else self.M
+        if not (flags & 4) or sd is None:
+            self.ctx.check(lib().dspx_sk_detect_res(self.h, ptr(sums), ptr(counts), ndat, flags, ptr(mask)))
+            return
+        import numpy as np
+        self.ctx.check(lib().dspx_sk_detect_res(self.h, ptr(sums), ptr(counts), ndat, flags & 3, ptr(mask)))
+        t = np.zeros(self.nchan, np.uint64)
+        self.ctx.check(lib().dspx_sk_res_totals(self.h, t.ctypes.data))
+        need = sorted({int(x) for x in t if Mlast < x < 32768 and int(x) not in self._tscr})
+        if need:
+            from .sklimits import SKLimits
+            for x in need:
+                L = SKLimits(x, sd, self.Nd, self.npol)
+                self._tscr[x] = (L.lower, L.upper)
+            self.set_tscr_limits(self._tscr)
+        self.ctx.check(lib().dspx_sk_res_tscr(self.h, ptr(mask)))
+
+    def mask_res(self, mask, x, ics, ips, out, ocs, ops, ndat):
+        self.ctx.check(lib().dspx_sk_mask_res(self.h, ptr(mask), ptr(x), ics, ips, ptr(out), ocs, ops, ndat))
 
 
 class LoadToFold:

@@ -1520,7 +1520,6 @@ int dspx_ltf_stream_filterbank(dspx_ltf* L, const void* d_raw, float* d_out)
   dspx_ctx* ctx = L ? L->ctx : nullptr;
   return guard(ctx, [&] {
     DSPX_REQUIRE(L && d_raw && d_out, DSPX_E_PARAM, "null argument");
-    const auto& d = L->d;
     L->nstage = 0;
     L->nstage_open = false;
     stream_call(L, d_raw, [&](auto& ld, uint32_t np) {

@@ -1,7 +1,9 @@
 // sk.hip -- spectral kurtosis engine: block sums, the three detectors of
-// SpectralKurtosis::detect and the mask, for one resolution (blocks of M
-// samples starting every M / noverlap samples; the averaging detectors
-// restricted to the included channels).
+// SpectralKurtosis::detect and the mask (blocks of M samples starting every
+// M / noverlap samples; the averaging detectors restricted to the included
+// channels), for one resolution or several (the coarser ones' sums
+// time-scrunched from the finer ones', their zaps mapped onto the finest
+// resolution's mask rows).
 //
 // Reference: SpectralKurtosis.C:674-798 (compute), 830-909 (detect),
 // 1105-1222 (detect_skfb), 1312-1462 (detect_fscr), 1014-1103 (detect_tscr),
@@ -11,6 +13,8 @@
 // detect_fscr; sequential double sums of detect_tscr.
 #include <algorithm>
 #include <cstring>
+#include <memory>
+#include <vector>
 
 #include "dspx_internal.hpp"
 #include "stages.hpp"
@@ -31,6 +35,24 @@ struct dspx_sk {
   uint32_t nov = 1, off = 0;  // noverlap; blocks start every off = M / noverlap samples
   double pcsd = 0;
   bool have_limits = false;
+  // resolutions after the first (dspx_sk_add_resolution), increasing M
+  struct Res {
+    uint32_t M = 0, nov = 1, off = 0;
+    dspx::DevBuf lim;    // lower[M+1], upper[M+1], symmetric[M+1]
+    dspx::DevBuf chans;  // include flags, if have_chans
+    bool have_chans = false;
+  };
+  std::vector<std::unique_ptr<Res>> res;
+  dspx::DevBuf rsum[2], rcnt[2];  // sums and counts of the coarser resolutions, used in turn
+  dspx::DevBuf rtest, rrow;       // fscr of a coarser resolution: per-block test, per-row zap
+  // the sums dspx_sk_detect_res left for detect_tscr, and the last
+  // resolution's shape and thresholds
+  const float* last_sums = nullptr;
+  const uint32_t* last_counts = nullptr;
+  uint64_t last_npart = 0, last_rows = 0;
+  uint32_t last_M = 0, last_nov = 1;
+  const float* last_lim = nullptr;
+  bool have_last = false;
 };
 
 namespace dspx {
@@ -212,11 +234,19 @@ __global__ void k_fill_u8(uint8_t* p, uint64_t n, uint8_t v)
   if (i < n) p[i] = v;
 }
 
+// Rows of the first resolution's mask that a block of a coarser resolution
+// flags (detect_skfb / detect_fscr, SpectralKurtosis.C:1121-1131, 1355-1364):
+// block b flags rows b * fo + k * fs, k < nflag; (1, 1, 1) for the first.
+struct SkFlagMap {
+  uint64_t fo;
+  uint32_t fs, nflag;
+};
+
 // detect_skfb (SpectralKurtosis.C:1148-1210): one thread per (block, channel).
 // Counts above M (never produced by the compute kernels) are not tested.
 __global__ void k_sk_skfb(float* __restrict__ sums, uint32_t* __restrict__ counts, uint64_t n, uint32_t npol,
                           uint32_t M, uint32_t Nd, int omit, const float* __restrict__ lower, const float* __restrict__ upper,
-                          uint8_t* __restrict__ mask)
+                          uint8_t* __restrict__ mask, uint32_t nchan, SkFlagMap fm)
 {
 #pragma clang fp contract(off)
   uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -239,26 +269,31 @@ __global__ void k_sk_skfb(float* __restrict__ sums, uint32_t* __restrict__ count
       counts[i] = 0;
       for (uint32_t pol = 0; pol < npol; pol++) sd[2 * pol] = sd[2 * pol + 1] = 0.f;
     }
-    mask[i] = 1;
+    const uint64_t b = i / nchan, ch = i % nchan;
+    for (uint32_t k = 0; k < fm.nflag; k++) mask[(b * fm.fo + uint64_t(k) * fm.fs) * nchan + ch] = 1;
   }
 }
 
 // detect_fscr (SpectralKurtosis.C:1369-1451): one workgroup per block; the
 // channel sums of each pol are reduced in a fixed order, over the included
 // channels (chans, or every channel when null; Resolution::get_channels).
+// A coarser resolution's block reads its first mask row (part * fo) and,
+// with test set, records its outcome there instead of zapping (k_sk_fscr_scan
+// then applies the outcomes in block order).
 __global__ __launch_bounds__(kSkThreads) void k_sk_fscr(const float* __restrict__ sums,
                                                        const uint32_t* __restrict__ counts, uint32_t nchan,
                                                        uint32_t npol, uint32_t M, uint32_t Nd, int omit,
                                                        const float* __restrict__ symmetric,
                                                        const uint8_t* __restrict__ chans,
-                                                       uint8_t* __restrict__ mask)
+                                                       uint8_t* __restrict__ mask, uint64_t fo,
+                                                       uint8_t* __restrict__ test)
 {
 #pragma clang fp contract(off)
   __shared__ double ra[kSkThreads], rm[kSkThreads], rq[kSkThreads], rn[kSkThreads];
   __shared__ int zap;
   const int tid = threadIdx.x;
   const uint64_t part = blockIdx.x;
-  uint8_t* mrow = mask + part * nchan;
+  uint8_t* mrow = mask + part * fo * nchan;
   if (tid == 0) zap = 0;
   for (uint32_t pol = 0; pol < npol; pol++) {
     double sk_avg = 0, M_tot = 0, Msq_tot = 0, chan_count = 0;
@@ -302,8 +337,59 @@ __global__ __launch_bounds__(kSkThreads) void k_sk_fscr(const float* __restrict_
     }
     __syncthreads();
   }
+  if (test) {
+    if (tid == 0) test[part] = uint8_t(zap);
+    return;
+  }
   if (zap)
     for (uint32_t ch = tid; ch < nchan; ch += kSkThreads) mrow[ch] = 1;
+}
+
+// detect_fscr of a coarser resolution, in block order: a block whose first
+// row a preceding block's zap has filled has no channel left to average
+// (omit_outliers) and is not zapped itself (SpectralKurtosis.C:1376-1385,
+// 1418-1432); one thread, rows[] (the first resolution's rows, cleared by
+// the caller) marks the zapped rows.  Without omit_outliers every tested
+// block stands.
+__global__ void k_sk_fscr_scan(const uint8_t* __restrict__ test, uint64_t npart, SkFlagMap fm, int omit,
+                               uint8_t* __restrict__ rows)
+{
+  for (uint64_t b = 0; b < npart; b++) {
+    if (!test[b] || (omit && rows[b * fm.fo])) continue;
+    for (uint32_t k = 0; k < fm.nflag; k++) rows[b * fm.fo + uint64_t(k) * fm.fs] = 1;
+  }
+}
+
+// mask[row][ch] |= rows[row]: grid over nrow * nchan bytes
+__global__ void k_sk_rows_or(const uint8_t* __restrict__ rows, uint64_t nrow, uint32_t nchan,
+                             uint8_t* __restrict__ mask)
+{
+  const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < nrow * nchan && rows[i / nchan]) mask[i] = 1;
+}
+
+// tscrunch_sums (SpectralKurtosis.C:917-972) out of place: block b of the
+// coarser resolution sums blocks (b + i) * offset, i < nsum, of the finer, in
+// that order (float sums and integer counts); one thread per (block, chan).
+__global__ void k_sk_tscrunch(const float* __restrict__ in_s, const uint32_t* __restrict__ in_c, uint64_t npart,
+                              uint32_t nchan, uint32_t npol, uint32_t nsum, uint32_t offset,
+                              float* __restrict__ out_s, uint32_t* __restrict__ out_c)
+{
+#pragma clang fp contract(off)
+  const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= npart * nchan) return;
+  const uint64_t b = i / nchan, ch = i % nchan;
+  const uint32_t nf = 2 * npol;
+  float acc[4];
+  uint32_t c = 0;
+  for (uint32_t k = 0; k < nf; k++) acc[k] = 0.f;
+  for (uint32_t j = 0; j < nsum; j++) {
+    const uint64_t src = (b + j) * offset * nchan + ch;
+    for (uint32_t k = 0; k < nf; k++) acc[k] = j ? acc[k] + in_s[src * nf + k] : in_s[src * nf + k];
+    c += in_c[src];
+  }
+  for (uint32_t k = 0; k < nf; k++) out_s[i * nf + k] = acc[k];
+  out_c[i] = c;
 }
 
 // The per-channel totals of the counts (detect_tscr's M_t before its test,
@@ -476,20 +562,22 @@ __global__ void k_sk_status_init(unsigned long long* status)
 
 // detect_tscr over the blocks' sums, OR-ed into the mask; waits for the
 // stream and fails when a channel's total has no thresholds.
-void run_tscr(dspx_sk* s, const float* d_sums, const uint32_t* d_counts, uint64_t npart, uint8_t* d_mask)
+// M, nov, lim: the resolution whose blocks the sums are (the last one).
+void run_tscr(dspx_sk* s, const float* d_sums, const uint32_t* d_counts, uint64_t npart, uint8_t* d_mask,
+              uint32_t M, uint32_t nov, const float* lim)
 {
   const dspx_sk_desc& d = s->d;
   hipStream_t st = s->ctx->stream;
   s->status.reserve(2 * sizeof(unsigned long long));
   auto* status = s->status.as<unsigned long long>();
-  const float* lower = s->lim.as<float>();
-  const float* upper = lower + d.M + 1;
+  const float* lower = lim;
+  const float* upper = lower + M + 1;
   const auto* ttot = s->tscr.as<unsigned long long>();
   const float* tlo = reinterpret_cast<const float*>(ttot + s->ntscr);
   k_sk_status_init<<<1, 1, 0, st>>>(status);
-  k_sk_tscr<<<d.nchan, kSkThreads, 0, st>>>(d_sums, d_counts, npart, d.nchan, d.npol, d.M, d.Nd, lower, upper,
+  k_sk_tscr<<<d.nchan, kSkThreads, 0, st>>>(d_sums, d_counts, npart, d.nchan, d.npol, M, d.Nd, lower, upper,
                                             s->pcsd, ttot, tlo, tlo + s->ntscr, s->ntscr, status,
-                                            s->have_chans ? s->chans.as<uint8_t>() : nullptr, s->nov, d_mask);
+                                            s->have_chans ? s->chans.as<uint8_t>() : nullptr, nov, d_mask);
   DSPX_HIP(hipGetLastError());
   unsigned long long h[2];
   DSPX_HIP(hipMemcpyAsync(h, status, sizeof(h), hipMemcpyDeviceToHost, st));
@@ -617,13 +705,15 @@ int dspx_sk_detect(dspx_sk* s, float* d_sums, uint32_t* d_counts, uint64_t npart
     k_fill_u8<<<unsigned((n + 255) / 256), 256, 0, st>>>(d_mask, n, 0);
     if (flags & DSPX_SK_SKFB)
       k_sk_skfb<<<unsigned((n + 255) / 256), 256, 0, st>>>(d_sums, d_counts, n, d.npol, d.M, d.Nd,
-                                                          d.omit_outliers, lower, upper, d_mask);
+                                                          d.omit_outliers, lower, upper, d_mask, d.nchan,
+                                                          SkFlagMap{1, 1, 1});
     if (flags & DSPX_SK_FSCR)
       k_sk_fscr<<<unsigned(npart), kSkThreads, 0, st>>>(d_sums, d_counts, d.nchan, d.npol, d.M, d.Nd,
                                                         d.omit_outliers, sym,
-                                                        s->have_chans ? s->chans.as<uint8_t>() : nullptr, d_mask);
+                                                        s->have_chans ? s->chans.as<uint8_t>() : nullptr, d_mask,
+                                                        1, nullptr);
     DSPX_HIP(hipGetLastError());
-    if (flags & DSPX_SK_TSCR) run_tscr(s, d_sums, d_counts, npart, d_mask);
+    if (flags & DSPX_SK_TSCR) run_tscr(s, d_sums, d_counts, npart, d_mask, d.M, s->nov, lower);
   });
 }
 
@@ -699,7 +789,7 @@ int dspx_sk_detect_tscr(dspx_sk* s, const float* d_sums, const uint32_t* d_count
     DSPX_REQUIRE(s->have_limits, DSPX_E_STATE, "thresholds not set (dspx_sk_set_limits)");
     DSPX_REQUIRE(npart < (1u << 31), DSPX_E_UNSUPPORTED, "npart=%llu", (unsigned long long) npart);
     if (!npart) return;
-    run_tscr(s, d_sums, d_counts, npart, d_mask);
+    run_tscr(s, d_sums, d_counts, npart, d_mask, s->d.M, s->nov, s->lim.as<float>());
   });
 }
 
@@ -721,6 +811,227 @@ int dspx_sk_mask(dspx_sk* s, const uint8_t* d_mask, const float* d_in, uint64_t 
     const uint64_t n = m.n;
     DSPX_REQUIRE((n + 511) / 512 < (1ull << 31), DSPX_E_UNSUPPORTED, "block too long");
     dim3 grid(unsigned((n + 511) / 512), s->d.nchan * s->d.npol);
+    DSPX_REQUIRE(grid.y <= 65535, DSPX_E_UNSUPPORTED, "nchan*npol=%u", grid.y);
+    k_sk_mask<<<grid, 256, 0, s->ctx->stream>>>(d_mask, d_in, ics, ips, d_out, ocs, ops, m);
+    DSPX_HIP(hipGetLastError());
+  });
+}
+
+
+int dspx_sk_add_resolution(dspx_sk* s, uint32_t M, uint32_t noverlap, const float* lower, const float* upper,
+                           const float* symmetric, const uint8_t* include)
+{
+  return guard(s ? s->ctx : nullptr, [&] {
+    DSPX_REQUIRE(s && lower && upper && symmetric, DSPX_E_PARAM, "null argument");
+    const uint32_t nov = noverlap ? noverlap : 1;
+    DSPX_REQUIRE(M >= 2 && M % nov == 0, DSPX_E_STATE, "noverlap=%u does not divide M=%u", nov, M);
+    const uint32_t pM = s->res.empty() ? s->d.M : s->res.back()->M;
+    const uint32_t poff = s->res.empty() ? s->off : s->res.back()->off;
+    // Resolution::compatible (SpectralKurtosis.C:283-294), in increasing M
+    DSPX_REQUIRE(M > pM && M % pM == 0, DSPX_E_STATE, "M=%u not a larger multiple of the previous M=%u", M, pM);
+    DSPX_REQUIRE((M / nov) % poff == 0, DSPX_E_STATE, "overlap offset %u not divisible by the previous %u",
+                 M / nov, poff);
+    std::unique_ptr<dspx_sk::Res> r(new dspx_sk::Res);
+    r->M = M;
+    r->nov = nov;
+    r->off = M / nov;
+    const size_t n = size_t(M) + 1;
+    std::vector<float> h(3 * n);
+    std::copy(lower, lower + n, h.begin());
+    std::copy(upper, upper + n, h.begin() + n);
+    std::copy(symmetric, symmetric + n, h.begin() + 2 * n);
+    r->lim.reserve(h.size() * sizeof(float));
+    DSPX_HIP(hipMemcpy(r->lim.p, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice));
+    if (include) {
+      r->chans.reserve(s->d.nchan);
+      DSPX_HIP(hipMemcpy(r->chans.p, include, s->d.nchan, hipMemcpyHostToDevice));
+      r->have_chans = true;
+    }
+    s->res.push_back(std::move(r));
+    s->have_last = false;
+  });
+}
+
+int dspx_sk_clear_resolutions(dspx_sk* s)
+{
+  return guard(s ? s->ctx : nullptr, [&] {
+    DSPX_REQUIRE(s, DSPX_E_PARAM, "null argument");
+    DSPX_HIP(hipStreamSynchronize(s->ctx->stream));
+    s->res.clear();
+    s->have_last = false;
+  });
+}
+
+namespace {
+
+// Resolution::prepare (SpectralKurtosis.C:263-281) of blocks of M every off.
+uint64_t sk_nblock(uint64_t ndat, uint64_t M, uint64_t off, uint64_t* ndat_out)
+{
+  const uint64_t idat_start = M - off;
+  *ndat_out = 0;
+  if (ndat < idat_start) return 0;
+  const uint64_t nblock = (ndat - idat_start) / off;
+  if (nblock * off < idat_start) return 0;
+  *ndat_out = nblock * off - idat_start;
+  return nblock;
+}
+
+}  // namespace
+
+int dspx_sk_blocks_res(const dspx_sk* s, uint64_t ndat, uint64_t* npart, uint64_t* ndat_out)
+{
+  if (!s || !npart || !ndat_out) return DSPX_E_PARAM;
+  uint64_t nout0;
+  *npart = sk_nblock(ndat, s->d.M, s->off, &nout0);
+  if (s->res.empty()) {
+    *ndat_out = nout0;
+  } else {
+    const auto& r = *s->res.back();
+    sk_nblock(ndat, r.M, r.off, ndat_out);
+  }
+  return DSPX_OK;
+}
+
+int dspx_sk_detect_res(dspx_sk* s, float* d_sums, uint32_t* d_counts, uint64_t ndat, uint32_t flags, uint8_t* d_mask)
+{
+  return guard(s ? s->ctx : nullptr, [&] {
+    DSPX_REQUIRE(s && d_sums && d_counts && d_mask, DSPX_E_PARAM, "null argument");
+    DSPX_REQUIRE(s->have_limits, DSPX_E_STATE, "thresholds not set (dspx_sk_set_limits)");
+    DSPX_REQUIRE((flags & ~7u) == 0, DSPX_E_PARAM, "flags=%u", flags);
+    const dspx_sk_desc& d = s->d;
+    hipStream_t st = s->ctx->stream;
+    uint64_t nout;
+    const uint64_t np0 = sk_nblock(ndat, d.M, s->off, &nout);
+    DSPX_REQUIRE(np0 < (1u << 31), DSPX_E_UNSUPPORTED, "npart=%llu", (unsigned long long) np0);
+    s->have_last = false;
+    const uint64_t n0 = np0 * d.nchan;
+    if (!n0) return;
+    // the first resolution, as dspx_sk_detect without its tscr
+    k_fill_u8<<<unsigned((n0 + 255) / 256), 256, 0, st>>>(d_mask, n0, 0);
+    const float* lim0 = s->lim.as<float>();
+    if (flags & DSPX_SK_SKFB)
+      k_sk_skfb<<<unsigned((n0 + 255) / 256), 256, 0, st>>>(d_sums, d_counts, n0, d.npol, d.M, d.Nd,
+                                                           d.omit_outliers, lim0, lim0 + d.M + 1, d_mask, d.nchan,
+                                                           SkFlagMap{1, 1, 1});
+    if (flags & DSPX_SK_FSCR)
+      k_sk_fscr<<<unsigned(np0), kSkThreads, 0, st>>>(d_sums, d_counts, d.nchan, d.npol, d.M, d.Nd,
+                                                      d.omit_outliers, lim0 + 2 * (d.M + 1),
+                                                      s->have_chans ? s->chans.as<uint8_t>() : nullptr, d_mask, 1,
+                                                      nullptr);
+    DSPX_HIP(hipGetLastError());
+    const float* cs = d_sums;
+    const uint32_t* cc = d_counts;
+    uint64_t have = np0, np = np0;
+    uint32_t pM = d.M, poff = s->off, M = d.M, nov = s->nov;
+    const float* lim = lim0;
+    const size_t fpd = size_t(d.nchan) * d.npol * 2;
+    for (size_t ir = 0; ir < s->res.size(); ir++) {
+      const auto& r = *s->res[ir];
+      uint64_t nr_out;
+      np = sk_nblock(ndat, r.M, r.off, &nr_out);
+      const uint32_t nsum = r.M / pM, offset = r.off / poff;
+      // tscrunch_sums asserts that the last block it reads exists
+      DSPX_REQUIRE(np == 0 || (np - 1) * offset + uint64_t(nsum - 1) * offset < have, DSPX_E_STATE,
+                   "resolution M=%u noverlap=%u reads block %llu of %llu (tscrunch_sums)", r.M, r.nov,
+                   (unsigned long long) ((np - 1) * offset + uint64_t(nsum - 1) * offset),
+                   (unsigned long long) have);
+      if (!np) break;
+      DevBuf& os = s->rsum[ir & 1];
+      DevBuf& oc = s->rcnt[ir & 1];
+      os.reserve(np * fpd * sizeof(float));
+      oc.reserve(np * d.nchan * sizeof(uint32_t));
+      const uint64_t nn = np * d.nchan;
+      k_sk_tscrunch<<<unsigned((nn + 255) / 256), 256, 0, st>>>(cs, cc, np, d.nchan, d.npol, nsum, offset,
+                                                               os.as<float>(), oc.as<uint32_t>());
+      cs = os.as<float>();
+      cc = oc.as<uint32_t>();
+      const uint32_t nflag = r.M / d.M;
+      const SkFlagMap fm{uint64_t(nflag) * s->nov / r.nov, s->nov, nflag};
+      const float* rl = r.lim.as<float>();
+      if (flags & DSPX_SK_SKFB)
+        k_sk_skfb<<<unsigned((nn + 255) / 256), 256, 0, st>>>(os.as<float>(), oc.as<uint32_t>(), nn, d.npol, r.M,
+                                                             d.Nd, d.omit_outliers, rl, rl + r.M + 1, d_mask,
+                                                             d.nchan, fm);
+      if (flags & DSPX_SK_FSCR) {
+        s->rtest.reserve(np);
+        s->rrow.reserve(np0);
+        k_sk_fscr<<<unsigned(np), kSkThreads, 0, st>>>(cs, cc, d.nchan, d.npol, r.M, d.Nd, d.omit_outliers,
+                                                       rl + 2 * (r.M + 1),
+                                                       r.have_chans ? r.chans.as<uint8_t>() : nullptr, d_mask, fm.fo,
+                                                       s->rtest.as<uint8_t>());
+        k_fill_u8<<<unsigned((np0 + 255) / 256), 256, 0, st>>>(s->rrow.as<uint8_t>(), np0, 0);
+        k_sk_fscr_scan<<<1, 1, 0, st>>>(s->rtest.as<uint8_t>(), np, fm, d.omit_outliers, s->rrow.as<uint8_t>());
+        k_sk_rows_or<<<unsigned((n0 + 255) / 256), 256, 0, st>>>(s->rrow.as<uint8_t>(), np0, d.nchan, d_mask);
+      }
+      DSPX_HIP(hipGetLastError());
+      have = np;
+      pM = r.M;
+      poff = r.off;
+      M = r.M;
+      nov = r.nov;
+      lim = rl;
+    }
+    s->last_sums = cs;
+    s->last_counts = cc;
+    s->last_npart = np;
+    s->last_rows = np0;
+    s->last_M = M;
+    s->last_nov = nov;
+    s->last_lim = lim;
+    s->have_last = true;
+    if ((flags & DSPX_SK_TSCR) && np) run_tscr(s, cs, cc, np, d_mask, M, nov, lim);
+  });
+}
+
+int dspx_sk_res_totals(dspx_sk* s, uint64_t* h_totals)
+{
+  return guard(s ? s->ctx : nullptr, [&] {
+    DSPX_REQUIRE(s && h_totals, DSPX_E_PARAM, "null argument");
+    DSPX_REQUIRE(s->have_last, DSPX_E_STATE, "no dspx_sk_detect_res to follow");
+    const uint32_t nchan = s->d.nchan;
+    hipStream_t st = s->ctx->stream;
+    s->tot.reserve(size_t(nchan) * sizeof(unsigned long long));
+    if (s->last_npart)
+      k_sk_totals<<<nchan, kSkThreads, 0, st>>>(s->last_counts, s->last_npart, nchan, s->last_nov,
+                                                s->tot.as<unsigned long long>());
+    else
+      k_fill_u8<<<unsigned((8 * nchan + 255) / 256), 256, 0, st>>>(s->tot.as<uint8_t>(), 8ull * nchan, 0);
+    DSPX_HIP(hipGetLastError());
+    DSPX_HIP(hipMemcpyAsync(h_totals, s->tot.p, size_t(nchan) * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
+    DSPX_HIP(hipStreamSynchronize(st));
+  });
+}
+
+int dspx_sk_res_tscr(dspx_sk* s, uint8_t* d_mask)
+{
+  return guard(s ? s->ctx : nullptr, [&] {
+    DSPX_REQUIRE(s && d_mask, DSPX_E_PARAM, "null argument");
+    DSPX_REQUIRE(s->have_last, DSPX_E_STATE, "no dspx_sk_detect_res to follow");
+    if (!s->last_npart) return;
+    run_tscr(s, s->last_sums, s->last_counts, s->last_npart, d_mask, s->last_M, s->last_nov, s->last_lim);
+  });
+}
+
+int dspx_sk_mask_res(dspx_sk* s, const uint8_t* d_mask, const float* d_in, uint64_t ics, uint64_t ips,
+                     float* d_out, uint64_t ocs, uint64_t ops, uint64_t ndat)
+{
+  return guard(s ? s->ctx : nullptr, [&] {
+    DSPX_REQUIRE(s && d_mask && d_in && d_out, DSPX_E_PARAM, "null argument");
+    const uint64_t M = s->d.M, off = s->off;
+    const uint64_t Ml = s->res.empty() ? M : s->res.back()->M;
+    const uint64_t offl = s->res.empty() ? off : s->res.back()->off;
+    uint64_t nout0, nout;
+    const uint64_t np0 = sk_nblock(ndat, M, off, &nout0);
+    sk_nblock(ndat, Ml, offl, &nout);
+    if (nout < M) return;  // no whole block of the first resolution
+    // SpectralKurtosis::mask (:1480-1510): nskip rows skipped, npm blocks
+    SkMaskShape m{s->d.nchan, s->d.npol, uint32_t(off), uint32_t((Ml - offl) / off), 0, Ml - offl, nout};
+    DSPX_REQUIRE((Ml - offl) % off == 0 && (nout - M) % off == 0, DSPX_E_STATE, "mask blocks not aligned");
+    m.npm = (nout - M) / off + 1;
+    DSPX_REQUIRE(m.npm + m.nskip <= np0, DSPX_E_INTERNAL, "mask blocks");
+    DSPX_REQUIRE(d_in != d_out || (Ml == offl && M == off), DSPX_E_PARAM, "in place with overlapping blocks");
+    DSPX_REQUIRE((nout + 511) / 512 < (1ull << 31), DSPX_E_UNSUPPORTED, "block too long");
+    dim3 grid(unsigned((nout + 511) / 512), s->d.nchan * s->d.npol);
     DSPX_REQUIRE(grid.y <= 65535, DSPX_E_UNSUPPORTED, "nchan*npol=%u", grid.y);
     k_sk_mask<<<grid, 256, 0, s->ctx->stream>>>(d_mask, d_in, ics, ips, d_out, ocs, ops, m);
     DSPX_HIP(hipGetLastError());

@@ -417,9 +417,10 @@ int dspx_timediv_block(dspx_timediv* td, const dspx_mjd* input_start, double rat
    (set_channel_range, :848-851; Resolution::get_channels, :342-360).  Sums
    [npart][nchan][npol][2] float (S1, S2), counts [npart][nchan] uint32, mask
    [npart][nchan] bytes (1 = zapped), as the reference's sums
-   WeightedTimeSeries and zapmask BitSeries.  Several resolutions (adaptive
-   SK) are not offered: the reference's Engine path runs the first only
-   (SpectralKurtosis.C:1131-1135). */
+   WeightedTimeSeries and zapmask BitSeries.  Coarser resolutions (adaptive
+   SK, the reference's CPU path; its Engine path runs the first only,
+   SpectralKurtosis.C:1131-1135) are added with dspx_sk_add_resolution and
+   run by the _res entry points below. */
 typedef struct dspx_sk dspx_sk;
 
 typedef struct {
@@ -502,6 +503,54 @@ int dspx_sk_detect_tscr(dspx_sk* sk, const float* d_sums, const uint32_t* d_coun
 int dspx_sk_mask(dspx_sk* sk, const uint8_t* d_mask, const float* d_in, uint64_t in_chan_stride,
                  uint64_t in_pol_stride, float* d_out, uint64_t out_chan_stride, uint64_t out_pol_stride,
                  uint64_t npart);
+
+/* Several resolutions (SpectralKurtosis::Resolution, SpectralKurtosis.C:
+   101-211, 251-294, 360-380).  The engine's own M and noverlap are the first
+   (finest); each added resolution has M a multiple of the previous one's
+   and a block offset M/noverlap a multiple of the previous one's
+   (Resolution::compatible), added in increasing M, with its thresholds by
+   count 0..M (as dspx_sk_set_limits) and its included channels (include
+   [nchan] host bytes, NULL: every channel).  Synchronous. */
+int dspx_sk_add_resolution(dspx_sk* sk, uint32_t M, uint32_t noverlap, const float* lower, const float* upper,
+                           const float* symmetric, const uint8_t* include);
+/* Removes the added resolutions. */
+int dspx_sk_clear_resolutions(dspx_sk* sk);
+/* The blocks of a series of ndat samples with the resolutions: npart of the
+   first (the mask's rows) and the ndat_out samples of the masked output
+   (those of the last resolution, SpectralKurtosis::reserve, :454-480).  Host
+   only. */
+int dspx_sk_blocks_res(const dspx_sk* sk, uint64_t ndat, uint64_t* npart, uint64_t* ndat_out);
+/* SpectralKurtosis::detect with every resolution (:849-905) over the first
+   resolution's sums of a series of ndat samples (dspx_sk_compute over
+   dspx_sk_blocks_res's npart blocks): for each resolution in turn, the sums
+   time-scrunched from the previous one (tscrunch_sums, :917-972: block b of
+   the coarser is the sum of blocks (b + i) * offset, i < nsum, of the finer,
+   nsum the ratio of their M and offset that of their block offsets; the
+   reference asserts that the last block read exists, DSPX_E_STATE here),
+   then detect_skfb and detect_fscr as flagged, their zaps flagging the
+   first resolution's rows that the block spans (:1121-1131, 1355-1364);
+   finally detect_tscr (DSPX_SK_TSCR) over the last resolution's blocks and
+   thresholds with the first resolution's channels, flagging the mask's first
+   rows, as many as the last resolution has blocks (:1087-1093).  The
+   coarser sums are the engine's own: d_sums and d_counts keep the first
+   resolution's (zeroed where omit_outliers zaps).  With DSPX_SK_TSCR the
+   call waits for the stream; DSPX_E_STATE as dspx_sk_detect. */
+int dspx_sk_detect_res(dspx_sk* sk, float* d_sums, uint32_t* d_counts, uint64_t ndat, uint32_t flags,
+                       uint8_t* d_mask);
+/* After a dspx_sk_detect_res without DSPX_SK_TSCR: the per-channel totals
+   detect_tscr will test (h_totals[nchan], host), and detect_tscr itself
+   OR-ed into d_mask (waits for the stream; DSPX_E_STATE as dspx_sk_detect).
+   They read the last resolution's sums that call left. */
+int dspx_sk_res_totals(dspx_sk* sk, uint64_t* h_totals);
+int dspx_sk_res_tscr(dspx_sk* sk, uint8_t* d_mask);
+/* SpectralKurtosis::mask with the resolutions (:1465-1566): the output's
+   ndat_out samples (dspx_sk_blocks_res) are input samples M_last -
+   M_last/noverlap_last on, written in blocks of the first resolution's M
+   every M/noverlap samples from mask row (M_last - M_last/noverlap_last) /
+   (M/noverlap), each block zeroed where its row is flagged. */
+int dspx_sk_mask_res(dspx_sk* sk, const uint8_t* d_mask, const float* d_in, uint64_t in_chan_stride,
+                     uint64_t in_pol_stride, float* d_out, uint64_t out_chan_stride, uint64_t out_pol_stride,
+                     uint64_t ndat);
 
 /* ------------------------------------------------------ fused LoadToFold */
 

@@ -960,6 +960,254 @@ uint64_t ox_sk_detect(float* sums, uint32_t* counts, uint64_t npart, unsigned nc
   return nset;
 }
 
+/* SpectralKurtosis::detect with several resolutions (SpectralKurtosis.C:
+   360-380 prepare/compatible, 849-905 detect, 917-972 tscrunch_sums, 1105-1222
+   detect_skfb, 1312-1462 detect_fscr, 1014-1103 detect_tscr), sorted by M:
+   resolution r has blocks of Ms[r] samples every off_r = Ms[r] / novs[r]
+   samples, npart_r of them in ndat samples (Resolution::prepare), thresholds
+   tables[r] = lower, upper, symmetric (Ms[r]+1 each, consecutive; the tables
+   of the resolutions follow one another), and include flags channels[r][nchan].
+   The sums and counts of resolution 0 (npart_0 blocks) are tscrunched in
+   place from resolution r-1 to r before its tests:
+     out[ipart] = sum_{isum < nsum} in[(ipart + isum) * offset],
+     nsum = Ms[r] / Ms[r-1], offset = off_r / off_{r-1}
+   (the reference's stride for isum is the blocks' offset; it asserts that
+   the last block read lies inside the input, an error here).  The mask has
+   resolution 0's rows; the tests of resolution r > 0 flag rows ipart *
+   flag_offset + iflag * flag_step (iflag < nflag; nflag = Ms[r] / Ms[0],
+   flag_step = novs[0], flag_offset = nflag * flag_step / novs[r]), and its
+   fscr with omit_outliers reads row ipart * flag_offset.  detect_tscr takes
+   the last resolution's sums (every novs[last]-th block of npart_last), its
+   tables and tscr_* for larger totals, resolution 0's channels, and flags the
+   first npart_last rows of the mask.  Returns the number of mask bytes set;
+   UINT64_MAX for a tscr total without thresholds, UINT64_MAX - 1 for
+   resolutions that are not compatible. */
+uint64_t ox_sk_detect_res(float* sums, uint32_t* counts, uint64_t ndat, unsigned nchan, unsigned npol,
+                          unsigned nres, const unsigned* Ms, const unsigned* novs, const float* tables,
+                          const uint8_t* channels, unsigned Nd, int omit_outliers, unsigned flags, double pcsd,
+                          const uint64_t* tscr_totals, const float* tscr_lower, const float* tscr_upper,
+                          unsigned ntscr, uint8_t* mask)
+{
+  uint64_t npart[16], off[16], nd_out, i, nset = 0, have;
+  const float* tab[16];
+  unsigned r, ichan, ipol;
+  const uint64_t fpd = (uint64_t) nchan * npol * 2;
+  if (nres < 1 || nres > 16)
+    return UINT64_MAX - 1;
+  tab[0] = tables;
+  for (r = 0; r < nres; r++)
+  {
+    if (Ms[r] % novs[r])
+      return UINT64_MAX - 1;
+    off[r] = Ms[r] / novs[r];
+    npart[r] = ox_sk_blocks(ndat, Ms[r], novs[r], &nd_out);
+    if (r > 0)
+    {
+      if (Ms[r] % Ms[r - 1] || off[r] % off[r - 1])
+        return UINT64_MAX - 1;
+      tab[r] = tab[r - 1] + 3 * (uint64_t) (Ms[r - 1] + 1);
+    }
+  }
+  memset(mask, 0, npart[0] * nchan);
+  have = npart[0];
+  for (r = 0; r < nres; r++)
+  {
+    const float *lower = tab[r], *upper = tab[r] + Ms[r] + 1, *symmetric = tab[r] + 2 * (Ms[r] + 1);
+    const uint8_t* chans = channels + (uint64_t) r * nchan;
+    uint64_t nflag = 1, flag_step = 1, flag_offset = 1, ipart, iflag;
+    if (r > 0)
+    {
+      /* tscrunch_sums (SpectralKurtosis.C:917-972), in place, in order */
+      const uint64_t nsum = Ms[r] / Ms[r - 1], offset = off[r] / off[r - 1];
+      uint64_t isum, k;
+      if (npart[r] && (npart[r] - 1) * offset + (nsum - 1) * offset >= have)
+        return UINT64_MAX - 1;
+      for (ipart = 0; ipart < npart[r]; ipart++)
+      {
+        for (k = 0; k < fpd; k++)
+          sums[ipart * fpd + k] = sums[ipart * offset * fpd + k];
+        for (k = 0; k < nchan; k++)
+          counts[ipart * nchan + k] = counts[ipart * offset * nchan + k];
+        for (isum = 1; isum < nsum; isum++)
+        {
+          for (k = 0; k < fpd; k++)
+            sums[ipart * fpd + k] += sums[(ipart + isum) * offset * fpd + k];
+          for (k = 0; k < nchan; k++)
+            counts[ipart * nchan + k] += counts[(ipart + isum) * offset * nchan + k];
+        }
+      }
+      have = npart[r];
+      nflag = Ms[r] / Ms[0];
+      flag_step = novs[0];
+      flag_offset = nflag * flag_step / novs[r];
+    }
+    if (flags & 1)
+      for (ipart = 0; ipart < npart[r]; ipart++)
+        for (ichan = 0; ichan < nchan; ichan++)
+        {
+          uint32_t* wt = counts + ipart * nchan + ichan;
+          float* sd = sums + (ipart * nchan + ichan) * npol * 2;
+          unsigned Mprime = *wt;
+          int zap = 1;
+          if (Mprime == 0)
+            continue;
+          if (Mprime > 2)
+          {
+            const float M_fac = (float) (Mprime * Nd + 1) / (Mprime - 1);
+            zap = 0;
+            for (ipol = 0; ipol < npol; ipol++)
+            {
+              float S1_sum = sd[2 * ipol], S2_sum = sd[2 * ipol + 1];
+              float V = M_fac * (Mprime * (S2_sum / (S1_sum * S1_sum)) - 1);
+              if (V > upper[Mprime] || V < lower[Mprime])
+                zap = 1;
+            }
+          }
+          if (zap)
+          {
+            if (omit_outliers)
+            {
+              *wt = 0;
+              for (ipol = 0; ipol < npol; ipol++)
+                sd[2 * ipol] = sd[2 * ipol + 1] = 0.0f;
+            }
+            for (iflag = 0; iflag < nflag; iflag++)
+              mask[(ipart * flag_offset + iflag * flag_step) * nchan + ichan] = 1;
+          }
+        }
+    if (flags & 2)
+      for (ipart = 0; ipart < npart[r]; ipart++)
+      {
+        const uint8_t* row = mask + ipart * flag_offset * nchan;
+        int zap_ipart = 0;
+        for (ipol = 0; ipol < npol; ipol++)
+        {
+          double sk_avg = 0, M_tot = 0, Msq_tot = 0, chan_count = 0;
+          for (ichan = 0; ichan < nchan; ichan++)
+            if (chans[ichan] && (!omit_outliers || row[ichan] == 0))
+            {
+              const float* sd = sums + ((ipart * nchan + ichan) * npol + ipol) * 2;
+              float S1_sum = sd[0], S2_sum = sd[1];
+              unsigned Mprime = counts[ipart * nchan + ichan];
+              const float M_fac = (float) (Mprime * Nd + 1) / (Mprime - 1);
+              float V = M_fac * (Mprime * (S2_sum / (S1_sum * S1_sum)) - 1);
+              sk_avg += V * Mprime;
+              M_tot += Mprime;
+              Msq_tot += Mprime * Mprime;
+              chan_count++;
+            }
+          if (chan_count > 0)
+          {
+            double M_mean, Msq_mean, Deff, effective_count;
+            float threshold, up, lo;
+            sk_avg /= M_tot;
+            M_mean = M_tot / chan_count;
+            Msq_mean = Msq_tot / chan_count;
+            Deff = Msq_mean / (M_mean * M_mean);
+            effective_count = chan_count / Deff;
+            threshold = symmetric[(unsigned) (M_mean + 0.5)] / sqrt(effective_count);
+            up = 1.0 + threshold;
+            lo = 1.0 - threshold;
+            if (sk_avg > up || sk_avg < lo)
+              zap_ipart = 1;
+          }
+        }
+        if (zap_ipart)
+          for (iflag = 0; iflag < nflag; iflag++)
+            memset(mask + (ipart * flag_offset + iflag * flag_step) * nchan, 1, nchan);
+      }
+  }
+  if (flags & 4)
+  {
+    const unsigned L = nres - 1, M = Ms[L], nov = novs[L];
+    const float *lower = tab[L], *upper = tab[L] + M + 1;
+    const uint64_t np = npart[L];
+    uint64_t ipart;
+    for (ichan = 0; ichan < nchan; ichan++)
+    {
+      uint64_t count = 0;
+      float lo, up, M_t, M_fac;
+      int zap_chan = 0;
+      if (!channels[ichan])
+        continue;
+      for (ipart = 0; ipart < np; ipart += nov)
+        count += counts[ipart * nchan + ichan];
+      if (count == 0)
+        continue;
+      if (count <= M)
+      {
+        lo = lower[count];
+        up = upper[count];
+      }
+      else
+      {
+        unsigned k;
+        for (k = 0; k < ntscr && tscr_totals[k] != count; k++)
+          ;
+        if (k < ntscr)
+        {
+          lo = tscr_lower[k];
+          up = tscr_upper[k];
+        }
+        else if (count >= 32768)
+          ox_sk_limits_large(count, pcsd, &lo, &up);
+        else
+          return UINT64_MAX;
+      }
+      M_t = (float) count;
+      M_fac = (M_t * Nd + 1) / (M_t - 1);
+      for (ipol = 0; ipol < npol; ipol++)
+      {
+        double S1 = 0.0, S2 = 0.0, SK;
+        for (ipart = 0; ipart < np; ipart += nov)
+        {
+          uint64_t idx = (ipart * nchan + ichan) * npol + ipol;
+          S1 += sums[idx * 2];
+          S2 += sums[idx * 2 + 1];
+        }
+        SK = M_fac * (M_t * (S2 / (S1 * S1)) - 1);
+        if (SK > up || SK < lo)
+          zap_chan = 1;
+      }
+      if (zap_chan)
+        for (ipart = 0; ipart < np; ipart++)
+          mask[ipart * nchan + ichan] = 1;
+    }
+  }
+  for (i = 0; i < npart[0] * nchan; i++)
+    nset += mask[i];
+  return nset;
+}
+
+/* SpectralKurtosis::mask (SpectralKurtosis.C:1480-1554) with several
+   resolutions: blocks of the finest (M0 every off0 samples), the output
+   starting idat_start = M_last - off_last samples in and holding ndat_out =
+   the last resolution's output samples; nskip = idat_start / off0 mask rows
+   are skipped and npm = (ndat_out - M0) / off0 + 1 blocks written in order,
+   block ipart zeroed in every pol where mask[ipart + nskip][ichan] is set.
+   out rows hold ndat samples.  Returns ndat_out. */
+uint64_t ox_sk_mask_res(const float* in, unsigned nchan, unsigned npol, uint64_t ndat, unsigned M0, unsigned nov0,
+                        unsigned Ml, unsigned novl, const uint8_t* mask, float* out)
+{
+  const uint64_t off = M0 / nov0, offl = Ml / novl, idat_start = Ml - offl, nskip = idat_start / off;
+  uint64_t ndat_out, npm, ipart, j;
+  unsigned ichan, ipol;
+  ox_sk_blocks(ndat, Ml, novl, &ndat_out);
+  if (ndat_out < M0)
+    return 0;
+  npm = (ndat_out - M0) / off + 1;
+  for (ichan = 0; ichan < nchan; ichan++)
+    for (ipol = 0; ipol < npol; ipol++)
+    {
+      const float* x = in + ((uint64_t) ichan * npol + ipol) * ndat * 2 + idat_start * 2;
+      float* y = out + ((uint64_t) ichan * npol + ipol) * ndat * 2;
+      for (ipart = 0; ipart < npm; ipart++)
+        for (j = 0; j < 2 * (uint64_t) M0; j++)
+          y[ipart * off * 2 + j] = mask[(ipart + nskip) * nchan + ichan] ? 0.0f : x[ipart * off * 2 + j];
+    }
+  return ndat_out;
+}
+
 /* SpectralKurtosis::mask (SpectralKurtosis.C:1480-1554), one resolution,
    FPT complex input of ndat samples per row, mask of the npart blocks
    [npart][nchan]: the blocks written in order, block ipart (of npm) the M

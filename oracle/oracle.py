@@ -99,6 +99,11 @@ def lib():
         fn.argtypes = [p, u, u, U, u, u, u, u, p, p]
     L.ox_sk_mask.restype = U
     L.ox_sk_mask.argtypes = [f32p, u, u, U, u, u, u8p, U, f32p]
+    L.ox_sk_detect_res.restype = U
+    L.ox_sk_detect_res.argtypes = [f32p, u32p, U, u, u, u, u32p, u32p, f32p, u8p, u, i, u, d, C.c_void_p,
+                                   C.c_void_p, C.c_void_p, u, u8p]
+    L.ox_sk_mask_res.restype = U
+    L.ox_sk_mask_res.argtypes = [f32p, u, u, U, u, u, u, u, u8p, f32p]
     _lib = L
     return L
 
@@ -345,6 +350,50 @@ def sk_mask(x, M, mask, noverlap=1):
     lib().ox_sk_mask(x.view(np.float32).reshape(-1), nchan, npol, ndat, M, noverlap,
                      np.ascontiguousarray(mask, np.uint8).reshape(-1), npart, out.view(np.float32).reshape(-1))
     return out
+
+
+def sk_detect_res(sums, counts, ndat, resolutions, flags=SKFB, Nd=1, omit_outliers=False, pcsd=3.0,
+                  tscr_limits=None):
+    """SpectralKurtosis::detect with several resolutions.  sums/counts: the
+    finest resolution's [npart_0][nchan][npol][2] and [npart_0][nchan];
+    resolutions: [(M, noverlap, lower, upper, symmetric, channels or None)]
+    sorted by M (tables of M+1 by count, channels the include flags).
+    Returns the mask [npart_0][nchan] and the sums and counts after the
+    in-place tscrunches (the coarsest resolution's in the first rows)."""
+    sums = np.array(sums, np.float32, copy=True)
+    counts = np.array(counts, np.uint32, copy=True)
+    npart, nchan, npol, _ = sums.shape
+    Ms = np.array([r[0] for r in resolutions], np.uint32)
+    novs = np.array([r[1] for r in resolutions], np.uint32)
+    tables = np.concatenate([np.concatenate([np.asarray(r[2], np.float32), np.asarray(r[3], np.float32),
+                                             np.asarray(r[4], np.float32)]) for r in resolutions])
+    chans = np.concatenate([np.ones(nchan, np.uint8) if r[5] is None else np.asarray(r[5], np.uint8)
+                            for r in resolutions])
+    lim = tscr_limits or {}
+    tt = np.array(sorted(lim), np.uint64)
+    tl = np.array([lim[int(t)][0] for t in tt], np.float32)
+    tu = np.array([lim[int(t)][1] for t in tt], np.float32)
+    mask = np.zeros(npart * nchan, np.uint8)
+    n = lib().ox_sk_detect_res(sums.reshape(-1), counts.reshape(-1), ndat, nchan, npol, len(resolutions), Ms, novs,
+                               tables, chans, Nd, int(omit_outliers), flags, pcsd, tt.ctypes.data, tl.ctypes.data,
+                               tu.ctypes.data, tt.size, mask)
+    if n == 2 ** 64 - 1:
+        raise ValueError("detect_tscr: a channel total in (M, 32768) has no thresholds")
+    if n == 2 ** 64 - 2:
+        raise ValueError("resolutions not compatible")
+    return mask.reshape(npart, nchan), sums, counts
+
+
+def sk_mask_res(x, M0, nov0, Ml, novl, mask):
+    """SpectralKurtosis::mask with several resolutions (finest M0, noverlap
+    nov0; coarsest Ml, novl) -> the masked output, rows of ndat samples of
+    which the first ndat_out are written."""
+    x = np.ascontiguousarray(x, np.complex64)
+    nchan, npol, ndat = x.shape
+    out = np.zeros_like(x)
+    n = lib().ox_sk_mask_res(x.view(np.float32).reshape(-1), nchan, npol, ndat, M0, nov0, Ml, novl,
+                             np.ascontiguousarray(mask, np.uint8).reshape(-1), out.view(np.float32).reshape(-1))
+    return out, int(n)
 
 
 def filterbank(x, nsub, fres, nfilt_pos, nfilt_neg, resp, f64=False):

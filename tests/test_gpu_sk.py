@@ -254,3 +254,83 @@ def test_sk_channel_range(dspx, ctx, oracle, tables):
     sk.detect(to_dev(sums_o), to_dev(counts_o.view(np.int32)), nblk, 7, mask)
     assert np.array_equal(from_dev(mask), mask_all) and not np.array_equal(mask_all, mask_o)
     sk.close()
+
+
+RES_SPECS = {"64-256-1024": [(64, 1), (256, 4), (1024, 16)], "128-512": [(128, 2), (512, 8)]}
+
+
+@pytest.fixture(scope="module")
+def rtables():
+    z = np.load(GOLDEN)
+    return {M: (z["rlower%d" % M], z["rupper%d" % M], z["rsym%d" % M]) for M in (64, 128, 256, 512, 1024)}
+
+
+@pytest.mark.parametrize("omit", [False, True])
+@pytest.mark.parametrize("flags", [1, 2, 3, 7])
+@pytest.mark.parametrize("spec", list(RES_SPECS))
+def test_sk_resolutions(dspx, ctx, oracle, rtables, spec, flags, omit):
+    """Adaptive SK: the detectors of every resolution over the finest
+    resolution's sums (the oracle's, so that the comparison is of the
+    detection), masks bit-exact with the oracle's; a tone, a burst and a
+    channel whose power steps every 256 samples (seen only by the coarser
+    blocks); the last resolution's time-scrunched test with SKLimits of the
+    totals it meets; channel ranges on the 64-256-1024 case."""
+    import torch
+    res = RES_SPECS[spec]
+    M0, nov0 = res[0]
+    nchan, nblk = 24, 96
+    x = sk_rfi_input(nchan, 2, nblk * 64 // M0, M0, seed=31, tone_chan=5, impulse_blk=3)
+    n = x.shape[2]
+    x[11] *= np.where((np.arange(n) // 256) % 2 == 0, 1.0, 2.5).astype(np.float32)
+    sums_o, counts_o = oracle.sk_compute(x, M0, noverlap=nov0)
+    np0 = sums_o.shape[0]
+    include = None
+    if spec == "64-256-1024":
+        include = np.ones(nchan, np.uint8)
+        include[18:] = 0
+    sk = dspx.SpectralKurtosis(ctx, nchan, 2, M0, omit_outliers=omit, noverlap=nov0)
+    lo, up, sy = rtables[M0]
+    z = np.load(GOLDEN)
+    sk.set_limits(lo, up, sy, float(z["pcsd1024"]), std_devs=3.0)
+    if include is not None:
+        sk.set_channels(include)
+    for M, nov in res[1:]:
+        sk.add_resolution(M, nov, *rtables[M], include=include, std_devs=3.0)
+    assert sk.blocks_res(n)[0] == np0
+    sums, counts = to_dev(sums_o), to_dev(counts_o.view(np.int32))
+    mask = torch.zeros((np0, nchan), dtype=torch.uint8, device="cuda")
+    print("sk resolutions", spec, flags, omit, flush=True)
+    sk.detect_res(sums, counts, n, flags, mask)
+    tl = dict(sk._tscr)
+    m_o, s_o, c_o = oracle.sk_detect_res(sums_o, counts_o, n,
+                                         [(M, nov) + rtables[M] + (include,) for M, nov in res],
+                                         flags=flags, omit_outliers=omit, pcsd=float(z["pcsd1024"]),
+                                         tscr_limits=tl)
+    got = from_dev(mask)
+    print("rows flagged %d, oracle %d" % (got.sum(), m_o.sum()), flush=True)
+    assert np.array_equal(got, m_o)
+    assert 0 < m_o.mean() < 1
+    # the finest resolution's sums and counts as the oracle leaves its first rows
+    if len(res) == 1:
+        assert np.array_equal(from_dev(counts).view(np.uint32), c_o)
+    sk.close()
+
+
+def test_sk_mask_with_resolutions(dspx, ctx, oracle, rtables):
+    import torch
+    M0, nchan, nblk = 64, 6, 50
+    x = sk_rfi_input(nchan, 2, nblk, M0, seed=5)
+    n = x.shape[2]
+    rng = np.random.default_rng(2)
+    mask_h = (rng.random((nblk, nchan)) < 0.25).astype(np.uint8)
+    sk = dspx.SpectralKurtosis(ctx, nchan, 2, M0)
+    sk.set_limits(*rtables[64], 3.0)
+    sk.add_resolution(256, 4, *rtables[256])
+    np0, nout = sk.blocks_res(n)
+    expect, nout_o = oracle.sk_mask_res(x, 64, 1, 256, 4, mask_h)
+    assert nout == nout_o
+    out = torch.zeros((nchan, 2, n, 2), dtype=torch.float32, device="cuda")
+    sk.mask_res(to_dev(mask_h), to_dev(x.view(np.float32)), 2 * n * 2, n * 2, out, 2 * n * 2, n * 2, n)
+    got = from_dev(out).view(np.complex64).reshape(nchan, 2, n)
+    assert np.array_equal(got[:, :, :nout], expect[:, :, :nout])
+    sk.close()
