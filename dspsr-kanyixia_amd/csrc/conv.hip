@@ -1618,19 +1618,16 @@ dspx_ltf_desc with_geometry(const dspx_ltf_desc& d0, uint32_t nfft)
   return d;
 }
 
-// Seconds per output sample of the fused LoadToFold at FFT length nfft, from
-// three timed executions on a zero-filled block (the kernels' run time does
-// not depend on the sample values) of at least four parts and about 2^20
-// samples per channel where the beam has them, so that the short transforms'
-// grids fill the device as they do on the whole block.
+// Seconds per output sample of the fused LoadToFold at FFT length nfft: the
+// fastest of three timed executions of the beam's whole block, zero-filled
+// (the kernels' run time does not depend on the sample values), so that
+// grids, tails and cache footprints are those of the replay.  (Timing a
+// 2^20-sample slice instead let lengths within a few per cent of each other
+// trade places from run to run.)
 double time_per_sample(dspx_batch* B, const dspx_ltf_desc& dn)
 {
   dspx_ctx* ctx = B->ctx;
-  const uint32_t nov = dn.nfilt_pos + dn.nfilt_neg;
   dspx_ltf_desc d = dn;
-  const uint64_t step = d.nfft - nov;
-  const uint64_t nparts = std::max<uint64_t>(4, ((1u << 20) + step - 1) / step);
-  d.ndat = std::min<uint64_t>(dn.ndat, nov + nparts * step);
   if (d.ndat < d.nfft) return 1e30;
   std::unique_ptr<dspx_ltf> L(ltf_create(ctx, &d, &B->sc));
   if (!L->npart) return 1e30;
@@ -1649,16 +1646,22 @@ double time_per_sample(dspx_batch* B, const dspx_ltf_desc& dn)
     DSPX_REQUIRE(rc == DSPX_OK, rc, "autotune execute: %s", ctx->last_error.c_str());
   };
   run();
-  DSPX_HIP(hipEventRecord(ctx->ev[0], ctx->stream));
-  for (int i = 0; i < 3; i++) run();
-  DSPX_HIP(hipEventRecord(ctx->ev[1], ctx->stream));
-  DSPX_HIP(hipEventSynchronize(ctx->ev[1]));
-  float ms = 0;
-  DSPX_HIP(hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]));
-  return ms * 1e-3 / 3.0 / double(L->ndat_out);
+  for (int i = 0; i < 3; i++) {
+    DSPX_HIP(hipEventRecord(ctx->ev[i], ctx->stream));
+    run();
+  }
+  DSPX_HIP(hipEventRecord(ctx->ev[3], ctx->stream));
+  DSPX_HIP(hipEventSynchronize(ctx->ev[3]));
+  float best = 1e30f;
+  for (int i = 0; i < 3; i++) {
+    float ms = 0;
+    DSPX_HIP(hipEventElapsedTime(&ms, ctx->ev[i], ctx->ev[i + 1]));
+    best = std::min(best, ms);
+  }
+  return double(best) * 1e-3 / double(L->ndat_out);
 }
 
-// Candidate lengths: the reference's choice and the next two powers of two
+// Candidate lengths: the reference's choice and the next three powers of two
 // (longer transforms waste less on the overlap, shorter ones keep the
 // transforms in their fastest shapes), within the fused forms' range; each
 // runs in the form the plan picks for it (one pass where it fits).
@@ -1667,7 +1670,7 @@ uint32_t autotune_nfft(dspx_batch* B, const dspx_ltf_desc& d0)
   dspx_ltf_desc ref = with_geometry(d0, 0);
   uint32_t best = ref.nfft;
   double best_t = 1e30;
-  for (uint32_t n = ref.nfft; n <= (1u << 22) && n <= 4 * ref.nfft; n *= 2) {
+  for (uint32_t n = ref.nfft; n <= (1u << 22) && n <= 8 * ref.nfft; n *= 2) {
     dspx_ltf_desc d = with_geometry(d0, n);
     if (d.nfilt_pos + d.nfilt_neg >= n || d.ndat < n) continue;
     double t = time_per_sample(B, d);

@@ -334,11 +334,16 @@ __device__ __forceinline__ void fft_regs(float2* lds, float2* v, const float2* _
 
 // Returns p unchanged, opaquely: loads through the result cannot be hoisted
 // out of the loop that calls this.  Keeps per-pass twiddle loads inside the
-// loop over rows instead of holding every pass's twiddles in registers.
+// loop over rows instead of holding every pass's twiddles in registers.  p
+// must point to global memory (the twiddle tables): the value passes through
+// the asm as a global-address-space pointer, so that the loads stay global
+// loads (through a generic pointer they would be flat loads, which also
+// count against the LDS wait counter and hold up the LDS exchanges).
 template <class T> __device__ __forceinline__ const T* opaque(const T* p)
 {
-  asm volatile("" : "+s"(p));
-  return p;
+  auto q = (const __attribute__((address_space(1))) T*) p;
+  asm volatile("" : "+s"(q));
+  return (const T*) q;
 }
 
 // Returns x unchanged, opaquely (a vector-register value): arithmetic on the

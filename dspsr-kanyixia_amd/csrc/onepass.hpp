@@ -112,10 +112,25 @@ __global__ __launch_bounds__(OnePass<N>::NT) void k_onepass(LD ld, OnePassArgs a
   // workgroup barriers); a slot past the group's last window transforms its
   // group's first window again and neither stores nor folds it
   const uint32_t niter = (w_end - w_begin + WF - 1) / WF;
+  // the raw words of a window (both pols packed), in the forward plan's
+  // first-pass order; the next window's are loaded while this one is
+  // transformed
+  using Word = typename LD::Word;
+  auto load_words = [&](uint32_t it_, Word* wd) {
+    const uint32_t w_ = w_begin + it_ * WF + uint32_t(slot);
+    const uint64_t t_ = uint64_t(w_ < w_end ? w_ : w_begin) * a.step;
+#pragma unroll
+    for (int q = 0; q < E / R0; q++)
+#pragma unroll
+      for (int r = 0; r < R0; r++) wd[q * R0 + r] = ld.word(chan, t_ + uint32_t(tid0 + q * NTW + r * (N / R0)));
+  };
+  Word wcur[E];
+  if (niter) load_words(0, wcur);
   for (uint32_t it = 0; it < niter; it++) {
     const uint32_t w = w_begin + it * WF + uint32_t(slot);
     const bool valid = w < w_end;
-    const uint64_t t0 = uint64_t(valid ? w : w_begin) * a.step;
+    Word wnext[E];
+    if (it + 1 < niter) load_words(it + 1, wnext);
     // the lane's index arithmetic (LDS addresses of every pass) is redone
     // per window rather than held in registers across the loop
 #ifndef DSPX_OP_REINDEX
@@ -128,10 +143,7 @@ __global__ __launch_bounds__(OnePass<N>::NT) void k_onepass(LD ld, OnePassArgs a
     float2 y0[E], v[E];
     for (uint32_t pol = 0; pol < a.npol; pol++) {
 #pragma unroll
-      for (int q = 0; q < E / R0; q++)
-#pragma unroll
-        for (int r = 0; r < R0; r++)
-          v[q * R0 + r] = ld.pol(ld.word(chan, t0 + uint32_t(tid + q * NTW + r * (N / R0))), pol);
+      for (int e = 0; e < E; e++) v[e] = ld.pol(wcur[e], pol);
       // opaque: the pass twiddles are reloaded per window rather than held
       // in registers across the loop (registers go to occupancy)
       fft_regs<LAY, N, 1, NTW, -1>(lds, v, opaque(tw), tid);
@@ -191,6 +203,9 @@ __global__ __launch_bounds__(OnePass<N>::NT) void k_onepass(LD ld, OnePassArgs a
         fold_lane<NV, 4, ZH>(hist, fa, a.npol, kq, y0 + g * 4, b, chan, i0);
       }
     }
+    if (it + 1 < niter)
+#pragma unroll
+      for (int e = 0; e < E; e++) wcur[e] = wnext[e];
   }
   if constexpr (MODE == kStageCFold) {
     __syncthreads();
