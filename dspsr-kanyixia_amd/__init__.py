@@ -52,7 +52,7 @@ class Geometry(C.Structure):
 class ConvDesc(C.Structure):
     _fields_ = [("dedisp", DedispDesc), ("nchan", C.c_uint32), ("npol", C.c_uint32),
                 ("nfft", C.c_uint32), ("nfilt_pos", C.c_uint32), ("nfilt_neg", C.c_uint32),
-                ("chirp_mode", C.c_uint32), ("form", C.c_uint32), ("reserved", C.c_uint32)]
+                ("chirp_mode", C.c_uint32), ("form", C.c_uint32), ("nyquist", C.c_uint32)]
 
 
 class FoldChunk(C.Structure):
@@ -479,12 +479,14 @@ class Convolution:
 
     def __init__(self, ctx, nchan, npol, nfft, nfilt_pos, nfilt_neg, chirp_mode=CHIRP_PARITY,
                  form=FORM_FUSED, centre_frequency=0.0, bandwidth=0.0, dm=0.0, dc_centred=False,
-                 interchan=False):
+                 interchan=False, nyquist=False):
+        """nyquist: real-sampled input (ndat real samples per chan/pol; the
+        complex output at half the rate)."""
         self.ctx = ctx
         self.h = None
         self._lib = lib()
         d = ConvDesc(_dedisp(centre_frequency, bandwidth, dm, nchan, dc_centred, interchan), nchan, npol, nfft,
-                     nfilt_pos, nfilt_neg, chirp_mode, form, 0)
+                     nfilt_pos, nfilt_neg, chirp_mode, form, int(nyquist))
         h = P()
         ctx.check(lib().dspx_conv_create(ctx.h, C.byref(d), C.byref(h)))
         self.h = h

@@ -64,6 +64,9 @@ struct ConvCore {
   // form is not used) and whether it is selected
   float in_max = 0.f;
   bool row_mfma = false;
+  // real-sampled input (k_stageB_real): W_2N^{k1 + N1*k2} at [k1][k2]
+  bool nyquist = false;
+  DevBuf splitw;
 
   void init(dspx_ctx* c, uint32_t nchan_, uint32_t npol_, uint32_t N_, uint32_t pos, uint32_t neg,
             uint32_t mode, const dspx_dedisp_desc& d, bool one = false)
@@ -188,7 +191,23 @@ struct ConvCore {
     c.cf = chan_par.as<double>() ? chan_par.as<double>() + nchan : nullptr;
     c.fdelay = chan_par.as<double>() ? chan_par.as<double>() + 2 * nchan : nullptr;
     c.table = table.as<float2>();
+    c.nyquist = nyquist ? 1u : 0u;
     return c;
+  }
+
+  // Real-sampled input: the split twiddles of k_stageB_real.
+  void set_nyquist()
+  {
+    DSPX_REQUIRE(!onepass, DSPX_E_UNSUPPORTED, "real-sampled input in the one-pass form");
+    nyquist = true;
+    std::vector<float2> t(static_cast<size_t>(N));
+    for (uint32_t k1 = 0; k1 < N1; k1++)
+      for (uint32_t k2 = 0; k2 < N2; k2++) {
+        const double a = -M_PI * double(k1 + uint64_t(N1) * k2) / double(N);
+        t[size_t(k1) * N2 + k2] = make_float2(float(std::cos(a)), float(std::sin(a)));
+      }
+    splitw.reserve(t.size() * sizeof(float2));
+    DSPX_HIP(hipMemcpy(splitw.p, t.data(), t.size() * sizeof(float2), hipMemcpyHostToDevice));
   }
 
   uint32_t ncb() const { return N2 / (kColTile / N1); }
@@ -225,6 +244,20 @@ struct ConvCore {
     uint64_t nb = rows * npg;
     DSPX_REQUIRE(nb < (1ull << 31), DSPX_E_UNSUPPORTED, "grid too large");
     const float2* twr = twiddles(ctx, N2);
+    if (nyquist) {
+      // row pairs (k1, N1 - k1), rows 0 and N1/2 alone
+      const uint64_t nbr = uint64_t(N1 / 2 + 1) * nchan * npg;
+      DSPX_REQUIRE(nbr < (1ull << 31), DSPX_E_UNSUPPORTED, "grid too large");
+      switch (N2) {
+#define CASE_BR(n) \
+  case n: k_stageB_real<n><<<unsigned(nbr), n / 16, 0, ctx->stream>>>(s, c, twr, splitw.as<float2>(), ppg); break;
+        CASE_BR(256) CASE_BR(512) CASE_BR(1024) CASE_BR(2048) CASE_BR(4096)
+#undef CASE_BR
+        default: fail(DSPX_E_UNSUPPORTED, "N2=%u", N2);
+      }
+      DSPX_HIP(hipGetLastError());
+      return;
+    }
     if (row_mfma && N2 == 4096 && in_max > 0.f) {
       // scale on load so that no complex magnitude exceeds 46341 (its FP16
       // components then stay below 65504 through every pass): stage A's
@@ -741,12 +774,16 @@ int dspx_conv_create(dspx_ctx* ctx, const dspx_conv_desc* d, dspx_conv** out)
     c->ctx = ctx;
     c->d = *d;
     try {
-      if (d->form == DSPX_FORM_PLAIN)
+      DSPX_REQUIRE(d->nyquist <= 1, DSPX_E_PARAM, "nyquist=%u", d->nyquist);
+      if (d->form == DSPX_FORM_PLAIN) {
+        DSPX_REQUIRE(!d->nyquist, DSPX_E_UNSUPPORTED, "real-sampled input in the plain form");
         c->plain.init(ctx, d->nchan, d->npol, d->nfft, d->nfilt_pos, d->nfilt_neg, d->chirp_mode, d->dedisp);
-      else if (d->form == DSPX_FORM_FUSED || d->form == DSPX_FORM_TWOLEVEL)
+      } else if (d->form == DSPX_FORM_FUSED || d->form == DSPX_FORM_TWOLEVEL) {
         c->core.init(ctx, d->nchan, d->npol, d->nfft, d->nfilt_pos, d->nfilt_neg, d->chirp_mode, d->dedisp);
-      else
+        if (d->nyquist) c->core.set_nyquist();
+      } else {
         fail(DSPX_E_PARAM, "form=%u", d->form);
+      }
     } catch (...) {
       delete c;
       throw;
@@ -782,7 +819,10 @@ int dspx_conv_perform(dspx_conv* c, const float* d_in, uint64_t ics, uint64_t ip
   return guard(ctx, [&] {
     DSPX_REQUIRE(c && d_in && d_out, DSPX_E_PARAM, "null argument");
     const uint32_t N = c->d.nfft, nov = c->d.nfilt_pos + c->d.nfilt_neg, step = N - nov;
-    uint64_t max_part = ndat < N ? 0 : (ndat - nov) / step;
+    // real-sampled input: parts of 2N real samples every 2*step
+    uint64_t max_part = c->d.nyquist ? (ndat < 2ull * N ? 0 : (ndat - 2ull * nov) / (2ull * step))
+                                     : (ndat < N ? 0 : (ndat - nov) / step);
+    DSPX_REQUIRE(!(c->d.nyquist && d_zdm), DSPX_E_UNSUPPORTED, "zero-DM copy of real-sampled input");
     DSPX_REQUIRE(npart <= max_part, DSPX_E_PARAM, "npart=%u > %llu parts available in ndat=%llu", npart,
                  (unsigned long long) max_part, (unsigned long long) ndat);
     if (npart == 0) return;

@@ -130,6 +130,9 @@ struct ChirpArgs {
   const double* cf;    // per channel centre frequency (MHz)
   const double* fdelay; // per channel fractional inter-channel delay (us), 0 without -K
   const float2* table; // DSPX_CHIRP_TABLE: [nchan][N1][N2] (transposed bins)
+  // real-sampled input: bins in frequency order (no dual-sideband swap) and
+  // the first channel's DC zero (Response.C:335-345, 418-420)
+  uint32_t nyquist = 0;
 };
 
 // The SpectralKurtosis zap mask a fused fold applies (LoadToFold1.C:
@@ -238,7 +241,8 @@ __global__ __launch_bounds__(kColThreads) void k_stageA(LD ld, ConvShape s, cons
 __device__ __forceinline__ float2 chirp_onchip(const ChirpArgs& c, uint32_t chan, uint32_t k, uint32_t N)
 {
 #pragma clang fp contract(off)
-  uint32_t ipt = (k + (N >> 1)) & (N - 1);
+  if (c.nyquist && chan == 0 && k == 0) return make_float2(0.f, 0.f);
+  uint32_t ipt = c.nyquist ? k : (k + (N >> 1)) & (N - 1);
   double freq = double(ipt) * c.binwidth - c.half_cw;
   double cf = c.cf[chan];
   // coeff*sqr(freq)/(chan_freq+freq) + delay_phase, delay_phase = -2 pi freq delay
@@ -482,6 +486,140 @@ __global__ __launch_bounds__(N2 / 16) void k_stageB_fb(ConvShape s, ChirpArgs c,
       const int n2 = tid + e * NT;
       row[n2] = cmulc(lds[LAY::idx(n2, 0)], wo[e]);
     }
+  }
+}
+
+// ---------------------------------- stage B of real-sampled (Nyquist) input
+
+// Real-sampled input (Convolution.C:385-413, 700-790): a window of 2N real
+// samples x is taken as N complex samples z[n] = x[2n] + i x[2n+1] (stage A
+// unchanged), whose transform Z gives the real transform's bins 0..N-1 as
+//   X[k] = E[k] + W_2N^k O[k],  E = (Z[k] + Z*[N-k]) / 2,
+//                               O = (Z[k] - Z*[N-k]) / 2i
+// (the reference's frc1d forward transform); X times the response, then the
+// N-point backward transform (bcc1d) as for complex input.  In the four-step
+// split bin k = k1 + N1*k2 sits at row k1, position k2, and its mirror N - k
+// at row N1 - k1, position N2 - 1 - k2 (row 0: position (N2 - k2) mod N2),
+// so a workgroup takes rows a = ip and b = N1 - ip together (rows 0 and
+// N1/2 alone): both forward transforms, both rows' bins into LDS, each
+// bin's mirror from there, the response products, both backward
+// transforms.  splitw: W_2N^{k1 + N1*k2} at [k1][k2].  grid.x = (N1/2 + 1) *
+// npg * nchan, block = N2/16.
+__device__ __forceinline__ float2 real_split(float2 z, float2 zm, float2 w)
+{
+  const float2 e = make_float2(0.5f * (z.x + zm.x), 0.5f * (z.y + zm.y));
+  const float2 d = make_float2(0.5f * (z.x - zm.x), 0.5f * (z.y - zm.y));
+  return cadd(e, cmul(w, make_float2(d.y, -d.x)));
+}
+
+template <int N2>
+__global__ __launch_bounds__(N2 / 16) void k_stageB_real(ConvShape s, ChirpArgs c, const float2* __restrict__ twr,
+                                                         const float2* __restrict__ splitw, uint32_t ppg)
+{
+  constexpr int E = 16, NT = N2 / E;
+  constexpr int LROW = N2 + N2 / 16;
+  using LAY = RowLayout<N2, NT>;
+  __shared__ float2 lds[2 * LROW + 2 * N2];
+  float2* bufA = lds;
+  float2* bufB = lds + LROW;
+  float2* hA = lds + 2 * LROW;
+  float2* hB = hA + N2;
+  constexpr int RF = FirstRadix<N2, E, -1>::v;  // forward first pass = backward last
+  constexpr int RI = FirstRadix<N2, E, +1>::v;  // forward last pass
+  static_assert(E / RI == 1, "row plan must end with a pass of radix E");
+  constexpr int Q = E / RF;
+
+  const int tid = threadIdx.x;
+  const uint32_t npair = s.N1 / 2 + 1;
+  const uint32_t npg = (s.npart + ppg - 1) / ppg;
+  uint32_t bid = blockIdx.x;
+  const uint32_t ip = bid % npair;
+  bid /= npair;
+  const uint32_t pg = bid % npg;
+  const uint32_t chan = bid / npg;
+  const uint32_t p_begin = pg * ppg;
+  const uint32_t p_end = min(s.npart, p_begin + ppg);
+  const uint32_t N = s.N1 * N2;
+  const uint64_t plane = uint64_t(s.N1) * N2;
+  const uint32_t ra = ip, rb = s.N1 - ip;
+  const bool pair = ip != 0 && ip != s.N1 / 2;
+
+  for (int k2 = tid; k2 < N2; k2 += NT) {
+    hA[k2] = c.mode == DSPX_CHIRP_TABLE ? c.table[(uint64_t(chan) * s.N1 + ra) * N2 + k2]
+                                        : chirp_onchip(c, chan, ra + s.N1 * k2, N);
+    if (pair)
+      hB[k2] = c.mode == DSPX_CHIRP_TABLE ? c.table[(uint64_t(chan) * s.N1 + rb) * N2 + k2]
+                                          : chirp_onchip(c, chan, rb + s.N1 * k2, N);
+  }
+  float2 wa[E], wb[E], sa[RI], sb[RI];
+#pragma unroll
+  for (int q = 0; q < Q; q++)
+#pragma unroll
+    for (int r = 0; r < RF; r++) {
+      const uint32_t n2 = uint32_t(tid + q * NT + r * (N2 / RF));
+      wa[q * RF + r] = tw_N(s.tw, n2 * ra);
+      wb[q * RF + r] = tw_N(s.tw, (n2 * rb) & (N - 1));
+    }
+#pragma unroll
+  for (int r = 0; r < RI; r++) {
+    const int k2 = tid + r * (N2 / RI);
+    sa[r] = splitw[uint64_t(ra) * N2 + k2];
+    sb[r] = pair ? splitw[uint64_t(rb) * N2 + k2] : make_float2(0.f, 0.f);
+  }
+  __syncthreads();
+
+  const uint32_t nrow = (p_end - p_begin) * s.npol;
+  for (uint32_t i = 0; i < nrow; i++) {
+    const uint32_t p = p_begin + i / s.npol, pol = i % s.npol;
+    float2* base = s.work + ((uint64_t(chan) * s.npart + p) * s.npol + pol) * plane;
+    float2* rowa = base + uint64_t(ra) * N2;
+    float2* rowb = base + uint64_t(rb) * N2;
+    float2 va[E], vb[E];
+#pragma unroll
+    for (int q = 0; q < Q; q++)
+#pragma unroll
+      for (int r = 0; r < RF; r++) {
+        const int n2 = tid + q * NT + r * (N2 / RF);
+        va[q * RF + r] = cmul(rowa[n2], wa[q * RF + r]);
+        vb[q * RF + r] = pair ? cmul(rowb[n2], wb[q * RF + r]) : make_float2(0.f, 0.f);
+      }
+    fft_regs<LAY, N2, 1, NT, -1>(bufA, va, twr, tid);
+    if (pair) fft_regs<LAY, N2, 1, NT, -1>(bufB, vb, twr, tid);
+    // the bins of both rows into LDS, then each bin's mirror from there
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < RI; r++) {
+      const int k2 = tid + r * (N2 / RI);
+      bufA[LAY::idx(k2, 0)] = va[r];
+      if (pair) bufB[LAY::idx(k2, 0)] = vb[r];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < RI; r++) {
+      const int k2 = tid + r * (N2 / RI);
+      const int mk = N2 - 1 - k2;
+      float2 ma;
+      if (pair)
+        ma = bufB[LAY::idx(mk, 0)];
+      else
+        ma = bufA[LAY::idx(ra == 0 ? (N2 - k2) & (N2 - 1) : mk, 0)];
+      va[r] = cmul(real_split(va[r], make_float2(ma.x, -ma.y), sa[r]), hA[k2]);
+      if (pair) {
+        const float2 mb = bufA[LAY::idx(mk, 0)];
+        vb[r] = cmul(real_split(vb[r], make_float2(mb.x, -mb.y), sb[r]), hB[k2]);
+      }
+    }
+    __syncthreads();  // the mirrors are read before the backward transforms reuse the buffers
+    fft_regs<LAY, N2, 1, NT, +1>(bufA, va, twr, tid);
+    if (pair) fft_regs<LAY, N2, 1, NT, +1>(bufB, vb, twr, tid);
+#pragma unroll
+    for (int q = 0; q < Q; q++)
+#pragma unroll
+      for (int r = 0; r < RF; r++) {
+        const int n2 = tid + q * NT + r * (N2 / RF);
+        rowa[n2] = cmulc(va[q * RF + r], wa[q * RF + r]);
+        if (pair) rowb[n2] = cmulc(vb[q * RF + r], wb[q * RF + r]);
+      }
   }
 }
 

@@ -68,7 +68,8 @@ void ConvolutionEngine::create(const Response* response)
   dspx_conv_desc d{};
   d.nchan = in->get_nchan();
   d.npol = in->get_npol();
-  d.nfft = parent->get_minimum_samples();
+  d.nfft = response->get_ndat();  // complex points; 2*nfft real samples of Nyquist input
+  d.nyquist = in->get_state() == Signal::Nyquist ? 1u : 0u;
   d.nfilt_pos = parent->get_nfilt_pos();
   d.nfilt_neg = parent->get_nfilt_neg();
   d.form = form;

@@ -323,8 +323,11 @@ void Convolution::prepare()
   if (prepared) return;
   if (!response) throw Error(InvalidState, "dsp::Convolution::prepare", "no frequency response");
   if (input->get_detected()) throw Error(InvalidState, "dsp::Convolution::prepare", "input data are detected");
-  if (input->get_state() != Signal::Analytic)
-    throw Error(InvalidState, "dsp::Convolution::prepare", "only Analytic (complex) input is on this path");
+  const Signal::State state = input->get_state();
+  if (state != Signal::Analytic && state != Signal::Nyquist)
+    throw Error(InvalidState, "dsp::Convolution::prepare", "Cannot transform Signal::State");  // Convolution.C:385-400
+  if (state == Signal::Nyquist && zero_DM)
+    throw Error(InvalidState, "dsp::Convolution::prepare", "no zero-DM output of real-sampled input on this path");
   if (!engine) throw Error(InvalidState, "dsp::Convolution::prepare", "no engine (the CPU branch is the oracle)");
   response->prepare(input);
   if (response->get_ndat() < 2)
@@ -332,10 +335,13 @@ void Convolution::prepare()
   if (response->get_nchan() != input->get_nchan())
     throw Error(InvalidState, "dsp::Convolution::prepare", "invalid response nsub=%d != nchan=%d",
                 response->get_nchan(), input->get_nchan());
-  nsamp_fft = response->get_ndat();
   nfilt_pos = response->get_impulse_pos();
   nfilt_neg = response->get_impulse_neg();
-  nsamp_overlap = nfilt_pos + nfilt_neg;
+  // real-sampled input: 2 real samples per complex point of the response
+  // (Convolution.C:385-395)
+  const unsigned nreal = state == Signal::Nyquist ? 2 : 1;
+  nsamp_fft = nreal * response->get_ndat();
+  nsamp_overlap = nreal * (nfilt_pos + nfilt_neg);
   if (nsamp_fft < nsamp_overlap)
     throw Error(InvalidState, "dsp::Convolution::prepare", "error nfft=%d < nfilt=%d", nsamp_fft, nsamp_overlap);
   prepare_output();
@@ -352,6 +358,7 @@ void Convolution::prepare_output()
   output->copy_configuration(input);
   output->set_state(Signal::Analytic);
   output->set_ndim(2);
+  if (input->get_state() == Signal::Nyquist) output->set_rate(0.5 * input->get_rate());  // Convolution.C:515-517
 }
 
 void Convolution::operate()
@@ -363,6 +370,7 @@ void Convolution::operate()
   }
   prepare_output();
   uint64_t output_ndat = npart * nsamp_step;  // Convolution::reserve (:601-611)
+  if (input->get_state() == Signal::Nyquist) output_ndat /= 2;
   output->resize(output_ndat);
   output->set_start_time(input->get_start_time());
   output->change_start_time(nfilt_pos);

@@ -108,7 +108,7 @@ class Convolution {
   const TimeSeries* get_input() const { return input; }
   TimeSeries* get_output() const { return output; }
 
-  //! Convolution.C:316-455 (Analytic input)
+  //! Convolution.C:316-455 (Analytic or real-sampled Nyquist input)
   void prepare();
   //! Transformation::operate -> Convolution::transformation (Convolution.C:646-698)
   void operate();

@@ -194,7 +194,8 @@ typedef struct {
   uint32_t nfilt_neg;
   uint32_t chirp_mode;
   uint32_t form;
-  uint32_t reserved;
+  uint32_t nyquist;        /* 0: Analytic (complex) input; 1: real-sampled
+                              (Signal::Nyquist) input, fused forms only */
 } dspx_conv_desc;
 
 int dspx_conv_create(dspx_ctx* ctx, const dspx_conv_desc* desc, dspx_conv** out);
@@ -205,7 +206,14 @@ int dspx_conv_set_response(dspx_conv* conv, const float* host_response);
 /* Engine::perform: FPT complex input (ndat samples per chan/pol), npart
    overlapping FFTs of nfft with step nfft - nfilt_pos - nfilt_neg; output
    npart*step samples.  d_zero_dm (nullable) receives the zero-DM copy.
-   Strides in floats. */
+   Strides in floats.  Real-sampled input (nyquist = 1, Convolution.C:385-413,
+   597-608, 700-790, ConvolutionCUDA.cu:776-944): ndat real samples per
+   chan/pol; part p is the 2*nfft real samples from 2*p*step, whose 2*nfft-
+   point real transform's bins 0..nfft-1 take the response (in frequency
+   order, not swapped; the on-chip chirp zeroes the first channel's DC, as
+   Response.C:418-420 does) before the nfft-point backward transform; the
+   output is complex, npart*step samples at half the input rate; no zero-DM
+   copy. */
 int dspx_conv_perform(dspx_conv* conv, const float* d_in, uint64_t in_chan_stride,
                       uint64_t in_pol_stride, uint64_t ndat, float* d_out,
                       uint64_t out_chan_stride, uint64_t out_pol_stride, float* d_zero_dm,

@@ -511,6 +511,64 @@ void ox_dedisp_response_f64(double cfreq, double bw, unsigned nchan, int dc_cent
 OX_DEFINE_CONV(float, ox_convolve_f32, ox_fft_f32)
 OX_DEFINE_CONV(double, ox_convolve_f64, ox_fft_f64)
 
+/* Overlap-save convolution of real-sampled (Nyquist) input: Convolution.C:
+   385-413 (nsamp_fft = 2*nfft real samples, nsamp_overlap = 2*nfilt_tot,
+   normaliser 1/sqrt(nfft)), 597-608 (output ndat = npart*step/2, Analytic),
+   700-790 (frc1d forward, response over the nfft bins, bcc1d backward of
+   nfft points, nfilt_pos complex samples dropped).  The forward transform
+   of the 2*nfft real window is its 2*nfft-point complex DFT (imaginary part
+   zero), of which bins 0..nfft-1 are used (the Nyquist bin is not).
+     in  [nchan][npol][ndat] real; resp [nchan][nfft][2] in frequency order
+     (real input is not dual-sideband: no swap, Observation.C:95-102,
+     Response.C:335-345), DC of the first channel zero (Response.C:418-420);
+     out [nchan][npol][npart*step][2], step = nfft - nfilt_pos - nfilt_neg,
+     npart = (ndat - 2*nfilt) / (2*step).  Returns npart. */
+#define OX_DEFINE_CONV_REAL(T, NAME, FFT)                                      \
+  uint64_t NAME(const T* in, unsigned nchan, unsigned npol, uint64_t ndat,     \
+                const T* resp, unsigned nfft, unsigned nfilt_pos,              \
+                unsigned nfilt_neg, T* out)                                    \
+  {                                                                            \
+    uint64_t nov = (uint64_t) nfilt_pos + nfilt_neg;                           \
+    uint64_t step = nfft - nov;                                                \
+    uint64_t npart = (ndat < 2 * (uint64_t) nfft) ? 0                          \
+                                                  : (ndat - 2 * nov) / (2 * step); \
+    uint64_t outndat = npart * step;                                           \
+    unsigned ichan, ipol;                                                      \
+    uint64_t ipart, k;                                                         \
+    T* buf = (T*) malloc(sizeof(T) * 4 * nfft);                                \
+    for (ichan = 0; ichan < nchan; ichan++)                                    \
+      for (ipol = 0; ipol < npol; ipol++)                                      \
+      {                                                                        \
+        const T* x = in + ((uint64_t) ichan * npol + ipol) * ndat;             \
+        T* y = out + ((uint64_t) ichan * npol + ipol) * outndat * 2;           \
+        const T* h = resp + (uint64_t) ichan * nfft * 2;                       \
+        for (ipart = 0; ipart < npart; ipart++)                                \
+        {                                                                      \
+          for (k = 0; k < 2 * (uint64_t) nfft; k++)                            \
+          {                                                                    \
+            buf[2 * k] = x[ipart * 2 * step + k];                              \
+            buf[2 * k + 1] = 0;                                                \
+          }                                                                    \
+          FFT(2 * nfft, buf, -1);                                              \
+          for (k = 0; k < nfft; k++)                                           \
+          {                                                                    \
+            T d_r = buf[2 * k], d_i = buf[2 * k + 1];                          \
+            T f_r = h[2 * k], f_i = h[2 * k + 1];                              \
+            buf[2 * k] = f_r * d_r - f_i * d_i;                                \
+            buf[2 * k + 1] = f_i * d_r + f_r * d_i;                            \
+          }                                                                    \
+          FFT(nfft, buf, +1);                                                  \
+          memcpy(y + ipart * step * 2, buf + 2 * (uint64_t) nfilt_pos,         \
+                 sizeof(T) * 2 * step);                                        \
+        }                                                                      \
+      }                                                                        \
+    free(buf);                                                                 \
+    return npart;                                                              \
+  }
+
+OX_DEFINE_CONV_REAL(float, ox_convolve_real_f32, ox_fft_f32)
+OX_DEFINE_CONV_REAL(double, ox_convolve_real_f64, ox_fft_f64)
+
 /* ------------------------------------------------------------------------ */
 /* Detection                                                                 */
 /* ------------------------------------------------------------------------ */

@@ -99,6 +99,10 @@ def lib():
         fn.argtypes = [p, u, u, U, u, u, u, u, p, p]
     L.ox_sk_mask.restype = U
     L.ox_sk_mask.argtypes = [f32p, u, u, U, u, u, u8p, U, f32p]
+    for name, pt in (("ox_convolve_real_f32", f32p), ("ox_convolve_real_f64", f64p)):
+        fn = getattr(L, name)
+        fn.restype = U
+        fn.argtypes = [pt, u, u, U, pt, u, u, u, pt]
     L.ox_sk_detect_res.restype = U
     L.ox_sk_detect_res.argtypes = [f32p, u32p, U, u, u, u, u32p, u32p, f32p, u8p, u, i, u, d, C.c_void_p,
                                    C.c_void_p, C.c_void_p, u, u8p]
@@ -180,6 +184,37 @@ def response(cfreq, bw, nchan, dm, nfft, mode=0, dc_centred=False, interchan=Fal
     out = np.empty(nchan * nfft * 2, np.float32)
     lib().ox_dedisp_response_f32(cfreq, bw, nchan, int(dc_centred), dm, nfft, mode, int(interchan), out)
     return out.view(np.complex64).reshape(nchan, nfft)
+
+
+def response_real(cfreq, bw, nchan, dm, nfft, mode=0):
+    """The response of real-sampled (Nyquist) input: frequency order (not
+    swapped; real input is not dual-sideband, Observation.C:95-102), DC of
+    the first channel zero (Response.C:418-420), normaliser 1/sqrt(nfft)."""
+    r = response(cfreq, bw, nchan, dm, nfft, mode)
+    r = np.roll(r, nfft // 2, axis=1)  # undo the dual-sideband swap
+    r[0, 0] = 0
+    return r
+
+
+def convolve_real(x, resp, nfilt_pos, nfilt_neg):
+    """Real-sampled overlap-save convolution.  x: [nchan][npol][ndat] float32
+    or float64 real samples; resp [nchan][nfft] (response_real).  Returns the
+    complex output [nchan][npol][npart*step]."""
+    x = np.ascontiguousarray(x)
+    nchan, npol, ndat = x.shape
+    nfft = resp.shape[1]
+    step = nfft - nfilt_pos - nfilt_neg
+    npart = 0 if ndat < 2 * nfft else (ndat - 2 * (nfilt_pos + nfilt_neg)) // (2 * step)
+    if x.dtype == np.float32:
+        ft, fn, ct = np.float32, lib().ox_convolve_real_f32, np.complex64
+    else:
+        ft, fn, ct = np.float64, lib().ox_convolve_real_f64, np.complex128
+        x = x.astype(np.float64)
+    resp = np.ascontiguousarray(resp, ct)
+    out = np.zeros(nchan * npol * npart * step * 2, ft)
+    n = fn(x.ravel(), nchan, npol, ndat, resp.view(ft).ravel(), nfft, nfilt_pos, nfilt_neg, out)
+    assert n == npart
+    return out.view(ct).reshape(nchan, npol, npart * step)
 
 
 def sample_delays(cfreq, bw, nchan, dm, dc_centred=False):

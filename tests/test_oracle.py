@@ -520,3 +520,44 @@ def test_sk_channel_range(oracle):
     m1a, _, _ = oracle.sk_detect(sums, counts, M, lower, upper, sym, pcsd, flags=oracle.SKFB)
     m1i, _, _ = oracle.sk_detect(sums, counts, M, lower, upper, sym, pcsd, flags=oracle.SKFB, channels=inc)
     assert np.array_equal(m1a, m1i) and m1a[:, 5].mean() > 0.9
+
+
+def test_convolve_real_tone_closed_form(oracle):
+    """Real-sampled convolution: a cosine at bin f of the 2N-point real
+    transform through a unit response 1/sqrt(N) comes out as
+    sqrt(N) exp(2 pi i f n / N) (the positive-frequency half, the band's
+    bottom at DC: Convolution.C:632-645)."""
+    N, f = 1024, 37
+    x = np.cos(2 * np.pi * f * np.arange(8 * N) / (2 * N)).reshape(1, 1, -1)
+    resp = np.ones((1, N), np.complex128) / np.sqrt(N)
+    y = oracle.convolve_real(x, resp, 0, 0)
+    n = np.arange(y.shape[2])
+    assert np.abs(y[0, 0] - np.sqrt(N) * np.exp(2j * np.pi * f * n / N)).max() < 1e-9
+
+
+def test_convolve_real_matches_numpy_rfft(oracle):
+    """Against numpy's real FFT (rfft bins 0..N-1, response product,
+    unnormalised inverse), with overlap, two channels and the dedispersion
+    response of real-sampled input (frequency order, first channel's DC
+    zero)."""
+    cfreq, bw, nchan, dm, N = 1400.0, 4.0, 2, 5.0, 4096
+    g = oracle.geometry(cfreq, bw, nchan, dm, N)
+    pos, neg = g["impulse_pos"], g["impulse_neg"]
+    step = N - pos - neg
+    rng = np.random.default_rng(2)
+    ndat = 6 * N
+    x = rng.standard_normal((nchan, 2, ndat))
+    resp = oracle.response_real(cfreq, bw, nchan, dm, N, 1).astype(np.complex128)
+    assert resp[0, 0] == 0 and resp[1, 0] != 0
+    y = oracle.convolve_real(x, resp, pos, neg)
+    npart = (ndat - 2 * (pos + neg)) // (2 * step)
+    assert y.shape == (nchan, 2, npart * step)
+    for c in range(nchan):
+        for p in range(2):
+            for ip in range(npart):
+                X = np.fft.rfft(x[c, p, 2 * ip * step:2 * ip * step + 2 * N])[:N]
+                z = np.fft.ifft(X * resp[c]) * N
+                np.testing.assert_allclose(y[c, p, ip * step:(ip + 1) * step], z[pos:pos + step], rtol=0, atol=1e-9)
+    # the frequency order: the response's point k is the analytic response's point (k + N/2) % N
+    ra = oracle.response(cfreq, bw, nchan, dm, N, 1)
+    assert np.array_equal(resp[1], np.roll(ra[1], N // 2).astype(np.complex128))
