@@ -87,7 +87,11 @@ __global__ __launch_bounds__(OnePass<N>::NT) void k_onepass(LD ld, OnePassArgs a
   const uint32_t hist_len = MODE == kStageCFold ? fa.nslot * fa.hrow : 0;
   if (MODE == kStageCFold)
     for (uint32_t i = threadIdx.x; i < hist_len; i += NT) hist[i] = 0.f;
-  constexpr int QG = E / 4;
+  // fold groups: QF consecutive samples per thread (runs of equal bins
+  // accumulated in registers, so that short periods with bins a few samples
+  // wide add to the LDS histogram once per run rather than once per sample)
+  constexpr int QF = 8;
+  constexpr int QG = E / QF;
 
   // the channel's chirp, evaluated once per workgroup: into LDS (read after
   // the first exchange's barrier), or the bins X[j + r*N/RL] (j = tid +
@@ -168,8 +172,8 @@ __global__ __launch_bounds__(OnePass<N>::NT) void k_onepass(LD ld, OnePassArgs a
             }
         }
       } else {
-        // regroup through LDS: each thread then holds QG groups of 4
-        // consecutive samples, (g*NTW + tid)*4 + k
+        // regroup through LDS: each thread then holds QG groups of QF
+        // consecutive samples, (g*NTW + tid)*QF + k
         xsync<NTW>();
 #pragma unroll
         for (int q = 0; q < E / R0; q++)
@@ -179,12 +183,12 @@ __global__ __launch_bounds__(OnePass<N>::NT) void k_onepass(LD ld, OnePassArgs a
 #pragma unroll
         for (int g = 0; g < QG; g++)
 #pragma unroll
-          for (int k = 0; k < 4; k++) {
-            const float2 x = lds[LAY::idx((g * NTW + tid) * 4 + k, 0)];
+          for (int k = 0; k < QF; k++) {
+            const float2 x = lds[LAY::idx((g * NTW + tid) * QF + k, 0)];
             if (pol == 0)
-              y0[g * 4 + k] = x;
+              y0[g * QF + k] = x;
             else
-              v[g * 4 + k] = x;
+              v[g * QF + k] = x;
           }
       }
     }
@@ -192,15 +196,17 @@ __global__ __launch_bounds__(OnePass<N>::NT) void k_onepass(LD ld, OnePassArgs a
       const uint32_t* kw = fa.keys + uint64_t(valid ? w : w_begin) * N;
 #pragma unroll
       for (int g = 0; g < QG; g++) {
-        uint32_t kq[4];
-        load_keys<4>(kw + (g * NTW + tid) * 4, kq);
-        if (!valid) kq[0] = kq[1] = kq[2] = kq[3] = 0xffffffffu;
-        float2 b[4];
+        uint32_t kq[QF];
+        load_keys<QF>(kw + (g * NTW + tid) * QF, kq);
+        if (!valid)
 #pragma unroll
-        for (int k = 0; k < 4; k++) b[k] = a.npol == 2 ? v[g * 4 + k] : make_float2(0.f, 0.f);
+          for (int k = 0; k < QF; k++) kq[k] = 0xffffffffu;
+        float2 b[QF];
+#pragma unroll
+        for (int k = 0; k < QF; k++) b[k] = a.npol == 2 ? v[g * QF + k] : make_float2(0.f, 0.f);
         // output index of the group's first sample
-        const int32_t i0 = int32_t(w * a.step + uint32_t(g * NTW + tid) * 4) - int32_t(a.nfilt_pos);
-        fold_lane<NV, 4, ZH>(hist, fa, a.npol, kq, y0 + g * 4, b, chan, i0);
+        const int32_t i0 = int32_t(w * a.step + uint32_t(g * NTW + tid) * QF) - int32_t(a.nfilt_pos);
+        fold_lane<NV, QF, ZH>(hist, fa, a.npol, kq, y0 + g * QF, b, chan, i0);
       }
     }
     if (it + 1 < niter)

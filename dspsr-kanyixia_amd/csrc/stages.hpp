@@ -876,13 +876,17 @@ __device__ __forceinline__ void fold_lane(float* hist, const FoldArgs& fa, uint3
 // each thread holds QN consecutive columns of R rows.  A multi-pass
 // transform's last pass delivers it directly (QN = E / last radix); a
 // single-pass transform (L = 16) would leave one column per thread, so its
-// outputs are regrouped through LDS into QN = 4 (a 16-lane row then spans
-// 64 consecutive samples and the fold's row sums serve 4 samples per lane).
+// outputs are regrouped through LDS into QN = 8 (a 16-lane row then spans
+// 128 consecutive samples and the fold's row sums serve 8 samples per lane;
+// bins a few samples wide are accumulated per run within a lane's 8).
 template <int L> struct FoldOwn {
   static constexpr int E = kColTile / kColThreads;
   static constexpr int RL = LastRadix<L, 16, +1>::v;
-  static constexpr bool REGROUP = RadixPlan<L, 16>::n == 1;
-  static constexpr int QN = REGROUP ? 4 : E / RL;
+  // regrouped through LDS where the last pass would leave fewer than 4
+  // consecutive columns per thread (16 points: one pass; 128 points and up:
+  // a last pass of radix 8 or 16)
+  static constexpr bool REGROUP = RadixPlan<L, 16>::n == 1 || E / RL < 4;
+  static constexpr int QN = REGROUP ? 8 : E / RL;
   static constexpr int R = E / QN;
 };
 
@@ -900,10 +904,17 @@ __device__ __forceinline__ void last_pass_cols(float2* lds, float2* v, const flo
   const int b0 = (tid % (C / QN)) * QN;
   const int j = tid / (C / QN);
   if constexpr (OWN::REGROUP) {
-    // single pass in the standard ownership (one column per thread, the
-    // caller has synchronised), then one exchange
-    stockham_compute<L, C, kColThreads, RL, 1, +1>(v, tw, tid);
-    lds_store_pass<LAY, L, C, kColThreads, RL, 1>(lds, v, tid);
+    if constexpr (NPASS == 1) {
+      // single pass in the standard ownership (one column per thread, the
+      // caller has synchronised), then one exchange
+      stockham_compute<L, C, kColThreads, RL, 1, +1>(v, tw, tid);
+      lds_store_pass<LAY, L, C, kColThreads, RL, 1>(lds, v, tid);
+    } else {
+      // every pass in the standard ownership, then one more exchange
+      run_passes<LAY, L, C, kColThreads, +1, 0, NPASS>(lds, v, tw, tid);
+      __syncthreads();
+      lds_store_pass<LAY, L, C, kColThreads, RL, InvNs<L, 16, NPASS - 1>::v>(lds, v, tid);
+    }
     __syncthreads();
     constexpr int R = OWN::R;
 #pragma unroll
