@@ -90,7 +90,7 @@ __global__ __launch_bounds__(OnePass<N>::NT) void k_onepass(LD ld, OnePassArgs a
   // fold groups: QF consecutive samples per thread (runs of equal bins
   // accumulated in registers, so that short periods with bins a few samples
   // wide add to the LDS histogram once per run rather than once per sample)
-  constexpr int QF = 8;
+  constexpr int QF = E;
   constexpr int QG = E / QF;
 
   // the channel's chirp, evaluated once per workgroup: into LDS (read after

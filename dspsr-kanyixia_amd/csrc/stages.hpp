@@ -886,7 +886,7 @@ template <int L> struct FoldOwn {
   // consecutive columns per thread (16 points: one pass; 128 points and up:
   // a last pass of radix 8 or 16)
   static constexpr bool REGROUP = RadixPlan<L, 16>::n == 1 || E / RL < 4;
-  static constexpr int QN = REGROUP ? 8 : E / RL;
+  static constexpr int QN = REGROUP ? (RadixPlan<L, 16>::n == 1 ? 16 : 8) : E / RL;
   static constexpr int R = E / QN;
 };
 
