@@ -25,10 +25,10 @@
 namespace dspx {
 
 template <int N> struct OnePass {
-  // complex values per thread per pol: 8 up to 512 points and 16 above, so
-  // that a transform of up to 1024 points is one wavefront's (or part of
-  // one), whose exchanges need no workgroup barrier
-  static constexpr int E = N <= 512 ? 8 : 16;
+  // complex values per thread per pol: 16, so that a transform of up to 1024
+  // points is one wavefront's (or part of one), whose exchanges need no
+  // workgroup barrier, and the fold takes 16 consecutive samples per lane
+  static constexpr int E = 16;
   static constexpr int NTW = N / E;               // threads per window
   static constexpr int NT = NTW < 256 ? 256 : NTW;  // threads per workgroup
   static constexpr int WF = NT / NTW;             // windows in flight
@@ -45,7 +45,7 @@ template <int N> struct OnePass {
 constexpr int kOnePassMinLog2 = 7, kOnePassMaxLog2 = 13;
 
 // LDS bytes of the exchange rows for an n-point transform (host side).
-inline uint32_t onepass_ntw(uint32_t n) { return n / (n <= 512 ? 8 : 16); }
+inline uint32_t onepass_ntw(uint32_t n) { return n / 16; }
 inline uint32_t onepass_threads(uint32_t n) { return onepass_ntw(n) < 256 ? 256 : onepass_ntw(n); }
 inline uint32_t onepass_wf(uint32_t n) { return onepass_threads(n) / onepass_ntw(n); }
 inline size_t onepass_lds_bytes(uint32_t n)
