@@ -874,19 +874,20 @@ __device__ __forceinline__ void fold_lane(float* hist, const FoldArgs& fa, uint3
 
 // The fold's ownership of an L-point inverse column transform's outputs:
 // each thread holds QN consecutive columns of R rows.  A multi-pass
-// transform's last pass delivers it directly (QN = E / last radix); a
-// single-pass transform (L = 16) would leave one column per thread, so its
-// outputs are regrouped through LDS into QN = 8 (a 16-lane row then spans
-// 128 consecutive samples and the fold's row sums serve 8 samples per lane;
-// bins a few samples wide are accumulated per run within a lane's 8).
+// transform's last pass delivers E / (last radix) of them; a single-pass
+// transform (L = 16) would leave one column per thread.  The outputs are
+// regrouped through LDS into QN = 16 (or the tile's column count, if fewer)
+// wherever the last pass leaves fewer: a lane's 16 consecutive samples are
+// accumulated per run of equal bins (bins a few samples wide, short periods,
+// add to the histogram once per run), and a 16-lane row spans 256 samples
+// for the row sums of wide bins.
 template <int L> struct FoldOwn {
   static constexpr int E = kColTile / kColThreads;
   static constexpr int RL = LastRadix<L, 16, +1>::v;
-  // regrouped through LDS where the last pass would leave fewer than 4
-  // consecutive columns per thread (16 points: one pass; 128 points and up:
-  // a last pass of radix 8 or 16)
-  static constexpr bool REGROUP = RadixPlan<L, 16>::n == 1 || E / RL < 4;
-  static constexpr int QN = REGROUP ? (RadixPlan<L, 16>::n == 1 ? 16 : 8) : E / RL;
+  static constexpr int C = kColTile / L;             // columns of a tile
+  static constexpr int QMAX = C < 16 ? C : 16;
+  static constexpr bool REGROUP = RadixPlan<L, 16>::n == 1 || E / RL < QMAX;
+  static constexpr int QN = REGROUP ? QMAX : E / RL;
   static constexpr int R = E / QN;
 };
 
