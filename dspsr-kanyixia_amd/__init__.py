@@ -571,10 +571,14 @@ class Fold:
 
 class Filterbank:
     """dspx_fb: the convolving filterbank engine.  Windows of 1024-4096
-    points (sub-bands of 32-512 bins) run as one kernel per window; other
-    lengths, or any length with the environment DSPX_FB_FORM=library, run as
-    batches of FFT-library transforms with kernels for the response product
-    and the overlap discard."""
+    points (sub-bands of 32-512 bins) run as one kernel per window; windows
+    of 2^13-2^22 points whose sub-bands hold at least 8*N1 bins (N1 the
+    column length of the N1*N2 split) run as three kernels (column
+    transform, row transform with the response and the sub-bands' backward
+    rows, backward columns with the discard); other lengths, or any length
+    with the environment DSPX_FB_FORM=library, run as batches of
+    FFT-library transforms with kernels for the response product and the
+    overlap discard."""
 
     def __init__(self, ctx, nchan, npol, nsub, fres, nfilt_pos=0, nfilt_neg=0):
         self.ctx = ctx
