@@ -1664,7 +1664,7 @@ dspx_ltf_desc with_geometry(const dspx_ltf_desc& d0, uint32_t nfft)
 // grids, tails and cache footprints are those of the replay.  (Timing a
 // 2^20-sample slice instead let lengths within a few per cent of each other
 // trade places from run to run.)
-double time_per_sample(dspx_batch* B, const dspx_ltf_desc& dn)
+double time_per_sample(dspx_batch* B, const dspx_ltf_desc& dn, double* block_s = nullptr)
 {
   dspx_ctx* ctx = B->ctx;
   dspx_ltf_desc d = dn;
@@ -1698,6 +1698,7 @@ double time_per_sample(dspx_batch* B, const dspx_ltf_desc& dn)
     DSPX_HIP(hipEventElapsedTime(&ms, ctx->ev[i], ctx->ev[i + 1]));
     best = std::min(best, ms);
   }
+  if (block_s) *block_s = double(best) * 1e-3;
   return double(best) * 1e-3 / double(L->ndat_out);
 }
 
@@ -1705,18 +1706,22 @@ double time_per_sample(dspx_batch* B, const dspx_ltf_desc& dn)
 // (longer transforms waste less on the overlap, shorter ones keep the
 // transforms in their fastest shapes), within the fused forms' range; each
 // runs in the form the plan picks for it (one pass where it fits).
-uint32_t autotune_nfft(dspx_batch* B, const dspx_ltf_desc& d0)
+// block_s: the block's time at the chosen length (0 if none was timed).
+uint32_t autotune_nfft(dspx_batch* B, const dspx_ltf_desc& d0, double& block_s)
 {
   dspx_ltf_desc ref = with_geometry(d0, 0);
   uint32_t best = ref.nfft;
   double best_t = 1e30;
+  block_s = 0;
   for (uint32_t n = ref.nfft; n <= (1u << 22) && n <= 8 * ref.nfft; n *= 2) {
     dspx_ltf_desc d = with_geometry(d0, n);
     if (d.nfilt_pos + d.nfilt_neg >= n || d.ndat < n) continue;
-    double t = time_per_sample(B, d);
+    double bs = 0;
+    double t = time_per_sample(B, d, &bs);
     if (t < best_t) {
       best_t = t;
       best = n;
+      block_s = bs;
     }
   }
   return best;
@@ -1740,12 +1745,13 @@ int dspx_batch_create(dspx_ctx* ctx, const dspx_ltf_desc* beams, uint32_t nbeam,
     if (!nbranch) nbranch = 1;
     nbranch = std::min(nbranch, nbeam);
     std::vector<dspx_ltf_desc> descs(nbeam);
+    std::vector<double> timed(nbeam, 0.0);  // the autotuner's time of each beam's block
     for (uint32_t i = 0; i < nbeam; i++) {
       const dspx_ltf_desc& d0 = beams[i];
       DSPX_REQUIRE(d0.form != DSPX_FORM_PLAIN, DSPX_E_PARAM, "beam %u: the batch runs the fused forms", i);
       dspx_ltf_desc d = d0;
       if (d0.nfft == 0) {
-        uint32_t n = (flags & DSPX_BATCH_AUTOTUNE) ? autotune_nfft(B.get(), d0) : 0;
+        uint32_t n = (flags & DSPX_BATCH_AUTOTUNE) ? autotune_nfft(B.get(), d0, timed[i]) : 0;
         d = with_geometry(d0, n);
       }
       descs[i] = d;
@@ -1765,13 +1771,16 @@ int dspx_batch_create(dspx_ctx* ctx, const dspx_ltf_desc* beams, uint32_t nbeam,
       DSPX_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
       B->branch_ev.push_back(e);
     }
-    // beams to branches, longest first to the least loaded (a beam's cost
-    // taken as its window samples times log2 of its length)
+    // beams to branches, longest first to the least loaded: a beam's cost is
+    // its block's time as the autotuner measured it where every beam was
+    // timed, else its samples times log2 of its length
     std::vector<uint32_t> order(nbeam), branch(nbeam, 0);
     std::vector<double> cost(nbeam), load(nbranch, 0.0);
+    const bool all_timed = std::all_of(timed.begin(), timed.end(), [](double t) { return t > 0; });
     for (uint32_t i = 0; i < nbeam; i++) {
       order[i] = i;
-      cost[i] = double(descs[i].ndat) * descs[i].nchan * descs[i].npol * std::log2(double(descs[i].nfft));
+      cost[i] = all_timed ? timed[i]
+                          : double(descs[i].ndat) * descs[i].nchan * descs[i].npol * std::log2(double(descs[i].nfft));
     }
     std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return cost[a] > cost[b]; });
     for (uint32_t i : order) {
