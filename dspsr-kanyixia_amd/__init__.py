@@ -126,6 +126,7 @@ _SIGS = {
     "dspx_fb_set_response": (I32, [P, P]),
     "dspx_fb_geometry": (I32, [P, U64, C.POINTER(U64), C.POINTER(U64)]),
     "dspx_fb_perform": (I32, [P, P, U64, U64, U64, P, U64, U64]),
+    "dspx_fb_perform_zdm": (I32, [P, P, U64, U64, U64, P, U64, U64, P]),
     "dspx_fb_batches": (I32, [P, C.POINTER(U32)]),
     "dspx_detect": (I32, [P, U32, U32, P, U64, U64, U32, U32, U64, P, U64, U64]),
     "dspx_fold_create": (I32, [P, U64, D, U32, C.POINTER(FoldChunk), U32, C.POINTER(P)]),
@@ -609,8 +610,14 @@ class Filterbank:
         self.ctx.check(lib().dspx_fb_geometry(self.h, ndat, C.byref(npart), C.byref(nout)))
         return npart.value, nout.value
 
-    def perform(self, x, ics, ips, ndat, out, ocs, ops):
-        self.ctx.check(lib().dspx_fb_perform(self.h, ptr(x), ics, ips, ndat, ptr(out), ocs, ops))
+    def perform(self, x, ics, ips, ndat, out, ocs, ops, zdm=None):
+        """zdm: the zero-DM output (same layout as out): each window's
+        sub-bands transformed back without the response."""
+        if zdm is None:
+            self.ctx.check(lib().dspx_fb_perform(self.h, ptr(x), ics, ips, ndat, ptr(out), ocs, ops))
+        else:
+            self.ctx.check(lib().dspx_fb_perform_zdm(self.h, ptr(x), ics, ips, ndat, ptr(out), ocs, ops,
+                                                     ptr(zdm)))
 
     def batches(self):
         """Library-form batches of the last perform (0: the one-kernel form)."""

@@ -1955,7 +1955,7 @@ void fb4_destroy(Fb4* f) { delete f; }
 void fb4_set_response(Fb4* f, const float* host) { f->core.set_table(host); }
 
 void fb4_perform(Fb4* f, const float* d_in, uint64_t ics, uint64_t ips, uint64_t npart, float* d_out, uint64_t ocs,
-                 uint64_t ops)
+                 uint64_t ops, float* d_zdm)
 {
   ConvCore& k = f->core;
   DSPX_REQUIRE(k.have_table, DSPX_E_STATE, "no response: call dspx_fb_set_response first");
@@ -1975,14 +1975,22 @@ void fb4_perform(Fb4* f, const float* d_in, uint64_t ics, uint64_t ips, uint64_t
     npg = (np + ppg - 1) / ppg;
     const uint64_t nb = rows * npg;
     DSPX_REQUIRE(nb < (1ull << 31), DSPX_E_UNSUPPORTED, "grid too large");
-    fb4_stage_b(k.N2, f->F2, unsigned(nb), k.ctx->stream, k.shape(np), k.chirp(), twiddles(k.ctx, int(k.N2)),
-                twiddles(k.ctx, int(f->F2)), ppg, f->nsub);
-    FilterbankArgs fb{d_out + 2 * p0 * f->nkeep, ocs, ops};
-    fb.f2 = f->F2;
-    fb.lf2 = uint32_t(__builtin_ctz(f->F2));
-    fb.nsub = f->nsub;
-    fb.nkeep = f->nkeep;
-    k.stage_c<kStageCFilterbank>(np, fb, FoldArgs{});
+    // the dedispersed output, then (stage B overwrote stage A's rows) the
+    // zero-DM output from the windows transformed again, with a null table
+    // (a unit response) in stage B
+    for (int pass = 0; pass < (d_zdm ? 2 : 1); pass++) {
+      if (pass) k.stage_a(ld, np);
+      ChirpArgs c = k.chirp();
+      if (pass) c.table = nullptr;
+      fb4_stage_b(k.N2, f->F2, unsigned(nb), k.ctx->stream, k.shape(np), c, twiddles(k.ctx, int(k.N2)),
+                  twiddles(k.ctx, int(f->F2)), ppg, f->nsub);
+      FilterbankArgs fb{(pass ? d_zdm : d_out) + 2 * p0 * f->nkeep, ocs, ops};
+      fb.f2 = f->F2;
+      fb.lf2 = uint32_t(__builtin_ctz(f->F2));
+      fb.nsub = f->nsub;
+      fb.nkeep = f->nkeep;
+      k.stage_c<kStageCFilterbank>(np, fb, FoldArgs{});
+    }
   }
 }
 

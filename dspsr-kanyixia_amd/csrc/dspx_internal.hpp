@@ -198,7 +198,8 @@ struct Fb4;
 Fb4* fb4_create(dspx_ctx* ctx, const dspx_fb_desc& d);
 void fb4_destroy(Fb4* f);
 void fb4_set_response(Fb4* f, const float* host);
+// d_zdm: the zero-DM output (sub-bands without the response), or null
 void fb4_perform(Fb4* f, const float* d_in, uint64_t ics, uint64_t ips, uint64_t npart, float* d_out, uint64_t ocs,
-                 uint64_t ops);
+                 uint64_t ops, float* d_zdm);
 
 }  // namespace dspx

@@ -94,15 +94,18 @@ __global__ void k_fb_discard(const float2* __restrict__ scratch, uint64_t nfft, 
 // transforms of those columns, and the kept rows [nfilt_pos, nfilt_pos +
 // nkeep) of every column written with consecutive lanes on consecutive
 // output samples.  Same transforms and product as the library form above,
-// without its four HBM round trips of the window.
-template <int NFFT, int FRES>
+// without its four HBM round trips of the window.  ZDM: the sub-bands of
+// the spectrum without the response are transformed back too, into the
+// zero-DM output zdm (same layout as out), from the forward transform kept
+// in registers (FilterbankEngineCPU.C:266-300).
+template <int NFFT, int FRES, bool ZDM = false>
 __global__ __launch_bounds__(NFFT / 16) void k_fb_fused(const float* __restrict__ in, uint64_t ics, uint64_t ips,
                                                         uint32_t npol, uint64_t in_step, uint32_t npart,
                                                         const float2* __restrict__ resp,
                                                         const float2* __restrict__ tw_f,
                                                         const float2* __restrict__ tw_b, uint32_t nkeep,
                                                         uint32_t nfilt_pos, float* __restrict__ out, uint64_t ocs,
-                                                        uint64_t ops)
+                                                        uint64_t ops, float* __restrict__ zdm)
 {
   constexpr int NT = NFFT / 16, E = 16, NSUB = NFFT / FRES, RS = NSUB + 1;
   using ROW = RowLayout<NFFT, NT>;
@@ -126,28 +129,42 @@ __global__ __launch_bounds__(NFFT / 16) void k_fb_fused(const float* __restrict_
   // the forward transform's last pass leaves bin g + r*NFFT/RL in v[q*RL + r]
   constexpr int RL = LastRadix<NFFT, E, -1>::v;
   const float2* f = resp + uint64_t(chan) * NFFT;
-  __syncthreads();
+  float2 v0[ZDM ? E : 1];
+  if constexpr (ZDM)
 #pragma unroll
-  for (int q = 0; q < E / RL; q++)
+    for (int e = 0; e < E; e++) v0[e] = v[e];
+  // the spectrum's sub-bands (times the response, or not) transformed back
+  // and their kept samples stored at dst
+  auto backward = [&](float2* vv, bool product, float* dst) {
+    __syncthreads();
 #pragma unroll
-    for (int r = 0; r < RL; r++) {
-      const int k = tid + q * NT + r * (NFFT / RL);
-      const float2 d = v[q * RL + r], w = f[k];
-      lds[COL::idx(k % FRES, k / FRES)] = make_float2(w.x * d.x - w.y * d.y, w.y * d.x + w.x * d.y);
-    }
-  __syncthreads();
-  lds_load_pass<COL, FRES, NSUB, NT, FirstRadix<FRES, E, +1>::v>(lds, v, tid);
-  fft_regs<COL, FRES, NSUB, NT, +1>(lds, v, tw_b, tid);
-  constexpr int NP = RadixPlan<FRES, E>::n;
-  __syncthreads();
-  lds_store_pass<COL, FRES, NSUB, NT, InvRadix<FRES, E, NP - 1>::v, InvNs<FRES, E, NP - 1>::v>(lds, v, tid);
-  __syncthreads();
-
-  float* o = out + uint64_t(chan) * NSUB * ocs + pol * ops + 2 * uint64_t(p) * nkeep;
+    for (int q = 0; q < E / RL; q++)
 #pragma unroll
-  for (int s = 0; s < NSUB; s++)
-    for (uint32_t m = tid; m < nkeep; m += NT)
-      *reinterpret_cast<float2*>(o + s * ocs + 2 * m) = lds[COL::idx(int(nfilt_pos + m), s)];
+      for (int r = 0; r < RL; r++) {
+        const int k = tid + q * NT + r * (NFFT / RL);
+        const float2 d = vv[q * RL + r];
+        if (product) {
+          const float2 w = f[k];
+          lds[COL::idx(k % FRES, k / FRES)] = make_float2(w.x * d.x - w.y * d.y, w.y * d.x + w.x * d.y);
+        } else {
+          lds[COL::idx(k % FRES, k / FRES)] = d;
+        }
+      }
+    __syncthreads();
+    lds_load_pass<COL, FRES, NSUB, NT, FirstRadix<FRES, E, +1>::v>(lds, vv, tid);
+    fft_regs<COL, FRES, NSUB, NT, +1>(lds, vv, tw_b, tid);
+    constexpr int NP = RadixPlan<FRES, E>::n;
+    __syncthreads();
+    lds_store_pass<COL, FRES, NSUB, NT, InvRadix<FRES, E, NP - 1>::v, InvNs<FRES, E, NP - 1>::v>(lds, vv, tid);
+    __syncthreads();
+    float* o = dst + uint64_t(chan) * NSUB * ocs + pol * ops + 2 * uint64_t(p) * nkeep;
+#pragma unroll
+    for (int s = 0; s < NSUB; s++)
+      for (uint32_t m = tid; m < nkeep; m += NT)
+        *reinterpret_cast<float2*>(o + s * ocs + 2 * m) = lds[COL::idx(int(nfilt_pos + m), s)];
+  };
+  backward(v, true, out);
+  if constexpr (ZDM) backward(v0, false, zdm);
 }
 
 }  // namespace dspx
@@ -234,11 +251,17 @@ int dspx_fb_geometry(const dspx_fb* F, uint64_t ndat, uint64_t* npart, uint64_t*
   return DSPX_OK;
 }
 
-int dspx_fb_perform(dspx_fb* F, const float* d_in, uint64_t ics, uint64_t ips, uint64_t ndat, float* d_out,
-                    uint64_t ocs, uint64_t ops)
+}  // extern "C"
+
+namespace {
+
+// d_zdm: the zero-DM output (the sub-bands without the response), same
+// layout as d_out, or null
+int fb_perform(dspx_fb* F, const float* d_in, uint64_t ics, uint64_t ips, uint64_t ndat, float* d_out, uint64_t ocs,
+               uint64_t ops, float* d_zdm, bool zdm_call)
 {
   return guard(F ? F->ctx : nullptr, [&] {
-    DSPX_REQUIRE(F && d_in && d_out, DSPX_E_PARAM, "null argument");
+    DSPX_REQUIRE(F && d_in && d_out && (d_zdm || !zdm_call), DSPX_E_PARAM, "null argument");
     DSPX_REQUIRE(F->have_resp, DSPX_E_STATE, "no response: dspx_fb_set_response");
     uint64_t npart, nout;
     dspx_fb_geometry(F, ndat, &npart, &nout);
@@ -256,21 +279,26 @@ int dspx_fb_perform(dspx_fb* F, const float* d_in, uint64_t ics, uint64_t ips, u
     // environment asks for it (DSPX_FB_FORM=library).
     const char* form = std::getenv("DSPX_FB_FORM");
     const bool library = form && std::string(form) == "library";
-#define FB_FUSED(n, m)                                                                                   \
-  if (!library && F->nfft == n && d.fres == m) {                                                         \
-    k_fb_fused<n, m><<<unsigned(nunit), n / 16, 0, st>>>(d_in, ics, ips, d.npol, F->in_step,             \
-                                                         uint32_t(npart), F->resp.as<float2>(),          \
-                                                         twiddles(F->ctx, n), twiddles(F->ctx, m),       \
-                                                         F->nkeep, d.nfilt_pos, d_out, ocs, ops);        \
-    DSPX_HIP(hipGetLastError());                                                                         \
-    return;                                                                                              \
+#define FB_FUSED(n, m)                                                                                        \
+  if (!library && F->nfft == n && d.fres == m) {                                                              \
+    if (d_zdm)                                                                                                \
+      k_fb_fused<n, m, true><<<unsigned(nunit), n / 16, 0, st>>>(                                            \
+          d_in, ics, ips, d.npol, F->in_step, uint32_t(npart), F->resp.as<float2>(), twiddles(F->ctx, n),     \
+          twiddles(F->ctx, m), F->nkeep, d.nfilt_pos, d_out, ocs, ops, d_zdm);                                \
+    else                                                                                                      \
+      k_fb_fused<n, m><<<unsigned(nunit), n / 16, 0, st>>>(d_in, ics, ips, d.npol, F->in_step,                \
+                                                           uint32_t(npart), F->resp.as<float2>(),             \
+                                                           twiddles(F->ctx, n), twiddles(F->ctx, m),          \
+                                                           F->nkeep, d.nfilt_pos, d_out, ocs, ops, nullptr);  \
+    DSPX_HIP(hipGetLastError());                                                                              \
+    return;                                                                                                   \
   }
     FB_FUSED(1024, 32) FB_FUSED(1024, 64) FB_FUSED(1024, 128) FB_FUSED(1024, 256) FB_FUSED(1024, 512)
     FB_FUSED(2048, 32) FB_FUSED(2048, 64) FB_FUSED(2048, 128) FB_FUSED(2048, 256) FB_FUSED(2048, 512)
     FB_FUSED(4096, 32) FB_FUSED(4096, 64) FB_FUSED(4096, 128) FB_FUSED(4096, 256) FB_FUSED(4096, 512)
 #undef FB_FUSED
     if (!library && F->fb4) {
-      fb4_perform(F->fb4, d_in, ics, ips, npart, d_out, ocs, ops);
+      fb4_perform(F->fb4, d_in, ics, ips, npart, d_out, ocs, ops, d_zdm);
       return;
     }
 
@@ -305,26 +333,49 @@ int dspx_fb_perform(dspx_fb* F, const float* d_in, uint64_t ics, uint64_t ips, u
     F->scratch.reserve(batch * per);
     F->last_batches = uint32_t(nbat);
     float2* s = F->scratch.as<float2>();
+    auto* sc = reinterpret_cast<hipfftComplex*>(s);
+    // each batch's windows transformed forward; then the product with the
+    // response, the backward transforms and the discard into d_out; with a
+    // zero-DM output the windows are transformed again and their sub-bands
+    // transformed back without the product, into d_zdm
     for (uint64_t u0 = 0; u0 < nunit; u0 += batch) {
       const uint64_t nb = std::min(batch, nunit - u0);
-      if (nb != batch)  // the unused rows of the last batch stay finite
-        DSPX_HIP(hipMemsetAsync(s + nb * F->nfft, 0, (batch - nb) * per, st));
-      dim3 g0(unsigned((F->nfft + 511) / 512), unsigned(nb));
-      k_fb_windows<<<g0, 256, 0, st>>>(d_in, ics, ips, d.npol, F->nfft, F->in_step, uint32_t(npart), u0, s);
-      dim3 g1(unsigned((F->nfft + 255) / 256), unsigned(nb));
-      DSPX_HIP(hipGetLastError());
-      DSPX_FB_FFT(hipfftExecC2C(F->fwd, reinterpret_cast<hipfftComplex*>(s), reinterpret_cast<hipfftComplex*>(s),
-                                HIPFFT_FORWARD));
-      k_fb_multiply<<<g1, 256, 0, st>>>(s, F->resp.as<float2>(), F->nfft, d.npol, uint32_t(npart), u0);
-      DSPX_HIP(hipGetLastError());
-      DSPX_FB_FFT(hipfftExecC2C(F->bwd, reinterpret_cast<hipfftComplex*>(s), reinterpret_cast<hipfftComplex*>(s),
-                                HIPFFT_BACKWARD));
-      dim3 g2(static_cast<unsigned>(nkb), static_cast<unsigned>(nb));
-      k_fb_discard<<<g2, 256, 0, st>>>(s, F->nfft, d.fres, d.nsub, d.npol, uint32_t(npart), u0, F->nkeep,
-                                       d.nfilt_pos, d_out, ocs, ops);
-      DSPX_HIP(hipGetLastError());
+      for (int pass = 0; pass < (d_zdm ? 2 : 1); pass++) {
+        if (nb != batch)  // the unused rows of the last batch stay finite
+          DSPX_HIP(hipMemsetAsync(s + nb * F->nfft, 0, (batch - nb) * per, st));
+        dim3 g0(unsigned((F->nfft + 511) / 512), unsigned(nb));
+        k_fb_windows<<<g0, 256, 0, st>>>(d_in, ics, ips, d.npol, F->nfft, F->in_step, uint32_t(npart), u0, s);
+        DSPX_HIP(hipGetLastError());
+        DSPX_FB_FFT(hipfftExecC2C(F->fwd, sc, sc, HIPFFT_FORWARD));
+        if (pass == 0) {
+          dim3 g1(unsigned((F->nfft + 255) / 256), unsigned(nb));
+          k_fb_multiply<<<g1, 256, 0, st>>>(s, F->resp.as<float2>(), F->nfft, d.npol, uint32_t(npart), u0);
+          DSPX_HIP(hipGetLastError());
+        }
+        DSPX_FB_FFT(hipfftExecC2C(F->bwd, sc, sc, HIPFFT_BACKWARD));
+        dim3 g2(static_cast<unsigned>(nkb), static_cast<unsigned>(nb));
+        k_fb_discard<<<g2, 256, 0, st>>>(s, F->nfft, d.fres, d.nsub, d.npol, uint32_t(npart), u0, F->nkeep,
+                                         d.nfilt_pos, pass == 0 ? d_out : d_zdm, ocs, ops);
+        DSPX_HIP(hipGetLastError());
+      }
     }
   });
+}
+
+}  // namespace
+
+extern "C" {
+
+int dspx_fb_perform(dspx_fb* F, const float* d_in, uint64_t ics, uint64_t ips, uint64_t ndat, float* d_out,
+                    uint64_t ocs, uint64_t ops)
+{
+  return fb_perform(F, d_in, ics, ips, ndat, d_out, ocs, ops, nullptr, false);
+}
+
+int dspx_fb_perform_zdm(dspx_fb* F, const float* d_in, uint64_t ics, uint64_t ips, uint64_t ndat, float* d_out,
+                        uint64_t ocs, uint64_t ops, float* d_zdm)
+{
+  return fb_perform(F, d_in, ics, ips, ndat, d_out, ocs, ops, d_zdm, true);
 }
 
 int dspx_fb_batches(const dspx_fb* F, uint32_t* nbatch)

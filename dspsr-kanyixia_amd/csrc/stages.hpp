@@ -439,7 +439,9 @@ __global__ __launch_bounds__(N2 / 16) void k_stageB_fb(ConvShape s, ChirpArgs c,
   const uint32_t p_end = min(s.npart, p_begin + ppg);
   const uint64_t plane = uint64_t(s.N1) * N2;
 
-  for (int k2 = tid; k2 < N2; k2 += NT) hlds[k2] = c.table[(uint64_t(chan) * s.N1 + k1) * N2 + k2];
+  // a null table: the unit response (the filterbank's zero-DM output)
+  for (int k2 = tid; k2 < N2; k2 += NT)
+    hlds[k2] = c.table ? c.table[(uint64_t(chan) * s.N1 + k1) * N2 + k2] : make_float2(1.f, 0.f);
   // inter-stage twiddles W_N^{n2 k1} of the loaded elements (n2 = tid + q*NT
   // + r*N2/RF), and the output twiddles conj(W_N^{nsub k1 ob}) of the stored
   // ones (n2 = tid + e*NT, ob = n2 % F2), both fixed for the workgroup

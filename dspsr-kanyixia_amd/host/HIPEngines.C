@@ -144,9 +144,14 @@ void FilterbankEngine::perform(const TimeSeries* in, TimeSeries* out, TimeSeries
                                uint64_t in_step, uint64_t out_step)
 {
   const char* where = "dsp::HIP::FilterbankEngine::perform";
-  if (zdm) throw Error(InvalidState, where, "zero-DM output is not on this path");
   require_device(in->get_datptr(), in->get_memory()->on_device, where);
   require_device(out->get_datptr(), out->get_memory()->on_device, where);
+  if (zdm) {
+    require_device(zdm->get_datptr(), zdm->get_memory()->on_device, where);
+    // the library writes the zero-DM output with the output's strides
+    if (zdm->chan_span() != out->chan_span() || zdm->get_pol_span() != out->get_pol_span())
+      throw Error(InvalidParam, where, "zero-DM output layout differs from the output's");
+  }
   const uint64_t nkeep = desc.fres - desc.nfilt_pos - desc.nfilt_neg;
   if (in_step != 2 * uint64_t(desc.nsub) * nkeep || out_step != 2 * nkeep)
     throw Error(InvalidParam, where, "in_step=%llu out_step=%llu do not match nsub=%u nkeep=%llu",
@@ -160,9 +165,14 @@ void FilterbankEngine::perform(const TimeSeries* in, TimeSeries* out, TimeSeries
   const uint64_t nfft = uint64_t(desc.nsub) * desc.fres;
   const uint64_t span = (npart - 1) * (in_step / 2) + nfft;
   if (span > in->get_ndat()) throw Error(InvalidParam, where, "npart=%llu exceeds the input", (unsigned long long) npart);
-  ctx->check(dspx_fb_perform(fb, in->get_datptr(), in->chan_span(), in->get_pol_span(), span, out->get_datptr(),
-                             out->chan_span(), out->get_pol_span()),
-             where);
+  if (zdm)
+    ctx->check(dspx_fb_perform_zdm(fb, in->get_datptr(), in->chan_span(), in->get_pol_span(), span,
+                                   out->get_datptr(), out->chan_span(), out->get_pol_span(), zdm->get_datptr()),
+               where);
+  else
+    ctx->check(dspx_fb_perform(fb, in->get_datptr(), in->chan_span(), in->get_pol_span(), span, out->get_datptr(),
+                               out->chan_span(), out->get_pol_span()),
+               where);
 }
 
 // ------------------------------------------------- SpectralKurtosisEngine

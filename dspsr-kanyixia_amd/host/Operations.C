@@ -189,8 +189,16 @@ void Filterbank::operate()
   output->set_rate(input->get_rate() / nchan_subband);  // Filterbank.C:356
   output->resize(npart * nkeep);
   output->change_start_time(nfilt_pos);  // Filterbank.C:387
+  if (zero_DM) {  // Filterbank.C:388-396
+    if (!zero_DM_output) throw Error(InvalidState, "dsp::Filterbank::operate", "zero_DM without output");
+    zero_DM_output->copy_configuration(output);
+    zero_DM_output->resize(output->get_ndat());
+  }
   if (!npart) return;
-  engine->perform(input, output, npart, uint64_t(nsamp_step) * 2, uint64_t(nkeep) * 2);
+  if (zero_DM)
+    engine->perform(input, output, zero_DM_output, npart, uint64_t(nsamp_step) * 2, uint64_t(nkeep) * 2);
+  else
+    engine->perform(input, output, npart, uint64_t(nsamp_step) * 2, uint64_t(nkeep) * 2);
   engine->finish();
 }
 

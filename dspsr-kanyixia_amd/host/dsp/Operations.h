@@ -162,6 +162,11 @@ class Filterbank {
   void set_engine(Engine* e) { engine.reset(e); }
   const TimeSeries* get_input() const { return input; }
   TimeSeries* get_output() const { return output; }
+  //! the zero-DM output: the sub-bands without the response, shaped as the
+  //! output (Filterbank.C:388-396, 549-555; set by LoadToFold1.C:291-303)
+  void set_zero_DM(bool z) { zero_DM = z; }
+  void set_zero_DM_output(TimeSeries* z) { zero_DM_output = z; }
+  bool get_zero_DM() const { return zero_DM; }
 
   //! Filterbank::do_prepare (Filterbank.C:75-200)
   void prepare();
@@ -188,7 +193,8 @@ class Filterbank {
   Response* response = nullptr;
   std::unique_ptr<Engine> engine;
   std::vector<float> table;
-  bool prepared = false, table_changed = true;
+  TimeSeries* zero_DM_output = nullptr;
+  bool prepared = false, table_changed = true, zero_DM = false;
   unsigned nchan = 0, nchan_subband = 0, freq_res = 0, nfilt_pos = 0, nfilt_neg = 0;
   unsigned nsamp_fft = 0, nsamp_overlap = 0, nsamp_step = 0;
   uint64_t npart = 0;

@@ -103,12 +103,16 @@ try {
   if (mode == "filterbank") {
     // Unpacker -> Filterbank[HIP::FilterbankEngine] (FilterbankConfig.C:115-133);
     // the output is the FPT time series [nchan_out][npol][ndat_out][2]
-    TimeSeries unpacked(Memory::device()), channels(Memory::device());
+    TimeSeries unpacked(Memory::device()), channels(Memory::device()), zdm(Memory::device());
     HIP::Unpacker unpacker(&ctx);
     unpacker.unpack(&raw, &unpacked);
     Filterbank filterbank;
     filterbank.set_input(&unpacked);
     filterbank.set_output(&channels);
+    if (a.count("zdm")) {  // the zero-DM output too (LoadToFold1.C:291-303)
+      filterbank.set_zero_DM(true);
+      filterbank.set_zero_DM_output(&zdm);
+    }
     unsigned nchan_out = std::stoul(need("fb_nchan"));
     filterbank.set_nchan(nchan_out);
     TableResponse table;
@@ -127,15 +131,20 @@ try {
     filterbank.set_engine(new HIP::FilterbankEngine(&ctx));
     filterbank.operate();
     ctx.synchronize();
-    const size_t per = size_t(channels.get_ndat()) * 2;
-    std::vector<float> h(size_t(nchan_out) * npol * per);
-    for (unsigned c = 0; c < nchan_out; c++)
-      for (unsigned p = 0; p < npol; p++)
-        if (per && hipMemcpy(&h[(size_t(c) * npol + p) * per], channels.get_datptr(c, p), per * sizeof(float),
-                             hipMemcpyDeviceToHost) != hipSuccess)
-          throw Error(FailedCall, "dspx_pipeline", "hipMemcpy");
-    std::ofstream o(need("out"), std::ios::binary);
-    o.write(reinterpret_cast<const char*>(h.data()), std::streamsize(h.size() * sizeof(float)));
+    // [nchan_out][npol][ndat_out][2] floats of a device time series to a file
+    auto save = [&](const TimeSeries& ts, const std::string& path) {
+      const size_t per = size_t(ts.get_ndat()) * 2;
+      std::vector<float> h(size_t(nchan_out) * npol * per);
+      for (unsigned c = 0; c < nchan_out; c++)
+        for (unsigned p = 0; p < npol; p++)
+          if (per && hipMemcpy(&h[(size_t(c) * npol + p) * per], ts.get_datptr(c, p), per * sizeof(float),
+                               hipMemcpyDeviceToHost) != hipSuccess)
+            throw Error(FailedCall, "dspx_pipeline", "hipMemcpy");
+      std::ofstream o(path, std::ios::binary);
+      o.write(reinterpret_cast<const char*>(h.data()), std::streamsize(h.size() * sizeof(float)));
+    };
+    save(channels, need("out"));
+    if (a.count("zdm")) save(zdm, a["zdm"]);
     std::cout << "dspx_pipeline filterbank: " << nchan_out << " channels x " << channels.get_ndat() << " samples"
               << std::endl;
     return 0;

@@ -259,6 +259,17 @@ int dspx_fb_geometry(const dspx_fb* fb, uint64_t ndat, uint64_t* npart, uint64_t
    output [nchan*nsub][npol][npart*nkeep]. */
 int dspx_fb_perform(dspx_fb* fb, const float* d_in, uint64_t in_chan_stride, uint64_t in_pol_stride,
                     uint64_t ndat, float* d_out, uint64_t out_chan_stride, uint64_t out_pol_stride);
+/* dspx_fb_perform with the zero-DM output as well: d_zdm (same layout and
+   strides as d_out) receives the sub-bands of each window's spectrum
+   transformed back without the response (no zero-DM response set, as in
+   LoadToFold::construct for a filterbank that convolves during,
+   LoadToFold1.C:291-303), for the spectral kurtosis of undispersed data.
+   Replaces Filterbank::Engine::perform(in, out, zero_DM_out, npart,
+   in_step, out_step) (Filterbank.C:549-555; FilterbankEngineCPU.C:266-300,
+   FilterbankEngineCUDA.cu:329-344). */
+int dspx_fb_perform_zdm(dspx_fb* fb, const float* d_in, uint64_t in_chan_stride, uint64_t in_pol_stride,
+                        uint64_t ndat, float* d_out, uint64_t out_chan_stride, uint64_t out_pol_stride,
+                        float* d_zdm);
 /* The number of FFT-library batches the last dspx_fb_perform ran (0 when it
    ran the one-kernel or the four-step form); for checking the batch loop. */
 int dspx_fb_batches(const dspx_fb* fb, uint32_t* nbatch);

@@ -10,18 +10,23 @@ from helpers import from_dev, rel_rms, to_dev
 pytestmark = pytest.mark.gpu
 
 
-def _run(dspx, ctx, x, nsub, fres, pos, neg, resp, batches=None):
+def _run(dspx, ctx, x, nsub, fres, pos, neg, resp, batches=None, zdm=False):
+    """The engine's output; with zdm, (output, zero-DM output) of one
+    dspx_fb_perform_zdm call."""
     import torch
     nchan, npol, ndat = x.shape
     F = dspx.Filterbank(ctx, nchan, npol, nsub, fres, pos, neg)
     F.set_response(resp)
     npart, nout = F.geometry(ndat)
     out = torch.zeros((nchan * nsub, npol, nout, 2), dtype=torch.float32, device="cuda")
-    F.perform(to_dev(x.view(np.float32)), npol * ndat * 2, ndat * 2, ndat, out, npol * nout * 2, nout * 2)
+    z = torch.zeros_like(out) if zdm else None
+    F.perform(to_dev(x.view(np.float32)), npol * ndat * 2, ndat * 2, ndat, out, npol * nout * 2, nout * 2, zdm=z)
     got = from_dev(out).view(np.complex64)[..., 0]
     if batches is not None:
         batches.append(F.batches())
     F.close()
+    if zdm:
+        return got, from_dev(z).view(np.complex64)[..., 0]
     return got
 
 
@@ -153,3 +158,37 @@ def test_dedispersing_filterbank_long_windows(ctx, dspx, oracle):
     e = rel_rms(got, expect)
     print("rel rms", e, flush=True)
     assert e < 1e-5
+
+
+@pytest.mark.parametrize("form,nsub,fres,nchan", [("fused", 16, 128, 2), ("library", 16, 128, 2),
+                                                  ("fused", 4, 256, 1), ("four-step", 16, 512, 2),
+                                                  ("four-step", 8, 32768, 1), ("library", 8, 2048, 1)])
+def test_filterbank_zero_dm_output(ctx, dspx, oracle, monkeypatch, form, nsub, fres, nchan):
+    """The zero-DM output (FilterbankEngineCPU.C:266-300: each window's
+    sub-bands transformed back without the response, no zero-DM response
+    set) in every form: against the oracle's filterbank with a unit response,
+    while the dedispersed output of the same call stays that of the plain
+    perform."""
+    if form == "library":
+        monkeypatch.setenv("DSPX_FB_FORM", "library")
+    else:
+        monkeypatch.delenv("DSPX_FB_FORM", raising=False)
+    rng = np.random.default_rng(nsub * 7 + fres)
+    npol, pos, neg = 2, 11, 6
+    nfft = nsub * fres
+    resp = (np.exp(1j * rng.uniform(0, 2 * np.pi, (nchan, nfft))) / np.sqrt(nfft * fres)).astype(np.complex64)
+    ndat = 3 * nfft + 57
+    x = (rng.standard_normal((nchan, npol, ndat)) + 1j * rng.standard_normal((nchan, npol, ndat))).astype(np.complex64)
+    expect = oracle.filterbank(x, nsub, fres, pos, neg, resp)
+    expect_z = oracle.filterbank(x, nsub, fres, pos, neg, np.ones((nchan, nfft), np.complex64))
+    print("filterbank zero-DM", form, nsub, fres, nchan, flush=True)
+    nb = []
+    got, got_z = _run(dspx, ctx, x, nsub, fres, pos, neg, resp, nb, zdm=True)
+    assert (nb[0] > 0) == (form == "library")
+    plain = _run(dspx, ctx, x, nsub, fres, pos, neg, resp)
+    assert got_z.shape == expect_z.shape == got.shape
+    e, ez, ep = rel_rms(got, expect), rel_rms(got_z, expect_z), rel_rms(got, plain)
+    print("rel rms: output %.3g, zero-DM %.3g, output vs plain perform %.3g" % (e, ez, ep), flush=True)
+    assert e < 1e-5 and ez < 1e-5 and ep < 1e-6
+    # not the dedispersed output: the response changes every sample
+    assert rel_rms(got_z / np.sqrt(nfft * fres), got) > 0.5

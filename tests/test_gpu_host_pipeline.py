@@ -55,7 +55,9 @@ def test_host_filterbank(dspx, oracle, tmp_path, with_response):
     """Unpacker -> Filterbank[HIP::FilterbankEngine] through the C++ host
     layer, against the oracle's FilterbankEngineCPU restatement on the same
     unpacked samples: 2 input channels x 16 sub-bands of 128 bins, with no
-    response (the normaliser alone) and with a response table."""
+    response (the normaliser alone) and with a response table; with the
+    response, the zero-DM output too (the sub-bands without the response or
+    the normaliser)."""
     assert os.path.exists(EXE), "dspx_pipeline not built"
     rng = np.random.default_rng(17)
     nchan, npol, nsub, fres = 2, 2, 16, 128
@@ -74,7 +76,8 @@ def test_host_filterbank(dspx, oracle, tmp_path, with_response):
         table = np.exp(1j * rng.uniform(0, 2 * np.pi, (nchan * nsub, fres))).astype(np.complex64)
         respf = tmp_path / "resp.bin"
         table.tofile(respf)
-        cmd += ["--response", str(respf), "--nfilt_pos", str(pos), "--nfilt_neg", str(neg)]
+        zdmf = tmp_path / "zdm.bin"
+        cmd += ["--response", str(respf), "--nfilt_pos", str(pos), "--nfilt_neg", str(neg), "--zdm", str(zdmf)]
         resp = (table.view(np.float32) * norm).view(np.complex64).reshape(nchan, nfft)
     else:
         resp = np.full((nchan, nfft), norm, np.complex64)
@@ -88,6 +91,12 @@ def test_host_filterbank(dspx, oracle, tmp_path, with_response):
     e = rel_rms(got, expect)
     print("filterbank rel rms %.3g" % e)
     assert e < 1e-5
+    if with_response:
+        expect_z = oracle.filterbank(x, nsub, fres, pos, neg, np.ones((nchan, nfft), np.complex64))
+        got_z = np.fromfile(zdmf, np.float32).view(np.complex64).reshape(expect_z.shape)
+        ez = rel_rms(got_z, expect_z)
+        print("zero-DM rel rms %.3g" % ez)
+        assert ez < 1e-5
 
 
 def test_host_pipeline_interchan(dspx, oracle, tmp_path):
