@@ -1048,7 +1048,15 @@ void ltf_reserve(dspx_ltf* L)
   if (d.form == DSPX_FORM_PLAIN || !L->npart) return;
   uint64_t wbytes = uint64_t(d.npol) * 2 * d.nbit / 8;
   if (d.nchan > 1) L->sc().ftp.reserve(d.ndat * d.nchan * wbytes);
-  if (L->have_fold) L->sc().partial2.reserve(uint64_t(d.nchan) * L->npart * L->nslot * L->nval * d.nbin * sizeof(float));
+  // the per-part reduction of stage C's column groups (fold_finish): the
+  // three-stage form with more than one group per part only (one-pass
+  // workgroups write per-unit partials that need no such pass)
+  if (L->have_fold && !L->onepass) {
+    const uint64_t per = uint64_t(d.nchan) * L->npart * L->nslot * d.nbin * sizeof(float);
+    uint64_t need = L->G > 1 ? per * L->nval : 0;
+    if (L->zh_ok && L->G_zh > 1) need = std::max<uint64_t>(need, per * (L->nval + 1));
+    if (need) L->sc().partial2.reserve(need);
+  }
 }
 
 
@@ -1703,9 +1711,12 @@ double time_per_sample(dspx_batch* B, const dspx_ltf_desc& dn, double* block_s =
 }
 
 // Candidate lengths: the reference's choice and the next three powers of two
-// (longer transforms waste less on the overlap, shorter ones keep the
-// transforms in their fastest shapes), within the fused forms' range; each
-// runs in the form the plan picks for it (one pass where it fits).
+// (longer transforms waste less on the overlap), and, where the reference's
+// choice runs in one pass, the shorter one-pass lengths whose step is at
+// least the overlap (a 256-point window runs about 1.7 times as many samples
+// per second through the one-pass kernel as a 512-point one: two passes per
+// transform against three), within the fused forms' range; each runs in the
+// form the plan picks for it (one pass where it fits).
 // block_s: the block's time at the chosen length (0 if none was timed).
 uint32_t autotune_nfft(dspx_batch* B, const dspx_ltf_desc& d0, double& block_s)
 {
@@ -1713,7 +1724,10 @@ uint32_t autotune_nfft(dspx_batch* B, const dspx_ltf_desc& d0, double& block_s)
   uint32_t best = ref.nfft;
   double best_t = 1e30;
   block_s = 0;
-  for (uint32_t n = ref.nfft; n <= (1u << 22) && n <= 8 * ref.nfft; n *= 2) {
+  const uint64_t nov = uint64_t(ref.nfilt_pos) + ref.nfilt_neg;
+  uint32_t lo = ref.nfft;
+  while (lo <= (1u << kOnePassMaxLog2) && lo / 2 >= (1u << kOnePassMinLog2) && lo / 2 >= 2 * nov) lo /= 2;
+  for (uint32_t n = lo; n <= (1u << 22) && n <= 8 * ref.nfft; n *= 2) {
     dspx_ltf_desc d = with_geometry(d0, n);
     if (d.nfilt_pos + d.nfilt_neg >= n || d.ndat < n) continue;
     double bs = 0;
