@@ -317,6 +317,9 @@ def run_uneven(args, dspx, synth, ctx):
         profs.append(torch.zeros((1, b["nchan"], 4 * b["nbin"]), dtype=torch.float32, device="cuda"))
         hits.append(torch.zeros((1, b["nbin"]), dtype=torch.int32, device="cuda"))
     B.execute(raws, profs, hits)
+    free, total = torch.cuda.mem_get_info()
+    print("[bench] batch ready: %d streams, device memory %.1f of %.1f GB in use" % (
+        args.branches, (total - free) / 1e9, total / 1e9), file=sys.stderr, flush=True)
     B.capture(raws, profs, hits)
     for _ in range(args.warmup):
         B.replay()
@@ -639,8 +642,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-mfma", action="store_true", help="skip the matrix-core stage B timing (uneven)")
-    ap.add_argument("--branches", type=int, default=2,
-                    help="streams the uneven batch's beams are split over (3 exceed the HBM with 94 GB of input)")
+    ap.add_argument("--branches", type=int, default=4,
+                    help="streams the uneven batch's beams are split over (4: one per hardware queue; 6 and 8 "
+                         "measure the same)")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB")
     args = ap.parse_args()
