@@ -141,14 +141,20 @@ template <int L, int B, int RS, int NT> struct ColLayout {
   __device__ static __forceinline__ int idx(int n, int b) { return n * RS + b; }
 };
 
-// The same with the two low column bits XOR-ed by bits 4-5 of the column:
-// a permutation within each row that keeps consecutive columns on distinct
-// banks and also spreads columns read with stride 4 (b = 4k + q, k < 16)
-// over 16 distinct bank pairs.
+// The same with the low four column bits (the low log2 B, for B < 16)
+// XOR-ed by a key of the row and of column bits 5 and up: a permutation of
+// each aligned group of 16 columns (the key is constant over the group), so
+// that 16 consecutive columns of a row still fall on 16 distinct bank pairs,
+// while a read of column q of each lane's group of 16 consecutive columns
+// (the fold's regrouping: lane l reads row j(l), column 16 l' + q, with
+// B/16 lanes l' per row) puts the 16 lanes of a DPP row on 16 distinct bank
+// pairs: lanes of one row differ in (l' & 1, l' >> 1), those of the next
+// rows are offset by M = B/32 in the key (float2 = two of the 64 banks).
 template <int L, int B, int RS, int NT> struct ColLayoutSwz {
-  static_assert(B % 4 == 0, "swizzle permutes aligned groups of 4 columns");
   static constexpr int E = L * B / NT;
-  __device__ static __forceinline__ int idx(int n, int b) { return n * RS + (b ^ ((b >> 4) & 3)); }
+  static constexpr int M = B >= 64 ? B / 32 : 1;
+  static constexpr int KM = (B < 16 ? B : 16) - 1;
+  __device__ static __forceinline__ int idx(int n, int b) { return n * RS + (b ^ (((b >> 5) + n * M) & KM)); }
 };
 
 // Row layout (B = 1): element n at n + (n >> 4).

@@ -956,7 +956,8 @@ __global__ __launch_bounds__(kColThreads) void k_stageC(ConvShape s, FilterbankA
   constexpr int E = kColTile / kColThreads;
   constexpr int RI = FirstRadix<N1, 16, +1>::v;
   constexpr int RL = LastRadix<N1, 16, +1>::v;
-  // swizzled: the last inverse pass reads columns with stride E/RL
+  // swizzled: the fold's regrouping reads each lane's 16 consecutive
+  // columns without bank conflicts
   using LAY = ColLayoutSwz<N1, C, C, kColThreads>;
   extern __shared__ float2 smem[];
   float2* lds = smem;
