@@ -296,11 +296,11 @@ struct ConvCore {
     if (MODE == kStageCFold) shm += size_t(fa.nslot) * fa.hrow * sizeof(float);
     DSPX_REQUIRE(shm <= 160 * 1024, DSPX_E_UNSUPPORTED, "LDS request %zu too large", shm);
     const float2* twc = twiddles(ctx, N1);
-    // The next tile's prefetch (64 registers) where it costs no occupancy:
-    // always for the single-pass 16-point columns; at 32 and 64 points only
-    // where the histogram already holds the CU to two workgroups (without
-    // it those kernels run three waves per SIMD)
-    const bool pf = MODE == kStageCFold && !ZH && (N1 <= 16 || (N1 <= 64 && 3 * shm > 160 * 1024));
+    // The next tile's prefetch (64 registers) only where it costs no
+    // occupancy: where the LDS already holds the CU to two workgroups (the
+    // prefetching kernels take 178-210 VGPRs, two waves per SIMD; without it
+    // 114-146, three or four)
+    const bool pf = MODE == kStageCFold && !ZH && N1 <= 64 && 3 * shm > 160 * 1024;
     switch (N1) {
 #define LAUNCH_C(n, nv, p)                                                                      \
   do {                                                                                          \

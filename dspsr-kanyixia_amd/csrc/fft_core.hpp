@@ -236,6 +236,59 @@ __device__ __forceinline__ void stockham_compute(float2* v, const float2* __rest
   }
 }
 
+// The twiddles stockham_compute applies in a pass (R, NS), for each of the
+// thread's E/R butterflies q: w[q*R + r], r >= 1 (w[q*R] unused), computed
+// from the same table entries and products, so that a pass applied with
+// them (stockham_apply) gives the same values.
+template <int L, int B, int NT, int R, int NS>
+__device__ __forceinline__ void pass_twiddles(const float2* __restrict__ tw, int tid, float2* wq)
+{
+  constexpr int E = L * B / NT;
+  static_assert(NS > 1, "the first pass has no twiddles");
+#pragma unroll
+  for (int q = 0; q < E / R; q++) {
+    int g = tid + q * NT;
+    int k = (g / B) % NS;
+    float2* w = wq + q * R;
+    w[0] = make_float2(1.f, 0.f);
+    if constexpr (R >= 8 && B == 1 && NS >= 8) {
+      constexpr int S = NS * R;
+      const float4* rec = reinterpret_cast<const float4*>(tw - L + kTwPowBase) + 2 * (S / 2 + k);
+      float4 a = rec[0], b = rec[1];
+      w[1] = make_float2(a.x, a.y);
+      w[2] = make_float2(a.z, a.w);
+      w[4] = make_float2(b.x, b.y);
+      w[3] = cmul(w[2], w[1]);
+      w[5] = cmul(w[4], w[1]);
+      w[6] = cmul(w[4], w[2]);
+      w[7] = cmul(w[4], w[3]);
+      if constexpr (R == 16) {
+        w[8] = make_float2(b.z, b.w);
+#pragma unroll
+        for (int r = 9; r < 16; r++) w[r] = cmul(w[8], w[r - 8]);
+      }
+    } else {
+#pragma unroll
+      for (int r = 1; r < R; r++) w[r] = tw[r * k * (L / (NS * R))];
+    }
+  }
+}
+
+// A Stockham pass (as stockham_compute) with its twiddles given (from
+// pass_twiddles; DIR > 0 applies their conjugates).
+template <int L, int B, int NT, int R, int DIR>
+__device__ __forceinline__ void stockham_apply(float2* v, const float2* wq)
+{
+  constexpr int E = L * B / NT;
+#pragma unroll
+  for (int q = 0; q < E / R; q++) {
+#pragma unroll
+    for (int r = 1; r < R; r++)
+      v[q * R + r] = DIR < 0 ? cmul(v[q * R + r], wq[q * R + r]) : cmulc(v[q * R + r], wq[q * R + r]);
+    dft<R, DIR>(v + q * R);
+  }
+}
+
 template <class LAY, int L, int B, int NT, int R>
 __device__ __forceinline__ void lds_load_pass(const float2* lds, float2* v, int tid)
 {
@@ -336,6 +389,25 @@ template <class LAY, int L, int B, int NT, int DIR>
 __device__ __forceinline__ void fft_regs(float2* lds, float2* v, const float2* __restrict__ tw, int tid)
 {
   run_passes<LAY, L, B, NT, DIR, 0, RadixPlan<L, L * B / NT>::n>(lds, v, tw, tid);
+}
+
+// A transform of two passes of one radix R (B = 1) whose second pass's
+// twiddles are given in registers (wq, from pass_twiddles<L, 1, NT, R, R>):
+// the forward and the inverse plan then share them, conjugated.  The same
+// operations, hence the same values, as fft_regs.
+template <class LAY, int L, int NT, int DIR>
+__device__ __forceinline__ void fft_regs_2w(float2* lds, float2* v, const float2* wq, const float2* __restrict__ tw,
+                                            int tid)
+{
+  constexpr int E = L / NT;
+  constexpr int R = RadixPlan<L, E>::r[0];
+  static_assert(RadixPlan<L, E>::n == 2 && RadixPlan<L, E>::r[1] == R, "two passes of one radix");
+  stockham_compute<L, 1, NT, R, 1, DIR>(v, tw, tid);  // NS = 1: no twiddles
+  xsync<NT>();
+  lds_store_pass<LAY, L, 1, NT, R, 1>(lds, v, tid);
+  xsync<NT>();
+  lds_load_pass<LAY, L, 1, NT, R>(lds, v, tid);
+  stockham_apply<L, 1, NT, R, DIR>(v, wq);
 }
 
 // Returns p unchanged, opaquely: loads through the result cannot be hoisted

@@ -111,14 +111,25 @@ __global__ __launch_bounds__(OnePass<N>::NT) void k_onepass(LD ld, OnePassArgs a
   __syncthreads();
   int tid = tid0;
   float2* lds = lds0;
+  // two passes of one radix (256 points): the second pass's twiddles, the
+  // same for every window (conjugated in the inverse), held in registers
+  // rather than rebuilt from the table per window and pol (a 512-channel
+  // beam: 16.6 -> 16.2 ms per block with 16-bit samples, 4.5 -> 4.3 ms with
+  // 2048 bins)
+  constexpr bool TW2 = RadixPlan<N, E>::n == 2 && RadixPlan<N, E>::r[0] == RadixPlan<N, E>::r[1];
+  float2 wq[TW2 ? E : 1];
+  if constexpr (TW2) pass_twiddles<N, 1, NTW, E, E>(tw, tid0, wq);
 
   // every slot runs the same number of iterations (the exchanges are
   // workgroup barriers); a slot past the group's last window transforms its
   // group's first window again and neither stores nor folds it
   const uint32_t niter = (w_end - w_begin + WF - 1) / WF;
   // the raw words of a window (both pols packed), in the forward plan's
-  // first-pass order; the next window's are loaded while this one is
-  // transformed
+  // first-pass order, loaded at the start of its iteration: loading the next
+  // window's during this one's transforms held 16-32 more registers, which
+  // cost the 16-bit kernels occupancy (one wave per SIMD from 512 points,
+  // with spills to AGPRs: a 2048-point beam 5.33 ms per block against 3.38
+  // without) and gained nothing for 8-bit samples
   using Word = typename LD::Word;
   auto load_words = [&](uint32_t it_, Word* wd) {
     const uint32_t w_ = w_begin + it_ * WF + uint32_t(slot);
@@ -128,13 +139,11 @@ __global__ __launch_bounds__(OnePass<N>::NT) void k_onepass(LD ld, OnePassArgs a
 #pragma unroll
       for (int r = 0; r < R0; r++) wd[q * R0 + r] = ld.word(chan, t_ + uint32_t(tid0 + q * NTW + r * (N / R0)));
   };
-  Word wcur[E];
-  if (niter) load_words(0, wcur);
   for (uint32_t it = 0; it < niter; it++) {
     const uint32_t w = w_begin + it * WF + uint32_t(slot);
     const bool valid = w < w_end;
-    Word wnext[E];
-    if (it + 1 < niter) load_words(it + 1, wnext);
+    Word wcur[E];
+    load_words(it, wcur);
     // the lane's index arithmetic (LDS addresses of every pass) is redone
     // per window rather than held in registers across the loop
 #ifndef DSPX_OP_REINDEX
@@ -150,14 +159,20 @@ __global__ __launch_bounds__(OnePass<N>::NT) void k_onepass(LD ld, OnePassArgs a
       for (int e = 0; e < E; e++) v[e] = ld.pol(wcur[e], pol);
       // opaque: the pass twiddles are reloaded per window rather than held
       // in registers across the loop (registers go to occupancy)
-      fft_regs<LAY, N, 1, NTW, -1>(lds, v, opaque(tw), tid);
+      if constexpr (TW2)
+        fft_regs_2w<LAY, N, NTW, -1>(lds, v, wq, tw, tid);
+      else
+        fft_regs<LAY, N, 1, NTW, -1>(lds, v, opaque(tw), tid);
       // X[j + r*N/RL] (j = tid + q*NTW) times the chirp of its bin
 #pragma unroll
       for (int q = 0; q < E / RL; q++)
 #pragma unroll
         for (int r = 0; r < RL; r++)
           v[q * RL + r] = cmul(v[q * RL + r], S::CH_LDS ? chl[tid + q * NTW + r * (N / RL)] : ch[q * RL + r]);
-      fft_regs<LAY, N, 1, NTW, +1>(lds, v, opaque(tw), tid);
+      if constexpr (TW2)
+        fft_regs_2w<LAY, N, NTW, +1>(lds, v, wq, tw, tid);
+      else
+        fft_regs<LAY, N, 1, NTW, +1>(lds, v, opaque(tw), tid);
       // v[q*R0 + r] = y[tid + q*NTW + r*N/R0]
       if constexpr (MODE == kStageCFilterbank) {
         if (valid) {
@@ -209,9 +224,6 @@ __global__ __launch_bounds__(OnePass<N>::NT) void k_onepass(LD ld, OnePassArgs a
         fold_lane<NV, QF, ZH>(hist, fa, a.npol, kq, y0 + g * QF, b, chan, i0);
       }
     }
-    if (it + 1 < niter)
-#pragma unroll
-      for (int e = 0; e < E; e++) wcur[e] = wnext[e];
   }
   if constexpr (MODE == kStageCFold) {
     __syncthreads();
