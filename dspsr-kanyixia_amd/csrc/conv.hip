@@ -1017,12 +1017,13 @@ dspx_ltf* ltf_create(dspx_ctx* ctx, const dspx_ltf_desc* d, LtfScratch* ext)
       } else {
         DSPX_REQUIRE(d->form == DSPX_FORM_FUSED || d->form == DSPX_FORM_TWOLEVEL || d->form == DSPX_FORM_ONEPASS,
                      DSPX_E_PARAM, "form=%u", d->form);
-        // the fused form: one kernel where the transform and the fold
-        // histogram fit the LDS, else the three stages
+        // the fused form: one kernel up to 4096 points where the transform
+        // and the fold histogram fit the LDS, else the three stages
         const bool fits = onepass_fits(d->nfft, L->nval, d->nbin);
         DSPX_REQUIRE(d->form != DSPX_FORM_ONEPASS || fits, DSPX_E_UNSUPPORTED,
                      "nfft=%u with %u x %u fold values: outside the one-pass form", d->nfft, L->nval, d->nbin);
-        L->onepass = d->form == DSPX_FORM_ONEPASS || (d->form == DSPX_FORM_FUSED && fits);
+        L->onepass = d->form == DSPX_FORM_ONEPASS ||
+                     (d->form == DSPX_FORM_FUSED && fits && d->nfft <= (1u << kOnePassFusedMaxLog2));
         L->core.init(ctx, d->nchan, d->npol, d->nfft, d->nfilt_pos, d->nfilt_neg, d->chirp_mode, d->dedisp,
                      L->onepass);
         // largest unpacked component (LoadI8: |(v + 0.5) * scale| <= 127.5 |scale|; LoadI16: 32767.5)

@@ -43,6 +43,12 @@ template <int N> struct OnePass {
 };
 
 constexpr int kOnePassMinLog2 = 7, kOnePassMaxLog2 = 13;
+// The longest transform the fused form (DSPX_FORM_FUSED) runs in one pass: at
+// 8192 points (one window per 512-thread workgroup, the chirp in registers,
+// scratch spills) the three stages were faster on each beam measured (an
+// 8-bit 512-channel beam 10.1 against 11.7 ms per block, a 16-bit one 24.4
+// against 27.2), at 4096 points the one-pass kernel (7.1 against 11.0 ms).
+constexpr int kOnePassFusedMaxLog2 = 12;
 
 // LDS bytes of the exchange rows for an n-point transform (host side).
 inline uint32_t onepass_ntw(uint32_t n) { return n / 16; }

@@ -176,7 +176,7 @@ enum {
 
 enum {
   DSPX_FORM_FUSED = 0,    /* hand-written: the one-pass kernel where the transform
-                             (2^7..2^13 points) and the fold histogram fit the LDS
+                             (2^7..2^12 points) and the fold histogram fit the LDS
                              (LoadToFold plans), else the three stages */
   DSPX_FORM_PLAIN = 1,    /* hipFFT + separate multiply / discard kernels */
   DSPX_FORM_TWOLEVEL = 2, /* the three-stage (four-step) form, 2^12..2^22 points */
