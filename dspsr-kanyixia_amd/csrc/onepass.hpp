@@ -163,8 +163,9 @@ __global__ __launch_bounds__(OnePass<N>::NT) void k_onepass(LD ld, OnePassArgs a
     for (uint32_t pol = 0; pol < a.npol; pol++) {
 #pragma unroll
       for (int e = 0; e < E; e++) v[e] = ld.pol(wcur[e], pol);
-      // opaque: the pass twiddles are reloaded per window rather than held
-      // in registers across the loop (registers go to occupancy)
+      // other lengths: the pass twiddles are reloaded per window (opaque)
+      // rather than held in registers across the loop (registers go to
+      // occupancy)
       if constexpr (TW2)
         fft_regs_2w<LAY, N, NTW, -1>(lds, v, wq, tw, tid);
       else
