@@ -152,13 +152,8 @@ __global__ __launch_bounds__(OnePass<N>::NT) void k_onepass(LD ld, OnePassArgs a
     load_words(it, wcur);
     // the lane's index arithmetic (LDS addresses of every pass) is redone
     // per window rather than held in registers across the loop
-#ifndef DSPX_OP_REINDEX
-#define DSPX_OP_REINDEX 1
-#endif
-    if (DSPX_OP_REINDEX) {
-      tid = opaque_i(tid0);
-      lds = smem + slot * S::LROW;
-    }
+    tid = opaque_i(tid0);
+    lds = smem + slot * S::LROW;
     float2 y0[E], v[E];
     for (uint32_t pol = 0; pol < a.npol; pol++) {
 #pragma unroll
