@@ -302,26 +302,38 @@ struct ConvCore {
     // 114-146, three or four)
     const bool pf = MODE == kStageCFold && !ZH && N1 <= 64 && 3 * shm > 160 * 1024;
     switch (N1) {
-#define LAUNCH_C(n, nv, p)                                                                      \
-  do {                                                                                          \
+#define LAUNCH_C(n, nv, p, w)                                                                           \
+  do {                                                                                                   \
     /* the whole LDS allowed once per kernel (not a stream operation, so \
-       also outside any graph capture that follows) */                                          \
-    static bool lds_attr = false;                                                               \
-    if (!lds_attr) {                                                                            \
-      allow_full_lds((const void*) k_stageC<n, MODE, nv, p, ZH>);                               \
-      lds_attr = true;                                                                          \
-    }                                                                                           \
-    k_stageC<n, MODE, nv, p, ZH><<<unsigned(nb), kColThreads, shm, ctx->stream>>>(s, fb, fa, twc);  \
+       also outside any graph capture that follows) */                                                   \
+    static bool lds_attr = false;                                                                        \
+    if (!lds_attr) {                                                                                     \
+      allow_full_lds((const void*) k_stageC<n, MODE, nv, p, ZH, w>);                                     \
+      lds_attr = true;                                                                                   \
+    }                                                                                                    \
+    k_stageC<n, MODE, nv, p, ZH, w><<<unsigned(nb), kColThreads, shm, ctx->stream>>>(s, fb, fa, twc);    \
+  } while (0)
+// the narrow-bin ownership only matters at 32 and 64 points (FoldOwn)
+#define LAUNCH_CW(n, nv, p)                                          \
+  do {                                                               \
+    if constexpr (MODE == kStageCFold && (n == 32 || n == 64)) {     \
+      if (fa.narrow)                                                 \
+        LAUNCH_C(n, nv, p, true);                                    \
+      else                                                           \
+        LAUNCH_C(n, nv, p, false);                                   \
+    } else {                                                         \
+      LAUNCH_C(n, nv, p, true);                                      \
+    }                                                                \
   } while (0)
 #define LAUNCH_C2(n, nv)                                     \
   do {                                                       \
     if constexpr (MODE == kStageCFold && n <= 64 && !ZH) {   \
       if (pf)                                                \
-        LAUNCH_C(n, nv, true);                               \
+        LAUNCH_CW(n, nv, true);                              \
       else                                                   \
-        LAUNCH_C(n, nv, false);                              \
+        LAUNCH_CW(n, nv, false);                             \
     } else {                                                 \
-      LAUNCH_C(n, nv, false);                                \
+      LAUNCH_CW(n, nv, false);                               \
     }                                                        \
   } while (0)
 #define CASE_C(n)                                    \
@@ -336,6 +348,7 @@ struct ConvCore {
       CASE_C(16) CASE_C(32) CASE_C(64) CASE_C(128) CASE_C(256) CASE_C(512) CASE_C(1024)
 #undef CASE_C
 #undef LAUNCH_C2
+#undef LAUNCH_CW
 #undef LAUNCH_C
       default: fail(DSPX_E_UNSUPPORTED, "N1=%u", N1);
     }
@@ -1188,6 +1201,8 @@ FoldArgs fold_args(dspx_ltf* L, const ZapArgs* zap = nullptr)
   for (uint32_t i = 0; i < 4; i++) fa.voff[i] = (i / L->ndim_out) * L->d.nbin * L->ndim_out + (i % L->ndim_out);
   fa.hrow = (L->nval + (zap ? 1 : 0)) * L->d.nbin;
   fa.zap = zap ? *zap : ZapArgs{};
+  // samples per bin 1 / (turns per sample * bins per turn) below 64
+  fa.narrow = L->fold.pps * L->d.nbin > 1.0 / 64 ? 1u : 0u;
   return fa;
 }
 

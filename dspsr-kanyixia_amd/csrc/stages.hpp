@@ -157,6 +157,9 @@ struct FoldArgs {
   // data) nbin hit counts
   uint32_t hrow;
   ZapArgs zap;
+  // bins narrower than 64 samples (the launch's choice of ownership at 32
+  // and 64 points, FoldOwn)
+  uint32_t narrow = 1;
 };
 
 struct FilterbankArgs {
@@ -879,16 +882,20 @@ __device__ __forceinline__ void fold_lane(float* hist, const FoldArgs& fa, uint3
 // transform's last pass delivers E / (last radix) of them; a single-pass
 // transform (L = 16) would leave one column per thread.  The outputs are
 // regrouped through LDS into QN = 16 (or the tile's column count, if fewer)
-// wherever the last pass leaves fewer: a lane's 16 consecutive samples are
+// wherever the last pass leaves fewer (at 32 and 64 points only for narrow
+// bins, NB): a lane's 16 consecutive samples are
 // accumulated per run of equal bins (bins a few samples wide, short periods,
 // add to the histogram once per run), and a 16-lane row spans 256 samples
 // for the row sums of wide bins.
-template <int L> struct FoldOwn {
+template <int L, bool NB = true> struct FoldOwn {
   static constexpr int E = kColTile / kColThreads;
   static constexpr int RL = LastRadix<L, 16, +1>::v;
   static constexpr int C = kColTile / L;             // columns of a tile
   static constexpr int QMAX = C < 16 ? C : 16;
-  static constexpr bool REGROUP = RadixPlan<L, 16>::n == 1 || E / RL < QMAX;
+  // with wide bins 32- and 64-point columns keep the last pass's 8 and 4
+  // samples per lane: the regrouping costs them one more exchange (the
+  // 128-channel workload's stage C 527 -> 564 us with it)
+  static constexpr bool REGROUP = RadixPlan<L, 16>::n == 1 || (E / RL < QMAX && (L >= 128 || NB));
   static constexpr int QN = REGROUP ? QMAX : E / RL;
   static constexpr int R = E / QN;
 };
@@ -897,10 +904,10 @@ template <int L> struct FoldOwn {
 // entry v holds the first pass's inputs in the standard ownership; on exit
 // v[q*R + r] holds y(row j + r*L/R, column (tid % (C/QN))*QN + q) with
 // j = tid / (C/QN).
-template <class LAY, int L, int C>
+template <class LAY, int L, int C, bool NB>
 __device__ __forceinline__ void last_pass_cols(float2* lds, float2* v, const float2* __restrict__ tw, int tid)
 {
-  using OWN = FoldOwn<L>;
+  using OWN = FoldOwn<L, NB>;
   constexpr int NPASS = RadixPlan<L, 16>::n;
   constexpr int RL = OWN::RL;
   constexpr int QN = OWN::QN;
@@ -945,10 +952,11 @@ __device__ __forceinline__ void last_pass_cols(float2* lds, float2* v, const flo
 }
 
 // grid.x = G * npart * nchan (fold) or ncb * npart * nchan (filterbank).
-// NV: detected values per sample in fold mode (1, 2 or 4).  PFT: the fold
+// NV: detected values per sample in fold mode (1, 2 or 4).  NB: narrow
+// bins (the fold's ownership, FoldOwn).  PFT: the fold
 // loads each tile's inputs while the previous tile is transformed (64 more
 // registers; the launch chooses it where occupancy is set by the LDS).
-template <int N1, int MODE, int NV = 1, bool PFT = false, bool ZH = false>
+template <int N1, int MODE, int NV = 1, bool PFT = false, bool ZH = false, bool NB = true>
 __global__ __launch_bounds__(kColThreads) void k_stageC(ConvShape s, FilterbankArgs fb, FoldArgs fa,
                                                        const float2* __restrict__ twc)
 {
@@ -1056,13 +1064,13 @@ __global__ __launch_bounds__(kColThreads) void k_stageC(ConvShape s, FilterbankA
       // gives each thread QN consecutive columns (consecutive samples) of the
       // rows j + r*N1/RL, so a 16-lane DPP row spans whole tile rows
       __syncthreads();
-      last_pass_cols<LAY, N1, C>(lds, va, tw, tid);
+      last_pass_cols<LAY, N1, C, NB>(lds, va, tw, tid);
       if (s.npol == 2) {
         __syncthreads();
-        last_pass_cols<LAY, N1, C>(lds, vb, opaque(tw), tid);
+        last_pass_cols<LAY, N1, C, NB>(lds, vb, opaque(tw), tid);
       }
-      constexpr int QN = FoldOwn<N1>::QN;
-      constexpr int RF = FoldOwn<N1>::R;
+      constexpr int QN = FoldOwn<N1, NB>::QN;
+      constexpr int RF = FoldOwn<N1, NB>::R;
       const uint32_t bq0 = (tid % (C / QN)) * QN;
       const uint32_t jr = tid / (C / QN);
       // key address: uniform row base plus a per-thread 32-bit byte offset

@@ -66,6 +66,37 @@ def test_load_to_fold_parity(ctx, dspx, oracle, case, form):
     assert np.array_equal(from_dev(hits).view(np.uint32), 2 * hits_o)
 
 
+@pytest.mark.parametrize("nfft", [1 << 17, 1 << 18])
+@pytest.mark.parametrize("period", [0.005, 0.1])
+def test_column_ownership_narrow_and_wide_bins(ctx, dspx, oracle, nfft, period):
+    """Stage C's 32- and 64-point columns (2^17 and 2^18-point transforms)
+    fold 16 consecutive samples per lane after a regrouping where bins are
+    narrower than 64 samples (5 ms: 15 samples per bin) and the last pass's
+    8 or 4 where they are wider (100 ms: 305): both against the oracle."""
+    import torch
+    nbit, nchan, npol, ndat, cfreq, bw, dm, state, ndim, nbin = 8, 16, 2, 1 << 19, 1400.0, 50.0, 300.0, 3, 4, 1024
+    geom = oracle.geometry(cfreq, bw, nchan, dm, nfft=nfft)
+    from importlib import import_module
+    synth = import_module("dspsr-kanyixia_amd.synth")
+    raw = synth.make_baseband(nchan, npol, ndat, cfreq, bw, dm, period, 0.03, 2.0, sigma=10.0, nbit=nbit, seed=5)
+    print("ltf ownership nfft %d period %g geom %s" % (nfft, period, geom), flush=True)
+    L = dspx.LoadToFold(ctx, nbit, nchan, npol, ndat, cfreq, bw, dm, nbin, state=state, ndim=ndim, form=0, geom=geom)
+    rate = abs(bw) / nchan * 1e6
+    chunks = dspx.fold_chunks(L.ndat_out, rate, period, geom["impulse_pos"], nsub=2)
+    pps = (1.0 / rate) / period
+    L.set_fold(pps, chunks)
+    _, prof_o, hits_o = oracle_load_to_fold(oracle, raw, nbit, nchan, npol, ndat, cfreq, bw, dm, geom, state, ndim,
+                                            nbin, pps, chunks, 2)
+    d_raw = to_dev(raw)
+    prof = torch.zeros(prof_o.shape, dtype=torch.float32, device="cuda")
+    hits = torch.zeros(hits_o.shape, dtype=torch.int32, device="cuda")
+    L.execute(d_raw, prof, hits)
+    e = rel_rms(from_dev(prof), prof_o)
+    print("profile rel rms %.3g" % e, flush=True)
+    assert e < 1e-5
+    assert np.array_equal(from_dev(hits).view(np.uint32), hits_o)
+
+
 def test_fold_plan_per_block(ctx, dspx, oracle):
     """The fold plan rebuilt before every block (as a stream of blocks does),
     with other phases and sub-integration layouts each time: each block's
