@@ -546,7 +546,7 @@ def run_engine(args, dspx, synth, ctx):
                                      "achieved": round(flops / t_f / 1e9, 2), "peak": 2500.0, "unit": "TFLOP/s",
                                      "frac": round(flops / t_f / 1e9 / 2500.0, 4), "traffic": None,
                                      "note": "algorithmic flops of the 4 layers over the forward's time; the two "
-                                             "32-32 layers (94% of them) run as BF16 16x16x16 matrix products with "
+                                             "32-32 layers (94% of them) run as BF16 16x16x32 matrix products with "
                                              "the activations split hi/lo (twice the matrix flops), the first and "
                                              "last layers on the VALU; peak: dense BF16 matrix cores"}})
         sk.close()

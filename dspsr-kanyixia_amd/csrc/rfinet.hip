@@ -164,16 +164,25 @@ __global__ __launch_bounds__(kTileX) void k_conv(const float* __restrict__ A, co
 // about 16 mantissa bits of the activation (relative error near 2^-17)
 // instead of 8.
 //
-// Lane layout of v_mfma_f32_16x16x16bf16_1k (lane l, r = l % 16, q = l / 16):
-// A operand A[r][4q + j], B operand B[4q + j][r], j < 4 (4 BF16 each);
+// Lane layout of v_mfma_f32_16x16x32_bf16 (lane l, r = l % 16, q = l / 16):
+// A operand A[r][8q + j], B operand B[8q + j][r], j < 8 (8 BF16 each);
 // result D[4q + j][r].  A workgroup of 4 waves computes 32 output channels
 // (two 16-row tiles) at kMT positions (kMT / 64 16-column tiles per wave); k runs
-// in 38 steps of 16 (tap s / 2, input channels 16 (s % 2) + 4q + j).
+// in 19 steps of 32 (one tap, input channels 8q + j).
 constexpr int kMT = 128;             // positions per workgroup (a multiple of 64)
 constexpr int kMXP = kMT + kK - 1;   // activation tile width
 constexpr int kMRS = kW + 4;         // LDS row stride (BF16 units): rows 72 bytes apart
 typedef short bf16x4_t __attribute__((ext_vector_type(4)));
+typedef short s16x8_t __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
 typedef float f32x4_t __attribute__((ext_vector_type(4)));
+
+// eight BF16 (bit patterns) from two 8-byte loads
+__device__ __forceinline__ bf16x8_t bf16x8_of(bf16x4_t lo, bf16x4_t hi)
+{
+  const s16x8_t v = __builtin_shufflevector(lo, hi, 0, 1, 2, 3, 4, 5, 6, 7);
+  return __builtin_bit_cast(bf16x8_t, v);
+}
 
 __device__ __forceinline__ uint16_t bf16_bits_rne(float f)
 {
@@ -226,21 +235,26 @@ __global__ __launch_bounds__(256) void k_conv_mfma(const float* __restrict__ A, 
 #pragma unroll
       for (int j = 0; j < 4; j++) acc[m][t][j] = bias[16 * m + 4 * q + j];
   const bf16x4_t* wg = reinterpret_cast<const bf16x4_t*>(Wb);
+  const int ci = 8 * q;
 #pragma unroll 2
-  for (int s = 0; s < 2 * kK; s++) {
-    const int tap = s >> 1, ci = 16 * (s & 1) + 4 * q;
-    bf16x4_t aw[2];
+  for (int tap = 0; tap < kK; tap++) {
+    bf16x8_t aw[2];
 #pragma unroll
-    for (int m = 0; m < 2; m++) aw[m] = wg[((tap * kW + 16 * m + r) * kW + ci) / 4];
+    for (int m = 0; m < 2; m++) {
+      const int w0 = ((tap * kW + 16 * m + r) * kW + ci) / 4;
+      aw[m] = bf16x8_of(wg[w0], wg[w0 + 1]);
+    }
 #pragma unroll
     for (int t = 0; t < NT; t++) {
       const int jj = 16 * (wv * NT + t) + r + tap;
-      const bf16x4_t bh = *reinterpret_cast<const bf16x4_t*>(&ahi[jj * kMRS + ci]);
-      const bf16x4_t bl = *reinterpret_cast<const bf16x4_t*>(&alo[jj * kMRS + ci]);
+      const bf16x4_t* ph = reinterpret_cast<const bf16x4_t*>(&ahi[jj * kMRS + ci]);
+      const bf16x4_t* pl = reinterpret_cast<const bf16x4_t*>(&alo[jj * kMRS + ci]);
+      const bf16x8_t bh = bf16x8_of(ph[0], ph[1]);
+      const bf16x8_t bl = bf16x8_of(pl[0], pl[1]);
 #pragma unroll
       for (int m = 0; m < 2; m++) {
-        acc[m][t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(aw[m], bh, acc[m][t], 0, 0, 0);
-        acc[m][t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(aw[m], bl, acc[m][t], 0, 0, 0);
+        acc[m][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(aw[m], bh, acc[m][t], 0, 0, 0);
+        acc[m][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(aw[m], bl, acc[m][t], 0, 0, 0);
       }
     }
   }
