@@ -157,7 +157,7 @@ __global__ __launch_bounds__(kTileX) void k_conv(const float* __restrict__ A, co
 // ------------------------------------------------- 32 -> 32 layers on MFMA
 // The same convolution as an implicit GEMM on the matrix cores:
 // Y[co][x] = bias[co] + sum_k W'[co][k] * B[k][x], k = tap * 32 + ci,
-// B[k][x] = A[ci][x + tap - P], with 16x16x16 BF16 products accumulated in
+// B[k][x] = A[ci][x + tap - P], with 16x16x32 BF16 products accumulated in
 // FP32.  The weights are BF16 already (the network's weights are kept
 // rounded); each activation is split into BF16 hi = bf16(a) and lo =
 // bf16(a - hi), and both halves are multiplied, so the products carry
