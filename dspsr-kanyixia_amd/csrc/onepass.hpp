@@ -155,7 +155,12 @@ __global__ __launch_bounds__(OnePass<N>::NT) void k_onepass(LD ld, OnePassArgs a
     tid = opaque_i(tid0);
     lds = smem + slot * S::LROW;
     float2 y0[E], v[E];
-    for (uint32_t pol = 0; pol < a.npol; pol++) {
+    // one polarisation: transform, chirp, transform back, and (folding)
+    // regroup into dst; written out for pol 0 (into y0) and pol 1 (into v)
+    // rather than looped over, so that neither the word's half nor the
+    // destination array is selected per value (a select per float of
+    // both arrays in the rolled loop)
+    auto one_pol = [&](const uint32_t pol, float2* dst) __attribute__((always_inline)) {
 #pragma unroll
       for (int e = 0; e < E; e++) v[e] = ld.pol(wcur[e], pol);
       // other lengths: the pass twiddles are reloaded per window (opaque)
@@ -200,15 +205,11 @@ __global__ __launch_bounds__(OnePass<N>::NT) void k_onepass(LD ld, OnePassArgs a
 #pragma unroll
         for (int g = 0; g < QG; g++)
 #pragma unroll
-          for (int k = 0; k < QF; k++) {
-            const float2 x = lds[LAY::idx((g * NTW + tid) * QF + k, 0)];
-            if (pol == 0)
-              y0[g * QF + k] = x;
-            else
-              v[g * QF + k] = x;
-          }
+          for (int k = 0; k < QF; k++) dst[g * QF + k] = lds[LAY::idx((g * NTW + tid) * QF + k, 0)];
       }
-    }
+    };
+    one_pol(0u, y0);
+    if (a.npol == 2) one_pol(1u, v);
     if constexpr (MODE == kStageCFold) {
       const uint32_t* kw = fa.keys + uint64_t(valid ? w : w_begin) * N;
 #pragma unroll

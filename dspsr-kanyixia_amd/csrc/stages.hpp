@@ -33,11 +33,11 @@ struct LoadI8 {
   uint32_t npol;
   uint32_t offset_binary;
   float scale;
-  __device__ __forceinline__ float cvt(uint32_t byte) const
-  {
-    return offset_binary ? (float(byte) - 127.5f) * scale
-                         : (float(int8_t(uint8_t(byte))) + 0.5f) * scale;
-  }
+  // offset binary b is two's complement b ^ 0x80 (b - 127.5 = int8(b ^ 0x80)
+  // + 0.5 exactly), so one exclusive or of the whole word with a uniform
+  // mask serves both encodings
+  __device__ __forceinline__ uint32_t flip() const { return offset_binary ? 0x80808080u : 0u; }
+  __device__ __forceinline__ float cvt(uint32_t byte) const { return (float(int8_t(uint8_t(byte))) + 0.5f) * scale; }
   // packed raw word of both pols of one sample
   using Word = uint32_t;
   __device__ __forceinline__ Word word(uint32_t chan, uint64_t t) const
@@ -47,7 +47,7 @@ struct LoadI8 {
   }
   __device__ __forceinline__ float2 pol(Word w, uint32_t ipol) const
   {
-    w >>= 16 * ipol;
+    w = (w ^ flip()) >> (16 * ipol);
     return make_float2(cvt(w & 0xff), cvt((w >> 8) & 0xff));
   }
   __device__ __forceinline__ void operator()(uint32_t chan, uint64_t t, float2& a, float2& b) const
@@ -66,11 +66,8 @@ struct LoadI16 {
   uint64_t cstride;
   uint32_t npol;
   uint32_t offset_binary;
-  __device__ __forceinline__ float cvt(int16_t v) const
-  {
-    if (offset_binary) v = int16_t(v ^ 0x8000);
-    return float(v) + 0.5f;
-  }
+  __device__ __forceinline__ uint32_t flip() const { return offset_binary ? 0x80008000u : 0u; }
+  __device__ __forceinline__ float cvt(int16_t v) const { return float(v) + 0.5f; }
   using Word = uint2;
   __device__ __forceinline__ Word word(uint32_t chan, uint64_t t) const
   {
@@ -79,7 +76,7 @@ struct LoadI16 {
   }
   __device__ __forceinline__ float2 pol(Word w, uint32_t ipol) const
   {
-    uint32_t v = ipol ? w.y : w.x;
+    const uint32_t v = (ipol ? w.y : w.x) ^ flip();
     return make_float2(cvt(int16_t(v & 0xffff)), cvt(int16_t(v >> 16)));
   }
   __device__ __forceinline__ void operator()(uint32_t chan, uint64_t t, float2& a, float2& b) const
