@@ -822,11 +822,14 @@ __device__ __forceinline__ void fold_lane(float* hist, const FoldArgs& fa, uint3
     detect_finish<NV>(stokes, c, d);
     if constexpr (ZH) d[NV] = c[NV];
   };
+  // the slot, the bin and the LDS histogram's row and bin strides are all
+  // below 2^24 (the histogram fits in LDS), so the offsets are 24-bit
+  // products (full-rate multiplies instead of quarter-rate 32-bit ones)
   auto add = [&](uint32_t key, const float* d) {
-    float* hs = hist + (key >> 16) * fa.hrow;
+    float* hs = hist + __umul24(key >> 16, fa.hrow);
     const uint32_t bin = key & 0xffffu;
 #pragma unroll
-    for (int i = 0; i < NV; i++) atomicAdd(hs + bin * fa.ndim + fa.voff[i], d[i]);
+    for (int i = 0; i < NV; i++) atomicAdd(hs + __umul24(bin, fa.ndim) + fa.voff[i], d[i]);
     if constexpr (ZH) atomicAdd(hs + NV * fa.nbin + bin, d[NV]);
   };
   float cacc[NA], racc[NA];
