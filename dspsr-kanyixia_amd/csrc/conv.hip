@@ -368,6 +368,9 @@ struct ConvCore {
     size_t shm = onepass_lds_bytes(N);
     if (MODE == kStageCFold) shm += size_t(fa.nslot) * fa.hrow * sizeof(float);
     DSPX_REQUIRE(shm <= 160 * 1024, DSPX_E_UNSUPPORTED, "one-pass LDS request %zu too large", shm);
+    // a window's word offsets j * tstride (j < N) are 32-bit products
+    DSPX_REQUIRE(uint64_t(N) * ld.tstride < (1ull << 32), DSPX_E_UNSUPPORTED, "sample stride %llu",
+                 (unsigned long long) ld.tstride);
     ChirpArgs c = chirp();
     const float2* tw = twiddles(ctx, int(N));
 #define LAUNCH_OP(n, nv)                                                                                        \

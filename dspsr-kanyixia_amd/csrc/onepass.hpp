@@ -143,7 +143,7 @@ __global__ __launch_bounds__(OnePass<N>::NT) void k_onepass(LD ld, OnePassArgs a
 #pragma unroll
     for (int q = 0; q < E / R0; q++)
 #pragma unroll
-      for (int r = 0; r < R0; r++) wd[q * R0 + r] = ld.word(chan, t_ + uint32_t(tid0 + q * NTW + r * (N / R0)));
+      for (int r = 0; r < R0; r++) wd[q * R0 + r] = ld.word(chan, t_, uint32_t(tid0 + q * NTW + r * (N / R0)));
   };
   for (uint32_t it = 0; it < niter; it++) {
     const uint32_t w = w_begin + it * WF + uint32_t(slot);

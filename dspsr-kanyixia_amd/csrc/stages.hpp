@@ -45,6 +45,13 @@ struct LoadI8 {
     const uint8_t* q = raw + t * tstride + uint64_t(chan) * cstride;
     return npol == 2 ? *reinterpret_cast<const uint32_t*>(q) : uint32_t(*reinterpret_cast<const uint16_t*>(q));
   }
+  // the word of sample t0 + j, where j * tstride < 2^32: the per-sample part
+  // of the address is a 32-bit product (the window's base once per window)
+  __device__ __forceinline__ Word word(uint32_t chan, uint64_t t0, uint32_t j) const
+  {
+    const uint8_t* q = raw + t0 * tstride + uint64_t(chan) * cstride + j * uint32_t(tstride);
+    return npol == 2 ? *reinterpret_cast<const uint32_t*>(q) : uint32_t(*reinterpret_cast<const uint16_t*>(q));
+  }
   __device__ __forceinline__ float2 pol(Word w, uint32_t ipol) const
   {
     w = (w ^ flip()) >> (16 * ipol);
@@ -74,6 +81,11 @@ struct LoadI16 {
     const int16_t* q = raw + t * tstride + uint64_t(chan) * cstride;
     return npol == 2 ? *reinterpret_cast<const uint2*>(q) : make_uint2(*reinterpret_cast<const uint32_t*>(q), 0u);
   }
+  __device__ __forceinline__ Word word(uint32_t chan, uint64_t t0, uint32_t j) const
+  {
+    const int16_t* q = raw + t0 * tstride + uint64_t(chan) * cstride + j * uint32_t(tstride);
+    return npol == 2 ? *reinterpret_cast<const uint2*>(q) : make_uint2(*reinterpret_cast<const uint32_t*>(q), 0u);
+  }
   __device__ __forceinline__ float2 pol(Word w, uint32_t ipol) const
   {
     const uint32_t v = (ipol ? w.y : w.x) ^ flip();
@@ -96,6 +108,7 @@ struct LoadF32 {
   // the "word" is the sample's offset; each pol is loaded when converted
   using Word = uint64_t;
   __device__ __forceinline__ Word word(uint32_t chan, uint64_t t) const { return uint64_t(chan) * chan_stride + 2 * t; }
+  __device__ __forceinline__ Word word(uint32_t chan, uint64_t t0, uint32_t j) const { return word(chan, t0 + j); }
   __device__ __forceinline__ float2 pol(Word w, uint32_t ipol) const
   {
     return *reinterpret_cast<const float2*>(in + w + (ipol ? pol_stride : 0));
